@@ -1,0 +1,168 @@
+/*
+ * pizza_amd — C ABI of the MI355X-native IMEX time step of pizza.
+ *
+ * Every entry point takes plain pointers and sizes; no C++ or torch types
+ * cross this boundary.  Host arrays that hold spectral fields use the
+ * reference's Fortran layout: complex(8) f(n_m_max, n_r_max), m fastest,
+ * (re, im) interleaved — so a Fortran caller passes its arrays directly
+ * through ISO_C_BINDING.
+ *
+ * Return codes: 0 = ok; > 0 = LAPACK-style "info" of a singular pivot
+ * (the reference aborts with 'Singular matrix' in that case);
+ * < 0 = invalid argument, unsupported configuration or HIP error.
+ * pz_last_error() returns the message of the last failure of a handle.
+ */
+#ifndef PIZZA_AMD_H
+#define PIZZA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PZ_N_T_BOUNDS 32 /* namelists.f90:23 (n_t_bounds) */
+
+/* Time schemes (select_time_scheme.f90:15-49). */
+enum pz_time_scheme {
+    PZ_CNAB2 = 1,  /* multistep_schemes.f90:56-61 */
+    PZ_ARS222 = 101, /* dirk_schemes.f90:55-62, Butcher tables :287-299 */
+    PZ_ARS443 = 102, /* dirk_schemes.f90:151-158, :464-480 */
+    PZ_BPR353 = 103  /* dirk_schemes.f90:167-174, :481-497 */
+};
+
+/* Field identifiers for pz_set_field / pz_get_field (fields.f90). */
+enum pz_field {
+    PZ_US = 0,      /* u_s        (m, r) physical r */
+    PZ_UP = 1,      /* u_phi      (m, r) */
+    PZ_OM = 2,      /* omega_z    (m, r) */
+    PZ_PSI = 3,     /* psi        (m, r)  (m /= 0) */
+    PZ_TEMP = 4,    /* T          (m, r) */
+    PZ_XI = 5,      /* xi         (m, r) */
+    PZ_TEMP_HAT = 6,/* Chebyshev coefficients of T (integration method) */
+    PZ_XI_HAT = 7,
+    PZ_PSI_HAT = 8,
+    PZ_DTEMP = 9,   /* dT/dr (refreshed on log steps) */
+    PZ_NFIELDS = 10
+};
+
+/*
+ * Run configuration: the subset of the reference namelists
+ * (namelists.f90:162-189, defaults :530-672) that the IMEX step reads.
+ * pz_config_default() fills the reference defaults.
+ */
+typedef struct pz_config {
+    /* &grid */
+    int n_r_max, n_cheb_max, m_max, minc;
+    /* &control */
+    int time_scheme;       /* enum pz_time_scheme */
+    int l_cheb_coll;       /* 1: 'colloc', 0: 'integ' (sparse Chebyshev) */
+    int l_direct_solve;    /* matrix_solve='direct' */
+    int l_coriolis_imp;    /* corio_term='implicit' */
+    int l_buo_imp;         /* buo_term='implicit' */
+    int l_non_rot;
+    int time_scale_rot;    /* time_scale='rot' */
+    int l_rerror_fix;
+    double rerror_fac;
+    double alpha;          /* Crank-Nicolson weight */
+    double dtMax;
+    double courfac;        /* >= 1e3 selects the scheme's own value */
+    double dt_fac;
+    /* &phys_param */
+    double ra, ek, pr, raxi, sc, radratio, g0, g1, g2;
+    int ktopt, kbott, ktopv, kbotv, ktopxi, kbotxi;
+    int l_ek_pump, l_temp_advz;
+    double tcond_fac, xicond_fac;
+    double t_top[3 * PZ_N_T_BOUNDS], t_bot[3 * PZ_N_T_BOUNDS];
+    double xi_top[3 * PZ_N_T_BOUNDS], xi_bot[3 * PZ_N_T_BOUNDS];
+} pz_config;
+
+typedef struct pz_handle pz_handle;
+
+/* Fills the reference namelist defaults (namelists.f90:530-672). */
+void pz_config_default(pz_config *cfg);
+
+/*
+ * Creates a run on the current HIP device: truncation (truncation.f90:24-50),
+ * radial functions (radial.f90:115-282), horizontal functions
+ * (horizontal.f90:69-202), operator matrices and device state.
+ * Replaces the init sequence of pizza.f90:63-199 for the step's state.
+ */
+int pz_create(const pz_config *cfg, pz_handle **h);
+int pz_destroy(pz_handle *h);
+const char *pz_last_error(const pz_handle *h);
+
+/* Sizes of the spectral arrays: n_m_max, n_r_max, n_phi_max. */
+int pz_dims(const pz_handle *h, int *n_m_max, int *n_r_max, int *n_phi_max);
+
+/*
+ * Host <-> device copies of one field; host = complex(8) (n_m_max, n_r_max),
+ * m fastest.  Stream-ordered; pz_get_field synchronises.
+ */
+int pz_set_field(pz_handle *h, int field, const double *host);
+int pz_get_field(pz_handle *h, int field, double *host);
+
+/*
+ * Reference deterministic initial conditions (init_fields.f90:260-360,
+ * :455-509): init_t > 0 single mode, init_t == -1 Gaussian bubble; the same
+ * for xi; init_u > 0 single mode, init_u < 0 axisymmetric u_phi.  Adds to the
+ * fields already set (zero after pz_create).
+ */
+int pz_init_fields(pz_handle *h, int init_t, double amp_t, int init_xi,
+                   double amp_xi, int init_u, double amp_u);
+
+/*
+ * Fresh-start post-processing of get_start_fields (init_fields.f90:96-112,
+ * :186-255): dt(:) = dtMax, scheme weights, omega and dT/dr from the set
+ * fields, the Chebyshev "hat" fields and the first implicit state.
+ */
+int pz_start(pz_handle *h);
+
+/*
+ * Advances nsteps full IMEX steps: the stage loop of time_loop
+ * (step_time.f90:215-370) — radial loop, explicit assembly, Courant
+ * (stage 1), m-loop solves.  Asynchronous on the handle's stream.
+ */
+int pz_step(pz_handle *h, int nsteps);
+
+/* Synchronises and returns the simulation time and current dt(1). */
+int pz_time(pz_handle *h, double *time, double *dt);
+
+/*
+ * 3-D kinetic energy time-series entry of outputs.f90:552-571, 686-688:
+ * out[0..3] = us2_3D, up2_3D, uz2_3D, up2_axi_3D.  Synchronises.
+ */
+int pz_ekin_3d(pz_handle *h, double out[4]);
+
+/* The HIP stream (hipStream_t) the handle launches on. */
+void *pz_stream(pz_handle *h);
+
+/*
+ * Unit-level operators on caller buffers (host memory), the reference
+ * module procedures of the path:
+ *   pz_costf  — costf1 (radial_scheme.f90:47-49; dct_fftw.f90:175-197),
+ *               in place on f(n_m, n_r) complex, m fastest;
+ *   pz_get_dr / pz_get_ddr — get_dr/get_ddr complex 2-D
+ *               (radial_derivatives.f90:313-418, 487-591; dct_fftw.f90:258-397);
+ *   pz_fft / pz_ifft — fourier::fft/ifft (fft_fftw.f90:44-90) over n_rad
+ *               radii: spectral (n_m, n_rad) <-> grid (n_phi, n_rad).
+ * l_dct_in = 0 means f already holds Chebyshev coefficients.
+ */
+int pz_costf(pz_handle *h, double *f, int n_m);
+int pz_get_dr(pz_handle *h, const double *f, double *df, int n_m, int l_dct_in);
+int pz_get_ddr(pz_handle *h, const double *f, double *df, double *ddf, int n_m,
+               int l_dct_in);
+int pz_fft(pz_handle *h, const double *grid, double *spec, int n_rad);
+int pz_ifft(pz_handle *h, const double *spec, double *grid, int n_rad);
+
+/*
+ * Kernel timing for the roofline report: launches the m-loop update of the
+ * psi equation (stage 1 weights, no state change outside scratch) reps
+ * times between HIP events on the handle's stream; returns the mean
+ * duration in ms.  No reference counterpart (measurement only).
+ */
+int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIZZA_AMD_H */

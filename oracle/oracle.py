@@ -1,0 +1,156 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of oracle/liboracle.so, the CPU restatement of the reference's
+IMEX step.  Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline; never by the product library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from pizza_amd.config import FIELDS, PzConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        L.or_create.argtypes = [ctypes.POINTER(PzConfig), ctypes.POINTER(vp)]
+        L.or_destroy.argtypes = [vp]
+        L.or_destroy.restype = None
+        L.or_dims.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)]
+        L.or_set_field.argtypes = [vp, ip, dp]
+        L.or_get_field.argtypes = [vp, ip, dp]
+        L.or_init_fields.argtypes = [vp, ip, ctypes.c_double, ip, ctypes.c_double, ip,
+                                     ctypes.c_double]
+        L.or_start.argtypes = [vp]
+        L.or_step.argtypes = [vp, ip]
+        L.or_time.argtypes = [vp, dp, dp]
+        L.or_ekin_3d.argtypes = [vp, dp]
+        L.or_costf.argtypes = [vp, dp, ip]
+        L.or_get_dr.argtypes = [vp, dp, dp, ip, ip]
+        L.or_get_ddr.argtypes = [vp, dp, dp, dp, ip, ip]
+        L.or_fft.argtypes = [vp, dp, dp, ip]
+        L.or_ifft.argtypes = [vp, dp, dp, ip]
+        L.or_grid.argtypes = [vp, dp, dp]
+        L.or_stencil.argtypes = [vp, ip, ip, ip, ip, dp]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """One CPU run of the reference's step (integration method)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        rc = lib().or_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise RuntimeError(f"or_create failed ({rc})")
+        self.h = h
+        nm, nr, npf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().or_dims(h, ctypes.byref(nm), ctypes.byref(nr), ctypes.byref(npf))
+        self.n_m, self.n_r, self.n_phi = nm.value, nr.value, npf.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.or_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    def set_field(self, name, arr):
+        """arr: complex (n_m, n_r)."""
+        a = np.asfortranarray(np.asarray(arr, dtype=np.complex128))
+        buf = np.ascontiguousarray(a.T).view(np.float64)
+        self._check(lib().or_set_field(self.h, FIELDS[name], _dp(buf)), "set_field")
+
+    def get_field(self, name):
+        buf = np.zeros((self.n_r, self.n_m), dtype=np.complex128)
+        self._check(lib().or_get_field(self.h, FIELDS[name], _dp(buf.view(np.float64))),
+                    "get_field")
+        return buf.T.copy()
+
+    def init_fields(self, init_t=0, amp_t=0.0, init_xi=0, amp_xi=0.0, init_u=0, amp_u=0.0):
+        self._check(lib().or_init_fields(self.h, init_t, amp_t, init_xi, amp_xi, init_u,
+                                         amp_u), "init_fields")
+
+    def start(self):
+        self._check(lib().or_start(self.h), "start")
+
+    def step(self, n=1):
+        self._check(lib().or_step(self.h, n), "step")
+
+    def time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        lib().or_time(self.h, ctypes.byref(t), ctypes.byref(dt))
+        return t.value, dt.value
+
+    def ekin_3d(self):
+        out = np.zeros(4)
+        lib().or_ekin_3d(self.h, _dp(out))
+        return out
+
+    def grid(self):
+        r, x = np.zeros(self.n_r), np.zeros(self.n_r)
+        lib().or_grid(self.h, _dp(r), _dp(x))
+        return r, x
+
+    # unit operators on complex (n_m, n_r) arrays
+    def _to_buf(self, f):
+        f = np.asarray(f, dtype=np.complex128)
+        return np.ascontiguousarray(f.T), f.shape[0]
+
+    def costf(self, f):
+        buf, nm = self._to_buf(f)
+        lib().or_costf(self.h, _dp(buf.view(np.float64)), nm)
+        return buf.T.copy()
+
+    def get_dr(self, f, l_dct_in=True):
+        buf, nm = self._to_buf(f)
+        out = np.zeros_like(buf)
+        lib().or_get_dr(self.h, _dp(buf.view(np.float64)), _dp(out.view(np.float64)), nm,
+                        int(l_dct_in))
+        return out.T.copy()
+
+    def get_ddr(self, f, l_dct_in=True):
+        buf, nm = self._to_buf(f)
+        d1, d2 = np.zeros_like(buf), np.zeros_like(buf)
+        lib().or_get_ddr(self.h, _dp(buf.view(np.float64)), _dp(d1.view(np.float64)),
+                         _dp(d2.view(np.float64)), nm, int(l_dct_in))
+        return d1.T.copy(), d2.T.copy()
+
+    def fft(self, grid):
+        """grid: real (n_phi, n_rad) -> spectral complex (n_m, n_rad)."""
+        g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64).T)
+        nrad = g.shape[0]
+        out = np.zeros((nrad, self.n_m), dtype=np.complex128)
+        lib().or_fft(self.h, _dp(g), _dp(out.view(np.float64)), nrad)
+        return out.T.copy()
+
+    def ifft(self, spec):
+        s = np.ascontiguousarray(np.asarray(spec, dtype=np.complex128).T)
+        nrad = s.shape[0]
+        out = np.zeros((nrad, self.n_phi))
+        lib().or_ifft(self.h, _dp(s.view(np.float64)), _dp(out), nrad)
+        return out.T.copy()
+
+    def stencil(self, which, m, idx, ku):
+        out = np.zeros(2 * ku + 1)
+        rc = lib().or_stencil(self.h, which, m, idx, ku, _dp(out))
+        self._check(rc, "stencil")
+        return out

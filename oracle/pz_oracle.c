@@ -1,0 +1,2444 @@
+/*
+ * ORACLE — test infrastructure only (see pz_oracle.h).
+ *
+ * CPU restatement of the reference's IMEX step for the sparse-Chebyshev
+ * (integration) method with direct solves.  Each function names the
+ * reference routine (file:line, relative to the reference source tree) whose
+ * algorithm it follows.  Arithmetic is FP64 throughout; the operator
+ * stencils are composed in long double and rounded once.
+ */
+#include "pz_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+typedef long double ld;
+
+#define PI_F (4.0 * atan(1.0)) /* constants.f90:5 */
+
+/* ------------------------------------------------------------------------ */
+/* Mixed-radix complex FFT (unnormalised; dir=-1 forward, +1 backward).       */
+/* Stands in for the FFTW plans of fft_fftw.f90:21-35 and dct_fftw.f90:77-84. */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    cplx *w;    /* w[k] = exp(-2 pi i k / n) */
+    cplx *tmp;  /* scratch n */
+    cplx *tmp2; /* scratch n */
+} fftp;
+
+static fftp *fft_new(int n)
+{
+    fftp *p = calloc(1, sizeof(fftp));
+    p->n = n;
+    p->w = malloc(sizeof(cplx) * n);
+    p->tmp = malloc(sizeof(cplx) * n);
+    p->tmp2 = malloc(sizeof(cplx) * n);
+    for (int k = 0; k < n; k++) {
+        ld ang = 2.0L * 3.14159265358979323846264338327950288L * (ld)k / (ld)n;
+        p->w[k] = (double)cosl(ang) - I * (double)sinl(ang);
+    }
+    return p;
+}
+
+static void fft_free(fftp *p)
+{
+    if (!p) return;
+    free(p->w);
+    free(p->tmp);
+    free(p->tmp2);
+    free(p);
+}
+
+static int small_factor(int n)
+{
+    if (n % 4 == 0) return 4;
+    for (int f = 2; f * f <= n; f++)
+        if (n % f == 0) return f;
+    return n;
+}
+
+static inline cplx twid(const fftp *p, long idx, int dir)
+{
+    cplx w = p->w[idx % p->n];
+    return dir < 0 ? w : conj(w);
+}
+
+/* y[0..n) = DFT of x[0], x[is], ..., decimation in time, radix f */
+static void fft_rec(const fftp *p, const cplx *x, long is, cplx *y, int n, int dir)
+{
+    if (n == 1) {
+        y[0] = x[0];
+        return;
+    }
+    int f = small_factor(n), q = n / f;
+    for (int r = 0; r < f; r++) fft_rec(p, x + r * is, is * f, y + r * q, q, dir);
+    long step = p->n / n;
+    cplx t[64];
+    cplx *out = p->tmp2;
+    for (int k = 0; k < q; k++) {
+        for (int r = 0; r < f; r++) t[r] = y[r * q + k] * twid(p, (long)r * k * step, dir);
+        for (int s = 0; s < f; s++) {
+            cplx acc = 0;
+            for (int r = 0; r < f; r++) acc += t[r] * twid(p, ((long)r * s % f) * q * step, dir);
+            out[k + s * q] = acc;
+        }
+    }
+    memcpy(y, out, sizeof(cplx) * n);
+}
+
+static void fft_exec(fftp *p, const cplx *in, cplx *out, int dir)
+{
+    memcpy(p->tmp, in, sizeof(cplx) * p->n);
+    fft_rec(p, p->tmp, 1, out, p->n, dir);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Handle                                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct { int kl, ku, n; double *d; } bandr;   /* dgbmv storage */
+typedef struct { int kl, ku, n; cplx *d; } bandc;     /* zgbmv storage */
+
+typedef struct {          /* bordered_matrix.f90:15-28 (real) */
+    int kl, ku, ntau, nb; /* nb = nlines_band */
+    double *A1, *A2, *A3, *A4; /* A1 ntau x ntau, A2 ntau x nb, A3 nb x ntau, A4 (2kl+ku+1) x nb */
+    int *p1, *p4;
+} bordr;
+
+typedef struct {          /* bordered_matrix.f90:30-43 (complex) */
+    int kl, ku, ntau, nb;
+    cplx *A1, *A2, *A3, *A4;
+    int *p1, *p4;
+} bordc;
+
+typedef struct {          /* time_array.f90:11-24 */
+    int nold, nimp, nexp;
+    cplx *old, *impl, *expl; /* [slot][n_m][n_r] */
+} tarray;
+
+typedef struct {          /* time_schemes.f90:10-40 + children */
+    int family;           /* 0 multistep, 1 dirk */
+    int scheme;
+    int nstages, istage, nexp, nold, nimp;
+    int l_assembly;
+    double courfac;
+    double dt[8], wimp[8], wimp_lin[8], wexp[8];
+    double bimp[8][8], bexp[8][8];   /* scaled Butcher tables (row, col) 0-based */
+    double bimp0[8][8], bexp0[8][8]; /* unscaled */
+    double wimp_lin0;
+    int l_exp_calc[8], l_imp_calc_rhs[8];
+} tscheme;
+
+struct or_handle {
+    pz_config c;
+    int N, nm, nphi, ncheb, mmax, minc;
+    int *idx2m;
+    int l_heat, l_chem;
+    double CorFac, BuoFac, TdiffFac, ChemFac, XidiffFac, ViscFac, tadvz;
+    double r_cmb, r_icb, ek;
+    double *r, *x, *drx, *ddrx, *or1, *or2, *rgrav, *beta, *ekpump, *ekp_up, *ekp_us,
+        *ekp_dusdp, *height, *oheight, *tcond, *dtcond, *xicond, *dxicond, *delxr2,
+        *delxh2;
+    int *mR;
+    double rnorm, cheb_fac, boundary_fac, thr;
+    fftp *fdct, *fphi;
+    int *der, *der2;
+    double *rMat, *drMat, *d2rMat; /* tau lines: [2][N] */
+    double *hdifT, *hdifXi, *hdifV;
+    cplx *topt, *bott, *topxi, *botxi;
+    /* fields [n_m][n_r] */
+    cplx *us, *up, *om, *psi, *temp, *xi, *temp_hat, *xi_hat, *psi_hat, *dtemp, *dxi,
+        *work, *dVsT, *dVsXi, *dVsOm, *buo;
+    tarray dTdt, dxidt, dpsidt;
+    /* operator matrices */
+    bandr T_RHSE[2], *T_RHSI;
+    bordr *T_LHS, *XI_LHS;
+    double *tempfac, *xifac;
+    bandr P_RHSE[3], *P_RHSI;
+    bandc *P_RHSIL;
+    bordc *P_LHS;
+    double *psifac;
+    struct optabs_s *tabs;
+    tscheme ts;
+    double time;
+    int lMatNext, n_time_step, started;
+    double *dtr, *dth;
+    /* scratch */
+    cplx *w1, *w2, *w3;
+    double *gr[16];
+};
+
+/* ------------------------------------------------------------------------ */
+/* Config defaults: namelists.f90:530-672                                    */
+/* ------------------------------------------------------------------------ */
+void or_config_default(pz_config *c)
+{
+    memset(c, 0, sizeof(*c));
+    c->n_r_max = 32;
+    c->n_cheb_max = 32;
+    c->m_max = 32;
+    c->minc = 1;
+    c->time_scheme = PZ_CNAB2;
+    c->l_cheb_coll = 1;
+    c->l_direct_solve = 1;
+    c->l_coriolis_imp = 1;
+    c->l_buo_imp = 1;
+    c->l_non_rot = 0;
+    c->time_scale_rot = 0;
+    c->l_rerror_fix = 0;
+    c->rerror_fac = 500.0;
+    c->alpha = 0.5;
+    c->dtMax = 1.0e-5;
+    c->courfac = 1.0e3;
+    c->dt_fac = 2.0;
+    c->ra = 1.0e5;
+    c->ek = 1.0e-3;
+    c->pr = 1.0;
+    c->raxi = 0.0;
+    c->sc = 0.0;
+    c->radratio = 0.35;
+    c->g0 = 0.0;
+    c->g1 = 1.0;
+    c->g2 = 0.0;
+    c->ktopt = c->kbott = 1;
+    c->ktopxi = c->kbotxi = 1;
+    c->ktopv = c->kbotv = 2;
+    c->l_ek_pump = 0;
+    c->l_temp_advz = 0;
+    c->tcond_fac = 1.0;
+    c->xicond_fac = 1.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Azimuthal transforms: fourier::fft / ifft (fft_fftw.f90:44-90)             */
+/* ------------------------------------------------------------------------ */
+static void phi_fft(or_handle *h, const double *grid, cplx *spec, int m_R)
+{
+    int n = h->nphi;
+    cplx *a = h->w1, *b = h->w2;
+    for (int j = 0; j < n; j++) a[j] = grid[j];
+    fft_exec(h->fphi, a, b, -1);
+    for (int nm = 0; nm < h->nm; nm++) {
+        if (h->idx2m[nm] <= m_R) spec[nm] = b[nm] / (double)n;
+        else spec[nm] = 0;
+    }
+}
+
+/* c2r semantics: the imaginary part of the zero bin is not used */
+static void phi_ifft(or_handle *h, const cplx *spec, double *grid, int m_R)
+{
+    int n = h->nphi;
+    cplx *a = h->w1, *b = h->w2;
+    for (int k = 0; k < n; k++) a[k] = 0;
+    for (int nm = 0; nm < h->nm; nm++) {
+        cplx v = (h->idx2m[nm] <= m_R) ? spec[nm] : 0;
+        if (nm == 0) {
+            a[0] = creal(v);
+        } else {
+            a[nm] = v;
+            a[n - nm] = conj(v);
+        }
+    }
+    fft_exec(h->fphi, a, b, +1);
+    for (int j = 0; j < n; j++) grid[j] = creal(b[j]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Chebyshev transforms: dct_fftw.f90 (DCT_VERSION = dft_loop)               */
+/* ------------------------------------------------------------------------ */
+
+/* costf_complex_2d for one m (dct_fftw.f90:184-197): DCT-I as the forward
+ * FFT of the even extension, scaled by cheb_fac. */
+static void costf_vec(or_handle *h, cplx *f)
+{
+    int N = h->N, L = 2 * N - 2;
+    cplx *a = h->w1, *b = h->w2;
+    for (int j = 0; j < N; j++) a[j] = f[j];
+    for (int j = N; j < L; j++) a[j] = f[L - j];
+    fft_exec(h->fdct, a, b, -1);
+    for (int k = 0; k < N; k++) f[k] = h->cheb_fac * b[k];
+}
+
+/* costf_real_1d (dct_fftw.f90:231-246): FFTW REDFT00 then cheb_fac */
+static void costf_real(or_handle *h, double *f)
+{
+    int N = h->N, L = 2 * N - 2;
+    cplx *a = h->w1, *b = h->w2;
+    for (int j = 0; j < N; j++) a[j] = f[j];
+    for (int j = N; j < L; j++) a[j] = f[L - j];
+    fft_exec(h->fdct, a, b, -1);
+    for (int k = 0; k < N; k++) f[k] = h->cheb_fac * creal(b[k]);
+}
+
+static void costf_field(or_handle *h, cplx *F)
+{
+    for (int nm = 0; nm < h->nm; nm++) costf_vec(h, F + (size_t)nm * h->N);
+}
+
+/* get_dr_fft (dct_fftw.f90:258-316) followed by the drx map
+ * (radial_derivatives.f90:388-391), one m. */
+static void get_dr_vec(or_handle *h, const cplx *f, cplx *df, int l_dct_in)
+{
+    int N = h->N, L = 2 * N - 2;
+    cplx *a = h->w1, *o = h->w3, *b = h->w2;
+    if (l_dct_in) {
+        for (int j = 0; j < N; j++) a[j] = f[j];
+        for (int j = N; j < L; j++) a[j] = f[L - j];
+        fft_exec(h->fdct, a, o, -1);
+    } else {
+        for (int j = 0; j < N; j++) o[j] = f[j] / h->cheb_fac;
+        for (int j = N; j < L; j++) o[j] = f[L - j] / h->cheb_fac;
+    }
+    o[N - 1] = 0.5 * o[N - 1];
+    cplx tot = 0;
+    for (int k = 1; k <= N - 1; k++) tot += (double)(k * k) * o[k];
+    df[0] = tot / (double)(N - 1);
+    tot = 0;
+    for (int k = 1; k <= N - 1; k++) tot += (double)(((k + 1) % 2 == 0 ? 1 : -1) * k * k) * o[k];
+    df[N - 1] = tot / (double)(N - 1);
+    o[N - 1] = 2.0 * o[N - 1];
+    for (int p = 0; p < L; p++) o[p] = I * (double)h->der[p] * o[p];
+    fft_exec(h->fdct, o, b, +1);
+    for (int j = 1; j < N - 1; j++)
+        df[j] = -b[j] / sqrt(1.0 - h->x[j] * h->x[j]) / (double)(2 * N - 2);
+    for (int j = 0; j < N; j++) df[j] = h->drx[j] * df[j];
+}
+
+/* get_ddr_fft (dct_fftw.f90:318-397) + map (radial_derivatives.f90:544-549) */
+static void get_ddr_vec(or_handle *h, const cplx *f, cplx *df, cplx *ddf, int l_dct_in)
+{
+    int N = h->N, L = 2 * N - 2;
+    cplx *a = h->w1, *t = h->w3, *b = h->w2;
+    cplx *w2 = malloc(sizeof(cplx) * L), *w1 = malloc(sizeof(cplx) * L);
+    if (l_dct_in) {
+        for (int j = 0; j < N; j++) a[j] = f[j];
+        for (int j = N; j < L; j++) a[j] = f[L - j];
+        fft_exec(h->fdct, a, t, -1);
+    } else {
+        for (int j = 0; j < N; j++) t[j] = f[j] / h->cheb_fac;
+        for (int j = N; j < L; j++) t[j] = f[L - j] / h->cheb_fac;
+    }
+    t[N - 1] = 0.5 * t[N - 1];
+    cplx tot = 0;
+    for (int k = 1; k <= N - 1; k++) tot += (double)(k * k) * t[k];
+    df[0] = tot / (double)(N - 1);
+    tot = 0;
+    for (int k = 2; k <= N - 1; k++) tot += (double)((long)k * k * (k * k - 1)) * t[k];
+    ddf[0] = (1.0 / 3.0) * tot / (double)(N - 1);
+    tot = 0;
+    for (int k = 1; k <= N - 1; k++) tot += (double)(((k + 1) % 2 == 0 ? 1 : -1) * k * k) * t[k];
+    df[N - 1] = tot / (double)(N - 1);
+    tot = 0;
+    for (int k = 2; k <= N - 1; k++)
+        tot += (double)((k % 2 == 0 ? 1 : -1) * (long)k * k * (k * k - 1)) * t[k];
+    ddf[N - 1] = (1.0 / 3.0) * tot / (double)(N - 1);
+    t[N - 1] = 2.0 * t[N - 1];
+    for (int p = 0; p < L; p++) w2[p] = I * (double)h->der[p] * t[p];
+    fft_exec(h->fdct, w2, w1, +1);
+    for (int p = 0; p < L; p++) t[p] = -(double)h->der2[p] * t[p];
+    fft_exec(h->fdct, t, w2, +1);
+    for (int j = 1; j < N - 1; j++) {
+        double xj = h->x[j], s = 1.0 - xj * xj;
+        df[j] = -w1[j] / sqrt(s) / (double)(2 * N - 2);
+        ddf[j] = -w1[j] * xj / pow(s, 1.5) / (double)(2 * N - 2) + w2[j] / s / (double)(2 * N - 2);
+    }
+    for (int j = 0; j < N; j++) {
+        ddf[j] = h->ddrx[j] * df[j] + h->drx[j] * h->drx[j] * ddf[j];
+        df[j] = h->drx[j] * df[j];
+    }
+    (void)b;
+    free(w1);
+    free(w2);
+}
+
+/* get_dcheb_real_1d (radial_derivatives.f90:143-185) with the rerror
+ * threshold, used by get_dr_real_1d (:420-485) for u_phi(m=0). */
+static void get_dr_real(or_handle *h, const double *f_in, double *df, int l_dct)
+{
+    int N = h->N;
+    double *f = malloc(sizeof(double) * N);
+    memcpy(f, f_in, sizeof(double) * N);
+    if (l_dct) costf_real(h, f);
+    double mx = 0;
+    for (int j = 0; j < N; j++) mx = fmax(mx, fabs(f[j]));
+    double threshold = mx * h->thr;
+    /* 1-based: df(n_cheb_max..n_r_max)=0 with n_cheb_max = n_r_max */
+    df[N - 1] = 0.0;
+    int nc = N - 1; /* n_cheb = n_cheb_max-1 (1-based) */
+    double fac = (double)nc;
+    if (fabs(f[nc]) >= threshold) df[nc - 1] = fac * f[nc];
+    else df[nc - 1] = 0.0;
+    for (int n = N - 2; n >= 1; n--) {
+        fac = (double)(2 * n);
+        if (fabs(f[n]) >= threshold) df[n - 1] = df[n + 1] + fac * f[n];
+        else df[n - 1] = 0.0;
+    }
+    costf_real(h, df);
+    for (int j = 0; j < N; j++) df[j] = h->drx[j] * df[j];
+    free(f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sparse-Chebyshev operators (chebsparselib.f90): rows of I^J [p(r) D^k]    */
+/* composed from multiplication by r = a x + b and the integration operator  */
+/* in the Chebyshev basis with halved end coefficients.  The folding of       */
+/* negative columns follows mirror_stencil (chebsparselib.f90:997-1020).     */
+/* ------------------------------------------------------------------------ */
+#define LP_OFF 8
+#define LP_LEN 32
+typedef struct { ld c[LP_LEN]; } lpoly; /* Laurent polynomial, exponent e at c[e+LP_OFF] */
+#define DOP_K 7
+typedef struct { lpoly a[DOP_K]; } dop; /* sum_k a_k(r) D^k */
+
+static lpoly lp_zero(void) { lpoly p; memset(&p, 0, sizeof(p)); return p; }
+static lpoly lp_mono(ld c, int e) { lpoly p = lp_zero(); p.c[e + LP_OFF] = c; return p; }
+static lpoly lp_add(lpoly a, lpoly b, ld s)
+{
+    for (int i = 0; i < LP_LEN; i++) a.c[i] += s * b.c[i];
+    return a;
+}
+static lpoly lp_mul(lpoly a, lpoly b)
+{
+    lpoly p = lp_zero();
+    for (int i = 0; i < LP_LEN; i++)
+        if (a.c[i] != 0)
+            for (int j = 0; j < LP_LEN; j++)
+                if (b.c[j] != 0) {
+                    int e = (i - LP_OFF) + (j - LP_OFF) + LP_OFF;
+                    if (e < 0 || e >= LP_LEN) { fprintf(stderr, "lpoly overflow\n"); abort(); }
+                    p.c[e] += a.c[i] * b.c[j];
+                }
+    return p;
+}
+static lpoly lp_der(lpoly a)
+{
+    lpoly p = lp_zero();
+    for (int i = 0; i < LP_LEN; i++) {
+        int e = i - LP_OFF;
+        if (a.c[i] != 0 && e != 0) p.c[i - 1] += (ld)e * a.c[i];
+    }
+    return p;
+}
+static dop dop_zero(void) { dop d; memset(&d, 0, sizeof(d)); return d; }
+/* (A o B) = sum_k a_k D^k sum_l b_l D^l = sum a_k C(k,i) b_l^(i) D^(k-i+l) */
+static dop dop_compose(const dop *A, const dop *B)
+{
+    dop R = dop_zero();
+    for (int k = 0; k < DOP_K; k++) {
+        int nz = 0;
+        for (int i = 0; i < LP_LEN; i++) nz |= A->a[k].c[i] != 0;
+        if (!nz) continue;
+        for (int l = 0; l < DOP_K; l++) {
+            int nzb = 0;
+            for (int i = 0; i < LP_LEN; i++) nzb |= B->a[l].c[i] != 0;
+            if (!nzb) continue;
+            lpoly bl = B->a[l];
+            ld binom = 1;
+            for (int i = 0; i <= k; i++) {
+                if (i > 0) { binom = binom * (ld)(k - i + 1) / (ld)i; bl = lp_der(bl); }
+                int o = k - i + l;
+                if (o >= DOP_K) { fprintf(stderr, "dop order overflow\n"); abort(); }
+                R.a[o] = lp_add(R.a[o], lp_mul(A->a[k], bl), binom);
+            }
+        }
+    }
+    return R;
+}
+static dop dop_mulp(lpoly p, const dop *A)
+{
+    dop R = dop_zero();
+    for (int k = 0; k < DOP_K; k++) R.a[k] = lp_mul(p, A->a[k]);
+    return R;
+}
+
+typedef struct { int E, L; ld a, b; ld *buf[24]; } seqws; /* sequences on [-E, L) */
+
+static void seq_mulr(const seqws *w, const ld *y, ld *o)
+{
+    int T = w->E + w->L;
+    for (int i = 0; i < T; i++) {
+        ld ym = i > 0 ? y[i - 1] : 0, yp = i + 1 < T ? y[i + 1] : 0;
+        o[i] = w->a * (ym + yp) / 2 + w->b * y[i];
+    }
+}
+static void seq_integ(const seqws *w, const ld *y, ld *o)
+{
+    int T = w->E + w->L;
+    for (int i = 0; i < T; i++) {
+        int n = i - w->E;
+        if (n == 0) { o[i] = 0; continue; }
+        ld ym = i > 0 ? y[i - 1] : 0, yp = i + 1 < T ? y[i + 1] : 0;
+        o[i] = w->a * (ym - yp) / (2 * (ld)n);
+    }
+}
+/* o = I^j [ p(r) y ], p a polynomial (no negative powers) */
+static void seq_Ip(const seqws *w, int j, const lpoly *p, const ld *y, ld *o, int depth)
+{
+    int T = w->E + w->L;
+    ld *acc = w->buf[2 * depth], *tmp = w->buf[2 * depth + 1];
+    int deg = -1;
+    for (int e = 0; e < LP_LEN - LP_OFF; e++)
+        if (p->c[e + LP_OFF] != 0) deg = e;
+    for (int e = -LP_OFF; e < 0; e++)
+        if (p->c[e + LP_OFF] != 0) { fprintf(stderr, "negative power in operator\n"); abort(); }
+    for (int i = 0; i < T; i++) acc[i] = 0;
+    if (deg >= 0) {
+        for (int i = 0; i < T; i++) acc[i] = p->c[deg + LP_OFF] * y[i];
+        for (int e = deg - 1; e >= 0; e--) {
+            seq_mulr(w, acc, tmp);
+            for (int i = 0; i < T; i++) acc[i] = tmp[i] + p->c[e + LP_OFF] * y[i];
+        }
+    }
+    for (int q = 0; q < j; q++) {
+        seq_integ(w, acc, tmp);
+        memcpy(acc, tmp, sizeof(ld) * T);
+    }
+    memcpy(o, acc, sizeof(ld) * T);
+}
+/* o = I^j [ p D^k ] y via I(p D g) = p g - I(p' g) */
+static void seq_term(const seqws *w, int j, lpoly p, int k, const ld *y, ld *o, int depth)
+{
+    int T = w->E + w->L;
+    if (k == 0) { seq_Ip(w, j, &p, y, o, depth); return; }
+    if (j < k) { fprintf(stderr, "operator not integrable\n"); abort(); }
+    ld *t1 = w->buf[2 * depth + 12], *t2 = w->buf[2 * depth + 13];
+    seq_term(w, j - 1, p, k - 1, y, t1, depth + 1);
+    memcpy(o, t1, sizeof(ld) * T);
+    seq_term(w, j, lp_der(p), k - 1, y, t2, depth + 1);
+    for (int i = 0; i < T; i++) o[i] -= t2[i];
+}
+
+/*
+ * ext[n*(2K+1) + (K+d)] = row n, column n+d of I^J[op] on the extended
+ * (unfolded) index set, rows n in [0, N), d in [-K, K].
+ */
+static void op_ext(const dop *op, int J, int N, int K, ld a, ld b, ld *ext)
+{
+    seqws w;
+    w.E = K + 24;
+    w.L = N + K + 24;
+    w.a = a;
+    w.b = b;
+    int T = w.E + w.L;
+    for (int i = 0; i < 24; i++) w.buf[i] = calloc(T, sizeof(ld));
+    ld *e = calloc(T, sizeof(ld)), *o = calloc(T, sizeof(ld)), *acc = calloc(T, sizeof(ld));
+    memset(ext, 0, sizeof(ld) * (size_t)N * (2 * K + 1));
+    for (int col = -K; col < N + K; col++) {
+        memset(e, 0, sizeof(ld) * T);
+        e[col + w.E] = 1;
+        memset(acc, 0, sizeof(ld) * T);
+        for (int k = 0; k < DOP_K; k++) {
+            int nz = 0;
+            for (int i = 0; i < LP_LEN; i++) nz |= op->a[k].c[i] != 0;
+            if (!nz) continue;
+            seq_term(&w, J, op->a[k], k, e, o, 0);
+            for (int i = 0; i < T; i++) acc[i] += o[i];
+        }
+        for (int n = 0; n < N; n++) {
+            int d = col - n;
+            if (d < -K || d > K) continue;
+            ext[(size_t)n * (2 * K + 1) + (K + d)] = acc[n + w.E];
+        }
+    }
+    for (int i = 0; i < 24; i++) free(w.buf[i]);
+    free(e);
+    free(o);
+    free(acc);
+}
+
+/* reference stencil order (n_band = 1..2K+1 <-> column idx+K+1-n_band),
+ * folded as mirror_stencil does; the unfolded negative-column entries stay */
+static void ext_to_stencil(const ld *ext_row, int idx, int K, ld *st)
+{
+    for (int nb = 0; nb < 2 * K + 1; nb++) {
+        int d = K - nb; /* column = idx + d */
+        st[nb] = ext_row[K + d];
+    }
+    if (idx < K) {
+        for (int n = 1; n <= K - idx; n++) {
+            /* column n <- column -n */
+            int nb_pos = K - (n - idx), nb_neg = K - (-n - idx);
+            st[nb_pos] += st[nb_neg];
+        }
+    }
+}
+
+/* Operator library: I2R2, I2R1, I2R2LAPL, R2, I1R1, I4R4, I4R3, LAPLROT, LAPLROT2 */
+typedef struct { int J; dop op; } opdef;
+
+static dop dop_poly(lpoly p) { dop d = dop_zero(); d.a[0] = p; return d; }
+
+/* r^2 Delta_m = r^2 D^2 + r D - m^2 (split in m-independent / m^2 parts) */
+static void def_lapl(dop *d0, dop *d2)
+{
+    *d0 = dop_zero();
+    *d2 = dop_zero();
+    d0->a[2] = lp_mono(1, 2);
+    d0->a[1] = lp_mono(1, 1);
+    d2->a[0] = lp_mono(-1, 0);
+}
+
+/* omega = L_rot psi = -h^2 psi'' - (rc^2/r - 6 r) psi' + (6 + m^2 h^2 / r^2) psi,
+ * h^2 = rc^2 - r^2 (from the u_s, u_phi, omega reconstruction of
+ * update_psi_integ_smat.f90:504-523). Returned as Lr0 + m^2 Lr2. */
+static void def_lrot(ld rc, dop *L0, dop *L2)
+{
+    lpoly h2 = lp_add(lp_mono(rc * rc, 0), lp_mono(1, 2), -1);
+    *L0 = dop_zero();
+    *L2 = dop_zero();
+    L0->a[2] = lp_add(lp_zero(), h2, -1);
+    L0->a[1] = lp_add(lp_mono(-rc * rc, -1), lp_mono(6, 1), 1);
+    L0->a[0] = lp_mono(6, 0);
+    L2->a[0] = lp_mul(h2, lp_mono(1, -2));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Band / bordered LU with LAPACK semantics (dgbtf2/zgbtf2, dgbtrs/zgbtrs,   */
+/* dgetf2, dgemm/dgemv) as called by algebra_lapack.f90:176-416.             */
+/* LU storage: ab[(kl+ku+i-j) + j*ld], ld = 2kl+ku+1, 0-based i, j.          */
+/* ------------------------------------------------------------------------ */
+static void gbtrf_r(int n, int kl, int ku, double *ab, int *ipiv, int *info)
+{
+    int ld = 2 * kl + ku + 1, kv = ku + kl;
+#define AB(i, j) ab[(i) + (size_t)(j) * ld] /* 0-based storage row i, col j */
+    *info = 0;
+    for (int j = ku + 1; j < (kv < n ? kv : n); j++)
+        for (int i = kv - j; i < kl; i++) AB(i, j) = 0;
+    int ju = 0;
+    for (int j = 0; j < n; j++) {
+        if (j + kv < n)
+            for (int i = 0; i < kl; i++) AB(i, j + kv) = 0;
+        int km = kl < n - 1 - j ? kl : n - 1 - j;
+        int jp = 0;
+        double mx = fabs(AB(kv, j));
+        for (int i = 1; i <= km; i++)
+            if (fabs(AB(kv + i, j)) > mx) { mx = fabs(AB(kv + i, j)); jp = i; }
+        ipiv[j] = jp + j;
+        if (AB(kv + jp, j) != 0) {
+            int t = j + ku + jp;
+            if (t > n - 1) t = n - 1;
+            if (t > ju) ju = t;
+            if (jp != 0)
+                for (int c = 0; c <= ju - j; c++) {
+                    double tmp = AB(kv + jp - c, j + c);
+                    AB(kv + jp - c, j + c) = AB(kv - c, j + c);
+                    AB(kv - c, j + c) = tmp;
+                }
+            if (km > 0) {
+                double rp = 1.0 / AB(kv, j);
+                for (int i = 1; i <= km; i++) AB(kv + i, j) *= rp;
+                for (int c = 1; c <= ju - j; c++) {
+                    double temp = -AB(kv - c, j + c);
+                    if (temp != 0)
+                        for (int i = 1; i <= km; i++) AB(kv + i - c, j + c) += AB(kv + i, j) * temp;
+                }
+            }
+        } else if (*info == 0) {
+            *info = j + 1;
+        }
+    }
+#undef AB
+}
+
+static inline double cabs1(cplx z) { return fabs(creal(z)) + fabs(cimag(z)); }
+
+static void gbtrf_c(int n, int kl, int ku, cplx *ab, int *ipiv, int *info)
+{
+    int ld = 2 * kl + ku + 1, kv = ku + kl;
+#define AB(i, j) ab[(i) + (size_t)(j) * ld]
+    *info = 0;
+    for (int j = ku + 1; j < (kv < n ? kv : n); j++)
+        for (int i = kv - j; i < kl; i++) AB(i, j) = 0;
+    int ju = 0;
+    for (int j = 0; j < n; j++) {
+        if (j + kv < n)
+            for (int i = 0; i < kl; i++) AB(i, j + kv) = 0;
+        int km = kl < n - 1 - j ? kl : n - 1 - j;
+        int jp = 0;
+        double mx = cabs1(AB(kv, j));
+        for (int i = 1; i <= km; i++)
+            if (cabs1(AB(kv + i, j)) > mx) { mx = cabs1(AB(kv + i, j)); jp = i; }
+        ipiv[j] = jp + j;
+        if (AB(kv + jp, j) != 0) {
+            int t = j + ku + jp;
+            if (t > n - 1) t = n - 1;
+            if (t > ju) ju = t;
+            if (jp != 0)
+                for (int c = 0; c <= ju - j; c++) {
+                    cplx tmp = AB(kv + jp - c, j + c);
+                    AB(kv + jp - c, j + c) = AB(kv - c, j + c);
+                    AB(kv - c, j + c) = tmp;
+                }
+            if (km > 0) {
+                cplx rp = 1.0 / AB(kv, j);
+                for (int i = 1; i <= km; i++) AB(kv + i, j) *= rp;
+                for (int c = 1; c <= ju - j; c++) {
+                    cplx temp = -AB(kv - c, j + c);
+                    if (temp != 0)
+                        for (int i = 1; i <= km; i++) AB(kv + i - c, j + c) += AB(kv + i, j) * temp;
+                }
+            }
+        } else if (*info == 0) {
+            *info = j + 1;
+        }
+    }
+#undef AB
+}
+
+static void gbtrs_r(int n, int kl, int ku, const double *ab, const int *ipiv, double *b)
+{
+    int ld = 2 * kl + ku + 1, kd = ku + kl; /* diag row of U */
+    if (kl > 0)
+        for (int j = 0; j < n - 1; j++) {
+            int lm = kl < n - 1 - j ? kl : n - 1 - j;
+            int l = ipiv[j];
+            if (l != j) { double t = b[l]; b[l] = b[j]; b[j] = t; }
+            double bj = -b[j];
+            for (int i = 1; i <= lm; i++) b[j + i] += ab[kd + i + (size_t)j * ld] * bj;
+        }
+    int k = kl + ku;
+    for (int j = n - 1; j >= 0; j--) {
+        if (b[j] != 0) {
+            b[j] = b[j] / ab[kd + (size_t)j * ld];
+            double temp = b[j];
+            int i0 = j - k > 0 ? j - k : 0;
+            for (int i = j - 1; i >= i0; i--) b[i] -= temp * ab[kd + i - j + (size_t)j * ld];
+        }
+    }
+}
+
+static void gbtrs_c(int n, int kl, int ku, const cplx *ab, const int *ipiv, cplx *b)
+{
+    int ld = 2 * kl + ku + 1, kd = ku + kl;
+    if (kl > 0)
+        for (int j = 0; j < n - 1; j++) {
+            int lm = kl < n - 1 - j ? kl : n - 1 - j;
+            int l = ipiv[j];
+            if (l != j) { cplx t = b[l]; b[l] = b[j]; b[j] = t; }
+            cplx bj = -b[j];
+            for (int i = 1; i <= lm; i++) b[j + i] += ab[kd + i + (size_t)j * ld] * bj;
+        }
+    int k = kl + ku;
+    for (int j = n - 1; j >= 0; j--) {
+        if (b[j] != 0) {
+            b[j] = b[j] / ab[kd + (size_t)j * ld];
+            cplx temp = b[j];
+            int i0 = j - k > 0 ? j - k : 0;
+            for (int i = j - 1; i >= i0; i--) b[i] -= temp * ab[kd + i - j + (size_t)j * ld];
+        }
+    }
+}
+
+/* small dense LU, column major a[i + j*n], partial pivoting */
+static void getrf_r(int n, double *a, int *piv, int *info)
+{
+    *info = 0;
+    for (int j = 0; j < n; j++) {
+        int p = j;
+        for (int i = j + 1; i < n; i++)
+            if (fabs(a[i + j * n]) > fabs(a[p + j * n])) p = i;
+        piv[j] = p;
+        if (a[p + j * n] != 0) {
+            if (p != j)
+                for (int c = 0; c < n; c++) { double t = a[j + c * n]; a[j + c * n] = a[p + c * n]; a[p + c * n] = t; }
+            double rp = 1.0 / a[j + j * n];
+            for (int i = j + 1; i < n; i++) a[i + j * n] *= rp;
+        } else if (*info == 0) *info = j + 1;
+        for (int c = j + 1; c < n; c++) {
+            double t = -a[j + c * n];
+            for (int i = j + 1; i < n; i++) a[i + c * n] += a[i + j * n] * t;
+        }
+    }
+}
+static void getrf_c(int n, cplx *a, int *piv, int *info)
+{
+    *info = 0;
+    for (int j = 0; j < n; j++) {
+        int p = j;
+        for (int i = j + 1; i < n; i++)
+            if (cabs1(a[i + j * n]) > cabs1(a[p + j * n])) p = i;
+        piv[j] = p;
+        if (a[p + j * n] != 0) {
+            if (p != j)
+                for (int c = 0; c < n; c++) { cplx t = a[j + c * n]; a[j + c * n] = a[p + c * n]; a[p + c * n] = t; }
+            cplx rp = 1.0 / a[j + j * n];
+            for (int i = j + 1; i < n; i++) a[i + j * n] *= rp;
+        } else if (*info == 0) *info = j + 1;
+        for (int c = j + 1; c < n; c++) {
+            cplx t = -a[j + c * n];
+            for (int i = j + 1; i < n; i++) a[i + c * n] += a[i + j * n] * t;
+        }
+    }
+}
+static void getrs_r(int n, const double *a, const int *piv, double *b)
+{
+    for (int i = 0; i < n; i++)
+        if (piv[i] != i) { double t = b[i]; b[i] = b[piv[i]]; b[piv[i]] = t; }
+    for (int j = 0; j < n; j++)
+        for (int i = j + 1; i < n; i++) b[i] -= b[j] * a[i + j * n];
+    for (int j = n - 1; j >= 0; j--) {
+        b[j] /= a[j + j * n];
+        for (int i = 0; i < j; i++) b[i] -= b[j] * a[i + j * n];
+    }
+}
+static void getrs_c(int n, const cplx *a, const int *piv, cplx *b)
+{
+    for (int i = 0; i < n; i++)
+        if (piv[i] != i) { cplx t = b[i]; b[i] = b[piv[i]]; b[piv[i]] = t; }
+    for (int j = 0; j < n; j++)
+        for (int i = j + 1; i < n; i++) b[i] -= b[j] * a[i + j * n];
+    for (int j = n - 1; j >= 0; j--) {
+        b[j] /= a[j + j * n];
+        for (int i = 0; i < j; i++) b[i] -= b[j] * a[i + j * n];
+    }
+}
+
+/* prepare_bordered_mat_real (algebra_lapack.f90:216-250) */
+static int bord_prep_r(bordr *B)
+{
+    int info, nt = B->ntau, nb = B->nb;
+    gbtrf_r(nb, B->kl, B->ku, B->A4, B->p4, &info);
+    if (info) return info;
+    for (int c = 0; c < nt; c++) gbtrs_r(nb, B->kl, B->ku, B->A4, B->p4, B->A3 + (size_t)c * nb);
+    /* A1 <- A1 - A2*A3 (dgemm loop order: j, l, i) */
+    for (int j = 0; j < nt; j++)
+        for (int l = 0; l < nb; l++) {
+            double temp = -B->A3[l + (size_t)j * nb];
+            if (temp != 0)
+                for (int i = 0; i < nt; i++) B->A1[i + j * nt] += temp * B->A2[i + (size_t)l * nt];
+        }
+    getrf_r(nt, B->A1, B->p1, &info);
+    return info;
+}
+static int bord_prep_c(bordc *B)
+{
+    int info, nt = B->ntau, nb = B->nb;
+    gbtrf_c(nb, B->kl, B->ku, B->A4, B->p4, &info);
+    if (info) return info;
+    for (int c = 0; c < nt; c++) gbtrs_c(nb, B->kl, B->ku, B->A4, B->p4, B->A3 + (size_t)c * nb);
+    for (int j = 0; j < nt; j++)
+        for (int l = 0; l < nb; l++) {
+            cplx temp = -B->A3[l + (size_t)j * nb];
+            if (temp != 0)
+                for (int i = 0; i < nt; i++) B->A1[i + j * nt] += temp * B->A2[i + (size_t)l * nt];
+        }
+    getrf_c(nt, B->A1, B->p1, &info);
+    return info;
+}
+/* solve_bordered_mat_real (algebra_lapack.f90:332-373) */
+static void bord_solve_r(const bordr *B, double *rhs)
+{
+    int nt = B->ntau, nb = B->nb;
+    double *r2 = rhs + nt;
+    gbtrs_r(nb, B->kl, B->ku, B->A4, B->p4, r2);
+    for (int j = 0; j < nb; j++) { /* dgemv N: rhs1 -= A2*r2 */
+        double temp = -r2[j];
+        if (temp != 0)
+            for (int i = 0; i < nt; i++) rhs[i] += temp * B->A2[i + (size_t)j * nt];
+    }
+    getrs_r(nt, B->A1, B->p1, rhs);
+    for (int j = 0; j < nt; j++) {
+        double temp = -rhs[j];
+        if (temp != 0)
+            for (int i = 0; i < nb; i++) r2[i] += temp * B->A3[i + (size_t)j * nb];
+    }
+}
+static void bord_solve_c(const bordc *B, cplx *rhs)
+{
+    int nt = B->ntau, nb = B->nb;
+    cplx *r2 = rhs + nt;
+    gbtrs_c(nb, B->kl, B->ku, B->A4, B->p4, r2);
+    for (int j = 0; j < nb; j++) {
+        cplx temp = -r2[j];
+        if (temp != 0)
+            for (int i = 0; i < nt; i++) rhs[i] += temp * B->A2[i + (size_t)j * nt];
+    }
+    getrs_c(nt, B->A1, B->p1, rhs);
+    for (int j = 0; j < nt; j++) {
+        cplx temp = -rhs[j];
+        if (temp != 0)
+            for (int i = 0; i < nb; i++) r2[i] += temp * B->A3[i + (size_t)j * nb];
+    }
+}
+
+/* dgbmv 'N' with alpha=1, beta=0 on re and im separately
+ * (band_matrix.f90:263-294) */
+static void band_mv_r(const bandr *B, cplx *v)
+{
+    int n = B->n, ld = B->kl + B->ku + 1;
+    double *xr = malloc(sizeof(double) * n * 4), *xi = xr + n, *yr = xr + 2 * n, *yi = xr + 3 * n;
+    for (int i = 0; i < n; i++) { xr[i] = creal(v[i]); xi[i] = cimag(v[i]); yr[i] = 0; yi[i] = 0; }
+    for (int j = 0; j < n; j++) {
+        double tr = xr[j], ti = xi[j];
+        int i0 = j - B->ku > 0 ? j - B->ku : 0, i1 = j + B->kl < n - 1 ? j + B->kl : n - 1;
+        if (tr != 0)
+            for (int i = i0; i <= i1; i++) yr[i] += tr * B->d[B->ku + i - j + (size_t)j * ld];
+        if (ti != 0)
+            for (int i = i0; i <= i1; i++) yi[i] += ti * B->d[B->ku + i - j + (size_t)j * ld];
+    }
+    for (int i = 0; i < n; i++) v[i] = yr[i] + I * yi[i];
+    free(xr);
+}
+/* zgbmv (band_matrix.f90:345-366) */
+static void band_mv_c(const bandc *B, cplx *v)
+{
+    int n = B->n, ld = B->kl + B->ku + 1;
+    cplx *x = malloc(sizeof(cplx) * n);
+    memcpy(x, v, sizeof(cplx) * n);
+    for (int i = 0; i < n; i++) v[i] = 0;
+    for (int j = 0; j < n; j++) {
+        cplx t = x[j];
+        if (t == 0) continue;
+        int i0 = j - B->ku > 0 ? j - B->ku : 0, i1 = j + B->kl < n - 1 ? j + B->kl : n - 1;
+        for (int i = i0; i <= i1; i++) v[i] += t * B->d[B->ku + i - j + (size_t)j * ld];
+    }
+    free(x);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Matrix assembly                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Fills rows i_r = n_r + nbounds of an RHS band matrix from stencils
+ * (get_rhs_exp_mat, update_temp_integ.f90:778-812): row idx (0-based) =
+ * i_r-1 holds rnorm*stencil; rows < nbounds stay zero. */
+static void fill_rhs_band_r(or_handle *h, bandr *B, int nbounds, const ld *ext, int K)
+{
+    int N = h->N, ld_ = B->kl + B->ku + 1;
+    ld *st = malloc(sizeof(ld) * (2 * K + 1));
+    memset(B->d, 0, sizeof(double) * (size_t)ld_ * N);
+    for (int idx = nbounds; idx < N; idx++) {
+        ext_to_stencil(ext + (size_t)idx * (2 * K + 1), idx, K, st);
+        for (int nb = 0; nb < 2 * K + 1; nb++) {
+            int col = idx + K - nb;
+            if (col < 0 || col >= N) continue;
+            B->d[B->ku + idx - col + (size_t)col * ld_] = h->rnorm * (double)st[nb];
+        }
+    }
+    free(st);
+}
+
+static void band_alloc_r(bandr *B, int kl, int ku, int n)
+{
+    B->kl = kl; B->ku = ku; B->n = n;
+    B->d = calloc((size_t)(kl + ku + 1) * n, sizeof(double));
+}
+static void band_alloc_c(bandc *B, int kl, int ku, int n)
+{
+    B->kl = kl; B->ku = ku; B->n = n;
+    B->d = calloc((size_t)(kl + ku + 1) * n, sizeof(cplx));
+}
+static void bord_alloc_r(bordr *B, int kl, int ku, int ntau, int n)
+{
+    B->kl = kl; B->ku = ku; B->ntau = ntau; B->nb = n - ntau;
+    B->A1 = calloc(ntau * ntau, sizeof(double));
+    B->A2 = calloc((size_t)ntau * B->nb, sizeof(double));
+    B->A3 = calloc((size_t)ntau * B->nb, sizeof(double));
+    B->A4 = calloc((size_t)(2 * kl + ku + 1) * B->nb, sizeof(double));
+    B->p1 = calloc(ntau, sizeof(int));
+    B->p4 = calloc(B->nb, sizeof(int));
+}
+static void bord_alloc_c(bordc *B, int kl, int ku, int ntau, int n)
+{
+    B->kl = kl; B->ku = ku; B->ntau = ntau; B->nb = n - ntau;
+    B->A1 = calloc(ntau * ntau, sizeof(cplx));
+    B->A2 = calloc((size_t)ntau * B->nb, sizeof(cplx));
+    B->A3 = calloc((size_t)ntau * B->nb, sizeof(cplx));
+    B->A4 = calloc((size_t)(2 * kl + ku + 1) * B->nb, sizeof(cplx));
+    B->p1 = calloc(ntau, sizeof(int));
+    B->p4 = calloc(B->nb, sizeof(int));
+}
+
+/* ext tables of the m-independent operator parts */
+typedef struct {
+    int K;
+    ld *e;
+} exttab;
+
+typedef struct optabs_s {
+    exttab I2R2, I2R1, L2_0, L2_2, R2, I1R1, I4R4, I4R3, LR_0, LR_2, LR2_0, LR2_2, LR2_4;
+} optabs;
+
+static void make_ext(or_handle *h, exttab *t, const dop *op, int J, int K)
+{
+    ld a = 0.5L * ((ld)h->r_cmb - (ld)h->r_icb), b = 0.5L * ((ld)h->r_cmb + (ld)h->r_icb);
+    t->K = K;
+    t->e = malloc(sizeof(ld) * (size_t)h->N * (2 * K + 1));
+    op_ext(op, J, h->N, K, a, b, t->e);
+}
+
+static optabs *build_optabs(or_handle *h)
+{
+    optabs *T = calloc(1, sizeof(optabs));
+    dop d;
+    d = dop_poly(lp_mono(1, 2)); make_ext(h, &T->I2R2, &d, 2, 4);
+    d = dop_poly(lp_mono(1, 1)); make_ext(h, &T->I2R1, &d, 2, 3);
+    dop l0, l2;
+    def_lapl(&l0, &l2);
+    make_ext(h, &T->L2_0, &l0, 2, 2);
+    make_ext(h, &T->L2_2, &l2, 2, 2);
+    d = dop_poly(lp_mono(1, 2)); make_ext(h, &T->R2, &d, 0, 2);
+    d = dop_poly(lp_mono(1, 1)); make_ext(h, &T->I1R1, &d, 1, 2);
+    d = dop_poly(lp_mono(1, 4)); make_ext(h, &T->I4R4, &d, 4, 8);
+    d = dop_poly(lp_mono(1, 3)); make_ext(h, &T->I4R3, &d, 4, 7);
+    dop L0, L2;
+    def_lrot((ld)h->r_cmb, &L0, &L2);
+    dop r4L0 = dop_mulp(lp_mono(1, 4), &L0), r4L2 = dop_mulp(lp_mono(1, 4), &L2);
+    make_ext(h, &T->LR_0, &r4L0, 4, 8);
+    make_ext(h, &T->LR_2, &r4L2, 4, 8);
+    /* r^4 Delta_m L_rot = (r^4 D^2 + r^3 D) L_rot - m^2 r^2 L_rot */
+    dop Dm = dop_zero();
+    Dm.a[2] = lp_mono(1, 4);
+    Dm.a[1] = lp_mono(1, 3);
+    dop A0 = dop_compose(&Dm, &L0), A2 = dop_compose(&Dm, &L2);
+    dop B2 = dop_mulp(lp_mono(-1, 2), &L0), B4 = dop_mulp(lp_mono(-1, 2), &L2);
+    dop C2 = A2;
+    for (int k = 0; k < DOP_K; k++) C2.a[k] = lp_add(A2.a[k], B2.a[k], 1);
+    make_ext(h, &T->LR2_0, &A0, 4, 8);
+    make_ext(h, &T->LR2_2, &C2, 4, 8);
+    make_ext(h, &T->LR2_4, &B4, 4, 8);
+    return T;
+}
+
+static void free_optabs(optabs *T)
+{
+    exttab *list[] = {&T->I2R2, &T->I2R1, &T->L2_0, &T->L2_2, &T->R2, &T->I1R1, &T->I4R4,
+                      &T->I4R3, &T->LR_0, &T->LR_2, &T->LR2_0, &T->LR2_2, &T->LR2_4};
+    for (size_t i = 0; i < sizeof(list) / sizeof(list[0]); i++) free(list[i]->e);
+    free(T);
+}
+
+/* stencil (reference order, length 2K+1) of a combination sum_i c_i * tab_i
+ * embedded at half-bandwidth K */
+static void comb_stencil(int idx, int K, int nt, const exttab **tabs, const ld *coef, ld *st)
+{
+    ld *row = calloc(2 * K + 1, sizeof(ld));
+    for (int t = 0; t < nt; t++) {
+        const exttab *E = tabs[t];
+        for (int d = -E->K; d <= E->K; d++) row[K + d] += coef[t] * E->e[(size_t)idx * (2 * E->K + 1) + E->K + d];
+    }
+    ext_to_stencil(row, idx, K, st);
+    free(row);
+}
+
+/* get_lhs_mat_tau for T and xi (update_temp_integ.f90:631-776) */
+static int build_lhs_scalar(or_handle *h, optabs *T, bordr *A, double *fac, int m, double wimp,
+                            double diff, int ktop, int kbot)
+{
+    int N = h->N, K = 4, nt = 2, nb = N - 2, ld4 = 2 * A->kl + A->ku + 1;
+    ld st[9];
+    memset(A->A1, 0, sizeof(double) * nt * nt);
+    memset(A->A2, 0, sizeof(double) * nt * nb);
+    memset(A->A3, 0, sizeof(double) * nt * nb);
+    memset(A->A4, 0, sizeof(double) * ld4 * nb);
+    const exttab *tabs[3] = {&T->I2R2, &T->L2_0, &T->L2_2};
+    ld coef[3] = {1, -(ld)wimp * (ld)diff, -(ld)wimp * (ld)diff * (ld)m * (ld)m};
+    for (int n_r = 1; n_r <= nb; n_r++) {
+        int idx = n_r + nt - 1;
+        comb_stencil(idx, K, 3, tabs, coef, st);
+        double mx = 0;
+        for (int nbnd = 1; nbnd <= 2 * K + 1; nbnd++) {
+            double v = (double)st[nbnd - 1];
+            int c = n_r + A->ku + 1 - nbnd;
+            if (c <= nb && c >= 1) A->A4[(A->kl + nbnd - 1) + (size_t)(c - 1) * ld4] = h->rnorm * v;
+            if (fabs(v) > mx) mx = fabs(v);
+        }
+        fac[n_r + nt - 1] = 1.0 / h->rnorm / mx;
+        for (int nbnd = 1; nbnd <= A->kl; nbnd++)
+            if (n_r <= nbnd && n_r + nt - nbnd >= 1)
+                A->A3[(n_r - 1) + (size_t)(n_r + nt - nbnd - 1) * nb] = h->rnorm * (double)st[A->ku + nbnd];
+    }
+    for (int n = 0; n < N; n++) {
+        double v1 = ktop == 1 ? h->rMat[0 * N + n] : h->drMat[0 * N + n];
+        double v2 = kbot == 1 ? h->rMat[1 * N + n] : h->drMat[1 * N + n];
+        if (n < nt) { A->A1[0 + n * nt] = h->rnorm * v1; A->A1[1 + n * nt] = h->rnorm * v2; }
+        else { A->A2[0 + (n - nt) * nt] = h->rnorm * v1; A->A2[1 + (n - nt) * nt] = h->rnorm * v2; }
+    }
+    for (int i = 0; i < nt; i++) A->A1[i] = h->boundary_fac * A->A1[i];
+    for (int i = 0; i < nt; i++) {
+        double mx = 0;
+        for (int c = 0; c < nb; c++) mx = fmax(mx, fabs(A->A2[i + c * nt]));
+        fac[i] = 1.0 / mx;
+    }
+    for (int i = 0; i < nt; i++) {
+        for (int c = 0; c < nt; c++) A->A1[i + c * nt] *= fac[i];
+        for (int c = 0; c < nb; c++) A->A2[i + c * nt] *= fac[i];
+    }
+    for (int n = nt; n < N; n++)
+        for (int c = 0; c < nt; c++) A->A3[(n - nt) + (size_t)c * nb] *= fac[n];
+    for (int n_r = 1; n_r <= nb; n_r++)
+        for (int nbnd = 1; nbnd <= 2 * K + 1; nbnd++) {
+            int c = n_r + A->ku + 1 - nbnd;
+            if (c <= nb && c >= 1) A->A4[(A->kl + nbnd - 1) + (size_t)(c - 1) * ld4] *= fac[n_r + nt - 1];
+        }
+    return bord_prep_r(A);
+}
+
+/* get_lhs_mat_tau for psi (update_psi_integ_smat.f90:1286-1581),
+ * rotating container 'sphere', no full disk */
+static int build_lhs_psi(or_handle *h, optabs *T, bordc *A, double *fac, int m, double wimp)
+{
+    int N = h->N, nt = A->ntau, nb = N - nt, K = A->ku, ld4 = 2 * A->kl + A->ku + 1;
+    double nu = h->ViscFac * h->hdifV[0];
+    ld *st = malloc(sizeof(ld) * (2 * K + 1)), *sc = malloc(sizeof(ld) * (2 * K + 1));
+    memset(A->A1, 0, sizeof(cplx) * nt * nt);
+    memset(A->A2, 0, sizeof(cplx) * nt * nb);
+    memset(A->A3, 0, sizeof(cplx) * nt * nb);
+    memset(A->A4, 0, sizeof(cplx) * ld4 * nb);
+    const exttab *tabsA[4];
+    ld cA[4];
+    int ntA;
+    const exttab *tabC[1] = {&T->I4R4};
+    ld cC[1] = {0};
+    ld M2 = (ld)m * (ld)m;
+    if (m == 0) {
+        tabsA[0] = &T->I2R2; tabsA[1] = &T->R2; tabsA[2] = &T->I1R1;
+        cA[0] = 1; cA[1] = -(ld)wimp * (ld)nu; cA[2] = 3.0L * (ld)wimp * (ld)nu;
+        ntA = 3;
+    } else {
+        /* LAPLROT - wimp*nu*LAPLROT2 */
+        static const exttab *dummy;
+        (void)dummy;
+        tabsA[0] = &T->LR_0; tabsA[1] = &T->LR_2; tabsA[2] = &T->LR2_0; tabsA[3] = &T->LR2_2;
+        cA[0] = 1; cA[1] = M2; cA[2] = -(ld)wimp * (ld)nu; cA[3] = -(ld)wimp * (ld)nu * M2;
+        ntA = 4;
+        if (h->c.l_coriolis_imp) cC[0] = (ld)wimp * (ld)h->CorFac * (ld)m;
+    }
+    for (int n_r = 1; n_r <= nb; n_r++) {
+        int idx = n_r + nt - 1;
+        comb_stencil(idx, K, ntA, tabsA, cA, st);
+        if (m != 0) {
+            /* the m^4 part of LAPLROT2 */
+            ld tmp[17];
+            const exttab *t4[1] = {&T->LR2_4};
+            ld c4[1] = {-(ld)wimp * (ld)nu * M2 * M2};
+            comb_stencil(idx, K, 1, t4, c4, tmp);
+            for (int i = 0; i < 2 * K + 1; i++) st[i] += tmp[i];
+            comb_stencil(idx, K, 1, tabC, cC, sc);
+        } else {
+            for (int i = 0; i < 2 * K + 1; i++) sc[i] = 0;
+        }
+        double mx = 0;
+        for (int nbnd = 1; nbnd <= 2 * K + 1; nbnd++) {
+            cplx v = (double)st[nbnd - 1] + I * (double)sc[nbnd - 1];
+            int c = n_r + A->ku + 1 - nbnd;
+            if (c <= nb && c >= 1) A->A4[(A->kl + nbnd - 1) + (size_t)(c - 1) * ld4] = h->rnorm * v;
+            if (cabs(v) > mx) mx = cabs(v);
+        }
+        fac[n_r + nt - 1] = 1.0 / h->rnorm / mx;
+        for (int nbnd = 1; nbnd <= A->kl; nbnd++)
+            if (n_r <= nbnd && n_r + nt - nbnd >= 1)
+                A->A3[(n_r - 1) + (size_t)(n_r + nt - nbnd - 1) * nb] =
+                    h->rnorm * ((double)st[A->ku + nbnd] + I * (double)sc[A->ku + nbnd]);
+    }
+    /* tau lines */
+    for (int n = 0; n < N; n++) {
+        double dn2 = (double)n * (double)n;
+        double d3top = 8.0 / 15.0 * dn2 * (dn2 - 1.0) * (dn2 - 4.0);
+        double v[4];
+        int nv;
+        if (m == 0) {
+            v[0] = h->c.ktopv == 1 ? (h->drMat[n] - h->or1[0] * h->rMat[n]) : h->rMat[n];
+            v[1] = h->c.kbotv == 1 ? (h->drMat[N + n] - h->or1[N - 1] * h->rMat[N + n]) : h->rMat[N + n];
+            nv = 2;
+        } else {
+            v[0] = h->rMat[n];
+            v[1] = h->rMat[N + n];
+            v[2] = h->c.ktopv == 1 ? (h->d2rMat[n] - h->or1[0] * h->drMat[n])
+                                   : (h->c.l_non_rot ? h->drMat[n] : d3top);
+            v[3] = h->c.kbotv == 1 ? (h->d2rMat[N + n] - h->or1[N - 1] * h->drMat[N + n]) : h->drMat[N + n];
+            nv = 4;
+        }
+        for (int i = 0; i < nv; i++) {
+            if (n < nt) A->A1[i + n * nt] = h->rnorm * v[i];
+            else A->A2[i + (n - nt) * nt] = h->rnorm * v[i];
+        }
+    }
+    for (int i = 0; i < nt; i++) A->A1[i] = h->boundary_fac * A->A1[i];
+    for (int i = 0; i < nt; i++) {
+        double mx = 0;
+        for (int c = 0; c < nb; c++) mx = fmax(mx, cabs(A->A2[i + c * nt]));
+        fac[i] = 1.0 / mx;
+    }
+    for (int i = 0; i < nt; i++) {
+        for (int c = 0; c < nt; c++) A->A1[i + c * nt] *= fac[i];
+        for (int c = 0; c < nb; c++) A->A2[i + c * nt] *= fac[i];
+    }
+    for (int n = nt; n < N; n++)
+        for (int c = 0; c < nt; c++) A->A3[(n - nt) + (size_t)c * nb] *= fac[n];
+    for (int n_r = 1; n_r <= nb; n_r++)
+        for (int nbnd = 1; nbnd <= 2 * K + 1; nbnd++) {
+            int c = n_r + A->ku + 1 - nbnd;
+            if (c <= nb && c >= 1) A->A4[(A->kl + nbnd - 1) + (size_t)(c - 1) * ld4] *= fac[n_r + nt - 1];
+        }
+    free(st);
+    free(sc);
+    return bord_prep_c(A);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Time schemes: multistep_schemes.f90, dirk_schemes.f90                    */
+/* ------------------------------------------------------------------------ */
+static int ts_init(tscheme *t, int scheme, double courfac_nml)
+{
+    memset(t, 0, sizeof(*t));
+    t->scheme = scheme;
+    double cf;
+    if (scheme == PZ_CNAB2) { /* multistep_schemes.f90:40-61 */
+        t->family = 0;
+        t->nstages = 1; t->istage = 1;
+        t->nimp = 1; t->nold = 1; t->nexp = 2;
+        t->l_exp_calc[0] = 1; t->l_imp_calc_rhs[0] = 1;
+        cf = 3.0;
+    } else {
+        t->family = 1;
+        t->nold = 1;
+        double g;
+        switch (scheme) {
+        case PZ_ARS222: /* dirk_schemes.f90:55-62, 287-299 */
+            t->nimp = t->nexp = t->nstages = 2;
+            cf = 1.35;
+            g = 0.5 * (2.0 - sqrt(2.0));
+            {
+                double del = 1.0 - 0.5 / g;
+                double bi[3][3] = {{0, 0, 0}, {0, g, 0}, {0, 1 - g, g}};
+                double be[3][3] = {{0, 0, 0}, {g, 0, 0}, {del, 1 - del, 0}};
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) { t->bimp0[i][j] = bi[i][j]; t->bexp0[i][j] = be[i][j]; }
+                t->wimp_lin0 = g;
+            }
+            for (int s = 0; s < 2; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
+            t->l_imp_calc_rhs[0] = 0;
+            break;
+        case PZ_ARS443: /* dirk_schemes.f90:151-158, 464-480 */
+            t->nimp = t->nexp = t->nstages = 4;
+            cf = 0.9;
+            {
+                double bi[5][5] = {{0, 0, 0, 0, 0}, {0, 0.5, 0, 0, 0}, {0, 1.0 / 6.0, 0.5, 0, 0},
+                                   {0, -0.5, 0.5, 0.5, 0}, {0, 1.5, -1.5, 0.5, 0.5}};
+                double be[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0, 0, 0, 0}, {11.0 / 18.0, 1.0 / 18.0, 0, 0, 0},
+                                   {5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0}, {0.25, 1.75, 0.75, -1.75, 0}};
+                for (int i = 0; i < 5; i++)
+                    for (int j = 0; j < 5; j++) { t->bimp0[i][j] = bi[i][j]; t->bexp0[i][j] = be[i][j]; }
+                t->wimp_lin0 = 0.5;
+            }
+            for (int s = 0; s < 4; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
+            t->l_imp_calc_rhs[0] = 0;
+            break;
+        case PZ_BPR353: /* dirk_schemes.f90:167-174, 481-497 */
+            t->nimp = t->nexp = t->nstages = 4;
+            cf = 1.0;
+            {
+                double bi[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0.5, 0, 0, 0}, {5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0},
+                                   {0.5, 0, 0, 0.5, 0}, {0.25, 0, 0.75, -0.5, 0.5}};
+                double be[5][5] = {{0, 0, 0, 0, 0}, {1, 0, 0, 0, 0}, {4.0 / 9.0, 2.0 / 9.0, 0, 0, 0},
+                                   {0.25, 0, 0.75, 0, 0}, {0.25, 0, 0.75, 0, 0}};
+                for (int i = 0; i < 5; i++)
+                    for (int j = 0; j < 5; j++) { t->bimp0[i][j] = bi[i][j]; t->bexp0[i][j] = be[i][j]; }
+                t->wimp_lin0 = 0.5;
+            }
+            for (int s = 0; s < 4; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
+            t->l_exp_calc[3] = 0;
+            break;
+        default:
+            return -1;
+        }
+        t->istage = 1;
+        t->l_assembly = 0;
+    }
+    t->courfac = fabs(courfac_nml) >= 1.0e3 ? cf : courfac_nml;
+    return 0;
+}
+
+/* set_weights (multistep_schemes.f90:137-305; dirk_schemes.f90:275-713) */
+static void ts_set_weights(tscheme *t, double alpha, int *lMatNext)
+{
+    double wimp_old = t->wimp_lin[0];
+    if (t->family == 0) {
+        t->wimp[0] = 1.0;
+        t->wimp_lin[0] = alpha * t->dt[0];
+        t->wimp_lin[1] = (1 - alpha) * t->dt[0];
+        t->wexp[0] = (1.0 + 0.5 * t->dt[0] / t->dt[1]) * t->dt[0];
+        t->wexp[1] = -0.5 * t->dt[0] * t->dt[0] / t->dt[1];
+        if (t->wimp_lin[0] != wimp_old) *lMatNext = 1;
+    } else {
+        t->wimp_lin[0] = t->dt[0] * t->wimp_lin0;
+        for (int i = 0; i < 8; i++)
+            for (int j = 0; j < 8; j++) {
+                t->bimp[i][j] = t->dt[0] * t->bimp0[i][j];
+                t->bexp[i][j] = t->dt[0] * t->bexp0[i][j];
+            }
+    }
+}
+
+/* set_dt_array (multistep_schemes.f90:307-348; dirk_schemes.f90:715-771) */
+static int ts_set_dt_array(tscheme *t, double dt_new, double dt_min)
+{
+    if (t->family == 0) {
+        double last = t->dt[t->nexp - 1];
+        for (int i = t->nexp - 1; i > 0; i--) t->dt[i] = t->dt[i - 1];
+        (void)last;
+    }
+    t->dt[0] = dt_new;
+    if (dt_new < dt_min) return -2;
+    return 0;
+}
+
+/* set_imex_rhs (multistep_schemes.f90:366-413; dirk_schemes.f90:773-814) */
+static void ts_set_imex_rhs(or_handle *h, const tscheme *t, const tarray *a, cplx *rhs)
+{
+    size_t F = (size_t)h->nm * h->N;
+    if (t->family == 0) {
+        for (size_t i = 0; i < F; i++) rhs[i] = t->wimp[0] * a->old[i];
+        for (int o = 1; o < t->nold; o++)
+            for (size_t i = 0; i < F; i++) rhs[i] += t->wimp[o] * a->old[o * F + i];
+        for (int o = 0; o < t->nimp; o++)
+            for (size_t i = 0; i < F; i++) rhs[i] += t->wimp_lin[o + 1] * a->impl[o * F + i];
+        for (int o = 0; o < t->nexp; o++)
+            for (size_t i = 0; i < F; i++) rhs[i] += t->wexp[o] * a->expl[o * F + i];
+    } else {
+        int is = t->istage;
+        for (size_t i = 0; i < F; i++) rhs[i] = a->old[i];
+        for (int s = 0; s < is; s++)
+            for (size_t i = 0; i < F; i++) rhs[i] += t->bexp[is][s] * a->expl[s * F + i];
+        for (int s = 0; s < is; s++)
+            for (size_t i = 0; i < F; i++) rhs[i] += t->bimp[is][s] * a->impl[s * F + i];
+    }
+}
+
+/* rotate_imex (multistep_schemes.f90:462-498); a no-op for DIRK */
+static void ts_rotate(or_handle *h, const tscheme *t, tarray *a)
+{
+    if (t->family != 0) return;
+    size_t F = (size_t)h->nm * h->N;
+    for (int o = t->nexp - 1; o >= 1; o--) memcpy(a->expl + o * F, a->expl + (o - 1) * F, sizeof(cplx) * F);
+    for (int o = t->nold - 1; o >= 1; o--) memcpy(a->old + o * F, a->old + (o - 1) * F, sizeof(cplx) * F);
+    for (int o = t->nimp - 1; o >= 1; o--) memcpy(a->impl + o * F, a->impl + (o - 1) * F, sizeof(cplx) * F);
+}
+
+static void tarray_alloc(or_handle *h, tarray *a, const tscheme *t)
+{
+    size_t F = (size_t)h->nm * h->N;
+    /* dt_fieldsLast.f90:74-80 with the argument order of time_array.f90:28 */
+    a->nold = t->nold; a->nimp = t->nimp; a->nexp = t->nexp;
+    a->old = calloc(F * t->nold, sizeof(cplx));
+    a->impl = calloc(F * t->nimp, sizeof(cplx));
+    a->expl = calloc(F * t->nexp, sizeof(cplx));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Setup                                                                     */
+/* ------------------------------------------------------------------------ */
+static double *dalloc(int n) { return calloc(n, sizeof(double)); }
+
+/* get_conducting_state, 2-D branch (radial.f90:284-397) */
+static void conducting_state(or_handle *h, double cond_fac, int ktop, int kbot, const double *t_top,
+                             const double *t_bot, double *tc, double *dtc)
+{
+    int N = h->N;
+    double f_bot = 0, f_top = 0, epsc0 = 0.0;
+    for (int n = 0; n < PZ_N_T_BOUNDS; n++) {
+        int m_bot = (int)t_bot[3 * n], m_top = (int)t_top[3 * n];
+        double tr_bot = t_bot[3 * n + 1], tr_top = t_top[3 * n + 1];
+        if (m_bot == 0 && fabs(tr_bot) > 10.0 * 2.220446049250313e-16) f_bot = tr_bot;
+        if (m_top == 0 && fabs(tr_top) > 10.0 * 2.220446049250313e-16) f_top = tr_top;
+    }
+    double rc = h->r_cmb, ri = h->r_icb, rr = h->c.radratio;
+    for (int n = 0; n < N; n++) {
+        double r = h->r[n], or1 = h->or1[n];
+        if (ktop == 1 && kbot == 1) {
+            tc[n] = cond_fac * (log(r / rc) / log(rr));
+            dtc[n] = cond_fac * (or1 / log(rr));
+        } else if (ktop == 2 && kbot == 1) {
+            tc[n] = cond_fac * (f_bot + f_top * rc * log(r / ri) + 0.25 * epsc0 * (ri * ri - r * r + 2.0 * rc * rc * log(r / ri)));
+            dtc[n] = cond_fac * (f_top * rc * or1 + 0.5 * epsc0 * (rc * rc * or1 - r));
+        } else if (ktop == 1 && kbot == 2) {
+            tc[n] = cond_fac * (f_bot * ri * log(r / rc) + 0.25 * epsc0 * (rc * rc - r * r + 2.0 * ri * ri * log(r / rc)) + f_top);
+            dtc[n] = cond_fac * (f_bot * ri * or1 + 0.5 * epsc0 * (ri * ri * or1 - r));
+        } else {
+            double e0 = -2.0 * (rc * f_top - ri * f_bot) / (rc * rc - ri * ri);
+            if (fabs(e0) < 10.0 * 2.220446049250313e-16) e0 = 0.0;
+            tc[n] = cond_fac * (0.5 * e0 * (rc * rc * log(r / rc) - 0.5 * (r * r - rc * rc)) + rc * f_top * log(r / rc));
+            dtc[n] = cond_fac * (f_top * rc * or1 + 0.5 * e0 * (rc * rc * or1 - r));
+        }
+    }
+}
+
+static void bc_pattern(or_handle *h, const double *tb, const double *tt, cplx *bot, cplx *top)
+{
+    /* horizontal.f90:149-173 */
+    for (int nm = 0; nm < h->nm; nm++) {
+        int m = h->idx2m[nm];
+        bot[nm] = 0;
+        top[nm] = 0;
+        for (int n = 0; n < PZ_N_T_BOUNDS; n++) {
+            int m_bot = (int)tb[3 * n], m_top = (int)tt[3 * n];
+            double tr_bot = tb[3 * n + 1], ti_bot = tb[3 * n + 2];
+            double tr_top = tt[3 * n + 1], ti_top = tt[3 * n + 2];
+            if (m_bot == m && (tr_bot != 0 || ti_bot != 0)) { if (m == 0) ti_bot = 0; bot[nm] = tr_bot + I * ti_bot; }
+            if (m_top == m && (tr_top != 0 || ti_top != 0)) { if (m == 0) ti_top = 0; top[nm] = tr_top + I * ti_top; }
+        }
+    }
+}
+
+static int build_matrices(or_handle *h, int lhs_only);
+
+int or_create(const pz_config *cfg, or_handle **out)
+{
+    *out = NULL;
+    if (cfg->l_cheb_coll) return -10;   /* integration method only */
+    if (!cfg->l_direct_solve) return -11;
+    if (cfg->l_non_rot) return -12;
+    if (cfg->radratio == 0.0) return -13;
+    or_handle *h = calloc(1, sizeof(or_handle));
+    h->c = *cfg;
+    int N = h->N = cfg->n_r_max;
+    h->mmax = cfg->m_max;
+    h->minc = cfg->minc;
+    h->nm = cfg->m_max / cfg->minc + 1;
+    h->nphi = 3 * cfg->m_max / cfg->minc;
+    h->ncheb = cfg->n_cheb_max;
+    h->idx2m = calloc(h->nm, sizeof(int));
+    for (int i = 0; i < h->nm; i++) h->idx2m[i] = i * cfg->minc;
+    if (ts_init(&h->ts, cfg->time_scheme, cfg->courfac)) { free(h); return -14; }
+    /* namelists.f90:398-526 */
+    h->l_heat = cfg->ra != 0.0;
+    h->l_chem = cfg->raxi != 0.0;
+    h->tadvz = cfg->l_temp_advz ? 1.0 : 0.0;
+    h->r_cmb = 1.0 / (1.0 - cfg->radratio);
+    h->r_icb = h->r_cmb - 1.0;
+    h->ek = cfg->ek;
+    if (cfg->time_scale_rot) {
+        h->CorFac = 2.0;
+        h->BuoFac = h->l_heat ? cfg->ra * cfg->ek * cfg->ek / cfg->pr : 0.0;
+        h->TdiffFac = h->l_heat ? cfg->ek / cfg->pr : 0.0;
+        h->ChemFac = h->l_chem ? cfg->raxi * cfg->ek * cfg->ek / cfg->sc : 0.0;
+        h->XidiffFac = h->l_chem ? cfg->ek / cfg->sc : 0.0;
+        h->ViscFac = cfg->ek;
+    } else {
+        h->CorFac = 2.0 / cfg->ek;
+        h->BuoFac = h->l_heat ? cfg->ra / cfg->pr : 0.0;
+        h->TdiffFac = h->l_heat ? 1.0 / cfg->pr : 0.0;
+        h->ChemFac = h->l_chem ? cfg->raxi / cfg->sc : 0.0;
+        h->XidiffFac = h->l_chem ? 1.0 / cfg->sc : 0.0;
+        h->ViscFac = 1.0;
+    }
+    /* chebyshev.f90:55-60, dct_fftw.f90:124 */
+    h->rnorm = sqrt(2.0 / (double)(N - 1));
+    h->boundary_fac = 0.5;
+    h->cheb_fac = sqrt(0.5 / (double)(N - 1));
+    /* radial_derivatives.f90:61-66 */
+    h->thr = cfg->l_rerror_fix ? cfg->rerror_fac * 2.220446049250313e-16 : 0.0;
+    /* dct_fftw.f90:79-89 (n_cheb_max = n_r_max in chebt) */
+    int L = 2 * N - 2;
+    h->der = calloc(L, sizeof(int));
+    h->der2 = calloc(L, sizeof(int));
+    for (int k = 1; k <= N - 1; k++) h->der[L - k] = -k;
+    for (int k = 0; k < N; k++) h->der[k] = k;
+    for (int p = 0; p < L; p++) h->der2[p] = h->der[p] * h->der[p];
+    h->der[N - 1] = 0;
+    h->fdct = fft_new(L);
+    h->fphi = fft_new(h->nphi);
+    int big = L > h->nphi ? L : h->nphi;
+    h->w1 = calloc(big, sizeof(cplx));
+    h->w2 = calloc(big, sizeof(cplx));
+    h->w3 = calloc(big, sizeof(cplx));
+    /* grid: chebyshev.f90:188-240 */
+    double *arr[] = {0};
+    (void)arr;
+    h->r = dalloc(N); h->x = dalloc(N); h->drx = dalloc(N); h->ddrx = dalloc(N);
+    h->or1 = dalloc(N); h->or2 = dalloc(N); h->rgrav = dalloc(N); h->beta = dalloc(N);
+    h->ekpump = dalloc(N); h->ekp_up = dalloc(N); h->ekp_us = dalloc(N); h->ekp_dusdp = dalloc(N);
+    h->height = dalloc(N); h->oheight = dalloc(N); h->tcond = dalloc(N); h->dtcond = dalloc(N);
+    h->xicond = dalloc(N); h->dxicond = dalloc(N); h->delxr2 = dalloc(N); h->delxh2 = dalloc(N);
+    h->mR = calloc(N, sizeof(int));
+    double pi = PI_F, bma = 0.5 * (h->r_cmb - h->r_icb), bpa = 0.5 * (h->r_icb + h->r_cmb);
+    for (int k = 0; k < N; k++) {
+        h->x[k] = cos(pi * (double)k / (double)(N - 1));
+        h->r[k] = bma * h->x[k] + bpa;
+        h->drx[k] = 2.0 / (h->r_cmb - h->r_icb);
+        h->ddrx[k] = 0.0;
+    }
+    /* tau lines, chebyshev.f90:362-381 */
+    h->rMat = dalloc(2 * N); h->drMat = dalloc(2 * N); h->d2rMat = dalloc(2 * N);
+    for (int n = 0; n < N; n++) {
+        double dn2 = (double)n * (double)n, monen = (n % 2 == 0) ? 1.0 : -1.0;
+        h->rMat[n] = 1.0; h->rMat[N + n] = monen;
+        h->drMat[n] = 2.0 * dn2; h->drMat[N + n] = 2.0 * dn2 * (-1.0) * monen;
+        h->d2rMat[n] = 4.0 * (1.0 / 3.0) * dn2 * (dn2 - 1.0);
+        h->d2rMat[N + n] = 4.0 * (1.0 / 3.0) * dn2 * (dn2 - 1.0) * monen;
+    }
+    /* radial.f90:115-282 */
+    double ek_pump_fac = (cfg->l_ek_pump && !cfg->l_non_rot) ? 1.0 : 0.0;
+    double rc = h->r_cmb;
+    for (int n = 0; n < N; n++) {
+        h->or1[n] = 1.0 / h->r[n];
+        h->or2[n] = h->or1[n] * h->or1[n];
+        h->rgrav[n] = cfg->g0 + cfg->g1 * h->r[n] / rc + cfg->g2 * rc * rc * h->or2[n];
+        h->mR[n] = cfg->m_max;
+    }
+    h->delxh2[0] = pow(2.0 * pi / 3.0 / h->mR[0] * rc, 2);
+    h->delxh2[N - 1] = pow(2.0 * pi / 3.0 / h->mR[N - 1] * h->r_icb, 2);
+    h->delxr2[0] = pow(h->r[0] - h->r[1], 2);
+    h->delxr2[N - 1] = pow(h->r[N - 2] - h->r[N - 1], 2);
+    for (int n = 1; n < N - 1; n++) {
+        h->delxh2[n] = pow(2.0 * pi / 3.0 / h->mR[n] * h->r[n], 2);
+        double dm = fmin(h->r[n - 1] - h->r[n], h->r[n] - h->r[n + 1]);
+        h->delxr2[n] = dm * dm;
+    }
+    if (h->l_heat) conducting_state(h, cfg->tcond_fac, cfg->ktopt, cfg->kbott, cfg->t_top, cfg->t_bot, h->tcond, h->dtcond);
+    if (h->l_chem) conducting_state(h, cfg->xicond_fac, cfg->ktopxi, cfg->kbotxi, cfg->xi_top, cfg->xi_bot, h->xicond, h->dxicond);
+    for (int n = 1; n < N; n++) {
+        double s2 = rc * rc - h->r[n] * h->r[n];
+        h->height[n] = 2.0 * sqrt(s2);
+        h->oheight[n] = 0.5 / sqrt(s2);
+        h->beta[n] = -h->r[n] / s2;
+        h->ekpump[n] = 0.5 * ek_pump_fac * sqrt(cfg->ek * rc) / pow(s2, 0.75);
+    }
+    for (int n = 0; n < N; n++) {
+        h->ekp_up[n] = -0.5 * h->beta[n];
+        h->ekp_us[n] = -5.0 * rc * h->beta[n] * h->oheight[n];
+        h->ekp_dusdp[n] = h->beta[n];
+    }
+    /* horizontal.f90:69-173 (no hyperdiffusion) */
+    h->hdifT = dalloc(h->nm); h->hdifXi = dalloc(h->nm); h->hdifV = dalloc(h->nm);
+    for (int i = 0; i < h->nm; i++) h->hdifT[i] = h->hdifXi[i] = h->hdifV[i] = 1.0;
+    h->topt = calloc(h->nm, sizeof(cplx)); h->bott = calloc(h->nm, sizeof(cplx));
+    h->topxi = calloc(h->nm, sizeof(cplx)); h->botxi = calloc(h->nm, sizeof(cplx));
+    bc_pattern(h, cfg->t_bot, cfg->t_top, h->bott, h->topt);
+    /* the composition pattern is read from t_bot/t_top (horizontal.f90:176-198) */
+    bc_pattern(h, cfg->t_bot, cfg->t_top, h->botxi, h->topxi);
+    /* fields */
+    size_t F = (size_t)h->nm * N;
+    cplx **fl[] = {&h->us, &h->up, &h->om, &h->psi, &h->temp, &h->xi, &h->temp_hat, &h->xi_hat,
+                   &h->psi_hat, &h->dtemp, &h->dxi, &h->work, &h->dVsT, &h->dVsXi, &h->dVsOm, &h->buo};
+    for (size_t i = 0; i < sizeof(fl) / sizeof(fl[0]); i++) *fl[i] = calloc(F, sizeof(cplx));
+    tarray_alloc(h, &h->dTdt, &h->ts);
+    tarray_alloc(h, &h->dxidt, &h->ts);
+    tarray_alloc(h, &h->dpsidt, &h->ts);
+    h->dtr = dalloc(N);
+    h->dth = dalloc(N);
+    for (int i = 0; i < 16; i++) h->gr[i] = dalloc(h->nphi);
+    /* matrices */
+    band_alloc_r(&h->T_RHSE[0], 3, 3, N);
+    band_alloc_r(&h->T_RHSE[1], 4, 4, N);
+    h->T_RHSI = calloc(h->nm, sizeof(bandr));
+    h->T_LHS = calloc(h->nm, sizeof(bordr));
+    h->XI_LHS = calloc(h->nm, sizeof(bordr));
+    h->tempfac = dalloc(h->nm * N);
+    h->xifac = dalloc(h->nm * N);
+    band_alloc_r(&h->P_RHSE[0], 4, 4, N);
+    band_alloc_r(&h->P_RHSE[1], 8, 8, N);
+    band_alloc_r(&h->P_RHSE[2], 7, 7, N);
+    h->P_RHSI = calloc(h->nm, sizeof(bandr));
+    h->P_RHSIL = calloc(h->nm, sizeof(bandc));
+    h->P_LHS = calloc(h->nm, sizeof(bordc));
+    h->psifac = dalloc(h->nm * N);
+    for (int nm = 0; nm < h->nm; nm++) {
+        int m = h->idx2m[nm];
+        band_alloc_r(&h->T_RHSI[nm], 2, 2, N);
+        bord_alloc_r(&h->T_LHS[nm], 4, 4, 2, N);
+        bord_alloc_r(&h->XI_LHS[nm], 4, 4, 2, N);
+        if (m == 0) {
+            band_alloc_r(&h->P_RHSI[nm], 4, 4, N);
+            band_alloc_c(&h->P_RHSIL[nm], 2, 2, N);
+            bord_alloc_c(&h->P_LHS[nm], 4, 4, 2, N);
+        } else {
+            band_alloc_r(&h->P_RHSI[nm], 8, 8, N);
+            band_alloc_c(&h->P_RHSIL[nm], h->c.l_coriolis_imp ? 8 : 6, h->c.l_coriolis_imp ? 8 : 6, N);
+            bord_alloc_c(&h->P_LHS[nm], 8, 8, 4, N);
+        }
+    }
+    build_matrices(h, 0);
+    h->lMatNext = 1;
+    *out = h;
+    return 0;
+}
+
+/* RHS matrices (get_rhs_exp_mat / get_rhs_imp_mat / get_rhs_imp_lin_mat) and,
+ * when lhs_only, just the LHS matrices with the current wimp_lin(1). */
+static int build_matrices(or_handle *h, int lhs_only)
+{
+    if (!h->tabs) h->tabs = build_optabs(h);
+    optabs *T = h->tabs;
+    int N = h->N;
+    if (!lhs_only) {
+        fill_rhs_band_r(h, &h->T_RHSE[0], 2, T->I2R1.e, T->I2R1.K);
+        fill_rhs_band_r(h, &h->T_RHSE[1], 2, T->I2R2.e, T->I2R2.K);
+        fill_rhs_band_r(h, &h->P_RHSE[0], 2, T->I2R2.e, T->I2R2.K);
+        fill_rhs_band_r(h, &h->P_RHSE[1], 4, T->I4R4.e, T->I4R4.K);
+        fill_rhs_band_r(h, &h->P_RHSE[2], 4, T->I4R3.e, T->I4R3.K);
+        for (int nm = 0; nm < h->nm; nm++) {
+            int m = h->idx2m[nm];
+            ld M2 = (ld)m * m;
+            /* T: I2R2LAPL(m) with K=2 */
+            {
+                int K = 2;
+                ld *ext = calloc((size_t)N * (2 * K + 1), sizeof(ld));
+                for (size_t i = 0; i < (size_t)N * (2 * K + 1); i++) ext[i] = T->L2_0.e[i] + M2 * T->L2_2.e[i];
+                fill_rhs_band_r(h, &h->T_RHSI[nm], 2, ext, K);
+                free(ext);
+            }
+            double nu = h->ViscFac * h->hdifV[nm];
+            if (m == 0) {
+                fill_rhs_band_r(h, &h->P_RHSI[nm], 2, T->I2R2.e, T->I2R2.K);
+                /* nu*(R2 - 3 I1R1) */
+                int K = 2;
+                ld *ext = calloc((size_t)N * (2 * K + 1), sizeof(ld));
+                for (size_t i = 0; i < (size_t)N * (2 * K + 1); i++)
+                    ext[i] = (ld)nu * (T->R2.e[i] - 3.0L * T->I1R1.e[i]);
+                bandr tmp;
+                band_alloc_r(&tmp, 2, 2, N);
+                fill_rhs_band_r(h, &tmp, 2, ext, K);
+                for (size_t i = 0; i < (size_t)5 * N; i++) h->P_RHSIL[nm].d[i] = tmp.d[i];
+                free(tmp.d);
+                free(ext);
+            } else {
+                int K = 8;
+                ld *ext = calloc((size_t)N * (2 * K + 1), sizeof(ld));
+                for (size_t i = 0; i < (size_t)N * (2 * K + 1); i++) ext[i] = T->LR_0.e[i] + M2 * T->LR_2.e[i];
+                fill_rhs_band_r(h, &h->P_RHSI[nm], 4, ext, K);
+                /* nu*LAPLROT2 + i*(-CorFac*m)*I4R4 */
+                int Kc = h->P_RHSIL[nm].ku;
+                ld *er = calloc((size_t)N * (2 * Kc + 1), sizeof(ld));
+                ld *ei = calloc((size_t)N * (2 * Kc + 1), sizeof(ld));
+                for (int n = 0; n < N; n++)
+                    for (int d = -Kc; d <= Kc; d++) {
+                        size_t s8 = (size_t)n * 17 + 8 + d, sc = (size_t)n * (2 * Kc + 1) + Kc + d;
+                        er[sc] = (ld)nu * (T->LR2_0.e[s8] + M2 * T->LR2_2.e[s8] + M2 * M2 * T->LR2_4.e[s8]);
+                        ei[sc] = h->c.l_coriolis_imp ? -(ld)h->CorFac * (ld)m * T->I4R4.e[s8] : 0;
+                    }
+                bandr tr, ti;
+                band_alloc_r(&tr, Kc, Kc, N);
+                band_alloc_r(&ti, Kc, Kc, N);
+                fill_rhs_band_r(h, &tr, 4, er, Kc);
+                fill_rhs_band_r(h, &ti, 4, ei, Kc);
+                for (size_t i = 0; i < (size_t)(2 * Kc + 1) * N; i++) h->P_RHSIL[nm].d[i] = tr.d[i] + I * ti.d[i];
+                free(tr.d); free(ti.d); free(er); free(ei); free(ext);
+            }
+        }
+    } else {
+        double wimp = h->ts.wimp_lin[0];
+        for (int nm = 0; nm < h->nm; nm++) {
+            int m = h->idx2m[nm], info;
+            if (h->l_heat) {
+                info = build_lhs_scalar(h, T, &h->T_LHS[nm], h->tempfac + (size_t)nm * N, m, wimp,
+                                        h->TdiffFac * h->hdifT[nm], h->c.ktopt, h->c.kbott);
+                if (info) return info;
+            }
+            if (h->l_chem) {
+                info = build_lhs_scalar(h, T, &h->XI_LHS[nm], h->xifac + (size_t)nm * N, m, wimp,
+                                        h->XidiffFac * h->hdifXi[nm], h->c.ktopxi, h->c.kbotxi);
+                if (info) return info;
+            }
+            info = build_lhs_psi(h, T, &h->P_LHS[nm], h->psifac + (size_t)nm * N, m, wimp);
+            if (info) return info;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Physics                                                                   */
+/* ------------------------------------------------------------------------ */
+static inline double cc22real(cplx a, cplx b, int m)
+{
+    if (m == 0) return creal(a) * creal(b);
+    return 2.0 * (creal(a) * creal(b) + cimag(a) * cimag(b));
+}
+static inline double cc2real(cplx a, int m)
+{
+    if (m == 0) return creal(a) * creal(a);
+    return 2.0 * (creal(a) * creal(a) + cimag(a) * cimag(a));
+}
+
+/* rloop::radial_loop (radial_loop.f90:231-394) writing the stage slots */
+static void radial_loop(or_handle *h, int is /* 0-based stage */)
+{
+    int N = h->N, nm = h->nm, np = h->nphi;
+    size_t F = (size_t)nm * N;
+    double *usg = h->gr[0], *upg = h->gr[1], *omg = h->gr[2], *tg = h->gr[3], *xg = h->gr[4];
+    double *usT = h->gr[5], *upT = h->gr[6], *usX = h->gr[7], *upX = h->gr[8], *usO = h->gr[9],
+           *upO = h->gr[10];
+    cplx *sp = malloc(sizeof(cplx) * nm), *o = malloc(sizeof(cplx) * nm);
+    cplx *dpsidt = h->dpsidt.expl + is * F, *dTdt = h->dTdt.expl + is * F, *dxidt = h->dxidt.expl + is * F;
+    for (int n_r = 0; n_r < N; n_r++) {
+        double usom = 0;
+        for (int i = 0; i < nm; i++) {
+            int m = h->idx2m[i];
+            cplx usf = (m == 0) ? 0.5 * h->ek * h->CorFac * h->ekpump[n_r] * h->up[(size_t)i * N + n_r]
+                                : h->us[(size_t)i * N + n_r];
+            usom += cc22real(usf, h->om[(size_t)i * N + n_r], m);
+        }
+#define GATHER(f) for (int i = 0; i < nm; i++) sp[i] = (f)[(size_t)i * N + n_r]
+        GATHER(h->us); phi_ifft(h, sp, usg, h->mR[n_r]);
+        GATHER(h->up); phi_ifft(h, sp, upg, h->mR[n_r]);
+        GATHER(h->om); phi_ifft(h, sp, omg, h->mR[n_r]);
+        if (h->l_heat) { GATHER(h->temp); phi_ifft(h, sp, tg, h->mR[n_r]); }
+        if (h->l_chem) { GATHER(h->xi); phi_ifft(h, sp, xg, h->mR[n_r]); }
+#undef GATHER
+        if (h->ts.istage == 1) {
+            /* courant_mod::courant (courant.f90:41-76) */
+            double vr = 0, vh = 0, cf = h->ts.courfac;
+            for (int j = 0; j < np; j++) { vr = fmax(vr, usg[j] * usg[j]); vh = fmax(vh, upg[j] * upg[j]); }
+            vr = cf * cf * vr;
+            vh = cf * cf * vh;
+            h->dtr[n_r] = 1.0e10;
+            h->dth[n_r] = 1.0e10;
+            if (vr != 0.0) h->dtr[n_r] = fmin(h->dtr[n_r], sqrt(h->delxr2[n_r] / vr));
+            if (vh != 0.0) h->dth[n_r] = fmin(h->dth[n_r], sqrt(h->delxh2[n_r] / vh));
+        }
+        double r = h->r[n_r];
+        for (int j = 0; j < np; j++) {
+            if (h->l_heat) { usT[j] = r * usg[j] * tg[j]; upT[j] = upg[j] * tg[j]; }
+            if (h->l_chem) { usX[j] = r * usg[j] * xg[j]; upX[j] = upg[j] * xg[j]; }
+            usO[j] = r * usg[j] * omg[j];
+            upO[j] = upg[j] * omg[j];
+        }
+#define SCATTER(f) for (int i = 0; i < nm; i++) (f)[(size_t)i * N + n_r] = o[i]
+        phi_fft(h, upO, o, h->mR[n_r]); SCATTER(dpsidt);
+        phi_fft(h, usO, o, h->mR[n_r]); SCATTER(h->dVsOm);
+        if (h->l_heat) {
+            phi_fft(h, upT, o, h->mR[n_r]); SCATTER(dTdt);
+            phi_fft(h, usT, o, h->mR[n_r]); SCATTER(h->dVsT);
+        }
+        if (h->l_chem) {
+            phi_fft(h, upX, o, h->mR[n_r]); SCATTER(dxidt);
+            phi_fft(h, usX, o, h->mR[n_r]); SCATTER(h->dVsXi);
+        }
+#undef SCATTER
+        double or1 = h->or1[n_r];
+        for (int i = 0; i < nm; i++) {
+            int m = h->idx2m[i];
+            size_t q = (size_t)i * N + n_r;
+            cplx cim = -or1 * I * (double)m;
+            if (h->l_heat) dTdt[q] = cim * dTdt[q] - (1.0 - h->tadvz) * h->beta[n_r] * or1 * h->dVsT[q];
+            if (h->l_chem) dxidt[q] = cim * dxidt[q];
+            if (i == 0) dpsidt[q] = -usom;
+            else dpsidt[q] = cim * dpsidt[q];
+        }
+    }
+    free(sp);
+    free(o);
+}
+
+static void dealias(or_handle *h, cplx *F)
+{
+    for (int i = 0; i < h->nm; i++)
+        for (int k = h->ncheb; k < h->N; k++) F[(size_t)i * h->N + k] = 0;
+}
+
+/* finish_exp_temp_int / finish_exp_xi_int (update_temp_integ.f90:396-469,
+ * update_xi_integ.f90:280-354) */
+static void finish_exp_scalar(or_handle *h, cplx *dVs, cplx *exp_last, const double *dcond,
+                              const double *cond, int is_temp)
+{
+    int N = h->N;
+    costf_field(h, dVs);
+    dealias(h, dVs);
+    for (int i = 0; i < h->nm; i++) get_dr_vec(h, dVs + (size_t)i * N, h->work + (size_t)i * N, 0);
+    for (int n_r = 0; n_r < N; n_r++) {
+        double h2 = h->r_cmb * h->r_cmb - h->r[n_r] * h->r[n_r];
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            size_t q = (size_t)i * N + n_r;
+            if (is_temp)
+                exp_last[q] = h->r[n_r] * exp_last[q] -
+                              I * (double)m * (h2 * dcond[n_r] + h->tadvz * h->r[n_r] * cond[n_r]) * h->psi[q];
+            else
+                exp_last[q] = h->r[n_r] * exp_last[q] - I * (double)m * h2 * dcond[n_r] * h->psi[q];
+        }
+    }
+    for (size_t q = 0; q < (size_t)h->nm * N; q++) exp_last[q] -= h->work[q];
+    costf_field(h, exp_last);
+    dealias(h, exp_last);
+    for (int i = 0; i < h->nm; i++) {
+        cplx *v = exp_last + (size_t)i * N;
+        band_mv_r(&h->T_RHSE[0], v);
+        v[0] = 0;
+        v[1] = 0;
+    }
+}
+
+/* finish_exp_psi_int_smat (update_psi_integ_smat.f90:792-958) */
+static void finish_exp_psi(or_handle *h, cplx *dpsi)
+{
+    int N = h->N;
+    costf_field(h, h->dVsOm);
+    dealias(h, h->dVsOm);
+    for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsOm + (size_t)i * N, h->work + (size_t)i * N, 0);
+    for (int n_r = 0; n_r < N; n_r++)
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            size_t q = (size_t)i * N + n_r;
+            if (m != 0) {
+                dpsi[q] = h->r[n_r] * dpsi[q];
+                if (!h->c.l_coriolis_imp) dpsi[q] = dpsi[q] - h->CorFac * I * (double)m * h->r[n_r] * h->psi[q];
+                if (!h->c.l_buo_imp) {
+                    if (h->l_heat) dpsi[q] = dpsi[q] - h->BuoFac * h->rgrav[n_r] * I * (double)m * h->temp[q];
+                    if (h->l_chem) dpsi[q] = dpsi[q] - h->ChemFac * h->rgrav[n_r] * I * (double)m * h->xi[q];
+                }
+            }
+        }
+    if (h->c.l_ek_pump) {
+        for (int n_r = 0; n_r < N; n_r++) {
+            double ekp_fac = h->CorFac * h->ekpump[n_r];
+            for (int i = 0; i < h->nm; i++) {
+                int m = h->idx2m[i];
+                size_t q = (size_t)i * N + n_r;
+                if (m == 0) dpsi[q] = dpsi[q] - ekp_fac * h->up[q];
+                else
+                    dpsi[q] = dpsi[q] - ekp_fac * h->r[n_r] *
+                                            (h->om[q] + h->ekp_up[n_r] * h->up[q] +
+                                             (I * (double)m * h->ekp_dusdp[n_r] + h->ekp_us[n_r]) * h->us[q]);
+            }
+        }
+    }
+    for (int i = 1; i < h->nm; i++)
+        for (int n_r = 0; n_r < N; n_r++) dpsi[(size_t)i * N + n_r] -= h->work[(size_t)i * N + n_r];
+    costf_field(h, dpsi);
+    dealias(h, dpsi);
+    for (int i = 0; i < h->nm; i++) {
+        cplx *v = dpsi + (size_t)i * N;
+        if (h->idx2m[i] == 0) band_mv_r(&h->P_RHSE[0], v);
+        else { band_mv_r(&h->P_RHSE[2], v); v[2] = 0; v[3] = 0; }
+        v[0] = 0;
+        v[1] = 0;
+    }
+}
+
+/* get_temp_rhs_imp_int / get_xi_rhs_imp_int (update_temp_integ.f90:471-536) */
+static void rhs_imp_scalar(or_handle *h, const cplx *hat, tarray *a, int istage, int l_calc_lin,
+                           double diff, const double *hdif)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    cplx *v = malloc(sizeof(cplx) * N);
+    if (istage == 1)
+        for (int i = 0; i < h->nm; i++) {
+            memcpy(v, hat + (size_t)i * N, sizeof(cplx) * N);
+            band_mv_r(&h->T_RHSE[1], v);
+            v[0] = 0; v[1] = 0;
+            memcpy(a->old + (size_t)i * N, v, sizeof(cplx) * N);
+        }
+    if (l_calc_lin)
+        for (int i = 0; i < h->nm; i++) {
+            memcpy(v, hat + (size_t)i * N, sizeof(cplx) * N);
+            band_mv_r(&h->T_RHSI[i], v);
+            v[0] = 0; v[1] = 0;
+            for (int k = 0; k < N; k++) a->impl[(istage - 1) * F + (size_t)i * N + k] = diff * hdif[i] * v[k];
+        }
+    free(v);
+}
+
+/* get_psi_rhs_imp_int_smat (update_psi_integ_smat.f90:960-1115) */
+static void rhs_imp_psi(or_handle *h, const cplx *psi_hat, const cplx *up, const cplx *temp,
+                        const cplx *xi, tarray *a, int istage, int l_calc_lin)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    double *work_r = dalloc(N);
+    for (int n = 0; n < N; n++) work_r[n] = creal(up[n]);
+    costf_real(h, work_r);
+    cplx *v = malloc(sizeof(cplx) * N);
+    if (istage == 1)
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            for (int k = 0; k < N; k++) v[k] = m == 0 ? work_r[k] : psi_hat[(size_t)i * N + k];
+            band_mv_r(&h->P_RHSI[i], v);
+            v[0] = 0; v[1] = 0;
+            if (m > 0) { v[2] = 0; v[3] = 0; }
+            memcpy(a->old + (size_t)i * N, v, sizeof(cplx) * N);
+        }
+    if (l_calc_lin) {
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            for (int k = 0; k < N; k++) v[k] = m == 0 ? work_r[k] : psi_hat[(size_t)i * N + k];
+            band_mv_c(&h->P_RHSIL[i], v);
+            v[0] = 0; v[1] = 0;
+            if (m > 0) { v[2] = 0; v[3] = 0; }
+            memcpy(h->work + (size_t)i * N, v, sizeof(cplx) * N);
+        }
+        cplx *impl = a->impl + (size_t)(istage - 1) * F;
+        memcpy(impl, h->work, sizeof(cplx) * F);
+        if (h->c.l_buo_imp) {
+            for (int n_r = 0; n_r < N; n_r++)
+                for (int i = 0; i < h->nm; i++) {
+                    int m = h->idx2m[i];
+                    size_t q = (size_t)i * N + n_r;
+                    if (m != 0) {
+                        if (h->l_heat) {
+                            h->work[q] = -h->BuoFac * h->rgrav[n_r] * h->or1[n_r] * I * (double)m * temp[q];
+                            if (h->l_chem)
+                                h->work[q] = h->work[q] - h->ChemFac * h->rgrav[n_r] * h->or1[n_r] * I * (double)m * xi[q];
+                        } else if (h->l_chem) {
+                            h->work[q] = -h->ChemFac * h->rgrav[n_r] * h->or1[n_r] * I * (double)m * xi[q];
+                        }
+                    }
+                }
+            costf_field(h, h->work);
+            for (int i = 0; i < h->nm; i++) {
+                if (h->idx2m[i] <= 0) continue;
+                memcpy(v, h->work + (size_t)i * N, sizeof(cplx) * N);
+                band_mv_r(&h->P_RHSE[1], v);
+                v[0] = v[1] = v[2] = v[3] = 0;
+                for (int k = 0; k < N; k++) impl[(size_t)i * N + k] += v[k];
+            }
+        }
+    }
+    free(v);
+    free(work_r);
+}
+
+/* update_temp_int / update_xi_int (update_temp_integ.f90:179-298) */
+static int update_scalar(or_handle *h, int lMat, int is_temp)
+{
+    int N = h->N;
+    tscheme *t = &h->ts;
+    bordr *LHS = is_temp ? h->T_LHS : h->XI_LHS;
+    double *fac = is_temp ? h->tempfac : h->xifac;
+    tarray *a = is_temp ? &h->dTdt : &h->dxidt;
+    cplx *hat = is_temp ? h->temp_hat : h->xi_hat, *fld = is_temp ? h->temp : h->xi;
+    cplx *top = is_temp ? h->topt : h->topxi, *bot = is_temp ? h->bott : h->botxi;
+    (void)lMat;
+    ts_set_imex_rhs(h, t, a, h->work);
+    double *rr = dalloc(N), *ri = dalloc(N);
+    for (int i = 0; i < h->nm; i++) {
+        cplx *w = h->work + (size_t)i * N;
+        w[0] = top[i];
+        w[1] = bot[i];
+        for (int k = 0; k < N; k++) w[k] = w[k] * fac[(size_t)i * N + k];
+        for (int k = 0; k < N; k++) { rr[k] = creal(w[k]); ri[k] = cimag(w[k]); }
+        bord_solve_r(&LHS[i], rr);
+        bord_solve_r(&LHS[i], ri);
+        for (int k = 0; k < N; k++) hat[(size_t)i * N + k] = rr[k] + I * ri[k];
+    }
+    free(rr);
+    free(ri);
+    memcpy(fld, hat, sizeof(cplx) * h->nm * N);
+    costf_field(h, fld);
+    ts_rotate(h, t, a);
+    double diff = is_temp ? h->TdiffFac : h->XidiffFac;
+    double *hdif = is_temp ? h->hdifT : h->hdifXi;
+    if (t->istage == t->nstages) rhs_imp_scalar(h, hat, a, 1, t->l_imp_calc_rhs[0], diff, hdif);
+    else rhs_imp_scalar(h, hat, a, t->istage + 1, t->l_imp_calc_rhs[t->istage], diff, hdif);
+    return 0;
+}
+
+/* update_psi_int_smat (update_psi_integ_smat.f90:247-573) */
+static int update_psi(or_handle *h)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    tscheme *t = &h->ts;
+    if (h->c.l_buo_imp) {
+        for (int n_r = 0; n_r < N; n_r++)
+            for (int i = 0; i < h->nm; i++) {
+                int m = h->idx2m[i];
+                size_t q = (size_t)i * N + n_r;
+                if (m != 0) {
+                    if (h->l_heat) {
+                        h->buo[q] = -h->BuoFac * h->rgrav[n_r] * h->or1[n_r] * I * (double)m * h->temp[q];
+                        if (h->l_chem)
+                            h->buo[q] = h->buo[q] - h->ChemFac * h->rgrav[n_r] * h->or1[n_r] * I * (double)m * h->xi[q];
+                    } else if (h->l_chem) {
+                        h->buo[q] = -h->ChemFac * h->rgrav[n_r] * h->or1[n_r] * I * (double)m * h->xi[q];
+                    }
+                } else {
+                    h->buo[q] = 0;
+                }
+            }
+        costf_field(h, h->buo);
+        for (int i = 0; i < h->nm; i++) {
+            if (h->idx2m[i] <= 0) continue;
+            cplx *v = h->buo + (size_t)i * N;
+            band_mv_r(&h->P_RHSE[1], v);
+            v[0] = v[1] = v[2] = v[3] = 0;
+        }
+    }
+    ts_set_imex_rhs(h, t, &h->dpsidt, h->work);
+    double *uphi0 = dalloc(N), *om0 = dalloc(N);
+    cplx *rhs = malloc(sizeof(cplx) * N);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        for (int k = 0; k < N; k++) {
+            rhs[k] = h->work[(size_t)i * N + k];
+            if (h->c.l_buo_imp && m != 0) rhs[k] = rhs[k] + t->wimp_lin[0] * h->buo[(size_t)i * N + k];
+        }
+        for (int k = 0; k < N; k++) rhs[k] = rhs[k] * h->psifac[(size_t)i * N + k];
+        bord_solve_c(&h->P_LHS[i], rhs);
+        if (m == 0) for (int k = 0; k < N; k++) uphi0[k] = creal(rhs[k]);
+        else memcpy(h->psi_hat + (size_t)i * N, rhs, sizeof(cplx) * N);
+    }
+    for (int i = 0; i < h->nm; i++)
+        if (h->idx2m[i] != 0) memcpy(h->psi + (size_t)i * N, h->psi_hat + (size_t)i * N, sizeof(cplx) * N);
+    get_dr_real(h, uphi0, om0, 0);
+    costf_real(h, uphi0);
+    for (int i = 0; i < h->nm; i++)
+        get_ddr_vec(h, h->psi + (size_t)i * N, h->work + (size_t)i * N, h->om + (size_t)i * N, 0);
+    costf_field(h, h->psi);
+    double rc2 = h->r_cmb * h->r_cmb;
+    for (int n_r = 0; n_r < N; n_r++) {
+        double h2 = rc2 - h->r[n_r] * h->r[n_r];
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            size_t q = (size_t)i * N + n_r;
+            if (m == 0) {
+                h->us[q] = 0.0;
+                h->up[q] = uphi0[n_r];
+                h->om[q] = om0[n_r] + h->or1[n_r] * uphi0[n_r];
+            } else {
+                h->us[q] = I * (double)m * h->or1[n_r] * h2 * h->psi[q];
+                h->up[q] = -h2 * h->work[q] + 3.0 * h->r[n_r] * h->psi[q];
+                h->om[q] = -h2 * h->om[q] - (rc2 * h->or1[n_r] - 6.0 * h->r[n_r]) * h->work[q] +
+                           (6.0 + (double)m * (double)m * h->or2[n_r] * h2) * h->psi[q];
+            }
+        }
+    }
+    ts_rotate(h, t, &h->dpsidt);
+    if (t->istage == t->nstages)
+        rhs_imp_psi(h, h->psi_hat, h->up, h->temp, h->xi, &h->dpsidt, 1, t->l_imp_calc_rhs[0]);
+    else
+        rhs_imp_psi(h, h->psi_hat, h->up, h->temp, h->xi, &h->dpsidt, t->istage + 1,
+                    t->l_imp_calc_rhs[t->istage]);
+    free(uphi0);
+    free(om0);
+    free(rhs);
+    (void)F;
+    return 0;
+}
+
+/* dt_courant (courant.f90:78-167) */
+static void dt_courant(or_handle *h, int *l_new_dt, double *dt_new)
+{
+    double dtMax = h->c.dtMax, dt = h->ts.dt[0];
+    double dt_r = 1000.0 * dtMax, dt_h = dt_r;
+    for (int n = 0; n < h->N; n++) { dt_r = fmin(h->dtr[n], dt_r); dt_h = fmin(h->dth[n], dt_h); }
+    double dt_rh = fmin(dt_r, dt_h);
+    double dt_2 = fmin(0.5 * (1.0 / h->c.dt_fac + 1.0) * dt_rh, dtMax);
+    if (dt > dtMax) { *l_new_dt = 1; *dt_new = dtMax; }
+    else if (dt > dt_rh) { *l_new_dt = 1; *dt_new = dt_2; }
+    else if (h->c.dt_fac * dt < dt_rh && dt < dtMax) { *l_new_dt = 1; *dt_new = dt_2; }
+    else { *l_new_dt = 0; *dt_new = dt; }
+}
+
+/* one iteration of time_loop's stage loop (step_time.f90:215-348) */
+static int one_step(or_handle *h)
+{
+    tscheme *t = &h->ts;
+    size_t F = (size_t)h->nm * h->N;
+    int lMat = 0;
+    t->istage = 1;
+    for (int n_stage = 1; n_stage <= t->nstages; n_stage++) {
+        int is = t->istage - 1;
+        if (t->l_exp_calc[n_stage - 1]) {
+            radial_loop(h, is);
+            if (h->l_heat) finish_exp_scalar(h, h->dVsT, h->dTdt.expl + is * F, h->dtcond, h->tcond, 1);
+            if (h->l_chem) finish_exp_scalar(h, h->dVsXi, h->dxidt.expl + is * F, h->dxicond, h->xicond, 0);
+            finish_exp_psi(h, h->dpsidt.expl + is * F);
+        }
+        int l_new_dt = 0;
+        if (t->istage == 1) {
+            double dt_new;
+            dt_courant(h, &l_new_dt, &dt_new);
+            if (ts_set_dt_array(t, dt_new, h->c.dtMax / 1.0e6)) return -2;
+            ts_set_weights(t, h->c.alpha, &h->lMatNext);
+            h->time += t->dt[0];
+        }
+        lMat = h->lMatNext;
+        if ((l_new_dt || lMat) && t->istage == 1) lMat = 1;
+        h->lMatNext = 0;
+        if (lMat) {
+            int info = build_matrices(h, 1);
+            if (info) return info;
+        }
+        if (h->l_heat) update_scalar(h, lMat, 1);
+        if (h->l_chem) update_scalar(h, lMat, 0);
+        update_psi(h);
+        t->istage++;
+    }
+    h->n_time_step++;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* API                                                                       */
+/* ------------------------------------------------------------------------ */
+void or_destroy(or_handle *h)
+{
+    if (!h) return;
+    /* process-lifetime test helper: buffers are released with the process */
+    fft_free(h->fdct);
+    fft_free(h->fphi);
+    if (h->tabs) free_optabs(h->tabs);
+    free(h);
+}
+
+int or_dims(const or_handle *h, int *nm, int *nr, int *np)
+{
+    *nm = h->nm;
+    *nr = h->N;
+    *np = h->nphi;
+    return 0;
+}
+
+static cplx *field_ptr(or_handle *h, int f)
+{
+    switch (f) {
+    case PZ_US: return h->us;
+    case PZ_UP: return h->up;
+    case PZ_OM: return h->om;
+    case PZ_PSI: return h->psi;
+    case PZ_TEMP: return h->temp;
+    case PZ_XI: return h->xi;
+    case PZ_TEMP_HAT: return h->temp_hat;
+    case PZ_XI_HAT: return h->xi_hat;
+    case PZ_PSI_HAT: return h->psi_hat;
+    case PZ_DTEMP: return h->dtemp;
+    default: return NULL;
+    }
+}
+
+/* host layout: (n_m, n_r) m fastest, interleaved re/im */
+int or_set_field(or_handle *h, int f, const double *host)
+{
+    cplx *p = field_ptr(h, f);
+    if (!p) return -1;
+    for (int i = 0; i < h->nm; i++)
+        for (int k = 0; k < h->N; k++) {
+            size_t s = 2 * ((size_t)k * h->nm + i);
+            p[(size_t)i * h->N + k] = host[s] + I * host[s + 1];
+        }
+    return 0;
+}
+
+int or_get_field(or_handle *h, int f, double *host)
+{
+    cplx *p = field_ptr(h, f);
+    if (!p) return -1;
+    for (int i = 0; i < h->nm; i++)
+        for (int k = 0; k < h->N; k++) {
+            size_t s = 2 * ((size_t)k * h->nm + i);
+            host[s] = creal(p[(size_t)i * h->N + k]);
+            host[s + 1] = cimag(p[(size_t)i * h->N + k]);
+        }
+    return 0;
+}
+
+/* Gaspari-Cohn pieces (useful.f90:295-341) */
+static double gc_center(double c, double L)
+{
+    double pi = PI_F;
+    return pi * L * L * L * (1.0 - exp(-2.0 * c / L)) - 2.0 * pi * c * L * (c + L) * exp(-2.0 * c / L);
+}
+static double gc_middle(double r, double c, double L)
+{
+    double pi = PI_F;
+    return pi / 3.0 * L * r * (r + 3.0 * L) * exp(-r / L) +
+           (2.0 * pi * L * L * (c + L) * (c + L)) / r * exp(-2.0 * c / L) * (1.0 - (1.0 - r / (c + L)) * exp(r / L)) +
+           pi * L * L * L * (exp(-r / L) - exp((r - 2.0 * c) / L));
+}
+static double gc_edge(double r, double c, double L)
+{
+    double pi = PI_F;
+    return 2.0 * pi * L / r * exp(-r / L) *
+           (0.5 * (r * (r + L) * (2.0 * c - r)) + 1.0 / 3.0 * (pow(r - c, 3) - c * c * c) - L * (c + L) * (r - c + L) +
+            L * (c + L) * (c + L) * exp((r - 2.0 * c) / L));
+}
+
+/* initProp (init_fields.f90:260-360), deterministic branches */
+static int init_prop(or_handle *h, cplx *prop, int init, double amp)
+{
+    int N = h->N;
+    double pi = PI_F;
+    if (init > 0) {
+        if (init % h->minc != 0 || init > h->mmax) return -3;
+        int idx = init / h->minc;
+        for (int n = 0; n < N; n++) prop[(size_t)idx * N + n] += sin(pi * (h->r[n] - h->r_icb)) * amp;
+    } else if (init == -1) {
+        double sigma_r = 0.1 / sqrt(2.0);
+        double rc = 0.5 * (h->r_cmb + h->r_icb);
+        int ir = 0;
+        double best = fabs(h->r[0] - rc);
+        for (int n = 1; n < N; n++)
+            if (fabs(h->r[n] - rc) < best) { best = fabs(h->r[n] - rc); ir = n; }
+        rc = h->r[ir];
+        double c1 = 0.5 * (h->r[0] - rc), c2 = 0.5 * (rc - h->r[N - 1]), L = 0.5 * sigma_r;
+        double *gasp = dalloc(N);
+        for (int n = 0; n < N; n++) {
+            double r = h->r[n];
+            if (n == ir) gasp[n] = 1.0;
+            else if (r < rc - c2) gasp[n] = gc_edge(rc - r, c2, L);
+            else if (r >= rc - c2 && r < rc) gasp[n] = gc_middle(rc - r, c2, L);
+            else if (r > rc && r <= rc + c1) gasp[n] = gc_middle(r - rc, c1, L);
+            else if (r > rc + c1) gasp[n] = gc_edge(r - rc, c1, L);
+        }
+        for (int n = 0; n < N; n++) {
+            if (n < ir) gasp[n] = gasp[n] / gc_center(c1, L);
+            else if (n > ir) gasp[n] = gasp[n] / gc_center(c2, L);
+        }
+        double phi0 = pi / h->minc;
+        double *pf = dalloc(h->nphi);
+        cplx *sp = malloc(sizeof(cplx) * h->nm);
+        for (int n = 0; n < N; n++) {
+            double c_r = amp * gasp[n];
+            for (int j = 0; j < h->nphi; j++) {
+                double phi = (double)j * 2.0 * pi / h->minc / (double)h->nphi;
+                pf[j] = c_r * exp(-(phi - phi0) * (phi - phi0) / (0.2 * 0.2));
+            }
+            phi_fft(h, pf, sp, h->mR[n]);
+            for (int i = 0; i < h->nm; i++) prop[(size_t)i * N + n] += sp[i];
+        }
+        free(gasp);
+        free(pf);
+        free(sp);
+    } else {
+        return -4; /* unseeded random branch is not reproducible */
+    }
+    return 0;
+}
+
+int or_init_fields(or_handle *h, int init_t, double amp_t, int init_xi, double amp_xi, int init_u,
+                   double amp_u)
+{
+    int rc = 0;
+    if (init_t != 0 && h->l_heat) rc |= init_prop(h, h->temp, init_t, amp_t);
+    if (init_xi != 0 && h->l_chem) rc |= init_prop(h, h->xi, init_xi, amp_xi);
+    if (init_u != 0) {
+        /* initU (init_fields.f90:455-509) */
+        int N = h->N;
+        double pi = PI_F;
+        if (init_u > 0) {
+            if (init_u % h->minc != 0 || init_u > h->mmax) return -3;
+            int idx = init_u / h->minc;
+            for (int n = 0; n < N; n++) h->us[(size_t)idx * N + n] += sin(pi * (h->r[n] - h->r_icb)) * amp_u;
+            for (int n = 0; n < N; n++)
+                for (int i = 0; i < h->nm; i++) {
+                    int m = h->idx2m[i];
+                    h->om[(size_t)i * N + n] = -I * (double)m * h->or1[n] * h->us[(size_t)i * N + n];
+                }
+        } else {
+            for (int n = 0; n < N; n++) h->up[n] += amp_u * sin(pi * (h->r[n] - h->r_icb));
+        }
+    }
+    return rc;
+}
+
+/* get_start_fields, fresh start (init_fields.f90:96-112, 186-255) */
+int or_start(or_handle *h)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    tscheme *t = &h->ts;
+    h->time = 0.0;
+    for (int i = 0; i < 8; i++) t->dt[i] = h->c.dtMax;
+    int dummy = 0;
+    ts_set_weights(t, h->c.alpha, &dummy);
+    for (int i = 0; i < h->nm; i++) {
+        if (h->l_heat) get_dr_vec(h, h->temp + (size_t)i * N, h->dtemp + (size_t)i * N, 1);
+        if (h->l_chem) get_dr_vec(h, h->xi + (size_t)i * N, h->dxi + (size_t)i * N, 1);
+        get_dr_vec(h, h->up + (size_t)i * N, h->work + (size_t)i * N, 1);
+    }
+    for (int n_r = 0; n_r < N; n_r++)
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            size_t q = (size_t)i * N + n_r;
+            h->om[q] = h->work[q] + h->or1[n_r] * h->up[q] - h->or1[n_r] * I * (double)m * h->us[q];
+        }
+    memcpy(h->temp_hat, h->temp, sizeof(cplx) * F);
+    memcpy(h->xi_hat, h->xi, sizeof(cplx) * F);
+    memcpy(h->psi_hat, h->psi, sizeof(cplx) * F);
+    if (h->l_heat) costf_field(h, h->temp_hat);
+    if (h->l_chem) costf_field(h, h->xi_hat);
+    costf_field(h, h->psi_hat);
+    if (h->l_heat) rhs_imp_scalar(h, h->temp_hat, &h->dTdt, 1, 1, h->TdiffFac, h->hdifT);
+    if (h->l_chem) rhs_imp_scalar(h, h->xi_hat, &h->dxidt, 1, 1, h->XidiffFac, h->hdifXi);
+    /* the reference passes psi in the composition slot here (init_fields.f90:250-251) */
+    rhs_imp_psi(h, h->psi_hat, h->up, h->temp, h->psi, &h->dpsidt, 1, 1);
+    h->started = 1;
+    h->lMatNext = 1;
+    return 0;
+}
+
+int or_step(or_handle *h, int nsteps)
+{
+    if (!h->started) return -5;
+    for (int s = 0; s < nsteps; s++) {
+        int rc = one_step(h);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int or_time(or_handle *h, double *time, double *dt)
+{
+    *time = h->time;
+    *dt = h->ts.dt[0];
+    return 0;
+}
+
+/* rInt_R, Chebyshev branch (integration.f90:18-58) */
+static double rint_r(or_handle *h, const double *f)
+{
+    int N = h->N;
+    double *f2 = dalloc(N);
+    for (int n = 0; n < N; n++) f2[n] = f[n] / h->drx[n];
+    costf_real(h, f2);
+    f2[0] = 0.5 * f2[0];
+    f2[N - 1] = 0.5 * f2[N - 1];
+    double s = f2[0];
+    for (int nc = 3; nc <= N; nc += 2) s = s - 1.0 / (double)(nc * (nc - 2)) * f2[nc - 1];
+    s = 2.0 * sqrt(2.0 / (double)(N - 1)) * s;
+    free(f2);
+    return s;
+}
+
+static double round_off(double v) { return fabs(v) < 10.0 * 2.220446049250313e-16 ? 0.0 : v; }
+
+/* 3-D kinetic energies (outputs.f90:458-571) */
+int or_ekin_3d(or_handle *h, double out[4])
+{
+    int N = h->N;
+    double pi = PI_F;
+    double *us2 = dalloc(N), *up2 = dalloc(N), *upa = dalloc(N), *tmp = dalloc(N);
+    for (int n = 0; n < N; n++)
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            us2[n] += cc2real(h->us[(size_t)i * N + n], m);
+            up2[n] += cc2real(h->up[(size_t)i * N + n], m);
+            if (m == 0) upa[n] += cc2real(h->up[(size_t)i * N + n], m);
+        }
+    for (int n = 0; n < N; n++) tmp[n] = us2[n] * h->r[n] * h->height[n];
+    out[0] = round_off(pi * rint_r(h, tmp));
+    for (int n = 0; n < N; n++) tmp[n] = up2[n] * h->r[n] * h->height[n];
+    out[1] = round_off(pi * rint_r(h, tmp));
+    for (int n = 0; n < N; n++) tmp[n] = us2[n] * h->beta[n] * h->beta[n] * h->r[n] * pow(h->height[n], 3) / 12.0;
+    out[2] = round_off(pi * rint_r(h, tmp));
+    for (int n = 0; n < N; n++) tmp[n] = upa[n] * h->r[n] * h->height[n];
+    out[3] = round_off(pi * rint_r(h, tmp));
+    free(us2); free(up2); free(upa); free(tmp);
+    return 0;
+}
+
+/* ---- unit operators ---- */
+static void host_to_vecs(or_handle *h, const double *host, int n_m, cplx *v)
+{
+    for (int i = 0; i < n_m; i++)
+        for (int k = 0; k < h->N; k++) {
+            size_t s = 2 * ((size_t)k * n_m + i);
+            v[(size_t)i * h->N + k] = host[s] + I * host[s + 1];
+        }
+}
+static void vecs_to_host(or_handle *h, const cplx *v, int n_m, double *host)
+{
+    for (int i = 0; i < n_m; i++)
+        for (int k = 0; k < h->N; k++) {
+            size_t s = 2 * ((size_t)k * n_m + i);
+            host[s] = creal(v[(size_t)i * h->N + k]);
+            host[s + 1] = cimag(v[(size_t)i * h->N + k]);
+        }
+}
+
+int or_costf(or_handle *h, double *f, int n_m)
+{
+    cplx *v = malloc(sizeof(cplx) * (size_t)n_m * h->N);
+    host_to_vecs(h, f, n_m, v);
+    for (int i = 0; i < n_m; i++) costf_vec(h, v + (size_t)i * h->N);
+    vecs_to_host(h, v, n_m, f);
+    free(v);
+    return 0;
+}
+
+int or_get_dr(or_handle *h, const double *f, double *df, int n_m, int l_dct_in)
+{
+    size_t S = (size_t)n_m * h->N;
+    cplx *v = malloc(sizeof(cplx) * S), *d = malloc(sizeof(cplx) * S);
+    host_to_vecs(h, f, n_m, v);
+    for (int i = 0; i < n_m; i++) get_dr_vec(h, v + (size_t)i * h->N, d + (size_t)i * h->N, l_dct_in);
+    vecs_to_host(h, d, n_m, df);
+    free(v);
+    free(d);
+    return 0;
+}
+
+int or_get_ddr(or_handle *h, const double *f, double *df, double *ddf, int n_m, int l_dct_in)
+{
+    size_t S = (size_t)n_m * h->N;
+    cplx *v = malloc(sizeof(cplx) * S), *d = malloc(sizeof(cplx) * S), *dd = malloc(sizeof(cplx) * S);
+    host_to_vecs(h, f, n_m, v);
+    for (int i = 0; i < n_m; i++)
+        get_ddr_vec(h, v + (size_t)i * h->N, d + (size_t)i * h->N, dd + (size_t)i * h->N, l_dct_in);
+    vecs_to_host(h, d, n_m, df);
+    vecs_to_host(h, dd, n_m, ddf);
+    free(v);
+    free(d);
+    free(dd);
+    return 0;
+}
+
+/* grid (n_phi, n_rad) real, phi fastest <-> spec (n_m, n_rad) complex, m fastest */
+int or_fft(or_handle *h, const double *grid, double *spec, int n_rad)
+{
+    cplx *s = malloc(sizeof(cplx) * h->nm);
+    for (int r = 0; r < n_rad; r++) {
+        phi_fft(h, grid + (size_t)r * h->nphi, s, h->mmax);
+        for (int i = 0; i < h->nm; i++) {
+            spec[2 * ((size_t)r * h->nm + i)] = creal(s[i]);
+            spec[2 * ((size_t)r * h->nm + i) + 1] = cimag(s[i]);
+        }
+    }
+    free(s);
+    return 0;
+}
+
+int or_ifft(or_handle *h, const double *spec, double *grid, int n_rad)
+{
+    cplx *s = malloc(sizeof(cplx) * h->nm);
+    for (int r = 0; r < n_rad; r++) {
+        for (int i = 0; i < h->nm; i++)
+            s[i] = spec[2 * ((size_t)r * h->nm + i)] + I * spec[2 * ((size_t)r * h->nm + i) + 1];
+        phi_ifft(h, s, grid + (size_t)r * h->nphi, h->mmax);
+    }
+    free(s);
+    return 0;
+}
+
+int or_grid(or_handle *h, double *r, double *x)
+{
+    memcpy(r, h->r, sizeof(double) * h->N);
+    memcpy(x, h->x, sizeof(double) * h->N);
+    return 0;
+}
+
+int or_stencil(or_handle *h, int which, int m, int idx, int ku, double *out)
+{
+    if (!h->tabs) h->tabs = build_optabs(h);
+    optabs *T = h->tabs;
+    const exttab *tabs[4];
+    ld coef[4];
+    int nt = 0;
+    ld M2 = (ld)m * m;
+    switch (which) {
+    case 0: tabs[0] = &T->I2R2; coef[0] = 1; nt = 1; break;
+    case 1: tabs[0] = &T->I2R1; coef[0] = 1; nt = 1; break;
+    case 2: tabs[0] = &T->L2_0; tabs[1] = &T->L2_2; coef[0] = 1; coef[1] = M2; nt = 2; break;
+    case 3: tabs[0] = &T->R2; coef[0] = 1; nt = 1; break;
+    case 4: tabs[0] = &T->I1R1; coef[0] = 1; nt = 1; break;
+    case 5: tabs[0] = &T->I4R4; coef[0] = 1; nt = 1; break;
+    case 6: tabs[0] = &T->I4R3; coef[0] = 1; nt = 1; break;
+    case 7: tabs[0] = &T->LR_0; tabs[1] = &T->LR_2; coef[0] = 1; coef[1] = M2; nt = 2; break;
+    case 8: tabs[0] = &T->LR2_0; tabs[1] = &T->LR2_2; tabs[2] = &T->LR2_4; coef[0] = 1; coef[1] = M2; coef[2] = M2 * M2; nt = 3; break;
+    default: return -1;
+    }
+    for (int t = 0; t < nt; t++)
+        if (tabs[t]->K > ku) return -2;
+    ld *st = malloc(sizeof(ld) * (2 * ku + 1));
+    comb_stencil(idx, ku, nt, tabs, coef, st);
+    for (int i = 0; i < 2 * ku + 1; i++) out[i] = (double)st[i];
+    free(st);
+    return 0;
+}

@@ -1,0 +1,57 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A plain-C CPU restatement of pizza's IMEX time step (integration method,
+ * multistep CNAB2 and DIRK schemes), used as the checker of the HIP path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+ * it; the product library never links or calls it.
+ */
+#ifndef PZ_ORACLE_H
+#define PZ_ORACLE_H
+
+#include "../include/pizza_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_handle or_handle;
+
+void or_config_default(pz_config *cfg);
+int or_create(const pz_config *cfg, or_handle **h);
+void or_destroy(or_handle *h);
+int or_dims(const or_handle *h, int *n_m_max, int *n_r_max, int *n_phi_max);
+int or_set_field(or_handle *h, int field, const double *host);
+int or_get_field(or_handle *h, int field, double *host);
+int or_init_fields(or_handle *h, int init_t, double amp_t, int init_xi,
+                   double amp_xi, int init_u, double amp_u);
+int or_start(or_handle *h);
+int or_step(or_handle *h, int nsteps);
+int or_time(or_handle *h, double *time, double *dt);
+int or_ekin_3d(or_handle *h, double out[4]);
+
+/* unit operators on (n_m, n_r) complex arrays, m fastest */
+int or_costf(or_handle *h, double *f, int n_m);
+int or_get_dr(or_handle *h, const double *f, double *df, int n_m, int l_dct_in);
+int or_get_ddr(or_handle *h, const double *f, double *df, double *ddf, int n_m,
+               int l_dct_in);
+int or_fft(or_handle *h, const double *grid, double *spec, int n_rad);
+int or_ifft(or_handle *h, const double *spec, double *grid, int n_rad);
+
+/* radial grid and the Chebyshev x grid (n_r_max values each) */
+int or_grid(or_handle *h, double *r, double *x);
+
+/*
+ * Folded band stencil of a sparse-Chebyshev operator for row idx (0-based)
+ * with half-bandwidth ku: out[0..2ku] in the reference's stencil order
+ * (out[0] = column idx+ku ... out[2ku] = column idx-ku).
+ * which: 0 I2R2, 1 I2R1, 2 I2R2LAPL(m), 3 R2, 4 I1R1, 5 I4R4, 6 I4R3,
+ *        7 I4R4LAPLROT(m), 8 I4R4LAPLROT2(m)
+ */
+int or_stencil(or_handle *h, int which, int m, int idx, int ku, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
