@@ -1,0 +1,1085 @@
+// libpizza_amd: host side of the MI355X-native IMEX step and its C ABI
+// (include/pizza_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pizza_amd.h"
+#include "pz_host_ops.h"
+#include "pz_mphase.h"
+
+using pzh::ld;
+
+namespace {
+
+struct Scheme {
+    int family = 0, nstages = 1, nexp = 2, nimp = 1, nold = 1;
+    double courfac = 0, wimp_lin0 = 0;
+    double bimp0[PZ_MAX_STAGES][PZ_MAX_STAGES] = {}, bexp0[PZ_MAX_STAGES][PZ_MAX_STAGES] = {};
+    int l_exp_calc[8] = {}, l_imp_calc_rhs[8] = {};
+};
+
+// select_time_scheme / initialize of multistep_schemes.f90:40-135 and
+// dirk_schemes.f90:34-263 for the supported schemes
+bool make_scheme(int id, double courfac_nml, Scheme &s)
+{
+    double cf;
+    if (id == PZ_CNAB2) {
+        s.family = 0;
+        s.nstages = 1;
+        s.nimp = 1;
+        s.nold = 1;
+        s.nexp = 2;
+        s.l_exp_calc[0] = s.l_imp_calc_rhs[0] = 1;
+        cf = 3.0;
+    } else {
+        s.family = 1;
+        s.nold = 1;
+        if (id == PZ_ARS222) {
+            s.nimp = s.nexp = s.nstages = 2;
+            cf = 1.35;
+            double g = 0.5 * (2.0 - std::sqrt(2.0)), del = 1.0 - 0.5 / g;
+            double bi[3][3] = {{0, 0, 0}, {0, g, 0}, {0, 1 - g, g}};
+            double be[3][3] = {{0, 0, 0}, {g, 0, 0}, {del, 1 - del, 0}};
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) { s.bimp0[i][j] = bi[i][j]; s.bexp0[i][j] = be[i][j]; }
+            s.wimp_lin0 = g;
+            for (int k = 0; k < 2; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
+            s.l_imp_calc_rhs[0] = 0;
+        } else if (id == PZ_ARS443) {
+            s.nimp = s.nexp = s.nstages = 4;
+            cf = 0.9;
+            double bi[5][5] = {{0, 0, 0, 0, 0}, {0, 0.5, 0, 0, 0}, {0, 1.0 / 6.0, 0.5, 0, 0},
+                               {0, -0.5, 0.5, 0.5, 0}, {0, 1.5, -1.5, 0.5, 0.5}};
+            double be[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0, 0, 0, 0}, {11.0 / 18.0, 1.0 / 18.0, 0, 0, 0},
+                               {5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0}, {0.25, 1.75, 0.75, -1.75, 0}};
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) { s.bimp0[i][j] = bi[i][j]; s.bexp0[i][j] = be[i][j]; }
+            s.wimp_lin0 = 0.5;
+            for (int k = 0; k < 4; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
+            s.l_imp_calc_rhs[0] = 0;
+        } else if (id == PZ_BPR353) {
+            s.nimp = s.nexp = s.nstages = 4;
+            cf = 1.0;
+            double bi[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0.5, 0, 0, 0}, {5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0},
+                               {0.5, 0, 0, 0.5, 0}, {0.25, 0, 0.75, -0.5, 0.5}};
+            double be[5][5] = {{0, 0, 0, 0, 0}, {1, 0, 0, 0, 0}, {4.0 / 9.0, 2.0 / 9.0, 0, 0, 0},
+                               {0.25, 0, 0.75, 0, 0}, {0.25, 0, 0.75, 0, 0}};
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) { s.bimp0[i][j] = bi[i][j]; s.bexp0[i][j] = be[i][j]; }
+            s.wimp_lin0 = 0.5;
+            for (int k = 0; k < 4; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
+            s.l_exp_calc[3] = 0;
+        } else {
+            return false;
+        }
+    }
+    s.courfac = std::fabs(courfac_nml) >= 1.0e3 ? cf : courfac_nml;
+    return true;
+}
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+} // namespace
+
+struct pz_handle {
+    pz_config c;
+    int N = 0, nm = 0, nphi = 0, ncheb = 0, mmax = 0, minc = 1;
+    std::vector<int> idx2m;
+    double r_cmb = 0, r_icb = 0;
+    bool l_heat = false, l_chem = false;
+    std::vector<double> r, x, or1;
+    std::vector<double> height, beta, drx;
+    double rnorm = 0, cheb_fac = 0;
+    Scheme sch;
+    PzDev d{};
+    std::vector<void *> allocs;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool started = false;
+    bool use_graph = true;
+    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0;
+    std::string err;
+
+    template <class T> T *dalloc(size_t n)
+    {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        HIPCHK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+    template <class T> T *upload(const std::vector<T> &v)
+    {
+        T *p = dalloc<T>(v.size());
+        HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~pz_handle()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static int fail(pz_handle *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    return code;
+}
+
+extern "C" void pz_config_default(pz_config *c)
+{
+    // namelists.f90:530-672
+    std::memset(c, 0, sizeof(*c));
+    c->n_r_max = 32;
+    c->n_cheb_max = 32;
+    c->m_max = 32;
+    c->minc = 1;
+    c->time_scheme = PZ_CNAB2;
+    c->l_cheb_coll = 1;
+    c->l_direct_solve = 1;
+    c->l_coriolis_imp = 1;
+    c->l_buo_imp = 1;
+    c->rerror_fac = 500.0;
+    c->alpha = 0.5;
+    c->dtMax = 1.0e-5;
+    c->courfac = 1.0e3;
+    c->dt_fac = 2.0;
+    c->ra = 1.0e5;
+    c->ek = 1.0e-3;
+    c->pr = 1.0;
+    c->radratio = 0.35;
+    c->g1 = 1.0;
+    c->ktopt = c->kbott = c->ktopxi = c->kbotxi = 1;
+    c->ktopv = c->kbotv = 2;
+    c->tcond_fac = c->xicond_fac = 1.0;
+}
+
+extern "C" const char *pz_last_error(const pz_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+// ---------------------------------------------------------------------------
+// set-up helpers
+// ---------------------------------------------------------------------------
+static void conducting_state(const pz_handle *h, double cond_fac, int ktop, int kbot, const double *t_top,
+                             const double *t_bot, std::vector<double> &tc, std::vector<double> &dtc)
+{
+    // get_conducting_state, 2-D branch (radial.f90:284-397)
+    const double eps10 = 10.0 * 2.220446049250313e-16;
+    double f_bot = 0, f_top = 0, epsc0 = 0.0;
+    for (int n = 0; n < PZ_N_T_BOUNDS; n++) {
+        if ((int)t_bot[3 * n] == 0 && std::fabs(t_bot[3 * n + 1]) > eps10) f_bot = t_bot[3 * n + 1];
+        if ((int)t_top[3 * n] == 0 && std::fabs(t_top[3 * n + 1]) > eps10) f_top = t_top[3 * n + 1];
+    }
+    const double rc = h->r_cmb, ri = h->r_icb, rr = h->c.radratio;
+    tc.assign(h->N, 0.0);
+    dtc.assign(h->N, 0.0);
+    for (int n = 0; n < h->N; n++) {
+        double r = h->r[n], o1 = h->or1[n];
+        if (ktop == 1 && kbot == 1) {
+            tc[n] = cond_fac * (std::log(r / rc) / std::log(rr));
+            dtc[n] = cond_fac * (o1 / std::log(rr));
+        } else if (ktop == 2 && kbot == 1) {
+            tc[n] = cond_fac * (f_bot + f_top * rc * std::log(r / ri) + 0.25 * epsc0 * (ri * ri - r * r + 2.0 * rc * rc * std::log(r / ri)));
+            dtc[n] = cond_fac * (f_top * rc * o1 + 0.5 * epsc0 * (rc * rc * o1 - r));
+        } else if (ktop == 1 && kbot == 2) {
+            tc[n] = cond_fac * (f_bot * ri * std::log(r / rc) + 0.25 * epsc0 * (rc * rc - r * r + 2.0 * ri * ri * std::log(r / rc)) + f_top);
+            dtc[n] = cond_fac * (f_bot * ri * o1 + 0.5 * epsc0 * (ri * ri * o1 - r));
+        } else {
+            double e0 = -2.0 * (rc * f_top - ri * f_bot) / (rc * rc - ri * ri);
+            if (std::fabs(e0) < eps10) e0 = 0.0;
+            tc[n] = cond_fac * (0.5 * e0 * (rc * rc * std::log(r / rc) - 0.5 * (r * r - rc * rc)) + rc * f_top * std::log(r / rc));
+            dtc[n] = cond_fac * (f_top * rc * o1 + 0.5 * e0 * (rc * rc * o1 - r));
+        }
+    }
+}
+
+static std::vector<double2> bc_pattern(const pz_handle *h, const double *tb, const double *tt, bool top)
+{
+    // horizontal.f90:149-173
+    std::vector<double2> v(h->nm, make_double2(0, 0));
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        for (int n = 0; n < PZ_N_T_BOUNDS; n++) {
+            const double *t = top ? tt : tb;
+            double tr = t[3 * n + 1], ti = t[3 * n + 2];
+            if ((int)t[3 * n] == m && (tr != 0 || ti != 0)) {
+                if (m == 0) ti = 0;
+                v[i] = make_double2(tr, ti);
+            }
+        }
+    }
+    return v;
+}
+
+// row-major mat-vec band [N][2K+1], column-offset order, x rnorm
+static std::vector<double> rhs_band(const pz_handle *h, int K, int nbounds, const std::vector<ld> *st_rows)
+{
+    const int N = h->N, W = 2 * K + 1;
+    std::vector<double> b((size_t)N * W, 0.0);
+    for (int i = nbounds; i < N; i++)
+        for (int d = -K; d <= K; d++) {
+            int col = i + d;
+            if (col < 0 || col >= N) continue;
+            b[(size_t)i * W + K + d] = h->rnorm * (double)st_rows[i][K - d];
+        }
+    return b;
+}
+
+static void build_operators(pz_handle *h)
+{
+    const int N = h->N, nm = h->nm;
+    pzh::OpLib L;
+    L.build(N, (ld)h->r_cmb, (ld)h->r_icb);
+    auto rows = [&](int K, std::vector<const pzh::ExtTab *> tabs, std::vector<ld> coef) {
+        std::vector<std::vector<ld>> out(N);
+        for (int i = 0; i < N; i++) out[i] = pzh::stencil(i, K, tabs, coef);
+        return out;
+    };
+    auto rnd = [](const std::vector<std::vector<ld>> &rw, std::vector<double> &dst, size_t base) {
+        size_t W = rw[0].size();
+        for (size_t i = 0; i < rw.size(); i++)
+            for (size_t k = 0; k < W; k++) dst[base + i * W + k] = (double)rw[i][k];
+    };
+    // m-independent RHS bands
+    {
+        auto s = rows(3, {&L.I2R1}, {1});
+        h->d.T_RHSE1 = h->upload(rhs_band(h, 3, 2, s.data()));
+        s = rows(4, {&L.I2R2}, {1});
+        h->d.T_RHSE2 = h->upload(rhs_band(h, 4, 2, s.data()));
+        h->d.P_RHSE1 = h->upload(rhs_band(h, 4, 2, s.data()));
+        s = rows(8, {&L.I4R4}, {1});
+        h->d.P_RHSE2 = h->upload(rhs_band(h, 8, 4, s.data()));
+        s = rows(7, {&L.I4R3}, {1});
+        h->d.P_RHSE3 = h->upload(rhs_band(h, 7, 4, s.data()));
+    }
+    std::vector<double> TRHSI((size_t)nm * N * 5, 0.0), PRHSI((size_t)nm * N * 17, 0.0);
+    std::vector<double2> PRHSIL((size_t)nm * N * 17, make_double2(0, 0));
+    std::vector<double> TS0((size_t)nm * N * 9), TS1((size_t)nm * N * 9);
+    std::vector<double> PS0((size_t)nm * N * 17, 0.0), PS1((size_t)nm * N * 17, 0.0), PS2((size_t)nm * N * 17, 0.0);
+    auto i2r2_4 = rows(4, {&L.I2R2}, {1});
+    auto i4r4_8 = rows(8, {&L.I4R4}, {1});
+    auto r2_4 = rows(4, {&L.R2}, {1}), i1r1_4 = rows(4, {&L.I1R1}, {1});
+    auto r2_2 = rows(2, {&L.R2}, {1}), i1r1_2 = rows(2, {&L.I1R1}, {1});
+    for (int mi = 0; mi < nm; mi++) {
+        const int m = h->idx2m[mi];
+        const ld M2 = (ld)m * (ld)m;
+        const double nu = h->d.ViscFac * 1.0; // hdif_V = 1
+        // T / xi (update_temp_integ.f90:538-846)
+        auto lapl2 = rows(2, {&L.L2_0, &L.L2_2}, {1, M2});
+        std::vector<double> band = rhs_band(h, 2, 2, lapl2.data());
+        std::copy(band.begin(), band.end(), TRHSI.begin() + (size_t)mi * N * 5);
+        rnd(i2r2_4, TS0, (size_t)mi * N * 9);
+        rnd(rows(4, {&L.L2_0, &L.L2_2}, {1, M2}), TS1, (size_t)mi * N * 9);
+        // psi (update_psi_integ_smat.f90:1117-1764), rotating sphere
+        const size_t pb = (size_t)mi * N * 17;
+        if (m == 0) {
+            band = rhs_band(h, 4, 2, i2r2_4.data());
+            std::copy(band.begin(), band.end(), PRHSI.begin() + pb);
+            // nu*(rmult2 - 3 intcheb1rmult1), K = 2
+            for (int i = 2; i < N; i++)
+                for (int d = -2; d <= 2; d++) {
+                    int col = i + d;
+                    if (col < 0 || col >= N) continue;
+                    double sc = nu * ((double)r2_2[i][2 - d] - 3.0 * (double)i1r1_2[i][2 - d]);
+                    PRHSIL[pb + (size_t)i * 5 + 2 + d] = make_double2(h->rnorm * sc, 0.0);
+                }
+            rnd(i2r2_4, PS0, pb);
+            for (int i = 0; i < N; i++)
+                for (int k = 0; k < 9; k++) PS1[pb + (size_t)i * 9 + k] = (double)r2_4[i][k] - 3.0 * (double)i1r1_4[i][k];
+        } else {
+            auto lr = rows(8, {&L.LR_0, &L.LR_2}, {1, M2});
+            auto lr2 = rows(8, {&L.LR2_0, &L.LR2_2, &L.LR2_4}, {1, M2, M2 * M2});
+            band = rhs_band(h, 8, 4, lr.data());
+            std::copy(band.begin(), band.end(), PRHSI.begin() + pb);
+            const double corf = h->c.l_coriolis_imp ? -h->d.CorFac * (double)m : 0.0;
+            for (int i = 4; i < N; i++)
+                for (int d = -8; d <= 8; d++) {
+                    int col = i + d;
+                    if (col < 0 || col >= N) continue;
+                    double sc = nu * (double)lr2[i][8 - d];
+                    double cs = corf * (double)i4r4_8[i][8 - d];
+                    PRHSIL[pb + (size_t)i * 17 + 8 + d] = make_double2(h->rnorm * sc, h->rnorm * cs);
+                }
+            rnd(lr, PS0, pb);
+            rnd(lr2, PS1, pb);
+            rnd(i4r4_8, PS2, pb);
+        }
+    }
+    h->d.T_RHSI = h->upload(TRHSI);
+    h->d.P_RHSI = h->upload(PRHSI);
+    h->d.P_RHSIL = h->upload(PRHSIL);
+    h->d.T_S0 = h->upload(TS0);
+    h->d.T_S1 = h->upload(TS1);
+    h->d.X_S1 = h->d.T_S1;
+    h->d.P_S0 = h->upload(PS0);
+    h->d.P_S1 = h->upload(PS1);
+    h->d.P_S2 = h->upload(PS2);
+}
+
+static std::vector<int> factor_radices(int n)
+{
+    std::vector<int> f;
+    while (n % 4 == 0) { f.push_back(4); n /= 4; }
+    while (n % 2 == 0) { f.push_back(2); n /= 2; }
+    while (n % 3 == 0) { f.push_back(3); n /= 3; }
+    for (int p = 5; n > 1; p += 2)
+        while (n % p == 0) { f.push_back(p); n /= p; }
+    return f;
+}
+
+static void setup(pz_handle *h)
+{
+    const pz_config &c = h->c;
+    const int N = h->N = c.n_r_max;
+    h->mmax = c.m_max;
+    h->minc = c.minc;
+    h->nm = c.m_max / c.minc + 1;             // truncation.f90:28
+    h->nphi = 3 * c.m_max / c.minc;           // truncation.f90:30-31
+    h->ncheb = c.n_cheb_max;
+    h->idx2m.resize(h->nm);
+    for (int i = 0; i < h->nm; i++) h->idx2m[i] = i * c.minc;
+    if (!make_scheme(c.time_scheme, c.courfac, h->sch)) throw std::invalid_argument("unsupported time scheme");
+    h->l_heat = c.ra != 0.0;
+    h->l_chem = c.raxi != 0.0;
+    h->r_cmb = 1.0 / (1.0 - c.radratio);
+    h->r_icb = h->r_cmb - 1.0;
+    PzDev &d = h->d;
+    d.N = N;
+    d.nm = h->nm;
+    d.nphi = h->nphi;
+    d.ncheb = c.n_cheb_max;
+    d.L = 2 * N - 2;
+    d.family = h->sch.family;
+    d.nstages = h->sch.nstages;
+    d.nexp = h->sch.nexp;
+    d.nimp = h->sch.nimp;
+    d.nold = h->sch.nold;
+    for (int k = 0; k < 8; k++) { d.l_exp_calc[k] = h->sch.l_exp_calc[k]; d.l_imp_calc_rhs[k] = h->sch.l_imp_calc_rhs[k]; }
+    std::memcpy(d.bimp0, h->sch.bimp0, sizeof(d.bimp0));
+    std::memcpy(d.bexp0, h->sch.bexp0, sizeof(d.bexp0));
+    d.wimp_lin0 = h->sch.wimp_lin0;
+    d.courfac = h->sch.courfac;
+    d.l_heat = h->l_heat;
+    d.l_chem = h->l_chem;
+    d.l_coriolis_imp = c.l_coriolis_imp;
+    d.l_buo_imp = c.l_buo_imp;
+    d.l_ek_pump = c.l_ek_pump;
+    d.ktopt = c.ktopt; d.kbott = c.kbott; d.ktopxi = c.ktopxi; d.kbotxi = c.kbotxi;
+    d.ktopv = c.ktopv; d.kbotv = c.kbotv;
+    // prefactors (namelists.f90:471-526)
+    if (c.time_scale_rot) {
+        d.CorFac = 2.0;
+        d.BuoFac = h->l_heat ? c.ra * c.ek * c.ek / c.pr : 0.0;
+        d.TdiffFac = h->l_heat ? c.ek / c.pr : 0.0;
+        d.ChemFac = h->l_chem ? c.raxi * c.ek * c.ek / c.sc : 0.0;
+        d.XidiffFac = h->l_chem ? c.ek / c.sc : 0.0;
+        d.ViscFac = c.ek;
+    } else {
+        d.CorFac = 2.0 / c.ek;
+        d.BuoFac = h->l_heat ? c.ra / c.pr : 0.0;
+        d.TdiffFac = h->l_heat ? 1.0 / c.pr : 0.0;
+        d.ChemFac = h->l_chem ? c.raxi / c.sc : 0.0;
+        d.XidiffFac = h->l_chem ? 1.0 / c.sc : 0.0;
+        d.ViscFac = 1.0;
+    }
+    d.tadvz = c.l_temp_advz ? 1.0 : 0.0;
+    d.r_cmb = h->r_cmb;
+    d.ek = c.ek;
+    d.alpha = c.alpha;
+    d.dtMax = c.dtMax;
+    d.dtMin = c.dtMax / 1.0e6; // pre_calculations.f90:30
+    d.dt_fac = c.dt_fac;
+    // Chebyshev constants (chebyshev.f90:55-60, dct_fftw.f90:124)
+    h->rnorm = d.rnorm = std::sqrt(2.0 / (double)(N - 1));
+    h->cheb_fac = d.cheb_fac = std::sqrt(0.5 / (double)(N - 1));
+    d.boundary_fac = 0.5;
+    d.thr = c.l_rerror_fix ? c.rerror_fac * 2.220446049250313e-16 : 0.0;
+    // grid (chebyshev.f90:188-240; radial.f90:115-282)
+    const double pi = 4.0 * std::atan(1.0);
+    const double bma = 0.5 * (h->r_cmb - h->r_icb), bpa = 0.5 * (h->r_icb + h->r_cmb), rc = h->r_cmb;
+    std::vector<double> x(N), r(N), drx(N), ddrx(N, 0.0), or1(N), or2(N), rgrav(N), beta(N, 0.0), ekpump(N, 0.0);
+    std::vector<double> ekp_up(N), ekp_us(N), ekp_dusdp(N), height(N, 0.0), oheight(N, 0.0), delxr2(N), delxh2(N);
+    for (int k = 0; k < N; k++) {
+        x[k] = std::cos(pi * (double)k / (double)(N - 1));
+        r[k] = bma * x[k] + bpa;
+        drx[k] = 2.0 / (h->r_cmb - h->r_icb);
+        or1[k] = 1.0 / r[k];
+        or2[k] = or1[k] * or1[k];
+        rgrav[k] = c.g0 + c.g1 * r[k] / rc + c.g2 * rc * rc * or2[k];
+    }
+    h->r = r;
+    h->x = x;
+    h->or1 = or1;
+    h->drx = drx;
+    const double ek_pump_fac = (c.l_ek_pump && !c.l_non_rot) ? 1.0 : 0.0;
+    for (int n = 1; n < N; n++) {
+        double s2 = rc * rc - r[n] * r[n];
+        height[n] = 2.0 * std::sqrt(s2);
+        oheight[n] = 0.5 / std::sqrt(s2);
+        beta[n] = -r[n] / s2;
+        ekpump[n] = 0.5 * ek_pump_fac * std::sqrt(c.ek * rc) / std::pow(s2, 0.75);
+    }
+    for (int n = 0; n < N; n++) {
+        ekp_up[n] = -0.5 * beta[n];
+        ekp_us[n] = -5.0 * rc * beta[n] * oheight[n];
+        ekp_dusdp[n] = beta[n];
+    }
+    h->height = height;
+    h->beta = beta;
+    const int mR = c.m_max;
+    delxh2[0] = std::pow(2.0 * pi / 3.0 / mR * rc, 2);
+    delxh2[N - 1] = std::pow(2.0 * pi / 3.0 / mR * h->r_icb, 2);
+    delxr2[0] = std::pow(r[0] - r[1], 2);
+    delxr2[N - 1] = std::pow(r[N - 2] - r[N - 1], 2);
+    for (int n = 1; n < N - 1; n++) {
+        delxh2[n] = std::pow(2.0 * pi / 3.0 / mR * r[n], 2);
+        double dm = std::min(r[n - 1] - r[n], r[n] - r[n + 1]);
+        delxr2[n] = dm * dm;
+    }
+    std::vector<double> tcond(N, 0.0), dtcond(N, 0.0), xicond(N, 0.0), dxicond(N, 0.0);
+    if (h->l_heat) conducting_state(h, c.tcond_fac, c.ktopt, c.kbott, c.t_top, c.t_bot, tcond, dtcond);
+    if (h->l_chem) conducting_state(h, c.xicond_fac, c.ktopxi, c.kbotxi, c.xi_top, c.xi_bot, xicond, dxicond);
+    d.x = h->upload(x);
+    d.r = h->upload(r);
+    d.drx = h->upload(drx);
+    d.ddrx = h->upload(ddrx);
+    d.or1 = h->upload(or1);
+    d.or2 = h->upload(or2);
+    d.rgrav = h->upload(rgrav);
+    d.beta = h->upload(beta);
+    d.ekpump = h->upload(ekpump);
+    d.ekp_up = h->upload(ekp_up);
+    d.ekp_us = h->upload(ekp_us);
+    d.ekp_dusdp = h->upload(ekp_dusdp);
+    d.delxr2 = h->upload(delxr2);
+    d.delxh2 = h->upload(delxh2);
+    d.tcond = h->upload(tcond);
+    d.dtcond = h->upload(dtcond);
+    d.xicond = h->upload(xicond);
+    d.dxicond = h->upload(dxicond);
+    // transform tables
+    const int L = 2 * N - 2;
+    std::vector<double> cosT(L), sinT(L);
+    const ld lpi = 3.14159265358979323846264338327950288L;
+    for (int q = 0; q < L; q++) {
+        cosT[q] = (double)cosl(lpi * (ld)q / (ld)(N - 1));
+        sinT[q] = (double)sinl(lpi * (ld)q / (ld)(N - 1));
+    }
+    d.cosT = h->upload(cosT);
+    d.sinT = h->upload(sinT);
+    std::vector<double> tau(6 * N);
+    for (int n = 0; n < N; n++) {
+        // chebyshev.f90:362-381
+        double dn2 = (double)n * (double)n, monen = (n % 2 == 0) ? 1.0 : -1.0;
+        tau[n] = 1.0;
+        tau[N + n] = monen;
+        tau[2 * N + n] = 2.0 * dn2;
+        tau[3 * N + n] = 2.0 * dn2 * (-1.0) * monen;
+        tau[4 * N + n] = 4.0 * (1.0 / 3.0) * dn2 * (dn2 - 1.0);
+        tau[5 * N + n] = 4.0 * (1.0 / 3.0) * dn2 * (dn2 - 1.0) * monen;
+    }
+    d.tau = h->upload(tau);
+    std::vector<double2> tw(h->nphi);
+    for (int k = 0; k < h->nphi; k++) {
+        ld ang = 2.0L * lpi * (ld)k / (ld)h->nphi;
+        tw[k] = make_double2((double)cosl(ang), -(double)sinl(ang));
+    }
+    d.twid = h->upload(tw);
+    std::vector<int> rad = factor_radices(h->nphi);
+    if ((int)rad.size() > PZ_MAX_RADICES || *std::max_element(rad.begin(), rad.end()) > 64)
+        throw std::invalid_argument("n_phi has a prime factor above 64");
+    d.nrad = (int)rad.size();
+    for (size_t i = 0; i < rad.size(); i++) d.radices[i] = rad[i];
+    d.idx2m = h->upload(h->idx2m);
+    std::vector<double> ones(h->nm, 1.0); // no hyperdiffusion (horizontal.f90:126-131)
+    d.hdifT = h->upload(ones);
+    d.hdifXi = h->upload(ones);
+    d.hdifV = h->upload(ones);
+    d.topt = h->upload(bc_pattern(h, c.t_bot, c.t_top, true));
+    d.bott = h->upload(bc_pattern(h, c.t_bot, c.t_top, false));
+    // the composition pattern is read from t_bot/t_top (horizontal.f90:176-198)
+    d.topxi = h->upload(bc_pattern(h, c.t_bot, c.t_top, true));
+    d.botxi = h->upload(bc_pattern(h, c.t_bot, c.t_top, false));
+    // fields and time arrays
+    const size_t F = (size_t)h->nm * N;
+    d.us = h->dalloc<cd>(F); d.up = h->dalloc<cd>(F); d.om = h->dalloc<cd>(F); d.psi = h->dalloc<cd>(F);
+    d.temp = h->dalloc<cd>(F); d.xi = h->dalloc<cd>(F); d.temp_hat = h->dalloc<cd>(F);
+    d.xi_hat = h->dalloc<cd>(F); d.psi_hat = h->dalloc<cd>(F); d.dtemp = h->dalloc<cd>(F);
+    d.dVsT = h->dalloc<cd>(F); d.dVsXi = h->dalloc<cd>(F); d.dVsOm = h->dalloc<cd>(F);
+    const Scheme &s = h->sch;
+    d.T_old = h->dalloc<cd>(F * s.nold); d.T_impl = h->dalloc<cd>(F * s.nimp); d.T_expl = h->dalloc<cd>(F * s.nexp);
+    d.X_old = h->dalloc<cd>(F * s.nold); d.X_impl = h->dalloc<cd>(F * s.nimp); d.X_expl = h->dalloc<cd>(F * s.nexp);
+    d.P_old = h->dalloc<cd>(F * s.nold); d.P_impl = h->dalloc<cd>(F * s.nimp); d.P_expl = h->dalloc<cd>(F * s.nexp);
+    d.dtr = h->dalloc<double>(N);
+    d.dth = h->dalloc<double>(N);
+    // factorised matrices
+    const int nm = h->nm, nb2 = N - 2;
+    d.T_A1 = h->dalloc<double>(nm * 4); d.T_A2 = h->dalloc<double>((size_t)nm * 2 * nb2);
+    d.T_A3 = h->dalloc<double>((size_t)nm * 2 * nb2); d.T_A4 = h->dalloc<double>((size_t)nm * PZ_T_LD * nb2);
+    d.T_p1 = h->dalloc<int>(nm * 2); d.T_p4 = h->dalloc<int>((size_t)nm * nb2); d.T_fac = h->dalloc<double>(F);
+    d.X_A1 = h->dalloc<double>(nm * 4); d.X_A2 = h->dalloc<double>((size_t)nm * 2 * nb2);
+    d.X_A3 = h->dalloc<double>((size_t)nm * 2 * nb2); d.X_A4 = h->dalloc<double>((size_t)nm * PZ_T_LD * nb2);
+    d.X_p1 = h->dalloc<int>(nm * 2); d.X_p4 = h->dalloc<int>((size_t)nm * nb2); d.X_fac = h->dalloc<double>(F);
+    d.P_A1 = h->dalloc<cd>(nm * 16); d.P_A2 = h->dalloc<cd>((size_t)nm * PZ_P_NT * N);
+    d.P_A3 = h->dalloc<cd>((size_t)nm * PZ_P_NT * N); d.P_A4 = h->dalloc<cd>((size_t)nm * PZ_P_LD * N);
+    d.P_p1 = h->dalloc<int>(nm * 4); d.P_p4 = h->dalloc<int>((size_t)nm * N); d.P_fac = h->dalloc<double>(F);
+    d.info = h->dalloc<int>(3 * nm);
+    d.ss = h->dalloc<PzScheme>(1);
+    build_operators(h);
+    // LDS budgets
+    h->lds_radial = (5 * (size_t)h->nm + 6 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
+    h->lds_mphase = (size_t)MPHASE_NVEC * N * sizeof(cd) + 2 * (size_t)N * sizeof(double);
+    h->lds_start = 15 * (size_t)N * sizeof(cd);
+    {
+        size_t nbp = N - 4;
+        size_t psi = (size_t)(PZ_P_LD * nbp + 2 * PZ_P_NT * nbp + 16) * sizeof(cd) + N * sizeof(double) +
+                     nbp * sizeof(int) + nbp * sizeof(cd) + 7 * 16;
+        h->lds_build = 4 * (size_t)N * sizeof(double) + psi + 64;
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    size_t lim = prop.sharedMemPerBlock;
+    size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start});
+    if (need > 65536) {
+        if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
+        HIPCHK(hipFuncSetAttribute((const void *)k_build_lhs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_build));
+        HIPCHK(hipFuncSetAttribute((const void *)k_mphase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_mphase));
+        HIPCHK(hipFuncSetAttribute((const void *)k_radial_loop, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_radial));
+        HIPCHK(hipFuncSetAttribute((const void *)k_start, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_start));
+    }
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+}
+
+extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
+{
+    if (!cfg || !out) return -1;
+    *out = nullptr;
+    pz_handle *h = new pz_handle();
+    h->c = *cfg;
+    const pz_config &c = *cfg;
+    std::string why;
+    if (c.l_cheb_coll) why = "only cheb_method='integ' is implemented";
+    else if (!c.l_direct_solve) why = "only matrix_solve='direct' is implemented";
+    else if (c.l_non_rot) why = "non-rotating runs are not implemented";
+    else if (c.radratio <= 0.0) why = "full-disk geometry is not implemented";
+    else if (c.n_r_max < 8 || c.n_r_max > 257) why = "n_r_max must be in [8, 257]";
+    else if (c.n_cheb_max < 1 || c.n_cheb_max > c.n_r_max) why = "n_cheb_max must be in [1, n_r_max]";
+    else if (c.m_max < 3 || c.minc < 1 || c.m_max % c.minc) why = "invalid m_max/minc";
+    if (!why.empty()) {
+        int rc = fail(h, -2, why);
+        std::fprintf(stderr, "pz_create: %s\n", why.c_str());
+        delete h;
+        return rc;
+    }
+    try {
+        setup(h);
+    } catch (std::exception &e) {
+        std::fprintf(stderr, "pz_create: %s\n", e.what());
+        delete h;
+        return -3;
+    }
+    *out = h;
+    return 0;
+}
+
+extern "C" int pz_destroy(pz_handle *h)
+{
+    delete h;
+    return 0;
+}
+
+extern "C" int pz_dims(const pz_handle *h, int *nm, int *nr, int *np)
+{
+    if (!h) return -1;
+    *nm = h->nm;
+    *nr = h->N;
+    *np = h->nphi;
+    return 0;
+}
+
+extern "C" void *pz_stream(pz_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+static cd *field_ptr(pz_handle *h, int f)
+{
+    PzDev &d = h->d;
+    switch (f) {
+    case PZ_US: return d.us;
+    case PZ_UP: return d.up;
+    case PZ_OM: return d.om;
+    case PZ_PSI: return d.psi;
+    case PZ_TEMP: return d.temp;
+    case PZ_XI: return d.xi;
+    case PZ_TEMP_HAT: return d.temp_hat;
+    case PZ_XI_HAT: return d.xi_hat;
+    case PZ_PSI_HAT: return d.psi_hat;
+    case PZ_DTEMP: return d.dtemp;
+    default: return nullptr;
+    }
+}
+
+// Fortran layout (n_m, n_r), m fastest  <->  device [m][r]
+static void to_device_layout(const pz_handle *h, const double *host, std::vector<double2> &v)
+{
+    v.resize((size_t)h->nm * h->N);
+    for (int k = 0; k < h->N; k++)
+        for (int i = 0; i < h->nm; i++) {
+            size_t s = 2 * ((size_t)k * h->nm + i);
+            v[(size_t)i * h->N + k] = make_double2(host[s], host[s + 1]);
+        }
+}
+static void from_device_layout(const pz_handle *h, const std::vector<double2> &v, double *host)
+{
+    for (int k = 0; k < h->N; k++)
+        for (int i = 0; i < h->nm; i++) {
+            size_t s = 2 * ((size_t)k * h->nm + i);
+            host[s] = v[(size_t)i * h->N + k].x;
+            host[s + 1] = v[(size_t)i * h->N + k].y;
+        }
+}
+
+extern "C" int pz_set_field(pz_handle *h, int f, const double *host)
+{
+    if (!h || !host) return -1;
+    cd *p = field_ptr(h, f);
+    if (!p) return fail(h, -1, "unknown field");
+    try {
+        std::vector<double2> v;
+        to_device_layout(h, host, v);
+        HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(cd), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_get_field(pz_handle *h, int f, double *host)
+{
+    if (!h || !host) return -1;
+    cd *p = field_ptr(h, f);
+    if (!p) return fail(h, -1, "unknown field");
+    try {
+        std::vector<double2> v((size_t)h->nm * h->N);
+        HIPCHK(hipMemcpyAsync(v.data(), p, v.size() * sizeof(cd), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        from_device_layout(h, v, host);
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+// Gaspari-Cohn pieces (useful.f90:295-341)
+static double gc_center(double c, double L)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    return pi * L * L * L * (1.0 - std::exp(-2.0 * c / L)) - 2.0 * pi * c * L * (c + L) * std::exp(-2.0 * c / L);
+}
+static double gc_middle(double r, double c, double L)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    return pi / 3.0 * L * r * (r + 3.0 * L) * std::exp(-r / L) +
+           (2.0 * pi * L * L * (c + L) * (c + L)) / r * std::exp(-2.0 * c / L) * (1.0 - (1.0 - r / (c + L)) * std::exp(r / L)) +
+           pi * L * L * L * (std::exp(-r / L) - std::exp((r - 2.0 * c) / L));
+}
+static double gc_edge(double r, double c, double L)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    return 2.0 * pi * L / r * std::exp(-r / L) *
+           (0.5 * (r * (r + L) * (2.0 * c - r)) + 1.0 / 3.0 * (std::pow(r - c, 3) - c * c * c) - L * (c + L) * (r - c + L) +
+            L * (c + L) * (c + L) * std::exp((r - 2.0 * c) / L));
+}
+
+// initProp (init_fields.f90:260-360), deterministic branches; adds to v[m][r]
+static int init_prop(pz_handle *h, std::vector<double2> &v, int init, double amp)
+{
+    const int N = h->N;
+    const double pi = 4.0 * std::atan(1.0);
+    if (init > 0) {
+        if (init % h->minc != 0 || init > h->mmax) return -3;
+        int idx = init / h->minc;
+        for (int n = 0; n < N; n++) v[(size_t)idx * N + n].x += std::sin(pi * (h->r[n] - h->r_icb)) * amp;
+        return 0;
+    }
+    if (init != -1) return -4; // the unseeded random branch is not reproducible
+    const double sigma_r = 0.1 / std::sqrt(2.0);
+    double rcm = 0.5 * (h->r_cmb + h->r_icb);
+    int ir = 0;
+    for (int n = 1; n < N; n++)
+        if (std::fabs(h->r[n] - rcm) < std::fabs(h->r[ir] - rcm)) ir = n;
+    rcm = h->r[ir];
+    const double c1 = 0.5 * (h->r[0] - rcm), c2 = 0.5 * (rcm - h->r[N - 1]), L = 0.5 * sigma_r;
+    std::vector<double> gasp(N, 0.0);
+    for (int n = 0; n < N; n++) {
+        double r = h->r[n];
+        if (n == ir) gasp[n] = 1.0;
+        else if (r < rcm - c2) gasp[n] = gc_edge(rcm - r, c2, L);
+        else if (r >= rcm - c2 && r < rcm) gasp[n] = gc_middle(rcm - r, c2, L);
+        else if (r > rcm && r <= rcm + c1) gasp[n] = gc_middle(r - rcm, c1, L);
+        else if (r > rcm + c1) gasp[n] = gc_edge(r - rcm, c1, L);
+    }
+    for (int n = 0; n < N; n++) {
+        if (n < ir) gasp[n] /= gc_center(c1, L);
+        else if (n > ir) gasp[n] /= gc_center(c2, L);
+    }
+    // fourier::fft of the azimuthal profile (1/n_phi normalisation)
+    const int np = h->nphi;
+    const double phi0 = pi / h->minc;
+    std::vector<double> prof(np);
+    for (int j = 0; j < np; j++) {
+        double phi = (double)j * 2.0 * pi / h->minc / (double)np;
+        prof[j] = std::exp(-(phi - phi0) * (phi - phi0) / (0.2 * 0.2));
+    }
+    const ld lpi = 3.14159265358979323846264338327950288L;
+    std::vector<double2> spec(h->nm);
+    for (int i = 0; i < h->nm; i++) {
+        ld sr = 0, si = 0;
+        for (int j = 0; j < np; j++) {
+            ld ang = 2.0L * lpi * (ld)((long)i * j % np) / (ld)np;
+            sr += (ld)prof[j] * cosl(ang);
+            si -= (ld)prof[j] * sinl(ang);
+        }
+        spec[i] = make_double2((double)(sr / np), (double)(si / np));
+    }
+    for (int n = 0; n < N; n++) {
+        double cr = amp * gasp[n];
+        for (int i = 0; i < h->nm; i++) {
+            v[(size_t)i * N + n].x += cr * spec[i].x;
+            v[(size_t)i * N + n].y += cr * spec[i].y;
+        }
+    }
+    return 0;
+}
+
+extern "C" int pz_init_fields(pz_handle *h, int init_t, double amp_t, int init_xi, double amp_xi, int init_u,
+                              double amp_u)
+{
+    if (!h) return -1;
+    try {
+        const size_t F = (size_t)h->nm * h->N;
+        auto edit = [&](cd *dptr, auto fn) {
+            std::vector<double2> v(F);
+            HIPCHK(hipMemcpy(v.data(), dptr, F * sizeof(cd), hipMemcpyDeviceToHost));
+            int rc = fn(v);
+            HIPCHK(hipMemcpy(dptr, v.data(), F * sizeof(cd), hipMemcpyHostToDevice));
+            return rc;
+        };
+        int rc = 0;
+        if (init_t != 0 && h->l_heat) rc |= edit(h->d.temp, [&](std::vector<double2> &v) { return init_prop(h, v, init_t, amp_t); });
+        if (init_xi != 0 && h->l_chem) rc |= edit(h->d.xi, [&](std::vector<double2> &v) { return init_prop(h, v, init_xi, amp_xi); });
+        if (init_u != 0) {
+            // initU (init_fields.f90:455-509)
+            const double pi = 4.0 * std::atan(1.0);
+            const int N = h->N;
+            if (init_u > 0) {
+                if (init_u % h->minc != 0 || init_u > h->mmax) return fail(h, -3, "init_u out of range");
+                int idx = init_u / h->minc;
+                rc |= edit(h->d.us, [&](std::vector<double2> &v) {
+                    for (int n = 0; n < N; n++) v[(size_t)idx * N + n].x += std::sin(pi * (h->r[n] - h->r_icb)) * amp_u;
+                    return 0;
+                });
+            } else {
+                rc |= edit(h->d.up, [&](std::vector<double2> &v) {
+                    for (int n = 0; n < N; n++) v[n].x += amp_u * std::sin(pi * (h->r[n] - h->r_icb));
+                    return 0;
+                });
+            }
+        }
+        if (rc) return fail(h, rc, "unsupported initial condition");
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_start(pz_handle *h)
+{
+    if (!h) return -1;
+    try {
+        PzScheme s;
+        std::memset(&s, 0, sizeof(s));
+        for (int i = 0; i < 8; i++) s.dt[i] = h->c.dtMax; // init_fields.f90:111
+        s.lMatNext = 1;                                     // step_time.f90:115
+        HIPCHK(hipMemcpyAsync(h->d.ss, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+        const size_t F = (size_t)h->nm * h->N;
+        const Scheme &sc = h->sch;
+        cd *slots[] = {h->d.T_old, h->d.T_impl, h->d.T_expl, h->d.X_old, h->d.X_impl, h->d.X_expl,
+                       h->d.P_old, h->d.P_impl, h->d.P_expl};
+        size_t cnt[] = {(size_t)sc.nold, (size_t)sc.nimp, (size_t)sc.nexp, (size_t)sc.nold, (size_t)sc.nimp,
+                        (size_t)sc.nexp, (size_t)sc.nold, (size_t)sc.nimp, (size_t)sc.nexp};
+        for (int i = 0; i < 9; i++) HIPCHK(hipMemsetAsync(slots[i], 0, cnt[i] * F * sizeof(cd), h->stream));
+        hipLaunchKernelGGL(k_set_weights, dim3(1), dim3(1), 0, h->stream, h->d);
+        hipLaunchKernelGGL(k_start, dim3(h->nm), dim3(256), h->lds_start, h->stream, h->d);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+        h->started = true;
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+// the stage loop of time_loop (step_time.f90:215-348) for one step
+static void launch_step(pz_handle *h)
+{
+    const Scheme &s = h->sch;
+    for (int stage = 1; stage <= s.nstages; stage++) {
+        const int exp_calc = s.l_exp_calc[stage - 1];
+        if (exp_calc)
+            hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, h->d, stage - 1,
+                               stage == 1 ? 1 : 0);
+        if (stage == 1) {
+            hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, h->d);
+            hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, h->d, 0);
+        }
+        hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(256), h->lds_mphase, h->stream, h->d, stage, exp_calc);
+    }
+}
+
+extern "C" int pz_step(pz_handle *h, int nsteps)
+{
+    if (!h) return -1;
+    if (!h->started) return fail(h, -5, "pz_start has not been called");
+    try {
+        if (h->use_graph && !h->gexec) {
+            HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            launch_step(h);
+            HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
+            HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < nsteps; i++) {
+            if (h->gexec) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+            else launch_step(h);
+        }
+        HIPCHK(hipGetLastError());
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+static int check_state(pz_handle *h, PzScheme *out)
+{
+    PzScheme s;
+    HIPCHK(hipMemcpyAsync(&s, h->d.ss, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+    std::vector<int> info(3 * h->nm);
+    HIPCHK(hipMemcpyAsync(info.data(), h->d.info, info.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (out) *out = s;
+    if (s.error) return fail(h, -6, "time step too small (dt < dtMin)");
+    for (int v : info)
+        if (v) return fail(h, v, "singular matrix in the implicit solve");
+    return 0;
+}
+
+extern "C" int pz_time(pz_handle *h, double *time, double *dt)
+{
+    if (!h) return -1;
+    try {
+        PzScheme s;
+        int rc = check_state(h, &s);
+        if (time) *time = s.time;
+        if (dt) *dt = s.dt[0];
+        return rc;
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+}
+
+// rInt_R, Chebyshev branch (integration.f90:18-58)
+static double rint_r(const pz_handle *h, const std::vector<double> &f)
+{
+    const int N = h->N;
+    std::vector<double> f2(N), c(N);
+    for (int n = 0; n < N; n++) f2[n] = f[n] / h->drx[n];
+    for (int k = 0; k < N; k++) {
+        double s = 0.0;
+        for (int j = 1; j < N - 1; j++) s += f2[j] * std::cos(4.0 * std::atan(1.0) * (double)((long)j * k % (2 * N - 2)) / (double)(N - 1));
+        c[k] = h->cheb_fac * (f2[0] + ((k & 1) ? -1.0 : 1.0) * f2[N - 1] + 2.0 * s);
+    }
+    c[0] *= 0.5;
+    c[N - 1] *= 0.5;
+    double s = c[0];
+    for (int nc = 3; nc <= N; nc += 2) s -= 1.0 / (double)(nc * (nc - 2)) * c[nc - 1];
+    return 2.0 * std::sqrt(2.0 / (double)(N - 1)) * s;
+}
+
+extern "C" int pz_ekin_3d(pz_handle *h, double out[4])
+{
+    if (!h || !out) return -1;
+    try {
+        const int N = h->N;
+        double *dprof = nullptr;
+        HIPCHK(hipMallocAsync((void **)&dprof, 3 * N * sizeof(double), h->stream));
+        hipLaunchKernelGGL(k_ekin_profiles, dim3((N + 63) / 64), dim3(64), 0, h->stream, h->d, dprof);
+        std::vector<double> prof(3 * N);
+        HIPCHK(hipMemcpyAsync(prof.data(), dprof, 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipFreeAsync(dprof, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        // outputs.f90:552-571
+        const double pi = 4.0 * std::atan(1.0), tiny = 10.0 * 2.220446049250313e-16;
+        auto ro = [&](double v) { return std::fabs(v) < tiny ? 0.0 : v; };
+        std::vector<double> t(N);
+        for (int n = 0; n < N; n++) t[n] = prof[n] * h->r[n] * h->height[n];
+        out[0] = ro(pi * rint_r(h, t));
+        for (int n = 0; n < N; n++) t[n] = prof[N + n] * h->r[n] * h->height[n];
+        out[1] = ro(pi * rint_r(h, t));
+        for (int n = 0; n < N; n++) t[n] = prof[n] * h->beta[n] * h->beta[n] * h->r[n] * std::pow(h->height[n], 3) / 12.0;
+        out[2] = ro(pi * rint_r(h, t));
+        for (int n = 0; n < N; n++) t[n] = prof[2 * N + n] * h->r[n] * h->height[n];
+        out[3] = ro(pi * rint_r(h, t));
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+// ---- unit operators on caller host buffers --------------------------------
+static int vec_op(pz_handle *h, const double *f, double *df, double *ddf, int n_m, int op, int l_dct_in)
+{
+    try {
+        const int N = h->N;
+        const size_t S = (size_t)n_m * N;
+        std::vector<double2> v(S);
+        for (int k = 0; k < N; k++)
+            for (int i = 0; i < n_m; i++) {
+                size_t s = 2 * ((size_t)k * n_m + i);
+                v[(size_t)i * N + k] = make_double2(f[s], f[s + 1]);
+            }
+        cd *a = nullptr, *b = nullptr, *c = nullptr;
+        HIPCHK(hipMalloc(&a, S * sizeof(cd)));
+        HIPCHK(hipMalloc(&b, S * sizeof(cd)));
+        HIPCHK(hipMalloc(&c, S * sizeof(cd)));
+        HIPCHK(hipMemcpy(a, v.data(), S * sizeof(cd), hipMemcpyHostToDevice));
+        if (op == 0) hipLaunchKernelGGL(k_costf_vecs, dim3(n_m), dim3(256), 2 * N * sizeof(cd), h->stream, h->d, a);
+        else hipLaunchKernelGGL(k_dr_vecs, dim3(n_m), dim3(256), 4 * N * sizeof(cd), h->stream, h->d, a, b,
+                                op == 2 ? c : nullptr, l_dct_in);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        auto back = [&](cd *src, double *dst) {
+            HIPCHK(hipMemcpy(v.data(), src, S * sizeof(cd), hipMemcpyDeviceToHost));
+            for (int k = 0; k < N; k++)
+                for (int i = 0; i < n_m; i++) {
+                    size_t s = 2 * ((size_t)k * n_m + i);
+                    dst[s] = v[(size_t)i * N + k].x;
+                    dst[s + 1] = v[(size_t)i * N + k].y;
+                }
+        };
+        if (op == 0) back(a, df);
+        else {
+            back(b, df);
+            if (op == 2) back(c, ddf);
+        }
+        (void)hipFree(a);
+        (void)hipFree(b);
+        (void)hipFree(c);
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_costf(pz_handle *h, double *f, int n_m)
+{
+    if (!h || !f || n_m <= 0) return -1;
+    return vec_op(h, f, f, nullptr, n_m, 0, 0);
+}
+extern "C" int pz_get_dr(pz_handle *h, const double *f, double *df, int n_m, int l_dct_in)
+{
+    if (!h || !f || !df || n_m <= 0) return -1;
+    return vec_op(h, f, df, nullptr, n_m, 1, l_dct_in);
+}
+extern "C" int pz_get_ddr(pz_handle *h, const double *f, double *df, double *ddf, int n_m, int l_dct_in)
+{
+    if (!h || !f || !df || !ddf || n_m <= 0) return -1;
+    return vec_op(h, f, df, ddf, n_m, 2, l_dct_in);
+}
+
+extern "C" int pz_fft(pz_handle *h, const double *grid, double *spec, int n_rad)
+{
+    if (!h || !grid || !spec || n_rad <= 0) return -1;
+    try {
+        double *g = nullptr;
+        cd *s = nullptr;
+        HIPCHK(hipMalloc(&g, (size_t)n_rad * h->nphi * sizeof(double)));
+        HIPCHK(hipMalloc(&s, (size_t)n_rad * h->nm * sizeof(cd)));
+        HIPCHK(hipMemcpy(g, grid, (size_t)n_rad * h->nphi * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_phi_fft, dim3(n_rad), dim3(256), 2 * h->nphi * sizeof(cd), h->stream, h->d, g, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(spec, s, (size_t)n_rad * h->nm * sizeof(cd), hipMemcpyDeviceToHost));
+        (void)hipFree(g);
+        (void)hipFree(s);
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_ifft(pz_handle *h, const double *spec, double *grid, int n_rad)
+{
+    if (!h || !grid || !spec || n_rad <= 0) return -1;
+    try {
+        double *g = nullptr;
+        cd *s = nullptr;
+        HIPCHK(hipMalloc(&g, (size_t)n_rad * h->nphi * sizeof(double)));
+        HIPCHK(hipMalloc(&s, (size_t)n_rad * h->nm * sizeof(cd)));
+        HIPCHK(hipMemcpy(s, spec, (size_t)n_rad * h->nm * sizeof(cd), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_phi_ifft, dim3(n_rad), dim3(256), 2 * h->nphi * sizeof(cd), h->stream, h->d, s, g);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(grid, g, (size_t)n_rad * h->nphi * sizeof(double), hipMemcpyDeviceToHost));
+        (void)hipFree(g);
+        (void)hipFree(s);
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+// which: 0 = m-loop stage kernel (stage 1 of the scheme, exp_calc), 1 = radial loop
+extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
+{
+    if (!h || !avg_ms || reps <= 0) return -1;
+    if (!h->started) return fail(h, -5, "pz_start has not been called");
+    try {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, h->stream));
+        for (int i = 0; i < reps; i++) {
+            if (which == 0)
+                hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(256), h->lds_mphase, h->stream, h->d, 1, 1);
+            else
+                hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, h->d, 0, 0);
+        }
+        HIPCHK(hipEventRecord(e1, h->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = (double)ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}

@@ -1,0 +1,873 @@
+// Device code of libpizza_amd: CDNA4 (gfx950) kernels of the IMEX step.
+// Included once by pz_api.hip.
+#pragma once
+#include "pz_internal.h"
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ cd mk(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cd cscale(double s, cd a) { return mk(s * a.x, s * a.y); }
+__device__ __forceinline__ cd cconj(cd a) { return mk(a.x, -a.y); }
+__device__ __forceinline__ cd cdivr(cd a, double s) { return mk(a.x / s, a.y / s); }
+// (0 + i s) * a
+__device__ __forceinline__ cd cimul(double s, cd a) { return mk(-s * a.y, s * a.x); }
+__device__ __forceinline__ cd cdiv(cd a, cd b)
+{
+    // Smith's algorithm
+    if (fabs(b.x) >= fabs(b.y)) {
+        double r = b.y / b.x, d = b.x + b.y * r;
+        return mk((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+    } else {
+        double r = b.x / b.y, d = b.y + b.x * r;
+        return mk((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+    }
+}
+// matrix element times vector element: real or complex matrix
+__device__ __forceinline__ cd mmul(double a, cd v) { return mk(a * v.x, a * v.y); }
+__device__ __forceinline__ cd mmul(cd a, cd v) { return cmul(a, v); }
+__device__ __forceinline__ cd mdiv(cd v, double a) { return mk(v.x / a, v.y / a); }
+__device__ __forceinline__ cd mdiv(cd v, cd a) { return cdiv(v, a); }
+__device__ __forceinline__ double mabs1(double a) { return fabs(a); }
+__device__ __forceinline__ double mabs1(cd a) { return fabs(a.x) + fabs(a.y); }
+__device__ __forceinline__ bool mnz(double a) { return a != 0.0; }
+__device__ __forceinline__ bool mnz(cd a) { return a.x != 0.0 || a.y != 0.0; }
+__device__ __forceinline__ double mrecip(double a) { return 1.0 / a; }
+__device__ __forceinline__ cd mrecip(cd a) { return cdiv(mk(1.0, 0.0), a); }
+__device__ __forceinline__ double mscale(double a, double s) { return a * s; }
+__device__ __forceinline__ cd mscale(cd a, double s) { return mk(a.x * s, a.y * s); }
+__device__ __forceinline__ double mmm(double a, double b) { return a * b; }
+__device__ __forceinline__ cd mmm(cd a, cd b) { return cmul(a, b); }
+__device__ __forceinline__ double mneg(double a) { return -a; }
+__device__ __forceinline__ cd mneg(cd a) { return mk(-a.x, -a.y); }
+__device__ __forceinline__ double madd(double a, double b) { return a + b; }
+template <class M> __device__ __forceinline__ M mzero();
+template <> __device__ __forceinline__ double mzero<double>() { return 0.0; }
+template <> __device__ __forceinline__ cd mzero<cd>() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ cd madd(cd a, cd b) { return cadd(a, b); }
+
+// wave-uniform lane read of a double2 (lane index must be uniform)
+__device__ __forceinline__ double rl(double v, int lane)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ cd rlc(cd v, int lane) { return mk(rl(v.x, lane), rl(v.y, lane)); }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Stockham mixed-radix FFT in LDS for ntr transforms of length n
+// (dir = -1 forward, +1 backward, unnormalised).  Result pointer returned.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ cd twd(const PzDev &p, int k, int dir)
+{
+    cd w = p.twid[k];
+    return dir < 0 ? w : mk(w.x, -w.y);
+}
+
+__device__ cd *fft_lds(const PzDev &p, cd *a, cd *b, int n, int ntr, int dir)
+{
+    int Ns = 1;
+    for (int s = 0; s < p.nrad; s++) {
+        const int R = p.radices[s];
+        const int nb = n / R;
+        const int tot = nb * ntr;
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+            int tr = t / nb, j = t - tr * nb;
+            const cd *x = a + tr * n;
+            cd *y = b + tr * n;
+            int jm = j % Ns;
+            int tstep = (n / (Ns * R)) * jm;
+            int idxD = (j / Ns) * Ns * R + jm;
+            if (R == 2) {
+                cd v0 = x[j], v1 = cmul(x[j + nb], twd(p, tstep, dir));
+                y[idxD] = cadd(v0, v1);
+                y[idxD + Ns] = csub(v0, v1);
+            } else if (R == 4) {
+                cd v0 = x[j];
+                cd v1 = cmul(x[j + nb], twd(p, tstep, dir));
+                cd v2 = cmul(x[j + 2 * nb], twd(p, 2 * tstep, dir));
+                cd v3 = cmul(x[j + 3 * nb], twd(p, 3 * tstep, dir));
+                cd a0 = cadd(v0, v2), b0 = csub(v0, v2), c0 = cadd(v1, v3);
+                cd d0 = cimul((double)dir, csub(v1, v3));
+                y[idxD] = cadd(a0, c0);
+                y[idxD + Ns] = cadd(b0, d0);
+                y[idxD + 2 * Ns] = csub(a0, c0);
+                y[idxD + 3 * Ns] = csub(b0, d0);
+            } else if (R == 3) {
+                const double s3 = 0.86602540378443864676;
+                cd v0 = x[j];
+                cd v1 = cmul(x[j + nb], twd(p, tstep, dir));
+                cd v2 = cmul(x[j + 2 * nb], twd(p, 2 * tstep, dir));
+                cd t1 = cadd(v1, v2);
+                cd t2 = cimul((double)dir * s3, csub(v1, v2));
+                cd m0 = mk(v0.x - 0.5 * t1.x, v0.y - 0.5 * t1.y);
+                y[idxD] = cadd(v0, t1);
+                y[idxD + Ns] = cadd(m0, t2);
+                y[idxD + 2 * Ns] = csub(m0, t2);
+            } else {
+                // generic radix: direct DFT of length R, inputs re-read from LDS
+                int q = n / R;
+                for (int k = 0; k < R; k++) {
+                    cd acc = mk(0.0, 0.0);
+                    for (int r = 0; r < R; r++) {
+                        cd v = cmul(x[j + r * nb], twd(p, r * tstep, dir));
+                        acc = cadd(acc, cmul(v, twd(p, ((r * k) % R) * q, dir)));
+                    }
+                    y[idxD + k * Ns] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        cd *t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+__device__ __forceinline__ double cc22real(cd a, cd b, int m)
+{
+    return m == 0 ? a.x * b.x : 2.0 * (a.x * b.x + a.y * b.y);
+}
+
+__device__ double block_reduce(double v, double *scratch, int op /*0 sum, 1 max*/)
+{
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        double u = __shfl_xor(v, o, 64);
+        v = op == 0 ? v + u : fmax(v, u);
+    }
+    __syncthreads();
+    if (lane == 0) scratch[w] = v;
+    __syncthreads();
+    double r = scratch[0];
+    for (int i = 1; i < nw; i++) r = op == 0 ? r + scratch[i] : fmax(r, scratch[i]);
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Radial loop: rloop::radial_loop (radial_loop.f90:264-392) for one radius
+// per workgroup.  Real fields travel in pairs through one complex FFT.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_radial_loop(PzDev p, int is, int courant)
+{
+    extern __shared__ double smem[];
+    const int n = p.nphi, nm = p.nm, N = p.N, nr = blockIdx.x;
+    const size_t F = (size_t)nm * N;
+    cd *spec = (cd *)smem;           // 5 x nm
+    cd *bufa = spec + 5 * nm;        // 3 x n
+    cd *bufb = bufa + 3 * n;         // 3 x n
+    double *red = (double *)(bufb + 3 * n);
+    const int nin = (p.l_heat || p.l_chem) ? (p.l_chem ? 3 : 2) : 2;
+
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) {
+        size_t q = (size_t)i * N + nr;
+        spec[i] = p.us[q];
+        spec[nm + i] = p.up[q];
+        spec[2 * nm + i] = p.om[q];
+        spec[3 * nm + i] = p.l_heat ? p.temp[q] : mk(0, 0);
+        spec[4 * nm + i] = p.l_chem ? p.xi[q] : mk(0, 0);
+    }
+    __syncthreads();
+    // Reynolds-stress m-sum (radial_loop.f90:267-278)
+    double part = 0.0;
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) {
+        int m = p.idx2m[i];
+        cd usf = m == 0 ? cscale(0.5 * p.ek * p.CorFac * p.ekpump[nr], spec[nm + i]) : spec[i];
+        part += cc22real(usf, spec[2 * nm + i], m);
+    }
+    double usom = block_reduce(part, red, 0);
+    // pack pairs (us,up), (om,T), (xi,0) as Hermitian spectra of z = a + i b
+    for (int t = threadIdx.x; t < nin * n; t += blockDim.x) {
+        int tr = t / n, k = t - tr * n;
+        const cd *A = spec + (2 * tr) * nm, *B = spec + (2 * tr + 1) * nm;
+        bool hasB = (2 * tr + 1) < 5;
+        cd z = mk(0, 0);
+        if (k == 0) {
+            z = mk(A[0].x, hasB ? B[0].x : 0.0);
+        } else if (k < nm) {
+            cd a = A[k], b = hasB ? B[k] : mk(0, 0);
+            z = mk(a.x - b.y, a.y + b.x); // a + i b
+        } else if (n - k < nm && n - k > 0) {
+            int kk = n - k;
+            cd a = cconj(A[kk]), b = hasB ? cconj(B[kk]) : mk(0, 0);
+            z = mk(a.x - b.y, a.y + b.x);
+        }
+        bufa[t] = z;
+    }
+    __syncthreads();
+    cd *g = fft_lds(p, bufa, bufb, n, nin, +1);
+    cd *o = (g == bufa) ? bufb : bufa;
+    // Courant (courant.f90:41-76)
+    if (courant) {
+        double vr = 0.0, vh = 0.0;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            cd z = g[j];
+            vr = fmax(vr, z.x * z.x);
+            vh = fmax(vh, z.y * z.y);
+        }
+        vr = block_reduce(vr, red, 1);
+        vh = block_reduce(vh, red, 1);
+        if (threadIdx.x == 0) {
+            double cf = p.courfac;
+            vr = cf * cf * vr;
+            vh = cf * cf * vh;
+            double dtrk = 1.0e10, dthk = 1.0e10;
+            if (vr != 0.0) dtrk = fmin(dtrk, sqrt(p.delxr2[nr] / vr));
+            if (vh != 0.0) dthk = fmin(dthk, sqrt(p.delxh2[nr] / vh));
+            p.dtr[nr] = dtrk;
+            p.dth[nr] = dthk;
+        }
+    }
+    // nonlinear products (radial_loop.f90:316-325): (upX + i r us X)
+    const double rr = p.r[nr];
+    const int nout = p.l_chem ? 3 : (p.l_heat ? 2 : 1);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        cd z0 = g[j], z1 = g[n + j];
+        double us = z0.x, up = z0.y, om = z1.x, T = z1.y;
+        o[j] = mk(up * om, rr * us * om);
+        if (p.l_heat) o[n + j] = mk(up * T, rr * us * T);
+        if (p.l_chem) {
+            double X = g[2 * n + j].x;
+            o[(p.l_heat ? 2 : 1) * n + j] = mk(up * X, rr * us * X);
+        }
+    }
+    __syncthreads();
+    cd *other = (o == bufa) ? bufb : bufa;
+    cd *Z = fft_lds(p, o, other, n, nout, -1);
+    // unpack, normalise and post-scale (radial_loop.f90:350-386)
+    const double or1 = p.or1[nr];
+    const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
+    for (int t = threadIdx.x; t < nout * nm; t += blockDim.x) {
+        int tr = t / nm, i = t - tr * nm;
+        cd zk = Z[tr * n + i], zc = cconj(Z[tr * n + (i == 0 ? 0 : n - i)]);
+        cd A = mk(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));  // spectrum of up*X
+        cd B = mk(0.5 * (zk.y - zc.y), -0.5 * (zk.x - zc.x)); // spectrum of r us X
+        A = cdivr(A, (double)n);
+        B = cdivr(B, (double)n);
+        int m = p.idx2m[i];
+        size_t q = (size_t)i * N + nr;
+        cd cim = mk(0.0, -or1 * (double)m); // -or1*ci*m
+        if (tr == 0) {
+            p.P_expl[is * F + q] = (i == 0) ? mk(-usom, 0.0) : cmul(cim, A);
+            p.dVsOm[q] = B;
+        } else if (tr == 1 && p.l_heat) {
+            cd v = cmul(cim, A);
+            p.T_expl[is * F + q] = mk(v.x - fbeta * B.x, v.y - fbeta * B.y);
+            p.dVsT[q] = B;
+        } else {
+            p.X_expl[is * F + q] = cmul(cim, A);
+            p.dVsXi[q] = B;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Courant decision and time-scheme weights on the device:
+// dt_courant (courant.f90:78-167), set_dt_array and set_weights
+// (multistep_schemes.f90:137-348; dirk_schemes.f90:275-771), lMat logic of
+// step_time.f90:277-304.
+// ---------------------------------------------------------------------------
+__device__ void scheme_weights(const PzDev &p, PzScheme *s, int *lMatNext)
+{
+    double wimp_old = s->wimp_lin[0];
+    if (p.family == 0) {
+        s->wimp[0] = 1.0;
+        s->wimp_lin[0] = p.alpha * s->dt[0];
+        s->wimp_lin[1] = (1 - p.alpha) * s->dt[0];
+        s->wexp[0] = (1.0 + 0.5 * s->dt[0] / s->dt[1]) * s->dt[0];
+        s->wexp[1] = -0.5 * s->dt[0] * s->dt[0] / s->dt[1];
+        if (s->wimp_lin[0] != wimp_old) *lMatNext = 1;
+    } else {
+        s->wimp_lin[0] = s->dt[0] * p.wimp_lin0;
+        for (int i = 0; i < PZ_MAX_STAGES; i++)
+            for (int j = 0; j < PZ_MAX_STAGES; j++) {
+                s->bimp[i][j] = s->dt[0] * p.bimp0[i][j];
+                s->bexp[i][j] = s->dt[0] * p.bexp0[i][j];
+            }
+    }
+}
+
+__global__ void k_set_weights(PzDev p)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        PzScheme *s = p.ss;
+        int dummy = 0;
+        scheme_weights(p, s, &dummy);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_courant(PzDev p)
+{
+    __shared__ double red[8];
+    PzScheme *s = p.ss;
+    double dtr = 1.0e300, dth = 1.0e300;
+    for (int i = threadIdx.x; i < p.N; i += blockDim.x) {
+        dtr = fmin(dtr, p.dtr[i]);
+        dth = fmin(dth, p.dth[i]);
+    }
+    dtr = -block_reduce(-dtr, red, 1);
+    dth = -block_reduce(-dth, red, 1);
+    if (threadIdx.x != 0) return;
+    double dtMax = p.dtMax, dt = s->dt[0];
+    double dt_r = fmin(dtr, 1000.0 * dtMax), dt_h = fmin(dth, 1000.0 * dtMax);
+    double dt_rh = fmin(dt_r, dt_h);
+    double dt_2 = fmin(0.5 * (1.0 / p.dt_fac + 1.0) * dt_rh, dtMax);
+    int l_new_dt;
+    double dt_new;
+    if (dt > dtMax) { l_new_dt = 1; dt_new = dtMax; }
+    else if (dt > dt_rh) { l_new_dt = 1; dt_new = dt_2; }
+    else if (p.dt_fac * dt < dt_rh && dt < dtMax) { l_new_dt = 1; dt_new = dt_2; }
+    else { l_new_dt = 0; dt_new = dt; }
+    s->dt_r = dt_r;
+    s->dt_h = dt_h;
+    if (p.family == 0)
+        for (int i = p.nexp - 1; i > 0; i--) s->dt[i] = s->dt[i - 1];
+    s->dt[0] = dt_new;
+    if (dt_new < p.dtMin) s->error = 1;
+    int lMatNext = s->lMatNext;
+    scheme_weights(p, s, &lMatNext);
+    s->time += s->dt[0];
+    int lMat = lMatNext;
+    if (l_new_dt || lMat) lMat = 1;
+    s->lMat = lMat;
+    s->lMatNext = 0;
+    s->l_new_dt = l_new_dt;
+    s->n_steps += 1;
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level banded solve (dgbtrs 'N' semantics: row interchanges with L,
+// then the banded upper solve of dtbsv) on one right-hand side in LDS.
+// Lane l of the wave holds element j0 + l of a 64-wide window; each step
+// broadcasts the pivot row element with v_readlane and updates kl (or
+// kl+ku) lanes, so no LDS traffic sits on the recurrence.
+// ---------------------------------------------------------------------------
+template <class M>
+__device__ void wave_gbtrs(int n, int kl, int ku, const M *ab, int ld, const int *ipiv, cd *b)
+{
+    const int lane = threadIdx.x & 63;
+    const int kd = kl + ku;
+    int j = 0;
+    while (j < n - 1) {
+        const int j0 = j;
+        cd v = (j0 + lane < n) ? b[j0 + lane] : mk(0, 0);
+        const int jend = min(n - 1, j0 + 64 - kl);
+        for (; j < jend; j++) {
+            const int h = j - j0;
+            const int lm = min(kl, n - 1 - j);
+            const int pv = ipiv[j] - j;
+            if (pv != 0) {
+                cd a = rlc(v, h), c = rlc(v, h + pv);
+                if (lane == h) v = c;
+                else if (lane == h + pv) v = a;
+            }
+            const cd bj = rlc(v, h);
+            const int d = lane - h;
+            if (d >= 1 && d <= lm) v = csub(v, mmul(ab[kd + d + (size_t)j * ld], bj));
+        }
+        if (j0 + lane < n) b[j0 + lane] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+    const int k = kd;
+    j = n - 1;
+    while (j >= 0) {
+        const int j0 = j;
+        cd v = (j0 - lane >= 0) ? b[j0 - lane] : mk(0, 0);
+        const int jend = max(-1, j0 - (64 - k));
+        for (; j > jend; j--) {
+            const int h = j0 - j;
+            cd xj = rlc(v, h);
+            if (mnz(xj)) xj = mdiv(xj, ab[kd + (size_t)j * ld]);
+            if (lane == h) v = xj;
+            const int d = lane - h;
+            if (d >= 1 && d <= min(k, j)) v = csub(v, mmul(ab[kd - d + (size_t)j * ld], xj));
+        }
+        if (j0 - lane >= 0) b[j0 - lane] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// small dense solve (dgetrs 'N'), one thread; a column-major nt x nt LU
+template <class M>
+__device__ void small_getrs(int nt, const M *a, const int *piv, cd *b)
+{
+    for (int i = 0; i < nt; i++)
+        if (piv[i] != i) { cd t = b[i]; b[i] = b[piv[i]]; b[piv[i]] = t; }
+    for (int jj = 0; jj < nt; jj++)
+        for (int i = jj + 1; i < nt; i++) b[i] = csub(b[i], mmul(a[i + jj * nt], b[jj]));
+    for (int jj = nt - 1; jj >= 0; jj--) {
+        b[jj] = mdiv(b[jj], a[jj + jj * nt]);
+        for (int i = 0; i < jj; i++) b[i] = csub(b[i], mmul(a[i + jj * nt], b[jj]));
+    }
+}
+
+// bordered solve (algebra_lapack.f90:332-416) by one wave; rhs in LDS
+template <class M>
+__device__ void wave_bordered_solve(int N, int nt, int kl, int ku, int ld, const M *A1, const int *p1,
+                                    const M *A2, const M *A3, const M *A4, const int *p4, cd *rhs)
+{
+    const int lane = threadIdx.x & 63;
+    const int nb = N - nt;
+    cd *r2 = rhs + nt;
+    wave_gbtrs<M>(nb, kl, ku, A4, ld, p4, r2);
+    // rhs1 <- rhs1 - A2 r2
+    for (int i = 0; i < nt; i++) {
+        double sx = 0.0, sy = 0.0;
+        for (int c = lane; c < nb; c += 64) {
+            cd t = mmul(A2[i + (size_t)c * nt], r2[c]);
+            sx += t.x;
+            sy += t.y;
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        if (lane == 0) rhs[i] = mk(rhs[i].x - sx, rhs[i].y - sy);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) small_getrs<M>(nt, A1, p1, rhs);
+    __builtin_amdgcn_wave_barrier();
+    cd y[4];
+    for (int c = 0; c < nt; c++) y[c] = rhs[c];
+    for (int i = lane; i < nb; i += 64) {
+        cd v = r2[i];
+        for (int c = 0; c < nt; c++) v = csub(v, mmul(A3[i + (size_t)c * nb], y[c]));
+        r2[i] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev transforms on LDS vectors, whole workgroup (dct_fftw.f90)
+// ---------------------------------------------------------------------------
+// costf (DCT-I with cheb_fac) of nf vectors: out[f][k]
+__device__ void dct_many(const PzDev &p, const cd *const *in, cd *const *out, int nf, int dealias)
+{
+    const int N = p.N, L = p.L;
+    for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
+        int f = t / N, k = t - f * N;
+        const cd *x = in[f];
+        double sx = 0.0, sy = 0.0;
+        int q = k;
+        for (int j = 1; j < N - 1; j++, q += k) {
+            if (q >= L) q -= L;
+            double c = p.cosT[q];
+            sx += x[j].x * c;
+            sy += x[j].y * c;
+        }
+        double sg = (k & 1) ? -1.0 : 1.0;
+        double ax = x[0].x + sg * x[N - 1].x + 2.0 * sx;
+        double ay = x[0].y + sg * x[N - 1].y + 2.0 * sy;
+        cd r = mk(p.cheb_fac * ax, p.cheb_fac * ay);
+        if (dealias && k >= p.ncheb) r = mk(0, 0);
+        out[f][k] = r;
+    }
+}
+
+// first (and optionally second) radial derivative from Chebyshev
+// coefficients (get_dr_fft/get_ddr_fft with l_dct_in = .false.) followed
+// by the drx/ddrx map; nf vectors
+__device__ void dr_many(const PzDev &p, const cd *const *c, cd *const *df, cd *const *ddf, int nf)
+{
+    const int N = p.N, L = p.L;
+    const double cf = p.cheb_fac;
+    for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
+        int f = t / N, j = t - f * N;
+        const cd *w = c[f];
+        cd d1, d2 = mk(0, 0);
+        if (j == 0 || j == N - 1) {
+            double sgn0 = 1.0;
+            double ax = 0, ay = 0, bx = 0, by = 0;
+            for (int k = 1; k <= N - 1; k++) {
+                double wx = w[k].x / cf, wy = w[k].y / cf;
+                if (k == N - 1) { wx *= 0.5; wy *= 0.5; }
+                double s1 = (j == 0) ? 1.0 : (((k + 1) & 1) ? -1.0 : 1.0);
+                double k2 = (double)k * k;
+                ax += s1 * k2 * wx;
+                ay += s1 * k2 * wy;
+                if (ddf && k >= 2) {
+                    double s2 = (j == 0) ? 1.0 : ((k & 1) ? -1.0 : 1.0);
+                    double kk = k2 * (k2 - 1.0);
+                    bx += s2 * kk * wx;
+                    by += s2 * kk * wy;
+                }
+            }
+            (void)sgn0;
+            d1 = mk(ax / (double)(N - 1), ay / (double)(N - 1));
+            if (ddf) d2 = mk((1.0 / 3.0) * bx / (double)(N - 1), (1.0 / 3.0) * by / (double)(N - 1));
+        } else {
+            // V1 = sum_{k=1}^{N-2} i k w_k (e^{i pi kj/(N-1)} - e^{-i...}) = -2 sum k w_k sin
+            double vx = 0, vy = 0, ux = 0, uy = 0;
+            int q = j;
+            for (int k = 1; k < N - 1; k++, q += j) {
+                if (q >= L) q -= L;
+                double wx = w[k].x / cf, wy = w[k].y / cf;
+                double s = p.sinT[q], co = p.cosT[q];
+                vx += (double)k * wx * s;
+                vy += (double)k * wy * s;
+                if (ddf) {
+                    double k2 = (double)k * k;
+                    ux += k2 * wx * co;
+                    uy += k2 * wy * co;
+                }
+            }
+            double V1x = -2.0 * vx, V1y = -2.0 * vy;
+            double xj = p.x[j], sj = 1.0 - xj * xj;
+            double nrm = (double)(2 * N - 2);
+            d1 = mk(-V1x / sqrt(sj) / nrm, -V1y / sqrt(sj) / nrm);
+            if (ddf) {
+                double wlx = w[N - 1].x / cf, wly = w[N - 1].y / cf;
+                double sgn = (j & 1) ? -1.0 : 1.0;
+                double n1 = (double)(N - 1) * (double)(N - 1);
+                double W2x = -(2.0 * ux + n1 * wlx * sgn), W2y = -(2.0 * uy + n1 * wly * sgn);
+                double p15 = pow(sj, 1.5);
+                d2 = mk(-V1x * xj / p15 / nrm + W2x / sj / nrm, -V1y * xj / p15 / nrm + W2y / sj / nrm);
+            }
+        }
+        double a = p.drx[j];
+        if (ddf) {
+            double b = p.ddrx[j];
+            ddf[f][j] = mk(b * d1.x + a * a * d2.x, b * d1.y + a * a * d2.y);
+        }
+        df[f][j] = mk(a * d1.x, a * d1.y);
+    }
+}
+
+// y = band * x for a row-major band [n][2K+1]; rows < nzero become zero
+__device__ void bandmv(const PzDev &p, const double *band, int K, const cd *x, cd *y, int nzero)
+{
+    const int N = p.N, W = 2 * K + 1;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double sx = 0.0, sy = 0.0;
+        if (i >= nzero) {
+            int j0 = max(0, i - K), j1 = min(N - 1, i + K);
+            for (int jj = j0; jj <= j1; jj++) {
+                double a = band[(size_t)i * W + K + (jj - i)];
+                sx += a * x[jj].x;
+                sy += a * x[jj].y;
+            }
+        }
+        y[i] = mk(sx, sy);
+    }
+}
+__device__ void bandmv_c(const PzDev &p, const cd *band, int K, const cd *x, cd *y, int nzero)
+{
+    const int N = p.N, W = 2 * K + 1;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        cd s = mk(0, 0);
+        if (i >= nzero) {
+            int j0 = max(0, i - K), j1 = min(N - 1, i + K);
+            for (int jj = j0; jj <= j1; jj++) s = cadd(s, cmul(band[(size_t)i * W + K + (jj - i)], x[jj]));
+        }
+        y[i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device assembly + factorisation of the bordered LHS matrices
+// (get_lhs_mat_tau: update_temp_integ.f90:631-776,
+//  update_psi_integ_smat.f90:1286-1581; prepare_bordered_mat:
+//  algebra_lapack.f90:216-286).  One wave per (field, m).
+// ---------------------------------------------------------------------------
+template <class M>
+__device__ void wave_gbtrf(int n, int kl, int ku, M *ab, int *ipiv, int *info)
+{
+    const int lane = threadIdx.x & 63;
+    const int ld = 2 * kl + ku + 1, kv = ku + kl;
+    __shared__ int s_jp, s_ju;
+    if (lane == 0) { *info = 0; s_ju = 0; }
+    for (int jj = ku + 1; jj < min(kv, n); jj++)
+        for (int i = kv - jj + lane; i < kl; i += 64) ab[i + (size_t)jj * ld] = mzero<M>();
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        if (j + kv < n)
+            for (int i = lane; i < kl; i += 64) ab[i + (size_t)(j + kv) * ld] = mzero<M>();
+        const int km = min(kl, n - 1 - j);
+        __syncthreads();
+        if (lane == 0) {
+            int jp = 0;
+            double mx = mabs1(ab[kv + (size_t)j * ld]);
+            for (int i = 1; i <= km; i++) {
+                double a = mabs1(ab[kv + i + (size_t)j * ld]);
+                if (a > mx) { mx = a; jp = i; }
+            }
+            s_jp = jp;
+            ipiv[j] = jp + j;
+            if (mnz(ab[kv + jp + (size_t)j * ld])) {
+                int t = min(j + ku + jp, n - 1);
+                if (t > s_ju) s_ju = t;
+            } else if (*info == 0) {
+                *info = j + 1;
+            }
+        }
+        __syncthreads();
+        const int jp = s_jp, ju = s_ju;
+        if (!mnz(ab[kv + jp + (size_t)j * ld])) continue;
+        if (jp != 0)
+            for (int c = lane; c <= ju - j; c += 64) {
+                M t = ab[kv + jp - c + (size_t)(j + c) * ld];
+                ab[kv + jp - c + (size_t)(j + c) * ld] = ab[kv - c + (size_t)(j + c) * ld];
+                ab[kv - c + (size_t)(j + c) * ld] = t;
+            }
+        __syncthreads();
+        if (km > 0) {
+            M rp = mrecip(ab[kv + (size_t)j * ld]);
+            for (int i = 1 + lane; i <= km; i += 64) ab[kv + i + (size_t)j * ld] = mmm(ab[kv + i + (size_t)j * ld], rp);
+            __syncthreads();
+            const int nc = ju - j;
+            for (int t = lane; t < km * nc; t += 64) {
+                int c = 1 + t / km, i = 1 + t % km;
+                M temp = mneg(ab[kv - c + (size_t)(j + c) * ld]);
+                ab[kv + i - c + (size_t)(j + c) * ld] =
+                    madd(ab[kv + i - c + (size_t)(j + c) * ld], mmm(ab[kv + i + (size_t)j * ld], temp));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class M> __device__ M mk_elem(double re, double im);
+template <> __device__ double mk_elem<double>(double re, double) { return re; }
+template <> __device__ cd mk_elem<cd>(double re, double im) { return mk(re, im); }
+
+// Assembles and factorises one bordered matrix from raw stencils.
+//   st = S0 - c1*S1 (+ i c2*S2), rows idx = n_r + nt - 1
+// tau rows taken from tauv[nt][N]; result written to global storage.
+template <class M>
+__device__ void build_bordered(const PzDev &p, int nt, int kl, int ku, int K, const double *S0,
+                               const double *S1, const double *S2, double c1, double c2,
+                               const double *tauv, M *A1g, M *A2g, M *A3g, M *A4g, int *p1g,
+                               int *p4g, double *facg, int *infog, char *lds)
+{
+    const int N = p.N, nb = N - nt, ld = 2 * kl + ku + 1, W = 2 * K + 1;
+    const int lane = threadIdx.x & 63;
+    // LDS carve-up, every array 16-byte aligned
+    size_t o = 0;
+    auto take = [&](size_t bytes) { char *q = lds + o; o += (bytes + 15) & ~(size_t)15; return q; };
+    M *A4 = (M *)take(sizeof(M) * ld * nb);
+    M *A3 = (M *)take(sizeof(M) * nt * nb);
+    M *A2 = (M *)take(sizeof(M) * nt * nb);
+    M *A1 = (M *)take(sizeof(M) * nt * nt);
+    double *fac = (double *)take(sizeof(double) * N);
+    int *p4 = (int *)take(sizeof(int) * nb);
+    cd *tmp = (cd *)take(sizeof(cd) * nb);
+    __shared__ int s_info;
+    __shared__ int s_p1[4];
+    for (int i = lane; i < ld * nb; i += 64) A4[i] = mzero<M>();
+    for (int i = lane; i < nt * nb; i += 64) { A3[i] = mzero<M>(); A2[i] = mzero<M>(); }
+    for (int i = lane; i < nt * nt; i += 64) A1[i] = mzero<M>();
+    __syncthreads();
+    // banded rows + A3 + precond of the banded rows
+    for (int n_r = 1 + lane; n_r <= nb; n_r += 64) {
+        const int idx = n_r + nt - 1;
+        double mx = 0.0;
+        for (int nbd = 1; nbd <= W; nbd++) {
+            double re = S0[(size_t)idx * W + nbd - 1] - c1 * S1[(size_t)idx * W + nbd - 1];
+            double im = S2 ? c2 * S2[(size_t)idx * W + nbd - 1] : 0.0;
+            M v = mk_elem<M>(re, im);
+            double a = S2 ? hypot(re, im) : fabs(re);
+            if (a > mx) mx = a;
+            int c = n_r + ku + 1 - nbd;
+            if (c <= nb && c >= 1) A4[(kl + nbd - 1) + (size_t)(c - 1) * ld] = mscale(v, p.rnorm);
+            if (nbd > ku + 1 && nbd <= ku + 1 + kl) {
+                int band = nbd - ku - 1; // 1..kl
+                if (n_r <= band && n_r + nt - band >= 1)
+                    A3[(n_r - 1) + (size_t)(n_r + nt - band - 1) * nb] = mscale(v, p.rnorm);
+            }
+        }
+        fac[idx] = 1.0 / p.rnorm / mx;
+    }
+    // tau lines
+    for (int n = lane; n < N; n += 64)
+        for (int i = 0; i < nt; i++) {
+            M v = mk_elem<M>(p.rnorm * tauv[i * N + n], 0.0);
+            if (n < nt) A1[i + n * nt] = v;
+            else A2[i + (size_t)(n - nt) * nt] = v;
+        }
+    __syncthreads();
+    if (lane < nt) A1[lane] = mscale(A1[lane], p.boundary_fac);
+    __syncthreads();
+    for (int i = 0; i < nt; i++) {
+        double mx = 0.0;
+        for (int c = lane; c < nb; c += 64) {
+            M a = A2[i + (size_t)c * nt];
+            double v;
+            if constexpr (sizeof(M) == 16) v = hypot(((cd *)&a)->x, ((cd *)&a)->y);
+            else v = fabs(*(double *)&a);
+            mx = fmax(mx, v);
+        }
+        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+        if (lane == 0) fac[i] = 1.0 / mx;
+    }
+    __syncthreads();
+    for (int t = lane; t < nt * (nt + nb); t += 64) {
+        int i = t % nt, c = t / nt;
+        if (c < nt) A1[i + c * nt] = mscale(A1[i + c * nt], fac[i]);
+        else A2[i + (size_t)(c - nt) * nt] = mscale(A2[i + (size_t)(c - nt) * nt], fac[i]);
+    }
+    for (int t = lane; t < nt * nb; t += 64) {
+        int r = t % nb, c = t / nb;
+        A3[r + (size_t)c * nb] = mscale(A3[r + (size_t)c * nb], fac[r + nt]);
+    }
+    for (int t = lane; t < nb * ld; t += 64) {
+        int c = t / ld, rowb = t % ld; // storage row, column c
+        int i = rowb - (kl + ku) + c;  // matrix row of this entry
+        if (rowb >= kl && i >= 0 && i < nb) A4[rowb + (size_t)c * ld] = mscale(A4[rowb + (size_t)c * ld], fac[i + nt]);
+    }
+    __syncthreads();
+    int info = 0;
+    wave_gbtrf<M>(nb, kl, ku, A4, p4, &s_info);
+    __syncthreads();
+    info = s_info;
+    // v = A4^{-1} A3, column by column through a complex LDS scratch vector
+    if (info == 0) {
+        for (int c = 0; c < nt; c++) {
+            for (int i = lane; i < nb; i += 64) {
+                M a = A3[i + (size_t)c * nb];
+                if constexpr (sizeof(M) == 16) tmp[i] = *(cd *)&a;
+                else tmp[i] = mk(*(double *)&a, 0.0);
+            }
+            __syncthreads();
+            wave_gbtrs<M>(nb, kl, ku, A4, ld, p4, tmp);
+            __syncthreads();
+            for (int i = lane; i < nb; i += 64) {
+                if constexpr (sizeof(M) == 16) *(cd *)&A3[i + (size_t)c * nb] = tmp[i];
+                else *(double *)&A3[i + (size_t)c * nb] = tmp[i].x;
+            }
+            __syncthreads();
+        }
+        // Schur complement A1 <- A1 - A2 A3 (dgemm loop order j, l, i)
+        if (lane == 0) {
+            for (int jj = 0; jj < nt; jj++)
+                for (int l = 0; l < nb; l++) {
+                    M temp = mneg(A3[l + (size_t)jj * nb]);
+                    if (mnz(temp))
+                        for (int i = 0; i < nt; i++)
+                            A1[i + jj * nt] = madd(A1[i + jj * nt], mmm(temp, A2[i + (size_t)l * nt]));
+                }
+            // dgetrf of the nt x nt Schur complement
+            for (int jj = 0; jj < nt; jj++) {
+                int pv = jj;
+                for (int i = jj + 1; i < nt; i++)
+                    if (mabs1(A1[i + jj * nt]) > mabs1(A1[pv + jj * nt])) pv = i;
+                s_p1[jj] = pv;
+                if (mnz(A1[pv + jj * nt])) {
+                    if (pv != jj)
+                        for (int c = 0; c < nt; c++) { M t = A1[jj + c * nt]; A1[jj + c * nt] = A1[pv + c * nt]; A1[pv + c * nt] = t; }
+                    M rp = mrecip(A1[jj + jj * nt]);
+                    for (int i = jj + 1; i < nt; i++) A1[i + jj * nt] = mmm(A1[i + jj * nt], rp);
+                } else if (info == 0) {
+                    info = 1000 + jj + 1;
+                }
+                for (int c = jj + 1; c < nt; c++) {
+                    M t = mneg(A1[jj + c * nt]);
+                    for (int i = jj + 1; i < nt; i++) A1[i + c * nt] = madd(A1[i + c * nt], mmm(A1[i + jj * nt], t));
+                }
+            }
+            s_info = info;
+        }
+        __syncthreads();
+        info = s_info;
+    }
+    // write back
+    for (int i = lane; i < ld * nb; i += 64) A4g[i] = A4[i];
+    for (int i = lane; i < nt * nb; i += 64) { A3g[i] = A3[i]; A2g[i] = A2[i]; }
+    for (int i = lane; i < nt * nt; i += 64) A1g[i] = A1[i];
+    for (int i = lane; i < N; i += 64) facg[i] = fac[i];
+    for (int i = lane; i < nb; i += 64) p4g[i] = p4[i];
+    if (lane < nt) p1g[lane] = s_p1[lane];
+    if (lane == 0) *infog = info;
+}
+
+// grid: (3, nm) blocks: field 0 = T, 1 = xi, 2 = psi; 64 threads
+__global__ void __launch_bounds__(64) k_build_lhs(PzDev p, int force)
+{
+    extern __shared__ char lds[];
+    if (!force && p.ss->lMat == 0) return;
+    const int fld = blockIdx.x, mi = blockIdx.y, N = p.N, m = p.idx2m[mi];
+    const double wimp = p.ss->wimp_lin[0];
+    if (fld < 2) {
+        if (fld == 0 && !p.l_heat) return;
+        if (fld == 1 && !p.l_chem) return;
+        const int nt = PZ_T_NT, kl = PZ_T_KL, ku = PZ_T_KU, ld = PZ_T_LD, nb = N - nt, K = 4;
+        const double diff = fld == 0 ? p.TdiffFac * p.hdifT[mi] : p.XidiffFac * p.hdifXi[mi];
+        const double c1 = wimp * (fld == 0 ? p.TdiffFac : p.XidiffFac) * (fld == 0 ? p.hdifT[mi] : p.hdifXi[mi]);
+        (void)diff;
+        const int ktop = fld == 0 ? p.ktopt : p.ktopxi, kbot = fld == 0 ? p.kbott : p.kbotxi;
+        double *tauv = (double *)lds;
+        for (int n = threadIdx.x; n < N; n += 64) {
+            tauv[n] = ktop == 1 ? p.tau[n] : p.tau[2 * N + n];
+            tauv[N + n] = kbot == 1 ? p.tau[N + n] : p.tau[3 * N + n];
+        }
+        __syncthreads();
+        const double *S0 = p.T_S0 + (size_t)mi * N * (2 * K + 1);
+        const double *S1 = (fld == 0 ? p.T_S1 : p.X_S1) + (size_t)mi * N * (2 * K + 1);
+        double *A1 = (fld == 0 ? p.T_A1 : p.X_A1) + (size_t)mi * nt * nt;
+        double *A2 = (fld == 0 ? p.T_A2 : p.X_A2) + (size_t)mi * nt * nb;
+        double *A3 = (fld == 0 ? p.T_A3 : p.X_A3) + (size_t)mi * nt * nb;
+        double *A4 = (fld == 0 ? p.T_A4 : p.X_A4) + (size_t)mi * ld * nb;
+        int *p1 = (fld == 0 ? p.T_p1 : p.X_p1) + (size_t)mi * nt;
+        int *p4 = (fld == 0 ? p.T_p4 : p.X_p4) + (size_t)mi * nb;
+        double *fac = (fld == 0 ? p.T_fac : p.X_fac) + (size_t)mi * N;
+        build_bordered<double>(p, nt, kl, ku, K, S0, S1, nullptr, c1, 0.0, tauv, A1, A2, A3, A4, p1, p4, fac,
+                               p.info + fld * p.nm + mi, lds + 4 * N * sizeof(double));
+    } else {
+        const int nt = m == 0 ? 2 : PZ_P_NT, kl = m == 0 ? 4 : PZ_P_KL, ku = kl;
+        const int nb = N - nt, K = PZ_P_KL;
+        const double nu = p.ViscFac * p.hdifV[mi];
+        const double c1 = wimp * p.ViscFac * p.hdifV[mi];
+        const double c2 = (m == 0 || !p.l_coriolis_imp) ? 0.0 : wimp * p.CorFac * (double)m;
+        (void)nu;
+        double *tauv = (double *)lds;
+        const double rc_or1 = p.or1[0], rb_or1 = p.or1[N - 1];
+        for (int n = threadIdx.x; n < N; n += 64) {
+            double rT = p.tau[n], rB = p.tau[N + n], dT = p.tau[2 * N + n], dB = p.tau[3 * N + n];
+            double d2T = p.tau[4 * N + n], d2B = p.tau[5 * N + n];
+            double dn2 = (double)n * (double)n;
+            double d3top = 8.0 / 15.0 * dn2 * (dn2 - 1.0) * (dn2 - 4.0);
+            if (m == 0) {
+                tauv[n] = p.ktopv == 1 ? (dT - rc_or1 * rT) : rT;
+                tauv[N + n] = p.kbotv == 1 ? (dB - rb_or1 * rB) : rB;
+            } else {
+                tauv[n] = rT;
+                tauv[N + n] = rB;
+                tauv[2 * N + n] = p.ktopv == 1 ? (d2T - rc_or1 * dT) : d3top;
+                tauv[3 * N + n] = p.kbotv == 1 ? (d2B - rb_or1 * dB) : dB;
+            }
+        }
+        __syncthreads();
+        const size_t W = 2 * K + 1;
+        const int ldm = 2 * kl + ku + 1;
+        // m = 0 stencils are stored 9 wide (K = 4) at the start of the m block
+        const double *S0 = p.P_S0 + (size_t)mi * N * W;
+        const double *S1 = p.P_S1 + (size_t)mi * N * W;
+        const double *S2 = p.P_S2 + (size_t)mi * N * W;
+        cd *A1 = p.P_A1 + (size_t)mi * 16;
+        cd *A2 = p.P_A2 + (size_t)mi * PZ_P_NT * N;
+        cd *A3 = p.P_A3 + (size_t)mi * PZ_P_NT * N;
+        cd *A4 = p.P_A4 + (size_t)mi * PZ_P_LD * N;
+        int *p1 = p.P_p1 + (size_t)mi * 4;
+        int *p4 = p.P_p4 + (size_t)mi * N;
+        double *fac = p.P_fac + (size_t)mi * N;
+        (void)ldm;
+        char *scr = lds + 4 * N * sizeof(double);
+        if (m == 0)
+            build_bordered<cd>(p, nt, kl, ku, 4, S0, S1, nullptr, c1, 0.0, tauv, A1, A2, A3, A4, p1, p4, fac,
+                               p.info + 2 * p.nm + mi, scr);
+        else
+            build_bordered<cd>(p, nt, kl, ku, K, S0, S1, S2, c1, c2, tauv, A1, A2, A3, A4, p1, p4, fac,
+                               p.info + 2 * p.nm + mi, scr);
+    }
+}

@@ -1,0 +1,96 @@
+// Internal device-side data structures of libpizza_amd (not part of the ABI).
+//
+// Layout of every spectral field on the device: complex FP64 stored as
+// double2 (x = re, y = im), "M-layout" f[m][r]: one contiguous radial column
+// of n_r values per azimuthal wavenumber.  Time-scheme slots of a field are
+// consecutive [slot][m][r] blocks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+typedef double2 cd;
+
+#define PZ_MAX_RADICES 24
+#define PZ_MAX_STAGES 6
+
+// Time-scheme state that changes during the run.  It lives in device memory
+// and is updated by k_courant, so a captured step replays without the host.
+struct PzScheme {
+    double dt[8];
+    double wimp[8];
+    double wimp_lin[8];
+    double wexp[8];
+    double bimp[PZ_MAX_STAGES][PZ_MAX_STAGES]; // dt-scaled Butcher tables
+    double bexp[PZ_MAX_STAGES][PZ_MAX_STAGES];
+    double time;
+    double dt_r, dt_h; // last Courant limits
+    int lMat;          // matrices must be rebuilt in this step
+    int lMatNext;
+    int l_new_dt;
+    int error;         // 1: dt fell below dtMin
+    int n_steps;
+    int pad[3];
+};
+
+// Everything a kernel needs: sizes, constants and device pointers.
+struct PzDev {
+    int N, nm, nphi, ncheb, L; // L = 2N-2 (Chebyshev even extension)
+    int family, nstages, nexp, nimp, nold;
+    int l_heat, l_chem, l_coriolis_imp, l_buo_imp, l_ek_pump;
+    int ktopt, kbott, ktopxi, kbotxi, ktopv, kbotv;
+    int nrad, radices[PZ_MAX_RADICES]; // azimuthal FFT plan
+    int l_exp_calc[8], l_imp_calc_rhs[8];
+    double rnorm, cheb_fac, boundary_fac, thr;
+    double CorFac, BuoFac, TdiffFac, ChemFac, XidiffFac, ViscFac, tadvz, r_cmb, ek;
+    double alpha, dtMax, dtMin, dt_fac, courfac, wimp_lin0;
+    double bimp0[PZ_MAX_STAGES][PZ_MAX_STAGES], bexp0[PZ_MAX_STAGES][PZ_MAX_STAGES];
+
+    // radial functions (n_r) and transform tables
+    const double *r, *or1, *or2, *rgrav, *beta, *ekpump, *ekp_up, *ekp_us, *ekp_dusdp;
+    const double *tcond, *dtcond, *xicond, *dxicond, *delxr2, *delxh2, *drx, *ddrx, *x;
+    const double *cosT, *sinT; // cos/sin(pi q/(N-1)), q in [0, L)
+    const double *tau;         // tau lines: [rMat(2,N) | drMat(2,N) | d2rMat(2,N)]
+    const cd *twid;            // exp(-2 pi i k / n_phi)
+    const int *idx2m;
+    const double *hdifT, *hdifXi, *hdifV;
+    const cd *topt, *bott, *topxi, *botxi;
+
+    // fields [m][r]
+    cd *us, *up, *om, *psi, *temp, *xi, *temp_hat, *xi_hat, *psi_hat, *dtemp;
+    cd *dVsT, *dVsXi, *dVsOm;
+    cd *T_old, *T_impl, *T_expl, *X_old, *X_impl, *X_expl, *P_old, *P_impl, *P_expl;
+    double *dtr, *dth; // per-radius Courant limits
+
+    // m-independent RHS bands, row-major [n_r][2K+1] in column-offset order
+    // (entry K+d is column i+d), already multiplied by rnorm
+    const double *T_RHSE1, *T_RHSE2;           // K = 3, 4
+    const double *P_RHSE1, *P_RHSE2, *P_RHSE3; // K = 4, 8, 7
+    // per-m RHS bands
+    const double *T_RHSI; // [nm][n_r][5]  K = 2
+    const double *P_RHSI; // [nm][n_r][17] K = 8 (m = 0 uses the central 9)
+    const cd *P_RHSIL;    // [nm][n_r][17]
+    // per-m raw LHS stencils in the reference stencil order (index nb <->
+    // column idx+K-nb), rows idx in [0, n_r)
+    const double *T_S0, *T_S1, *X_S1;   // [nm][n_r][9]
+    const double *P_S0, *P_S1, *P_S2;   // [nm][n_r][17]
+    // factorised bordered LHS (written by k_build_lhs)
+    double *T_A1, *T_A2, *T_A3, *T_A4, *T_fac;
+    int *T_p1, *T_p4;
+    double *X_A1, *X_A2, *X_A3, *X_A4, *X_fac;
+    int *X_p1, *X_p4;
+    cd *P_A1, *P_A2, *P_A3, *P_A4;
+    double *P_fac;
+    int *P_p1, *P_p4;
+    int *info; // [3 * nm] factorisation info per (field, m)
+
+    PzScheme *ss;
+};
+
+// sizes of the bordered storage per m
+#define PZ_T_KL 4
+#define PZ_T_KU 4
+#define PZ_T_NT 2
+#define PZ_T_LD (2 * PZ_T_KL + PZ_T_KU + 1) // 13
+#define PZ_P_KL 8
+#define PZ_P_KU 8
+#define PZ_P_NT 4
+#define PZ_P_LD (2 * PZ_P_KL + PZ_P_KU + 1) // 25

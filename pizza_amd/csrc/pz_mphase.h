@@ -1,0 +1,525 @@
+// Device code of libpizza_amd, part 2: the per-m stage kernel (explicit
+// assembly + implicit solves of the integration method), the start-up
+// kernel, diagnostics and unit-operator kernels.  Included once by pz_api.hip.
+#pragma once
+#include "pz_device.h"
+
+// set_imex_rhs for one m row (multistep_schemes.f90:366-413;
+// dirk_schemes.f90:773-814).  cur: LDS copy of the current explicit slot or
+// nullptr to read it from global memory.
+__device__ __forceinline__ cd imex_rhs(const PzDev &p, const PzScheme *s, int istage, const cd *old,
+                                       const cd *impl, const cd *expl, const cd *cur, size_t off, int k)
+{
+    const size_t F = (size_t)p.nm * p.N;
+    const int is = istage - 1;
+    cd r;
+    if (p.family == 0) {
+        r = cscale(s->wimp[0], old[off + k]);
+        for (int o = 1; o < p.nold; o++) r = cadd(r, cscale(s->wimp[o], old[o * F + off + k]));
+        for (int o = 0; o < p.nimp; o++) r = cadd(r, cscale(s->wimp_lin[o + 1], impl[o * F + off + k]));
+        for (int o = 0; o < p.nexp; o++) {
+            cd e = (o == is && cur) ? cur[k] : expl[o * F + off + k];
+            r = cadd(r, cscale(s->wexp[o], e));
+        }
+    } else {
+        r = old[off + k];
+        for (int st = 0; st <= is; st++) {
+            cd e = (st == is && cur) ? cur[k] : expl[st * F + off + k];
+            r = cadd(r, cscale(s->bexp[istage][st], e));
+        }
+        for (int st = 0; st <= is; st++) r = cadd(r, cscale(s->bimp[istage][st], impl[st * F + off + k]));
+    }
+    return r;
+}
+
+// rotate_imex for one m row (multistep_schemes.f90:462-498); DIRK: no-op
+__device__ void rotate_row(const PzDev &p, cd *old, cd *impl, cd *expl, size_t off)
+{
+    if (p.family != 0) return;
+    const size_t F = (size_t)p.nm * p.N;
+    for (int k = threadIdx.x; k < p.N; k += blockDim.x) {
+        for (int o = p.nexp - 1; o >= 1; o--) expl[o * F + off + k] = expl[(o - 1) * F + off + k];
+        for (int o = p.nold - 1; o >= 1; o--) old[o * F + off + k] = old[(o - 1) * F + off + k];
+        for (int o = p.nimp - 1; o >= 1; o--) impl[o * F + off + k] = impl[(o - 1) * F + off + k];
+    }
+}
+
+// -BuoFac rgrav/r i m T - ChemFac rgrav/r i m X (update_psi_integ_smat.f90:286-304)
+__device__ __forceinline__ cd buoyancy(const PzDev &p, int m, int k, cd T, cd X)
+{
+    cd b = mk(0, 0);
+    if (m == 0) return b;
+    if (p.l_heat) {
+        double s = -p.BuoFac * p.rgrav[k] * p.or1[k];
+        b = cmul(mk(0.0, s * (double)m), T);
+        if (p.l_chem) b = csub(b, cmul(mk(0.0, p.ChemFac * p.rgrav[k] * p.or1[k] * (double)m), X));
+    } else if (p.l_chem) {
+        double s = -p.ChemFac * p.rgrav[k] * p.or1[k];
+        b = cmul(mk(0.0, s * (double)m), X);
+    }
+    return b;
+}
+
+// get_dcheb_real_1d with the rerror threshold (radial_derivatives.f90:143-185),
+// one thread: Chebyshev coefficients of d/dx from those of f
+__device__ void dcheb_real(const PzDev &p, const double *f, double *df)
+{
+    const int N = p.N;
+    double mx = 0.0;
+    for (int j = 0; j < N; j++) mx = fmax(mx, fabs(f[j]));
+    const double thr = mx * p.thr;
+    df[N - 1] = 0.0;
+    const int nc = N - 1;
+    df[nc - 1] = fabs(f[nc]) >= thr ? (double)nc * f[nc] : 0.0;
+    for (int n = N - 2; n >= 1; n--) df[n - 1] = fabs(f[n]) >= thr ? df[n + 1] + (double)(2 * n) * f[n] : 0.0;
+}
+
+#define MPHASE_NVEC 18
+
+// One stage of the m-loop for one azimuthal wavenumber per workgroup:
+// finish_explicit_assembly (mloop.f90:169-229) when exp_calc, then
+// update_temp_int / update_xi_int (update_temp_integ.f90:179-298) and
+// update_psi_int_smat (update_psi_integ_smat.f90:247-573).
+__global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_calc)
+{
+    extern __shared__ double smem[];
+    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi];
+    const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
+    const int is = istage - 1;
+    const PzScheme *ss = p.ss;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    cd *V = (cd *)smem;
+    cd *aT = V, *cT = V + N, *eT = V + 2 * N;
+    cd *aX = V + 3 * N, *cX = V + 4 * N, *eX = V + 5 * N;
+    cd *aP = V + 6 * N, *cP = V + 7 * N, *eP = V + 8 * N;
+    cd *psiv = V + 9 * N, *omv = V + 10 * N, *upv = V + 11 * N, *usv = V + 12 * N;
+    cd *Tv = V + 13 * N, *Xv = V + 14 * N, *bv = V + 15 * N, *b2 = V + 16 * N, *wv = V + 17 * N;
+    double *dtmp = (double *)(V + MPHASE_NVEC * N);
+    const bool hT = p.l_heat, hX = p.l_chem;
+
+    // ---------------- explicit assembly -----------------------------------
+    if (exp_calc) {
+        for (int k = tid; k < N; k += blockDim.x) {
+            if (hT) { aT[k] = p.dVsT[off + k]; eT[k] = p.T_expl[is * F + off + k]; }
+            if (hX) { aX[k] = p.dVsXi[off + k]; eX[k] = p.X_expl[is * F + off + k]; }
+            aP[k] = p.dVsOm[off + k];
+            eP[k] = p.P_expl[is * F + off + k];
+            psiv[k] = p.psi[off + k];
+            omv[k] = p.om[off + k];
+            upv[k] = p.up[off + k];
+            usv[k] = p.us[off + k];
+            if (!p.l_buo_imp) { Tv[k] = hT ? p.temp[off + k] : mk(0, 0); Xv[k] = hX ? p.xi[off + k] : mk(0, 0); }
+        }
+        __syncthreads();
+        const cd *din[3];
+        cd *dout[3];
+        int nf = 0;
+        if (hT) { din[nf] = aT; dout[nf++] = cT; }
+        if (hX) { din[nf] = aX; dout[nf++] = cX; }
+        din[nf] = aP; dout[nf++] = cP;
+        dct_many(p, din, dout, nf, 1);
+        __syncthreads();
+        // derivative of the advective flux (into the a-vectors)
+        {
+            const cd *c3[3];
+            cd *d3[3];
+            int k3 = 0;
+            if (hT) { c3[k3] = cT; d3[k3++] = aT; }
+            if (hX) { c3[k3] = cX; d3[k3++] = aX; }
+            c3[k3] = cP; d3[k3++] = aP;
+            dr_many(p, c3, d3, nullptr, k3);
+        }
+        __syncthreads();
+        const double rc2 = p.r_cmb * p.r_cmb;
+        for (int k = tid; k < N; k += blockDim.x) {
+            const double rk = p.r[k], h2 = rc2 - rk * rk;
+            if (hT) {
+                // finish_exp_temp_int (update_temp_integ.f90:420-438)
+                double s = h2 * p.dtcond[k] + p.tadvz * rk * p.tcond[k];
+                cd t = cmul(mk(0.0, (double)m * s), psiv[k]);
+                cd e = csub(cscale(rk, eT[k]), t);
+                eT[k] = csub(e, aT[k]);
+            }
+            if (hX) {
+                cd t = cmul(mk(0.0, (double)m * h2 * p.dxicond[k]), psiv[k]);
+                cd e = csub(cscale(rk, eX[k]), t);
+                eX[k] = csub(e, aX[k]);
+            }
+            // finish_exp_psi_int_smat (update_psi_integ_smat.f90:832-930)
+            cd e = eP[k];
+            if (m != 0) {
+                e = cscale(rk, e);
+                if (!p.l_coriolis_imp) e = csub(e, cmul(mk(0.0, p.CorFac * (double)m * rk), psiv[k]));
+                if (!p.l_buo_imp) {
+                    if (hT) e = csub(e, cmul(mk(0.0, p.BuoFac * p.rgrav[k] * (double)m), Tv[k]));
+                    if (hX) e = csub(e, cmul(mk(0.0, p.ChemFac * p.rgrav[k] * (double)m), Xv[k]));
+                }
+            }
+            if (p.l_ek_pump) {
+                double ekp = p.CorFac * p.ekpump[k];
+                if (m == 0) {
+                    e = csub(e, cscale(ekp, upv[k]));
+                } else {
+                    cd t = cadd(omv[k], cscale(p.ekp_up[k], upv[k]));
+                    t = cadd(t, cmul(mk(p.ekp_us[k], (double)m * p.ekp_dusdp[k]), usv[k]));
+                    e = csub(e, cscale(ekp * rk, t));
+                }
+            }
+            if (m != 0) e = csub(e, aP[k]);
+            eP[k] = e;
+        }
+        __syncthreads();
+        nf = 0;
+        if (hT) { din[nf] = eT; dout[nf++] = cT; }
+        if (hX) { din[nf] = eX; dout[nf++] = cX; }
+        din[nf] = eP; dout[nf++] = cP;
+        dct_many(p, din, dout, nf, 1);
+        __syncthreads();
+        if (hT) bandmv(p, p.T_RHSE1, 3, cT, eT, 2);
+        if (hX) bandmv(p, p.T_RHSE1, 3, cX, eX, 2);
+        if (m == 0) bandmv(p, p.P_RHSE1, 4, cP, eP, 2);
+        else bandmv(p, p.P_RHSE3, 7, cP, eP, 4);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) {
+            if (hT) p.T_expl[is * F + off + k] = eT[k];
+            if (hX) p.X_expl[is * F + off + k] = eX[k];
+            p.P_expl[is * F + off + k] = eP[k];
+        }
+    }
+
+    // ---------------- T and xi: rhs, bordered solve ------------------------
+    const int nb2 = N - PZ_T_NT;
+    for (int k = tid; k < N; k += blockDim.x) {
+        if (hT) {
+            cd r = imex_rhs(p, ss, istage, p.T_old, p.T_impl, p.T_expl, exp_calc ? eT : nullptr, off, k);
+            if (k == 0) r = p.topt[mi];
+            if (k == 1) r = p.bott[mi];
+            aT[k] = cscale(p.T_fac[off + k], r);
+        }
+        if (hX) {
+            cd r = imex_rhs(p, ss, istage, p.X_old, p.X_impl, p.X_expl, exp_calc ? eX : nullptr, off, k);
+            if (k == 0) r = p.topxi[mi];
+            if (k == 1) r = p.botxi[mi];
+            aX[k] = cscale(p.X_fac[off + k], r);
+        }
+    }
+    __syncthreads();
+    if (wave == 0 && hT)
+        wave_bordered_solve<double>(N, PZ_T_NT, PZ_T_KL, PZ_T_KU, PZ_T_LD, p.T_A1 + mi * 4, p.T_p1 + mi * 2,
+                                    p.T_A2 + (size_t)mi * 2 * nb2, p.T_A3 + (size_t)mi * 2 * nb2,
+                                    p.T_A4 + (size_t)mi * PZ_T_LD * nb2, p.T_p4 + (size_t)mi * nb2, aT);
+    if (wave == 1 && hX)
+        wave_bordered_solve<double>(N, PZ_T_NT, PZ_T_KL, PZ_T_KU, PZ_T_LD, p.X_A1 + mi * 4, p.X_p1 + mi * 2,
+                                    p.X_A2 + (size_t)mi * 2 * nb2, p.X_A3 + (size_t)mi * 2 * nb2,
+                                    p.X_A4 + (size_t)mi * PZ_T_LD * nb2, p.X_p4 + (size_t)mi * nb2, aX);
+    __syncthreads();
+    {
+        const cd *din[2];
+        cd *dout[2];
+        int nf = 0;
+        if (hT) { din[nf] = aT; dout[nf++] = Tv; }
+        if (hX) { din[nf] = aX; dout[nf++] = Xv; }
+        if (nf) dct_many(p, din, dout, nf, 0);
+    }
+    __syncthreads();
+    const int inext = (istage == p.nstages) ? 1 : istage + 1;
+    const int lcalc = p.l_imp_calc_rhs[inext - 1];
+    if (hT) rotate_row(p, p.T_old, p.T_impl, p.T_expl, off);
+    if (hX) rotate_row(p, p.X_old, p.X_impl, p.X_expl, off);
+    __syncthreads();
+    for (int k = tid; k < N; k += blockDim.x) {
+        if (hT) { p.temp_hat[off + k] = aT[k]; p.temp[off + k] = Tv[k]; }
+        if (hX) { p.xi_hat[off + k] = aX[k]; p.xi[off + k] = Xv[k]; }
+    }
+    // get_temp_rhs_imp_int (update_temp_integ.f90:471-536)
+    if (inext == 1) {
+        if (hT) bandmv(p, p.T_RHSE2, 4, aT, p.T_old + off, 2);
+        if (hX) bandmv(p, p.T_RHSE2, 4, aX, p.X_old + off, 2);
+    }
+    if (lcalc) {
+        if (hT) bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, aT, cT, 2);
+        if (hX) bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, aX, cX, 2);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) {
+            if (hT) p.T_impl[(inext - 1) * F + off + k] = cscale(p.TdiffFac * p.hdifT[mi], cT[k]);
+            if (hX) p.X_impl[(inext - 1) * F + off + k] = cscale(p.XidiffFac * p.hdifXi[mi], cX[k]);
+        }
+    }
+    __syncthreads();
+
+    // ---------------- psi -------------------------------------------------
+    if (p.l_buo_imp) {
+        for (int k = tid; k < N; k += blockDim.x)
+            bv[k] = buoyancy(p, m, k, hT ? Tv[k] : mk(0, 0), hX ? Xv[k] : mk(0, 0));
+        __syncthreads();
+        const cd *din[1] = {bv};
+        cd *dout[1] = {wv};
+        dct_many(p, din, dout, 1, 0);
+        __syncthreads();
+        if (m > 0) bandmv(p, p.P_RHSE2, 8, wv, b2, 4);
+        __syncthreads();
+    }
+    const double wl = ss->wimp_lin[0];
+    for (int k = tid; k < N; k += blockDim.x) {
+        cd r = imex_rhs(p, ss, istage, p.P_old, p.P_impl, p.P_expl, exp_calc ? eP : nullptr, off, k);
+        if (p.l_buo_imp && m != 0) r = cadd(r, cscale(wl, b2[k]));
+        aP[k] = cscale(p.P_fac[off + k], r);
+    }
+    __syncthreads();
+    {
+        const int nt = m == 0 ? 2 : PZ_P_NT, kl = m == 0 ? 4 : PZ_P_KL, ld = 2 * kl + kl + 1;
+        if (wave == 0)
+            wave_bordered_solve<cd>(N, nt, kl, kl, ld, p.P_A1 + mi * 16, p.P_p1 + mi * 4, p.P_A2 + (size_t)mi * PZ_P_NT * N,
+                                    p.P_A3 + (size_t)mi * PZ_P_NT * N, p.P_A4 + (size_t)mi * PZ_P_LD * N,
+                                    p.P_p4 + (size_t)mi * N, aP);
+    }
+    __syncthreads();
+    const double rc2 = p.r_cmb * p.r_cmb;
+    if (m == 0) {
+        double *f = dtmp, *df = dtmp + N;
+        for (int k = tid; k < N; k += blockDim.x) f[k] = aP[k].x;
+        __syncthreads();
+        if (tid == 0) dcheb_real(p, f, df);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) { cP[k] = mk(df[k], 0.0); eP[k] = mk(f[k], 0.0); }
+        __syncthreads();
+        const cd *din[2] = {cP, eP};
+        cd *dout[2] = {omv, upv};
+        dct_many(p, din, dout, 2, 0);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) {
+            double u0 = upv[k].x, om0 = p.drx[k] * omv[k].x;
+            p.us[off + k] = mk(0, 0);
+            p.up[off + k] = mk(u0, 0.0);
+            p.om[off + k] = mk(om0 + p.or1[k] * u0, 0.0);
+            eP[k] = mk(u0, 0.0);
+        }
+        __syncthreads();
+        // work_r = costf(real(up(m=0))) for get_psi_rhs_imp_int_smat
+        const cd *di[1] = {eP};
+        cd *dd[1] = {wv};
+        dct_many(p, di, dd, 1, 0);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) wv[k] = mk(wv[k].x, 0.0);
+    } else {
+        const cd *c1[1] = {aP};
+        cd *d1[1] = {cP}, *d2[1] = {eP};
+        dr_many(p, c1, d1, d2, 1);
+        const cd *din[1] = {aP};
+        cd *dout[1] = {psiv};
+        dct_many(p, din, dout, 1, 0);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) {
+            const double rk = p.r[k], h2 = rc2 - rk * rk, or1 = p.or1[k];
+            cd ps = psiv[k], dps = cP[k], d2ps = eP[k];
+            p.psi_hat[off + k] = aP[k];
+            p.psi[off + k] = ps;
+            p.us[off + k] = cmul(mk(0.0, (double)m * or1 * h2), ps);
+            p.up[off + k] = cadd(cscale(-h2, dps), cscale(3.0 * rk, ps));
+            cd o = csub(cscale(-h2, d2ps), cscale(rc2 * or1 - 6.0 * rk, dps));
+            p.om[off + k] = cadd(o, cscale(6.0 + (double)m * (double)m * p.or2[k] * h2, ps));
+            wv[k] = aP[k];
+        }
+    }
+    __syncthreads();
+    rotate_row(p, p.P_old, p.P_impl, p.P_expl, off);
+    __syncthreads();
+    // get_psi_rhs_imp_int_smat (update_psi_integ_smat.f90:960-1115)
+    if (inext == 1) {
+        if (m == 0) bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 4, wv, p.P_old + off, 2);
+        else bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 8, wv, p.P_old + off, 4);
+    }
+    if (lcalc) {
+        if (m == 0) bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 2, wv, cT, 2);
+        else bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 8, wv, cT, 4);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) {
+            cd v = cT[k];
+            if (p.l_buo_imp && m != 0) v = cadd(v, b2[k]);
+            p.P_impl[(inext - 1) * F + off + k] = v;
+        }
+    }
+}
+
+// get_start_fields, fresh start of the integration method
+// (init_fields.f90:186-255): dT/dr, omega, hats and the first implicit state.
+__global__ void __launch_bounds__(256) k_start(PzDev p)
+{
+    extern __shared__ double smem[];
+    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi];
+    const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
+    (void)F;
+    const int tid = threadIdx.x;
+    cd *V = (cd *)smem;
+    cd *Tv = V, *Xv = V + N, *upv = V + 2 * N, *usv = V + 3 * N, *psv = V + 4 * N;
+    cd *cT = V + 5 * N, *cX = V + 6 * N, *cU = V + 7 * N, *cPs = V + 8 * N;
+    cd *dT = V + 9 * N, *dU = V + 10 * N, *bv = V + 11 * N, *wv = V + 12 * N, *b2 = V + 13 * N, *t1 = V + 14 * N;
+    const bool hT = p.l_heat, hX = p.l_chem;
+    for (int k = tid; k < N; k += blockDim.x) {
+        Tv[k] = hT ? p.temp[off + k] : mk(0, 0);
+        Xv[k] = hX ? p.xi[off + k] : mk(0, 0);
+        upv[k] = p.up[off + k];
+        usv[k] = p.us[off + k];
+        psv[k] = p.psi[off + k];
+    }
+    __syncthreads();
+    {
+        const cd *din[5] = {Tv, Xv, upv, psv, nullptr};
+        cd *dout[5] = {cT, cX, cU, cPs, nullptr};
+        dct_many(p, din, dout, 4, 0);
+    }
+    __syncthreads();
+    {
+        const cd *c2[2] = {cT, cU};
+        cd *d2[2] = {dT, dU};
+        dr_many(p, c2, d2, nullptr, 2);
+    }
+    __syncthreads();
+    for (int k = tid; k < N; k += blockDim.x) {
+        if (hT) { p.dtemp[off + k] = dT[k]; p.temp_hat[off + k] = cT[k]; }
+        if (hX) p.xi_hat[off + k] = cX[k];
+        p.psi_hat[off + k] = cPs[k];
+        cd o = cadd(dU[k], cscale(p.or1[k], upv[k]));
+        p.om[off + k] = csub(o, cmul(mk(0.0, p.or1[k] * (double)m), usv[k]));
+    }
+    // scalar implicit states (istage = 1, l_calc_lin = .true.)
+    if (hT) {
+        bandmv(p, p.T_RHSE2, 4, cT, p.T_old + off, 2);
+        bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, cT, t1, 2);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) p.T_impl[off + k] = cscale(p.TdiffFac * p.hdifT[mi], t1[k]);
+        __syncthreads();
+    }
+    if (hX) {
+        bandmv(p, p.T_RHSE2, 4, cX, p.X_old + off, 2);
+        bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, cX, t1, 2);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) p.X_impl[off + k] = cscale(p.XidiffFac * p.hdifXi[mi], t1[k]);
+        __syncthreads();
+    }
+    // psi: the reference passes psi in the composition slot (init_fields.f90:250-251)
+    if (m == 0) {
+        // work_r = costf(real(up(m=0)))
+        for (int k = tid; k < N; k += blockDim.x) bv[k] = mk(upv[k].x, 0.0);
+        __syncthreads();
+        const cd *di[1] = {bv};
+        cd *dd[1] = {wv};
+        dct_many(p, di, dd, 1, 0);
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) wv[k] = mk(wv[k].x, 0.0);
+        __syncthreads();
+        bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 4, wv, p.P_old + off, 2);
+        bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 2, wv, p.P_impl + off, 2);
+    } else {
+        bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 8, cPs, p.P_old + off, 4);
+        bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 8, cPs, t1, 4);
+        if (p.l_buo_imp) {
+            for (int k = tid; k < N; k += blockDim.x) bv[k] = buoyancy(p, m, k, Tv[k], psv[k]);
+            __syncthreads();
+            const cd *di[1] = {bv};
+            cd *dd[1] = {wv};
+            dct_many(p, di, dd, 1, 0);
+            __syncthreads();
+            bandmv(p, p.P_RHSE2, 8, wv, b2, 4);
+        }
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) {
+            cd v = t1[k];
+            if (p.l_buo_imp) v = cadd(v, b2[k]);
+            p.P_impl[off + k] = v;
+        }
+    }
+}
+
+// radial profiles for the kinetic energies (outputs.f90:458-467): per r,
+// sum over m of cc2real(us), cc2real(up) and the m = 0 part of up
+__global__ void k_ekin_profiles(PzDev p, double *out)
+{
+    const int nr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nr >= p.N) return;
+    double us2 = 0.0, up2 = 0.0, upa = 0.0;
+    for (int i = 0; i < p.nm; i++) {
+        const int m = p.idx2m[i];
+        cd u = p.us[(size_t)i * p.N + nr], v = p.up[(size_t)i * p.N + nr];
+        double a = m == 0 ? u.x * u.x : 2.0 * (u.x * u.x + u.y * u.y);
+        double b = m == 0 ? v.x * v.x : 2.0 * (v.x * v.x + v.y * v.y);
+        us2 += a;
+        up2 += b;
+        if (m == 0) upa += b;
+    }
+    out[nr] = us2;
+    out[p.N + nr] = up2;
+    out[2 * p.N + nr] = upa;
+}
+
+// ---- unit-operator kernels on [vec][n_r] device arrays ----
+__global__ void __launch_bounds__(256) k_costf_vecs(PzDev p, cd *f)
+{
+    extern __shared__ double smem[];
+    cd *a = (cd *)smem, *b = a + p.N;
+    cd *x = f + (size_t)blockIdx.x * p.N;
+    for (int k = threadIdx.x; k < p.N; k += blockDim.x) a[k] = x[k];
+    __syncthreads();
+    const cd *din[1] = {a};
+    cd *dout[1] = {b};
+    dct_many(p, din, dout, 1, 0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < p.N; k += blockDim.x) x[k] = b[k];
+}
+
+__global__ void __launch_bounds__(256) k_dr_vecs(PzDev p, const cd *f, cd *df, cd *ddf, int l_dct_in)
+{
+    extern __shared__ double smem[];
+    cd *a = (cd *)smem, *c = a + p.N, *d1 = c + p.N, *d2 = d1 + p.N;
+    const size_t off = (size_t)blockIdx.x * p.N;
+    for (int k = threadIdx.x; k < p.N; k += blockDim.x) a[k] = f[off + k];
+    __syncthreads();
+    if (l_dct_in) {
+        const cd *din[1] = {a};
+        cd *dout[1] = {c};
+        dct_many(p, din, dout, 1, 0);
+    } else {
+        for (int k = threadIdx.x; k < p.N; k += blockDim.x) c[k] = a[k];
+    }
+    __syncthreads();
+    const cd *cc[1] = {c};
+    cd *o1[1] = {d1}, *o2[1] = {d2};
+    dr_many(p, cc, o1, ddf ? o2 : nullptr, 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < p.N; k += blockDim.x) {
+        df[off + k] = d1[k];
+        if (ddf) ddf[off + k] = d2[k];
+    }
+}
+
+// azimuthal transforms, one radius per block: grid real [rad][n_phi],
+// spectral [rad][nm]
+__global__ void __launch_bounds__(256) k_phi_fft(PzDev p, const double *grid, cd *spec)
+{
+    extern __shared__ double smem[];
+    const int n = p.nphi, nm = p.nm;
+    cd *a = (cd *)smem, *b = a + n;
+    const double *g = grid + (size_t)blockIdx.x * n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) a[j] = mk(g[j], 0.0);
+    __syncthreads();
+    cd *Z = fft_lds(p, a, b, n, 1, -1);
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) spec[(size_t)blockIdx.x * nm + i] = cdivr(Z[i], (double)n);
+}
+
+__global__ void __launch_bounds__(256) k_phi_ifft(PzDev p, const cd *spec, double *grid)
+{
+    extern __shared__ double smem[];
+    const int n = p.nphi, nm = p.nm;
+    cd *a = (cd *)smem, *b = a + n;
+    const cd *s = spec + (size_t)blockIdx.x * nm;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        cd z = mk(0, 0);
+        if (k == 0) z = mk(s[0].x, 0.0);
+        else if (k < nm) z = s[k];
+        else if (n - k < nm) z = cconj(s[n - k]);
+        a[k] = z;
+    }
+    __syncthreads();
+    cd *Z = fft_lds(p, a, b, n, 1, +1);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) grid[(size_t)blockIdx.x * n + j] = Z[j].x;
+}

@@ -1,0 +1,174 @@
+"""ctypes binding of libpizza_amd (the C ABI in include/pizza_amd.h).
+
+This is the Python counterpart of the Fortran driver's ISO_C_BINDING layer:
+one ``Run`` object per simulation, mirroring the reference's module
+procedures of the time-step path (radial_loop.f90:231, mloop.f90:36,
+step_time.f90:215) behind pz_step, and its unit operators (costf1,
+get_dr/get_ddr, fft/ifft).  The library must be built (``make -C
+pizza_amd/csrc``); nothing here falls back to a CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .config import FIELDS, PzConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpizza_amd.so")
+_LIB = None
+
+
+class PizzaError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libpizza_amd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise PizzaError(f"{LIB_PATH} is missing: build it with make -C pizza_amd/csrc")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        L.pz_config_default.argtypes = [ctypes.POINTER(PzConfig)]
+        L.pz_config_default.restype = None
+        L.pz_create.argtypes = [ctypes.POINTER(PzConfig), ctypes.POINTER(vp)]
+        L.pz_destroy.argtypes = [vp]
+        L.pz_last_error.argtypes = [vp]
+        L.pz_last_error.restype = ctypes.c_char_p
+        L.pz_dims.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)]
+        L.pz_set_field.argtypes = [vp, ip, dp]
+        L.pz_get_field.argtypes = [vp, ip, dp]
+        L.pz_init_fields.argtypes = [vp, ip, ctypes.c_double, ip, ctypes.c_double, ip, ctypes.c_double]
+        L.pz_start.argtypes = [vp]
+        L.pz_step.argtypes = [vp, ip]
+        L.pz_time.argtypes = [vp, dp, dp]
+        L.pz_ekin_3d.argtypes = [vp, dp]
+        L.pz_stream.argtypes = [vp]
+        L.pz_stream.restype = vp
+        L.pz_costf.argtypes = [vp, dp, ip]
+        L.pz_get_dr.argtypes = [vp, dp, dp, ip, ip]
+        L.pz_get_ddr.argtypes = [vp, dp, dp, dp, ip, ip]
+        L.pz_fft.argtypes = [vp, dp, dp, ip]
+        L.pz_ifft.argtypes = [vp, dp, dp, ip]
+        L.pz_time_kernel.argtypes = [vp, ip, ip, dp]
+        _LIB = L
+    return _LIB
+
+
+EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_dims",
+           "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step",
+           "pz_time", "pz_ekin_3d", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
+           "pz_fft", "pz_ifft", "pz_time_kernel"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Run:
+    """One simulation on the current HIP device."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        rc = lib().pz_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise PizzaError(f"pz_create failed ({rc})")
+        self.h = h
+        nm, nr, npf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().pz_dims(h, ctypes.byref(nm), ctypes.byref(nr), ctypes.byref(npf))
+        self.n_m, self.n_r, self.n_phi = nm.value, nr.value, npf.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().pz_last_error(self.h)
+            raise PizzaError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def set_field(self, name, arr):
+        """arr: complex (n_m, n_r)."""
+        a = np.asarray(arr, dtype=np.complex128)
+        buf = np.ascontiguousarray(a.T)
+        self._check(lib().pz_set_field(self.h, FIELDS[name], _dp(buf.view(np.float64))), "pz_set_field")
+
+    def get_field(self, name):
+        buf = np.zeros((self.n_r, self.n_m), dtype=np.complex128)
+        self._check(lib().pz_get_field(self.h, FIELDS[name], _dp(buf.view(np.float64))), "pz_get_field")
+        return buf.T.copy()
+
+    def init_fields(self, init_t=0, amp_t=0.0, init_xi=0, amp_xi=0.0, init_u=0, amp_u=0.0):
+        self._check(lib().pz_init_fields(self.h, init_t, amp_t, init_xi, amp_xi, init_u, amp_u),
+                    "pz_init_fields")
+
+    def start(self):
+        self._check(lib().pz_start(self.h), "pz_start")
+
+    def step(self, n=1):
+        self._check(lib().pz_step(self.h, n), "pz_step")
+
+    def time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().pz_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "pz_time")
+        return t.value, dt.value
+
+    def ekin_3d(self):
+        out = np.zeros(4)
+        self._check(lib().pz_ekin_3d(self.h, _dp(out)), "pz_ekin_3d")
+        return out
+
+    def stream(self):
+        return lib().pz_stream(self.h)
+
+    def time_kernel(self, which, reps):
+        ms = ctypes.c_double()
+        self._check(lib().pz_time_kernel(self.h, which, reps, ctypes.byref(ms)), "pz_time_kernel")
+        return ms.value
+
+    # unit operators on complex (n_m, n_r) arrays
+    def costf(self, f):
+        f = np.asarray(f, dtype=np.complex128)
+        buf = np.ascontiguousarray(f.T)
+        self._check(lib().pz_costf(self.h, _dp(buf.view(np.float64)), f.shape[0]), "pz_costf")
+        return buf.T.copy()
+
+    def get_dr(self, f, l_dct_in=True):
+        f = np.asarray(f, dtype=np.complex128)
+        buf = np.ascontiguousarray(f.T)
+        out = np.zeros_like(buf)
+        self._check(lib().pz_get_dr(self.h, _dp(buf.view(np.float64)), _dp(out.view(np.float64)),
+                                    f.shape[0], int(l_dct_in)), "pz_get_dr")
+        return out.T.copy()
+
+    def get_ddr(self, f, l_dct_in=True):
+        f = np.asarray(f, dtype=np.complex128)
+        buf = np.ascontiguousarray(f.T)
+        d1, d2 = np.zeros_like(buf), np.zeros_like(buf)
+        self._check(lib().pz_get_ddr(self.h, _dp(buf.view(np.float64)), _dp(d1.view(np.float64)),
+                                     _dp(d2.view(np.float64)), f.shape[0], int(l_dct_in)), "pz_get_ddr")
+        return d1.T.copy(), d2.T.copy()
+
+    def fft(self, grid):
+        g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64).T)
+        nrad = g.shape[0]
+        out = np.zeros((nrad, self.n_m), dtype=np.complex128)
+        self._check(lib().pz_fft(self.h, _dp(g), _dp(out.view(np.float64)), nrad), "pz_fft")
+        return out.T.copy()
+
+    def ifft(self, spec):
+        s = np.ascontiguousarray(np.asarray(spec, dtype=np.complex128).T)
+        nrad = s.shape[0]
+        out = np.zeros((nrad, self.n_phi))
+        self._check(lib().pz_ifft(self.h, _dp(s.view(np.float64)), _dp(out), nrad), "pz_ifft")
+        return out.T.copy()
