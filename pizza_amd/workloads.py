@@ -1,0 +1,70 @@
+"""Named workloads of the IMEX step and their seeded initial conditions.
+
+``c2_production`` is the headline configuration: sparse-Chebyshev
+integration, n_r_max = 97, m_max = 128 (n_phi = 384), Ekman 1e-5, Ra 3e7,
+Pr 1, thermal + compositional (Sc 3), DIRK BPR353, dt 5e-6.
+``c1_collocation_shape`` keeps the C1 physical parameters (n_r 33, n_phi 96,
+Ekman 1e-3, Ra 1e5, Ekman pumping, CNAB2 dt 1e-4) on the integration method.
+
+Seeded perturbation (both runs get the same fields through set_field):
+    f(m, r) = A * g_m * sin(pi (r - r_icb)),  m = 1..m_max,
+g_m complex Gaussian drawn from numpy's PCG64 with a fixed seed.
+"""
+import numpy as np
+
+from .config import BPR353, CNAB2, default_config
+
+SEED = 20261202
+
+
+def c2_production(n_r=97, m_max=128, raxi=3.0e7):
+    return default_config(
+        n_r_max=n_r, n_cheb_max=n_r, m_max=m_max, minc=1, time_scheme="BPR353",
+        cheb_method="integ", matrix_solve="direct", dtMax=5.0e-6,
+        ra=3.0e7, ek=1.0e-5, pr=1.0, raxi=raxi, sc=3.0, radratio=0.35,
+        ktopt=1, kbott=1, ktopxi=1, kbotxi=1, ktopv=2, kbotv=2, l_ek_pump=True,
+        g1=1.0)
+
+
+def c1_collocation_shape():
+    c = default_config(
+        n_r_max=33, n_cheb_max=33, m_max=32, minc=1, time_scheme="CNAB2",
+        cheb_method="integ", matrix_solve="direct", dtMax=1.0e-4,
+        ra=1.0e5, ek=1.0e-3, pr=1.0, radratio=0.35, l_ek_pump=True)
+    assert c.time_scheme == CNAB2
+    return c
+
+
+def small_bpr353(n_r=33, m_max=32):
+    """A CPU-sized version of the production physics (parity tests)."""
+    c = c2_production(n_r=n_r, m_max=m_max)
+    assert c.time_scheme == BPR353
+    return c
+
+
+def seeded_field(n_m, r, r_icb, amp, seed=SEED, stream=0):
+    """complex (n_m, n_r) perturbation A g_m sin(pi (r - r_icb)), m >= 1."""
+    rng = np.random.Generator(np.random.PCG64([seed, stream]))
+    g = rng.standard_normal(n_m) + 1j * rng.standard_normal(n_m)
+    g[0] = 0.0
+    prof = np.sin(np.pi * (np.asarray(r) - r_icb))
+    return amp * g[:, None] * prof[None, :]
+
+
+def radial_grid(cfg):
+    """Gauss-Lobatto radii r_k (chebyshev.f90:188-240), outer boundary first."""
+    n = cfg.n_r_max
+    r_cmb = 1.0 / (1.0 - cfg.radratio)
+    r_icb = r_cmb - 1.0
+    x = np.cos(np.pi * np.arange(n) / (n - 1))
+    return 0.5 * (r_cmb - r_icb) * x + 0.5 * (r_cmb + r_icb), r_icb
+
+
+def seed_run(run, cfg, amp_t=1.0e-4, amp_xi=1.0e-4, seed=SEED):
+    """Injects the seeded T and xi perturbations into a Run or an oracle."""
+    r, r_icb = radial_grid(cfg)
+    n_m = cfg.m_max // cfg.minc + 1
+    if cfg.ra != 0.0:
+        run.set_field("temp", seeded_field(n_m, r, r_icb, amp_t, seed, 0))
+    if cfg.raxi != 0.0:
+        run.set_field("xi", seeded_field(n_m, r, r_icb, amp_xi, seed, 1))

@@ -1,0 +1,150 @@
+"""GPU parity tests: libpizza_amd.so (through its C ABI) against the oracle.
+
+Tolerances.  The relative field error is max|gpu - oracle| / max|oracle|.
+A field's conditioning floor is how far the oracle's own result moves when
+its initial T (and xi) are perturbed by 1e-16 relative; second radial
+derivatives amplify rounding, so the floor of omega and u_phi is far above
+1e-11 (n_r = 97, one step: omega 6e-9, u_phi 1e-9, u_s 9e-11, psi 2e-11,
+T and xi 2e-14).  Each field must agree within max(1e-11, 10 x floor) after
+one step and within max(1e-8, 10 x floor) after several steps.
+"""
+import numpy as np
+import pytest
+
+from helpers import golden_rows, rel, rotinteg_config
+
+pytestmark = pytest.mark.gpu
+
+TOL1 = {"temp": 1e-11, "xi": 1e-11, "psi": 1e-11, "us": 1e-11, "up": 1e-9, "om": 1e-8}
+TOLN = 1e-8
+
+
+def _pair(cfg):
+    from oracle.oracle import Oracle
+    from pizza_amd.runtime import Run
+    return Run(cfg), Oracle(cfg)
+
+
+def _fields(cfg):
+    f = ["us", "up", "om", "psi"]
+    if cfg.ra != 0:
+        f.append("temp")
+    if cfg.raxi != 0:
+        f.append("xi")
+    return f
+
+
+def test_unit_operators():
+    from pizza_amd.workloads import c2_production
+    cfg = c2_production(n_r=97, m_max=128)
+    g, o = _pair(cfg)
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((7, g.n_r)) + 1j * rng.standard_normal((7, g.n_r))
+    assert rel(g.costf(f), o.costf(f)) < 1e-14
+    assert rel(g.costf(g.costf(f)), f) < 1e-13           # involution
+    assert rel(g.get_dr(f), o.get_dr(f)) < 1e-13
+    assert rel(g.get_dr(f, False), o.get_dr(f, False)) < 1e-13
+    for a, b in zip(g.get_ddr(f, False), o.get_ddr(f, False)):
+        assert rel(a, b) < 1e-12
+    grid = rng.standard_normal((g.n_phi, 5))
+    assert rel(g.fft(grid), o.fft(grid)) < 1e-14
+    sp = o.fft(grid)
+    assert rel(g.ifft(sp), o.ifft(sp)) < 1e-14
+    assert rel(g.fft(g.ifft(sp)), sp) < 1e-14            # round trip
+
+
+def test_rotinteg_parity_and_golden():
+    """Reference sample RotInteg (integration, CNAB2): fields vs the oracle
+    after 1 and 100 steps, and the e_kin_3D series vs the reference's
+    golden file over 1000 steps."""
+    cfg, start, _ = rotinteg_config()
+    g, o = _pair(cfg)
+    for x in (g, o):
+        x.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+        x.start()
+    g.step(1)
+    o.step(1)
+    for f in _fields(cfg):
+        assert rel(g.get_field(f), o.get_field(f)) < TOL1[f], f
+    g.step(99)
+    o.step(99)
+    for f in _fields(cfg):
+        assert rel(g.get_field(f), o.get_field(f)) < TOLN, f
+    ref = golden_rows(51)
+    g2, _ = _pair(cfg)
+    g2.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+    g2.start()
+    rows = []
+    for k in range(51):
+        if k:
+            g2.step(20)
+        t, _ = g2.time()
+        rows.append(np.r_[t, g2.ekin_3d()])
+    np.testing.assert_allclose(np.array(rows), ref, rtol=1e-8, atol=1e-20)
+
+
+def _floor(cfg, steps, seed_fn):
+    """Per-field conditioning floor of the oracle after each count in steps."""
+    from oracle.oracle import Oracle
+    runs = []
+    for trial in range(2):
+        o = Oracle(cfg)
+        seed_fn(o)
+        if trial:
+            rng = np.random.default_rng(7)
+            for f in ("temp", "xi"):
+                if (f == "temp" and cfg.ra) or (f == "xi" and cfg.raxi):
+                    a = o.get_field(f)
+                    o.set_field(f, a * (1 + 1e-16 * rng.standard_normal(a.shape)))
+        o.start()
+        out, done = [], 0
+        for n in steps:
+            o.step(n - done)
+            done = n
+            out.append({f: o.get_field(f) for f in _fields(cfg)})
+        runs.append(out)
+    return [{f: rel(runs[1][i][f], runs[0][i][f]) for f in _fields(cfg)} for i in range(len(steps))]
+
+
+@pytest.mark.parametrize("n_r,m_max,nsteps", [(33, 32, 20), (97, 128, 3)])
+def test_bpr353_thermochemical_parity(n_r, m_max, nsteps):
+    """Production physics (BPR353, T + xi, Ekman pumping) at a small and at
+    the production shape."""
+    from pizza_amd import workloads as W
+    cfg = W.c2_production(n_r=n_r, m_max=m_max)
+    floors = _floor(cfg, [1, nsteps], lambda x: W.seed_run(x, cfg))
+    g, o = _pair(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg)
+        x.start()
+    g.step(1)
+    o.step(1)
+    assert abs(g.time()[0] - o.time()[0]) < 1e-15
+    for f in _fields(cfg):
+        err = rel(g.get_field(f), o.get_field(f))
+        assert err < max(1e-11, 10 * floors[0][f]), (f, err, floors[0][f])
+    for f in ("temp", "xi"):
+        assert rel(g.get_field(f), o.get_field(f)) < 1e-11, f
+    g.step(nsteps - 1)
+    o.step(nsteps - 1)
+    for f in _fields(cfg):
+        err = rel(g.get_field(f), o.get_field(f))
+        assert err < max(TOLN, 10 * floors[1][f]), (f, err, floors[1][f])
+    for f in ("temp_hat", "xi_hat", "psi_hat"):
+        assert rel(g.get_field(f), o.get_field(f)) < TOLN, f
+
+
+def test_graph_replay_is_deterministic():
+    """Ten steps in one call equal ten single-step calls bit for bit."""
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Run
+    cfg = W.small_bpr353()
+    a, b = Run(cfg), Run(cfg)
+    for x in (a, b):
+        W.seed_run(x, cfg)
+        x.start()
+    a.step(10)
+    for _ in range(10):
+        b.step(1)
+    for f in ("temp", "xi", "om", "psi"):
+        assert np.array_equal(a.get_field(f), b.get_field(f)), f

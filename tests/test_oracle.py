@@ -1,0 +1,188 @@
+"""CPU tests that pin the oracle (oracle/pz_oracle.c) before it is trusted.
+
+1. End-to-end against the reference's own golden time series
+   (samples/RotInteg/reference.out, first run: integration method, CNAB2,
+   rotating, implicit Coriolis and buoyancy, l_rerror_fix, n_cheb_max <
+   n_r_max), compared with the reference test's rtol 1e-8
+   (samples/RotInteg/unitTest.py:116-120).
+2. Closed forms for the transforms (DCT-I involution and Chebyshev-node
+   identities, single-bin azimuthal FFT, exact derivatives of polynomials).
+3. The sparse-Chebyshev stencils against an independent construction with
+   numpy.polynomial.chebyshev (apply the differential operator to T_j,
+   integrate J times, read the coefficient of T_n).
+"""
+import numpy as np
+import numpy.polynomial.chebyshev as C
+import pytest
+
+from helpers import golden_rows, rel, rotinteg_config
+from oracle.oracle import Oracle
+
+
+@pytest.fixture(scope="module")
+def rot():
+    cfg, start, _ = rotinteg_config()
+    return cfg, start
+
+
+def test_rotinteg_golden_series(rot):
+    """1000 CNAB2 steps (t = 0..100) of the RotInteg start run."""
+    cfg, start = rot
+    ref = golden_rows(51)
+    o = Oracle(cfg)
+    o.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+    o.start()
+    rows = []
+    for k in range(51):
+        if k:
+            o.step(20)
+        t, _ = o.time()
+        rows.append(np.r_[t, o.ekin_3d()])
+    rows = np.array(rows)
+    np.testing.assert_allclose(rows, ref, rtol=1e-8, atol=1e-20)
+
+
+def _oracle(n_r=33, m_max=32):
+    from pizza_amd.workloads import small_bpr353
+    return Oracle(small_bpr353(n_r, m_max))
+
+
+def test_costf_closed_forms():
+    o = _oracle(33)
+    N = o.n_r
+    r, x = o.grid()
+    # DCT-I of T_k at the Gauss-Lobatto nodes is a scaled unit vector
+    for k in (0, 1, 5, N - 2, N - 1):
+        f = np.cos(k * np.arccos(np.clip(x, -1, 1)))[None, :].astype(complex)
+        c = o.costf(f)[0]
+        expect = np.zeros(N)
+        expect[k] = (N - 1) * np.sqrt(0.5 / (N - 1)) * (2.0 if k in (0, N - 1) else 1.0)
+        assert np.max(np.abs(c - expect)) < 1e-13 * (N - 1)
+    # involution
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((4, N)) + 1j * rng.standard_normal((4, N))
+    assert rel(o.costf(o.costf(f)), f) < 1e-14
+    # brute-force cosine sum
+    j = np.arange(N)
+    w = np.where((j == 0) | (j == N - 1), 1.0, 2.0)
+    M = np.sqrt(0.5 / (N - 1)) * w[None, :] * np.cos(np.pi * np.outer(j, j) / (N - 1))
+    assert rel(o.costf(f), f @ M.T) < 1e-14
+
+
+def test_azimuthal_fft_closed_forms():
+    o = _oracle(17, 32)
+    n = o.n_phi
+    phi = 2 * np.pi * np.arange(n) / n
+    g = np.stack([np.cos(5 * phi + 0.3), 2.0 + 0 * phi, np.sin(32 * phi)], axis=1)
+    s = o.fft(g)
+    expect = np.zeros((o.n_m, 3), dtype=complex)
+    expect[5, 0] = 0.5 * np.exp(0.3j)
+    expect[0, 1] = 2.0
+    expect[32, 2] = -0.5j
+    assert np.max(np.abs(s - expect)) < 1e-15 * n
+    rng = np.random.default_rng(4)
+    sp = rng.standard_normal((o.n_m, 3)) + 1j * rng.standard_normal((o.n_m, 3))
+    sp[0] = sp[0].real
+    assert rel(o.fft(o.ifft(sp)), sp) < 1e-14
+
+
+def test_radial_derivatives_exact_on_polynomials():
+    o = _oracle(33)
+    r, _ = o.grid()
+    f = np.stack([r ** 5, 3 * r ** 2 - r ** 7]).astype(complex)
+    d1, d2 = o.get_ddr(f)
+    e1 = np.stack([5 * r ** 4, 6 * r - 7 * r ** 6])
+    e2 = np.stack([20 * r ** 3, 6 - 42 * r ** 5])
+    assert rel(d1, e1) < 1e-12
+    assert rel(d2, e2) < 1e-10
+    assert rel(o.get_dr(f), e1) < 1e-12
+
+
+# ---- stencils -------------------------------------------------------------
+def _cheb_of_poly_in_r(pcoef_r, a, b):
+    """Chebyshev series (in x) of sum_k p_k r^k with r = a x + b."""
+    rx = np.array([b, a])
+    out = np.zeros(1)
+    powk = np.array([1.0])
+    for k, pk in enumerate(pcoef_r):
+        if k:
+            powk = C.chebmul(powk, rx)
+        if pk:
+            out = C.chebadd(out, pk * powk)
+    return out
+
+
+def _apply(poly_ops, J, j, a, b):
+    """I_r^J [sum_k p_k(r) D_r^k] T_j (T_0 counted with weight 1/2)."""
+    basis = np.zeros(j + 1)
+    basis[j] = 0.5 if j == 0 else 1.0
+    acc = np.zeros(1)
+    for k, p in poly_ops.items():
+        d = C.chebder(basis, m=k) / a ** k if k else basis
+        acc = C.chebadd(acc, C.chebmul(_cheb_of_poly_in_r(p, a, b), d))
+    return C.chebint(acc, m=J) * a ** J
+
+
+def _stencil_ref(poly_ops, J, idx, K, a, b):
+    """Folded stencil of row idx in the reference's band order."""
+    st = np.zeros(2 * K + 1)
+    for nb in range(2 * K + 1):
+        col = idx + K - nb
+        if col < 0:
+            continue
+        g = _apply(poly_ops, J, col, a, b)
+        st[nb] = g[idx] if idx < len(g) else 0.0
+    return st
+
+
+@pytest.mark.parametrize("which,J,K,m", [(0, 2, 4, 0), (1, 2, 3, 0), (2, 2, 2, 7), (5, 4, 8, 0),
+                                         (6, 4, 7, 0), (7, 4, 8, 5), (8, 4, 8, 5), (4, 1, 2, 0)])
+def test_sparse_stencils(which, J, K, m):
+    o = _oracle(25, 16)
+    r, _ = o.grid()
+    r_cmb, r_icb = r[0], r[-1]
+    a, b = 0.5 * (r_cmb - r_icb), 0.5 * (r_cmb + r_icb)
+    rc2 = r_cmb ** 2
+    m2 = m * m
+    # polynomial coefficient lists (ascending powers of r) per derivative order
+    if which == 0:
+        ops = {0: [0, 0, 1]}
+    elif which == 1:
+        ops = {0: [0, 1]}
+    elif which == 2:  # r^2 Delta_m
+        ops = {2: [0, 0, 1], 1: [0, 1], 0: [-m2]}
+    elif which == 4:
+        ops = {0: [0, 1]}
+    elif which == 5:
+        ops = {0: [0, 0, 0, 0, 1]}
+    elif which == 6:
+        ops = {0: [0, 0, 0, 1]}
+    else:
+        # q = r^2 omega(psi) = -r^2 h^2 psi'' - (rc^2 r - 6 r^3) psi' + (6 r^2 + m^2 h^2) psi
+        q = {2: [0, 0, -rc2, 0, 1], 1: [0, -rc2, 0, 6], 0: [m2 * rc2, 0, 6 - m2]}
+        if which == 7:  # r^4 omega = r^2 q
+            ops = {k: [0, 0] + v for k, v in q.items()}
+        else:  # r^4 Delta_m omega = r^2 q'' - 3 r q' + (4 - m^2) q
+            P = np.polynomial.polynomial
+            ops = {}
+
+            def add(k, p):
+                cur = ops.get(k, [0])
+                ops[k] = list(P.polyadd(cur, p))
+            for k, p in q.items():
+                dp, ddp = P.polyder(p), P.polyder(p, 2)
+                # q'' = p'' psi^(k) + 2 p' psi^(k+1) + p psi^(k+2)
+                add(k, P.polymul([0, 0, 1], ddp))
+                add(k + 1, P.polymul([0, 0, 2], dp))
+                add(k + 2, P.polymul([0, 0, 1], p))
+                # -3 r q' = -3 r (p' psi^(k) + p psi^(k+1))
+                add(k, P.polymul([0, -3], dp))
+                add(k + 1, P.polymul([0, -3], p))
+                add(k, P.polymul([4 - m2], p))
+    for idx in range(J, 25):
+        got = o.stencil(which, m, idx, K)
+        ref = _stencil_ref(ops, J, idx, K, a, b)
+        cols = idx + K - np.arange(2 * K + 1)
+        got = np.where(cols >= 0, got, 0.0)  # unfolded negative-column leftovers
+        scale = max(np.max(np.abs(ref)), 1e-300)
+        assert np.max(np.abs(got - ref)) / scale < 1e-12, (which, idx, got, ref)
