@@ -1,0 +1,140 @@
+"""Benchmark of the FP64 IMEX step on one MI355X.
+
+Workload: the production configuration (sparse-Chebyshev integration,
+n_r_max = 97, m_max = 128 so n_phi = 384, Ekman 1e-5, Ra 3e7, Pr 1,
+thermal + compositional with Sc 3, DIRK BPR353, dt 5e-6), seeded random
+T and xi perturbations of amplitude 1e-4.  One "step" is one full IMEX
+time step (4 BPR353 stages: 3 radial loops, 4 m-loop solves) with all
+state resident in HBM.
+
+Prints one JSON line: steps/s, the roofline figure of the dominant kernel
+(the m-loop stage kernel k_mphase, timed with HIP events on the library
+stream) and a CPU baseline of the oracle (single-thread C restatement)
+timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+from pizza_amd import workloads as W  # noqa: E402
+from pizza_amd.runtime import Run  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def mphase_bytes(cfg, nstage_slots=1):
+    """Algorithmic HBM bytes of one k_mphase launch (stage 1, explicit part on).
+
+    Per azimuthal wavenumber m, with F = 16 n_r bytes (one complex radial
+    column):
+      reads : dVs x3, current explicit slot x3, psi/om/up/us (Ekman pumping),
+              old x3 and impl(stage 1) x3 for the IMEX right-hand side,
+              preconditioners x3 (8 n_r), bordered LU factors of T, xi
+              (A4 13 x (n_r-2) + A2, A3 2 x (n_r-2) reals + pivots) and psi
+              (A4 25 x (n_r-4) + A2, A3 4 x (n_r-4) complex + pivots), the
+              per-m bands RHSI_T (5 n_r reals), RHSI_psi (17 n_r reals) and
+              RHSIL_psi (17 n_r complex);
+      writes: explicit slot x3, T, xi, hats x3, psi/us/up/om, impl x3.
+    """
+    N = cfg.n_r_max
+    nm = cfg.m_max // cfg.minc + 1
+    F = 16 * N
+    nb2, nb4 = N - 2, N - 4
+    reads = 16 * F + 3 * 8 * N
+    reads += 2 * (13 * nb2 * 8 + 4 * nb2 * 8 + 4 * nb2 + 4 * 8 + 8)
+    reads += 25 * nb4 * 16 + 8 * nb4 * 16 + 4 * nb4 + 16 * 16 + 16
+    reads += 5 * N * 8 + 17 * N * 8 + 17 * N * 16
+    writes = 15 * F
+    return nm * (reads + writes)
+
+
+def cpu_baseline(cfg, seconds=15.0):
+    """Oracle (single-thread C restatement) steps/s on a bounded sample."""
+    from oracle.oracle import Oracle
+    o = Oracle(cfg)
+    W.seed_run(o, cfg)
+    o.start()
+    o.step(1)  # includes the first matrix factorisation
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        o.step(1)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} steps of the same workload (n_r=97, n_phi=384, BPR353) "
+                      f"after one untimed step, {dt:.1f} s on one host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs each simulation on a single "
+              f"MI355X (requested --gpus {args.gpus}); ensembles shard whole runs per GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    cfg = W.c2_production()
+    run = Run(cfg)
+    W.seed_run(run, cfg)
+    run.start()
+    run.step(args.warmup)
+    run.time()  # synchronises the library stream
+    t0 = time.perf_counter()
+    run.step(args.steps)
+    t_end, dt = run.time()
+    elapsed = time.perf_counter() - t0
+    steps_per_s = args.steps / elapsed
+
+    # dominant kernel: k_mphase, timed on a second handle so the timed run
+    # keeps a valid state
+    krun = Run(cfg)
+    W.seed_run(krun, cfg)
+    krun.start()
+    krun.step(2)
+    krun.time()
+    reps = 200
+    avg_ms = krun.time_kernel(0, reps)
+    bytes_launch = mphase_bytes(cfg)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    krun.close()
+
+    line = {
+        "metric": "FP64 IMEX steps/s (n_r=97,n_phi=384)",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded T/xi perturbation, amplitude 1e-4)",
+        "config": {"workload": "integ-BPR353 n_r=97 n_phi=384 E=1e-5 Ra=3e7 Pr=1 Sc=3 Raxi=3e7 "
+                               "Ekman pumping, dt=5e-6",
+                   "n_r_max": cfg.n_r_max, "n_phi_max": 3 * cfg.m_max, "m_max": cfg.m_max,
+                   "time_scheme": "BPR353", "sim_time_end": t_end, "dt": dt},
+        "roofline": {"bound": "hbm", "kernel": "k_mphase (m-loop stage kernel)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
