@@ -531,6 +531,8 @@ static void setup(pz_handle *h)
     d.T_A1 = h->dalloc<double>(nm * 4); d.T_A2 = h->dalloc<double>((size_t)nm * 2 * nb2);
     d.T_A3 = h->dalloc<double>((size_t)nm * 2 * nb2); d.T_A4 = h->dalloc<double>((size_t)nm * PZ_T_LD * nb2);
     d.T_p1 = h->dalloc<int>(nm * 2); d.T_p4 = h->dalloc<int>((size_t)nm * nb2); d.T_fac = h->dalloc<double>(F);
+    d.T_dinv = h->dalloc<double>((size_t)nm * nb2); d.X_dinv = h->dalloc<double>((size_t)nm * nb2);
+    d.P_dinv = h->dalloc<cd>((size_t)nm * N);
     d.X_A1 = h->dalloc<double>(nm * 4); d.X_A2 = h->dalloc<double>((size_t)nm * 2 * nb2);
     d.X_A3 = h->dalloc<double>((size_t)nm * 2 * nb2); d.X_A4 = h->dalloc<double>((size_t)nm * PZ_T_LD * nb2);
     d.X_p1 = h->dalloc<int>(nm * 2); d.X_p4 = h->dalloc<int>((size_t)nm * nb2); d.X_fac = h->dalloc<double>(F);
@@ -542,7 +544,7 @@ static void setup(pz_handle *h)
     build_operators(h);
     // LDS budgets
     h->lds_radial = (5 * (size_t)h->nm + 6 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
-    h->lds_mphase = (size_t)MPHASE_NVEC * N * sizeof(cd) + 2 * (size_t)N * sizeof(double);
+    h->lds_mphase = MpLayout(N).total;
     h->lds_start = 15 * (size_t)N * sizeof(cd);
     {
         size_t nbp = N - 4;

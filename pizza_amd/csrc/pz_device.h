@@ -415,6 +415,106 @@ __device__ void small_getrs(int nt, const M *a, const int *piv, cd *b)
     }
 }
 
+
+template <class M> __device__ __forceinline__ M rlm(M v, int lane);
+template <> __device__ __forceinline__ double rlm<double>(double v, int lane) { return rl(v, lane); }
+template <> __device__ __forceinline__ cd rlm<cd>(cd v, int lane) { return rlc(v, lane); }
+
+// Banded solve with the factors in LDS (same semantics as wave_gbtrs).
+// Off the recurrence: the next step's band coefficient is fetched one step
+// ahead, the pivot indices and the reciprocal diagonal of U sit in lane
+// registers and are broadcast with v_readlane.
+template <class M, int KL, int KU>
+__device__ void wave_gbtrs_fast(int n, const M *ab, int ld, const int *ipiv, const M *dinv, cd *b)
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int kd = KL + KU, K = KL + KU;
+    int j = 0;
+    while (j < n - 1) {
+        const int j0 = j;
+        cd v = (j0 + lane < n) ? b[j0 + lane] : mk(0, 0);
+        const int preg = (j0 + lane < n) ? ipiv[j0 + lane] : 0;
+        const int jend = min(n - 1, j0 + 64 - KL);
+        const int base = kd + lane + j0 * ld;
+        M cn = mzero<M>();
+        if (lane >= 1 && lane <= KL && j0 + lane < n) cn = ab[base];
+        for (int h = 0; j < jend; j++, h++) {
+            const M c = cn;
+            {
+                const int dn = lane - h - 1;
+                cn = (dn >= 1 && dn <= KL && j + 1 + dn < n) ? ab[base + (h + 1) * (ld - 1)] : mzero<M>();
+            }
+            const int pv = __builtin_amdgcn_readlane(preg, h) - j;
+            if (pv != 0) {
+                cd a = rlc(v, h), e = rlc(v, h + pv);
+                if (lane == h) v = e;
+                else if (lane == h + pv) v = a;
+            }
+            const cd bj = rlc(v, h);
+            const int d = lane - h;
+            if (d >= 1 && d <= KL && j + d < n) v = csub(v, mmul(c, bj));
+        }
+        if (j0 + lane < n) b[j0 + lane] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+    j = n - 1;
+    while (j >= 0) {
+        const int j0 = j;
+        cd v = (j0 - lane >= 0) ? b[j0 - lane] : mk(0, 0);
+        const M dreg = (j0 - lane >= 0) ? dinv[j0 - lane] : mzero<M>();
+        const int jend = max(-1, j0 - (64 - K));
+        const int base = kd - lane + j0 * ld;
+        M cn = mzero<M>();
+        if (lane >= 1 && lane <= K && j0 - lane >= 0) cn = ab[base];
+        for (int h = 0; j > jend; j--, h++) {
+            const M c = cn;
+            {
+                const int dn = lane - h - 1;
+                cn = (dn >= 1 && dn <= K && j - 1 - dn >= 0) ? ab[base - (h + 1) * (ld - 1)] : mzero<M>();
+            }
+            const cd xj = mmul(rlm<M>(dreg, h), rlc(v, h));
+            if (lane == h) v = xj;
+            const int d = lane - h;
+            if (d >= 1 && d <= K && j - d >= 0) v = csub(v, mmul(c, xj));
+        }
+        if (j0 - lane >= 0) b[j0 - lane] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// bordered solve with all factors in LDS
+template <class M, int KL, int KU>
+__device__ void wave_bordered_fast(int N, int nt, const M *A1, const int *p1, const M *A2, const M *A3,
+                                   const M *A4, const int *p4, const M *dinv, cd *rhs)
+{
+    const int lane = threadIdx.x & 63;
+    const int nb = N - nt, ld = 2 * KL + KU + 1;
+    cd *r2 = rhs + nt;
+    wave_gbtrs_fast<M, KL, KU>(nb, A4, ld, p4, dinv, r2);
+    for (int i = 0; i < nt; i++) {
+        double sx = 0.0, sy = 0.0;
+        for (int c = lane; c < nb; c += 64) {
+            cd t = mmul(A2[i + (size_t)c * nt], r2[c]);
+            sx += t.x;
+            sy += t.y;
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        if (lane == 0) rhs[i] = mk(rhs[i].x - sx, rhs[i].y - sy);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) small_getrs<M>(nt, A1, p1, rhs);
+    __builtin_amdgcn_wave_barrier();
+    cd y[4];
+    for (int c = 0; c < nt; c++) y[c] = rhs[c];
+    for (int i = lane; i < nb; i += 64) {
+        cd v = r2[i];
+        for (int c = 0; c < nt; c++) v = csub(v, mmul(A3[i + (size_t)c * nb], y[c]));
+        r2[i] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // bordered solve (algebra_lapack.f90:332-416) by one wave; rhs in LDS
 template <class M>
 __device__ void wave_bordered_solve(int N, int nt, int kl, int ku, int ld, const M *A1, const int *p1,
@@ -453,17 +553,20 @@ __device__ void wave_bordered_solve(int N, int nt, int kl, int ku, int ld, const
 // Chebyshev transforms on LDS vectors, whole workgroup (dct_fftw.f90)
 // ---------------------------------------------------------------------------
 // costf (DCT-I with cheb_fac) of nf vectors: out[f][k]
-__device__ void dct_many(const PzDev &p, const cd *const *in, cd *const *out, int nf, int dealias)
+__device__ void dct_many(const PzDev &p, const cd *const *in, cd *const *out, int nf, int dealias,
+                         const double *cosT = nullptr)
 {
     const int N = p.N, L = p.L;
+    if (!cosT) cosT = p.cosT;
     for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
         int f = t / N, k = t - f * N;
         const cd *x = in[f];
         double sx = 0.0, sy = 0.0;
         int q = k;
+#pragma unroll 4
         for (int j = 1; j < N - 1; j++, q += k) {
             if (q >= L) q -= L;
-            double c = p.cosT[q];
+            double c = cosT[q];
             sx += x[j].x * c;
             sy += x[j].y * c;
         }
@@ -479,9 +582,12 @@ __device__ void dct_many(const PzDev &p, const cd *const *in, cd *const *out, in
 // first (and optionally second) radial derivative from Chebyshev
 // coefficients (get_dr_fft/get_ddr_fft with l_dct_in = .false.) followed
 // by the drx/ddrx map; nf vectors
-__device__ void dr_many(const PzDev &p, const cd *const *c, cd *const *df, cd *const *ddf, int nf)
+__device__ void dr_many(const PzDev &p, const cd *const *c, cd *const *df, cd *const *ddf, int nf,
+                        const double *cosT = nullptr, const double *sinT = nullptr)
 {
     const int N = p.N, L = p.L;
+    if (!cosT) cosT = p.cosT;
+    if (!sinT) sinT = p.sinT;
     const double cf = p.cheb_fac;
     for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
         int f = t / N, j = t - f * N;
@@ -511,10 +617,11 @@ __device__ void dr_many(const PzDev &p, const cd *const *c, cd *const *df, cd *c
             // V1 = sum_{k=1}^{N-2} i k w_k (e^{i pi kj/(N-1)} - e^{-i...}) = -2 sum k w_k sin
             double vx = 0, vy = 0, ux = 0, uy = 0;
             int q = j;
+#pragma unroll 4
             for (int k = 1; k < N - 1; k++, q += j) {
                 if (q >= L) q -= L;
                 double wx = w[k].x / cf, wy = w[k].y / cf;
-                double s = p.sinT[q], co = p.cosT[q];
+                double s = sinT[q], co = cosT[q];
                 vx += (double)k * wx * s;
                 vy += (double)k * wy * s;
                 if (ddf) {
@@ -649,7 +756,7 @@ template <class M>
 __device__ void build_bordered(const PzDev &p, int nt, int kl, int ku, int K, const double *S0,
                                const double *S1, const double *S2, double c1, double c2,
                                const double *tauv, M *A1g, M *A2g, M *A3g, M *A4g, int *p1g,
-                               int *p4g, double *facg, int *infog, char *lds)
+                               int *p4g, M *dinvg, double *facg, int *infog, char *lds)
 {
     const int N = p.N, nb = N - nt, ld = 2 * kl + ku + 1, W = 2 * K + 1;
     const int lane = threadIdx.x & 63;
@@ -787,6 +894,7 @@ __device__ void build_bordered(const PzDev &p, int nt, int kl, int ku, int K, co
     for (int i = lane; i < nt * nt; i += 64) A1g[i] = A1[i];
     for (int i = lane; i < N; i += 64) facg[i] = fac[i];
     for (int i = lane; i < nb; i += 64) p4g[i] = p4[i];
+    for (int i = lane; i < nb; i += 64) dinvg[i] = mrecip(A4[(kl + ku) + (size_t)i * ld]);
     if (lane < nt) p1g[lane] = s_p1[lane];
     if (lane == 0) *infog = info;
 }
@@ -821,7 +929,8 @@ __global__ void __launch_bounds__(64) k_build_lhs(PzDev p, int force)
         int *p1 = (fld == 0 ? p.T_p1 : p.X_p1) + (size_t)mi * nt;
         int *p4 = (fld == 0 ? p.T_p4 : p.X_p4) + (size_t)mi * nb;
         double *fac = (fld == 0 ? p.T_fac : p.X_fac) + (size_t)mi * N;
-        build_bordered<double>(p, nt, kl, ku, K, S0, S1, nullptr, c1, 0.0, tauv, A1, A2, A3, A4, p1, p4, fac,
+        double *dinv = (fld == 0 ? p.T_dinv : p.X_dinv) + (size_t)mi * nb;
+        build_bordered<double>(p, nt, kl, ku, K, S0, S1, nullptr, c1, 0.0, tauv, A1, A2, A3, A4, p1, p4, dinv, fac,
                                p.info + fld * p.nm + mi, lds + 4 * N * sizeof(double));
     } else {
         const int nt = m == 0 ? 2 : PZ_P_NT, kl = m == 0 ? 4 : PZ_P_KL, ku = kl;
@@ -861,13 +970,14 @@ __global__ void __launch_bounds__(64) k_build_lhs(PzDev p, int force)
         int *p1 = p.P_p1 + (size_t)mi * 4;
         int *p4 = p.P_p4 + (size_t)mi * N;
         double *fac = p.P_fac + (size_t)mi * N;
+        cd *dinv = p.P_dinv + (size_t)mi * N;
         (void)ldm;
         char *scr = lds + 4 * N * sizeof(double);
         if (m == 0)
-            build_bordered<cd>(p, nt, kl, ku, 4, S0, S1, nullptr, c1, 0.0, tauv, A1, A2, A3, A4, p1, p4, fac,
-                               p.info + 2 * p.nm + mi, scr);
+            build_bordered<cd>(p, nt, kl, ku, 4, S0, S1, nullptr, c1, 0.0, tauv, A1, A2, A3, A4, p1, p4, dinv,
+                               fac, p.info + 2 * p.nm + mi, scr);
         else
-            build_bordered<cd>(p, nt, kl, ku, K, S0, S1, S2, c1, c2, tauv, A1, A2, A3, A4, p1, p4, fac,
+            build_bordered<cd>(p, nt, kl, ku, K, S0, S1, S2, c1, c2, tauv, A1, A2, A3, A4, p1, p4, dinv, fac,
                                p.info + 2 * p.nm + mi, scr);
     }
 }

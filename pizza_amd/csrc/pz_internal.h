@@ -73,11 +73,12 @@ struct PzDev {
     const double *T_S0, *T_S1, *X_S1;   // [nm][n_r][9]
     const double *P_S0, *P_S1, *P_S2;   // [nm][n_r][17]
     // factorised bordered LHS (written by k_build_lhs)
-    double *T_A1, *T_A2, *T_A3, *T_A4, *T_fac;
+    // dinv: reciprocals of the diagonal of U of the banded block
+    double *T_A1, *T_A2, *T_A3, *T_A4, *T_fac, *T_dinv;
     int *T_p1, *T_p4;
-    double *X_A1, *X_A2, *X_A3, *X_A4, *X_fac;
+    double *X_A1, *X_A2, *X_A3, *X_A4, *X_fac, *X_dinv;
     int *X_p1, *X_p4;
-    cd *P_A1, *P_A2, *P_A3, *P_A4;
+    cd *P_A1, *P_A2, *P_A3, *P_A4, *P_dinv;
     double *P_fac;
     int *P_p1, *P_p4;
     int *info; // [3 * nm] factorisation info per (field, m)

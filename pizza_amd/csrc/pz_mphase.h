@@ -76,6 +76,49 @@ __device__ void dcheb_real(const PzDev &p, const double *f, double *df)
 
 #define MPHASE_NVEC 18
 
+// LDS layout of k_mphase: working vectors, transform tables and the LU
+// factors of the three bordered systems of one m (16-byte aligned blocks).
+struct MpLayout {
+    size_t vec, dtmp, cosT, sinT;
+    size_t TA4, TA2, TA3, TA1, Tp4, Tp1, Tdi;
+    size_t XA4, XA2, XA3, XA1, Xp4, Xp1, Xdi;
+    size_t PA4, PA2, PA3, PA1, Pp4, Pp1, Pdi;
+    size_t total;
+    __host__ __device__ MpLayout(int N)
+    {
+        size_t o = 0;
+        auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
+        const int L = 2 * N - 2, nb2 = N - 2, nbp = N - 2; // psi: m = 0 has the larger nb
+        vec = take(sizeof(cd) * MPHASE_NVEC * N);
+        dtmp = take(sizeof(double) * 2 * N);
+        cosT = take(sizeof(double) * L);
+        sinT = take(sizeof(double) * L);
+        TA4 = take(sizeof(double) * PZ_T_LD * nb2); TA2 = take(sizeof(double) * 2 * nb2);
+        TA3 = take(sizeof(double) * 2 * nb2); TA1 = take(sizeof(double) * 4);
+        Tp4 = take(sizeof(int) * nb2); Tp1 = take(sizeof(int) * 2); Tdi = take(sizeof(double) * nb2);
+        XA4 = take(sizeof(double) * PZ_T_LD * nb2); XA2 = take(sizeof(double) * 2 * nb2);
+        XA3 = take(sizeof(double) * 2 * nb2); XA1 = take(sizeof(double) * 4);
+        Xp4 = take(sizeof(int) * nb2); Xp1 = take(sizeof(int) * 2); Xdi = take(sizeof(double) * nb2);
+        PA4 = take(sizeof(cd) * PZ_P_LD * nbp); PA2 = take(sizeof(cd) * PZ_P_NT * nbp);
+        PA3 = take(sizeof(cd) * PZ_P_NT * nbp); PA1 = take(sizeof(cd) * 16);
+        Pp4 = take(sizeof(int) * nbp); Pp1 = take(sizeof(int) * 4); Pdi = take(sizeof(cd) * nbp);
+        total = o;
+    }
+};
+
+__device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t bytes)
+{
+    if ((bytes & 7) == 0 && ((size_t)src & 7) == 0) {
+        const double *s = (const double *)src;
+        double *d = (double *)dst;
+        for (size_t i = threadIdx.x; i < bytes / 8; i += blockDim.x) d[i] = s[i];
+    } else {
+        const int *s = (const int *)src;
+        int *d = (int *)dst;
+        for (size_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
+    }
+}
+
 // One stage of the m-loop for one azimuthal wavenumber per workgroup:
 // finish_explicit_assembly (mloop.f90:169-229) when exp_calc, then
 // update_temp_int / update_xi_int (update_temp_integ.f90:179-298) and
@@ -88,14 +131,48 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
     const int tid = threadIdx.x, wave = tid >> 6;
-    cd *V = (cd *)smem;
+    const MpLayout lay(N);
+    char *lb = (char *)smem;
+    cd *V = (cd *)(lb + lay.vec);
     cd *aT = V, *cT = V + N, *eT = V + 2 * N;
     cd *aX = V + 3 * N, *cX = V + 4 * N, *eX = V + 5 * N;
     cd *aP = V + 6 * N, *cP = V + 7 * N, *eP = V + 8 * N;
     cd *psiv = V + 9 * N, *omv = V + 10 * N, *upv = V + 11 * N, *usv = V + 12 * N;
     cd *Tv = V + 13 * N, *Xv = V + 14 * N, *bv = V + 15 * N, *b2 = V + 16 * N, *wv = V + 17 * N;
-    double *dtmp = (double *)(V + MPHASE_NVEC * N);
+    double *dtmp = (double *)(lb + lay.dtmp);
+    double *cosT = (double *)(lb + lay.cosT), *sinT = (double *)(lb + lay.sinT);
     const bool hT = p.l_heat, hX = p.l_chem;
+    const int nb2 = N - 2;
+    const int ntP = m == 0 ? 2 : PZ_P_NT, klP = m == 0 ? 4 : PZ_P_KL, ldP = 3 * klP + 1, nbP = N - ntP;
+    // stage transform tables and this m's LU factors into LDS
+    lds_copy(cosT, p.cosT, sizeof(double) * p.L);
+    lds_copy(sinT, p.sinT, sizeof(double) * p.L);
+    if (hT) {
+        lds_copy(lb + lay.TA4, p.T_A4 + (size_t)mi * PZ_T_LD * nb2, sizeof(double) * PZ_T_LD * nb2);
+        lds_copy(lb + lay.TA2, p.T_A2 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
+        lds_copy(lb + lay.TA3, p.T_A3 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
+        lds_copy(lb + lay.TA1, p.T_A1 + (size_t)mi * 4, sizeof(double) * 4);
+        lds_copy(lb + lay.Tp4, p.T_p4 + (size_t)mi * nb2, sizeof(int) * nb2);
+        lds_copy(lb + lay.Tp1, p.T_p1 + (size_t)mi * 2, sizeof(int) * 2);
+        lds_copy(lb + lay.Tdi, p.T_dinv + (size_t)mi * nb2, sizeof(double) * nb2);
+    }
+    if (hX) {
+        lds_copy(lb + lay.XA4, p.X_A4 + (size_t)mi * PZ_T_LD * nb2, sizeof(double) * PZ_T_LD * nb2);
+        lds_copy(lb + lay.XA2, p.X_A2 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
+        lds_copy(lb + lay.XA3, p.X_A3 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
+        lds_copy(lb + lay.XA1, p.X_A1 + (size_t)mi * 4, sizeof(double) * 4);
+        lds_copy(lb + lay.Xp4, p.X_p4 + (size_t)mi * nb2, sizeof(int) * nb2);
+        lds_copy(lb + lay.Xp1, p.X_p1 + (size_t)mi * 2, sizeof(int) * 2);
+        lds_copy(lb + lay.Xdi, p.X_dinv + (size_t)mi * nb2, sizeof(double) * nb2);
+    }
+    lds_copy(lb + lay.PA4, p.P_A4 + (size_t)mi * PZ_P_LD * N, sizeof(cd) * ldP * nbP);
+    lds_copy(lb + lay.PA2, p.P_A2 + (size_t)mi * PZ_P_NT * N, sizeof(cd) * ntP * nbP);
+    lds_copy(lb + lay.PA3, p.P_A3 + (size_t)mi * PZ_P_NT * N, sizeof(cd) * ntP * nbP);
+    lds_copy(lb + lay.PA1, p.P_A1 + (size_t)mi * 16, sizeof(cd) * ntP * ntP);
+    lds_copy(lb + lay.Pp4, p.P_p4 + (size_t)mi * N, sizeof(int) * nbP);
+    lds_copy(lb + lay.Pp1, p.P_p1 + (size_t)mi * 4, sizeof(int) * ntP);
+    lds_copy(lb + lay.Pdi, p.P_dinv + (size_t)mi * N, sizeof(cd) * nbP);
+    __syncthreads();
 
     // ---------------- explicit assembly -----------------------------------
     if (exp_calc) {
@@ -117,7 +194,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         if (hT) { din[nf] = aT; dout[nf++] = cT; }
         if (hX) { din[nf] = aX; dout[nf++] = cX; }
         din[nf] = aP; dout[nf++] = cP;
-        dct_many(p, din, dout, nf, 1);
+        dct_many(p, din, dout, nf, 1, cosT);
         __syncthreads();
         // derivative of the advective flux (into the a-vectors)
         {
@@ -127,7 +204,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
             if (hT) { c3[k3] = cT; d3[k3++] = aT; }
             if (hX) { c3[k3] = cX; d3[k3++] = aX; }
             c3[k3] = cP; d3[k3++] = aP;
-            dr_many(p, c3, d3, nullptr, k3);
+            dr_many(p, c3, d3, nullptr, k3, cosT, sinT);
         }
         __syncthreads();
         const double rc2 = p.r_cmb * p.r_cmb;
@@ -173,7 +250,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         if (hT) { din[nf] = eT; dout[nf++] = cT; }
         if (hX) { din[nf] = eX; dout[nf++] = cX; }
         din[nf] = eP; dout[nf++] = cP;
-        dct_many(p, din, dout, nf, 1);
+        dct_many(p, din, dout, nf, 1, cosT);
         __syncthreads();
         if (hT) bandmv(p, p.T_RHSE1, 3, cT, eT, 2);
         if (hX) bandmv(p, p.T_RHSE1, 3, cX, eX, 2);
@@ -188,7 +265,6 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     }
 
     // ---------------- T and xi: rhs, bordered solve ------------------------
-    const int nb2 = N - PZ_T_NT;
     for (int k = tid; k < N; k += blockDim.x) {
         if (hT) {
             cd r = imex_rhs(p, ss, istage, p.T_old, p.T_impl, p.T_expl, exp_calc ? eT : nullptr, off, k);
@@ -205,13 +281,15 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     }
     __syncthreads();
     if (wave == 0 && hT)
-        wave_bordered_solve<double>(N, PZ_T_NT, PZ_T_KL, PZ_T_KU, PZ_T_LD, p.T_A1 + mi * 4, p.T_p1 + mi * 2,
-                                    p.T_A2 + (size_t)mi * 2 * nb2, p.T_A3 + (size_t)mi * 2 * nb2,
-                                    p.T_A4 + (size_t)mi * PZ_T_LD * nb2, p.T_p4 + (size_t)mi * nb2, aT);
+        wave_bordered_fast<double, PZ_T_KL, PZ_T_KU>(N, PZ_T_NT, (double *)(lb + lay.TA1), (int *)(lb + lay.Tp1),
+                                                     (double *)(lb + lay.TA2), (double *)(lb + lay.TA3),
+                                                     (double *)(lb + lay.TA4), (int *)(lb + lay.Tp4),
+                                                     (double *)(lb + lay.Tdi), aT);
     if (wave == 1 && hX)
-        wave_bordered_solve<double>(N, PZ_T_NT, PZ_T_KL, PZ_T_KU, PZ_T_LD, p.X_A1 + mi * 4, p.X_p1 + mi * 2,
-                                    p.X_A2 + (size_t)mi * 2 * nb2, p.X_A3 + (size_t)mi * 2 * nb2,
-                                    p.X_A4 + (size_t)mi * PZ_T_LD * nb2, p.X_p4 + (size_t)mi * nb2, aX);
+        wave_bordered_fast<double, PZ_T_KL, PZ_T_KU>(N, PZ_T_NT, (double *)(lb + lay.XA1), (int *)(lb + lay.Xp1),
+                                                     (double *)(lb + lay.XA2), (double *)(lb + lay.XA3),
+                                                     (double *)(lb + lay.XA4), (int *)(lb + lay.Xp4),
+                                                     (double *)(lb + lay.Xdi), aX);
     __syncthreads();
     {
         const cd *din[2];
@@ -219,7 +297,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         int nf = 0;
         if (hT) { din[nf] = aT; dout[nf++] = Tv; }
         if (hX) { din[nf] = aX; dout[nf++] = Xv; }
-        if (nf) dct_many(p, din, dout, nf, 0);
+        if (nf) dct_many(p, din, dout, nf, 0, cosT);
     }
     __syncthreads();
     const int inext = (istage == p.nstages) ? 1 : istage + 1;
@@ -254,7 +332,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         __syncthreads();
         const cd *din[1] = {bv};
         cd *dout[1] = {wv};
-        dct_many(p, din, dout, 1, 0);
+        dct_many(p, din, dout, 1, 0, cosT);
         __syncthreads();
         if (m > 0) bandmv(p, p.P_RHSE2, 8, wv, b2, 4);
         __syncthreads();
@@ -266,12 +344,12 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         aP[k] = cscale(p.P_fac[off + k], r);
     }
     __syncthreads();
-    {
-        const int nt = m == 0 ? 2 : PZ_P_NT, kl = m == 0 ? 4 : PZ_P_KL, ld = 2 * kl + kl + 1;
-        if (wave == 0)
-            wave_bordered_solve<cd>(N, nt, kl, kl, ld, p.P_A1 + mi * 16, p.P_p1 + mi * 4, p.P_A2 + (size_t)mi * PZ_P_NT * N,
-                                    p.P_A3 + (size_t)mi * PZ_P_NT * N, p.P_A4 + (size_t)mi * PZ_P_LD * N,
-                                    p.P_p4 + (size_t)mi * N, aP);
+    if (wave == 0) {
+        cd *A1 = (cd *)(lb + lay.PA1), *A2 = (cd *)(lb + lay.PA2), *A3 = (cd *)(lb + lay.PA3);
+        cd *A4 = (cd *)(lb + lay.PA4), *di = (cd *)(lb + lay.Pdi);
+        int *p1 = (int *)(lb + lay.Pp1), *p4 = (int *)(lb + lay.Pp4);
+        if (m == 0) wave_bordered_fast<cd, 4, 4>(N, ntP, A1, p1, A2, A3, A4, p4, di, aP);
+        else wave_bordered_fast<cd, PZ_P_KL, PZ_P_KU>(N, ntP, A1, p1, A2, A3, A4, p4, di, aP);
     }
     __syncthreads();
     const double rc2 = p.r_cmb * p.r_cmb;
@@ -285,7 +363,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         __syncthreads();
         const cd *din[2] = {cP, eP};
         cd *dout[2] = {omv, upv};
-        dct_many(p, din, dout, 2, 0);
+        dct_many(p, din, dout, 2, 0, cosT);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             double u0 = upv[k].x, om0 = p.drx[k] * omv[k].x;
@@ -298,16 +376,16 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         // work_r = costf(real(up(m=0))) for get_psi_rhs_imp_int_smat
         const cd *di[1] = {eP};
         cd *dd[1] = {wv};
-        dct_many(p, di, dd, 1, 0);
+        dct_many(p, di, dd, 1, 0, cosT);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) wv[k] = mk(wv[k].x, 0.0);
     } else {
         const cd *c1[1] = {aP};
         cd *d1[1] = {cP}, *d2[1] = {eP};
-        dr_many(p, c1, d1, d2, 1);
+        dr_many(p, c1, d1, d2, 1, cosT, sinT);
         const cd *din[1] = {aP};
         cd *dout[1] = {psiv};
-        dct_many(p, din, dout, 1, 0);
+        dct_many(p, din, dout, 1, 0, cosT);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             const double rk = p.r[k], h2 = rc2 - rk * rk, or1 = p.or1[k];
