@@ -109,7 +109,8 @@ struct pz_handle {
     hipGraphExec_t gexec = nullptr;
     bool started = false;
     bool use_graph = true;
-    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0;
+    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0;
+    int inv_waves = 16;
     std::string err;
 
     template <class T> T *dalloc(size_t n)
@@ -406,6 +407,7 @@ static void setup(pz_handle *h)
     // Chebyshev constants (chebyshev.f90:55-60, dct_fftw.f90:124)
     h->rnorm = d.rnorm = std::sqrt(2.0 / (double)(N - 1));
     h->cheb_fac = d.cheb_fac = std::sqrt(0.5 / (double)(N - 1));
+    d.icheb_fac = 1.0 / d.cheb_fac;
     d.boundary_fac = 0.5;
     d.thr = c.l_rerror_fix ? c.rerror_fac * 2.220446049250313e-16 : 0.0;
     // grid (chebyshev.f90:188-240; radial.f90:115-282)
@@ -481,6 +483,20 @@ static void setup(pz_handle *h)
     }
     d.cosT = h->upload(cosT);
     d.sinT = h->upload(sinT);
+    {
+        // bulk factors of get_dr_fft / get_ddr_fft at the interior nodes:
+        // 1/(sqrt(1-x^2) nrm), x/((1-x^2)^1.5 nrm), 1/((1-x^2) nrm)
+        std::vector<double> drb(3 * N, 0.0);
+        const ld nrm = (ld)(2 * N - 2);
+        for (int j = 1; j < N - 1; j++) {
+            const double xj = x[j];
+            const ld sj = (ld)(1.0 - xj * xj);
+            drb[j] = (double)(1.0L / (sqrtl(sj) * nrm));
+            drb[N + j] = (double)((ld)xj / (powl(sj, 1.5L) * nrm));
+            drb[2 * N + j] = (double)(1.0L / (sj * nrm));
+        }
+        d.drb = h->upload(drb);
+    }
     std::vector<double> tau(6 * N);
     for (int n = 0; n < N; n++) {
         // chebyshev.f90:362-381
@@ -504,6 +520,11 @@ static void setup(pz_handle *h)
         throw std::invalid_argument("n_phi has a prime factor above 64");
     d.nrad = (int)rad.size();
     for (size_t i = 0; i < rad.size(); i++) d.radices[i] = rad[i];
+    // Chebyshev transforms by FFTs of length L unless L has a prime factor
+    // above 64 (then the direct cosine sums are used)
+    std::vector<int> radL = factor_radices(2 * N - 2);
+    d.nradL = (radL.size() <= PZ_MAX_RADICES && *std::max_element(radL.begin(), radL.end()) <= 64) ? (int)radL.size() : 0;
+    for (int i = 0; i < d.nradL; i++) d.radicesL[i] = radL[i];
     d.idx2m = h->upload(h->idx2m);
     std::vector<double> ones(h->nm, 1.0); // no hyperdiffusion (horizontal.f90:126-131)
     d.hdifT = h->upload(ones);
@@ -540,6 +561,10 @@ static void setup(pz_handle *h)
     d.P_A3 = h->dalloc<cd>((size_t)nm * PZ_P_NT * N); d.P_A4 = h->dalloc<cd>((size_t)nm * PZ_P_LD * N);
     d.P_p1 = h->dalloc<int>(nm * 4); d.P_p4 = h->dalloc<int>((size_t)nm * N); d.P_fac = h->dalloc<double>(F);
     d.info = h->dalloc<int>(3 * nm);
+    d.ldi = (N + 1) & ~1;
+    d.T_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
+    d.X_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
+    d.P_inv = h->dalloc<cd>((size_t)nm * N * d.ldi);
     d.ss = h->dalloc<PzScheme>(1);
     build_operators(h);
     // LDS budgets
@@ -557,13 +582,17 @@ static void setup(pz_handle *h)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     size_t lim = prop.sharedMemPerBlock;
-    size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start});
+    // waves of k_build_inv: as many right-hand sides as fit next to the factors
+    while (h->inv_waves > 1 && InvLayout(N, h->inv_waves).total > lim) h->inv_waves--;
+    h->lds_inv = InvLayout(N, h->inv_waves).total;
+    size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
     if (need > 65536) {
         if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
         HIPCHK(hipFuncSetAttribute((const void *)k_build_lhs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_build));
         HIPCHK(hipFuncSetAttribute((const void *)k_mphase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_mphase));
         HIPCHK(hipFuncSetAttribute((const void *)k_radial_loop, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_radial));
         HIPCHK(hipFuncSetAttribute((const void *)k_start, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_start));
+        HIPCHK(hipFuncSetAttribute((const void *)k_build_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_inv));
     }
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
 }
@@ -851,6 +880,7 @@ static void launch_step(pz_handle *h)
         if (stage == 1) {
             hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, h->d);
             hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, h->d, 0);
+            hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream, h->d, 0);
         }
         hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(256), h->lds_mphase, h->stream, h->d, stage, exp_calc);
     }
@@ -971,8 +1001,8 @@ static int vec_op(pz_handle *h, const double *f, double *df, double *ddf, int n_
         HIPCHK(hipMalloc(&b, S * sizeof(cd)));
         HIPCHK(hipMalloc(&c, S * sizeof(cd)));
         HIPCHK(hipMemcpy(a, v.data(), S * sizeof(cd), hipMemcpyHostToDevice));
-        if (op == 0) hipLaunchKernelGGL(k_costf_vecs, dim3(n_m), dim3(256), 2 * N * sizeof(cd), h->stream, h->d, a);
-        else hipLaunchKernelGGL(k_dr_vecs, dim3(n_m), dim3(256), 4 * N * sizeof(cd), h->stream, h->d, a, b,
+        if (op == 0) hipLaunchKernelGGL(k_costf_vecs, dim3(n_m), dim3(256), (2 * N + 2 * (size_t)h->d.L) * sizeof(cd), h->stream, h->d, a);
+        else hipLaunchKernelGGL(k_dr_vecs, dim3(n_m), dim3(256), (4 * N + 4 * (size_t)h->d.L + 16) * sizeof(cd), h->stream, h->d, a, b,
                                 op == 2 ? c : nullptr, l_dct_in);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));

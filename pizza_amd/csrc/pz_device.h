@@ -65,6 +65,44 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Element access through a generic pointer known to address LDS, retyped so
+// that it compiles to ds_read/ds_write instead of flat loads and stores.
+typedef __attribute__((address_space(3))) double lds_double;
+__device__ __forceinline__ cd lds_load(const cd *q, int i)
+{
+    const lds_double *d = (const lds_double *)q;
+    return mk(d[2 * i], d[2 * i + 1]);
+}
+__device__ __forceinline__ void lds_store(cd *q, int i, cd v)
+{
+    lds_double *d = (lds_double *)q;
+    d[2 * i] = v.x;
+    d[2 * i + 1] = v.y;
+}
+
+// Up to four vector pointers passed by value; indexing by selects keeps
+// them in registers (a private array indexed at run time would live in
+// scratch memory and turn LDS accesses into flat ones).
+template <class T>
+struct VecSet {
+    T *v0 = nullptr, *v1 = nullptr, *v2 = nullptr, *v3 = nullptr;
+    int n = 0;
+    __device__ VecSet() {}
+    // unused slots repeat an earlier pointer, so every slot stays in the
+    // address space of the vectors (no null among the selected pointers)
+    __device__ explicit VecSet(T *a, T *b = nullptr, T *c = nullptr, T *d = nullptr)
+        : v0(a), v1(b ? b : a), v2(c ? c : a), v3(d ? d : a), n(d ? 4 : c ? 3 : b ? 2 : a ? 1 : 0) {}
+    __device__ __forceinline__ void push(T *q)
+    {
+        v0 = n == 0 ? q : v0;
+        v1 = n <= 1 ? q : v1;
+        v2 = n <= 2 ? q : v2;
+        v3 = n <= 3 ? q : v3;
+        n++;
+    }
+    __device__ __forceinline__ T *operator[](int f) const { return f == 0 ? v0 : f == 1 ? v1 : f == 2 ? v2 : v3; }
+};
+
 // ---------------------------------------------------------------------------
 // Stockham mixed-radix FFT in LDS for ntr transforms of length n
 // (dir = -1 forward, +1 backward, unnormalised).  Result pointer returned.
@@ -75,11 +113,12 @@ __device__ __forceinline__ cd twd(const PzDev &p, int k, int dir)
     return dir < 0 ? w : mk(w.x, -w.y);
 }
 
-__device__ cd *fft_lds(const PzDev &p, cd *a, cd *b, int n, int ntr, int dir)
+template <class TW>
+__device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *radices, int nrad, TW tw)
 {
     int Ns = 1;
-    for (int s = 0; s < p.nrad; s++) {
-        const int R = p.radices[s];
+    for (int s = 0; s < nrad; s++) {
+        const int R = radices[s];
         const int nb = n / R;
         const int tot = nb * ntr;
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
@@ -90,14 +129,14 @@ __device__ cd *fft_lds(const PzDev &p, cd *a, cd *b, int n, int ntr, int dir)
             int tstep = (n / (Ns * R)) * jm;
             int idxD = (j / Ns) * Ns * R + jm;
             if (R == 2) {
-                cd v0 = x[j], v1 = cmul(x[j + nb], twd(p, tstep, dir));
+                cd v0 = x[j], v1 = cmul(x[j + nb], tw(tstep));
                 y[idxD] = cadd(v0, v1);
                 y[idxD + Ns] = csub(v0, v1);
             } else if (R == 4) {
                 cd v0 = x[j];
-                cd v1 = cmul(x[j + nb], twd(p, tstep, dir));
-                cd v2 = cmul(x[j + 2 * nb], twd(p, 2 * tstep, dir));
-                cd v3 = cmul(x[j + 3 * nb], twd(p, 3 * tstep, dir));
+                cd v1 = cmul(x[j + nb], tw(tstep));
+                cd v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
+                cd v3 = cmul(x[j + 3 * nb], tw(3 * tstep));
                 cd a0 = cadd(v0, v2), b0 = csub(v0, v2), c0 = cadd(v1, v3);
                 cd d0 = cimul((double)dir, csub(v1, v3));
                 y[idxD] = cadd(a0, c0);
@@ -107,8 +146,8 @@ __device__ cd *fft_lds(const PzDev &p, cd *a, cd *b, int n, int ntr, int dir)
             } else if (R == 3) {
                 const double s3 = 0.86602540378443864676;
                 cd v0 = x[j];
-                cd v1 = cmul(x[j + nb], twd(p, tstep, dir));
-                cd v2 = cmul(x[j + 2 * nb], twd(p, 2 * tstep, dir));
+                cd v1 = cmul(x[j + nb], tw(tstep));
+                cd v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
                 cd t1 = cadd(v1, v2);
                 cd t2 = cimul((double)dir * s3, csub(v1, v2));
                 cd m0 = mk(v0.x - 0.5 * t1.x, v0.y - 0.5 * t1.y);
@@ -121,8 +160,8 @@ __device__ cd *fft_lds(const PzDev &p, cd *a, cd *b, int n, int ntr, int dir)
                 for (int k = 0; k < R; k++) {
                     cd acc = mk(0.0, 0.0);
                     for (int r = 0; r < R; r++) {
-                        cd v = cmul(x[j + r * nb], twd(p, r * tstep, dir));
-                        acc = cadd(acc, cmul(v, twd(p, ((r * k) % R) * q, dir)));
+                        cd v = cmul(x[j + r * nb], tw(r * tstep));
+                        acc = cadd(acc, cmul(v, tw(((r * k) % R) * q)));
                     }
                     y[idxD + k * Ns] = acc;
                 }
@@ -135,6 +174,21 @@ __device__ cd *fft_lds(const PzDev &p, cd *a, cd *b, int n, int ntr, int dir)
         Ns *= R;
     }
     return a;
+}
+
+// azimuthal FFT of length n_phi (twiddles exp(-2 pi i k / n_phi) in p.twid)
+__device__ cd *fft_lds(const PzDev &p, cd *a, cd *b, int n, int ntr, int dir)
+{
+    return fft_stockham(a, b, n, ntr, dir, p.radices, p.nrad, [&](int k) { return twd(p, k, dir); });
+}
+
+// FFT of length L = 2 n_r - 2 with twiddles exp(dir 2 pi i k / L) =
+// (cosT[k], dir sinT[k]) read from the Chebyshev tables
+__device__ cd *fft_cheb(const PzDev &p, cd *a, cd *b, int ntr, int dir, const double *cosT, const double *sinT)
+{
+    const double sd = (double)dir;
+    return fft_stockham(a, b, p.L, ntr, dir, p.radicesL, p.nradL,
+                        [&](int k) { return mk(cosT[k], sd * sinT[k]); });
 }
 
 __device__ __forceinline__ double cc22real(cd a, cd b, int m)
@@ -553,7 +607,7 @@ __device__ void wave_bordered_solve(int N, int nt, int kl, int ku, int ld, const
 // Chebyshev transforms on LDS vectors, whole workgroup (dct_fftw.f90)
 // ---------------------------------------------------------------------------
 // costf (DCT-I with cheb_fac) of nf vectors: out[f][k]
-__device__ void dct_many(const PzDev &p, const cd *const *in, cd *const *out, int nf, int dealias,
+__device__ void dct_many(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int nf, int dealias,
                          const double *cosT = nullptr)
 {
     const int N = p.N, L = p.L;
@@ -579,76 +633,207 @@ __device__ void dct_many(const PzDev &p, const cd *const *in, cd *const *out, in
     }
 }
 
-// first (and optionally second) radial derivative from Chebyshev
-// coefficients (get_dr_fft/get_ddr_fft with l_dct_in = .false.) followed
-// by the drx/ddrx map; nf vectors
-__device__ void dr_many(const PzDev &p, const cd *const *c, cd *const *df, cd *const *ddf, int nf,
+// Radial derivatives from Chebyshev coefficients w (get_dr_fft /
+// get_ddr_fft with l_dct_in = .false., dct_fftw.f90), split in three parts:
+// the boundary points x = +-1 (closed k^2 sums), the bulk points from the
+// sums V = sum_{k=1}^{N-2} k w_k sin(pi k j/(N-1)) and U = sum k^2 w_k
+// cos(pi k j/(N-1)), and the map to r by drx/ddrx.
+__device__ __forceinline__ void dr_boundary(const PzDev &p, const cd *w, int j, bool second, cd &d1, cd &d2)
+{
+    const int N = p.N;
+    const double cf = p.cheb_fac;
+    double ax = 0, ay = 0, bx = 0, by = 0;
+    for (int k = 1; k <= N - 1; k++) {
+        double wx = w[k].x, wy = w[k].y;
+        if (k == N - 1) { wx *= 0.5; wy *= 0.5; }
+        const double s1 = (j == 0) ? 1.0 : (((k + 1) & 1) ? -1.0 : 1.0);
+        const double k2 = (double)k * k;
+        ax += s1 * k2 * wx;
+        ay += s1 * k2 * wy;
+        if (second && k >= 2) {
+            const double s2 = (j == 0) ? 1.0 : ((k & 1) ? -1.0 : 1.0);
+            const double kk = k2 * (k2 - 1.0);
+            bx += s2 * kk * wx;
+            by += s2 * kk * wy;
+        }
+    }
+    // the coefficient vector enters divided by cheb_fac
+    ax /= cf; ay /= cf; bx /= cf; by /= cf;
+    d1 = mk(ax / (double)(N - 1), ay / (double)(N - 1));
+    d2 = second ? mk((1.0 / 3.0) * bx / (double)(N - 1), (1.0 / 3.0) * by / (double)(N - 1)) : mk(0, 0);
+}
+
+// bulk point j from V = vx + i vy and U = ux + i uy (sums over the raw
+// coefficients); drb holds 1/(sqrt(1-x^2) nrm), x/((1-x^2)^1.5 nrm) and
+// 1/((1-x^2) nrm) with nrm = 2 N - 2
+__device__ __forceinline__ void dr_bulk(const PzDev &p, cd wl, int j, bool second, double vx, double vy,
+                                        double ux, double uy, cd &d1, cd &d2)
+{
+    const int N = p.N;
+    const double icf = p.icheb_fac;
+    vx *= icf; vy *= icf; ux *= icf; uy *= icf;
+    const double V1x = -2.0 * vx, V1y = -2.0 * vy;
+    const double g1 = p.drb[j], g2 = p.drb[N + j], g3 = p.drb[2 * N + j];
+    d1 = mk(-V1x * g1, -V1y * g1);
+    d2 = mk(0, 0);
+    if (second) {
+        const double wlx = wl.x * icf, wly = wl.y * icf;
+        const double sgn = (j & 1) ? -1.0 : 1.0;
+        const double n1 = (double)(N - 1) * (double)(N - 1);
+        const double W2x = -(2.0 * ux + n1 * wlx * sgn), W2y = -(2.0 * uy + n1 * wly * sgn);
+        d2 = mk(-V1x * g2 + W2x * g3, -V1y * g2 + W2y * g3);
+    }
+}
+
+__device__ __forceinline__ void dr_store(const PzDev &p, int j, cd d1, cd d2, bool second, cd *df, cd *ddf)
+{
+    const double a = p.drx[j];
+    if (second) {
+        const double b = p.ddrx[j];
+        lds_store(ddf, j, mk(b * d1.x + a * a * d2.x, b * d1.y + a * a * d2.y));
+    }
+    lds_store(df, j, mk(a * d1.x, a * d1.y));
+}
+
+// first (and optionally second) derivative of nf vectors, direct sums
+__device__ void dr_many(const PzDev &p, VecSet<const cd> c, VecSet<cd> df, VecSet<cd> ddf, int nf,
                         const double *cosT = nullptr, const double *sinT = nullptr)
 {
     const int N = p.N, L = p.L;
     if (!cosT) cosT = p.cosT;
     if (!sinT) sinT = p.sinT;
-    const double cf = p.cheb_fac;
     for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
-        int f = t / N, j = t - f * N;
+        const int f = t / N, j = t - f * N;
         const cd *w = c[f];
-        cd d1, d2 = mk(0, 0);
+        cd d1, d2;
         if (j == 0 || j == N - 1) {
-            double sgn0 = 1.0;
-            double ax = 0, ay = 0, bx = 0, by = 0;
-            for (int k = 1; k <= N - 1; k++) {
-                double wx = w[k].x / cf, wy = w[k].y / cf;
-                if (k == N - 1) { wx *= 0.5; wy *= 0.5; }
-                double s1 = (j == 0) ? 1.0 : (((k + 1) & 1) ? -1.0 : 1.0);
-                double k2 = (double)k * k;
-                ax += s1 * k2 * wx;
-                ay += s1 * k2 * wy;
-                if (ddf && k >= 2) {
-                    double s2 = (j == 0) ? 1.0 : ((k & 1) ? -1.0 : 1.0);
-                    double kk = k2 * (k2 - 1.0);
-                    bx += s2 * kk * wx;
-                    by += s2 * kk * wy;
-                }
-            }
-            (void)sgn0;
-            d1 = mk(ax / (double)(N - 1), ay / (double)(N - 1));
-            if (ddf) d2 = mk((1.0 / 3.0) * bx / (double)(N - 1), (1.0 / 3.0) * by / (double)(N - 1));
+            dr_boundary(p, w, j, ddf.v0 != nullptr, d1, d2);
         } else {
-            // V1 = sum_{k=1}^{N-2} i k w_k (e^{i pi kj/(N-1)} - e^{-i...}) = -2 sum k w_k sin
             double vx = 0, vy = 0, ux = 0, uy = 0;
             int q = j;
 #pragma unroll 4
             for (int k = 1; k < N - 1; k++, q += j) {
                 if (q >= L) q -= L;
-                double wx = w[k].x / cf, wy = w[k].y / cf;
-                double s = sinT[q], co = cosT[q];
+                const double wx = w[k].x, wy = w[k].y;
+                const double s = sinT[q], co = cosT[q];
                 vx += (double)k * wx * s;
                 vy += (double)k * wy * s;
-                if (ddf) {
-                    double k2 = (double)k * k;
+                if (ddf.v0) {
+                    const double k2 = (double)k * k;
                     ux += k2 * wx * co;
                     uy += k2 * wy * co;
                 }
             }
-            double V1x = -2.0 * vx, V1y = -2.0 * vy;
-            double xj = p.x[j], sj = 1.0 - xj * xj;
-            double nrm = (double)(2 * N - 2);
-            d1 = mk(-V1x / sqrt(sj) / nrm, -V1y / sqrt(sj) / nrm);
-            if (ddf) {
-                double wlx = w[N - 1].x / cf, wly = w[N - 1].y / cf;
-                double sgn = (j & 1) ? -1.0 : 1.0;
-                double n1 = (double)(N - 1) * (double)(N - 1);
-                double W2x = -(2.0 * ux + n1 * wlx * sgn), W2y = -(2.0 * uy + n1 * wly * sgn);
-                double p15 = pow(sj, 1.5);
-                d2 = mk(-V1x * xj / p15 / nrm + W2x / sj / nrm, -V1y * xj / p15 / nrm + W2y / sj / nrm);
+            dr_bulk(p, w[N - 1], j, ddf.v0 != nullptr, vx, vy, ux, uy, d1, d2);
+        }
+        dr_store(p, j, d1, d2, ddf.v0 != nullptr, df[f], ddf.v0 ? ddf[f] : df[f]);
+    }
+}
+
+// costf of nf <= 3 vectors through one FFT of length L = 2 N - 2 of the
+// even extension y_j = x_j, y_{L-j} = x_j: Y_k = x_0 + (-1)^k x_{N-1}
+// + 2 sum_{j=1}^{N-2} x_j cos(pi j k/(N-1)), the DCT-I of dct_fftw.f90.
+// buf: LDS scratch of 2 nf L complex values; no trailing barrier.
+__device__ void dct_fft(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int nf, int dealias,
+                        const double *cosT, const double *sinT, cd *buf)
+{
+    if (p.nradL == 0) {
+        dct_many(p, in, out, nf, dealias, cosT);
+        return;
+    }
+    const int N = p.N, L = p.L;
+    cd *A = buf, *B = buf + (size_t)nf * L;
+    for (int t = threadIdx.x; t < nf * L; t += blockDim.x) {
+        const int f = t / L, j = t - f * L;
+        A[t] = lds_load(in[f], j < N ? j : L - j);
+    }
+    __syncthreads();
+    const cd *Z = fft_cheb(p, A, B, nf, -1, cosT, sinT);
+    const double cf = p.cheb_fac;
+    for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
+        const int f = t / N, k = t - f * N;
+        const cd z = Z[(size_t)f * L + k];
+        lds_store(out[f], k, (dealias && k >= p.ncheb) ? mk(0.0, 0.0) : mk(cf * z.x, cf * z.y));
+    }
+}
+
+// derivatives of nf <= 3 vectors through FFTs of length L: the odd
+// extension of k w_k gives -2i V, the even extension of k^2 w_k (zero at
+// k = 0, N-1) gives 2U.  The boundary points x = +-1 are k^2-weighted sums,
+// one half-wave reduction per (vector, side).  buf: LDS scratch of
+// 2 ntr L + 4 nf complex values (ntr = nf, or 2 nf with the second
+// derivative); no trailing barrier.
+__device__ void dr_fft(const PzDev &p, VecSet<const cd> c, VecSet<cd> df, VecSet<cd> ddf, int nf,
+                       const double *cosT, const double *sinT, cd *buf)
+{
+    if (p.nradL == 0) {
+        dr_many(p, c, df, ddf, nf, cosT, sinT);
+        return;
+    }
+    const bool second = ddf.v0 != nullptr;
+    const int N = p.N, L = p.L, ntr = second ? 2 * nf : nf;
+    cd *A = buf, *B = buf + (size_t)ntr * L, *bnd = buf + (size_t)2 * ntr * L;
+    for (int t = threadIdx.x; t < ntr * L; t += blockDim.x) {
+        const int tr = t / L, j = t - tr * L;
+        const int f = tr < nf ? tr : tr - nf;
+        const int k = j < N ? j : L - j;
+        cd y = mk(0.0, 0.0);
+        if (k != 0 && k != N - 1) {
+            const cd w = lds_load(c[f], k);
+            const double sc = tr < nf ? (j < N ? (double)k : -(double)k) : (double)k * (double)k;
+            y = mk(sc * w.x, sc * w.y);
+        }
+        A[t] = y;
+    }
+    {
+        const int seg = threadIdx.x >> 5, hl = threadIdx.x & 31;
+        if (seg < 2 * nf) {
+            const int f = seg >> 1, side = seg & 1; // side 0: x = 1 (j = 0), 1: x = -1 (j = N-1)
+            const cd *w = c[f];
+            double ax = 0, ay = 0, bx = 0, by = 0;
+            for (int k = 1 + hl; k <= N - 1; k += 32) {
+                const cd wk = lds_load(w, k);
+                double wx = wk.x, wy = wk.y;
+                if (k == N - 1) { wx *= 0.5; wy *= 0.5; }
+                const double k2 = (double)k * k;
+                const double s1 = side == 0 ? 1.0 : ((k & 1) ? 1.0 : -1.0);
+                ax += s1 * k2 * wx;
+                ay += s1 * k2 * wy;
+                if (second && k >= 2) {
+                    const double s2 = side == 0 ? 1.0 : ((k & 1) ? -1.0 : 1.0);
+                    const double kk = k2 * (k2 - 1.0);
+                    bx += s2 * kk * wx;
+                    by += s2 * kk * wy;
+                }
+            }
+            for (int o = 16; o > 0; o >>= 1) {
+                ax += __shfl_xor(ax, o, 32);
+                ay += __shfl_xor(ay, o, 32);
+                bx += __shfl_xor(bx, o, 32);
+                by += __shfl_xor(by, o, 32);
+            }
+            if (hl == 0) {
+                const double sc = p.icheb_fac / (double)(N - 1);
+                bnd[2 * seg] = mk(ax * sc, ay * sc);
+                bnd[2 * seg + 1] = mk((1.0 / 3.0) * bx * sc, (1.0 / 3.0) * by * sc);
             }
         }
-        double a = p.drx[j];
-        if (ddf) {
-            double b = p.ddrx[j];
-            ddf[f][j] = mk(b * d1.x + a * a * d2.x, b * d1.y + a * a * d2.y);
+    }
+    __syncthreads();
+    const cd *Z = fft_cheb(p, A, B, ntr, -1, cosT, sinT);
+    for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
+        const int f = t / N, j = t - f * N;
+        cd d1, d2;
+        if (j == 0 || j == N - 1) {
+            const int seg = 2 * f + (j == 0 ? 0 : 1);
+            d1 = bnd[2 * seg];
+            d2 = second ? bnd[2 * seg + 1] : mk(0.0, 0.0);
+        } else {
+            const cd zv = Z[(size_t)f * L + j];
+            const cd zu = second ? Z[(size_t)(nf + f) * L + j] : mk(0.0, 0.0);
+            dr_bulk(p, lds_load(c[f], N - 1), j, second, -0.5 * zv.y, 0.5 * zv.x, 0.5 * zu.x, 0.5 * zu.y, d1, d2);
         }
-        df[f][j] = mk(a * d1.x, a * d1.y);
+        dr_store(p, j, d1, d2, second, df[f], second ? ddf[f] : df[f]);
     }
 }
 
@@ -979,5 +1164,183 @@ __global__ void __launch_bounds__(64) k_build_lhs(PzDev p, int force)
         else
             build_bordered<cd>(p, nt, kl, ku, K, S0, S1, S2, c1, c2, tauv, A1, A2, A3, A4, p1, p4, dinv, fac,
                                p.info + 2 * p.nm + mi, scr);
+    }
+}
+
+// global -> LDS copy by the whole block, four loads in flight per thread
+__device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t bytes)
+{
+    const size_t st = blockDim.x;
+    size_t i = threadIdx.x;
+    if ((bytes & 7) == 0 && ((size_t)src & 7) == 0) {
+        const double *s = (const double *)src;
+        double *d = (double *)dst;
+        const size_t n = bytes / 8;
+        for (; i + 3 * st < n; i += 4 * st) {
+            const double a = s[i], b = s[i + st], c = s[i + 2 * st], e = s[i + 3 * st];
+            d[i] = a; d[i + st] = b; d[i + 2 * st] = c; d[i + 3 * st] = e;
+        }
+        for (; i < n; i += st) d[i] = s[i];
+    } else {
+        const int *s = (const int *)src;
+        int *d = (int *)dst;
+        const size_t n = bytes / 4;
+        for (; i + 3 * st < n; i += 4 * st) {
+            const int a = s[i], b = s[i + st], c = s[i + 2 * st], e = s[i + 3 * st];
+            d[i] = a; d[i + st] = b; d[i + 2 * st] = c; d[i + 3 * st] = e;
+        }
+        for (; i < n; i += st) d[i] = s[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense solve by explicit inverse: y_i = sum_j INV[j][i] x_j for NMAT
+// matrices at once (column-major, leading dimension ldi, columns 16-byte
+// aligned).  E = double: a thread owns two consecutive rows (one 16-byte
+// load per column); E = cd: one row.  Threads split the columns into CG
+// groups, each thread keeps 16 column loads in flight, and the partial sums
+// are combined in a fixed order (bitwise reproducible).  x and y may alias;
+// every thread of the block must call it.  red: LDS scratch of
+// 512 * NMAT complex values.
+// ---------------------------------------------------------------------------
+template <class E, int NMAT>
+__device__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y, cd *red)
+{
+    constexpr int RPT = sizeof(E) == 8 ? 2 : 1, U = 16 / NMAT;
+    const int R = (N + RPT - 1) / RPT;
+    const int CG = min((int)blockDim.x / R, N);
+    const int t = threadIdx.x, cg = t / R, ru = t - cg * R;
+    cd acc[NMAT][RPT];
+#pragma unroll
+    for (int a = 0; a < NMAT; a++)
+#pragma unroll
+        for (int r = 0; r < RPT; r++) acc[a][r] = mk(0.0, 0.0);
+    if (cg < CG) {
+        const int nj = (N - cg + CG - 1) / CG;
+        for (int jb = 0; jb < nj; jb += U) {
+            double2 v[NMAT][U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int jj = cg + (jb + u) * CG;
+                const bool ok = jb + u < nj;
+#pragma unroll
+                for (int a = 0; a < NMAT; a++)
+                    v[a][u] = ok ? *(const double2 *)(inv[a] + (size_t)jj * ldi + (size_t)ru * RPT) : make_double2(0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int a = 0; a < NMAT; a++) {
+                    const int jj = min(cg + (jb + u) * CG, N - 1);
+                    const cd xj = x[a][jj];
+                    if constexpr (RPT == 2) {
+                        acc[a][0].x = fma(v[a][u].x, xj.x, acc[a][0].x);
+                        acc[a][0].y = fma(v[a][u].x, xj.y, acc[a][0].y);
+                        acc[a][1].x = fma(v[a][u].y, xj.x, acc[a][1].x);
+                        acc[a][1].y = fma(v[a][u].y, xj.y, acc[a][1].y);
+                    } else {
+                        acc[a][0] = cadd(acc[a][0], cmul(v[a][u], xj));
+                    }
+                }
+        }
+        if (cg > 0)
+#pragma unroll
+            for (int a = 0; a < NMAT; a++)
+#pragma unroll
+                for (int r = 0; r < RPT; r++) red[(size_t)a * 512 + (cg - 1) * R * RPT + ru * RPT + r] = acc[a][r];
+    }
+    __syncthreads();
+    if (cg == 0) {
+#pragma unroll
+        for (int a = 0; a < NMAT; a++)
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                cd s = acc[a][r];
+                for (int c = 1; c < CG; c++) s = cadd(s, red[(size_t)a * 512 + (c - 1) * R * RPT + ru * RPT + r]);
+                const int row = ru * RPT + r;
+                if (row < N) y[a][row] = s;
+            }
+    }
+    __syncthreads();
+}
+
+// LDS layout of k_build_inv: one field's bordered factors (sized for the
+// largest, psi with m != 0) and one right-hand side per wave.
+struct InvLayout {
+    size_t A4, A2, A3, A1, p4, p1, dinv, rhs, total;
+    __host__ __device__ InvLayout(int N, int nw)
+    {
+        size_t o = 0;
+        auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
+        A4 = take(sizeof(cd) * PZ_P_LD * N);
+        A2 = take(sizeof(cd) * PZ_P_NT * N);
+        A3 = take(sizeof(cd) * PZ_P_NT * N);
+        A1 = take(sizeof(cd) * 16);
+        p4 = take(sizeof(int) * N);
+        p1 = take(sizeof(int) * 4);
+        dinv = take(sizeof(cd) * N);
+        rhs = take(sizeof(cd) * (size_t)nw * N);
+        total = o;
+    }
+};
+
+// Columns of the inverses INV = A^{-1} diag(fac) of the bordered systems
+// factorised by k_build_lhs: wave w solves A y = fac_j e_j for the columns
+// j = w, w + nw, ... with the same bordered solve (algebra_lapack.f90:332-416)
+// as one time step would.  grid (nm, 3): field 0 = T, 1 = xi, 2 = psi.
+__global__ void __launch_bounds__(1024) k_build_inv(PzDev p, int force)
+{
+    extern __shared__ char lds[];
+    if (!force && p.ss->lMat == 0) return;
+    const int mi = blockIdx.x, fld = blockIdx.y, N = p.N, m = p.idx2m[mi];
+    if ((fld == 0 && !p.l_heat) || (fld == 1 && !p.l_chem)) return;
+    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63, ldi = p.ldi;
+    const InvLayout lay(N, nw);
+    cd *rhs = (cd *)(lds + lay.rhs) + (size_t)wave * N;
+    if (fld < 2) {
+        const int nt = PZ_T_NT, nb = N - nt;
+        const double *fac = (fld == 0 ? p.T_fac : p.X_fac) + (size_t)mi * N;
+        lds_copy(lds + lay.A4, (fld == 0 ? p.T_A4 : p.X_A4) + (size_t)mi * PZ_T_LD * nb, sizeof(double) * PZ_T_LD * nb);
+        lds_copy(lds + lay.A2, (fld == 0 ? p.T_A2 : p.X_A2) + (size_t)mi * nt * nb, sizeof(double) * nt * nb);
+        lds_copy(lds + lay.A3, (fld == 0 ? p.T_A3 : p.X_A3) + (size_t)mi * nt * nb, sizeof(double) * nt * nb);
+        lds_copy(lds + lay.A1, (fld == 0 ? p.T_A1 : p.X_A1) + (size_t)mi * nt * nt, sizeof(double) * nt * nt);
+        lds_copy(lds + lay.p4, (fld == 0 ? p.T_p4 : p.X_p4) + (size_t)mi * nb, sizeof(int) * nb);
+        lds_copy(lds + lay.p1, (fld == 0 ? p.T_p1 : p.X_p1) + (size_t)mi * nt, sizeof(int) * nt);
+        lds_copy(lds + lay.dinv, (fld == 0 ? p.T_dinv : p.X_dinv) + (size_t)mi * nb, sizeof(double) * nb);
+        __syncthreads();
+        double *inv = (fld == 0 ? p.T_inv : p.X_inv) + (size_t)mi * N * ldi;
+        for (int j = wave; j < N; j += nw) {
+            for (int k = lane; k < N; k += 64) rhs[k] = mk(k == j ? fac[j] : 0.0, 0.0);
+            __builtin_amdgcn_wave_barrier();
+            wave_bordered_fast<double, PZ_T_KL, PZ_T_KU>(N, nt, (double *)(lds + lay.A1), (int *)(lds + lay.p1),
+                                                         (double *)(lds + lay.A2), (double *)(lds + lay.A3),
+                                                         (double *)(lds + lay.A4), (int *)(lds + lay.p4),
+                                                         (double *)(lds + lay.dinv), rhs);
+            for (int k = lane; k < ldi; k += 64) inv[(size_t)j * ldi + k] = k < N ? rhs[k].x : 0.0;
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else {
+        const int nt = m == 0 ? 2 : PZ_P_NT, kl = m == 0 ? 4 : PZ_P_KL, ld = 3 * kl + 1, nb = N - nt;
+        const double *fac = p.P_fac + (size_t)mi * N;
+        lds_copy(lds + lay.A4, p.P_A4 + (size_t)mi * PZ_P_LD * N, sizeof(cd) * ld * nb);
+        lds_copy(lds + lay.A2, p.P_A2 + (size_t)mi * PZ_P_NT * N, sizeof(cd) * nt * nb);
+        lds_copy(lds + lay.A3, p.P_A3 + (size_t)mi * PZ_P_NT * N, sizeof(cd) * nt * nb);
+        lds_copy(lds + lay.A1, p.P_A1 + (size_t)mi * 16, sizeof(cd) * nt * nt);
+        lds_copy(lds + lay.p4, p.P_p4 + (size_t)mi * N, sizeof(int) * nb);
+        lds_copy(lds + lay.p1, p.P_p1 + (size_t)mi * 4, sizeof(int) * nt);
+        lds_copy(lds + lay.dinv, p.P_dinv + (size_t)mi * N, sizeof(cd) * nb);
+        __syncthreads();
+        cd *A1 = (cd *)(lds + lay.A1), *A2 = (cd *)(lds + lay.A2), *A3 = (cd *)(lds + lay.A3);
+        cd *A4 = (cd *)(lds + lay.A4), *di = (cd *)(lds + lay.dinv);
+        int *p1 = (int *)(lds + lay.p1), *p4 = (int *)(lds + lay.p4);
+        cd *inv = p.P_inv + (size_t)mi * N * ldi;
+        for (int j = wave; j < N; j += nw) {
+            for (int k = lane; k < N; k += 64) rhs[k] = mk(k == j ? fac[j] : 0.0, 0.0);
+            __builtin_amdgcn_wave_barrier();
+            if (m == 0) wave_bordered_fast<cd, 4, 4>(N, nt, A1, p1, A2, A3, A4, p4, di, rhs);
+            else wave_bordered_fast<cd, PZ_P_KL, PZ_P_KU>(N, nt, A1, p1, A2, A3, A4, p4, di, rhs);
+            for (int k = lane; k < ldi; k += 64) inv[(size_t)j * ldi + k] = k < N ? rhs[k] : mk(0.0, 0.0);
+            __builtin_amdgcn_wave_barrier();
+        }
     }
 }

@@ -38,8 +38,9 @@ struct PzDev {
     int l_heat, l_chem, l_coriolis_imp, l_buo_imp, l_ek_pump;
     int ktopt, kbott, ktopxi, kbotxi, ktopv, kbotv;
     int nrad, radices[PZ_MAX_RADICES]; // azimuthal FFT plan
+    int nradL, radicesL[PZ_MAX_RADICES]; // plan of length L (Chebyshev); 0: direct sums
     int l_exp_calc[8], l_imp_calc_rhs[8];
-    double rnorm, cheb_fac, boundary_fac, thr;
+    double rnorm, cheb_fac, icheb_fac, boundary_fac, thr;
     double CorFac, BuoFac, TdiffFac, ChemFac, XidiffFac, ViscFac, tadvz, r_cmb, ek;
     double alpha, dtMax, dtMin, dt_fac, courfac, wimp_lin0;
     double bimp0[PZ_MAX_STAGES][PZ_MAX_STAGES], bexp0[PZ_MAX_STAGES][PZ_MAX_STAGES];
@@ -48,6 +49,7 @@ struct PzDev {
     const double *r, *or1, *or2, *rgrav, *beta, *ekpump, *ekp_up, *ekp_us, *ekp_dusdp;
     const double *tcond, *dtcond, *xicond, *dxicond, *delxr2, *delxh2, *drx, *ddrx, *x;
     const double *cosT, *sinT; // cos/sin(pi q/(N-1)), q in [0, L)
+    const double *drb;         // [3][N] bulk factors of the radial derivatives
     const double *tau;         // tau lines: [rMat(2,N) | drMat(2,N) | d2rMat(2,N)]
     const cd *twid;            // exp(-2 pi i k / n_phi)
     const int *idx2m;
@@ -82,6 +84,12 @@ struct PzDev {
     double *P_fac;
     int *P_p1, *P_p4;
     int *info; // [3 * nm] factorisation info per (field, m)
+    // explicit inverses of the preconditioned bordered matrices with the row
+    // preconditioner folded in: x = INV * rhs.  Column-major per m,
+    // [nm][n_r columns][ldi rows], ldi = n_r rounded up to even (zero pad)
+    double *T_inv, *X_inv;
+    cd *P_inv;
+    int ldi;
 
     PzScheme *ss;
 };

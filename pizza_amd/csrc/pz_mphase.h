@@ -5,10 +5,11 @@
 #include "pz_device.h"
 
 // set_imex_rhs for one m row (multistep_schemes.f90:366-413;
-// dirk_schemes.f90:773-814).  cur: LDS copy of the current explicit slot or
-// nullptr to read it from global memory.
+// dirk_schemes.f90:773-814).  cur: LDS copy of the current explicit slot,
+// used when use_cur (else the slot is read from global memory).
 __device__ __forceinline__ cd imex_rhs(const PzDev &p, const PzScheme *s, int istage, const cd *old,
-                                       const cd *impl, const cd *expl, const cd *cur, size_t off, int k)
+                                       const cd *impl, const cd *expl, const cd *cur, bool use_cur, size_t off,
+                                       int k)
 {
     const size_t F = (size_t)p.nm * p.N;
     const int is = istage - 1;
@@ -18,13 +19,13 @@ __device__ __forceinline__ cd imex_rhs(const PzDev &p, const PzScheme *s, int is
         for (int o = 1; o < p.nold; o++) r = cadd(r, cscale(s->wimp[o], old[o * F + off + k]));
         for (int o = 0; o < p.nimp; o++) r = cadd(r, cscale(s->wimp_lin[o + 1], impl[o * F + off + k]));
         for (int o = 0; o < p.nexp; o++) {
-            cd e = (o == is && cur) ? cur[k] : expl[o * F + off + k];
+            cd e = (o == is && use_cur) ? lds_load(cur, k) : expl[o * F + off + k];
             r = cadd(r, cscale(s->wexp[o], e));
         }
     } else {
         r = old[off + k];
         for (int st = 0; st <= is; st++) {
-            cd e = (st == is && cur) ? cur[k] : expl[st * F + off + k];
+            cd e = (st == is && use_cur) ? lds_load(cur, k) : expl[st * F + off + k];
             r = cadd(r, cscale(s->bexp[istage][st], e));
         }
         for (int st = 0; st <= is; st++) r = cadd(r, cscale(s->bimp[istage][st], impl[st * F + off + k]));
@@ -76,48 +77,26 @@ __device__ void dcheb_real(const PzDev &p, const double *f, double *df)
 
 #define MPHASE_NVEC 18
 
-// LDS layout of k_mphase: working vectors, transform tables and the LU
-// factors of the three bordered systems of one m (16-byte aligned blocks).
+// LDS layout of k_mphase: working vectors, transform tables and the
+// reduction scratch of the dense solves (16-byte aligned blocks).
 struct MpLayout {
-    size_t vec, dtmp, cosT, sinT;
-    size_t TA4, TA2, TA3, TA1, Tp4, Tp1, Tdi;
-    size_t XA4, XA2, XA3, XA1, Xp4, Xp1, Xdi;
-    size_t PA4, PA2, PA3, PA1, Pp4, Pp1, Pdi;
-    size_t total;
+    size_t vec, dtmp, cosT, sinT, red, total;
     __host__ __device__ MpLayout(int N)
     {
         size_t o = 0;
         auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
-        const int L = 2 * N - 2, nb2 = N - 2, nbp = N - 2; // psi: m = 0 has the larger nb
+        const int L = 2 * N - 2;
         vec = take(sizeof(cd) * MPHASE_NVEC * N);
         dtmp = take(sizeof(double) * 2 * N);
         cosT = take(sizeof(double) * L);
         sinT = take(sizeof(double) * L);
-        TA4 = take(sizeof(double) * PZ_T_LD * nb2); TA2 = take(sizeof(double) * 2 * nb2);
-        TA3 = take(sizeof(double) * 2 * nb2); TA1 = take(sizeof(double) * 4);
-        Tp4 = take(sizeof(int) * nb2); Tp1 = take(sizeof(int) * 2); Tdi = take(sizeof(double) * nb2);
-        XA4 = take(sizeof(double) * PZ_T_LD * nb2); XA2 = take(sizeof(double) * 2 * nb2);
-        XA3 = take(sizeof(double) * 2 * nb2); XA1 = take(sizeof(double) * 4);
-        Xp4 = take(sizeof(int) * nb2); Xp1 = take(sizeof(int) * 2); Xdi = take(sizeof(double) * nb2);
-        PA4 = take(sizeof(cd) * PZ_P_LD * nbp); PA2 = take(sizeof(cd) * PZ_P_NT * nbp);
-        PA3 = take(sizeof(cd) * PZ_P_NT * nbp); PA1 = take(sizeof(cd) * 16);
-        Pp4 = take(sizeof(int) * nbp); Pp1 = take(sizeof(int) * 4); Pdi = take(sizeof(cd) * nbp);
+        // FFT scratch of the Chebyshev transforms (6 L + 12), shared with the
+        // reduction scratch of the dense solves (2 x 512)
+        red = take(sizeof(cd) * (6 * L + 16 > 1024 ? 6 * L + 16 : 1024));
         total = o;
     }
 };
 
-__device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t bytes)
-{
-    if ((bytes & 7) == 0 && ((size_t)src & 7) == 0) {
-        const double *s = (const double *)src;
-        double *d = (double *)dst;
-        for (size_t i = threadIdx.x; i < bytes / 8; i += blockDim.x) d[i] = s[i];
-    } else {
-        const int *s = (const int *)src;
-        int *d = (int *)dst;
-        for (size_t i = threadIdx.x; i < bytes / 4; i += blockDim.x) d[i] = s[i];
-    }
-}
 
 // One stage of the m-loop for one azimuthal wavenumber per workgroup:
 // finish_explicit_assembly (mloop.f90:169-229) when exp_calc, then
@@ -130,7 +109,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
-    const int tid = threadIdx.x, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const MpLayout lay(N);
     char *lb = (char *)smem;
     cd *V = (cd *)(lb + lay.vec);
@@ -142,36 +121,11 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     double *dtmp = (double *)(lb + lay.dtmp);
     double *cosT = (double *)(lb + lay.cosT), *sinT = (double *)(lb + lay.sinT);
     const bool hT = p.l_heat, hX = p.l_chem;
-    const int nb2 = N - 2;
-    const int ntP = m == 0 ? 2 : PZ_P_NT, klP = m == 0 ? 4 : PZ_P_KL, ldP = 3 * klP + 1, nbP = N - ntP;
-    // stage transform tables and this m's LU factors into LDS
+    cd *red = (cd *)(lb + lay.red), *fbuf = red;
+    const int ldi = p.ldi;
+    // stage the transform tables into LDS
     lds_copy(cosT, p.cosT, sizeof(double) * p.L);
     lds_copy(sinT, p.sinT, sizeof(double) * p.L);
-    if (hT) {
-        lds_copy(lb + lay.TA4, p.T_A4 + (size_t)mi * PZ_T_LD * nb2, sizeof(double) * PZ_T_LD * nb2);
-        lds_copy(lb + lay.TA2, p.T_A2 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
-        lds_copy(lb + lay.TA3, p.T_A3 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
-        lds_copy(lb + lay.TA1, p.T_A1 + (size_t)mi * 4, sizeof(double) * 4);
-        lds_copy(lb + lay.Tp4, p.T_p4 + (size_t)mi * nb2, sizeof(int) * nb2);
-        lds_copy(lb + lay.Tp1, p.T_p1 + (size_t)mi * 2, sizeof(int) * 2);
-        lds_copy(lb + lay.Tdi, p.T_dinv + (size_t)mi * nb2, sizeof(double) * nb2);
-    }
-    if (hX) {
-        lds_copy(lb + lay.XA4, p.X_A4 + (size_t)mi * PZ_T_LD * nb2, sizeof(double) * PZ_T_LD * nb2);
-        lds_copy(lb + lay.XA2, p.X_A2 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
-        lds_copy(lb + lay.XA3, p.X_A3 + (size_t)mi * 2 * nb2, sizeof(double) * 2 * nb2);
-        lds_copy(lb + lay.XA1, p.X_A1 + (size_t)mi * 4, sizeof(double) * 4);
-        lds_copy(lb + lay.Xp4, p.X_p4 + (size_t)mi * nb2, sizeof(int) * nb2);
-        lds_copy(lb + lay.Xp1, p.X_p1 + (size_t)mi * 2, sizeof(int) * 2);
-        lds_copy(lb + lay.Xdi, p.X_dinv + (size_t)mi * nb2, sizeof(double) * nb2);
-    }
-    lds_copy(lb + lay.PA4, p.P_A4 + (size_t)mi * PZ_P_LD * N, sizeof(cd) * ldP * nbP);
-    lds_copy(lb + lay.PA2, p.P_A2 + (size_t)mi * PZ_P_NT * N, sizeof(cd) * ntP * nbP);
-    lds_copy(lb + lay.PA3, p.P_A3 + (size_t)mi * PZ_P_NT * N, sizeof(cd) * ntP * nbP);
-    lds_copy(lb + lay.PA1, p.P_A1 + (size_t)mi * 16, sizeof(cd) * ntP * ntP);
-    lds_copy(lb + lay.Pp4, p.P_p4 + (size_t)mi * N, sizeof(int) * nbP);
-    lds_copy(lb + lay.Pp1, p.P_p1 + (size_t)mi * 4, sizeof(int) * ntP);
-    lds_copy(lb + lay.Pdi, p.P_dinv + (size_t)mi * N, sizeof(cd) * nbP);
     __syncthreads();
 
     // ---------------- explicit assembly -----------------------------------
@@ -188,23 +142,21 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
             if (!p.l_buo_imp) { Tv[k] = hT ? p.temp[off + k] : mk(0, 0); Xv[k] = hX ? p.xi[off + k] : mk(0, 0); }
         }
         __syncthreads();
-        const cd *din[3];
-        cd *dout[3];
-        int nf = 0;
-        if (hT) { din[nf] = aT; dout[nf++] = cT; }
-        if (hX) { din[nf] = aX; dout[nf++] = cX; }
-        din[nf] = aP; dout[nf++] = cP;
-        dct_many(p, din, dout, nf, 1, cosT);
+        VecSet<const cd> din;
+        VecSet<cd> dout;
+        if (hT) { din.push(aT); dout.push(cT); }
+        if (hX) { din.push(aX); dout.push(cX); }
+        din.push(aP); dout.push(cP);
+        dct_fft(p, din, dout, din.n, 1, cosT, sinT, fbuf);
         __syncthreads();
         // derivative of the advective flux (into the a-vectors)
         {
-            const cd *c3[3];
-            cd *d3[3];
-            int k3 = 0;
-            if (hT) { c3[k3] = cT; d3[k3++] = aT; }
-            if (hX) { c3[k3] = cX; d3[k3++] = aX; }
-            c3[k3] = cP; d3[k3++] = aP;
-            dr_many(p, c3, d3, nullptr, k3, cosT, sinT);
+            VecSet<const cd> c3;
+            VecSet<cd> d3;
+            if (hT) { c3.push(cT); d3.push(aT); }
+            if (hX) { c3.push(cX); d3.push(aX); }
+            c3.push(cP); d3.push(aP);
+            dr_fft(p, c3, d3, VecSet<cd>(), c3.n, cosT, sinT, fbuf);
         }
         __syncthreads();
         const double rc2 = p.r_cmb * p.r_cmb;
@@ -246,11 +198,12 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
             eP[k] = e;
         }
         __syncthreads();
-        nf = 0;
-        if (hT) { din[nf] = eT; dout[nf++] = cT; }
-        if (hX) { din[nf] = eX; dout[nf++] = cX; }
-        din[nf] = eP; dout[nf++] = cP;
-        dct_many(p, din, dout, nf, 1, cosT);
+        din = VecSet<const cd>();
+        dout = VecSet<cd>();
+        if (hT) { din.push(eT); dout.push(cT); }
+        if (hX) { din.push(eX); dout.push(cX); }
+        din.push(eP); dout.push(cP);
+        dct_fft(p, din, dout, din.n, 1, cosT, sinT, fbuf);
         __syncthreads();
         if (hT) bandmv(p, p.T_RHSE1, 3, cT, eT, 2);
         if (hX) bandmv(p, p.T_RHSE1, 3, cX, eX, 2);
@@ -267,37 +220,35 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     // ---------------- T and xi: rhs, bordered solve ------------------------
     for (int k = tid; k < N; k += blockDim.x) {
         if (hT) {
-            cd r = imex_rhs(p, ss, istage, p.T_old, p.T_impl, p.T_expl, exp_calc ? eT : nullptr, off, k);
+            cd r = imex_rhs(p, ss, istage, p.T_old, p.T_impl, p.T_expl, eT, exp_calc != 0, off, k);
             if (k == 0) r = p.topt[mi];
             if (k == 1) r = p.bott[mi];
-            aT[k] = cscale(p.T_fac[off + k], r);
+            aT[k] = r;
         }
         if (hX) {
-            cd r = imex_rhs(p, ss, istage, p.X_old, p.X_impl, p.X_expl, exp_calc ? eX : nullptr, off, k);
+            cd r = imex_rhs(p, ss, istage, p.X_old, p.X_impl, p.X_expl, eX, exp_calc != 0, off, k);
             if (k == 0) r = p.topxi[mi];
             if (k == 1) r = p.botxi[mi];
-            aX[k] = cscale(p.X_fac[off + k], r);
+            aX[k] = r;
         }
     }
     __syncthreads();
-    if (wave == 0 && hT)
-        wave_bordered_fast<double, PZ_T_KL, PZ_T_KU>(N, PZ_T_NT, (double *)(lb + lay.TA1), (int *)(lb + lay.Tp1),
-                                                     (double *)(lb + lay.TA2), (double *)(lb + lay.TA3),
-                                                     (double *)(lb + lay.TA4), (int *)(lb + lay.Tp4),
-                                                     (double *)(lb + lay.Tdi), aT);
-    if (wave == 1 && hX)
-        wave_bordered_fast<double, PZ_T_KL, PZ_T_KU>(N, PZ_T_NT, (double *)(lb + lay.XA1), (int *)(lb + lay.Xp1),
-                                                     (double *)(lb + lay.XA2), (double *)(lb + lay.XA3),
-                                                     (double *)(lb + lay.XA4), (int *)(lb + lay.Xp4),
-                                                     (double *)(lb + lay.Xdi), aX);
-    __syncthreads();
+    // bordered solves of T and xi (update_temp_integ.f90:226-240) by their inverses
+    if (hT && hX) {
+        const VecSet<const double> inv(p.T_inv + off * ldi, p.X_inv + off * ldi);
+        const VecSet<cd> xs(aT, aX);
+        block_matvec<double, 2>(N, ldi, inv, xs, xs, red);
+    } else if (hT || hX) {
+        const VecSet<const double> inv(hT ? p.T_inv + off * ldi : p.X_inv + off * ldi);
+        const VecSet<cd> xs(hT ? aT : aX);
+        block_matvec<double, 1>(N, ldi, inv, xs, xs, red);
+    }
     {
-        const cd *din[2];
-        cd *dout[2];
-        int nf = 0;
-        if (hT) { din[nf] = aT; dout[nf++] = Tv; }
-        if (hX) { din[nf] = aX; dout[nf++] = Xv; }
-        if (nf) dct_many(p, din, dout, nf, 0, cosT);
+        VecSet<const cd> din;
+        VecSet<cd> dout;
+        if (hT) { din.push(aT); dout.push(Tv); }
+        if (hX) { din.push(aX); dout.push(Xv); }
+        if (din.n) dct_fft(p, din, dout, din.n, 0, cosT, sinT, fbuf);
     }
     __syncthreads();
     const int inext = (istage == p.nstages) ? 1 : istage + 1;
@@ -330,28 +281,26 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         for (int k = tid; k < N; k += blockDim.x)
             bv[k] = buoyancy(p, m, k, hT ? Tv[k] : mk(0, 0), hX ? Xv[k] : mk(0, 0));
         __syncthreads();
-        const cd *din[1] = {bv};
-        cd *dout[1] = {wv};
-        dct_many(p, din, dout, 1, 0, cosT);
+        VecSet<const cd> din(bv);
+        VecSet<cd> dout(wv);
+        dct_fft(p, din, dout, 1, 0, cosT, sinT, fbuf);
         __syncthreads();
         if (m > 0) bandmv(p, p.P_RHSE2, 8, wv, b2, 4);
         __syncthreads();
     }
     const double wl = ss->wimp_lin[0];
     for (int k = tid; k < N; k += blockDim.x) {
-        cd r = imex_rhs(p, ss, istage, p.P_old, p.P_impl, p.P_expl, exp_calc ? eP : nullptr, off, k);
+        cd r = imex_rhs(p, ss, istage, p.P_old, p.P_impl, p.P_expl, eP, exp_calc != 0, off, k);
         if (p.l_buo_imp && m != 0) r = cadd(r, cscale(wl, b2[k]));
-        aP[k] = cscale(p.P_fac[off + k], r);
+        aP[k] = r;
     }
     __syncthreads();
-    if (wave == 0) {
-        cd *A1 = (cd *)(lb + lay.PA1), *A2 = (cd *)(lb + lay.PA2), *A3 = (cd *)(lb + lay.PA3);
-        cd *A4 = (cd *)(lb + lay.PA4), *di = (cd *)(lb + lay.Pdi);
-        int *p1 = (int *)(lb + lay.Pp1), *p4 = (int *)(lb + lay.Pp4);
-        if (m == 0) wave_bordered_fast<cd, 4, 4>(N, ntP, A1, p1, A2, A3, A4, p4, di, aP);
-        else wave_bordered_fast<cd, PZ_P_KL, PZ_P_KU>(N, ntP, A1, p1, A2, A3, A4, p4, di, aP);
+    {
+        // bordered solve of psi (update_psi_integ_smat.f90:417-432) by its inverse
+        VecSet<const cd> inv(p.P_inv + off * ldi);
+        VecSet<cd> xs(aP);
+        block_matvec<cd, 1>(N, ldi, inv, xs, xs, red);
     }
-    __syncthreads();
     const double rc2 = p.r_cmb * p.r_cmb;
     if (m == 0) {
         double *f = dtmp, *df = dtmp + N;
@@ -361,9 +310,9 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) { cP[k] = mk(df[k], 0.0); eP[k] = mk(f[k], 0.0); }
         __syncthreads();
-        const cd *din[2] = {cP, eP};
-        cd *dout[2] = {omv, upv};
-        dct_many(p, din, dout, 2, 0, cosT);
+        VecSet<const cd> din(cP, eP);
+        VecSet<cd> dout(omv, upv);
+        dct_fft(p, din, dout, 2, 0, cosT, sinT, fbuf);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             double u0 = upv[k].x, om0 = p.drx[k] * omv[k].x;
@@ -374,18 +323,19 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         }
         __syncthreads();
         // work_r = costf(real(up(m=0))) for get_psi_rhs_imp_int_smat
-        const cd *di[1] = {eP};
-        cd *dd[1] = {wv};
-        dct_many(p, di, dd, 1, 0, cosT);
+        VecSet<const cd> di(eP);
+        VecSet<cd> dd(wv);
+        dct_fft(p, di, dd, 1, 0, cosT, sinT, fbuf);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) wv[k] = mk(wv[k].x, 0.0);
     } else {
-        const cd *c1[1] = {aP};
-        cd *d1[1] = {cP}, *d2[1] = {eP};
-        dr_many(p, c1, d1, d2, 1, cosT, sinT);
-        const cd *din[1] = {aP};
-        cd *dout[1] = {psiv};
-        dct_many(p, din, dout, 1, 0, cosT);
+        VecSet<const cd> c1(aP);
+        VecSet<cd> d1(cP), d2(eP);
+        dr_fft(p, c1, d1, d2, 1, cosT, sinT, fbuf);
+        __syncthreads();
+        VecSet<const cd> din(aP);
+        VecSet<cd> dout(psiv);
+        dct_fft(p, din, dout, 1, 0, cosT, sinT, fbuf);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             const double rk = p.r[k], h2 = rc2 - rk * rk, or1 = p.or1[k];
@@ -442,15 +392,15 @@ __global__ void __launch_bounds__(256) k_start(PzDev p)
     }
     __syncthreads();
     {
-        const cd *din[5] = {Tv, Xv, upv, psv, nullptr};
-        cd *dout[5] = {cT, cX, cU, cPs, nullptr};
+        VecSet<const cd> din(Tv, Xv, upv, psv);
+        VecSet<cd> dout(cT, cX, cU, cPs);
         dct_many(p, din, dout, 4, 0);
     }
     __syncthreads();
     {
-        const cd *c2[2] = {cT, cU};
-        cd *d2[2] = {dT, dU};
-        dr_many(p, c2, d2, nullptr, 2);
+        VecSet<const cd> c2(cT, cU);
+        VecSet<cd> d2(dT, dU);
+        dr_many(p, c2, d2, VecSet<cd>(), 2);
     }
     __syncthreads();
     for (int k = tid; k < N; k += blockDim.x) {
@@ -480,8 +430,8 @@ __global__ void __launch_bounds__(256) k_start(PzDev p)
         // work_r = costf(real(up(m=0)))
         for (int k = tid; k < N; k += blockDim.x) bv[k] = mk(upv[k].x, 0.0);
         __syncthreads();
-        const cd *di[1] = {bv};
-        cd *dd[1] = {wv};
+        VecSet<const cd> di(bv);
+        VecSet<cd> dd(wv);
         dct_many(p, di, dd, 1, 0);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) wv[k] = mk(wv[k].x, 0.0);
@@ -494,8 +444,8 @@ __global__ void __launch_bounds__(256) k_start(PzDev p)
         if (p.l_buo_imp) {
             for (int k = tid; k < N; k += blockDim.x) bv[k] = buoyancy(p, m, k, Tv[k], psv[k]);
             __syncthreads();
-            const cd *di[1] = {bv};
-            cd *dd[1] = {wv};
+            VecSet<const cd> di(bv);
+            VecSet<cd> dd(wv);
             dct_many(p, di, dd, 1, 0);
             __syncthreads();
             bandmv(p, p.P_RHSE2, 8, wv, b2, 4);
@@ -534,13 +484,13 @@ __global__ void k_ekin_profiles(PzDev p, double *out)
 __global__ void __launch_bounds__(256) k_costf_vecs(PzDev p, cd *f)
 {
     extern __shared__ double smem[];
-    cd *a = (cd *)smem, *b = a + p.N;
+    cd *a = (cd *)smem, *b = a + p.N, *buf = b + p.N;
     cd *x = f + (size_t)blockIdx.x * p.N;
     for (int k = threadIdx.x; k < p.N; k += blockDim.x) a[k] = x[k];
     __syncthreads();
-    const cd *din[1] = {a};
-    cd *dout[1] = {b};
-    dct_many(p, din, dout, 1, 0);
+    VecSet<const cd> din(a);
+    VecSet<cd> dout(b);
+    dct_fft(p, din, dout, 1, 0, p.cosT, p.sinT, buf);
     __syncthreads();
     for (int k = threadIdx.x; k < p.N; k += blockDim.x) x[k] = b[k];
 }
@@ -548,21 +498,21 @@ __global__ void __launch_bounds__(256) k_costf_vecs(PzDev p, cd *f)
 __global__ void __launch_bounds__(256) k_dr_vecs(PzDev p, const cd *f, cd *df, cd *ddf, int l_dct_in)
 {
     extern __shared__ double smem[];
-    cd *a = (cd *)smem, *c = a + p.N, *d1 = c + p.N, *d2 = d1 + p.N;
+    cd *a = (cd *)smem, *c = a + p.N, *d1 = c + p.N, *d2 = d1 + p.N, *buf = d2 + p.N;
     const size_t off = (size_t)blockIdx.x * p.N;
     for (int k = threadIdx.x; k < p.N; k += blockDim.x) a[k] = f[off + k];
     __syncthreads();
     if (l_dct_in) {
-        const cd *din[1] = {a};
-        cd *dout[1] = {c};
-        dct_many(p, din, dout, 1, 0);
+        VecSet<const cd> din(a);
+        VecSet<cd> dout(c);
+        dct_fft(p, din, dout, 1, 0, p.cosT, p.sinT, buf);
     } else {
         for (int k = threadIdx.x; k < p.N; k += blockDim.x) c[k] = a[k];
     }
     __syncthreads();
-    const cd *cc[1] = {c};
-    cd *o1[1] = {d1}, *o2[1] = {d2};
-    dr_many(p, cc, o1, ddf ? o2 : nullptr, 1);
+    VecSet<const cd> cc(c);
+    VecSet<cd> o1(d1), o2(d2);
+    dr_fft(p, cc, o1, ddf ? o2 : VecSet<cd>(), 1, p.cosT, p.sinT, buf);
     __syncthreads();
     for (int k = threadIdx.x; k < p.N; k += blockDim.x) {
         df[off + k] = d1[k];
