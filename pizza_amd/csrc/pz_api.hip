@@ -111,6 +111,7 @@ struct pz_handle {
     bool use_graph = true;
     size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0;
     int inv_waves = 16;
+    PzDev *ddev = nullptr;
     std::string err;
 
     template <class T> T *dalloc(size_t n)
@@ -334,6 +335,7 @@ static void build_operators(pz_handle *h)
 static std::vector<int> factor_radices(int n)
 {
     std::vector<int> f;
+    while (n % 8 == 0) { f.push_back(8); n /= 8; }
     while (n % 4 == 0) { f.push_back(4); n /= 4; }
     while (n % 2 == 0) { f.push_back(2); n /= 2; }
     while (n % 3 == 0) { f.push_back(3); n /= 3; }
@@ -455,24 +457,6 @@ static void setup(pz_handle *h)
     std::vector<double> tcond(N, 0.0), dtcond(N, 0.0), xicond(N, 0.0), dxicond(N, 0.0);
     if (h->l_heat) conducting_state(h, c.tcond_fac, c.ktopt, c.kbott, c.t_top, c.t_bot, tcond, dtcond);
     if (h->l_chem) conducting_state(h, c.xicond_fac, c.ktopxi, c.kbotxi, c.xi_top, c.xi_bot, xicond, dxicond);
-    d.x = h->upload(x);
-    d.r = h->upload(r);
-    d.drx = h->upload(drx);
-    d.ddrx = h->upload(ddrx);
-    d.or1 = h->upload(or1);
-    d.or2 = h->upload(or2);
-    d.rgrav = h->upload(rgrav);
-    d.beta = h->upload(beta);
-    d.ekpump = h->upload(ekpump);
-    d.ekp_up = h->upload(ekp_up);
-    d.ekp_us = h->upload(ekp_us);
-    d.ekp_dusdp = h->upload(ekp_dusdp);
-    d.delxr2 = h->upload(delxr2);
-    d.delxh2 = h->upload(delxh2);
-    d.tcond = h->upload(tcond);
-    d.dtcond = h->upload(dtcond);
-    d.xicond = h->upload(xicond);
-    d.dxicond = h->upload(dxicond);
     // transform tables
     const int L = 2 * N - 2;
     std::vector<double> cosT(L), sinT(L);
@@ -483,10 +467,10 @@ static void setup(pz_handle *h)
     }
     d.cosT = h->upload(cosT);
     d.sinT = h->upload(sinT);
+    // bulk factors of get_dr_fft / get_ddr_fft at the interior nodes:
+    // 1/(sqrt(1-x^2) nrm), x/((1-x^2)^1.5 nrm), 1/((1-x^2) nrm)
+    std::vector<double> drb(3 * N, 0.0);
     {
-        // bulk factors of get_dr_fft / get_ddr_fft at the interior nodes:
-        // 1/(sqrt(1-x^2) nrm), x/((1-x^2)^1.5 nrm), 1/((1-x^2) nrm)
-        std::vector<double> drb(3 * N, 0.0);
         const ld nrm = (ld)(2 * N - 2);
         for (int j = 1; j < N - 1; j++) {
             const double xj = x[j];
@@ -495,7 +479,19 @@ static void setup(pz_handle *h)
             drb[N + j] = (double)((ld)xj / (powl(sj, 1.5L) * nrm));
             drb[2 * N + j] = (double)(1.0L / (sj * nrm));
         }
-        d.drb = h->upload(drb);
+    }
+    {
+        // all radial functions in one block of PZ_NRADC rows of n_r values
+        // (row order of set_radial_pointers), so that a kernel can stage
+        // them into LDS with one copy
+        const std::vector<double> *rows[18] = {&x, &r, &drx, &ddrx, &or1, &or2, &rgrav, &beta, &ekpump,
+                                               &ekp_up, &ekp_us, &ekp_dusdp, &delxr2, &delxh2, &tcond,
+                                               &dtcond, &xicond, &dxicond};
+        std::vector<double> rc((size_t)PZ_NRADC * N, 0.0);
+        for (int i = 0; i < 18; i++) std::copy(rows[i]->begin(), rows[i]->end(), rc.begin() + (size_t)i * N);
+        std::copy(drb.begin(), drb.end(), rc.begin() + (size_t)18 * N);
+        d.radc = h->upload(rc);
+        set_radial_pointers(d, d.radc, (size_t)N);
     }
     std::vector<double> tau(6 * N);
     for (int n = 0; n < N; n++) {
@@ -595,6 +591,10 @@ static void setup(pz_handle *h)
         HIPCHK(hipFuncSetAttribute((const void *)k_build_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_inv));
     }
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    // device copy of the (now final) descriptor for kernels that read it
+    // through a pointer
+    h->ddev = h->dalloc<PzDev>(1);
+    HIPCHK(hipMemcpy(h->ddev, &h->d, sizeof(PzDev), hipMemcpyHostToDevice));
 }
 
 extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
@@ -882,7 +882,7 @@ static void launch_step(pz_handle *h)
             hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, h->d, 0);
             hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream, h->d, 0);
         }
-        hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(256), h->lds_mphase, h->stream, h->d, stage, exp_calc);
+        hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream, (const PzDev *)h->ddev, stage, exp_calc);
     }
 }
 
@@ -1099,7 +1099,7 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
         HIPCHK(hipEventRecord(e0, h->stream));
         for (int i = 0; i < reps; i++) {
             if (which == 0)
-                hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(256), h->lds_mphase, h->stream, h->d, 1, 1);
+                hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream, (const PzDev *)h->ddev, 1, 1);
             else
                 hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, h->d, 0, 0);
         }

@@ -143,6 +143,33 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
                 y[idxD + Ns] = cadd(b0, d0);
                 y[idxD + 2 * Ns] = csub(a0, c0);
                 y[idxD + 3 * Ns] = csub(b0, d0);
+            } else if (R == 8) {
+                // radix 8 as 2 x 4: a_r = v_r + v_{r+4} gives the even outputs,
+                // (v_r - v_{r+4}) w8^r the odd ones, w8 = exp(dir i pi/4)
+                const cd v0 = x[j], v1 = cmul(x[j + nb], tw(tstep)), v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
+                const cd v3 = cmul(x[j + 3 * nb], tw(3 * tstep)), v4 = cmul(x[j + 4 * nb], tw(4 * tstep));
+                const cd v5 = cmul(x[j + 5 * nb], tw(5 * tstep)), v6 = cmul(x[j + 6 * nb], tw(6 * tstep));
+                const cd v7 = cmul(x[j + 7 * nb], tw(7 * tstep));
+                const double h = 0.70710678118654752440, sd = (double)dir;
+                const cd a0 = cadd(v0, v4), a1 = cadd(v1, v5), a2 = cadd(v2, v6), a3 = cadd(v3, v7);
+                const cd b0 = csub(v0, v4), c1 = csub(v1, v5), c2 = csub(v2, v6), c3 = csub(v3, v7);
+                const cd b1 = mk(h * (c1.x - sd * c1.y), h * (c1.y + sd * c1.x));
+                const cd b2 = cimul(sd, c2);
+                const cd b3 = mk(h * (-c3.x - sd * c3.y), h * (-c3.y + sd * c3.x));
+                {
+                    const cd s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = cimul(sd, csub(a1, a3));
+                    y[idxD] = cadd(s02, s13);
+                    y[idxD + 2 * Ns] = cadd(d02, d13);
+                    y[idxD + 4 * Ns] = csub(s02, s13);
+                    y[idxD + 6 * Ns] = csub(d02, d13);
+                }
+                {
+                    const cd s02 = cadd(b0, b2), d02 = csub(b0, b2), s13 = cadd(b1, b3), d13 = cimul(sd, csub(b1, b3));
+                    y[idxD + Ns] = cadd(s02, s13);
+                    y[idxD + 3 * Ns] = cadd(d02, d13);
+                    y[idxD + 5 * Ns] = csub(s02, s13);
+                    y[idxD + 7 * Ns] = csub(d02, d13);
+                }
             } else if (R == 3) {
                 const double s3 = 0.86602540378443864676;
                 cd v0 = x[j];
@@ -216,7 +243,7 @@ __device__ double block_reduce(double v, double *scratch, int op /*0 sum, 1 max*
 // Radial loop: rloop::radial_loop (radial_loop.f90:264-392) for one radius
 // per workgroup.  Real fields travel in pairs through one complex FFT.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_radial_loop(PzDev p, int is, int courant)
+__global__ void __launch_bounds__(256) k_radial_loop(const PzDev p __attribute__((grid_constant)), int is, int courant)
 {
     extern __shared__ double smem[];
     const int n = p.nphi, nm = p.nm, N = p.N, nr = blockIdx.x;
@@ -355,7 +382,7 @@ __device__ void scheme_weights(const PzDev &p, PzScheme *s, int *lMatNext)
     }
 }
 
-__global__ void k_set_weights(PzDev p)
+__global__ void k_set_weights(const PzDev p __attribute__((grid_constant)))
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         PzScheme *s = p.ss;
@@ -364,7 +391,7 @@ __global__ void k_set_weights(PzDev p)
     }
 }
 
-__global__ void __launch_bounds__(256) k_courant(PzDev p)
+__global__ void __launch_bounds__(256) k_courant(const PzDev p __attribute__((grid_constant)))
 {
     __shared__ double red[8];
     PzScheme *s = p.ss;
@@ -666,14 +693,14 @@ __device__ __forceinline__ void dr_boundary(const PzDev &p, const cd *w, int j, 
 // bulk point j from V = vx + i vy and U = ux + i uy (sums over the raw
 // coefficients); drb holds 1/(sqrt(1-x^2) nrm), x/((1-x^2)^1.5 nrm) and
 // 1/((1-x^2) nrm) with nrm = 2 N - 2
-__device__ __forceinline__ void dr_bulk(const PzDev &p, cd wl, int j, bool second, double vx, double vy,
+__device__ __forceinline__ void dr_bulk(const PzDev &p, const RadView &R, cd wl, int j, bool second, double vx, double vy,
                                         double ux, double uy, cd &d1, cd &d2)
 {
     const int N = p.N;
     const double icf = p.icheb_fac;
     vx *= icf; vy *= icf; ux *= icf; uy *= icf;
     const double V1x = -2.0 * vx, V1y = -2.0 * vy;
-    const double g1 = p.drb[j], g2 = p.drb[N + j], g3 = p.drb[2 * N + j];
+    const double g1 = R.drb[j], g2 = R.drb[N + j], g3 = R.drb[2 * N + j];
     d1 = mk(-V1x * g1, -V1y * g1);
     d2 = mk(0, 0);
     if (second) {
@@ -685,18 +712,18 @@ __device__ __forceinline__ void dr_bulk(const PzDev &p, cd wl, int j, bool secon
     }
 }
 
-__device__ __forceinline__ void dr_store(const PzDev &p, int j, cd d1, cd d2, bool second, cd *df, cd *ddf)
+__device__ __forceinline__ void dr_store(const PzDev &p, const RadView &R, int j, cd d1, cd d2, bool second, cd *df, cd *ddf)
 {
-    const double a = p.drx[j];
+    const double a = R.drx[j];
     if (second) {
-        const double b = p.ddrx[j];
+        const double b = R.ddrx[j];
         lds_store(ddf, j, mk(b * d1.x + a * a * d2.x, b * d1.y + a * a * d2.y));
     }
     lds_store(df, j, mk(a * d1.x, a * d1.y));
 }
 
 // first (and optionally second) derivative of nf vectors, direct sums
-__device__ void dr_many(const PzDev &p, VecSet<const cd> c, VecSet<cd> df, VecSet<cd> ddf, int nf,
+__device__ void dr_many(const PzDev &p, const RadView &R, VecSet<const cd> c, VecSet<cd> df, VecSet<cd> ddf, int nf,
                         const double *cosT = nullptr, const double *sinT = nullptr)
 {
     const int N = p.N, L = p.L;
@@ -724,9 +751,9 @@ __device__ void dr_many(const PzDev &p, VecSet<const cd> c, VecSet<cd> df, VecSe
                     uy += k2 * wy * co;
                 }
             }
-            dr_bulk(p, w[N - 1], j, ddf.v0 != nullptr, vx, vy, ux, uy, d1, d2);
+            dr_bulk(p, R, w[N - 1], j, ddf.v0 != nullptr, vx, vy, ux, uy, d1, d2);
         }
-        dr_store(p, j, d1, d2, ddf.v0 != nullptr, df[f], ddf.v0 ? ddf[f] : df[f]);
+        dr_store(p, R, j, d1, d2, ddf.v0 != nullptr, df[f], ddf.v0 ? ddf[f] : df[f]);
     }
 }
 
@@ -760,25 +787,29 @@ __device__ void dct_fft(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int
 // derivatives of nf <= 3 vectors through FFTs of length L: the odd
 // extension of k w_k gives -2i V, the even extension of k^2 w_k (zero at
 // k = 0, N-1) gives 2U.  The boundary points x = +-1 are k^2-weighted sums,
-// one half-wave reduction per (vector, side).  buf: LDS scratch of
-// 2 ntr L + 4 nf complex values (ntr = nf, or 2 nf with the second
-// derivative); no trailing barrier.
-__device__ void dr_fft(const PzDev &p, VecSet<const cd> c, VecSet<cd> df, VecSet<cd> ddf, int nf,
-                       const double *cosT, const double *sinT, cd *buf)
+// one half-wave reduction per (vector, side).  With dct_out (nf = 1) the
+// same FFT pass also returns costf(c) (even extension of c itself).
+// buf: LDS scratch of 2 ntr L + 4 nf complex values (ntr = nf, + nf with
+// the second derivative, + 1 with dct_out); no trailing barrier.
+__device__ void dr_fft(const PzDev &p, const RadView &R, VecSet<const cd> c, VecSet<cd> df, VecSet<cd> ddf, int nf,
+                       const double *cosT, const double *sinT, cd *buf, cd *dct_out = nullptr)
 {
     if (p.nradL == 0) {
-        dr_many(p, c, df, ddf, nf, cosT, sinT);
+        dr_many(p, R, c, df, ddf, nf, cosT, sinT);
+        if (dct_out) dct_many(p, VecSet<const cd>(c[0]), VecSet<cd>(dct_out), 1, 0, cosT);
         return;
     }
-    const bool second = ddf.v0 != nullptr;
-    const int N = p.N, L = p.L, ntr = second ? 2 * nf : nf;
+    const bool second = ddf.v0 != nullptr, with_dct = dct_out != nullptr;
+    const int N = p.N, L = p.L, nder = second ? 2 * nf : nf, ntr = nder + (with_dct ? 1 : 0);
     cd *A = buf, *B = buf + (size_t)ntr * L, *bnd = buf + (size_t)2 * ntr * L;
     for (int t = threadIdx.x; t < ntr * L; t += blockDim.x) {
         const int tr = t / L, j = t - tr * L;
-        const int f = tr < nf ? tr : tr - nf;
+        const int f = tr < nf ? tr : (tr < nder ? tr - nf : 0);
         const int k = j < N ? j : L - j;
         cd y = mk(0.0, 0.0);
-        if (k != 0 && k != N - 1) {
+        if (tr >= nder) {
+            y = lds_load(c[0], k);
+        } else if (k != 0 && k != N - 1) {
             const cd w = lds_load(c[f], k);
             const double sc = tr < nf ? (j < N ? (double)k : -(double)k) : (double)k * (double)k;
             y = mk(sc * w.x, sc * w.y);
@@ -831,39 +862,79 @@ __device__ void dr_fft(const PzDev &p, VecSet<const cd> c, VecSet<cd> df, VecSet
         } else {
             const cd zv = Z[(size_t)f * L + j];
             const cd zu = second ? Z[(size_t)(nf + f) * L + j] : mk(0.0, 0.0);
-            dr_bulk(p, lds_load(c[f], N - 1), j, second, -0.5 * zv.y, 0.5 * zv.x, 0.5 * zu.x, 0.5 * zu.y, d1, d2);
+            dr_bulk(p, R, lds_load(c[f], N - 1), j, second, -0.5 * zv.y, 0.5 * zv.x, 0.5 * zu.x, 0.5 * zu.y, d1, d2);
         }
-        dr_store(p, j, d1, d2, second, df[f], second ? ddf[f] : df[f]);
+        dr_store(p, R, j, d1, d2, second, df[f], second ? ddf[f] : df[f]);
+    }
+    if (with_dct) {
+        const double cf = p.cheb_fac;
+        for (int k = threadIdx.x; k < N; k += blockDim.x) {
+            const cd z = Z[(size_t)nder * L + k];
+            lds_store(dct_out, k, mk(cf * z.x, cf * z.y));
+        }
     }
 }
 
-// y = band * x for a row-major band [n][2K+1]; rows < nzero become zero
-__device__ void bandmv(const PzDev &p, const double *band, int K, const cd *x, cd *y, int nzero)
+// y = band * x for a row-major band [n][2K+1]; rows < nzero become zero.
+// Four lanes share a row: lane s takes the diagonals s, s+4, ... (loads
+// issued together), and two xor-shuffles combine the partial sums in a
+// fixed order.
+template <int K>
+__device__ void bandmv(const PzDev &p, const double *band, const cd *x, cd *y, int nzero)
 {
-    const int N = p.N, W = 2 * K + 1;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    constexpr int W = 2 * K + 1, G = 4, PER = (W + G - 1) / G;
+    const int N = p.N, sub = threadIdx.x & (G - 1);
+    for (int i = threadIdx.x / G; i < N; i += blockDim.x / G) {
         double sx = 0.0, sy = 0.0;
         if (i >= nzero) {
-            int j0 = max(0, i - K), j1 = min(N - 1, i + K);
-            for (int jj = j0; jj <= j1; jj++) {
-                double a = band[(size_t)i * W + K + (jj - i)];
-                sx += a * x[jj].x;
-                sy += a * x[jj].y;
+            double b[PER];
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int d = sub + u * G, jj = i - K + d;
+                b[u] = (d < W && jj >= 0 && jj < N) ? band[(size_t)i * W + d] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int d = sub + u * G, jj = i - K + d;
+                if (d < W && jj >= 0 && jj < N) {
+                    const cd xv = x[jj];
+                    sx += b[u] * xv.x;
+                    sy += b[u] * xv.y;
+                }
             }
         }
-        y[i] = mk(sx, sy);
+        sx += __shfl_xor(sx, 1, G);
+        sy += __shfl_xor(sy, 1, G);
+        sx += __shfl_xor(sx, 2, G);
+        sy += __shfl_xor(sy, 2, G);
+        if (sub == 0) y[i] = mk(sx, sy);
     }
 }
-__device__ void bandmv_c(const PzDev &p, const cd *band, int K, const cd *x, cd *y, int nzero)
+template <int K>
+__device__ void bandmv_c(const PzDev &p, const cd *band, const cd *x, cd *y, int nzero)
 {
-    const int N = p.N, W = 2 * K + 1;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    constexpr int W = 2 * K + 1, G = 4, PER = (W + G - 1) / G;
+    const int N = p.N, sub = threadIdx.x & (G - 1);
+    for (int i = threadIdx.x / G; i < N; i += blockDim.x / G) {
         cd s = mk(0, 0);
         if (i >= nzero) {
-            int j0 = max(0, i - K), j1 = min(N - 1, i + K);
-            for (int jj = j0; jj <= j1; jj++) s = cadd(s, cmul(band[(size_t)i * W + K + (jj - i)], x[jj]));
+            cd b[PER];
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int d = sub + u * G, jj = i - K + d;
+                b[u] = (d < W && jj >= 0 && jj < N) ? band[(size_t)i * W + d] : mk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int d = sub + u * G, jj = i - K + d;
+                if (d < W && jj >= 0 && jj < N) s = cadd(s, cmul(b[u], x[jj]));
+            }
         }
-        y[i] = s;
+        s.x += __shfl_xor(s.x, 1, G);
+        s.y += __shfl_xor(s.y, 1, G);
+        s.x += __shfl_xor(s.x, 2, G);
+        s.y += __shfl_xor(s.y, 2, G);
+        if (sub == 0) y[i] = s;
     }
 }
 
@@ -1085,7 +1156,7 @@ __device__ void build_bordered(const PzDev &p, int nt, int kl, int ku, int K, co
 }
 
 // grid: (3, nm) blocks: field 0 = T, 1 = xi, 2 = psi; 64 threads
-__global__ void __launch_bounds__(64) k_build_lhs(PzDev p, int force)
+__global__ void __launch_bounds__(64) k_build_lhs(const PzDev p __attribute__((grid_constant)), int force)
 {
     extern __shared__ char lds[];
     if (!force && p.ss->lMat == 0) return;
@@ -1201,7 +1272,7 @@ __device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t byte
 // groups, each thread keeps 16 column loads in flight, and the partial sums
 // are combined in a fixed order (bitwise reproducible).  x and y may alias;
 // every thread of the block must call it.  red: LDS scratch of
-// 512 * NMAT complex values.
+// 2 * blockDim.x * NMAT complex values.
 // ---------------------------------------------------------------------------
 template <class E, int NMAT>
 __device__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y, cd *red)
@@ -1210,6 +1281,7 @@ __device__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, 
     const int R = (N + RPT - 1) / RPT;
     const int CG = min((int)blockDim.x / R, N);
     const int t = threadIdx.x, cg = t / R, ru = t - cg * R;
+    const size_t RS = 2 * (size_t)blockDim.x; // scratch per matrix
     cd acc[NMAT][RPT];
 #pragma unroll
     for (int a = 0; a < NMAT; a++)
@@ -1247,7 +1319,7 @@ __device__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, 
 #pragma unroll
             for (int a = 0; a < NMAT; a++)
 #pragma unroll
-                for (int r = 0; r < RPT; r++) red[(size_t)a * 512 + (cg - 1) * R * RPT + ru * RPT + r] = acc[a][r];
+                for (int r = 0; r < RPT; r++) red[(size_t)a * RS + (cg - 1) * R * RPT + ru * RPT + r] = acc[a][r];
     }
     __syncthreads();
     if (cg == 0) {
@@ -1256,7 +1328,7 @@ __device__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, 
 #pragma unroll
             for (int r = 0; r < RPT; r++) {
                 cd s = acc[a][r];
-                for (int c = 1; c < CG; c++) s = cadd(s, red[(size_t)a * 512 + (c - 1) * R * RPT + ru * RPT + r]);
+                for (int c = 1; c < CG; c++) s = cadd(s, red[(size_t)a * RS + (c - 1) * R * RPT + ru * RPT + r]);
                 const int row = ru * RPT + r;
                 if (row < N) y[a][row] = s;
             }
@@ -1288,7 +1360,7 @@ struct InvLayout {
 // factorised by k_build_lhs: wave w solves A y = fac_j e_j for the columns
 // j = w, w + nw, ... with the same bordered solve (algebra_lapack.f90:332-416)
 // as one time step would.  grid (nm, 3): field 0 = T, 1 = xi, 2 = psi.
-__global__ void __launch_bounds__(1024) k_build_inv(PzDev p, int force)
+__global__ void __launch_bounds__(1024) k_build_inv(const PzDev p __attribute__((grid_constant)), int force)
 {
     extern __shared__ char lds[];
     if (!force && p.ss->lMat == 0) return;

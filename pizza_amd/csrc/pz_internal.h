@@ -50,6 +50,7 @@ struct PzDev {
     const double *tcond, *dtcond, *xicond, *dxicond, *delxr2, *delxh2, *drx, *ddrx, *x;
     const double *cosT, *sinT; // cos/sin(pi q/(N-1)), q in [0, L)
     const double *drb;         // [3][N] bulk factors of the radial derivatives
+    const double *radc;        // block holding all radial rows above (PZ_NRADC x n_r)
     const double *tau;         // tau lines: [rMat(2,N) | drMat(2,N) | d2rMat(2,N)]
     const cd *twid;            // exp(-2 pi i k / n_phi)
     const int *idx2m;
@@ -92,6 +93,41 @@ struct PzDev {
     int ldi;
 
     PzScheme *ss;
+};
+
+// rows of the radial block: x, r, drx, ddrx, or1, or2, rgrav, beta, ekpump,
+// ekp_up, ekp_us, ekp_dusdp, delxr2, delxh2, tcond, dtcond, xicond, dxicond,
+// drb (3 rows)
+#define PZ_NRADC 21
+template <class D>
+__host__ __device__ inline void set_radial_pointers(D &d, const double *rc, size_t N)
+{
+    d.x = rc;
+    d.r = rc + N;
+    d.drx = rc + 2 * N;
+    d.ddrx = rc + 3 * N;
+    d.or1 = rc + 4 * N;
+    d.or2 = rc + 5 * N;
+    d.rgrav = rc + 6 * N;
+    d.beta = rc + 7 * N;
+    d.ekpump = rc + 8 * N;
+    d.ekp_up = rc + 9 * N;
+    d.ekp_us = rc + 10 * N;
+    d.ekp_dusdp = rc + 11 * N;
+    d.delxr2 = rc + 12 * N;
+    d.delxh2 = rc + 13 * N;
+    d.tcond = rc + 14 * N;
+    d.dtcond = rc + 15 * N;
+    d.xicond = rc + 16 * N;
+    d.dxicond = rc + 17 * N;
+    d.drb = rc + 18 * N;
+}
+
+// The radial rows of one block (global memory or a kernel's LDS copy).
+struct RadView {
+    const double *x, *r, *drx, *ddrx, *or1, *or2, *rgrav, *beta, *ekpump, *ekp_up, *ekp_us, *ekp_dusdp;
+    const double *delxr2, *delxh2, *tcond, *dtcond, *xicond, *dxicond, *drb;
+    __host__ __device__ RadView(const double *rc, int N) { set_radial_pointers(*this, rc, (size_t)N); }
 };
 
 // sizes of the bordered storage per m

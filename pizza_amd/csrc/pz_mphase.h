@@ -5,32 +5,39 @@
 #include "pz_device.h"
 
 // set_imex_rhs for one m row (multistep_schemes.f90:366-413;
-// dirk_schemes.f90:773-814).  cur: LDS copy of the current explicit slot,
-// used when use_cur (else the slot is read from global memory).
-__device__ __forceinline__ cd imex_rhs(const PzDev &p, const PzScheme *s, int istage, const cd *old,
-                                       const cd *impl, const cd *expl, const cd *cur, bool use_cur, size_t off,
-                                       int k)
+// dirk_schemes.f90:773-814), split around the current explicit slot: P sums
+// (in the reference's order) every term before it, Q every term after it, and the
+// right-hand side is (P + w cur) + Q with w = imex_weight().  Lets the
+// global-memory terms be loaded and summed before cur is known.
+__device__ __forceinline__ void imex_split(const PzDev &p, const PzScheme *s, int istage, const cd *old,
+                                           const cd *impl, const cd *expl, size_t off, int k, cd &P, cd &Q)
 {
     const size_t F = (size_t)p.nm * p.N;
     const int is = istage - 1;
-    cd r;
+    cd r, q = mk(0.0, 0.0);
     if (p.family == 0) {
         r = cscale(s->wimp[0], old[off + k]);
         for (int o = 1; o < p.nold; o++) r = cadd(r, cscale(s->wimp[o], old[o * F + off + k]));
         for (int o = 0; o < p.nimp; o++) r = cadd(r, cscale(s->wimp_lin[o + 1], impl[o * F + off + k]));
-        for (int o = 0; o < p.nexp; o++) {
-            cd e = (o == is && use_cur) ? lds_load(cur, k) : expl[o * F + off + k];
-            r = cadd(r, cscale(s->wexp[o], e));
+        for (int o = 0; o < is; o++) r = cadd(r, cscale(s->wexp[o], expl[o * F + off + k]));
+        for (int o = is + 1; o < p.nexp; o++) {
+            const cd t = cscale(s->wexp[o], expl[o * F + off + k]);
+            q = o == is + 1 ? t : cadd(q, t);
         }
     } else {
         r = old[off + k];
+        for (int st = 0; st < is; st++) r = cadd(r, cscale(s->bexp[istage][st], expl[st * F + off + k]));
         for (int st = 0; st <= is; st++) {
-            cd e = (st == is && use_cur) ? lds_load(cur, k) : expl[st * F + off + k];
-            r = cadd(r, cscale(s->bexp[istage][st], e));
+            const cd t = cscale(s->bimp[istage][st], impl[st * F + off + k]);
+            q = st == 0 ? t : cadd(q, t);
         }
-        for (int st = 0; st <= is; st++) r = cadd(r, cscale(s->bimp[istage][st], impl[st * F + off + k]));
     }
-    return r;
+    P = r;
+    Q = q;
+}
+__device__ __forceinline__ double imex_weight(const PzDev &p, const PzScheme *s, int istage)
+{
+    return p.family == 0 ? s->wexp[istage - 1] : s->bexp[istage][istage - 1];
 }
 
 // rotate_imex for one m row (multistep_schemes.f90:462-498); DIRK: no-op
@@ -46,16 +53,16 @@ __device__ void rotate_row(const PzDev &p, cd *old, cd *impl, cd *expl, size_t o
 }
 
 // -BuoFac rgrav/r i m T - ChemFac rgrav/r i m X (update_psi_integ_smat.f90:286-304)
-__device__ __forceinline__ cd buoyancy(const PzDev &p, int m, int k, cd T, cd X)
+__device__ __forceinline__ cd buoyancy(const PzDev &p, const RadView &R, int m, int k, cd T, cd X)
 {
     cd b = mk(0, 0);
     if (m == 0) return b;
     if (p.l_heat) {
-        double s = -p.BuoFac * p.rgrav[k] * p.or1[k];
+        double s = -p.BuoFac * R.rgrav[k] * R.or1[k];
         b = cmul(mk(0.0, s * (double)m), T);
-        if (p.l_chem) b = csub(b, cmul(mk(0.0, p.ChemFac * p.rgrav[k] * p.or1[k] * (double)m), X));
+        if (p.l_chem) b = csub(b, cmul(mk(0.0, p.ChemFac * R.rgrav[k] * R.or1[k] * (double)m), X));
     } else if (p.l_chem) {
-        double s = -p.ChemFac * p.rgrav[k] * p.or1[k];
+        double s = -p.ChemFac * R.rgrav[k] * R.or1[k];
         b = cmul(mk(0.0, s * (double)m), X);
     }
     return b;
@@ -75,12 +82,13 @@ __device__ void dcheb_real(const PzDev &p, const double *f, double *df)
     for (int n = N - 2; n >= 1; n--) df[n - 1] = fabs(f[n]) >= thr ? df[n + 1] + (double)(2 * n) * f[n] : 0.0;
 }
 
-#define MPHASE_NVEC 18
+#define MPHASE_NVEC 24
+#define PZ_MPHASE_THREADS 512
 
 // LDS layout of k_mphase: working vectors, transform tables and the
 // reduction scratch of the dense solves (16-byte aligned blocks).
 struct MpLayout {
-    size_t vec, dtmp, cosT, sinT, red, total;
+    size_t vec, dtmp, cosT, sinT, rad, red, total;
     __host__ __device__ MpLayout(int N)
     {
         size_t o = 0;
@@ -90,9 +98,10 @@ struct MpLayout {
         dtmp = take(sizeof(double) * 2 * N);
         cosT = take(sizeof(double) * L);
         sinT = take(sizeof(double) * L);
+        rad = take(sizeof(double) * PZ_NRADC * N);
         // FFT scratch of the Chebyshev transforms (6 L + 12), shared with the
-        // reduction scratch of the dense solves (2 x 512)
-        red = take(sizeof(cd) * (6 * L + 16 > 1024 ? 6 * L + 16 : 1024));
+        // reduction scratch of the dense solves (2 matrices x 2 rows x 512 threads)
+        red = take(sizeof(cd) * (6 * L + 16 > 2048 ? 6 * L + 16 : 2048));
         total = o;
     }
 };
@@ -102,8 +111,11 @@ struct MpLayout {
 // finish_explicit_assembly (mloop.f90:169-229) when exp_calc, then
 // update_temp_int / update_xi_int (update_temp_integ.f90:179-298) and
 // update_psi_int_smat (update_psi_integ_smat.f90:247-573).
-__global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_calc)
+__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__restrict__ pd, int istage, int exp_calc)
 {
+    // the descriptor is read through a pointer (scalar loads): a by-value
+    // copy of this size would be materialised in scratch memory
+    const PzDev &p = *pd;
     extern __shared__ double smem[];
     const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi];
     const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
@@ -118,29 +130,44 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     cd *aP = V + 6 * N, *cP = V + 7 * N, *eP = V + 8 * N;
     cd *psiv = V + 9 * N, *omv = V + 10 * N, *upv = V + 11 * N, *usv = V + 12 * N;
     cd *Tv = V + 13 * N, *Xv = V + 14 * N, *bv = V + 15 * N, *b2 = V + 16 * N, *wv = V + 17 * N;
+    cd *PT = V + 18 * N, *QT = V + 19 * N, *PX = V + 20 * N, *QX = V + 21 * N, *PP = V + 22 * N, *QP = V + 23 * N;
     double *dtmp = (double *)(lb + lay.dtmp);
     double *cosT = (double *)(lb + lay.cosT), *sinT = (double *)(lb + lay.sinT);
     const bool hT = p.l_heat, hX = p.l_chem;
     cd *red = (cd *)(lb + lay.red), *fbuf = red;
     const int ldi = p.ldi;
-    // stage the transform tables into LDS
+    // stage the transform tables and the radial functions into LDS (the
+    // first barrier below orders them before their first use)
     lds_copy(cosT, p.cosT, sizeof(double) * p.L);
     lds_copy(sinT, p.sinT, sizeof(double) * p.L);
-    __syncthreads();
+    lds_copy(lb + lay.rad, p.radc, sizeof(double) * PZ_NRADC * N);
+    const RadView R((const double *)(lb + lay.rad), N);
 
-    // ---------------- explicit assembly -----------------------------------
-    if (exp_calc) {
-        for (int k = tid; k < N; k += blockDim.x) {
-            if (hT) { aT[k] = p.dVsT[off + k]; eT[k] = p.T_expl[is * F + off + k]; }
-            if (hX) { aX[k] = p.dVsXi[off + k]; eX[k] = p.X_expl[is * F + off + k]; }
+    // ---------------- global inputs of this m in one phase ----------------
+    // explicit-assembly inputs (or the current explicit slot) and the
+    // right-hand-side terms that do not depend on this stage's explicit part
+    for (int k = tid; k < N; k += blockDim.x) {
+        if (hT) eT[k] = p.T_expl[is * F + off + k];
+        if (hX) eX[k] = p.X_expl[is * F + off + k];
+        eP[k] = p.P_expl[is * F + off + k];
+        if (exp_calc) {
+            if (hT) aT[k] = p.dVsT[off + k];
+            if (hX) aX[k] = p.dVsXi[off + k];
             aP[k] = p.dVsOm[off + k];
-            eP[k] = p.P_expl[is * F + off + k];
             psiv[k] = p.psi[off + k];
             omv[k] = p.om[off + k];
             upv[k] = p.up[off + k];
             usv[k] = p.us[off + k];
             if (!p.l_buo_imp) { Tv[k] = hT ? p.temp[off + k] : mk(0, 0); Xv[k] = hX ? p.xi[off + k] : mk(0, 0); }
         }
+        if (hT) imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, PT[k], QT[k]);
+        if (hX) imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, PX[k], QX[k]);
+        imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, PP[k], QP[k]);
+    }
+    const double wcur = imex_weight(p, ss, istage);
+
+    // ---------------- explicit assembly -----------------------------------
+    if (exp_calc) {
         __syncthreads();
         VecSet<const cd> din;
         VecSet<cd> dout;
@@ -156,21 +183,21 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
             if (hT) { c3.push(cT); d3.push(aT); }
             if (hX) { c3.push(cX); d3.push(aX); }
             c3.push(cP); d3.push(aP);
-            dr_fft(p, c3, d3, VecSet<cd>(), c3.n, cosT, sinT, fbuf);
+            dr_fft(p, R, c3, d3, VecSet<cd>(), c3.n, cosT, sinT, fbuf);
         }
         __syncthreads();
         const double rc2 = p.r_cmb * p.r_cmb;
         for (int k = tid; k < N; k += blockDim.x) {
-            const double rk = p.r[k], h2 = rc2 - rk * rk;
+            const double rk = R.r[k], h2 = rc2 - rk * rk;
             if (hT) {
                 // finish_exp_temp_int (update_temp_integ.f90:420-438)
-                double s = h2 * p.dtcond[k] + p.tadvz * rk * p.tcond[k];
+                double s = h2 * R.dtcond[k] + p.tadvz * rk * R.tcond[k];
                 cd t = cmul(mk(0.0, (double)m * s), psiv[k]);
                 cd e = csub(cscale(rk, eT[k]), t);
                 eT[k] = csub(e, aT[k]);
             }
             if (hX) {
-                cd t = cmul(mk(0.0, (double)m * h2 * p.dxicond[k]), psiv[k]);
+                cd t = cmul(mk(0.0, (double)m * h2 * R.dxicond[k]), psiv[k]);
                 cd e = csub(cscale(rk, eX[k]), t);
                 eX[k] = csub(e, aX[k]);
             }
@@ -180,17 +207,17 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
                 e = cscale(rk, e);
                 if (!p.l_coriolis_imp) e = csub(e, cmul(mk(0.0, p.CorFac * (double)m * rk), psiv[k]));
                 if (!p.l_buo_imp) {
-                    if (hT) e = csub(e, cmul(mk(0.0, p.BuoFac * p.rgrav[k] * (double)m), Tv[k]));
-                    if (hX) e = csub(e, cmul(mk(0.0, p.ChemFac * p.rgrav[k] * (double)m), Xv[k]));
+                    if (hT) e = csub(e, cmul(mk(0.0, p.BuoFac * R.rgrav[k] * (double)m), Tv[k]));
+                    if (hX) e = csub(e, cmul(mk(0.0, p.ChemFac * R.rgrav[k] * (double)m), Xv[k]));
                 }
             }
             if (p.l_ek_pump) {
-                double ekp = p.CorFac * p.ekpump[k];
+                double ekp = p.CorFac * R.ekpump[k];
                 if (m == 0) {
                     e = csub(e, cscale(ekp, upv[k]));
                 } else {
-                    cd t = cadd(omv[k], cscale(p.ekp_up[k], upv[k]));
-                    t = cadd(t, cmul(mk(p.ekp_us[k], (double)m * p.ekp_dusdp[k]), usv[k]));
+                    cd t = cadd(omv[k], cscale(R.ekp_up[k], upv[k]));
+                    t = cadd(t, cmul(mk(R.ekp_us[k], (double)m * R.ekp_dusdp[k]), usv[k]));
                     e = csub(e, cscale(ekp * rk, t));
                 }
             }
@@ -205,10 +232,10 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         din.push(eP); dout.push(cP);
         dct_fft(p, din, dout, din.n, 1, cosT, sinT, fbuf);
         __syncthreads();
-        if (hT) bandmv(p, p.T_RHSE1, 3, cT, eT, 2);
-        if (hX) bandmv(p, p.T_RHSE1, 3, cX, eX, 2);
-        if (m == 0) bandmv(p, p.P_RHSE1, 4, cP, eP, 2);
-        else bandmv(p, p.P_RHSE3, 7, cP, eP, 4);
+        if (hT) bandmv<3>(p, p.T_RHSE1, cT, eT, 2);
+        if (hX) bandmv<3>(p, p.T_RHSE1, cX, eX, 2);
+        if (m == 0) bandmv<4>(p, p.P_RHSE1, cP, eP, 2);
+        else bandmv<7>(p, p.P_RHSE3, cP, eP, 4);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             if (hT) p.T_expl[is * F + off + k] = eT[k];
@@ -220,13 +247,13 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     // ---------------- T and xi: rhs, bordered solve ------------------------
     for (int k = tid; k < N; k += blockDim.x) {
         if (hT) {
-            cd r = imex_rhs(p, ss, istage, p.T_old, p.T_impl, p.T_expl, eT, exp_calc != 0, off, k);
+            cd r = cadd(cadd(PT[k], cscale(wcur, eT[k])), QT[k]);
             if (k == 0) r = p.topt[mi];
             if (k == 1) r = p.bott[mi];
             aT[k] = r;
         }
         if (hX) {
-            cd r = imex_rhs(p, ss, istage, p.X_old, p.X_impl, p.X_expl, eX, exp_calc != 0, off, k);
+            cd r = cadd(cadd(PX[k], cscale(wcur, eX[k])), QX[k]);
             if (k == 0) r = p.topxi[mi];
             if (k == 1) r = p.botxi[mi];
             aX[k] = r;
@@ -262,12 +289,12 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     }
     // get_temp_rhs_imp_int (update_temp_integ.f90:471-536)
     if (inext == 1) {
-        if (hT) bandmv(p, p.T_RHSE2, 4, aT, p.T_old + off, 2);
-        if (hX) bandmv(p, p.T_RHSE2, 4, aX, p.X_old + off, 2);
+        if (hT) bandmv<4>(p, p.T_RHSE2, aT, p.T_old + off, 2);
+        if (hX) bandmv<4>(p, p.T_RHSE2, aX, p.X_old + off, 2);
     }
     if (lcalc) {
-        if (hT) bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, aT, cT, 2);
-        if (hX) bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, aX, cX, 2);
+        if (hT) bandmv<2>(p, p.T_RHSI + (size_t)mi * N * 5, aT, cT, 2);
+        if (hX) bandmv<2>(p, p.T_RHSI + (size_t)mi * N * 5, aX, cX, 2);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             if (hT) p.T_impl[(inext - 1) * F + off + k] = cscale(p.TdiffFac * p.hdifT[mi], cT[k]);
@@ -279,18 +306,18 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     // ---------------- psi -------------------------------------------------
     if (p.l_buo_imp) {
         for (int k = tid; k < N; k += blockDim.x)
-            bv[k] = buoyancy(p, m, k, hT ? Tv[k] : mk(0, 0), hX ? Xv[k] : mk(0, 0));
+            bv[k] = buoyancy(p, R, m, k, hT ? Tv[k] : mk(0, 0), hX ? Xv[k] : mk(0, 0));
         __syncthreads();
         VecSet<const cd> din(bv);
         VecSet<cd> dout(wv);
         dct_fft(p, din, dout, 1, 0, cosT, sinT, fbuf);
         __syncthreads();
-        if (m > 0) bandmv(p, p.P_RHSE2, 8, wv, b2, 4);
+        if (m > 0) bandmv<8>(p, p.P_RHSE2, wv, b2, 4);
         __syncthreads();
     }
     const double wl = ss->wimp_lin[0];
     for (int k = tid; k < N; k += blockDim.x) {
-        cd r = imex_rhs(p, ss, istage, p.P_old, p.P_impl, p.P_expl, eP, exp_calc != 0, off, k);
+        cd r = cadd(cadd(PP[k], cscale(wcur, eP[k])), QP[k]);
         if (p.l_buo_imp && m != 0) r = cadd(r, cscale(wl, b2[k]));
         aP[k] = r;
     }
@@ -315,10 +342,10 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
         dct_fft(p, din, dout, 2, 0, cosT, sinT, fbuf);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
-            double u0 = upv[k].x, om0 = p.drx[k] * omv[k].x;
+            double u0 = upv[k].x, om0 = R.drx[k] * omv[k].x;
             p.us[off + k] = mk(0, 0);
             p.up[off + k] = mk(u0, 0.0);
-            p.om[off + k] = mk(om0 + p.or1[k] * u0, 0.0);
+            p.om[off + k] = mk(om0 + R.or1[k] * u0, 0.0);
             eP[k] = mk(u0, 0.0);
         }
         __syncthreads();
@@ -331,21 +358,18 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     } else {
         VecSet<const cd> c1(aP);
         VecSet<cd> d1(cP), d2(eP);
-        dr_fft(p, c1, d1, d2, 1, cosT, sinT, fbuf);
-        __syncthreads();
-        VecSet<const cd> din(aP);
-        VecSet<cd> dout(psiv);
-        dct_fft(p, din, dout, 1, 0, cosT, sinT, fbuf);
+        // derivatives and grid values of psi from one FFT pass
+        dr_fft(p, R, c1, d1, d2, 1, cosT, sinT, fbuf, psiv);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
-            const double rk = p.r[k], h2 = rc2 - rk * rk, or1 = p.or1[k];
+            const double rk = R.r[k], h2 = rc2 - rk * rk, or1 = R.or1[k];
             cd ps = psiv[k], dps = cP[k], d2ps = eP[k];
             p.psi_hat[off + k] = aP[k];
             p.psi[off + k] = ps;
             p.us[off + k] = cmul(mk(0.0, (double)m * or1 * h2), ps);
             p.up[off + k] = cadd(cscale(-h2, dps), cscale(3.0 * rk, ps));
             cd o = csub(cscale(-h2, d2ps), cscale(rc2 * or1 - 6.0 * rk, dps));
-            p.om[off + k] = cadd(o, cscale(6.0 + (double)m * (double)m * p.or2[k] * h2, ps));
+            p.om[off + k] = cadd(o, cscale(6.0 + (double)m * (double)m * R.or2[k] * h2, ps));
             wv[k] = aP[k];
         }
     }
@@ -354,12 +378,12 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
     __syncthreads();
     // get_psi_rhs_imp_int_smat (update_psi_integ_smat.f90:960-1115)
     if (inext == 1) {
-        if (m == 0) bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 4, wv, p.P_old + off, 2);
-        else bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 8, wv, p.P_old + off, 4);
+        if (m == 0) bandmv<4>(p, p.P_RHSI + (size_t)mi * N * 17, wv, p.P_old + off, 2);
+        else bandmv<8>(p, p.P_RHSI + (size_t)mi * N * 17, wv, p.P_old + off, 4);
     }
     if (lcalc) {
-        if (m == 0) bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 2, wv, cT, 2);
-        else bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 8, wv, cT, 4);
+        if (m == 0) bandmv_c<2>(p, p.P_RHSIL + (size_t)mi * N * 17, wv, cT, 2);
+        else bandmv_c<8>(p, p.P_RHSIL + (size_t)mi * N * 17, wv, cT, 4);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             cd v = cT[k];
@@ -371,7 +395,7 @@ __global__ void __launch_bounds__(256) k_mphase(PzDev p, int istage, int exp_cal
 
 // get_start_fields, fresh start of the integration method
 // (init_fields.f90:186-255): dT/dr, omega, hats and the first implicit state.
-__global__ void __launch_bounds__(256) k_start(PzDev p)
+__global__ void __launch_bounds__(256) k_start(const PzDev p __attribute__((grid_constant)))
 {
     extern __shared__ double smem[];
     const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi];
@@ -400,7 +424,7 @@ __global__ void __launch_bounds__(256) k_start(PzDev p)
     {
         VecSet<const cd> c2(cT, cU);
         VecSet<cd> d2(dT, dU);
-        dr_many(p, c2, d2, VecSet<cd>(), 2);
+        dr_many(p, RadView(p.radc, p.N), c2, d2, VecSet<cd>(), 2);
     }
     __syncthreads();
     for (int k = tid; k < N; k += blockDim.x) {
@@ -412,15 +436,15 @@ __global__ void __launch_bounds__(256) k_start(PzDev p)
     }
     // scalar implicit states (istage = 1, l_calc_lin = .true.)
     if (hT) {
-        bandmv(p, p.T_RHSE2, 4, cT, p.T_old + off, 2);
-        bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, cT, t1, 2);
+        bandmv<4>(p, p.T_RHSE2, cT, p.T_old + off, 2);
+        bandmv<2>(p, p.T_RHSI + (size_t)mi * N * 5, cT, t1, 2);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) p.T_impl[off + k] = cscale(p.TdiffFac * p.hdifT[mi], t1[k]);
         __syncthreads();
     }
     if (hX) {
-        bandmv(p, p.T_RHSE2, 4, cX, p.X_old + off, 2);
-        bandmv(p, p.T_RHSI + (size_t)mi * N * 5, 2, cX, t1, 2);
+        bandmv<4>(p, p.T_RHSE2, cX, p.X_old + off, 2);
+        bandmv<2>(p, p.T_RHSI + (size_t)mi * N * 5, cX, t1, 2);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) p.X_impl[off + k] = cscale(p.XidiffFac * p.hdifXi[mi], t1[k]);
         __syncthreads();
@@ -436,19 +460,19 @@ __global__ void __launch_bounds__(256) k_start(PzDev p)
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) wv[k] = mk(wv[k].x, 0.0);
         __syncthreads();
-        bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 4, wv, p.P_old + off, 2);
-        bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 2, wv, p.P_impl + off, 2);
+        bandmv<4>(p, p.P_RHSI + (size_t)mi * N * 17, wv, p.P_old + off, 2);
+        bandmv_c<2>(p, p.P_RHSIL + (size_t)mi * N * 17, wv, p.P_impl + off, 2);
     } else {
-        bandmv(p, p.P_RHSI + (size_t)mi * N * 17, 8, cPs, p.P_old + off, 4);
-        bandmv_c(p, p.P_RHSIL + (size_t)mi * N * 17, 8, cPs, t1, 4);
+        bandmv<8>(p, p.P_RHSI + (size_t)mi * N * 17, cPs, p.P_old + off, 4);
+        bandmv_c<8>(p, p.P_RHSIL + (size_t)mi * N * 17, cPs, t1, 4);
         if (p.l_buo_imp) {
-            for (int k = tid; k < N; k += blockDim.x) bv[k] = buoyancy(p, m, k, Tv[k], psv[k]);
+            for (int k = tid; k < N; k += blockDim.x) bv[k] = buoyancy(p, RadView(p.radc, p.N), m, k, Tv[k], psv[k]);
             __syncthreads();
             VecSet<const cd> di(bv);
             VecSet<cd> dd(wv);
             dct_many(p, di, dd, 1, 0);
             __syncthreads();
-            bandmv(p, p.P_RHSE2, 8, wv, b2, 4);
+            bandmv<8>(p, p.P_RHSE2, wv, b2, 4);
         }
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
@@ -461,7 +485,7 @@ __global__ void __launch_bounds__(256) k_start(PzDev p)
 
 // radial profiles for the kinetic energies (outputs.f90:458-467): per r,
 // sum over m of cc2real(us), cc2real(up) and the m = 0 part of up
-__global__ void k_ekin_profiles(PzDev p, double *out)
+__global__ void k_ekin_profiles(const PzDev p __attribute__((grid_constant)), double *out)
 {
     const int nr = blockIdx.x * blockDim.x + threadIdx.x;
     if (nr >= p.N) return;
@@ -481,7 +505,7 @@ __global__ void k_ekin_profiles(PzDev p, double *out)
 }
 
 // ---- unit-operator kernels on [vec][n_r] device arrays ----
-__global__ void __launch_bounds__(256) k_costf_vecs(PzDev p, cd *f)
+__global__ void __launch_bounds__(256) k_costf_vecs(const PzDev p __attribute__((grid_constant)), cd *f)
 {
     extern __shared__ double smem[];
     cd *a = (cd *)smem, *b = a + p.N, *buf = b + p.N;
@@ -495,7 +519,7 @@ __global__ void __launch_bounds__(256) k_costf_vecs(PzDev p, cd *f)
     for (int k = threadIdx.x; k < p.N; k += blockDim.x) x[k] = b[k];
 }
 
-__global__ void __launch_bounds__(256) k_dr_vecs(PzDev p, const cd *f, cd *df, cd *ddf, int l_dct_in)
+__global__ void __launch_bounds__(256) k_dr_vecs(const PzDev p __attribute__((grid_constant)), const cd *f, cd *df, cd *ddf, int l_dct_in)
 {
     extern __shared__ double smem[];
     cd *a = (cd *)smem, *c = a + p.N, *d1 = c + p.N, *d2 = d1 + p.N, *buf = d2 + p.N;
@@ -512,7 +536,7 @@ __global__ void __launch_bounds__(256) k_dr_vecs(PzDev p, const cd *f, cd *df, c
     __syncthreads();
     VecSet<const cd> cc(c);
     VecSet<cd> o1(d1), o2(d2);
-    dr_fft(p, cc, o1, ddf ? o2 : VecSet<cd>(), 1, p.cosT, p.sinT, buf);
+    dr_fft(p, RadView(p.radc, p.N), cc, o1, ddf ? o2 : VecSet<cd>(), 1, p.cosT, p.sinT, buf);
     __syncthreads();
     for (int k = threadIdx.x; k < p.N; k += blockDim.x) {
         df[off + k] = d1[k];
@@ -522,7 +546,7 @@ __global__ void __launch_bounds__(256) k_dr_vecs(PzDev p, const cd *f, cd *df, c
 
 // azimuthal transforms, one radius per block: grid real [rad][n_phi],
 // spectral [rad][nm]
-__global__ void __launch_bounds__(256) k_phi_fft(PzDev p, const double *grid, cd *spec)
+__global__ void __launch_bounds__(256) k_phi_fft(const PzDev p __attribute__((grid_constant)), const double *grid, cd *spec)
 {
     extern __shared__ double smem[];
     const int n = p.nphi, nm = p.nm;
@@ -534,7 +558,7 @@ __global__ void __launch_bounds__(256) k_phi_fft(PzDev p, const double *grid, cd
     for (int i = threadIdx.x; i < nm; i += blockDim.x) spec[(size_t)blockIdx.x * nm + i] = cdivr(Z[i], (double)n);
 }
 
-__global__ void __launch_bounds__(256) k_phi_ifft(PzDev p, const cd *spec, double *grid)
+__global__ void __launch_bounds__(256) k_phi_ifft(const PzDev p __attribute__((grid_constant)), const cd *spec, double *grid)
 {
     extern __shared__ double smem[];
     const int n = p.nphi, nm = p.nm;
