@@ -700,7 +700,7 @@ __device__ __forceinline__ void dr_bulk(const PzDev &p, const RadView &R, cd wl,
     const double icf = p.icheb_fac;
     vx *= icf; vy *= icf; ux *= icf; uy *= icf;
     const double V1x = -2.0 * vx, V1y = -2.0 * vy;
-    const double g1 = R.drb[j], g2 = R.drb[N + j], g3 = R.drb[2 * N + j];
+    const double g1 = R.drb(j), g2 = R.drb(N + j), g3 = R.drb(2 * N + j);
     d1 = mk(-V1x * g1, -V1y * g1);
     d2 = mk(0, 0);
     if (second) {
@@ -714,9 +714,9 @@ __device__ __forceinline__ void dr_bulk(const PzDev &p, const RadView &R, cd wl,
 
 __device__ __forceinline__ void dr_store(const PzDev &p, const RadView &R, int j, cd d1, cd d2, bool second, cd *df, cd *ddf)
 {
-    const double a = R.drx[j];
+    const double a = R.drx(j);
     if (second) {
-        const double b = R.ddrx[j];
+        const double b = R.ddrx(j);
         lds_store(ddf, j, mk(b * d1.x + a * a * d2.x, b * d1.y + a * a * d2.y));
     }
     lds_store(df, j, mk(a * d1.x, a * d1.y));

@@ -123,11 +123,33 @@ __host__ __device__ inline void set_radial_pointers(D &d, const double *rc, size
     d.drb = rc + 18 * N;
 }
 
-// The radial rows of one block (global memory or a kernel's LDS copy).
+// The radial rows of one block (global memory or a kernel's LDS copy):
+// one base pointer, rows addressed in the order of set_radial_pointers
+// (a struct of nineteen pointers would hold 38 scalar registers).
 struct RadView {
-    const double *x, *r, *drx, *ddrx, *or1, *or2, *rgrav, *beta, *ekpump, *ekp_up, *ekp_us, *ekp_dusdp;
-    const double *delxr2, *delxh2, *tcond, *dtcond, *xicond, *dxicond, *drb;
-    __host__ __device__ RadView(const double *rc, int N) { set_radial_pointers(*this, rc, (size_t)N); }
+    const double *b;
+    int N;
+    __host__ __device__ RadView(const double *rc, int n) : b(rc), N(n) {}
+    __host__ __device__ double row(int i, int k) const { return b[(size_t)i * N + k]; }
+    __host__ __device__ double x(int k) const { return row(0, k); }
+    __host__ __device__ double r(int k) const { return row(1, k); }
+    __host__ __device__ double drx(int k) const { return row(2, k); }
+    __host__ __device__ double ddrx(int k) const { return row(3, k); }
+    __host__ __device__ double or1(int k) const { return row(4, k); }
+    __host__ __device__ double or2(int k) const { return row(5, k); }
+    __host__ __device__ double rgrav(int k) const { return row(6, k); }
+    __host__ __device__ double beta(int k) const { return row(7, k); }
+    __host__ __device__ double ekpump(int k) const { return row(8, k); }
+    __host__ __device__ double ekp_up(int k) const { return row(9, k); }
+    __host__ __device__ double ekp_us(int k) const { return row(10, k); }
+    __host__ __device__ double ekp_dusdp(int k) const { return row(11, k); }
+    __host__ __device__ double delxr2(int k) const { return row(12, k); }
+    __host__ __device__ double delxh2(int k) const { return row(13, k); }
+    __host__ __device__ double tcond(int k) const { return row(14, k); }
+    __host__ __device__ double dtcond(int k) const { return row(15, k); }
+    __host__ __device__ double xicond(int k) const { return row(16, k); }
+    __host__ __device__ double dxicond(int k) const { return row(17, k); }
+    __host__ __device__ double drb(int k) const { return row(18, k); } // k in [0, 3N)
 };
 
 // sizes of the bordered storage per m

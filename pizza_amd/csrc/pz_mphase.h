@@ -58,11 +58,11 @@ __device__ __forceinline__ cd buoyancy(const PzDev &p, const RadView &R, int m, 
     cd b = mk(0, 0);
     if (m == 0) return b;
     if (p.l_heat) {
-        double s = -p.BuoFac * R.rgrav[k] * R.or1[k];
+        double s = -p.BuoFac * R.rgrav(k) * R.or1(k);
         b = cmul(mk(0.0, s * (double)m), T);
-        if (p.l_chem) b = csub(b, cmul(mk(0.0, p.ChemFac * R.rgrav[k] * R.or1[k] * (double)m), X));
+        if (p.l_chem) b = csub(b, cmul(mk(0.0, p.ChemFac * R.rgrav(k) * R.or1(k) * (double)m), X));
     } else if (p.l_chem) {
-        double s = -p.ChemFac * R.rgrav[k] * R.or1[k];
+        double s = -p.ChemFac * R.rgrav(k) * R.or1(k);
         b = cmul(mk(0.0, s * (double)m), X);
     }
     return b;
@@ -188,16 +188,16 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
         __syncthreads();
         const double rc2 = p.r_cmb * p.r_cmb;
         for (int k = tid; k < N; k += blockDim.x) {
-            const double rk = R.r[k], h2 = rc2 - rk * rk;
+            const double rk = R.r(k), h2 = rc2 - rk * rk;
             if (hT) {
                 // finish_exp_temp_int (update_temp_integ.f90:420-438)
-                double s = h2 * R.dtcond[k] + p.tadvz * rk * R.tcond[k];
+                double s = h2 * R.dtcond(k) + p.tadvz * rk * R.tcond(k);
                 cd t = cmul(mk(0.0, (double)m * s), psiv[k]);
                 cd e = csub(cscale(rk, eT[k]), t);
                 eT[k] = csub(e, aT[k]);
             }
             if (hX) {
-                cd t = cmul(mk(0.0, (double)m * h2 * R.dxicond[k]), psiv[k]);
+                cd t = cmul(mk(0.0, (double)m * h2 * R.dxicond(k)), psiv[k]);
                 cd e = csub(cscale(rk, eX[k]), t);
                 eX[k] = csub(e, aX[k]);
             }
@@ -207,17 +207,17 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
                 e = cscale(rk, e);
                 if (!p.l_coriolis_imp) e = csub(e, cmul(mk(0.0, p.CorFac * (double)m * rk), psiv[k]));
                 if (!p.l_buo_imp) {
-                    if (hT) e = csub(e, cmul(mk(0.0, p.BuoFac * R.rgrav[k] * (double)m), Tv[k]));
-                    if (hX) e = csub(e, cmul(mk(0.0, p.ChemFac * R.rgrav[k] * (double)m), Xv[k]));
+                    if (hT) e = csub(e, cmul(mk(0.0, p.BuoFac * R.rgrav(k) * (double)m), Tv[k]));
+                    if (hX) e = csub(e, cmul(mk(0.0, p.ChemFac * R.rgrav(k) * (double)m), Xv[k]));
                 }
             }
             if (p.l_ek_pump) {
-                double ekp = p.CorFac * R.ekpump[k];
+                double ekp = p.CorFac * R.ekpump(k);
                 if (m == 0) {
                     e = csub(e, cscale(ekp, upv[k]));
                 } else {
-                    cd t = cadd(omv[k], cscale(R.ekp_up[k], upv[k]));
-                    t = cadd(t, cmul(mk(R.ekp_us[k], (double)m * R.ekp_dusdp[k]), usv[k]));
+                    cd t = cadd(omv[k], cscale(R.ekp_up(k), upv[k]));
+                    t = cadd(t, cmul(mk(R.ekp_us(k), (double)m * R.ekp_dusdp(k)), usv[k]));
                     e = csub(e, cscale(ekp * rk, t));
                 }
             }
@@ -342,10 +342,10 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
         dct_fft(p, din, dout, 2, 0, cosT, sinT, fbuf);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
-            double u0 = upv[k].x, om0 = R.drx[k] * omv[k].x;
+            double u0 = upv[k].x, om0 = R.drx(k) * omv[k].x;
             p.us[off + k] = mk(0, 0);
             p.up[off + k] = mk(u0, 0.0);
-            p.om[off + k] = mk(om0 + R.or1[k] * u0, 0.0);
+            p.om[off + k] = mk(om0 + R.or1(k) * u0, 0.0);
             eP[k] = mk(u0, 0.0);
         }
         __syncthreads();
@@ -362,14 +362,14 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
         dr_fft(p, R, c1, d1, d2, 1, cosT, sinT, fbuf, psiv);
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
-            const double rk = R.r[k], h2 = rc2 - rk * rk, or1 = R.or1[k];
+            const double rk = R.r(k), h2 = rc2 - rk * rk, or1 = R.or1(k);
             cd ps = psiv[k], dps = cP[k], d2ps = eP[k];
             p.psi_hat[off + k] = aP[k];
             p.psi[off + k] = ps;
             p.us[off + k] = cmul(mk(0.0, (double)m * or1 * h2), ps);
             p.up[off + k] = cadd(cscale(-h2, dps), cscale(3.0 * rk, ps));
             cd o = csub(cscale(-h2, d2ps), cscale(rc2 * or1 - 6.0 * rk, dps));
-            p.om[off + k] = cadd(o, cscale(6.0 + (double)m * (double)m * R.or2[k] * h2, ps));
+            p.om[off + k] = cadd(o, cscale(6.0 + (double)m * (double)m * R.or2(k) * h2, ps));
             wv[k] = aP[k];
         }
     }
