@@ -25,6 +25,8 @@ from pizza_amd import workloads as W  # noqa: E402
 from pizza_amd.runtime import Run  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+PROFILE = os.path.join(HERE, "profiles", "k_mphase_integ_bpr353_nr97_nphi384.md")
+COPY_BYTES = 64 << 20  # calibration copy of the counter runs
 
 
 def mphase_bytes(cfg, nstage_slots=1):
@@ -53,6 +55,40 @@ def mphase_bytes(cfg, nstage_slots=1):
     return nm * (reads + writes)
 
 
+def measured_traffic():
+    """HBM bytes per k_mphase launch from the committed counter summary:
+    FETCH_SIZE and WRITE_SIZE of the kernel, each scaled by the same counter
+    read for a plain device copy of COPY_BYTES (rocprofv3 --pmc runs of
+    --pmc-sample).  None when no summary is present."""
+    if not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        line = line.strip()
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def pmc_sample():
+    """Short sample for the counter runs: 5 k_mphase launches of the C2
+    workload after two steps, then 3 plain copies of COPY_BYTES."""
+    import torch
+    cfg = W.c2_production()
+    run = Run(cfg)
+    W.seed_run(run, cfg)
+    run.start()
+    run.step(2)
+    run.time()
+    run.time_kernel(0, 5)
+    x = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    run.close()
+    print(json.dumps({"pmc_sample": "k_mphase x5 (stage 1, C2)", "copy_bytes": COPY_BYTES, "copies": 3}))
+
+
 def cpu_baseline(cfg, seconds=15.0):
     """Oracle (single-thread C restatement) steps/s on a bounded sample."""
     from oracle.oracle import Oracle
@@ -77,12 +113,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--pmc-sample", action="store_true",
+                    help="run only the short sample profiled by the counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs each simulation on a single "
               f"MI355X (requested --gpus {args.gpus}); ensembles shard whole runs per GPU.",
               file=sys.stderr)
         sys.exit(2)
+    if args.pmc_sample:
+        pmc_sample()
+        return
 
     cfg = W.c2_production()
     run = Run(cfg)
@@ -128,7 +169,7 @@ def main():
                    "time_scheme": "BPR353", "sim_time_end": t_end, "dt": dt},
         "roofline": {"bound": "hbm", "kernel": "k_mphase (m-loop stage kernel)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms},
     }
     if not args.no_cpu_baseline:
