@@ -167,6 +167,14 @@ struct or_handle {
     double time;
     int lMatNext, n_time_step, started;
     double *dtr, *dth;
+    /* collocation method (l_cheb_coll) */
+    int coll;
+    double *dbeta;
+    double *cM, *cdM, *cd2M; /* T_n(x_k), d/dr, d2/dr2 at point k: [k * N + n] */
+    double *cTmat, *cXmat, *cTfac, *cXfac, *cUmat;
+    int *cTpiv, *cXpiv, *cUpiv, *cPpiv;
+    cplx *cPmat;
+    double *cPfac;
     /* scratch */
     cplx *w1, *w2, *w3;
     double *gr[16];
@@ -1386,11 +1394,11 @@ static void bc_pattern(or_handle *h, const double *tb, const double *tt, cplx *b
 }
 
 static int build_matrices(or_handle *h, int lhs_only);
+static void coll_alloc(or_handle *h);
 
 int or_create(const pz_config *cfg, or_handle **out)
 {
     *out = NULL;
-    if (cfg->l_cheb_coll) return -10;   /* integration method only */
     if (!cfg->l_direct_solve) return -11;
     if (cfg->l_non_rot) return -12;
     if (cfg->radratio == 0.0) return -13;
@@ -1404,6 +1412,7 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->ncheb = cfg->n_cheb_max;
     h->idx2m = calloc(h->nm, sizeof(int));
     for (int i = 0; i < h->nm; i++) h->idx2m[i] = i * cfg->minc;
+    h->coll = cfg->l_cheb_coll != 0;
     if (ts_init(&h->ts, cfg->time_scheme, cfg->courfac)) { free(h); return -14; }
     /* namelists.f90:398-526 */
     h->l_heat = cfg->ra != 0.0;
@@ -1524,6 +1533,12 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->dth = dalloc(N);
     for (int i = 0; i < 16; i++) h->gr[i] = dalloc(h->nphi);
     /* matrices */
+    if (h->coll) {
+        coll_alloc(h);
+        h->lMatNext = 1;
+        *out = h;
+        return 0;
+    }
     band_alloc_r(&h->T_RHSE[0], 3, 3, N);
     band_alloc_r(&h->T_RHSE[1], 4, 4, N);
     h->T_RHSI = calloc(h->nm, sizeof(bandr));
@@ -2027,6 +2042,442 @@ static void dt_courant(or_handle *h, int *l_new_dt, double *dt_new)
 }
 
 /* one iteration of time_loop's stage loop (step_time.f90:215-348) */
+/* ------------------------------------------------------------------------ */
+/* Collocation method (cheb_method = 'colloc', matrix_solve = 'direct')      */
+/* ------------------------------------------------------------------------ */
+
+/* get_der_mat, collocation branch (chebyshev.f90:296-321): T_n at the grid
+ * points by the three-term recursion, with the r-map drx/ddrx folded into
+ * the derivatives; stored (point k, degree n) as after the transposition */
+static void coll_der_mat(or_handle *h)
+{
+    int N = h->N;
+    h->cM = dalloc(N * N); h->cdM = dalloc(N * N); h->cd2M = dalloc(N * N);
+    for (int k = 0; k < N; k++) {
+        double *r0 = h->cM + (size_t)k * N, *r1 = h->cdM + (size_t)k * N, *r2 = h->cd2M + (size_t)k * N;
+        double xk = h->x[k], a = h->drx[k], b = h->ddrx[k];
+        r0[0] = 1.0; r0[1] = xk;
+        r1[0] = 0.0; r1[1] = a;
+        r2[0] = 0.0; r2[1] = b;
+        for (int n = 2; n < N; n++) {
+            r0[n] = 2.0 * xk * r0[n - 1] - r0[n - 2];
+            r1[n] = 2.0 * a * r0[n - 1] + 2.0 * xk * r1[n - 1] - r1[n - 2];
+            r2[n] = 2.0 * b * r0[n - 1] + 4.0 * a * r1[n - 1] + 2.0 * xk * r2[n - 1] - r2[n - 2];
+        }
+    }
+}
+#define CM(k, n) h->cM[(size_t)(k) * N + (n)]
+#define CDM(k, n) h->cdM[(size_t)(k) * N + (n)]
+#define CD2M(k, n) h->cd2M[(size_t)(k) * N + (n)]
+
+/* get_tempMat / get_xiMat (update_temp_coll.f90:310-393, update_xi_coll.f90):
+ * boundary rows, diffusion operator on the interior rows, boundary_fac on
+ * the first and last columns, row preconditioning, dgetrf */
+static int coll_scalar_mat(or_handle *h, int m, double diff, int ktop, int kbot, double *A, double *fac,
+                           int *piv)
+{
+    int N = h->N, nc = h->ncheb;
+    double wimp = h->ts.wimp_lin[0], dm2 = (double)m * (double)m;
+    for (int c = 0; c < N; c++) {
+        A[0 + c * N] = c < nc ? h->rnorm * (ktop == 1 ? CM(0, c) : CDM(0, c)) : 0.0;
+        A[(N - 1) + c * N] = c < nc ? h->rnorm * (kbot == 1 ? CM(N - 1, c) : CDM(N - 1, c)) : 0.0;
+        for (int r = 1; r < N - 1; r++)
+            A[r + c * N] = h->rnorm * (CM(r, c) - wimp * diff *
+                                                      (CD2M(r, c) + h->or1[r] * CDM(r, c) - dm2 * h->or2[r] * CM(r, c)));
+    }
+    for (int r = 0; r < N; r++) {
+        A[r] *= h->boundary_fac;
+        A[r + (N - 1) * N] *= h->boundary_fac;
+    }
+    for (int r = 0; r < N; r++) {
+        double mx = 0.0;
+        for (int c = 0; c < N; c++) mx = fmax(mx, fabs(A[r + c * N]));
+        fac[r] = 1.0 / mx;
+        for (int c = 0; c < N; c++) A[r + c * N] *= fac[r];
+    }
+    int info;
+    getrf_r(N, A, piv, &info);
+    return info;
+}
+
+/* get_psiMat (update_psi_coll_smat.f90:674-837): the coupled [omega; psi]
+ * system of one m /= 0, row then column max-scaling, zgetrf */
+static int coll_psi_mat(or_handle *h, int m, cplx *P, double *fac, int *piv)
+{
+    int N = h->N, nc = h->ncheb, N2 = 2 * N;
+    double wimp = h->ts.wimp_lin[0], dm2 = (double)m * (double)m, rn = h->rnorm;
+    cplx cim = I * (double)m;
+    memset(P, 0, sizeof(cplx) * N2 * N2);
+    for (int c = 0; c < nc; c++) {
+        int cp = c + N;
+        P[0 + (size_t)cp * N2] = rn * CM(0, c);
+        P[(N - 1) + (size_t)cp * N2] = rn * CM(N - 1, c);
+        P[N + (size_t)cp * N2] = h->c.ktopv == 1 ? rn * (CD2M(0, c) - h->or1[0] * CDM(0, c)) : rn * CDM(0, c);
+        P[(N2 - 1) + (size_t)cp * N2] = h->c.kbotv == 1 ? rn * (CD2M(N - 1, c) - h->or1[N - 1] * CDM(N - 1, c))
+                                                         : rn * CDM(N - 1, c);
+    }
+    for (int c = 0; c < N; c++) {
+        int cp = c + N;
+        for (int r = 1; r < N - 1; r++) {
+            int rp = r + N;
+            double ekp = h->CorFac * h->ekpump[r];
+            P[r + (size_t)c * N2] = rn * (CM(r, c) - wimp * (h->ViscFac * (CD2M(r, c) + h->or1[r] * CDM(r, c) -
+                                                                          dm2 * h->or2[r] * CM(r, c)) -
+                                                              ekp * CM(r, c)));
+            cplx v = -rn * wimp * (ekp * h->ekp_up[r] * CDM(r, c) +
+                                   ekp * (h->ekp_up[r] * h->beta[r] - (h->ekp_dusdp[r] * cim + h->ekp_us[r]) * cim *
+                                                                          h->or1[r]) * CM(r, c));
+            if (h->c.l_coriolis_imp) v -= rn * wimp * h->CorFac * h->beta[r] * h->or1[r] * cim * CM(r, c);
+            P[r + (size_t)cp * N2] = v;
+            P[rp + (size_t)c * N2] = rn * CM(r, c);
+            P[rp + (size_t)cp * N2] = rn * (CD2M(r, c) + (h->or1[r] + h->beta[r]) * CDM(r, c) +
+                                            (h->or1[r] * h->beta[r] + h->dbeta[r] - dm2 * h->or2[r]) * CM(r, c));
+        }
+    }
+    int bc[4] = {0, N - 1, N, N2 - 1};
+    for (int r = 0; r < N2; r++)
+        for (int q = 0; q < 4; q++) P[r + (size_t)bc[q] * N2] *= h->boundary_fac;
+    for (int r = 0; r < N2; r++) {
+        double mx = 0.0;
+        for (int c = 0; c < N2; c++) mx = fmax(mx, cabs(P[r + (size_t)c * N2]));
+        fac[r] = 1.0 / mx;
+        for (int c = 0; c < N2; c++) P[r + (size_t)c * N2] *= fac[r];
+    }
+    for (int c = 0; c < N2; c++) {
+        double mx = 0.0;
+        for (int r = 0; r < N2; r++) mx = fmax(mx, cabs(P[r + (size_t)c * N2]));
+        fac[N2 + c] = 1.0 / mx;
+        for (int r = 0; r < N2; r++) P[r + (size_t)c * N2] *= fac[N2 + c];
+    }
+    int info;
+    getrf_c(N2, P, piv, &info);
+    return info;
+}
+
+/* get_uphiMat (update_psi_coll_smat.f90:919-987): m = 0, no preconditioning */
+static int coll_uphi_mat(or_handle *h, double *A, int *piv)
+{
+    int N = h->N, nc = h->ncheb;
+    double wimp = h->ts.wimp_lin[0], rn = h->rnorm;
+    for (int c = 0; c < N; c++) {
+        A[0 + c * N] = c >= nc ? 0.0 : (h->c.ktopv == 1 ? rn * (CDM(0, c) - h->or1[0] * CM(0, c)) : rn * CM(0, c));
+        A[(N - 1) + c * N] = c >= nc ? 0.0
+                                     : (h->c.kbotv == 1 ? rn * (CDM(N - 1, c) - h->or1[N - 1] * CM(N - 1, c))
+                                                        : rn * CM(N - 1, c));
+        for (int r = 1; r < N - 1; r++)
+            A[r + c * N] = rn * (CM(r, c) - wimp * (h->ViscFac * (CD2M(r, c) + h->or1[r] * CDM(r, c) -
+                                                                 h->or2[r] * CM(r, c)) -
+                                                     h->CorFac * h->ekpump[r] * CM(r, c)));
+    }
+    for (int r = 0; r < N; r++) {
+        A[r] *= h->boundary_fac;
+        A[r + (N - 1) * N] *= h->boundary_fac;
+    }
+    int info;
+    getrf_r(N, A, piv, &info);
+    return info;
+}
+
+/* the per-m LU factorisations of one lMat rebuild */
+static int coll_build(or_handle *h)
+{
+    int N = h->N, N2 = 2 * N;
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i], info;
+        if (h->l_heat) {
+            info = coll_scalar_mat(h, m, h->TdiffFac * h->hdifT[i], h->c.ktopt, h->c.kbott, h->cTmat + (size_t)i * N * N,
+                                   h->cTfac + (size_t)i * N, h->cTpiv + (size_t)i * N);
+            if (info) return info;
+        }
+        if (h->l_chem) {
+            info = coll_scalar_mat(h, m, h->XidiffFac * h->hdifXi[i], h->c.ktopxi, h->c.kbotxi,
+                                   h->cXmat + (size_t)i * N * N, h->cXfac + (size_t)i * N, h->cXpiv + (size_t)i * N);
+            if (info) return info;
+        }
+        if (m == 0) info = coll_uphi_mat(h, h->cUmat, h->cUpiv);
+        else info = coll_psi_mat(h, m, h->cPmat + (size_t)i * N2 * N2, h->cPfac + (size_t)i * 2 * N2,
+                                 h->cPpiv + (size_t)i * N2);
+        if (info) return info;
+    }
+    return 0;
+}
+
+static void coll_alloc(or_handle *h)
+{
+    int N = h->N, N2 = 2 * N;
+    size_t nm = h->nm;
+    h->dbeta = dalloc(N);
+    for (int n = 1; n < N; n++) {
+        double s2 = h->r_cmb * h->r_cmb - h->r[n] * h->r[n];
+        h->dbeta[n] = -(h->r_cmb * h->r_cmb + h->r[n] * h->r[n]) / (s2 * s2); /* radial.f90:222 */
+    }
+    coll_der_mat(h);
+    h->cTmat = calloc(nm * N * N, sizeof(double)); h->cXmat = calloc(nm * N * N, sizeof(double));
+    h->cTfac = dalloc(nm * N); h->cXfac = dalloc(nm * N);
+    h->cTpiv = calloc(nm * N, sizeof(int)); h->cXpiv = calloc(nm * N, sizeof(int));
+    h->cUmat = dalloc(N * N); h->cUpiv = calloc(N, sizeof(int));
+    h->cPmat = calloc(nm * N2 * N2, sizeof(cplx));
+    h->cPfac = dalloc(nm * 2 * N2);
+    h->cPpiv = calloc(nm * N2, sizeof(int));
+}
+
+/* get_ddr_real_1d with get_ddcheb_real_1d (radial_derivatives.f90:255-311,
+ * 593-667): grid values in, derivatives out; below the rerror threshold the
+ * recursion carries the previous value instead of adding the term */
+static void get_ddr_real(or_handle *h, const double *f_in, double *df, double *ddf)
+{
+    int N = h->N;
+    double *f = malloc(sizeof(double) * N);
+    memcpy(f, f_in, sizeof(double) * N);
+    costf_real(h, f);
+    double mx = 0;
+    for (int j = 0; j < N; j++) mx = fmax(mx, fabs(f[j]));
+    double thr = mx * h->thr;
+    df[N - 1] = 0.0;
+    ddf[N - 1] = 0.0;
+    int nc = N - 1;
+    df[nc - 1] = fabs(f[nc]) >= thr ? (double)nc * f[nc] : 0.0;
+    ddf[nc - 1] = 0.0;
+    for (int n = N - 2; n >= 1; n--) {
+        double fac = (double)(2 * n);
+        df[n - 1] = fabs(f[n]) >= thr ? df[n + 1] + fac * f[n] : df[n + 1];
+        ddf[n - 1] = fabs(df[n]) >= thr ? ddf[n + 1] + fac * df[n] : ddf[n + 1];
+    }
+    costf_real(h, df);
+    costf_real(h, ddf);
+    for (int j = 0; j < N; j++) {
+        ddf[j] = h->ddrx[j] * df[j] + h->drx[j] * h->drx[j] * ddf[j];
+        df[j] = h->drx[j] * df[j];
+    }
+    free(f);
+}
+
+/* finish_exp_temp_coll / finish_exp_xi_coll (update_temp_coll.f90:152-176,
+ * update_xi_coll.f90:148-166) and finish_exp_psi_coll_smat
+ * (update_psi_coll_smat.f90:323-378) */
+static void coll_finish_exp(or_handle *h, int is)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    if (h->l_heat) {
+        cplx *e = h->dTdt.expl + is * F;
+        for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsT + (size_t)i * N, h->work + (size_t)i * N, 1);
+        for (int i = 0; i < h->nm; i++)
+            for (int k = 0; k < N; k++) {
+                size_t q = (size_t)i * N + k;
+                e[q] = e[q] - h->or1[k] * h->work[q] - h->us[q] * (h->dtcond[k] - h->tadvz * h->beta[k] * h->tcond[k]);
+            }
+    }
+    if (h->l_chem) {
+        cplx *e = h->dxidt.expl + is * F;
+        for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsXi + (size_t)i * N, h->work + (size_t)i * N, 1);
+        for (int i = 0; i < h->nm; i++)
+            for (int k = 0; k < N; k++) {
+                size_t q = (size_t)i * N + k;
+                e[q] = e[q] - h->or1[k] * h->work[q] - h->us[q] * h->dxicond[k];
+            }
+    }
+    cplx *e = h->dpsidt.expl + is * F;
+    for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsOm + (size_t)i * N, h->work + (size_t)i * N, 1);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        if (m == 0) continue;
+        for (int k = 0; k < N; k++) {
+            size_t q = (size_t)i * N + k;
+            e[q] = e[q] - h->or1[k] * h->work[q];
+            if (!h->c.l_coriolis_imp) e[q] += h->CorFac * h->beta[k] * h->us[q];
+            if (!h->c.l_buo_imp) {
+                if (h->l_heat) e[q] -= h->BuoFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->temp[q];
+                if (h->l_chem) e[q] -= h->ChemFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->xi[q];
+            }
+        }
+    }
+}
+
+/* get_temp_rhs_imp_coll / get_xi_rhs_imp_coll (update_temp_coll.f90:178-236):
+ * fld holds Chebyshev coefficients when in_cheb (left as grid values) */
+static void coll_rhs_imp_scalar(or_handle *h, cplx *fld, cplx *dfld, tarray *a, int istage, int l_calc_lin,
+                                int in_cheb, double diff, const double *hdif)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    if (l_calc_lin)
+        for (int i = 0; i < h->nm; i++)
+            get_ddr_vec(h, fld + (size_t)i * N, dfld + (size_t)i * N, h->work + (size_t)i * N, !in_cheb);
+    if (in_cheb) costf_field(h, fld);
+    if (istage == 1) memcpy(a->old, fld, sizeof(cplx) * F);
+    if (l_calc_lin)
+        for (int i = 0; i < h->nm; i++) {
+            double dm2 = (double)h->idx2m[i] * (double)h->idx2m[i];
+            for (int k = 0; k < N; k++) {
+                size_t q = (size_t)i * N + k;
+                a->impl[(istage - 1) * F + q] =
+                    diff * hdif[i] * (h->work[q] + h->or1[k] * dfld[q] - dm2 * h->or2[k] * fld[q]);
+            }
+        }
+}
+
+/* get_psi_rhs_imp_coll_smat (update_psi_coll_smat.f90:380-527) */
+static void coll_rhs_imp_psi(or_handle *h, tarray *a, int istage, int l_calc_lin, int in_cheb)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    int nm0 = -1;
+    for (int i = 0; i < h->nm; i++)
+        if (h->idx2m[i] == 0) nm0 = i;
+    double *om0 = dalloc(N);
+    if (nm0 >= 0)
+        for (int k = 0; k < N; k++) om0[k] = creal(h->om[(size_t)nm0 * N + k]);
+    cplx *dom = calloc(F, sizeof(cplx));
+    if (l_calc_lin)
+        for (int i = 0; i < h->nm; i++)
+            get_ddr_vec(h, h->om + (size_t)i * N, dom + (size_t)i * N, h->work + (size_t)i * N, !in_cheb);
+    if (in_cheb) costf_field(h, h->om);
+    if (nm0 >= 0)
+        for (int k = 0; k < N; k++) h->om[(size_t)nm0 * N + k] = om0[k];
+    if (istage == 1)
+        for (int i = 0; i < h->nm; i++)
+            memcpy(a->old + (size_t)i * N, (h->idx2m[i] == 0 ? h->up : h->om) + (size_t)i * N, sizeof(cplx) * N);
+    if (l_calc_lin) {
+        double *u0 = dalloc(N), *du0 = dalloc(N), *d2u0 = dalloc(N);
+        if (nm0 >= 0) {
+            for (int k = 0; k < N; k++) u0[k] = creal(h->up[(size_t)nm0 * N + k]);
+            get_ddr_real(h, u0, du0, d2u0);
+        }
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            double dm2 = (double)m * (double)m;
+            for (int k = 0; k < N; k++) {
+                size_t q = (size_t)i * N + k;
+                cplx v;
+                if (m == 0) {
+                    v = h->ViscFac * h->hdifV[i] * (d2u0[k] + h->or1[k] * du0[k] - h->or2[k] * u0[k]) -
+                        h->CorFac * h->ekpump[k] * u0[k];
+                } else {
+                    v = h->ViscFac * h->hdifV[i] * (h->work[q] + h->or1[k] * dom[q] - dm2 * h->or2[k] * h->om[q]) -
+                        h->CorFac * h->ekpump[k] *
+                            (h->om[q] + h->ekp_up[k] * h->up[q] +
+                             (h->ekp_dusdp[k] * I * (double)m + h->ekp_us[k]) * h->us[q]);
+                    if (h->c.l_coriolis_imp) v += h->CorFac * h->beta[k] * h->us[q];
+                    if (h->c.l_buo_imp) {
+                        if (h->l_heat) v -= h->rgrav[k] * h->or1[k] * h->BuoFac * I * (double)m * h->temp[q];
+                        if (h->l_chem) v -= h->rgrav[k] * h->or1[k] * h->ChemFac * I * (double)m * h->xi[q];
+                    }
+                }
+                a->impl[(istage - 1) * F + q] = v;
+            }
+        }
+        free(u0); free(du0); free(d2u0);
+    }
+    free(dom);
+    free(om0);
+}
+
+/* update_temp_co / update_xi_co (update_temp_coll.f90:66-150) */
+static void coll_update_scalar(or_handle *h, int is_temp)
+{
+    int N = h->N;
+    tscheme *t = &h->ts;
+    tarray *a = is_temp ? &h->dTdt : &h->dxidt;
+    cplx *fld = is_temp ? h->temp : h->xi, *dfld = is_temp ? h->dtemp : h->dxi;
+    cplx *top = is_temp ? h->topt : h->topxi, *bot = is_temp ? h->bott : h->botxi;
+    double *Am = is_temp ? h->cTmat : h->cXmat, *fac = is_temp ? h->cTfac : h->cXfac;
+    int *piv = is_temp ? h->cTpiv : h->cXpiv;
+    ts_set_imex_rhs(h, t, a, h->work);
+    double *rr = dalloc(N), *ri = dalloc(N);
+    for (int i = 0; i < h->nm; i++) {
+        cplx *w = h->work + (size_t)i * N;
+        w[0] = top[i];
+        w[N - 1] = bot[i];
+        for (int k = 0; k < N; k++) {
+            w[k] *= fac[(size_t)i * N + k];
+            rr[k] = creal(w[k]);
+            ri[k] = cimag(w[k]);
+        }
+        getrs_r(N, Am + (size_t)i * N * N, piv + (size_t)i * N, rr);
+        getrs_r(N, Am + (size_t)i * N * N, piv + (size_t)i * N, ri);
+        for (int k = 0; k < N; k++) fld[(size_t)i * N + k] = k < h->ncheb ? rr[k] + I * ri[k] : 0.0;
+    }
+    free(rr);
+    free(ri);
+    ts_rotate(h, t, a);
+    double diff = is_temp ? h->TdiffFac : h->XidiffFac;
+    double *hdif = is_temp ? h->hdifT : h->hdifXi;
+    if (t->istage == t->nstages) coll_rhs_imp_scalar(h, fld, dfld, a, 1, t->l_imp_calc_rhs[0], 1, diff, hdif);
+    else coll_rhs_imp_scalar(h, fld, dfld, a, t->istage + 1, t->l_imp_calc_rhs[t->istage], 1, diff, hdif);
+}
+
+/* update_om_coll_smat (update_psi_coll_smat.f90:88-321) */
+static void coll_update_psi(or_handle *h)
+{
+    int N = h->N, N2 = 2 * N;
+    tscheme *t = &h->ts;
+    tarray *a = &h->dpsidt;
+    double wimp = t->wimp_lin[0];
+    ts_set_imex_rhs(h, t, a, h->work);
+    double *u0 = dalloc(N), *om0 = dalloc(N);
+    cplx *rhs = calloc(N2, sizeof(cplx));
+    int nm0 = -1;
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        cplx *w = h->work + (size_t)i * N;
+        if (m == 0) {
+            nm0 = i;
+            u0[0] = 0.0;
+            u0[N - 1] = 0.0;
+            for (int k = 1; k < N - 1; k++) u0[k] = creal(w[k]);
+            getrs_r(N, h->cUmat, h->cUpiv, u0);
+            for (int k = h->ncheb; k < N; k++) u0[k] = 0.0;
+            continue;
+        }
+        memset(rhs, 0, sizeof(cplx) * N2);
+        for (int k = 1; k < N - 1; k++) {
+            size_t q = (size_t)i * N + k;
+            cplx v = w[k];
+            if (h->c.l_buo_imp && (h->l_heat || h->l_chem)) {
+                if (h->l_heat) v -= wimp * h->BuoFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->temp[q];
+                if (h->l_chem) v -= wimp * h->ChemFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->xi[q];
+            }
+            rhs[k] = v;
+        }
+        const double *fac = h->cPfac + (size_t)i * 2 * N2;
+        for (int k = 0; k < N2; k++) rhs[k] *= fac[k];
+        getrs_c(N2, h->cPmat + (size_t)i * N2 * N2, h->cPpiv + (size_t)i * N2, rhs);
+        for (int k = 0; k < N2; k++) rhs[k] *= fac[N2 + k];
+        for (int k = 0; k < N; k++) {
+            h->om[(size_t)i * N + k] = k < h->ncheb ? rhs[k] : 0.0;
+            h->psi[(size_t)i * N + k] = k < h->ncheb ? rhs[N + k] : 0.0;
+        }
+    }
+    if (nm0 >= 0) {
+        get_dr_real(h, u0, om0, 0);
+        costf_real(h, u0);
+        for (int k = 0; k < N; k++) h->psi[(size_t)nm0 * N + k] = 0.0;
+    }
+    for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->psi + (size_t)i * N, h->work + (size_t)i * N, 0);
+    costf_field(h, h->psi);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        for (int k = 0; k < N; k++) {
+            size_t q = (size_t)i * N + k;
+            if (m == 0) {
+                h->us[q] = 0.0;
+                h->up[q] = u0[k];
+                h->om[q] = om0[k] + h->or1[k] * u0[k];
+            } else {
+                h->us[q] = I * (double)m * h->or1[k] * h->psi[q];
+                h->up[q] = -h->work[q] - h->beta[k] * h->psi[q];
+            }
+        }
+    }
+    ts_rotate(h, t, a);
+    if (t->istage == t->nstages) coll_rhs_imp_psi(h, a, 1, t->l_imp_calc_rhs[0], 1);
+    else coll_rhs_imp_psi(h, a, t->istage + 1, t->l_imp_calc_rhs[t->istage], 1);
+    free(rhs);
+    free(u0);
+    free(om0);
+}
+
 static int one_step(or_handle *h)
 {
     tscheme *t = &h->ts;
@@ -2035,7 +2486,10 @@ static int one_step(or_handle *h)
     t->istage = 1;
     for (int n_stage = 1; n_stage <= t->nstages; n_stage++) {
         int is = t->istage - 1;
-        if (t->l_exp_calc[n_stage - 1]) {
+        if (t->l_exp_calc[n_stage - 1] && h->coll) {
+            radial_loop(h, is);
+            coll_finish_exp(h, is);
+        } else if (t->l_exp_calc[n_stage - 1]) {
             radial_loop(h, is);
             if (h->l_heat) finish_exp_scalar(h, h->dVsT, h->dTdt.expl + is * F, h->dtcond, h->tcond, 1);
             if (h->l_chem) finish_exp_scalar(h, h->dVsXi, h->dxidt.expl + is * F, h->dxicond, h->xicond, 0);
@@ -2053,12 +2507,18 @@ static int one_step(or_handle *h)
         if ((l_new_dt || lMat) && t->istage == 1) lMat = 1;
         h->lMatNext = 0;
         if (lMat) {
-            int info = build_matrices(h, 1);
+            int info = h->coll ? coll_build(h) : build_matrices(h, 1);
             if (info) return info;
         }
-        if (h->l_heat) update_scalar(h, lMat, 1);
-        if (h->l_chem) update_scalar(h, lMat, 0);
-        update_psi(h);
+        if (h->coll) {
+            if (h->l_heat) coll_update_scalar(h, 1);
+            if (h->l_chem) coll_update_scalar(h, 0);
+            coll_update_psi(h);
+        } else {
+            if (h->l_heat) update_scalar(h, lMat, 1);
+            if (h->l_chem) update_scalar(h, lMat, 0);
+            update_psi(h);
+        }
         t->istage++;
     }
     h->n_time_step++;
@@ -2249,6 +2709,15 @@ int or_start(or_handle *h)
             size_t q = (size_t)i * N + n_r;
             h->om[q] = h->work[q] + h->or1[n_r] * h->up[q] - h->or1[n_r] * I * (double)m * h->us[q];
         }
+    if (h->coll) {
+        /* get_start_fields, collocation branch (init_fields.f90:228-244) */
+        if (h->l_heat) coll_rhs_imp_scalar(h, h->temp, h->dtemp, &h->dTdt, 1, 1, 0, h->TdiffFac, h->hdifT);
+        if (h->l_chem) coll_rhs_imp_scalar(h, h->xi, h->dxi, &h->dxidt, 1, 1, 0, h->XidiffFac, h->hdifXi);
+        coll_rhs_imp_psi(h, &h->dpsidt, 1, 1, 0);
+        h->started = 1;
+        h->lMatNext = 1;
+        return 0;
+    }
     memcpy(h->temp_hat, h->temp, sizeof(cplx) * F);
     memcpy(h->xi_hat, h->xi, sizeof(cplx) * F);
     memcpy(h->psi_hat, h->psi, sizeof(cplx) * F);

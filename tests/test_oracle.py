@@ -10,6 +10,10 @@
 3. The sparse-Chebyshev stencils against an independent construction with
    numpy.polynomial.chebyshev (apply the differential operator to T_j,
    integrate J times, read the coefficient of T_n).
+4. The collocation method end to end against the reference's golden series
+   samples/RotE1e3EkPump/reference.out, first run (input.nml: collocation,
+   CNAB2, Ekman pumping, n_r = 32, single-mode start), at rtol 1e-8
+   (samples/RotE1e3EkPump/unitTest.py:116-120).
 """
 import numpy as np
 import numpy.polynomial.chebyshev as C
@@ -40,6 +44,26 @@ def test_rotinteg_golden_series(rot):
         rows.append(np.r_[t, o.ekin_3d()])
     rows = np.array(rows)
     np.testing.assert_allclose(rows, ref, rtol=1e-8, atol=1e-20)
+
+
+def test_rotekpump_collocation_golden_series():
+    """1000 CNAB2 steps (t = 0..0.1) of the collocation start run."""
+    import os
+    from helpers import GOLDEN
+    from pizza_amd.config import config_from_namelist
+    cfg, start, _ = config_from_namelist(os.path.join(GOLDEN, "rotekpump_input_start.nml"))
+    assert cfg.l_cheb_coll == 1
+    ref = np.loadtxt(os.path.join(GOLDEN, "rotekpump_start_e_kin_3D.txt"))
+    o = Oracle(cfg)
+    o.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+    o.start()
+    rows = []
+    for k in range(len(ref)):
+        if k:
+            o.step(100)
+        t, _ = o.time()
+        rows.append(np.r_[t, o.ekin_3d()])
+    np.testing.assert_allclose(np.array(rows), ref, rtol=1e-8, atol=1e-20)
 
 
 def _oracle(n_r=33, m_max=32):
