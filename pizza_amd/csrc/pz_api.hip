@@ -13,6 +13,7 @@
 #include "../../include/pizza_amd.h"
 #include "pz_host_ops.h"
 #include "pz_mphase.h"
+#include "pz_coll.h"
 
 using pzh::ld;
 
@@ -109,7 +110,7 @@ struct pz_handle {
     hipGraphExec_t gexec = nullptr;
     bool started = false;
     bool use_graph = true;
-    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0;
+    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0;
     int inv_waves = 16;
     PzDev *ddev = nullptr;
     std::string err;
@@ -437,6 +438,11 @@ static void setup(pz_handle *h)
         beta[n] = -r[n] / s2;
         ekpump[n] = 0.5 * ek_pump_fac * std::sqrt(c.ek * rc) / std::pow(s2, 0.75);
     }
+    std::vector<double> dbeta(N, 0.0); // radial.f90:222
+    for (int n = 1; n < N; n++) {
+        double s2 = rc * rc - r[n] * r[n];
+        dbeta[n] = -(rc * rc + r[n] * r[n]) / (s2 * s2);
+    }
     for (int n = 0; n < N; n++) {
         ekp_up[n] = -0.5 * beta[n];
         ekp_us[n] = -5.0 * rc * beta[n] * oheight[n];
@@ -490,6 +496,7 @@ static void setup(pz_handle *h)
         std::vector<double> rc((size_t)PZ_NRADC * N, 0.0);
         for (int i = 0; i < 18; i++) std::copy(rows[i]->begin(), rows[i]->end(), rc.begin() + (size_t)i * N);
         std::copy(drb.begin(), drb.end(), rc.begin() + (size_t)18 * N);
+        std::copy(dbeta.begin(), dbeta.end(), rc.begin() + (size_t)21 * N);
         d.radc = h->upload(rc);
         set_radial_pointers(d, d.radc, (size_t)N);
     }
@@ -561,12 +568,39 @@ static void setup(pz_handle *h)
     d.T_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
     d.X_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
     d.P_inv = h->dalloc<cd>((size_t)nm * N * d.ldi);
+    d.coll = c.l_cheb_coll ? 1 : 0;
+    if (d.coll) {
+        // get_der_mat, collocation branch (chebyshev.f90:296-321)
+        std::vector<double> cm((size_t)N * N), cdm((size_t)N * N), cd2m((size_t)N * N);
+        for (int k = 0; k < N; k++) {
+            double *r0 = &cm[(size_t)k * N], *r1 = &cdm[(size_t)k * N], *r2 = &cd2m[(size_t)k * N];
+            const double xk = h->x[k], a = h->drx[k], b = 0.0;
+            r0[0] = 1.0; r0[1] = xk;
+            r1[0] = 0.0; r1[1] = a;
+            r2[0] = 0.0; r2[1] = b;
+            for (int n = 2; n < N; n++) {
+                r0[n] = 2.0 * xk * r0[n - 1] - r0[n - 2];
+                r1[n] = 2.0 * a * r0[n - 1] + 2.0 * xk * r1[n - 1] - r1[n - 2];
+                r2[n] = 2.0 * b * r0[n - 1] + 4.0 * a * r1[n - 1] + 2.0 * xk * r2[n - 1] - r2[n - 2];
+            }
+        }
+        d.cM = h->upload(cm);
+        d.cdM = h->upload(cdm);
+        d.cd2M = h->upload(cd2m);
+        d.PC_inv = h->dalloc<cd>((size_t)nm * N * 2 * N);
+        d.U_inv = h->dalloc<double>((size_t)N * d.ldi);
+        // Gauss-Jordan blocks per m: T and xi (N x 2N real), psi (2N x 4N complex)
+        d.coll_ws = (2 * (size_t)N * 2 * N * sizeof(double) + (size_t)2 * N * 4 * N * sizeof(cd) + 255) & ~(size_t)255;
+        d.coll_work = h->dalloc<char>((size_t)nm * d.coll_ws);
+    }
     d.ss = h->dalloc<PzScheme>(1);
-    build_operators(h);
+    if (!d.coll) build_operators(h);
     // LDS budgets
     h->lds_radial = (5 * (size_t)h->nm + 6 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
     h->lds_mphase = MpLayout(N).total;
     h->lds_start = 15 * (size_t)N * sizeof(cd);
+    h->lds_start_coll = (13 * (size_t)N + 6 * (size_t)(2 * N - 2) + 3 * (size_t)N + 16) * sizeof(cd) +
+                        3 * (size_t)N * sizeof(double);
     {
         size_t nbp = N - 4;
         size_t psi = (size_t)(PZ_P_LD * nbp + 2 * PZ_P_NT * nbp + 16) * sizeof(cd) + N * sizeof(double) +
@@ -581,7 +615,8 @@ static void setup(pz_handle *h)
     // waves of k_build_inv: as many right-hand sides as fit next to the factors
     while (h->inv_waves > 1 && InvLayout(N, h->inv_waves).total > lim) h->inv_waves--;
     h->lds_inv = InvLayout(N, h->inv_waves).total;
-    size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
+    size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv,
+                            h->lds_start_coll});
     if (need > 65536) {
         if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
         HIPCHK(hipFuncSetAttribute((const void *)k_build_lhs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_build));
@@ -589,6 +624,9 @@ static void setup(pz_handle *h)
         HIPCHK(hipFuncSetAttribute((const void *)k_radial_loop, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_radial));
         HIPCHK(hipFuncSetAttribute((const void *)k_start, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_start));
         HIPCHK(hipFuncSetAttribute((const void *)k_build_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_inv));
+        HIPCHK(hipFuncSetAttribute((const void *)k_mcoll, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_mphase));
+        HIPCHK(hipFuncSetAttribute((const void *)k_start_coll, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)h->lds_start_coll));
     }
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // device copy of the (now final) descriptor for kernels that read it
@@ -605,11 +643,11 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
     h->c = *cfg;
     const pz_config &c = *cfg;
     std::string why;
-    if (c.l_cheb_coll) why = "only cheb_method='integ' is implemented";
-    else if (!c.l_direct_solve) why = "only matrix_solve='direct' is implemented";
+    if (!c.l_direct_solve) why = "only matrix_solve='direct' is implemented";
     else if (c.l_non_rot) why = "non-rotating runs are not implemented";
     else if (c.radratio <= 0.0) why = "full-disk geometry is not implemented";
     else if (c.n_r_max < 8 || c.n_r_max > 257) why = "n_r_max must be in [8, 257]";
+    else if (c.l_cheb_coll && c.n_r_max > 256) why = "the collocation method needs n_r_max <= 256";
     else if (c.n_cheb_max < 1 || c.n_cheb_max > c.n_r_max) why = "n_cheb_max must be in [1, n_r_max]";
     else if (c.m_max < 3 || c.minc < 1 || c.m_max % c.minc) why = "invalid m_max/minc";
     if (!why.empty()) {
@@ -856,7 +894,10 @@ extern "C" int pz_start(pz_handle *h)
                         (size_t)sc.nexp, (size_t)sc.nold, (size_t)sc.nimp, (size_t)sc.nexp};
         for (int i = 0; i < 9; i++) HIPCHK(hipMemsetAsync(slots[i], 0, cnt[i] * F * sizeof(cd), h->stream));
         hipLaunchKernelGGL(k_set_weights, dim3(1), dim3(1), 0, h->stream, h->d);
-        hipLaunchKernelGGL(k_start, dim3(h->nm), dim3(256), h->lds_start, h->stream, h->d);
+        if (h->d.coll)
+            hipLaunchKernelGGL(k_start_coll, dim3(h->nm), dim3(256), h->lds_start_coll, h->stream, h->d);
+        else
+            hipLaunchKernelGGL(k_start, dim3(h->nm), dim3(256), h->lds_start, h->stream, h->d);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));
         if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -879,10 +920,20 @@ static void launch_step(pz_handle *h)
                                stage == 1 ? 1 : 0);
         if (stage == 1) {
             hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, h->d);
-            hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, h->d, 0);
-            hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream, h->d, 0);
+            if (h->d.coll) {
+                hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), 0, h->stream, h->d, 0);
+            } else {
+                hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, h->d, 0);
+                hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream, h->d,
+                                   0);
+            }
         }
-        hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream, (const PzDev *)h->ddev, stage, exp_calc);
+        if (h->d.coll)
+            hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+                               (const PzDev *)h->ddev, stage, exp_calc);
+        else
+            hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+                               (const PzDev *)h->ddev, stage, exp_calc);
     }
 }
 
@@ -1098,8 +1149,12 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, h->stream));
         for (int i = 0; i < reps; i++) {
-            if (which == 0)
-                hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream, (const PzDev *)h->ddev, 1, 1);
+            if (which == 0 && h->d.coll)
+                hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+                                   (const PzDev *)h->ddev, 1, 1);
+            else if (which == 0)
+                hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+                                   (const PzDev *)h->ddev, 1, 1);
             else
                 hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, h->d, 0, 0);
         }

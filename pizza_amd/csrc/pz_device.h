@@ -1265,9 +1265,9 @@ __device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t byte
 }
 
 // ---------------------------------------------------------------------------
-// Dense solve by explicit inverse: y_i = sum_j INV[j][i] x_j for NMAT
-// matrices at once (column-major, leading dimension ldi, columns 16-byte
-// aligned).  E = double: a thread owns two consecutive rows (one 16-byte
+// Dense solve by explicit inverse: y_i = sum_j INV[j][i] x_j (i < nrow,
+// j < ncol) for NMAT matrices at once (column-major, leading dimension
+// ldi, columns 16-byte aligned).  E = double: a thread owns two consecutive rows (one 16-byte
 // load per column); E = cd: one row.  Threads split the columns into CG
 // groups, each thread keeps 16 column loads in flight, and the partial sums
 // are combined in a fixed order (bitwise reproducible).  x and y may alias;
@@ -1275,10 +1275,12 @@ __device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t byte
 // 2 * blockDim.x * NMAT complex values.
 // ---------------------------------------------------------------------------
 template <class E, int NMAT>
-__device__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y, cd *red)
+__device__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y,
+                                cd *red)
 {
     constexpr int RPT = sizeof(E) == 8 ? 2 : 1, U = 16 / NMAT;
-    const int R = (N + RPT - 1) / RPT;
+    const int N = ncol;
+    const int R = (nrow + RPT - 1) / RPT;
     const int CG = min((int)blockDim.x / R, N);
     const int t = threadIdx.x, cg = t / R, ru = t - cg * R;
     const size_t RS = 2 * (size_t)blockDim.x; // scratch per matrix
@@ -1330,10 +1332,17 @@ __device__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, 
                 cd s = acc[a][r];
                 for (int c = 1; c < CG; c++) s = cadd(s, red[(size_t)a * RS + (c - 1) * R * RPT + ru * RPT + r]);
                 const int row = ru * RPT + r;
-                if (row < N) y[a][row] = s;
+                if (row < nrow) y[a][row] = s;
             }
     }
     __syncthreads();
+}
+// square n x n form
+template <class E, int NMAT>
+__device__ __forceinline__ void block_matvec(int N, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y,
+                                             cd *red)
+{
+    block_matvec_rc<E, NMAT>(N, N, ldi, inv, x, y, red);
 }
 
 // LDS layout of k_build_inv: one field's bordered factors (sized for the

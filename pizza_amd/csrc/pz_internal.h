@@ -50,6 +50,7 @@ struct PzDev {
     const double *tcond, *dtcond, *xicond, *dxicond, *delxr2, *delxh2, *drx, *ddrx, *x;
     const double *cosT, *sinT; // cos/sin(pi q/(N-1)), q in [0, L)
     const double *drb;         // [3][N] bulk factors of the radial derivatives
+    const double *dbeta;       // d beta / dr (collocation psi operator)
     const double *radc;        // block holding all radial rows above (PZ_NRADC x n_r)
     const double *tau;         // tau lines: [rMat(2,N) | drMat(2,N) | d2rMat(2,N)]
     const cd *twid;            // exp(-2 pi i k / n_phi)
@@ -92,13 +93,25 @@ struct PzDev {
     cd *P_inv;
     int ldi;
 
+    // collocation method (l_cheb_coll): T_n and its r-derivatives at the
+    // grid points, [k][n] (chebyshev.f90:296-321); T/xi inverses reuse
+    // T_inv/X_inv; psi: the first n_r columns of the inverse of the coupled
+    // omega-psi system, [m][col][2 n_r]; u_phi(m = 0): [col][ldi]; scratch of
+    // the Gauss-Jordan inversions, coll_ws bytes per m
+    int coll;
+    const double *cM, *cdM, *cd2M;
+    cd *PC_inv;
+    double *U_inv;
+    char *coll_work;
+    size_t coll_ws;
+
     PzScheme *ss;
 };
 
 // rows of the radial block: x, r, drx, ddrx, or1, or2, rgrav, beta, ekpump,
 // ekp_up, ekp_us, ekp_dusdp, delxr2, delxh2, tcond, dtcond, xicond, dxicond,
-// drb (3 rows)
-#define PZ_NRADC 21
+// drb (3 rows), dbeta
+#define PZ_NRADC 22
 template <class D>
 __host__ __device__ inline void set_radial_pointers(D &d, const double *rc, size_t N)
 {
@@ -121,6 +134,7 @@ __host__ __device__ inline void set_radial_pointers(D &d, const double *rc, size
     d.xicond = rc + 16 * N;
     d.dxicond = rc + 17 * N;
     d.drb = rc + 18 * N;
+    d.dbeta = rc + 21 * N;
 }
 
 // The radial rows of one block (global memory or a kernel's LDS copy):
@@ -150,6 +164,7 @@ struct RadView {
     __host__ __device__ double xicond(int k) const { return row(16, k); }
     __host__ __device__ double dxicond(int k) const { return row(17, k); }
     __host__ __device__ double drb(int k) const { return row(18, k); } // k in [0, 3N)
+    __host__ __device__ double dbeta(int k) const { return row(21, k); }
 };
 
 // sizes of the bordered storage per m

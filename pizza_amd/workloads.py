@@ -3,8 +3,9 @@
 ``c2_production`` is the headline configuration: sparse-Chebyshev
 integration, n_r_max = 97, m_max = 128 (n_phi = 384), Ekman 1e-5, Ra 3e7,
 Pr 1, thermal + compositional (Sc 3), DIRK BPR353, dt 5e-6.
-``c1_collocation_shape`` keeps the C1 physical parameters (n_r 33, n_phi 96,
-Ekman 1e-3, Ra 1e5, Ekman pumping, CNAB2 dt 1e-4) on the integration method.
+``c1_collocation`` is the reference's small collocation case (n_r 33,
+n_phi 96, Ekman 1e-3, Ra 1e5, Ekman pumping, CNAB2 dt 1e-4);
+``c1_collocation_shape`` keeps those parameters on the integration method.
 
 Seeded perturbation (both runs get the same fields through set_field):
     f(m, r) = A * g_m * sin(pi (r - r_icb)),  m = 1..m_max,
@@ -24,6 +25,15 @@ def c2_production(n_r=97, m_max=128, raxi=3.0e7):
         ra=3.0e7, ek=1.0e-5, pr=1.0, raxi=raxi, sc=3.0, radratio=0.35,
         ktopt=1, kbott=1, ktopxi=1, kbotxi=1, ktopv=2, kbotv=2, l_ek_pump=True,
         g1=1.0)
+
+
+def c1_collocation():
+    c = default_config(
+        n_r_max=33, n_cheb_max=33, m_max=32, minc=1, time_scheme="CNAB2",
+        cheb_method="colloc", matrix_solve="direct", dtMax=1.0e-4,
+        ra=1.0e5, ek=1.0e-3, pr=1.0, radratio=0.35, l_ek_pump=True)
+    assert c.time_scheme == CNAB2 and c.l_cheb_coll == 1
+    return c
 
 
 def c1_collocation_shape():

@@ -148,3 +148,55 @@ def test_graph_replay_is_deterministic():
         b.step(1)
     for f in ("temp", "xi", "om", "psi"):
         assert np.array_equal(a.get_field(f), b.get_field(f)), f
+
+
+def test_rotekpump_collocation_golden():
+    """Reference sample RotE1e3EkPump (collocation, CNAB2, Ekman pumping):
+    the e_kin_3D series of 1000 steps against the reference's golden file
+    (samples/RotE1e3EkPump/reference.out) at the reference test's rtol."""
+    import os
+    from helpers import GOLDEN
+    from pizza_amd.config import config_from_namelist
+    from pizza_amd.runtime import Run
+    cfg, start, _ = config_from_namelist(os.path.join(GOLDEN, "rotekpump_input_start.nml"))
+    assert cfg.l_cheb_coll == 1
+    ref = np.loadtxt(os.path.join(GOLDEN, "rotekpump_start_e_kin_3D.txt"))
+    g = Run(cfg)
+    g.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+    g.start()
+    rows = []
+    for k in range(len(ref)):
+        if k:
+            g.step(100)
+        t, _ = g.time()
+        rows.append(np.r_[t, g.ekin_3d()])
+    np.testing.assert_allclose(np.array(rows), ref, rtol=1e-8, atol=1e-20)
+
+
+@pytest.mark.parametrize("case", ["c1", "bpr353_xi"])
+def test_collocation_parity(case):
+    """Collocation method: fields against the oracle after 1 and 20 steps,
+    on the small collocation case (CNAB2) and on the thermochemical DIRK
+    physics (BPR353, T + xi) at n_r = 33."""
+    from pizza_amd import workloads as W
+    if case == "c1":
+        cfg = W.c1_collocation()
+    else:
+        cfg = W.small_bpr353()
+        cfg.l_cheb_coll = 1
+    floors = _floor(cfg, [1, 20], lambda x: W.seed_run(x, cfg))
+    g, o = _pair(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg)
+        x.start()
+    g.step(1)
+    o.step(1)
+    for f in _fields(cfg):
+        err = rel(g.get_field(f), o.get_field(f))
+        assert err < max(1e-11, 10 * floors[0][f]), (f, err, floors[0][f])
+    g.step(19)
+    o.step(19)
+    assert abs(g.time()[0] - o.time()[0]) < 1e-14
+    for f in _fields(cfg):
+        err = rel(g.get_field(f), o.get_field(f))
+        assert err < max(TOLN, 10 * floors[1][f]), (f, err, floors[1][f])
