@@ -23,10 +23,25 @@ extern "C" {
 
 /* Time schemes (select_time_scheme.f90:15-49). */
 enum pz_time_scheme {
-    PZ_CNAB2 = 1,  /* multistep_schemes.f90:56-61 */
-    PZ_ARS222 = 101, /* dirk_schemes.f90:55-62, Butcher tables :287-299 */
-    PZ_ARS443 = 102, /* dirk_schemes.f90:151-158, :464-480 */
-    PZ_BPR353 = 103  /* dirk_schemes.f90:167-174, :481-497 */
+    /* multistep family: initialize multistep_schemes.f90:56-104, weights :137-305 */
+    PZ_CNAB2 = 1,
+    PZ_CNLF = 2,
+    PZ_MODCNAB = 3,
+    PZ_SBDF2 = 4,  /* also BDF2AB2 */
+    PZ_SBDF3 = 5,  /* also BDF3AB3 */
+    PZ_SBDF4 = 6,  /* also BDF4AB4 */
+    PZ_TVB33 = 7,
+    /* DIRK family without an assembly stage: initialize dirk_schemes.f90:55-182,
+       Butcher tables :286-525 */
+    PZ_ARS222 = 101,
+    PZ_ARS443 = 102,
+    PZ_BPR353 = 103,
+    PZ_DP1_242 = 104,
+    PZ_DP2_242 = 105,
+    PZ_LZ232 = 106,
+    PZ_CK232 = 107,
+    PZ_LZ453 = 108,
+    PZ_PC2 = 109
 };
 
 /* Field identifiers for pz_set_field / pz_get_field (fields.f90). */

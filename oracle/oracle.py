@@ -42,12 +42,30 @@ def lib():
         L.or_ifft.argtypes = [vp, dp, dp, ip]
         L.or_grid.argtypes = [vp, dp, dp]
         L.or_stencil.argtypes = [vp, ip, ip, ip, ip, dp]
+        L.or_scheme_weights.argtypes = [ip, ctypes.c_double, dp, ctypes.POINTER(ip), ctypes.POINTER(ip),
+                                        dp, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def scheme_weights(scheme, dt, alpha=0.6):
+    """Weights of one time scheme for the dt history dt[0..3] (newest
+    first): dict with family, nstages, wimp, wimp_lin, wexp (multistep) and
+    the dt-scaled Butcher tables bimp, bexp (DIRK)."""
+    d = np.ascontiguousarray(np.r_[np.asarray(dt, float), np.ones(4)][:4])
+    w = {k: np.zeros(8) for k in ("wimp", "wimp_lin", "wexp")}
+    b = {k: np.zeros((8, 8)) for k in ("bimp", "bexp")}
+    fam, nst = ctypes.c_int(), ctypes.c_int()
+    rc = lib().or_scheme_weights(scheme, alpha, _dp(d), ctypes.byref(fam), ctypes.byref(nst),
+                                 _dp(w["wimp"]), _dp(w["wimp_lin"]), _dp(w["wexp"]),
+                                 _dp(b["bimp"]), _dp(b["bexp"]))
+    if rc:
+        raise ValueError(f"unknown scheme {scheme}")
+    return dict(family=fam.value, nstages=nst.value, **w, **b)
 
 
 class Oracle:

@@ -1191,75 +1191,215 @@ static int build_lhs_psi(or_handle *h, optabs *T, bordc *A, double *fac, int m, 
 /* ------------------------------------------------------------------------ */
 /* Time schemes: multistep_schemes.f90, dirk_schemes.f90                    */
 /* ------------------------------------------------------------------------ */
+static void ts_butcher(tscheme *t, int nst, const double *bi, const double *be)
+{
+    int n = nst + 1;
+    t->nimp = t->nexp = t->nstages = nst;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) { t->bimp0[i][j] = bi[i * n + j]; t->bexp0[i][j] = be[i * n + j]; }
+    for (int s = 0; s < nst; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
+}
+
+/* initialize (multistep_schemes.f90:35-125; dirk_schemes.f90:34-263) and
+ * the DIRK tables of set_weights (dirk_schemes.f90:286-525) */
 static int ts_init(tscheme *t, int scheme, double courfac_nml)
 {
     memset(t, 0, sizeof(*t));
     t->scheme = scheme;
+    t->nold = 1;
+    t->istage = 1;
     double cf;
-    if (scheme == PZ_CNAB2) { /* multistep_schemes.f90:40-61 */
+    if (scheme >= PZ_CNAB2 && scheme <= PZ_TVB33) {
         t->family = 0;
-        t->nstages = 1; t->istage = 1;
-        t->nimp = 1; t->nold = 1; t->nexp = 2;
+        t->nstages = 1;
         t->l_exp_calc[0] = 1; t->l_imp_calc_rhs[0] = 1;
-        cf = 3.0;
+        switch (scheme) {
+        case PZ_CNAB2: t->nimp = 1; t->nold = 1; t->nexp = 2; cf = 3.0; break;           /* :58-63 */
+        case PZ_MODCNAB: t->nold = 2; t->nimp = 2; t->nexp = 2; cf = 2.9; break;         /* :64-69 */
+        case PZ_CNLF: t->nold = 2; t->nimp = 2; t->nexp = 2; cf = 3.0; break;            /* :70-75 */
+        case PZ_SBDF2: t->nold = 2; t->nimp = 1; t->nexp = 2; t->l_imp_calc_rhs[0] = 0; cf = 2.8; break;
+        case PZ_SBDF3: t->nold = 3; t->nimp = 1; t->nexp = 3; t->l_imp_calc_rhs[0] = 0; cf = 4.0; break;
+        case PZ_TVB33: t->nold = 3; t->nimp = 3; t->nexp = 3; cf = 4.0; break;           /* :90-95 */
+        default: t->nold = 4; t->nimp = 1; t->nexp = 4; t->l_imp_calc_rhs[0] = 0; cf = 5.5; break; /* SBDF4 */
+        }
     } else {
         t->family = 1;
-        t->nold = 1;
-        double g;
+        double g, r2 = sqrt(2.0);
         switch (scheme) {
-        case PZ_ARS222: /* dirk_schemes.f90:55-62, 287-299 */
-            t->nimp = t->nexp = t->nstages = 2;
+        case PZ_ARS222: { /* :287-299 */
+            g = 0.5 * (2.0 - r2);
+            double del = 1.0 - 0.5 / g;
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - g, g}, be[9] = {0, 0, 0, g, 0, 0, del, 1 - del, 0};
+            ts_butcher(t, 2, bi, be);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_ARS443: { /* :464-480 */
+            double bi[25] = {0, 0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1.0 / 6.0, 0.5, 0, 0,
+                             0, -0.5, 0.5, 0.5, 0, 0, 1.5, -1.5, 0.5, 0.5};
+            double be[25] = {0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
+                             5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0, 0.25, 1.75, 0.75, -1.75, 0};
+            ts_butcher(t, 4, bi, be);
+            t->wimp_lin0 = 0.5; t->l_imp_calc_rhs[0] = 0; cf = 0.9;
+            break;
+        }
+        case PZ_BPR353: { /* :481-497 */
+            double bi[25] = {0, 0, 0, 0, 0, 0.5, 0.5, 0, 0, 0, 5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
+                             0.5, 0, 0, 0.5, 0, 0.25, 0, 0.75, -0.5, 0.5};
+            double be[25] = {0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
+                             0.25, 0, 0.75, 0, 0, 0.25, 0, 0.75, 0, 0};
+            ts_butcher(t, 4, bi, be);
+            t->wimp_lin0 = 0.5; t->l_exp_calc[3] = 0; cf = 1.0;
+            break;
+        }
+        case PZ_DP1_242: { /* :393-409 */
+            double bi[25] = {0, 0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1.0 / 6.0, 0.5, 0, 0,
+                             0, -0.5, 0.5, 0.5, 0, 0, 1.5, -1.5, 0.5, 0.5};
+            double be[25] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1.0 / 3.0, 0, 0, 0,
+                             0, 1, 0, 0, 0, 0, 0.5, 0, 0.5, 0};
+            ts_butcher(t, 4, bi, be);
+            t->wimp_lin0 = 0.5; t->l_imp_calc_rhs[0] = 0; t->l_exp_calc[0] = 0; t->l_exp_calc[2] = 0;
             cf = 1.35;
-            g = 0.5 * (2.0 - sqrt(2.0));
-            {
-                double del = 1.0 - 0.5 / g;
-                double bi[3][3] = {{0, 0, 0}, {0, g, 0}, {0, 1 - g, g}};
-                double be[3][3] = {{0, 0, 0}, {g, 0, 0}, {del, 1 - del, 0}};
-                for (int i = 0; i < 3; i++)
-                    for (int j = 0; j < 3; j++) { t->bimp0[i][j] = bi[i][j]; t->bexp0[i][j] = be[i][j]; }
-                t->wimp_lin0 = g;
-            }
-            for (int s = 0; s < 2; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
-            t->l_imp_calc_rhs[0] = 0;
             break;
-        case PZ_ARS443: /* dirk_schemes.f90:151-158, 464-480 */
-            t->nimp = t->nexp = t->nstages = 4;
-            cf = 0.9;
-            {
-                double bi[5][5] = {{0, 0, 0, 0, 0}, {0, 0.5, 0, 0, 0}, {0, 1.0 / 6.0, 0.5, 0, 0},
-                                   {0, -0.5, 0.5, 0.5, 0}, {0, 1.5, -1.5, 0.5, 0.5}};
-                double be[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0, 0, 0, 0}, {11.0 / 18.0, 1.0 / 18.0, 0, 0, 0},
-                                   {5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0}, {0.25, 1.75, 0.75, -1.75, 0}};
-                for (int i = 0; i < 5; i++)
-                    for (int j = 0; j < 5; j++) { t->bimp0[i][j] = bi[i][j]; t->bexp0[i][j] = be[i][j]; }
-                t->wimp_lin0 = 0.5;
-            }
-            for (int s = 0; s < 4; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
-            t->l_imp_calc_rhs[0] = 0;
+        }
+        case PZ_DP2_242: { /* :410-426 */
+            g = 1.0 / 3.0;
+            double bi[25] = {0, 0, 0, 0, 0, 0, g, 0, 0, 0, 0, -g, g, 0, 0,
+                             0, 0, 1 - g, g, 0, 0, 0, 0.5, 0.5 - g, g};
+            double be[25] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                             0, 0, 1, 0, 0, 0, 0, 0.5, 0.5, 0};
+            ts_butcher(t, 4, bi, be);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; t->l_exp_calc[0] = 0; cf = 1.35;
             break;
-        case PZ_BPR353: /* dirk_schemes.f90:167-174, 481-497 */
-            t->nimp = t->nexp = t->nstages = 4;
-            cf = 1.0;
-            {
-                double bi[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0.5, 0, 0, 0}, {5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0},
-                                   {0.5, 0, 0, 0.5, 0}, {0.25, 0, 0.75, -0.5, 0.5}};
-                double be[5][5] = {{0, 0, 0, 0, 0}, {1, 0, 0, 0, 0}, {4.0 / 9.0, 2.0 / 9.0, 0, 0, 0},
-                                   {0.25, 0, 0.75, 0, 0}, {0.25, 0, 0.75, 0, 0}};
-                for (int i = 0; i < 5; i++)
-                    for (int j = 0; j < 5; j++) { t->bimp0[i][j] = bi[i][j]; t->bexp0[i][j] = be[i][j]; }
-                t->wimp_lin0 = 0.5;
-            }
-            for (int s = 0; s < 4; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
-            t->l_exp_calc[3] = 0;
+        }
+        case PZ_LZ232: { /* :442-451 */
+            double bi[9] = {0, 0, 0, -0.25, 0.5, 0, 0.5, 0, 0.5}, be[9] = {0, 0, 0, 0.25, 0, 0, -1, 2, 0};
+            ts_butcher(t, 2, bi, be);
+            t->wimp_lin0 = 0.5; cf = 1.25;
             break;
+        }
+        case PZ_CK232: { /* :452-463 */
+            g = 1.0 - 0.5 * r2;
+            double bi[9] = {0, 0, 0, -1.0 / 3.0 + 0.5 * r2, g, 0, 0.75 - 0.25 * r2, -0.75 + 0.75 * r2, g};
+            double be[9] = {0, 0, 0, 2.0 / 3.0, 0, 0, 0.25, 0.75, 0};
+            ts_butcher(t, 2, bi, be);
+            t->wimp_lin0 = g; cf = 1.25;
+            break;
+        }
+        case PZ_LZ453: { /* :498-513 */
+            double bi[25] = {0, 0, 0, 0, 0, -44.0 / 45.0, 1.2, 0, 0, 0, -47.0 / 300.0, -0.71, 1.2, 0, 0,
+                             3.375, -3.25, -59.0 / 120.0, 1.2, 0, 89.0 / 50.0, -486.0 / 55.0, 8.9, -562.0 / 275.0, 1.2};
+            double be[25] = {0, 0, 0, 0, 0, 2.0 / 9.0, 0, 0, 0, 0, 71.0 / 420.0, 23.0 / 140.0, 0, 0, 0,
+                             -281.0 / 336.0, 187.0 / 112.0, 0, 0, 0, 0.1, 0, 0.5, 0.4, 0};
+            ts_butcher(t, 4, bi, be);
+            t->wimp_lin0 = 1.2; cf = 1.15;
+            break;
+        }
+        case PZ_PC2: { /* :514-525 */
+            double bi[16] = {0, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0, 0.5, 0, 0, 0.5};
+            double be[16] = {0, 0, 0, 0, 1, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0};
+            ts_butcher(t, 3, bi, be);
+            t->wimp_lin0 = 0.5; cf = 0.8;
+            break;
+        }
         default:
             return -1;
         }
-        t->istage = 1;
         t->l_assembly = 0;
     }
     t->courfac = fabs(courfac_nml) >= 1.0e3 ? cf : courfac_nml;
     return 0;
+}
+
+/* multistep weights for the dt history (multistep_schemes.f90:137-305) */
+static void ms_weights(int scheme, double alpha, const double *dt, double *wimp, double *wl, double *wexp)
+{
+    double d0 = dt[0];
+    if (scheme == PZ_CNLF) { /* :158-167 */
+        double de = dt[0] / dt[1];
+        wimp[0] = (1.0 - de) * (1.0 + de);
+        wimp[1] = de * de;
+        wl[0] = 0.5 * (1.0 + de) / de * d0;
+        wl[1] = 0.5 * (1.0 + de) * (de - 1.0) / de * d0;
+        wl[2] = 0.5 * (1.0 + de) * d0;
+        wexp[0] = (1.0 + de) * d0;
+        wexp[1] = 0.0;
+    } else if (scheme == PZ_MODCNAB) { /* :168-177 */
+        double de = dt[0] / dt[1];
+        wimp[0] = 1.0;
+        wimp[1] = 0.0;
+        wl[0] = (0.5 + 1.0 / de / 16.0) * d0;
+        wl[1] = (7.0 / 16.0 - 1.0 / de / 16.0) * d0;
+        wl[2] = 1.0 / 16.0 * d0;
+        wexp[0] = (1.0 + 0.5 * de) * d0;
+        wexp[1] = -0.5 * de * d0;
+    } else if (scheme == PZ_SBDF2) { /* :178-186 */
+        double de = dt[0] / dt[1];
+        wl[0] = (1.0 + de) / (1.0 + 2.0 * de) * d0;
+        wl[1] = 0.0;
+        wimp[0] = (1.0 + de) * (1.0 + de) / (1.0 + 2.0 * de);
+        wimp[1] = -de * de / (1.0 + 2.0 * de);
+        wexp[0] = (1.0 + de) * (1.0 + de) * d0 / (1.0 + 2.0 * de);
+        wexp[1] = -de * (1.0 + de) * d0 / (1.0 + 2.0 * de);
+    } else if (scheme == PZ_SBDF3) { /* :187-212 */
+        double dn = dt[1] / dt[0], dn1 = dt[2] / dt[0];
+        double a0 = 1.0 + 1.0 / (1.0 + dn) + 1.0 / (1.0 + dn + dn1);
+        double a1 = (1.0 + dn) * (1.0 + dn + dn1) / (dn * (dn + dn1));
+        double a2 = -(1.0 + dn + dn1) / (dn * dn1 * (1.0 + dn));
+        double a3 = (1.0 + dn) / (dn1 * (dn + dn1) * (1.0 + dn + dn1));
+        double b0 = (1.0 + dn) * (1.0 + dn + dn1) / (dn * (dn + dn1));
+        double b1 = -(1.0 + dn + dn1) / (dn * dn1);
+        double b2 = (1.0 + dn) / (dn1 * (dn + dn1));
+        wl[0] = 1.0 / a0 * d0; wl[1] = 0.0;
+        wimp[0] = a1 / a0; wimp[1] = a2 / a0; wimp[2] = a3 / a0;
+        wexp[0] = b0 / a0 * d0; wexp[1] = b1 / a0 * d0; wexp[2] = b2 / a0 * d0;
+    } else if (scheme == PZ_TVB33) { /* :213-263 */
+        double gam = (2619.0 - sqrt(5995281.0)) / 345.0;
+        double th = (3525.0 * gam * gam - 3118.0 * gam - 5084.0) / 5468.0;
+        double c = (2178.0 * th - 959.0 * gam * gam + 130.0 * gam + 726.0) / 4096.0;
+        double dn = dt[0] / dt[1], dn1 = dt[1] / dt[2];
+        double a3 = -pow(dn1, 3) * dn * dn * (3.0 * gam * gam * dn + 2.0 * gam * (1.0 - dn) - 1.0) / (1.0 + dn1) /
+                    (1.0 + dn1 * (1.0 + dn));
+        double a2 = -dn * dn * (gam * dn * dn1 * (2.0 - 3.0 * gam) + (1.0 - 2.0 * gam) * (1.0 + dn1)) / (1.0 + dn);
+        double a1 = -(dn1 * (1.0 + gam * dn) * (1.0 + dn * (3.0 * gam - 2.0)) + dn * (2.0 * gam - 1.0) + 1.0) /
+                        (1.0 + dn1) - th;
+        double a0 = th + (1.0 + 2.0 * gam * dn + dn1 * (1.0 + gam * dn) * (1.0 + 3.0 * gam * dn)) / (1.0 + dn) /
+                             (1.0 + dn1 * (1.0 + dn));
+        double b2 = dn1 * dn1 * dn * (6.0 * gam * (1.0 + gam * dn) + th * (3.0 + 2.0 * dn)) / (6.0 * (1.0 + dn1));
+        double b1 = -gam * dn * (1.0 + dn1 * (1.0 + gam * dn)) - th / 6.0 * dn * (3.0 + dn1 * (3.0 + 2.0 * dn));
+        double b0 = (1.0 + gam * dn) * (1.0 + dn1 * (1.0 + gam * dn)) / (1.0 + dn1) +
+                    th * (1.0 + 0.5 * dn + dn1 * dn * (3.0 + 2.0 * dn) / (6.0 * (1.0 + dn1)));
+        double c3 = th * dn1 * dn1 * dn * (3.0 + 2.0 * dn) / (6.0 * (1.0 + dn1)) - c;
+        double c2 = (c * (1.0 + dn1) * (1.0 + dn1 * (1.0 + dn)) - dn1 * dn1 * dn * dn * gam * (1.0 - gam)) /
+                        (dn1 * dn1 * (1.0 + dn)) - th * dn * (3.0 + dn1 * (3.0 + 2.0 * dn)) / 6.0;
+        double c1 = (dn1 * dn1 * dn * (1.0 - gam) * (1.0 + gam * dn) - c * (1.0 + dn1 * (1.0 + dn))) / (dn1 * dn1) / dn +
+                    th * (1.0 + 0.5 * dn + dn1 * dn * (3.0 + 2.0 * dn) / (6.0 * (1.0 + dn1)));
+        double c0 = (dn1 * dn1 * dn * gam * (1.0 + gam * dn) + c * (1.0 + dn1)) / (dn1 * dn1 * dn * (1.0 + dn));
+        wl[0] = c0 / a0 * d0; wl[1] = c1 / a0 * d0; wl[2] = c2 / a0 * d0; wl[3] = c3 / a0 * d0;
+        wimp[0] = -a1 / a0; wimp[1] = -a2 / a0; wimp[2] = -a3 / a0;
+        wexp[0] = b0 / a0 * d0; wexp[1] = b1 / a0 * d0; wexp[2] = b2 / a0 * d0;
+    } else if (scheme == PZ_SBDF4) { /* :264-302 */
+        double dn = dt[0] / dt[1], dn1 = dt[1] / dt[2], dn2 = dt[2] / dt[3];
+        double c1 = 1.0 + dn2 * (1.0 + dn1), c2 = 1.0 + dn1 * (1.0 + dn), c3 = 1.0 + dn2 * c2;
+        double a4 = (1.0 + dn) / (1.0 + dn2) * c2 / c1 / c3 * (pow(dn2, 4) * pow(dn1, 3) * dn * dn);
+        double a3 = -pow(dn1, 3) * dn * dn * (1.0 + dn) / (1.0 + dn1) * c3 / c2;
+        double a2 = dn * (dn / (1.0 + dn) + dn1 * dn * (c3 + dn2) / (1.0 + dn2));
+        double a1 = -1.0 - dn * (1.0 + dn1 * (1.0 + dn) * (1.0 + dn2 * c2 / c1) / (1.0 + dn1));
+        double a0 = 1.0 + dn / (1.0 + dn) + dn1 * dn / c2 + dn2 * dn1 * dn / c3;
+        double b3 = -pow(dn2, 3) * dn1 * dn1 * dn * (1.0 + dn) / (1.0 + dn2) * c2 / c1;
+        double b2 = dn1 * dn1 * dn * (1.0 + dn) / (1.0 + dn1) * c3;
+        double b1 = -c2 * c3 * dn / (1.0 + dn2);
+        double b0 = dn1 * (1.0 + dn) / (1.0 + dn1) * ((1.0 + dn) * (c3 + dn2) + (1.0 + dn2) / dn1) / c1;
+        wl[0] = 1.0 / a0 * d0; wl[1] = 0.0;
+        wimp[0] = -a1 / a0; wimp[1] = -a2 / a0; wimp[2] = -a3 / a0; wimp[3] = -a4 / a0;
+        wexp[0] = b0 / a0 * d0; wexp[1] = b1 / a0 * d0; wexp[2] = b2 / a0 * d0; wexp[3] = b3 / a0 * d0;
+    } else { /* CNAB2, :151-157 */
+        wimp[0] = 1.0;
+        wl[0] = alpha * d0;
+        wl[1] = (1 - alpha) * d0;
+        wexp[0] = (1.0 + 0.5 * dt[0] / dt[1]) * d0;
+        wexp[1] = -0.5 * dt[0] * dt[0] / dt[1];
+    }
 }
 
 /* set_weights (multistep_schemes.f90:137-305; dirk_schemes.f90:275-713) */
@@ -1267,11 +1407,7 @@ static void ts_set_weights(tscheme *t, double alpha, int *lMatNext)
 {
     double wimp_old = t->wimp_lin[0];
     if (t->family == 0) {
-        t->wimp[0] = 1.0;
-        t->wimp_lin[0] = alpha * t->dt[0];
-        t->wimp_lin[1] = (1 - alpha) * t->dt[0];
-        t->wexp[0] = (1.0 + 0.5 * t->dt[0] / t->dt[1]) * t->dt[0];
-        t->wexp[1] = -0.5 * t->dt[0] * t->dt[0] / t->dt[1];
+        ms_weights(t->scheme, alpha, t->dt, t->wimp, t->wimp_lin, t->wexp);
         if (t->wimp_lin[0] != wimp_old) *lMatNext = 1;
     } else {
         t->wimp_lin[0] = t->dt[0] * t->wimp_lin0;
@@ -1281,6 +1417,34 @@ static void ts_set_weights(tscheme *t, double alpha, int *lMatNext)
                 t->bexp[i][j] = t->dt[0] * t->bexp0[i][j];
             }
     }
+}
+
+/* bridge_with_cnab2 (multistep_schemes.f90:500-524): CNAB2 weights, the
+ * remaining ones zero */
+static void ts_bridge(tscheme *t, double alpha)
+{
+    int save = t->scheme;
+    t->scheme = PZ_CNAB2;
+    ms_weights(PZ_CNAB2, alpha, t->dt, t->wimp, t->wimp_lin, t->wexp);
+    t->scheme = save;
+    for (int i = 1; i < 8; i++) t->wimp[i] = 0.0;
+    for (int i = 2; i < 8; i++) { t->wimp_lin[i] = 0.0; t->wexp[i] = 0.0; }
+}
+
+int or_scheme_weights(int scheme, double alpha, const double *dt, int *family, int *nstages,
+                      double *wimp, double *wimp_lin, double *wexp, double *bimp, double *bexp)
+{
+    tscheme t;
+    if (ts_init(&t, scheme, 1.0e4)) return -1;
+    for (int i = 0; i < 4; i++) t.dt[i] = dt[i];
+    int dummy = 0;
+    ts_set_weights(&t, alpha, &dummy);
+    *family = t.family;
+    *nstages = t.nstages;
+    for (int i = 0; i < 8; i++) { wimp[i] = t.wimp[i]; wimp_lin[i] = t.wimp_lin[i]; wexp[i] = t.wexp[i]; }
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) { bimp[i * 8 + j] = t.bimp[i][j]; bexp[i * 8 + j] = t.bexp[i][j]; }
+    return 0;
 }
 
 /* set_dt_array (multistep_schemes.f90:307-348; dirk_schemes.f90:715-771) */
@@ -2506,6 +2670,23 @@ static int one_step(or_handle *h)
         lMat = h->lMatNext;
         if ((l_new_dt || lMat) && t->istage == 1) lMat = 1;
         h->lMatNext = 0;
+        /* start_from_another_scheme (step_time.f90:417-503) on a fresh start
+         * (l_bridge_step, namelists.f90:365): the first nold-1 steps of a
+         * multistep scheme other than CNAB2 recompute the implicit terms and
+         * take the CNAB2 weights */
+        if (t->family == 0 && t->scheme != PZ_CNAB2 && h->n_time_step + 1 <= t->nold - 1) {
+            if (h->coll) {
+                if (h->l_heat) coll_rhs_imp_scalar(h, h->temp, h->dtemp, &h->dTdt, 1, 1, 0, h->TdiffFac, h->hdifT);
+                if (h->l_chem) coll_rhs_imp_scalar(h, h->xi, h->dxi, &h->dxidt, 1, 1, 0, h->XidiffFac, h->hdifXi);
+                coll_rhs_imp_psi(h, &h->dpsidt, 1, 1, 0);
+            } else {
+                if (h->l_heat) rhs_imp_scalar(h, h->temp_hat, &h->dTdt, 1, 1, h->TdiffFac, h->hdifT);
+                if (h->l_chem) rhs_imp_scalar(h, h->xi_hat, &h->dxidt, 1, 1, h->XidiffFac, h->hdifXi);
+                /* psi in the composition slot, as at step_time.f90:475-478 */
+                rhs_imp_psi(h, h->psi_hat, h->up, h->temp, h->psi, &h->dpsidt, 1, 1);
+            }
+            ts_bridge(t, h->c.alpha);
+        }
         if (lMat) {
             int info = h->coll ? coll_build(h) : build_matrices(h, 1);
             if (info) return info;
@@ -2695,9 +2876,12 @@ int or_start(or_handle *h)
     size_t F = (size_t)h->nm * N;
     tscheme *t = &h->ts;
     h->time = 0.0;
+    h->n_time_step = 0;
     for (int i = 0; i < 8; i++) t->dt[i] = h->c.dtMax;
     int dummy = 0;
     ts_set_weights(t, h->c.alpha, &dummy);
+    /* dummy Courant limits before the first radial loop (step_time.f90:124-126) */
+    for (int n = 0; n < N; n++) { h->dtr[n] = 1.0e10; h->dth[n] = 1.0e10; }
     for (int i = 0; i < h->nm; i++) {
         if (h->l_heat) get_dr_vec(h, h->temp + (size_t)i * N, h->dtemp + (size_t)i * N, 1);
         if (h->l_chem) get_dr_vec(h, h->xi + (size_t)i * N, h->dxi + (size_t)i * N, 1);

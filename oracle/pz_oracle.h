@@ -38,6 +38,13 @@ int or_get_ddr(or_handle *h, const double *f, double *df, double *ddf, int n_m,
 int or_fft(or_handle *h, const double *grid, double *spec, int n_rad);
 int or_ifft(or_handle *h, const double *spec, double *grid, int n_rad);
 
+/* time-scheme weights for the dt history dt[0..3] (dt[0] newest): family
+ * (0 multistep, 1 DIRK), stage count, multistep weights (8 each) and the
+ * dt-scaled Butcher tables (8 x 8, row-major); returns -1 for an unknown
+ * scheme */
+int or_scheme_weights(int scheme, double alpha, const double *dt, int *family, int *nstages,
+                      double *wimp, double *wimp_lin, double *wexp, double *bimp, double *bexp);
+
 /* radial grid and the Chebyshev x grid (n_r_max values each) */
 int or_grid(or_handle *h, double *r, double *x);
 

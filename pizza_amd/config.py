@@ -10,8 +10,35 @@ import re
 
 N_T_BOUNDS = 32  # namelists.f90:23
 
-CNAB2, ARS222, ARS443, BPR353 = 1, 101, 102, 103
-SCHEMES = {"CNAB2": CNAB2, "ARS222": ARS222, "ARS443": ARS443, "BPR353": BPR353}
+CNAB2, CNLF, MODCNAB, SBDF2, SBDF3, SBDF4, TVB33 = 1, 2, 3, 4, 5, 6, 7
+ARS222, ARS443, BPR353, DP1_242, DP2_242, LZ232, CK232, LZ453, PC2 = range(101, 110)
+
+# The DIRK names, in the order initialize() of dirk_schemes.f90:55-230 tests
+# them; None marks schemes with an assembly stage, which are not implemented.
+DIRK_ORDER = [("ARS222", ARS222), ("SSP222", None), ("LSDIRK222", None), ("ARS232", None),
+              ("ARS233", None), ("SSP322", None), ("SSP332", None), ("SSP433", None),
+              ("DP1-242", DP1_242), ("DP2-242", DP2_242), ("LZ232", LZ232), ("CK232", CK232),
+              ("ARS443", ARS443), ("LZ453", LZ453), ("BPR353", BPR353), ("PC2", PC2),
+              ("ARS343", None), ("MARS343", None), ("KC343", None), ("KC564", None),
+              ("KC785", None), ("DBM453", None)]
+# The multistep names in the order of multistep_schemes.f90:56-104
+MULTISTEP_ORDER = [("CNAB2", CNAB2), ("MODCNAB", MODCNAB), ("CNLF", CNLF), ("BDF2AB2", SBDF2),
+                   ("SBDF2", SBDF2), ("BDF3AB3", SBDF3), ("SBDF3", SBDF3), ("TVB33", TVB33),
+                   ("BDF4AB4", SBDF4), ("SBDF4", SBDF4)]
+SCHEMES = dict(MULTISTEP_ORDER + [(n, v) for n, v in DIRK_ORDER if v is not None])
+
+
+def scheme_id(name):
+    """The scheme a time_scheme string selects: the family test of
+    select_time_scheme.f90:15-49, then the first name initialize() matches."""
+    name = str(name).upper()
+    order = DIRK_ORDER if any(n in name for n, _ in DIRK_ORDER) else MULTISTEP_ORDER
+    for n, v in order:
+        if n in name:
+            if v is None:
+                raise ValueError(f"time scheme {n} (with an assembly stage) is not implemented")
+            return v
+    raise ValueError(f"time scheme {name!r} was not found")
 
 US, UP, OM, PSI, TEMP, XI, TEMP_HAT, XI_HAT, PSI_HAT, DTEMP = range(10)
 FIELDS = {"us": US, "up": UP, "om": OM, "psi": PSI, "temp": TEMP, "xi": XI,
@@ -87,11 +114,7 @@ def set_option(c, key, value):
     """Sets one namelist entry (case-insensitive key) on a PzConfig."""
     k = key.lower()
     if k == "time_scheme":
-        name = str(value).upper()
-        hit = [v for s, v in SCHEMES.items() if s in name]
-        if not hit:
-            raise ValueError(f"time scheme {value!r} is not supported")
-        c.time_scheme = hit[0]
+        c.time_scheme = scheme_id(value)
     elif k == "cheb_method":
         c.l_cheb_coll = 1 if "COLL" in str(value).upper() else 0
     elif k == "matrix_solve":

@@ -20,64 +20,140 @@ using pzh::ld;
 namespace {
 
 struct Scheme {
-    int family = 0, nstages = 1, nexp = 2, nimp = 1, nold = 1;
+    int id = PZ_CNAB2, family = 0, nstages = 1, nexp = 2, nimp = 1, nold = 1;
     double courfac = 0, wimp_lin0 = 0;
     double bimp0[PZ_MAX_STAGES][PZ_MAX_STAGES] = {}, bexp0[PZ_MAX_STAGES][PZ_MAX_STAGES] = {};
     int l_exp_calc[8] = {}, l_imp_calc_rhs[8] = {};
 };
 
+// Butcher tables of a DIRK scheme without an assembly stage: n = nstages + 1
+// rows, row-major; the implicit diagonal coefficient is bi[1][1].
+void set_butcher(Scheme &s, int nst, const double *bi, const double *be)
+{
+    const int n = nst + 1;
+    s.nstages = s.nimp = s.nexp = nst;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) { s.bimp0[i][j] = bi[i * n + j]; s.bexp0[i][j] = be[i * n + j]; }
+    for (int k = 0; k < nst; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
+}
+
 // select_time_scheme / initialize of multistep_schemes.f90:40-135 and
-// dirk_schemes.f90:34-263 for the supported schemes
+// dirk_schemes.f90:34-263, with the tables of set_weights
+// (dirk_schemes.f90:286-525) for the schemes without an assembly stage
 bool make_scheme(int id, double courfac_nml, Scheme &s)
 {
     double cf;
-    if (id == PZ_CNAB2) {
+    s.id = id;
+    s.nold = 1;
+    if (id >= PZ_CNAB2 && id <= PZ_TVB33) {
         s.family = 0;
         s.nstages = 1;
-        s.nimp = 1;
-        s.nold = 1;
-        s.nexp = 2;
         s.l_exp_calc[0] = s.l_imp_calc_rhs[0] = 1;
-        cf = 3.0;
+        switch (id) {
+        case PZ_CNAB2: s.nimp = 1; s.nold = 1; s.nexp = 2; cf = 3.0; break;
+        case PZ_MODCNAB: s.nimp = 2; s.nold = 2; s.nexp = 2; cf = 2.9; break;
+        case PZ_CNLF: s.nimp = 2; s.nold = 2; s.nexp = 2; cf = 3.0; break;
+        case PZ_SBDF2: s.nimp = 1; s.nold = 2; s.nexp = 2; s.l_imp_calc_rhs[0] = 0; cf = 2.8; break;
+        case PZ_SBDF3: s.nimp = 1; s.nold = 3; s.nexp = 3; s.l_imp_calc_rhs[0] = 0; cf = 4.0; break;
+        case PZ_TVB33: s.nimp = 3; s.nold = 3; s.nexp = 3; cf = 4.0; break;
+        default: s.nimp = 1; s.nold = 4; s.nexp = 4; s.l_imp_calc_rhs[0] = 0; cf = 5.5; break; // SBDF4
+        }
     } else {
         s.family = 1;
-        s.nold = 1;
-        if (id == PZ_ARS222) {
-            s.nimp = s.nexp = s.nstages = 2;
-            cf = 1.35;
-            double g = 0.5 * (2.0 - std::sqrt(2.0)), del = 1.0 - 0.5 / g;
-            double bi[3][3] = {{0, 0, 0}, {0, g, 0}, {0, 1 - g, g}};
-            double be[3][3] = {{0, 0, 0}, {g, 0, 0}, {del, 1 - del, 0}};
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) { s.bimp0[i][j] = bi[i][j]; s.bexp0[i][j] = be[i][j]; }
+        const double r2 = std::sqrt(2.0);
+        switch (id) {
+        case PZ_ARS222: {
+            const double g = 0.5 * (2.0 - r2), del = 1.0 - 0.5 / g;
+            const double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - g, g}, be[9] = {0, 0, 0, g, 0, 0, del, 1 - del, 0};
+            set_butcher(s, 2, bi, be);
             s.wimp_lin0 = g;
-            for (int k = 0; k < 2; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
             s.l_imp_calc_rhs[0] = 0;
-        } else if (id == PZ_ARS443) {
-            s.nimp = s.nexp = s.nstages = 4;
+            cf = 1.35;
+            break;
+        }
+        case PZ_ARS443: {
+            const double bi[25] = {0, 0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1.0 / 6.0, 0.5, 0, 0,
+                                   0, -0.5, 0.5, 0.5, 0, 0, 1.5, -1.5, 0.5, 0.5};
+            const double be[25] = {0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 11.0 / 18.0, 1.0 / 18.0, 0, 0, 0,
+                                   5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0, 0.25, 1.75, 0.75, -1.75, 0};
+            set_butcher(s, 4, bi, be);
+            s.wimp_lin0 = 0.5;
+            s.l_imp_calc_rhs[0] = 0;
             cf = 0.9;
-            double bi[5][5] = {{0, 0, 0, 0, 0}, {0, 0.5, 0, 0, 0}, {0, 1.0 / 6.0, 0.5, 0, 0},
-                               {0, -0.5, 0.5, 0.5, 0}, {0, 1.5, -1.5, 0.5, 0.5}};
-            double be[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0, 0, 0, 0}, {11.0 / 18.0, 1.0 / 18.0, 0, 0, 0},
-                               {5.0 / 6.0, -5.0 / 6.0, 0.5, 0, 0}, {0.25, 1.75, 0.75, -1.75, 0}};
-            for (int i = 0; i < 5; i++)
-                for (int j = 0; j < 5; j++) { s.bimp0[i][j] = bi[i][j]; s.bexp0[i][j] = be[i][j]; }
+            break;
+        }
+        case PZ_BPR353: {
+            const double bi[25] = {0, 0, 0, 0, 0, 0.5, 0.5, 0, 0, 0, 5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0,
+                                   0.5, 0, 0, 0.5, 0, 0.25, 0, 0.75, -0.5, 0.5};
+            const double be[25] = {0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 4.0 / 9.0, 2.0 / 9.0, 0, 0, 0,
+                                   0.25, 0, 0.75, 0, 0, 0.25, 0, 0.75, 0, 0};
+            set_butcher(s, 4, bi, be);
             s.wimp_lin0 = 0.5;
-            for (int k = 0; k < 4; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
-            s.l_imp_calc_rhs[0] = 0;
-        } else if (id == PZ_BPR353) {
-            s.nimp = s.nexp = s.nstages = 4;
-            cf = 1.0;
-            double bi[5][5] = {{0, 0, 0, 0, 0}, {0.5, 0.5, 0, 0, 0}, {5.0 / 18.0, -1.0 / 9.0, 0.5, 0, 0},
-                               {0.5, 0, 0, 0.5, 0}, {0.25, 0, 0.75, -0.5, 0.5}};
-            double be[5][5] = {{0, 0, 0, 0, 0}, {1, 0, 0, 0, 0}, {4.0 / 9.0, 2.0 / 9.0, 0, 0, 0},
-                               {0.25, 0, 0.75, 0, 0}, {0.25, 0, 0.75, 0, 0}};
-            for (int i = 0; i < 5; i++)
-                for (int j = 0; j < 5; j++) { s.bimp0[i][j] = bi[i][j]; s.bexp0[i][j] = be[i][j]; }
-            s.wimp_lin0 = 0.5;
-            for (int k = 0; k < 4; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
             s.l_exp_calc[3] = 0;
-        } else {
+            cf = 1.0;
+            break;
+        }
+        case PZ_DP1_242: {
+            const double bi[25] = {0, 0, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1.0 / 6.0, 0.5, 0, 0,
+                                   0, -0.5, 0.5, 0.5, 0, 0, 1.5, -1.5, 0.5, 0.5};
+            const double be[25] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1.0 / 3.0, 0, 0, 0,
+                                   0, 1, 0, 0, 0, 0, 0.5, 0, 0.5, 0};
+            set_butcher(s, 4, bi, be);
+            s.wimp_lin0 = 0.5;
+            s.l_imp_calc_rhs[0] = 0;
+            s.l_exp_calc[0] = s.l_exp_calc[2] = 0;
+            cf = 1.35;
+            break;
+        }
+        case PZ_DP2_242: {
+            const double g = 1.0 / 3.0;
+            const double bi[25] = {0, 0, 0, 0, 0, 0, g, 0, 0, 0, 0, -g, g, 0, 0,
+                                   0, 0, 1 - g, g, 0, 0, 0, 0.5, 0.5 - g, g};
+            const double be[25] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                                   0, 0, 1, 0, 0, 0, 0, 0.5, 0.5, 0};
+            set_butcher(s, 4, bi, be);
+            s.wimp_lin0 = g;
+            s.l_imp_calc_rhs[0] = 0;
+            s.l_exp_calc[0] = 0;
+            cf = 1.35;
+            break;
+        }
+        case PZ_LZ232: {
+            const double bi[9] = {0, 0, 0, -0.25, 0.5, 0, 0.5, 0, 0.5}, be[9] = {0, 0, 0, 0.25, 0, 0, -1, 2, 0};
+            set_butcher(s, 2, bi, be);
+            s.wimp_lin0 = 0.5;
+            cf = 1.25;
+            break;
+        }
+        case PZ_CK232: {
+            const double g = 1.0 - 0.5 * r2;
+            const double bi[9] = {0, 0, 0, -1.0 / 3.0 + 0.5 * r2, g, 0, 0.75 - 0.25 * r2, -0.75 + 0.75 * r2, g};
+            const double be[9] = {0, 0, 0, 2.0 / 3.0, 0, 0, 0.25, 0.75, 0};
+            set_butcher(s, 2, bi, be);
+            s.wimp_lin0 = g;
+            cf = 1.25;
+            break;
+        }
+        case PZ_LZ453: {
+            const double bi[25] = {0, 0, 0, 0, 0, -44.0 / 45.0, 1.2, 0, 0, 0, -47.0 / 300.0, -0.71, 1.2, 0, 0,
+                                   3.375, -3.25, -59.0 / 120.0, 1.2, 0,
+                                   89.0 / 50.0, -486.0 / 55.0, 8.9, -562.0 / 275.0, 1.2};
+            const double be[25] = {0, 0, 0, 0, 0, 2.0 / 9.0, 0, 0, 0, 0, 71.0 / 420.0, 23.0 / 140.0, 0, 0, 0,
+                                   -281.0 / 336.0, 187.0 / 112.0, 0, 0, 0, 0.1, 0, 0.5, 0.4, 0};
+            set_butcher(s, 4, bi, be);
+            s.wimp_lin0 = 1.2;
+            cf = 1.15;
+            break;
+        }
+        case PZ_PC2: {
+            const double bi[16] = {0, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0, 0.5, 0, 0, 0.5};
+            const double be[16] = {0, 0, 0, 0, 1, 0, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0.5, 0};
+            set_butcher(s, 3, bi, be);
+            s.wimp_lin0 = 0.5;
+            cf = 0.8;
+            break;
+        }
+        default:
             return false;
         }
     }
@@ -368,6 +444,10 @@ static void setup(pz_handle *h)
     d.ncheb = c.n_cheb_max;
     d.L = 2 * N - 2;
     d.family = h->sch.family;
+    d.scheme = h->sch.id;
+    // a fresh start sets l_bridge_step (namelists.f90:364-365): the first
+    // nold-1 steps of a multistep scheme other than CNAB2 are CNAB2 steps
+    d.bridge = h->sch.family == 0 && h->sch.id != PZ_CNAB2;
     d.nstages = h->sch.nstages;
     d.nexp = h->sch.nexp;
     d.nimp = h->sch.nimp;
@@ -650,6 +730,10 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
     else if (c.l_cheb_coll && c.n_r_max > 256) why = "the collocation method needs n_r_max <= 256";
     else if (c.n_cheb_max < 1 || c.n_cheb_max > c.n_r_max) why = "n_cheb_max must be in [1, n_r_max]";
     else if (c.m_max < 3 || c.minc < 1 || c.m_max % c.minc) why = "invalid m_max/minc";
+    else {
+        Scheme probe;
+        if (!make_scheme(c.time_scheme, c.courfac, probe)) why = "time scheme not implemented";
+    }
     if (!why.empty()) {
         int rc = fail(h, -2, why);
         std::fprintf(stderr, "pz_create: %s\n", why.c_str());

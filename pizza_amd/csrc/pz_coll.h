@@ -421,7 +421,9 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll(const PzDev *__rest
     for (int k = nc + tid; k < N; k += blockDim.x) { aT[k] = mk(0, 0); aX[k] = mk(0, 0); }
     __syncthreads();
     const int inext = (istage == p.nstages) ? 1 : istage + 1;
-    const int lcalc = p.l_imp_calc_rhs[inext - 1];
+    // implicit terms for the next stage; also before a CNAB2 start-up step
+    // of a multistep scheme, which recomputes them (step_time.f90:417-503)
+    const int lcalc = p.l_imp_calc_rhs[inext - 1] || (p.bridge && ss->n_steps + 1 <= p.nold - 1);
     if (lcalc) {
         // get_ddr (coefficients in) and costf: derivatives and grid values
         if (hT) {

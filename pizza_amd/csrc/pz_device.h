@@ -362,15 +362,118 @@ __global__ void __launch_bounds__(256) k_radial_loop(const PzDev p __attribute__
 // (multistep_schemes.f90:137-348; dirk_schemes.f90:275-771), lMat logic of
 // step_time.f90:277-304.
 // ---------------------------------------------------------------------------
+// Weights of the multistep schemes for the dt history dt[0..3]
+// (multistep_schemes.f90:137-305): rhs = sum wimp[o] old[o] +
+// sum wimp_lin[o+1] impl[o] + sum wexp[o] expl[o], LHS weight wimp_lin[0].
+__device__ void multistep_weights(int id, double alpha, const double *dt, double *wimp, double *wl, double *wexp)
+{
+    const double d0 = dt[0];
+    switch (id) {
+    case PZ_CNLF: {
+        const double de = dt[0] / dt[1];
+        wimp[0] = (1.0 - de) * (1.0 + de);
+        wimp[1] = de * de;
+        wl[0] = 0.5 * (1.0 + de) / de * d0;
+        wl[1] = 0.5 * (1.0 + de) * (de - 1.0) / de * d0;
+        wl[2] = 0.5 * (1.0 + de) * d0;
+        wexp[0] = (1.0 + de) * d0;
+        wexp[1] = 0.0;
+        break;
+    }
+    case PZ_MODCNAB: {
+        const double de = dt[0] / dt[1];
+        wimp[0] = 1.0;
+        wimp[1] = 0.0;
+        wl[0] = (0.5 + 1.0 / de / 16.0) * d0;
+        wl[1] = (7.0 / 16.0 - 1.0 / de / 16.0) * d0;
+        wl[2] = 1.0 / 16.0 * d0;
+        wexp[0] = (1.0 + 0.5 * de) * d0;
+        wexp[1] = -0.5 * de * d0;
+        break;
+    }
+    case PZ_SBDF2: {
+        const double de = dt[0] / dt[1], q = 1.0 + 2.0 * de;
+        wl[0] = (1.0 + de) / q * d0;
+        wl[1] = 0.0;
+        wimp[0] = (1.0 + de) * (1.0 + de) / q;
+        wimp[1] = -de * de / q;
+        wexp[0] = (1.0 + de) * (1.0 + de) * d0 / q;
+        wexp[1] = -de * (1.0 + de) * d0 / q;
+        break;
+    }
+    case PZ_SBDF3: {
+        const double a = dt[1] / dt[0], b = dt[2] / dt[0];
+        const double a0 = 1.0 + 1.0 / (1.0 + a) + 1.0 / (1.0 + a + b);
+        const double a1 = (1.0 + a) * (1.0 + a + b) / (a * (a + b));
+        const double a2 = -(1.0 + a + b) / (a * b * (1.0 + a));
+        const double a3 = (1.0 + a) / (b * (a + b) * (1.0 + a + b));
+        const double b0 = (1.0 + a) * (1.0 + a + b) / (a * (a + b));
+        const double b1 = -(1.0 + a + b) / (a * b);
+        const double b2 = (1.0 + a) / (b * (a + b));
+        wl[0] = 1.0 / a0 * d0;
+        wl[1] = 0.0;
+        wimp[0] = a1 / a0; wimp[1] = a2 / a0; wimp[2] = a3 / a0;
+        wexp[0] = b0 / a0 * d0; wexp[1] = b1 / a0 * d0; wexp[2] = b2 / a0 * d0;
+        break;
+    }
+    case PZ_TVB33: {
+        const double g = (2619.0 - sqrt(5995281.0)) / 345.0;
+        const double th = (3525.0 * g * g - 3118.0 * g - 5084.0) / 5468.0;
+        const double c = (2178.0 * th - 959.0 * g * g + 130.0 * g + 726.0) / 4096.0;
+        const double a = dt[0] / dt[1], b = dt[1] / dt[2];
+        const double a3 = -b * b * b * a * a * (3.0 * g * g * a + 2.0 * g * (1.0 - a) - 1.0) / (1.0 + b) /
+                          (1.0 + b * (1.0 + a));
+        const double a2 = -a * a * (g * a * b * (2.0 - 3.0 * g) + (1.0 - 2.0 * g) * (1.0 + b)) / (1.0 + a);
+        const double a1 = -(b * (1.0 + g * a) * (1.0 + a * (3.0 * g - 2.0)) + a * (2.0 * g - 1.0) + 1.0) / (1.0 + b) - th;
+        const double a0 = th + (1.0 + 2.0 * g * a + b * (1.0 + g * a) * (1.0 + 3.0 * g * a)) / (1.0 + a) /
+                                   (1.0 + b * (1.0 + a));
+        const double q = 1.0 + 0.5 * a + b * a * (3.0 + 2.0 * a) / (6.0 * (1.0 + b));
+        const double b2 = b * b * a * (6.0 * g * (1.0 + g * a) + th * (3.0 + 2.0 * a)) / (6.0 * (1.0 + b));
+        const double b1 = -g * a * (1.0 + b * (1.0 + g * a)) - th / 6.0 * a * (3.0 + b * (3.0 + 2.0 * a));
+        const double b0 = (1.0 + g * a) * (1.0 + b * (1.0 + g * a)) / (1.0 + b) + th * q;
+        const double c3 = th * b * b * a * (3.0 + 2.0 * a) / (6.0 * (1.0 + b)) - c;
+        const double c2 = (c * (1.0 + b) * (1.0 + b * (1.0 + a)) - b * b * a * a * g * (1.0 - g)) / (b * b * (1.0 + a)) -
+                          th * a * (3.0 + b * (3.0 + 2.0 * a)) / 6.0;
+        const double c1 = (b * b * a * (1.0 - g) * (1.0 + g * a) - c * (1.0 + b * (1.0 + a))) / (b * b) / a + th * q;
+        const double c0 = (b * b * a * g * (1.0 + g * a) + c * (1.0 + b)) / (b * b * a * (1.0 + a));
+        wl[0] = c0 / a0 * d0; wl[1] = c1 / a0 * d0; wl[2] = c2 / a0 * d0; wl[3] = c3 / a0 * d0;
+        wimp[0] = -a1 / a0; wimp[1] = -a2 / a0; wimp[2] = -a3 / a0;
+        wexp[0] = b0 / a0 * d0; wexp[1] = b1 / a0 * d0; wexp[2] = b2 / a0 * d0;
+        break;
+    }
+    case PZ_SBDF4: {
+        const double a = dt[0] / dt[1], b = dt[1] / dt[2], e = dt[2] / dt[3];
+        const double c1 = 1.0 + e * (1.0 + b), c2 = 1.0 + b * (1.0 + a), c3 = 1.0 + e * c2;
+        const double a4 = (1.0 + a) / (1.0 + e) * c2 / c1 / c3 * (e * e * e * e * b * b * b * a * a);
+        const double a3 = -b * b * b * a * a * (1.0 + a) / (1.0 + b) * c3 / c2;
+        const double a2 = a * (a / (1.0 + a) + b * a * (c3 + e) / (1.0 + e));
+        const double a1 = -1.0 - a * (1.0 + b * (1.0 + a) * (1.0 + e * c2 / c1) / (1.0 + b));
+        const double a0 = 1.0 + a / (1.0 + a) + b * a / c2 + e * b * a / c3;
+        const double b3 = -e * e * e * b * b * a * (1.0 + a) / (1.0 + e) * c2 / c1;
+        const double b2 = b * b * a * (1.0 + a) / (1.0 + b) * c3;
+        const double b1 = -c2 * c3 * a / (1.0 + e);
+        const double b0 = b * (1.0 + a) / (1.0 + b) * ((1.0 + a) * (c3 + e) + (1.0 + e) / b) / c1;
+        wl[0] = 1.0 / a0 * d0;
+        wl[1] = 0.0;
+        wimp[0] = -a1 / a0; wimp[1] = -a2 / a0; wimp[2] = -a3 / a0; wimp[3] = -a4 / a0;
+        wexp[0] = b0 / a0 * d0; wexp[1] = b1 / a0 * d0; wexp[2] = b2 / a0 * d0; wexp[3] = b3 / a0 * d0;
+        break;
+    }
+    default: // CNAB2
+        wimp[0] = 1.0;
+        wl[0] = alpha * d0;
+        wl[1] = (1 - alpha) * d0;
+        wexp[0] = (1.0 + 0.5 * dt[0] / dt[1]) * d0;
+        wexp[1] = -0.5 * dt[0] * dt[0] / dt[1];
+        break;
+    }
+}
+
 __device__ void scheme_weights(const PzDev &p, PzScheme *s, int *lMatNext)
 {
     double wimp_old = s->wimp_lin[0];
     if (p.family == 0) {
-        s->wimp[0] = 1.0;
-        s->wimp_lin[0] = p.alpha * s->dt[0];
-        s->wimp_lin[1] = (1 - p.alpha) * s->dt[0];
-        s->wexp[0] = (1.0 + 0.5 * s->dt[0] / s->dt[1]) * s->dt[0];
-        s->wexp[1] = -0.5 * s->dt[0] * s->dt[0] / s->dt[1];
+        multistep_weights(p.scheme, p.alpha, s->dt, s->wimp, s->wimp_lin, s->wexp);
         if (s->wimp_lin[0] != wimp_old) *lMatNext = 1;
     } else {
         s->wimp_lin[0] = s->dt[0] * p.wimp_lin0;
@@ -388,6 +491,8 @@ __global__ void k_set_weights(const PzDev p __attribute__((grid_constant)))
         PzScheme *s = p.ss;
         int dummy = 0;
         scheme_weights(p, s, &dummy);
+        // dummy Courant limits before the first radial loop (step_time.f90:124-126)
+        for (int i = 0; i < p.N; i++) { p.dtr[i] = 1.0e10; p.dth[i] = 1.0e10; }
     }
 }
 
@@ -422,6 +527,17 @@ __global__ void __launch_bounds__(256) k_courant(const PzDev p __attribute__((gr
     int lMatNext = s->lMatNext;
     scheme_weights(p, s, &lMatNext);
     s->time += s->dt[0];
+    if (p.bridge && s->n_steps + 1 <= p.nold - 1) {
+        // bridge_with_cnab2 (multistep_schemes.f90:500-524): this step uses
+        // the CNAB2 weights, the higher-order ones are zero
+        double w[8] = {}, wl[8] = {}, we[8] = {};
+        multistep_weights(PZ_CNAB2, p.alpha, s->dt, w, wl, we);
+        for (int i = 0; i < 8; i++) {
+            s->wimp[i] = i < 1 ? w[i] : 0.0;
+            s->wimp_lin[i] = i < 2 ? wl[i] : 0.0;
+            s->wexp[i] = i < 2 ? we[i] : 0.0;
+        }
+    }
     int lMat = lMatNext;
     if (l_new_dt || lMat) lMat = 1;
     s->lMat = lMat;

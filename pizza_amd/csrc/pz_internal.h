@@ -7,6 +7,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/pizza_amd.h"
+
 typedef double2 cd;
 
 #define PZ_MAX_RADICES 24
@@ -35,6 +37,7 @@ struct PzScheme {
 struct PzDev {
     int N, nm, nphi, ncheb, L; // L = 2N-2 (Chebyshev even extension)
     int family, nstages, nexp, nimp, nold;
+    int scheme, bridge; // enum pz_time_scheme; 1: multistep start-up steps use CNAB2
     int l_heat, l_chem, l_coriolis_imp, l_buo_imp, l_ek_pump;
     int ktopt, kbott, ktopxi, kbotxi, ktopv, kbotv;
     int nrad, radices[PZ_MAX_RADICES]; // azimuthal FFT plan

@@ -279,7 +279,10 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
     }
     __syncthreads();
     const int inext = (istage == p.nstages) ? 1 : istage + 1;
-    const int lcalc = p.l_imp_calc_rhs[inext - 1];
+    // implicit terms for the next stage; also before a CNAB2 start-up step
+    // of a multistep scheme, which recomputes them (step_time.f90:417-503)
+    const int bridge_next = p.bridge && ss->n_steps + 1 <= p.nold - 1;
+    const int lcalc = p.l_imp_calc_rhs[inext - 1] || bridge_next;
     if (hT) rotate_row(p, p.T_old, p.T_impl, p.T_expl, off);
     if (hX) rotate_row(p, p.X_old, p.X_impl, p.X_expl, off);
     __syncthreads();
@@ -384,10 +387,22 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
     if (lcalc) {
         if (m == 0) bandmv_c<2>(p, p.P_RHSIL + (size_t)mi * N * 17, wv, cT, 2);
         else bandmv_c<8>(p, p.P_RHSIL + (size_t)mi * N * 17, wv, cT, 4);
+        cd *bimp = b2;
+        if (bridge_next && p.l_buo_imp && hX && m != 0) {
+            // the start-up recomputation passes psi in the composition slot
+            // (step_time.f90:475-478): buoyancy of (T, psi) instead of (T, xi)
+            for (int k = tid; k < N; k += blockDim.x)
+                bv[k] = buoyancy(p, R, m, k, hT ? Tv[k] : mk(0, 0), psiv[k]);
+            __syncthreads();
+            dct_fft(p, VecSet<const cd>(bv), VecSet<cd>(aT), 1, 0, cosT, sinT, fbuf);
+            __syncthreads();
+            bandmv<8>(p, p.P_RHSE2, aT, aX, 4);
+            bimp = aX;
+        }
         __syncthreads();
         for (int k = tid; k < N; k += blockDim.x) {
             cd v = cT[k];
-            if (p.l_buo_imp && m != 0) v = cadd(v, b2[k]);
+            if (p.l_buo_imp && m != 0) v = cadd(v, bimp[k]);
             p.P_impl[(inext - 1) * F + off + k] = v;
         }
     }

@@ -156,23 +156,55 @@ contains
       flag = merge(1_c_int, 0_c_int, l)
    end function flag
 
+   ! The scheme a time_scheme string selects: the family test of
+   ! select_time_scheme.f90:15-49, then the first name that initialize()
+   ! matches (dirk_schemes.f90:55-230, multistep_schemes.f90:56-104).
+   ! Schemes with an assembly stage stop the run.
+   integer(c_int) function scheme_id(name)
+      character(len=*), intent(in) :: name
+      character(len=10), parameter :: dirk(22) = [character(len=10) :: 'ARS222', 'SSP222', 'LSDIRK222', &
+         'ARS232', 'ARS233', 'SSP322', 'SSP332', 'SSP433', 'DP1-242', 'DP2-242', 'LZ232', 'CK232', &
+         'ARS443', 'LZ453', 'BPR353', 'PC2', 'ARS343', 'MARS343', 'KC343', 'KC564', 'KC785', 'DBM453']
+      integer(c_int), parameter :: dirk_id(22) = [PZ_ARS222, -1, -1, -1, -1, -1, -1, -1, PZ_DP1_242, &
+         PZ_DP2_242, PZ_LZ232, PZ_CK232, PZ_ARS443, PZ_LZ453, PZ_BPR353, PZ_PC2, -1, -1, -1, -1, -1, -1]
+      character(len=10), parameter :: ms(10) = [character(len=10) :: 'CNAB2', 'MODCNAB', 'CNLF', &
+         'BDF2AB2', 'SBDF2', 'BDF3AB3', 'SBDF3', 'TVB33', 'BDF4AB4', 'SBDF4']
+      integer(c_int), parameter :: ms_id(10) = [PZ_CNAB2, PZ_MODCNAB, PZ_CNLF, PZ_SBDF2, PZ_SBDF2, &
+         PZ_SBDF3, PZ_SBDF3, PZ_TVB33, PZ_SBDF4, PZ_SBDF4]
+      integer :: i
+      logical :: l_dirk
+      l_dirk = .false.
+      do i = 1, size(dirk)
+         if (has(name, trim(dirk(i)))) l_dirk = .true.
+      end do
+      scheme_id = 0
+      if (l_dirk) then
+         do i = 1, size(dirk)
+            if (has(name, trim(dirk(i)))) then
+               scheme_id = dirk_id(i)
+               exit
+            end if
+         end do
+      else
+         do i = 1, size(ms)
+            if (has(name, trim(ms(i)))) then
+               scheme_id = ms_id(i)
+               exit
+            end if
+         end do
+      end if
+      if (scheme_id <= 0) then
+         write(*, '(a)') 'pizza_gpu: time scheme '//trim(name)//' is not supported'
+         error stop 1
+      end if
+   end function scheme_id
+
    ! namelist values -> pz_config (the mapping of namelists.f90:233-300)
    subroutine fill_config(cfg)
       type(pz_config), intent(out) :: cfg
       call pz_config_default(cfg)
       cfg%n_r_max = n_r_max; cfg%n_cheb_max = n_cheb_max; cfg%m_max = m_max; cfg%minc = minc
-      if (has(time_scheme, 'BPR353')) then
-         cfg%time_scheme = PZ_BPR353
-      else if (has(time_scheme, 'ARS443')) then
-         cfg%time_scheme = PZ_ARS443
-      else if (has(time_scheme, 'ARS222')) then
-         cfg%time_scheme = PZ_ARS222
-      else if (has(time_scheme, 'CNAB2')) then
-         cfg%time_scheme = PZ_CNAB2
-      else
-         write(*, '(a)') 'pizza_gpu: time scheme '//trim(time_scheme)//' is not supported'
-         error stop 1
-      end if
+      cfg%time_scheme = scheme_id(time_scheme)
       cfg%l_cheb_coll = flag(has(cheb_method, 'COLL'))
       cfg%l_direct_solve = flag(has(matrix_solve, 'DIRECT'))
       cfg%l_coriolis_imp = flag(has(corio_term, 'IMP'))

@@ -10,7 +10,10 @@ module pizza_amd_gpu
    integer, parameter :: PZ_N_T_BOUNDS = 32          ! namelists.f90:23
 
    ! time schemes (select_time_scheme.f90:15-49)
-   integer(c_int), parameter :: PZ_CNAB2 = 1, PZ_ARS222 = 101, PZ_ARS443 = 102, PZ_BPR353 = 103
+   integer(c_int), parameter :: PZ_CNAB2 = 1, PZ_CNLF = 2, PZ_MODCNAB = 3, PZ_SBDF2 = 4, PZ_SBDF3 = 5, &
+                                PZ_SBDF4 = 6, PZ_TVB33 = 7
+   integer(c_int), parameter :: PZ_ARS222 = 101, PZ_ARS443 = 102, PZ_BPR353 = 103, PZ_DP1_242 = 104, &
+                                PZ_DP2_242 = 105, PZ_LZ232 = 106, PZ_CK232 = 107, PZ_LZ453 = 108, PZ_PC2 = 109
    ! fields (fields.f90)
    integer(c_int), parameter :: PZ_US = 0, PZ_UP = 1, PZ_OM = 2, PZ_PSI = 3, PZ_TEMP = 4, PZ_XI = 5, &
                                 PZ_TEMP_HAT = 6, PZ_XI_HAT = 7, PZ_PSI_HAT = 8, PZ_DTEMP = 9

@@ -21,3 +21,27 @@ def rotinteg_config():
 
 def golden_rows(n):
     return np.loadtxt(os.path.join(GOLDEN, "rotinteg_start_e_kin_3D.txt"))[:n]
+
+
+def sample_config(name):
+    """(cfg, start, output) of a reference sample's start namelist kept in
+    tests/golden/<name>_input_start.nml."""
+    from pizza_amd.config import config_from_namelist
+    return config_from_namelist(os.path.join(GOLDEN, f"{name}_input_start.nml"))
+
+
+def sample_series(run, name, steps_per_row):
+    """Initialises `run` (a Run or an oracle) from the sample's start
+    namelist, steps it, and returns (rows, golden): the e_kin_3D rows at the
+    golden file's log steps and the golden rows themselves."""
+    _, start, _ = sample_config(name)
+    ref = np.loadtxt(os.path.join(GOLDEN, f"{name}_start_e_kin_3D.txt"))
+    run.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+    run.start()
+    rows = []
+    for k in range(len(ref)):
+        if k:
+            run.step(steps_per_row)
+        t, _ = run.time()
+        rows.append(np.r_[t, run.ekin_3d()][: ref.shape[1]])
+    return np.array(rows), ref

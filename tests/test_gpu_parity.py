@@ -173,6 +173,56 @@ def test_rotekpump_collocation_golden():
     np.testing.assert_allclose(np.array(rows), ref, rtol=1e-8, atol=1e-20)
 
 
+def test_dirkschemes_ars222_collocation_golden():
+    """Reference sample DIRKSchemes, start run (ARS222, collocation, n_r =
+    33): 100 steps against samples/DIRKSchemes/reference.out."""
+    from helpers import sample_config, sample_series
+    from pizza_amd.runtime import Run
+    cfg, _, _ = sample_config("dirkschemes")
+    rows, ref = sample_series(Run(cfg), "dirkschemes", 10)
+    np.testing.assert_allclose(rows, ref, rtol=1e-8, atol=1e-20)
+
+
+def test_multistep_sbdf3_collocation_golden():
+    """Reference sample MultistepSchemes, start run (SBDF3 with its CNAB2
+    start-up steps, collocation, n_r = 32): 200 steps, every one logged."""
+    from helpers import sample_config, sample_series
+    from pizza_amd.runtime import Run
+    cfg, _, _ = sample_config("multistep")
+    rows, ref = sample_series(Run(cfg), "multistep", 1)
+    np.testing.assert_allclose(rows, ref, rtol=1e-8, atol=1e-20)
+
+
+SCHEMES = ["CNLF", "MODCNAB", "SBDF2", "SBDF3", "SBDF4", "TVB33", "ARS222", "ARS443", "BPR353",
+           "DP1-242", "DP2-242", "LZ232", "CK232", "LZ453", "PC2"]
+
+
+@pytest.mark.parametrize("method", ["coll", "integ"])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_time_scheme_parity(scheme, method):
+    """Every time scheme on both radial methods: fields against the oracle
+    after 12 steps (past every multistep start-up step) of the n_r = 33
+    thermochemical physics (T + xi, Ekman pumping)."""
+    from pizza_amd import workloads as W
+    from pizza_amd.config import scheme_id
+    cfg = W.small_bpr353()
+    cfg.time_scheme = scheme_id(scheme)
+    cfg.l_cheb_coll = 1 if method == "coll" else 0
+    cfg.dtMax = 2.0e-6
+    print(f"case: scheme {scheme} method {method}", flush=True)
+    floors = _floor(cfg, [12], lambda x: W.seed_run(x, cfg))
+    g, o = _pair(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg)
+        x.start()
+    g.step(12)
+    o.step(12)
+    assert abs(g.time()[0] - o.time()[0]) < 1e-15
+    for f in _fields(cfg):
+        err = rel(g.get_field(f), o.get_field(f))
+        assert err < max(1e-10, 10 * floors[0][f]), (f, err, floors[0][f])
+
+
 @pytest.mark.parametrize("case", ["c1", "bpr353_xi"])
 def test_collocation_parity(case):
     """Collocation method: fields against the oracle after 1 and 20 steps,

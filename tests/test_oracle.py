@@ -66,6 +66,25 @@ def test_rotekpump_collocation_golden_series():
     np.testing.assert_allclose(np.array(rows), ref, rtol=1e-8, atol=1e-20)
 
 
+def test_dirkschemes_ars222_collocation_golden_series():
+    """100 ARS222 steps of the reference's DIRKSchemes start run
+    (collocation, n_r = 33; samples/DIRKSchemes/reference.out rows 1-11)."""
+    from helpers import sample_config, sample_series
+    cfg, _, _ = sample_config("dirkschemes")
+    rows, ref = sample_series(Oracle(cfg), "dirkschemes", 10)
+    np.testing.assert_allclose(rows, ref, rtol=1e-8, atol=1e-20)
+
+
+def test_multistep_sbdf3_collocation_golden_series():
+    """200 SBDF3 steps of the reference's MultistepSchemes start run
+    (collocation, n_r = 32, CNAB2 start-up steps; every step logged;
+    samples/MultistepSchemes/reference.out rows 1-201)."""
+    from helpers import sample_config, sample_series
+    cfg, _, _ = sample_config("multistep")
+    rows, ref = sample_series(Oracle(cfg), "multistep", 1)
+    np.testing.assert_allclose(rows, ref, rtol=1e-8, atol=1e-20)
+
+
 def _oracle(n_r=33, m_max=32):
     from pizza_amd.workloads import small_bpr353
     return Oracle(small_bpr353(n_r, m_max))
