@@ -80,6 +80,26 @@ __device__ __forceinline__ void lds_store(cd *q, int i, cd v)
     d[2 * i + 1] = v.y;
 }
 
+// Loads through a generic pointer known to address global memory, retyped
+// so that they compile to global_load (which counts only against vmcnt)
+// instead of flat loads, whose waits also stall on pending LDS accesses.
+typedef double pz_d2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const pz_d2 g_d2;
+typedef __attribute__((address_space(1))) const double g_double;
+typedef __attribute__((address_space(1))) const int g_int;
+__device__ __forceinline__ double gld(const double *q, size_t i) { return ((g_double *)q)[i]; }
+__device__ __forceinline__ cd gld(const cd *q, size_t i)
+{
+    const pz_d2 v = ((g_d2 *)q)[i];
+    return mk(v.x, v.y);
+}
+// two consecutive doubles at q (16-byte aligned)
+__device__ __forceinline__ cd gld2(const double *q)
+{
+    const pz_d2 v = *(g_d2 *)q;
+    return mk(v.x, v.y);
+}
+
 // Up to four vector pointers passed by value; indexing by selects keeps
 // them in registers (a private array indexed at run time would live in
 // scratch memory and turn LDS accesses into flat ones).
@@ -1007,13 +1027,13 @@ __device__ void bandmv(const PzDev &p, const double *band, const cd *x, cd *y, i
 #pragma unroll
             for (int u = 0; u < PER; u++) {
                 const int d = sub + u * G, jj = i - K + d;
-                b[u] = (d < W && jj >= 0 && jj < N) ? band[(size_t)i * W + d] : 0.0;
+                b[u] = (d < W && jj >= 0 && jj < N) ? gld(band, (size_t)i * W + d) : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < PER; u++) {
                 const int d = sub + u * G, jj = i - K + d;
                 if (d < W && jj >= 0 && jj < N) {
-                    const cd xv = x[jj];
+                    const cd xv = lds_load(x, jj);
                     sx += b[u] * xv.x;
                     sy += b[u] * xv.y;
                 }
@@ -1038,12 +1058,12 @@ __device__ void bandmv_c(const PzDev &p, const cd *band, const cd *x, cd *y, int
 #pragma unroll
             for (int u = 0; u < PER; u++) {
                 const int d = sub + u * G, jj = i - K + d;
-                b[u] = (d < W && jj >= 0 && jj < N) ? band[(size_t)i * W + d] : mk(0.0, 0.0);
+                b[u] = (d < W && jj >= 0 && jj < N) ? gld(band, (size_t)i * W + d) : mk(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < PER; u++) {
                 const int d = sub + u * G, jj = i - K + d;
-                if (d < W && jj >= 0 && jj < N) s = cadd(s, cmul(b[u], x[jj]));
+                if (d < W && jj >= 0 && jj < N) s = cadd(s, cmul(b[u], lds_load(x, jj)));
             }
         }
         s.x += __shfl_xor(s.x, 1, G);
@@ -1355,13 +1375,14 @@ __global__ void __launch_bounds__(64) k_build_lhs(const PzDev p __attribute__((g
 }
 
 // global -> LDS copy by the whole block, four loads in flight per thread
+// global -> LDS copy of a byte range (dst in LDS, src in global memory)
 __device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t bytes)
 {
     const size_t st = blockDim.x;
     size_t i = threadIdx.x;
     if ((bytes & 7) == 0 && ((size_t)src & 7) == 0) {
-        const double *s = (const double *)src;
-        double *d = (double *)dst;
+        g_double *s = (g_double *)src;
+        lds_double *d = (lds_double *)dst;
         const size_t n = bytes / 8;
         for (; i + 3 * st < n; i += 4 * st) {
             const double a = s[i], b = s[i + st], c = s[i + 2 * st], e = s[i + 3 * st];
@@ -1369,8 +1390,8 @@ __device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t byte
         }
         for (; i < n; i += st) d[i] = s[i];
     } else {
-        const int *s = (const int *)src;
-        int *d = (int *)dst;
+        g_int *s = (g_int *)src;
+        __attribute__((address_space(3))) int *d = (__attribute__((address_space(3))) int *)dst;
         const size_t n = bytes / 4;
         for (; i + 3 * st < n; i += 4 * st) {
             const int a = s[i], b = s[i + st], c = s[i + 2 * st], e = s[i + 3 * st];
@@ -1415,14 +1436,14 @@ __device__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv
                 const bool ok = jb + u < nj;
 #pragma unroll
                 for (int a = 0; a < NMAT; a++)
-                    v[a][u] = ok ? *(const double2 *)(inv[a] + (size_t)jj * ldi + (size_t)ru * RPT) : make_double2(0, 0);
+                    v[a][u] = ok ? gld2((const double *)(inv[a] + (size_t)jj * ldi + (size_t)ru * RPT)) : mk(0, 0);
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
 #pragma unroll
                 for (int a = 0; a < NMAT; a++) {
                     const int jj = min(cg + (jb + u) * CG, N - 1);
-                    const cd xj = x[a][jj];
+                    const cd xj = lds_load(x[a], jj);
                     if constexpr (RPT == 2) {
                         acc[a][0].x = fma(v[a][u].x, xj.x, acc[a][0].x);
                         acc[a][0].y = fma(v[a][u].x, xj.y, acc[a][0].y);
@@ -1437,7 +1458,7 @@ __device__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv
 #pragma unroll
             for (int a = 0; a < NMAT; a++)
 #pragma unroll
-                for (int r = 0; r < RPT; r++) red[(size_t)a * RS + (cg - 1) * R * RPT + ru * RPT + r] = acc[a][r];
+                for (int r = 0; r < RPT; r++) lds_store(red, (size_t)a * RS + (cg - 1) * R * RPT + ru * RPT + r, acc[a][r]);
     }
     __syncthreads();
     if (cg == 0) {
@@ -1446,9 +1467,9 @@ __device__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv
 #pragma unroll
             for (int r = 0; r < RPT; r++) {
                 cd s = acc[a][r];
-                for (int c = 1; c < CG; c++) s = cadd(s, red[(size_t)a * RS + (c - 1) * R * RPT + ru * RPT + r]);
+                for (int c = 1; c < CG; c++) s = cadd(s, lds_load(red, (size_t)a * RS + (c - 1) * R * RPT + ru * RPT + r));
                 const int row = ru * RPT + r;
-                if (row < nrow) y[a][row] = s;
+                if (row < nrow) lds_store(y[a], row, s);
             }
     }
     __syncthreads();

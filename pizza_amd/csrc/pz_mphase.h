@@ -16,19 +16,19 @@ __device__ __forceinline__ void imex_split(const PzDev &p, const PzScheme *s, in
     const int is = istage - 1;
     cd r, q = mk(0.0, 0.0);
     if (p.family == 0) {
-        r = cscale(s->wimp[0], old[off + k]);
-        for (int o = 1; o < p.nold; o++) r = cadd(r, cscale(s->wimp[o], old[o * F + off + k]));
-        for (int o = 0; o < p.nimp; o++) r = cadd(r, cscale(s->wimp_lin[o + 1], impl[o * F + off + k]));
-        for (int o = 0; o < is; o++) r = cadd(r, cscale(s->wexp[o], expl[o * F + off + k]));
+        r = cscale(s->wimp[0], gld(old, off + k));
+        for (int o = 1; o < p.nold; o++) r = cadd(r, cscale(s->wimp[o], gld(old, o * F + off + k)));
+        for (int o = 0; o < p.nimp; o++) r = cadd(r, cscale(s->wimp_lin[o + 1], gld(impl, o * F + off + k)));
+        for (int o = 0; o < is; o++) r = cadd(r, cscale(s->wexp[o], gld(expl, o * F + off + k)));
         for (int o = is + 1; o < p.nexp; o++) {
-            const cd t = cscale(s->wexp[o], expl[o * F + off + k]);
+            const cd t = cscale(s->wexp[o], gld(expl, o * F + off + k));
             q = o == is + 1 ? t : cadd(q, t);
         }
     } else {
-        r = old[off + k];
-        for (int st = 0; st < is; st++) r = cadd(r, cscale(s->bexp[istage][st], expl[st * F + off + k]));
+        r = gld(old, off + k);
+        for (int st = 0; st < is; st++) r = cadd(r, cscale(s->bexp[istage][st], gld(expl, st * F + off + k)));
         for (int st = 0; st <= is; st++) {
-            const cd t = cscale(s->bimp[istage][st], impl[st * F + off + k]);
+            const cd t = cscale(s->bimp[istage][st], gld(impl, st * F + off + k));
             q = st == 0 ? t : cadd(q, t);
         }
     }
@@ -147,22 +147,29 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
     // explicit-assembly inputs (or the current explicit slot) and the
     // right-hand-side terms that do not depend on this stage's explicit part
     for (int k = tid; k < N; k += blockDim.x) {
-        if (hT) eT[k] = p.T_expl[is * F + off + k];
-        if (hX) eX[k] = p.X_expl[is * F + off + k];
-        eP[k] = p.P_expl[is * F + off + k];
+        const size_t q = off + k, qe = is * F + q;
+        if (hT) lds_store(eT, k, gld(p.T_expl, qe));
+        if (hX) lds_store(eX, k, gld(p.X_expl, qe));
+        lds_store(eP, k, gld(p.P_expl, qe));
         if (exp_calc) {
-            if (hT) aT[k] = p.dVsT[off + k];
-            if (hX) aX[k] = p.dVsXi[off + k];
-            aP[k] = p.dVsOm[off + k];
-            psiv[k] = p.psi[off + k];
-            omv[k] = p.om[off + k];
-            upv[k] = p.up[off + k];
-            usv[k] = p.us[off + k];
-            if (!p.l_buo_imp) { Tv[k] = hT ? p.temp[off + k] : mk(0, 0); Xv[k] = hX ? p.xi[off + k] : mk(0, 0); }
+            if (hT) lds_store(aT, k, gld(p.dVsT, q));
+            if (hX) lds_store(aX, k, gld(p.dVsXi, q));
+            lds_store(aP, k, gld(p.dVsOm, q));
+            lds_store(psiv, k, gld(p.psi, q));
+            lds_store(omv, k, gld(p.om, q));
+            lds_store(upv, k, gld(p.up, q));
+            lds_store(usv, k, gld(p.us, q));
+            if (!p.l_buo_imp) {
+                lds_store(Tv, k, hT ? gld(p.temp, q) : mk(0, 0));
+                lds_store(Xv, k, hX ? gld(p.xi, q) : mk(0, 0));
+            }
         }
-        if (hT) imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, PT[k], QT[k]);
-        if (hX) imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, PX[k], QX[k]);
-        imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, PP[k], QP[k]);
+        cd a, b;
+        if (hT) { imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, a, b); lds_store(PT, k, a); lds_store(QT, k, b); }
+        if (hX) { imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, a, b); lds_store(PX, k, a); lds_store(QX, k, b); }
+        imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, a, b);
+        lds_store(PP, k, a);
+        lds_store(QP, k, b);
     }
     const double wcur = imex_weight(p, ss, istage);
 
