@@ -176,6 +176,25 @@ int pz_ifft(pz_handle *h, const double *spec, double *grid, int n_rad);
  */
 int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms);
 
+/*
+ * Ensembles of independent runs.  The reference runs one case per process
+ * and has no counterpart; the reference interface each member replaces is
+ * pz_step's.  The members are ordinary handles, created and started with
+ * pz_create / pz_start (their sizes and physical parameters may all differ);
+ * they must share the time scheme and the radial method.  pz_ensemble_step
+ * advances every member by nsteps steps with one batched launch per kernel
+ * and stage for all of them (graph-captured), so that small runs share the
+ * GPU instead of queueing; each member's result equals its own pz_step.
+ * pz_get_field, pz_time and the other per-handle calls work on the members
+ * afterwards (they are ordered after the ensemble's work).  The members must
+ * outlive the ensemble.
+ */
+typedef struct pz_ensemble pz_ensemble;
+int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble **e);
+int pz_ensemble_step(pz_ensemble *e, int nsteps);
+int pz_ensemble_destroy(pz_ensemble *e);
+const char *pz_ensemble_last_error(const pz_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

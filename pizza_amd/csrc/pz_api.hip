@@ -421,6 +421,27 @@ static std::vector<int> factor_radices(int n)
     return f;
 }
 
+// Lets every kernel with dynamic LDS use the whole per-workgroup budget, once
+// per process: a per-run value would shrink the limit for a larger run
+// created earlier (runs of different sizes share the kernels).
+static void allow_full_lds(int lim)
+{
+    static int done = 0;
+    if (done >= lim) return;
+    const void *kernels[] = {(const void *)k_build_lhs, (const void *)k_mphase, (const void *)k_radial_loop,
+                             (const void *)k_start, (const void *)k_build_inv, (const void *)k_mcoll,
+                             (const void *)k_start_coll, (const void *)k_build_lhs_batch,
+                             (const void *)k_mphase_batch, (const void *)k_radial_loop_batch,
+                             (const void *)k_build_inv_batch, (const void *)k_mcoll_batch};
+    for (const void *k : kernels) {
+        hipFuncAttributes a;
+        HIPCHK(hipFuncGetAttributes(&a, k));
+        // the dynamic part may use what the kernel's static LDS leaves
+        HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lim - (int)a.sharedSizeBytes));
+    }
+    done = lim;
+}
+
 static void setup(pz_handle *h)
 {
     const pz_config &c = h->c;
@@ -697,17 +718,8 @@ static void setup(pz_handle *h)
     h->lds_inv = InvLayout(N, h->inv_waves).total;
     size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv,
                             h->lds_start_coll});
-    if (need > 65536) {
-        if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
-        HIPCHK(hipFuncSetAttribute((const void *)k_build_lhs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_build));
-        HIPCHK(hipFuncSetAttribute((const void *)k_mphase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_mphase));
-        HIPCHK(hipFuncSetAttribute((const void *)k_radial_loop, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_radial));
-        HIPCHK(hipFuncSetAttribute((const void *)k_start, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_start));
-        HIPCHK(hipFuncSetAttribute((const void *)k_build_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_inv));
-        HIPCHK(hipFuncSetAttribute((const void *)k_mcoll, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds_mphase));
-        HIPCHK(hipFuncSetAttribute((const void *)k_start_coll, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)h->lds_start_coll));
-    }
+    if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
+    allow_full_lds((int)lim);
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // device copy of the (now final) descriptor for kernels that read it
     // through a pointer
@@ -1000,16 +1012,16 @@ static void launch_step(pz_handle *h)
     for (int stage = 1; stage <= s.nstages; stage++) {
         const int exp_calc = s.l_exp_calc[stage - 1];
         if (exp_calc)
-            hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, h->d, stage - 1,
+            hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, (const PzDev *)h->ddev, stage - 1,
                                stage == 1 ? 1 : 0);
         if (stage == 1) {
-            hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, h->d);
+            hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev);
             if (h->d.coll) {
-                hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), 0, h->stream, h->d, 0);
+                hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), 0, h->stream, (const PzDev *)h->ddev, 0);
             } else {
-                hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, h->d, 0);
-                hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream, h->d,
-                                   0);
+                hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, (const PzDev *)h->ddev, 0);
+                hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream,
+                                   (const PzDev *)h->ddev, 0);
             }
         }
         if (h->d.coll)
@@ -1240,7 +1252,7 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
                 hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else
-                hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, h->d, 0, 0);
+                hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, (const PzDev *)h->ddev, 0, 0);
         }
         HIPCHK(hipEventRecord(e1, h->stream));
         HIPCHK(hipEventSynchronize(e1));
@@ -1254,3 +1266,156 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Ensembles: every kernel of the stage loop is launched once for all
+// members, each block finding its run and its own block index in a map.
+// ---------------------------------------------------------------------------
+struct pz_ensemble {
+    std::vector<pz_handle *> runs;
+    Scheme sch;
+    int coll = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<void *> allocs;
+    const PzDev **druns = nullptr;
+    int2 *map_r = nullptr, *map_c = nullptr, *map_3 = nullptr, *map_m = nullptr;
+    int nb_r = 0, nb_c = 0, nb_3 = 0, nb_m = 0;
+    size_t lds_r = 0, lds_m = 0, lds_b = 0, lds_i = 0;
+    int inv_threads = 64;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+    ~pz_ensemble()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (done) (void)hipEventDestroy(done);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (void *p : allocs) (void)hipFree(p);
+    }
+    template <class T> T *upload(const std::vector<T> &v)
+    {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)));
+        allocs.push_back(p);
+        HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return (T *)p;
+    }
+};
+
+static void launch_ensemble_step(pz_ensemble *e)
+{
+    const Scheme &s = e->sch;
+    const PzDev *const *runs = e->druns;
+    for (int stage = 1; stage <= s.nstages; stage++) {
+        const int exp_calc = s.l_exp_calc[stage - 1];
+        if (exp_calc)
+            hipLaunchKernelGGL(k_radial_loop_batch, dim3(e->nb_r), dim3(256), e->lds_r, e->stream,
+                               PzBatch{runs, e->map_r}, stage - 1, stage == 1 ? 1 : 0);
+        if (stage == 1) {
+            hipLaunchKernelGGL(k_courant_batch, dim3(e->nb_c), dim3(256), 0, e->stream, PzBatch{runs, e->map_c});
+            if (e->coll) {
+                hipLaunchKernelGGL(k_build_coll_batch, dim3(e->nb_3), dim3(1024), 0, e->stream,
+                                   PzBatch{runs, e->map_3}, 0);
+            } else {
+                hipLaunchKernelGGL(k_build_lhs_batch, dim3(e->nb_3), dim3(64), e->lds_b, e->stream,
+                                   PzBatch{runs, e->map_3}, 0);
+                hipLaunchKernelGGL(k_build_inv_batch, dim3(e->nb_3), dim3(e->inv_threads), e->lds_i, e->stream,
+                                   PzBatch{runs, e->map_3}, 0);
+            }
+        }
+        if (e->coll)
+            hipLaunchKernelGGL(k_mcoll_batch, dim3(e->nb_m), dim3(PZ_MPHASE_THREADS), e->lds_m, e->stream,
+                               PzBatch{runs, e->map_m}, stage, exp_calc);
+        else
+            hipLaunchKernelGGL(k_mphase_batch, dim3(e->nb_m), dim3(PZ_MPHASE_THREADS), e->lds_m, e->stream,
+                               PzBatch{runs, e->map_m}, stage, exp_calc);
+    }
+}
+
+extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble **out)
+{
+    if (!runs || nruns < 1 || !out) return -1;
+    *out = nullptr;
+    pz_ensemble *e = new pz_ensemble();
+    try {
+        const pz_handle *h0 = runs[0];
+        std::vector<const PzDev *> dr;
+        std::vector<int2> mr, mc, m3, mm;
+        int nw = 16;
+        for (int r = 0; r < nruns; r++) {
+            pz_handle *h = runs[r];
+            if (!h) throw std::invalid_argument("null run handle");
+            if (!h->started) throw std::invalid_argument("every run must be started (pz_start) first");
+            if (h->sch.id != h0->sch.id || h->d.coll != h0->d.coll)
+                throw std::invalid_argument("the runs of an ensemble must share the time scheme and the radial method");
+            e->runs.push_back(h);
+            dr.push_back(h->ddev);
+            for (int i = 0; i < h->N; i++) mr.push_back(make_int2(r, i));
+            mc.push_back(make_int2(r, 0));
+            for (int i = 0; i < 3 * h->nm; i++) m3.push_back(make_int2(r, i));
+            for (int i = 0; i < h->nm; i++) mm.push_back(make_int2(r, i));
+            e->lds_r = std::max(e->lds_r, h->lds_radial);
+            e->lds_m = std::max(e->lds_m, h->lds_mphase);
+            e->lds_b = std::max(e->lds_b, h->lds_build);
+            nw = std::min(nw, h->inv_waves);
+        }
+        for (const pz_handle *h : e->runs) e->lds_i = std::max(e->lds_i, InvLayout(h->N, nw).total);
+        e->inv_threads = 64 * nw;
+        e->sch = h0->sch;
+        e->coll = h0->d.coll;
+        e->druns = e->upload(dr);
+        e->map_r = e->upload(mr);
+        e->map_c = e->upload(mc);
+        e->map_3 = e->upload(m3);
+        e->map_m = e->upload(mm);
+        e->nb_r = (int)mr.size();
+        e->nb_c = (int)mc.size();
+        e->nb_3 = (int)m3.size();
+        e->nb_m = (int)mm.size();
+        HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
+    } catch (std::exception &ex) {
+        std::fprintf(stderr, "pz_ensemble_create: %s\n", ex.what());
+        delete e;
+        return -2;
+    }
+    *out = e;
+    return 0;
+}
+
+extern "C" int pz_ensemble_step(pz_ensemble *e, int nsteps)
+{
+    if (!e) return -1;
+    try {
+        // work already queued on the members comes first
+        for (pz_handle *h : e->runs) HIPCHK(hipStreamSynchronize(h->stream));
+        if (!e->gexec) {
+            HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            launch_ensemble_step(e);
+            HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
+            HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < nsteps; i++) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+        HIPCHK(hipGetLastError());
+        // later calls on a member wait for the ensemble's work
+        HIPCHK(hipEventRecord(e->done, e->stream));
+        for (pz_handle *h : e->runs) HIPCHK(hipStreamWaitEvent(h->stream, e->done, 0));
+    } catch (std::exception &ex) {
+        e->err = ex.what();
+        return -4;
+    }
+    return 0;
+}
+
+extern "C" int pz_ensemble_destroy(pz_ensemble *e)
+{
+    if (e) {
+        (void)hipStreamSynchronize(e->stream);
+        delete e;
+    }
+    return 0;
+}
+
+extern "C" const char *pz_ensemble_last_error(const pz_ensemble *e) { return e ? e->err.c_str() : "null ensemble"; }

@@ -195,12 +195,12 @@ __device__ void coll_assemble(const PzDev &p, const RadView &R, int fld, int m, 
 // grid (n_m, 3): field 0 = T, 1 = xi, 2 = psi (m /= 0) or u_phi (m = 0).
 // Forms the inverses, with the reference's scalings folded in, whenever the
 // matrices change (lMat) or when forced.
-__global__ void __launch_bounds__(1024) k_build_coll(const PzDev p __attribute__((grid_constant)), int force)
+__device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld, int force)
 {
     __shared__ cd fcol[2 * 257];
     __shared__ double facs[4 * 257];
     if (!force && p.ss->lMat == 0) return;
-    const int mi = blockIdx.x, fld = blockIdx.y, N = p.N, m = p.idx2m[mi], tid = threadIdx.x;
+    const int N = p.N, m = p.idx2m[mi], tid = threadIdx.x;
     if ((fld == 0 && !p.l_heat) || (fld == 1 && !p.l_chem)) return;
     const RadView R(p.radc, N);
     char *ws = p.coll_work + (size_t)mi * p.coll_ws;
@@ -238,6 +238,18 @@ __global__ void __launch_bounds__(1024) k_build_coll(const PzDev p __attribute__
             out[(size_t)c * n + r] = cscale(facs[n + r] * facs[c], W[(size_t)r * 2 * n + n + c]);
         }
     }
+}
+
+__global__ void __launch_bounds__(1024) k_build_coll(const PzDev *__restrict__ pd, int force)
+{
+    build_coll_body(*pd, blockIdx.x, blockIdx.y, force);
+}
+// batched: local block index = mi + nm fld
+__global__ void __launch_bounds__(1024) k_build_coll_batch(PzBatch b, int force)
+{
+    const int2 e = b.map[blockIdx.x];
+    const PzDev &p = *b.runs[e.x];
+    build_coll_body(p, e.y % p.nm, e.y / p.nm, force);
 }
 
 // get_ddcheb_real_1d on Chebyshev coefficients f (radial_derivatives.f90:
@@ -301,11 +313,10 @@ __device__ void coll_uphi0_impl(const PzDev &p, const RadView &R, const cd *u0, 
 // exp_calc, update_temp_co / update_xi_co (update_temp_coll.f90:66-150) and
 // update_om_coll_smat (update_psi_coll_smat.f90:88-321) with their implicit
 // terms (get_*_rhs_imp_coll*).
-__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll(const PzDev *__restrict__ pd, int istage, int exp_calc)
+__device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, int exp_calc)
 {
-    const PzDev &p = *pd;
     extern __shared__ double smem[];
-    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi];
+    const int N = p.N, m = p.idx2m[mi];
     const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
@@ -565,6 +576,16 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll(const PzDev *__rest
             p.P_impl[(inext - 1) * F + off + k] = v;
         }
     }
+}
+
+__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll(const PzDev *__restrict__ pd, int istage, int exp_calc)
+{
+    mcoll_body(*pd, blockIdx.x, istage, exp_calc);
+}
+__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll_batch(PzBatch b, int istage, int exp_calc)
+{
+    const int2 e = b.map[blockIdx.x];
+    mcoll_body(*b.runs[e.x], e.y, istage, exp_calc);
 }
 
 // get_start_fields, collocation branch (init_fields.f90:191-244): dT/dr,

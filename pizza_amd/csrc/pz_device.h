@@ -263,10 +263,10 @@ __device__ double block_reduce(double v, double *scratch, int op /*0 sum, 1 max*
 // Radial loop: rloop::radial_loop (radial_loop.f90:264-392) for one radius
 // per workgroup.  Real fields travel in pairs through one complex FFT.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_radial_loop(const PzDev p __attribute__((grid_constant)), int is, int courant)
+__device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is, int courant)
 {
     extern __shared__ double smem[];
-    const int n = p.nphi, nm = p.nm, N = p.N, nr = blockIdx.x;
+    const int n = p.nphi, nm = p.nm, N = p.N;
     const size_t F = (size_t)nm * N;
     cd *spec = (cd *)smem;           // 5 x nm
     cd *bufa = spec + 5 * nm;        // 3 x n
@@ -374,6 +374,16 @@ __global__ void __launch_bounds__(256) k_radial_loop(const PzDev p __attribute__
             p.dVsXi[q] = B;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_radial_loop(const PzDev *__restrict__ pd, int is, int courant)
+{
+    radial_loop_body(*pd, blockIdx.x, is, courant);
+}
+__global__ void __launch_bounds__(256) k_radial_loop_batch(PzBatch b, int is, int courant)
+{
+    const int2 e = b.map[blockIdx.x];
+    radial_loop_body(*b.runs[e.x], e.y, is, courant);
 }
 
 // ---------------------------------------------------------------------------
@@ -516,7 +526,7 @@ __global__ void k_set_weights(const PzDev p __attribute__((grid_constant)))
     }
 }
 
-__global__ void __launch_bounds__(256) k_courant(const PzDev p __attribute__((grid_constant)))
+__device__ __forceinline__ void courant_body(const PzDev &p)
 {
     __shared__ double red[8];
     PzScheme *s = p.ss;
@@ -565,6 +575,9 @@ __global__ void __launch_bounds__(256) k_courant(const PzDev p __attribute__((gr
     s->l_new_dt = l_new_dt;
     s->n_steps += 1;
 }
+
+__global__ void __launch_bounds__(256) k_courant(const PzDev *__restrict__ pd) { courant_body(*pd); }
+__global__ void __launch_bounds__(256) k_courant_batch(PzBatch b) { courant_body(*b.runs[b.map[blockIdx.x].x]); }
 
 // ---------------------------------------------------------------------------
 // Wave-level banded solve (dgbtrs 'N' semantics: row interchanges with L,
@@ -1292,11 +1305,11 @@ __device__ void build_bordered(const PzDev &p, int nt, int kl, int ku, int K, co
 }
 
 // grid: (3, nm) blocks: field 0 = T, 1 = xi, 2 = psi; 64 threads
-__global__ void __launch_bounds__(64) k_build_lhs(const PzDev p __attribute__((grid_constant)), int force)
+__device__ __forceinline__ void build_lhs_body(const PzDev &p, int fld, int mi, int force)
 {
     extern __shared__ char lds[];
     if (!force && p.ss->lMat == 0) return;
-    const int fld = blockIdx.x, mi = blockIdx.y, N = p.N, m = p.idx2m[mi];
+    const int N = p.N, m = p.idx2m[mi];
     const double wimp = p.ss->wimp_lin[0];
     if (fld < 2) {
         if (fld == 0 && !p.l_heat) return;
@@ -1372,6 +1385,17 @@ __global__ void __launch_bounds__(64) k_build_lhs(const PzDev p __attribute__((g
             build_bordered<cd>(p, nt, kl, ku, K, S0, S1, S2, c1, c2, tauv, A1, A2, A3, A4, p1, p4, dinv, fac,
                                p.info + 2 * p.nm + mi, scr);
     }
+}
+
+__global__ void __launch_bounds__(64) k_build_lhs(const PzDev *__restrict__ pd, int force)
+{
+    build_lhs_body(*pd, blockIdx.x, blockIdx.y, force);
+}
+// batched: local block index = fld + 3 mi
+__global__ void __launch_bounds__(64) k_build_lhs_batch(PzBatch b, int force)
+{
+    const int2 e = b.map[blockIdx.x];
+    build_lhs_body(*b.runs[e.x], e.y % 3, e.y / 3, force);
 }
 
 // global -> LDS copy by the whole block, four loads in flight per thread
@@ -1506,11 +1530,11 @@ struct InvLayout {
 // factorised by k_build_lhs: wave w solves A y = fac_j e_j for the columns
 // j = w, w + nw, ... with the same bordered solve (algebra_lapack.f90:332-416)
 // as one time step would.  grid (nm, 3): field 0 = T, 1 = xi, 2 = psi.
-__global__ void __launch_bounds__(1024) k_build_inv(const PzDev p __attribute__((grid_constant)), int force)
+__device__ __forceinline__ void build_inv_body(const PzDev &p, int mi, int fld, int force)
 {
     extern __shared__ char lds[];
     if (!force && p.ss->lMat == 0) return;
-    const int mi = blockIdx.x, fld = blockIdx.y, N = p.N, m = p.idx2m[mi];
+    const int N = p.N, m = p.idx2m[mi];
     if ((fld == 0 && !p.l_heat) || (fld == 1 && !p.l_chem)) return;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63, ldi = p.ldi;
     const InvLayout lay(N, nw);
@@ -1561,4 +1585,16 @@ __global__ void __launch_bounds__(1024) k_build_inv(const PzDev p __attribute__(
             __builtin_amdgcn_wave_barrier();
         }
     }
+}
+
+__global__ void __launch_bounds__(1024) k_build_inv(const PzDev *__restrict__ pd, int force)
+{
+    build_inv_body(*pd, blockIdx.x, blockIdx.y, force);
+}
+// batched: local block index = mi + nm fld
+__global__ void __launch_bounds__(1024) k_build_inv_batch(PzBatch b, int force)
+{
+    const int2 e = b.map[blockIdx.x];
+    const PzDev &p = *b.runs[e.x];
+    build_inv_body(p, e.y % p.nm, e.y / p.nm, force);
 }

@@ -33,6 +33,15 @@ struct PzScheme {
     int pad[3];
 };
 
+struct PzDev;
+// A batch of independent runs advanced by one launch per kernel: block b of
+// a batched launch does the work of block map[b].y of run map[b].x's own
+// (flattened) grid.
+struct PzBatch {
+    const PzDev *const *runs;
+    const int2 *map;
+};
+
 // Everything a kernel needs: sizes, constants and device pointers.
 struct PzDev {
     int N, nm, nphi, ncheb, L; // L = 2N-2 (Chebyshev even extension)

@@ -111,13 +111,10 @@ struct MpLayout {
 // finish_explicit_assembly (mloop.f90:169-229) when exp_calc, then
 // update_temp_int / update_xi_int (update_temp_integ.f90:179-298) and
 // update_psi_int_smat (update_psi_integ_smat.f90:247-573).
-__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__restrict__ pd, int istage, int exp_calc)
+__device__ __forceinline__ void mphase_body(const PzDev &p, int mi, int istage, int exp_calc)
 {
-    // the descriptor is read through a pointer (scalar loads): a by-value
-    // copy of this size would be materialised in scratch memory
-    const PzDev &p = *pd;
     extern __shared__ double smem[];
-    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi];
+    const int N = p.N, m = p.idx2m[mi];
     const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
@@ -413,6 +410,18 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__res
             p.P_impl[(inext - 1) * F + off + k] = v;
         }
     }
+}
+
+// the descriptor is read through a pointer (scalar loads): a by-value copy
+// of this size would be materialised in scratch memory
+__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase(const PzDev *__restrict__ pd, int istage, int exp_calc)
+{
+    mphase_body(*pd, blockIdx.x, istage, exp_calc);
+}
+__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mphase_batch(PzBatch b, int istage, int exp_calc)
+{
+    const int2 e = b.map[blockIdx.x];
+    mphase_body(*b.runs[e.x], e.y, istage, exp_calc);
 }
 
 // get_start_fields, fresh start of the integration method

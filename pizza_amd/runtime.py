@@ -53,6 +53,11 @@ def lib():
         L.pz_fft.argtypes = [vp, dp, dp, ip]
         L.pz_ifft.argtypes = [vp, dp, dp, ip]
         L.pz_time_kernel.argtypes = [vp, ip, ip, dp]
+        L.pz_ensemble_create.argtypes = [ctypes.POINTER(vp), ip, ctypes.POINTER(vp)]
+        L.pz_ensemble_step.argtypes = [vp, ip]
+        L.pz_ensemble_destroy.argtypes = [vp]
+        L.pz_ensemble_last_error.argtypes = [vp]
+        L.pz_ensemble_last_error.restype = ctypes.c_char_p
         _LIB = L
     return _LIB
 
@@ -60,7 +65,8 @@ def lib():
 EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_dims",
            "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step",
            "pz_time", "pz_ekin_3d", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
-           "pz_fft", "pz_ifft", "pz_time_kernel"]
+           "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
+           "pz_ensemble_destroy", "pz_ensemble_last_error"]
 
 
 def _dp(a):
@@ -172,3 +178,35 @@ class Run:
         out = np.zeros((nrad, self.n_phi))
         self._check(lib().pz_ifft(self.h, _dp(s.view(np.float64)), _dp(out), nrad), "pz_ifft")
         return out.T.copy()
+
+
+class Ensemble:
+    """Started runs advanced together (pz_ensemble_*): one batched launch
+    per kernel and stage for all members.  The members stay usable as
+    ordinary runs (get_field, time, ekin_3d) and must outlive the ensemble."""
+
+    def __init__(self, runs):
+        self.runs = list(runs)
+        arr = (ctypes.c_void_p * len(self.runs))(*[r.h for r in self.runs])
+        e = ctypes.c_void_p()
+        rc = lib().pz_ensemble_create(arr, len(self.runs), ctypes.byref(e))
+        if rc:
+            raise PizzaError(f"pz_ensemble_create failed ({rc})")
+        self.e = e
+
+    def step(self, n=1):
+        rc = lib().pz_ensemble_step(self.e, n)
+        if rc:
+            msg = lib().pz_ensemble_last_error(self.e)
+            raise PizzaError(f"pz_ensemble_step failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "e", None):
+            lib().pz_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -78,3 +78,32 @@ def seed_run(run, cfg, amp_t=1.0e-4, amp_xi=1.0e-4, seed=SEED):
         run.set_field("temp", seeded_field(n_m, r, r_icb, amp_t, seed, 0))
     if cfg.raxi != 0.0:
         run.set_field("xi", seeded_field(n_m, r, r_icb, amp_xi, seed, 1))
+
+
+C3_N_R = (33, 65, 97, 129)
+C3_M_MAX = (32, 64, 128, 170)  # n_phi 96, 192, 384, 510 (512 is not reachable)
+
+
+def c3_ensemble(nruns=64, seed=SEED):
+    """The uneven ensemble: per run, n_r and n_phi drawn from the sets above,
+    Ra log-uniform in [1e5, 1e8], Ekman 1e-4 or 1e-5, collocation, CNAB2,
+    dtMax = Ek / 10.  Returns the configurations in run order."""
+    rng = np.random.Generator(np.random.PCG64([seed, 99]))
+    cfgs = []
+    for _ in range(nruns):
+        n_r = int(rng.choice(C3_N_R))
+        m_max = int(rng.choice(C3_M_MAX))
+        ra = float(10.0 ** rng.uniform(5.0, 8.0))
+        ek = float(rng.choice((1.0e-4, 1.0e-5)))
+        cfgs.append(default_config(
+            n_r_max=n_r, n_cheb_max=n_r, m_max=m_max, minc=1, time_scheme="CNAB2",
+            cheb_method="colloc", matrix_solve="direct", dtMax=0.1 * ek,
+            ra=ra, ek=ek, pr=1.0, radratio=0.35))
+    return cfgs
+
+
+def seed_member(run, cfg, k, amp_t=1.0e-3, seed=SEED):
+    """Seeded T perturbation of ensemble member k (seed base + member id)."""
+    r, r_icb = radial_grid(cfg)
+    n_m = cfg.m_max // cfg.minc + 1
+    run.set_field("temp", seeded_field(n_m, r, r_icb, amp_t, seed + 1000 + k, 0))

@@ -1,0 +1,66 @@
+"""Ensembles (pz_ensemble_*): uneven runs advanced together with one batched
+launch per kernel and stage.  Each member must equal the same run advanced
+alone through pz_step — the batched kernels run the very same per-block
+code — so the comparison is bitwise."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("temp", "us", "up", "om", "psi")
+
+
+def _members(cfgs, seed_fn):
+    from pizza_amd.runtime import Run
+    runs = []
+    for k, c in enumerate(cfgs):
+        r = Run(c)
+        seed_fn(r, c, k)
+        r.start()
+        runs.append(r)
+    return runs
+
+
+def _check_equal(cfgs, seed_fn, nsteps):
+    from pizza_amd.runtime import Ensemble
+    print(f"case: ensemble of {len(cfgs)} runs, {nsteps} steps", flush=True)
+    alone = _members(cfgs, seed_fn)
+    for r in alone:
+        r.step(nsteps)
+    batch = _members(cfgs, seed_fn)
+    ens = Ensemble(batch)
+    ens.step(nsteps // 2)
+    ens.step(nsteps - nsteps // 2)
+    ens.close()
+    for k, (a, b) in enumerate(zip(alone, batch)):
+        assert a.time() == b.time(), k
+        for f in FIELDS:
+            assert np.array_equal(a.get_field(f), b.get_field(f)), (k, f)
+
+
+def test_uneven_collocation_ensemble_equals_single_runs():
+    from pizza_amd import workloads as W
+    cfgs = W.c3_ensemble(nruns=64)
+    pick = []
+    for c in cfgs:   # one member of each n_r, plus the largest n_phi
+        if c.n_r_max not in [p.n_r_max for p in pick] or (c.m_max == 170 and all(p.m_max != 170 for p in pick)):
+            pick.append(c)
+    assert len({c.n_r_max for c in pick}) == 4
+    _check_equal(pick, W.seed_member, 6)
+
+
+def test_integration_ensemble_equals_single_runs():
+    from pizza_amd import workloads as W
+    cfgs = [W.small_bpr353(33, 32), W.small_bpr353(49, 48), W.small_bpr353(33, 64)]
+    cfgs[1].ra = 1.0e7
+    _check_equal(cfgs, lambda r, c, k: W.seed_run(r, c, seed=W.SEED + k), 4)
+
+
+def test_ensemble_rejects_mixed_methods():
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Ensemble, PizzaError, Run
+    a, b = Run(W.c1_collocation()), Run(W.c1_collocation_shape())
+    for r in (a, b):
+        r.start()
+    with pytest.raises(PizzaError):
+        Ensemble([a, b])
