@@ -9,8 +9,10 @@ state resident in HBM.
 
 Prints one JSON line: steps/s, the roofline figure of the dominant kernel
 (the m-loop stage kernel k_mphase, timed with HIP events on the library
-stream) and a CPU baseline of the oracle (single-thread C restatement)
-timed on a bounded sample of the same workload.
+stream), the ensemble throughput factor of the uneven 64-run collocation
+ensemble (batched launches versus the plain step run by run) and a CPU
+baseline of the oracle (single-thread C restatement) timed on a bounded
+sample of the same workload.
 """
 import argparse
 import json
@@ -106,6 +108,51 @@ def cpu_baseline(cfg, seconds=15.0):
                       f"after one untimed step, {dt:.1f} s on one host core"}
 
 
+def ensemble_bench(nruns=64, steps=100, warmup=3):
+    """Ensemble throughput x: the uneven 64-run collocation ensemble (CNAB2),
+    advanced `steps` steps after `warmup` steps, once run by run through
+    pz_step (the plain sequential GPU step) and once through one
+    pz_ensemble_step; two identical sets of runs, so both time the same
+    step window (including any Courant-driven matrix rebuilds)."""
+    from pizza_amd.runtime import Ensemble
+    cfgs = W.c3_ensemble(nruns)
+
+    def members():
+        runs = []
+        for k, c in enumerate(cfgs):
+            r = Run(c)
+            W.seed_member(r, c, k)
+            r.start()
+            r.step(warmup)
+            r.time()
+            runs.append(r)
+        return runs
+
+    seq = members()
+    t0 = time.perf_counter()
+    for r in seq:
+        r.step(steps)
+        r.time()
+    t_seq = time.perf_counter() - t0
+    for r in seq:
+        r.close()
+    ens_runs = members()
+    ens = Ensemble(ens_runs)
+    ens.step(1)                         # graph capture
+    ens_runs[0].time()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    for r in ens_runs:
+        r.time()
+    t_ens = time.perf_counter() - t0
+    ens.close()
+    total = nruns * steps
+    return {"runs": nruns, "steps_per_run": steps, "x": t_seq / t_ens,
+            "ensemble_run_steps_per_s": total / t_ens, "sequential_run_steps_per_s": total / t_seq,
+            "workload": "64 uneven collocation runs, n_r in {33,65,97,129}, n_phi in {96,192,384,510}, "
+                        "Ra log-uniform [1e5,1e8], E in {1e-4,1e-5}, CNAB2, seeded T perturbation 1e-3"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +160,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-only", action="store_true", help="measure only the ensemble factor")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--pmc-sample", action="store_true",
                     help="run only the short sample profiled by the counter runs")
     args = ap.parse_args()
@@ -123,6 +173,9 @@ def main():
         sys.exit(2)
     if args.pmc_sample:
         pmc_sample()
+        return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(steps=args.ensemble_steps)}))
         return
 
     cfg = W.c2_production()
@@ -172,6 +225,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms},
     }
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_bench(steps=args.ensemble_steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(line))

@@ -188,6 +188,7 @@ struct pz_handle {
     bool use_graph = true;
     size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0;
     int inv_waves = 16;
+    int mthreads = PZ_MPHASE_THREADS; // threads of k_mphase / k_mcoll
     PzDev *ddev = nullptr;
     std::string err;
 
@@ -698,7 +699,8 @@ static void setup(pz_handle *h)
     if (!d.coll) build_operators(h);
     // LDS budgets
     h->lds_radial = (5 * (size_t)h->nm + 6 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
-    h->lds_mphase = MpLayout(N).total;
+    h->mthreads = mloop_threads(N);
+    h->lds_mphase = MpLayout(N, h->mthreads).total;
     h->lds_start = 15 * (size_t)N * sizeof(cd);
     h->lds_start_coll = (13 * (size_t)N + 6 * (size_t)(2 * N - 2) + 3 * (size_t)N + 16) * sizeof(cd) +
                         3 * (size_t)N * sizeof(double);
@@ -1025,10 +1027,10 @@ static void launch_step(pz_handle *h)
             }
         }
         if (h->d.coll)
-            hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+            hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                (const PzDev *)h->ddev, stage, exp_calc);
         else
-            hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+            hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                (const PzDev *)h->ddev, stage, exp_calc);
     }
 }
@@ -1246,10 +1248,10 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
         HIPCHK(hipEventRecord(e0, h->stream));
         for (int i = 0; i < reps; i++) {
             if (which == 0 && h->d.coll)
-                hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+                hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else if (which == 0)
-                hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(PZ_MPHASE_THREADS), h->lds_mphase, h->stream,
+                hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else
                 hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, (const PzDev *)h->ddev, 0, 0);
@@ -1271,27 +1273,37 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
 // Ensembles: every kernel of the stage loop is launched once for all
 // members, each block finding its run and its own block index in a map.
 // ---------------------------------------------------------------------------
-struct pz_ensemble {
-    std::vector<pz_handle *> runs;
-    Scheme sch;
-    int coll = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
-    std::vector<void *> allocs;
+// One size class of an ensemble (members of one n_r): its own maps, LDS
+// budgets, stream and captured step graph, so that classes run side by
+// side and the small members are not given the LDS of the largest.
+struct EnsClass {
+    std::vector<int> members;
     const PzDev **druns = nullptr;
     int2 *map_r = nullptr, *map_c = nullptr, *map_3 = nullptr, *map_m = nullptr;
     int nb_r = 0, nb_c = 0, nb_3 = 0, nb_m = 0;
     size_t lds_r = 0, lds_m = 0, lds_b = 0, lds_i = 0;
-    int inv_threads = 64;
+    int inv_threads = 64, mthreads = PZ_MPHASE_THREADS;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+};
+
+struct pz_ensemble {
+    std::vector<pz_handle *> runs;
+    std::vector<EnsClass> cls;
+    Scheme sch;
+    int coll = 0;
+    std::vector<void *> allocs;
     std::string err;
     ~pz_ensemble()
     {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (done) (void)hipEventDestroy(done);
-        if (stream) (void)hipStreamDestroy(stream);
+        for (EnsClass &c : cls) {
+            if (c.gexec) (void)hipGraphExecDestroy(c.gexec);
+            if (c.graph) (void)hipGraphDestroy(c.graph);
+            if (c.done) (void)hipEventDestroy(c.done);
+            if (c.stream) (void)hipStreamDestroy(c.stream);
+        }
         for (void *p : allocs) (void)hipFree(p);
     }
     template <class T> T *upload(const std::vector<T> &v)
@@ -1304,33 +1316,33 @@ struct pz_ensemble {
     }
 };
 
-static void launch_ensemble_step(pz_ensemble *e)
+static void launch_class_step(const pz_ensemble *e, const EnsClass &c)
 {
     const Scheme &s = e->sch;
-    const PzDev *const *runs = e->druns;
+    const PzDev *const *runs = c.druns;
     for (int stage = 1; stage <= s.nstages; stage++) {
         const int exp_calc = s.l_exp_calc[stage - 1];
         if (exp_calc)
-            hipLaunchKernelGGL(k_radial_loop_batch, dim3(e->nb_r), dim3(256), e->lds_r, e->stream,
-                               PzBatch{runs, e->map_r}, stage - 1, stage == 1 ? 1 : 0);
+            hipLaunchKernelGGL(k_radial_loop_batch, dim3(c.nb_r), dim3(256), c.lds_r, c.stream,
+                               PzBatch{runs, c.map_r, (int)c.members.size()}, stage - 1, stage == 1 ? 1 : 0);
         if (stage == 1) {
-            hipLaunchKernelGGL(k_courant_batch, dim3(e->nb_c), dim3(256), 0, e->stream, PzBatch{runs, e->map_c});
+            hipLaunchKernelGGL(k_courant_batch, dim3(c.nb_c), dim3(256), 0, c.stream, PzBatch{runs, c.map_c, (int)c.members.size()});
             if (e->coll) {
-                hipLaunchKernelGGL(k_build_coll_batch, dim3(e->nb_3), dim3(1024), 0, e->stream,
-                                   PzBatch{runs, e->map_3}, 0);
+                hipLaunchKernelGGL(k_build_coll_batch, dim3(std::min(c.nb_3, 512)), dim3(1024), 0, c.stream,
+                                   PzBatch{runs, c.map_3, (int)c.members.size()}, c.nb_3, 0);
             } else {
-                hipLaunchKernelGGL(k_build_lhs_batch, dim3(e->nb_3), dim3(64), e->lds_b, e->stream,
-                                   PzBatch{runs, e->map_3}, 0);
-                hipLaunchKernelGGL(k_build_inv_batch, dim3(e->nb_3), dim3(e->inv_threads), e->lds_i, e->stream,
-                                   PzBatch{runs, e->map_3}, 0);
+                hipLaunchKernelGGL(k_build_lhs_batch, dim3(std::min(c.nb_3, 2048)), dim3(64), c.lds_b, c.stream,
+                                   PzBatch{runs, c.map_3, (int)c.members.size()}, c.nb_3, 0);
+                hipLaunchKernelGGL(k_build_inv_batch, dim3(std::min(c.nb_3, 512)), dim3(c.inv_threads), c.lds_i,
+                                   c.stream, PzBatch{runs, c.map_3, (int)c.members.size()}, c.nb_3, 0);
             }
         }
         if (e->coll)
-            hipLaunchKernelGGL(k_mcoll_batch, dim3(e->nb_m), dim3(PZ_MPHASE_THREADS), e->lds_m, e->stream,
-                               PzBatch{runs, e->map_m}, stage, exp_calc);
+            hipLaunchKernelGGL(k_mcoll_batch, dim3(c.nb_m), dim3(c.mthreads), c.lds_m, c.stream,
+                               PzBatch{runs, c.map_m, (int)c.members.size()}, stage, exp_calc);
         else
-            hipLaunchKernelGGL(k_mphase_batch, dim3(e->nb_m), dim3(PZ_MPHASE_THREADS), e->lds_m, e->stream,
-                               PzBatch{runs, e->map_m}, stage, exp_calc);
+            hipLaunchKernelGGL(k_mphase_batch, dim3(c.nb_m), dim3(c.mthreads), c.lds_m, c.stream,
+                               PzBatch{runs, c.map_m, (int)c.members.size()}, stage, exp_calc);
     }
 }
 
@@ -1341,9 +1353,7 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
     pz_ensemble *e = new pz_ensemble();
     try {
         const pz_handle *h0 = runs[0];
-        std::vector<const PzDev *> dr;
-        std::vector<int2> mr, mc, m3, mm;
-        int nw = 16;
+        std::vector<int> sizes;
         for (int r = 0; r < nruns; r++) {
             pz_handle *h = runs[r];
             if (!h) throw std::invalid_argument("null run handle");
@@ -1351,31 +1361,47 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
             if (h->sch.id != h0->sch.id || h->d.coll != h0->d.coll)
                 throw std::invalid_argument("the runs of an ensemble must share the time scheme and the radial method");
             e->runs.push_back(h);
-            dr.push_back(h->ddev);
-            for (int i = 0; i < h->N; i++) mr.push_back(make_int2(r, i));
-            mc.push_back(make_int2(r, 0));
-            for (int i = 0; i < 3 * h->nm; i++) m3.push_back(make_int2(r, i));
-            for (int i = 0; i < h->nm; i++) mm.push_back(make_int2(r, i));
-            e->lds_r = std::max(e->lds_r, h->lds_radial);
-            e->lds_m = std::max(e->lds_m, h->lds_mphase);
-            e->lds_b = std::max(e->lds_b, h->lds_build);
-            nw = std::min(nw, h->inv_waves);
+            if (std::find(sizes.begin(), sizes.end(), h->N) == sizes.end()) sizes.push_back(h->N);
         }
-        for (const pz_handle *h : e->runs) e->lds_i = std::max(e->lds_i, InvLayout(h->N, nw).total);
-        e->inv_threads = 64 * nw;
+        std::sort(sizes.begin(), sizes.end());
         e->sch = h0->sch;
         e->coll = h0->d.coll;
-        e->druns = e->upload(dr);
-        e->map_r = e->upload(mr);
-        e->map_c = e->upload(mc);
-        e->map_3 = e->upload(m3);
-        e->map_m = e->upload(mm);
-        e->nb_r = (int)mr.size();
-        e->nb_c = (int)mc.size();
-        e->nb_3 = (int)m3.size();
-        e->nb_m = (int)mm.size();
-        HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&e->done, hipEventDisableTiming));
+        for (int n : sizes) {
+            EnsClass c;
+            std::vector<const PzDev *> dr;
+            std::vector<int2> mr, mc, m3, mm;
+            int nw = 16;
+            for (int r = 0; r < nruns; r++) {
+                const pz_handle *h = runs[r];
+                if (h->N != n) continue;
+                const int k = (int)dr.size();
+                c.members.push_back(r);
+                dr.push_back(h->ddev);
+                for (int i = 0; i < h->N; i++) mr.push_back(make_int2(k, i));
+                mc.push_back(make_int2(k, 0));
+                for (int i = 0; i < 3 * h->nm; i++) m3.push_back(make_int2(k, i));
+                for (int i = 0; i < h->nm; i++) mm.push_back(make_int2(k, i));
+                c.lds_r = std::max(c.lds_r, h->lds_radial);
+                c.lds_m = std::max(c.lds_m, h->lds_mphase);
+                c.lds_b = std::max(c.lds_b, h->lds_build);
+                nw = std::min(nw, h->inv_waves);
+            }
+            c.lds_i = InvLayout(n, nw).total;
+            c.mthreads = mloop_threads(n);
+            c.inv_threads = 64 * nw;
+            c.druns = e->upload(dr);
+            c.map_r = e->upload(mr);
+            c.map_c = e->upload(mc);
+            c.map_3 = e->upload(m3);
+            c.map_m = e->upload(mm);
+            c.nb_r = (int)mr.size();
+            c.nb_c = (int)mc.size();
+            c.nb_3 = (int)m3.size();
+            c.nb_m = (int)mm.size();
+            HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+            e->cls.push_back(c);
+        }
     } catch (std::exception &ex) {
         std::fprintf(stderr, "pz_ensemble_create: %s\n", ex.what());
         delete e;
@@ -1391,17 +1417,22 @@ extern "C" int pz_ensemble_step(pz_ensemble *e, int nsteps)
     try {
         // work already queued on the members comes first
         for (pz_handle *h : e->runs) HIPCHK(hipStreamSynchronize(h->stream));
-        if (!e->gexec) {
-            HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            launch_ensemble_step(e);
-            HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
-            HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
-        }
-        for (int i = 0; i < nsteps; i++) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+        for (EnsClass &c : e->cls)
+            if (!c.gexec) {
+                HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+                launch_class_step(e, c);
+                HIPCHK(hipStreamEndCapture(c.stream, &c.graph));
+                HIPCHK(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
+            }
+        // the classes advance side by side, each on its own stream
+        for (int i = 0; i < nsteps; i++)
+            for (EnsClass &c : e->cls) HIPCHK(hipGraphLaunch(c.gexec, c.stream));
         HIPCHK(hipGetLastError());
-        // later calls on a member wait for the ensemble's work
-        HIPCHK(hipEventRecord(e->done, e->stream));
-        for (pz_handle *h : e->runs) HIPCHK(hipStreamWaitEvent(h->stream, e->done, 0));
+        // later calls on a member wait for its class's work
+        for (EnsClass &c : e->cls) {
+            HIPCHK(hipEventRecord(c.done, c.stream));
+            for (int r : c.members) HIPCHK(hipStreamWaitEvent(e->runs[r]->stream, c.done, 0));
+        }
     } catch (std::exception &ex) {
         e->err = ex.what();
         return -4;
@@ -1412,7 +1443,7 @@ extern "C" int pz_ensemble_step(pz_ensemble *e, int nsteps)
 extern "C" int pz_ensemble_destroy(pz_ensemble *e)
 {
     if (e) {
-        (void)hipStreamSynchronize(e->stream);
+        for (EnsClass &c : e->cls) (void)hipStreamSynchronize(c.stream);
         delete e;
     }
     return 0;

@@ -244,12 +244,17 @@ __global__ void __launch_bounds__(1024) k_build_coll(const PzDev *__restrict__ p
 {
     build_coll_body(*pd, blockIdx.x, blockIdx.y, force);
 }
-// batched: local block index = mi + nm fld
-__global__ void __launch_bounds__(1024) k_build_coll_batch(PzBatch b, int force)
+// batched: local block index = mi + nm fld; a grid-stride loop over the
+// nb blocks of all runs, so that the usual early exit costs few workgroups
+__global__ void __launch_bounds__(1024) k_build_coll_batch(PzBatch b, int nb, int force)
 {
-    const int2 e = b.map[blockIdx.x];
-    const PzDev &p = *b.runs[e.x];
-    build_coll_body(p, e.y % p.nm, e.y / p.nm, force);
+    if (!batch_any_lmat(b, force)) return;
+    for (int i = blockIdx.x; i < nb; i += gridDim.x) {
+        const int2 e = b.map[i];
+        const PzDev &p = *b.runs[e.x];
+        build_coll_body(p, e.y % p.nm, e.y / p.nm, force);
+        __syncthreads();
+    }
 }
 
 // get_ddcheb_real_1d on Chebyshev coefficients f (radial_derivatives.f90:
@@ -321,7 +326,7 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
     const int tid = threadIdx.x;
-    const MpLayout lay(N);
+    const MpLayout lay(N, blockDim.x);
     char *lb = (char *)smem;
     cd *V = (cd *)(lb + lay.vec);
     cd *aT = V, *cT = V + N, *eT = V + 2 * N;

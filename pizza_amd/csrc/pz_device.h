@@ -1391,11 +1391,31 @@ __global__ void __launch_bounds__(64) k_build_lhs(const PzDev *__restrict__ pd, 
 {
     build_lhs_body(*pd, blockIdx.x, blockIdx.y, force);
 }
-// batched: local block index = fld + 3 mi
-__global__ void __launch_bounds__(64) k_build_lhs_batch(PzBatch b, int force)
+// 1 when any run of a batch rebuilds its matrices in this step (the lanes of
+// the first wave read the runs' flags); one dependent load chain instead of
+// one per block of the batch
+__device__ int batch_any_lmat(const PzBatch &b, int force)
 {
-    const int2 e = b.map[blockIdx.x];
-    build_lhs_body(*b.runs[e.x], e.y % 3, e.y / 3, force);
+    __shared__ int s_any;
+    if (threadIdx.x < 64) {
+        int any = force;
+        for (int r = threadIdx.x; r < b.nruns; r += 64) any |= b.runs[r]->ss->lMat != 0;
+        any = __any(any);
+        if (threadIdx.x == 0) s_any = any;
+    }
+    __syncthreads();
+    return s_any;
+}
+
+// batched: local block index = fld + 3 mi (grid-stride over the nb blocks)
+__global__ void __launch_bounds__(64) k_build_lhs_batch(PzBatch b, int nb, int force)
+{
+    if (!batch_any_lmat(b, force)) return;
+    for (int i = blockIdx.x; i < nb; i += gridDim.x) {
+        const int2 e = b.map[i];
+        build_lhs_body(*b.runs[e.x], e.y % 3, e.y / 3, force);
+        __syncthreads();
+    }
 }
 
 // global -> LDS copy by the whole block, four loads in flight per thread
@@ -1591,10 +1611,14 @@ __global__ void __launch_bounds__(1024) k_build_inv(const PzDev *__restrict__ pd
 {
     build_inv_body(*pd, blockIdx.x, blockIdx.y, force);
 }
-// batched: local block index = mi + nm fld
-__global__ void __launch_bounds__(1024) k_build_inv_batch(PzBatch b, int force)
+// batched: local block index = mi + nm fld (grid-stride over the nb blocks)
+__global__ void __launch_bounds__(1024) k_build_inv_batch(PzBatch b, int nb, int force)
 {
-    const int2 e = b.map[blockIdx.x];
-    const PzDev &p = *b.runs[e.x];
-    build_inv_body(p, e.y % p.nm, e.y / p.nm, force);
+    if (!batch_any_lmat(b, force)) return;
+    for (int i = blockIdx.x; i < nb; i += gridDim.x) {
+        const int2 e = b.map[i];
+        const PzDev &p = *b.runs[e.x];
+        build_inv_body(p, e.y % p.nm, e.y / p.nm, force);
+        __syncthreads();
+    }
 }

@@ -40,6 +40,7 @@ struct PzDev;
 struct PzBatch {
     const PzDev *const *runs;
     const int2 *map;
+    int nruns;
 };
 
 // Everything a kernel needs: sizes, constants and device pointers.

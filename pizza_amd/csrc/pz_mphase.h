@@ -84,12 +84,16 @@ __device__ void dcheb_real(const PzDev &p, const double *f, double *df)
 
 #define MPHASE_NVEC 24
 #define PZ_MPHASE_THREADS 512
+// threads of the m-loop kernels for n_r: 256 leave room for more
+// workgroups per CU on small grids (the coupled collocation system needs
+// 2 n_r <= threads)
+__host__ __device__ inline int mloop_threads(int N) { return N <= 65 ? 256 : PZ_MPHASE_THREADS; }
 
 // LDS layout of k_mphase: working vectors, transform tables and the
 // reduction scratch of the dense solves (16-byte aligned blocks).
 struct MpLayout {
     size_t vec, dtmp, cosT, sinT, rad, red, total;
-    __host__ __device__ MpLayout(int N)
+    __host__ __device__ MpLayout(int N, int threads)
     {
         size_t o = 0;
         auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
@@ -100,8 +104,8 @@ struct MpLayout {
         sinT = take(sizeof(double) * L);
         rad = take(sizeof(double) * PZ_NRADC * N);
         // FFT scratch of the Chebyshev transforms (6 L + 12), shared with the
-        // reduction scratch of the dense solves (2 matrices x 2 rows x 512 threads)
-        red = take(sizeof(cd) * (6 * L + 16 > 2048 ? 6 * L + 16 : 2048));
+        // reduction scratch of the dense solves (2 matrices x 2 rows x threads)
+        red = take(sizeof(cd) * (6 * L + 16 > 4 * threads ? 6 * L + 16 : 4 * threads));
         total = o;
     }
 };
@@ -119,7 +123,7 @@ __device__ __forceinline__ void mphase_body(const PzDev &p, int mi, int istage, 
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
     const int tid = threadIdx.x;
-    const MpLayout lay(N);
+    const MpLayout lay(N, blockDim.x);
     char *lb = (char *)smem;
     cd *V = (cd *)(lb + lay.vec);
     cd *aT = V, *cT = V + N, *eT = V + 2 * N;
