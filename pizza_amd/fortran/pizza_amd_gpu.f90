@@ -159,7 +159,22 @@ contains
             import :: c_size_t, c_ptr
             type(c_ptr), value :: str
          end function c_strlen
-      end interface
+         integer(c_int) function pz_ensemble_create(runs, nruns, e) bind(c, name='pz_ensemble_create')
+         import :: c_int, c_ptr
+         type(c_ptr), intent(in) :: runs(*)
+         integer(c_int), value :: nruns
+         type(c_ptr), intent(out) :: e
+      end function
+      integer(c_int) function pz_ensemble_step(e, nsteps) bind(c, name='pz_ensemble_step')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: e
+         integer(c_int), value :: nsteps
+      end function
+      integer(c_int) function pz_ensemble_destroy(e) bind(c, name='pz_ensemble_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: e
+      end function
+   end interface
       p = pz_last_error_c(h)
       n = int(c_strlen(p))
       call c_f_pointer(p, s, [n])
