@@ -88,6 +88,10 @@ typedef struct pz_config {
     double tcond_fac, xicond_fac;
     double t_top[3 * PZ_N_T_BOUNDS], t_bot[3 * PZ_N_T_BOUNDS];
     double xi_top[3 * PZ_N_T_BOUNDS], xi_bot[3 * PZ_N_T_BOUNDS];
+    /* &grid / &control: radial_scheme='FD' (second-order finite differences,
+       radial.f90:86-99) on the stretched grid of finite_differences.f90:87-188 */
+    double fd_stretch, fd_ratio;
+    int l_finite_diff, fd_pad;
 } pz_config;
 
 typedef struct pz_handle pz_handle;

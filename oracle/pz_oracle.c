@@ -167,6 +167,16 @@ struct or_handle {
     double time;
     int lMatNext, n_time_step, started;
     double *dtr, *dth;
+    /* finite differences (radial_scheme = 'FD', finite_differences.f90) */
+    int fd;
+    double *or3, *d2beta, *d3beta;
+    double *fdr, *fddr;     /* [N][3]: stencil of points n-1, n, n+1 */
+    double *fdddr, *fddddr; /* [N][5]: points n-2 .. n+2 */
+    double fdr_top[3], fdr_bot[3];
+    double *tTd, *tTl, *tTu, *tXd, *tXl, *tXu; /* tridiagonal T, xi per m: [m][N] */
+    double *uD, *uL, *uU;                      /* tridiagonal u_phi (m = 0) */
+    cplx *pD, *pL1, *pL2, *pU1, *pU2;          /* pentadiagonal psi per m: [m][N] */
+    cplx *tg, *xg, *pg, *u0g; /* ghosted T, xi [m][N+2], psi [m][N+4], u_phi(m=0) [N+2] */
     /* collocation method (l_cheb_coll) */
     int coll;
     double *dbeta;
@@ -219,6 +229,9 @@ void or_config_default(pz_config *c)
     c->l_temp_advz = 0;
     c->tcond_fac = 1.0;
     c->xicond_fac = 1.0;
+    c->fd_stretch = 0.3; /* namelists.f90:544-545 */
+    c->fd_ratio = 0.1;
+    c->l_finite_diff = 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1559,6 +1572,8 @@ static void bc_pattern(or_handle *h, const double *tb, const double *tt, cplx *b
 
 static int build_matrices(or_handle *h, int lhs_only);
 static void coll_alloc(or_handle *h);
+static int fd_grid(or_handle *h, double ratio1, double ratio2);
+static void fd_alloc(or_handle *h);
 
 int or_create(const pz_config *cfg, or_handle **out)
 {
@@ -1577,6 +1592,8 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->idx2m = calloc(h->nm, sizeof(int));
     for (int i = 0; i < h->nm; i++) h->idx2m[i] = i * cfg->minc;
     h->coll = cfg->l_cheb_coll != 0;
+    h->fd = cfg->l_finite_diff != 0;
+    if (h->fd) h->coll = 0;
     if (ts_init(&h->ts, cfg->time_scheme, cfg->courfac)) { free(h); return -14; }
     /* namelists.f90:398-526 */
     h->l_heat = cfg->ra != 0.0;
@@ -1636,6 +1653,7 @@ int or_create(const pz_config *cfg, or_handle **out)
         h->drx[k] = 2.0 / (h->r_cmb - h->r_icb);
         h->ddrx[k] = 0.0;
     }
+    if (h->fd && fd_grid(h, cfg->fd_stretch, cfg->fd_ratio)) { free(h); return -15; }
     /* tau lines, chebyshev.f90:362-381 */
     h->rMat = dalloc(2 * N); h->drMat = dalloc(2 * N); h->d2rMat = dalloc(2 * N);
     for (int n = 0; n < N; n++) {
@@ -1697,6 +1715,12 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->dth = dalloc(N);
     for (int i = 0; i < 16; i++) h->gr[i] = dalloc(h->nphi);
     /* matrices */
+    if (h->fd) {
+        fd_alloc(h);
+        h->lMatNext = 1;
+        *out = h;
+        return 0;
+    }
     if (h->coll) {
         coll_alloc(h);
         h->lMatNext = 1;
@@ -2642,6 +2666,589 @@ static void coll_update_psi(or_handle *h)
     free(om0);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Finite differences in radius (radial_scheme = 'FD'): second-order        */
+/* stencils on the stretched grid, tridiagonal T / xi / u_phi(m=0) and      */
+/* pentadiagonal psi systems with ghost zones                               */
+/* ------------------------------------------------------------------------ */
+
+/* Fornberg's weights (Math. Comp. 51 (1988) 699-706), the algorithm of
+ * populate_fd_weights (finite_differences.f90:501-546): c[j * (m + 1) + k] is
+ * the weight of the value at x[j] in the k-th derivative at z. */
+static void fd_weights(double z, const double *x, int nd, int m, double *c)
+{
+    const int w = m + 1;
+    double c1 = 1.0, c4 = x[0] - z;
+    for (int i = 0; i < (nd + 1) * w; i++) c[i] = 0.0;
+    c[0] = 1.0;
+    for (int i = 1; i <= nd; i++) {
+        int mn = i < m ? i : m;
+        double c2 = 1.0, c5 = c4;
+        c4 = x[i] - z;
+        for (int j = 0; j < i; j++) {
+            double c3 = x[i] - x[j];
+            c2 *= c3;
+            if (j == i - 1) {
+                for (int k = mn; k >= 1; k--)
+                    c[i * w + k] = c1 * ((double)k * c[(i - 1) * w + k - 1] - c5 * c[(i - 1) * w + k]) / c2;
+                c[i * w] = -c1 * c5 * c[(i - 1) * w] / c2;
+            }
+            for (int k = mn; k >= 1; k--) c[j * w + k] = (c4 * c[j * w + k] - (double)k * c[j * w + k - 1]) / c3;
+            c[j * w] = c4 * c[j * w] / c3;
+        }
+        c1 = c2;
+    }
+}
+
+/* get_FD_grid (finite_differences.f90:87-188), annulus (r_icb > 0):
+ * r[0] = r_cmb, decreasing; ratio1 = fd_stretch, ratio2 = fd_ratio */
+static int fd_grid(or_handle *h, double ratio1, double ratio2)
+{
+    int N = h->N;
+    double *r = h->r, drb, dra;
+    r[0] = h->r_cmb;
+    if (ratio2 == 1.0) {
+        drb = 1.0 / ((double)N - 1.0);
+        for (int k = 1; k < N; k++) r[k] = r[k - 1] - drb;
+    } else {
+        int nbp = (int)((double)(N - 1) / (2.0 * (1.0 + ratio1)));
+        int nbulk = N - 1 - 2 * nbp;
+        dra = exp(log(ratio2) / (double)nbp);
+        drb = 1.0;
+        for (int i = 0; i < nbp; i++) drb *= dra;
+        drb = 1.0 / ((double)nbulk + 2.0 * dra * ((1.0 - drb) / (1.0 - dra)));
+        for (int i = 0; i < nbp; i++) drb *= dra;
+        for (int k = 1; k <= nbp; k++) {
+            r[k] = r[k - 1] - drb;
+            drb /= dra;
+        }
+        for (int k = 1; k <= nbulk; k++) r[k + nbp] = r[k + nbp - 1] - drb;
+        for (int k = 1; k <= nbp; k++) {
+            drb *= dra;
+            r[k + nbp + nbulk] = r[k + nbp + nbulk - 1] - drb;
+        }
+    }
+    if (r[N - 1] != h->r_icb) {
+        if (fabs(r[N - 1] - h->r_icb) > drb) return -1;
+        r[N - 1] = h->r_icb;
+    }
+    return 0;
+}
+
+/* get_FD_coeffs, order 2 and boundary order 2 (finite_differences.f90:190-446):
+ * the stencils the step uses */
+static void fd_coeffs(or_handle *h)
+{
+    int N = h->N;
+    const double *r = h->r;
+    double x[5], c[25];
+    for (int k = 0; k < N; k++) {
+        double *d1 = h->fdr + 3 * k, *d2 = h->fddr + 3 * k;
+        if (k == 0 || k == N - 1) {
+            double drl = k == 0 ? r[1] - r[0] : r[N - 1] - r[N - 2];
+            d1[0] = -0.5 / drl; d1[1] = 0.0; d1[2] = 0.5 / drl;
+            d2[0] = 1.0 / drl / drl; d2[1] = -2.0 / drl / drl; d2[2] = 1.0 / drl / drl;
+        } else {
+            for (int od = 0; od < 3; od++) x[od] = r[k - 1 + od] - r[k];
+            fd_weights(0.0, x, 2, 2, c);
+            for (int od = 0; od < 3; od++) { d1[od] = c[od * 3 + 1]; d2[od] = c[od * 3 + 2]; }
+        }
+    }
+    for (int od = 0; od < 3; od++) x[od] = r[od] - r[0];
+    fd_weights(0.0, x, 2, 2, c);
+    for (int od = 0; od < 3; od++) h->fdr_top[od] = c[od * 3 + 1];
+    for (int od = 0; od < 3; od++) x[od] = r[N - 1 - od] - r[N - 1];
+    fd_weights(0.0, x, 2, 2, c);
+    for (int od = 0; od < 3; od++) h->fdr_bot[od] = c[od * 3 + 1];
+    /* 3rd and 4th derivatives: symmetric ghost points beyond the walls */
+    for (int k = 1; k < N - 1; k++) {
+        for (int od = 0; od < 5; od++) {
+            if (k == 1 && od == 0) x[od] = 2.0 * (r[0] - r[1]);
+            else if (k == N - 2 && od == 4) x[od] = 2.0 * (r[N - 1] - r[N - 2]);
+            else x[od] = r[k - 2 + od] - r[k];
+        }
+        fd_weights(0.0, x, 4, 4, c);
+        for (int od = 0; od < 5; od++) { h->fdddr[5 * k + od] = c[od * 5 + 3]; h->fddddr[5 * k + od] = c[od * 5 + 4]; }
+    }
+}
+
+#define TG(g, i, k) (g)[(size_t)(i) * (N + 2) + (k) + 1]
+#define PG(g, i, k) (g)[(size_t)(i) * (N + 4) + (k) + 2]
+
+static void fd_alloc(or_handle *h)
+{
+    int N = h->N, nm = h->nm;
+    double rc = h->r_cmb;
+    h->or3 = dalloc(N); h->dbeta = dalloc(N); h->d2beta = dalloc(N); h->d3beta = dalloc(N);
+    for (int n = 0; n < N; n++) h->or3[n] = h->or2[n] * h->or1[n];
+    /* radial.f90:212-226 (sphere container; zero at r_cmb) */
+    for (int n = 1; n < N; n++) {
+        double r = h->r[n], s2 = rc * rc - r * r, hh = h->height[n];
+        h->dbeta[n] = -(rc * rc + r * r) / (s2 * s2);
+        h->d2beta[n] = -128.0 * r * (3.0 * rc * rc + r * r) / pow(hh, 6);
+        h->d3beta[n] = -1536.0 * (pow(rc, 4) + pow(r, 4) + 6.0 * r * r * rc * rc) / pow(hh, 8);
+    }
+    h->fdr = dalloc(3 * N); h->fddr = dalloc(3 * N); h->fdddr = dalloc(5 * N); h->fddddr = dalloc(5 * N);
+    fd_coeffs(h);
+    h->tTd = dalloc(nm * N); h->tTl = dalloc(nm * N); h->tTu = dalloc(nm * N);
+    h->tXd = dalloc(nm * N); h->tXl = dalloc(nm * N); h->tXu = dalloc(nm * N);
+    h->uD = dalloc(N); h->uL = dalloc(N); h->uU = dalloc(N);
+    size_t F = (size_t)nm * N;
+    h->pD = calloc(F, sizeof(cplx)); h->pL1 = calloc(F, sizeof(cplx)); h->pL2 = calloc(F, sizeof(cplx));
+    h->pU1 = calloc(F, sizeof(cplx)); h->pU2 = calloc(F, sizeof(cplx));
+    h->tg = calloc((size_t)nm * (N + 2), sizeof(cplx));
+    h->xg = calloc((size_t)nm * (N + 2), sizeof(cplx));
+    h->pg = calloc((size_t)nm * (N + 4), sizeof(cplx));
+    h->u0g = calloc(N + 2, sizeof(cplx));
+}
+
+/* prepare_mat_3 (parallel_solver.f90:135-187): diag <- 1/pivot */
+static void fd_lu3(int N, double *d, double *lo, double *up)
+{
+    lo[0] = 0.0;
+    up[N - 1] = 0.0;
+    d[0] = 1.0 / d[0];
+    for (int k = 1; k < N; k++) d[k] = 1.0 / (d[k] - lo[k] * up[k - 1] * d[k - 1]);
+}
+
+/* prepare_mat_5 (parallel_solver.f90:189-288) */
+static void fd_lu5(int N, cplx *d, cplx *l1, cplx *l2, cplx *u1, cplx *u2)
+{
+    l1[0] = 0.0; l2[0] = 0.0; l2[1] = 0.0;
+    u1[N - 1] = 0.0; u2[N - 1] = 0.0; u2[N - 2] = 0.0;
+    u1[1] = u1[1] - l1[1] * u2[0] / d[0];
+    d[1] = d[1] - l1[1] * u1[0] / d[0];
+    for (int k = 2; k < N; k++) {
+        l1[k] = l1[k] - l2[k] * u1[k - 2] / d[k - 2];
+        u1[k] = u1[k] - l1[k] * u2[k - 1] / d[k - 1];
+        d[k] = d[k] - l1[k] * u1[k - 1] / d[k - 1] - l2[k] * u2[k - 2] / d[k - 2];
+    }
+    for (int k = 0; k < N; k++) {
+        d[k] = 1.0 / d[k];
+        u1[k] *= d[k]; u2[k] *= d[k]; l1[k] *= d[k]; l2[k] *= d[k];
+    }
+}
+
+/* get_tMat_Rdist (update_temp_fd.f90:387-483) and its xi twin */
+static void fd_scalar_mat(or_handle *h, int i, double diff, int ktop, int kbot, double *d, double *lo, double *up)
+{
+    int N = h->N, m = h->idx2m[i];
+    double w = h->ts.wimp_lin[0], dm2 = (double)m * (double)m;
+    for (int k = 0; k < N; k++) {
+        const double *a = h->fdr + 3 * k, *b = h->fddr + 3 * k;
+        d[k] = 1.0 - w * diff * (b[1] + h->or1[k] * a[1] - dm2 * h->or2[k]);
+        lo[k] = -w * diff * (b[0] + h->or1[k] * a[0]);
+        up[k] = -w * diff * (b[2] + h->or1[k] * a[2]);
+    }
+    if (ktop == 1) { d[0] = 1.0; up[0] = 0.0; lo[0] = 0.0; }
+    else up[0] = up[0] + lo[0];
+    if (kbot == 1) { d[N - 1] = 1.0; up[N - 1] = 0.0; lo[N - 1] = 0.0; }
+    else lo[N - 1] = up[N - 1] + lo[N - 1];
+    fd_lu3(N, d, lo, up);
+}
+
+/* get_uphimat_Rdist (update_psi_fd.f90:772-836) */
+static void fd_uphi_mat(or_handle *h)
+{
+    int N = h->N;
+    double w = h->ts.wimp_lin[0], nu = h->ViscFac;
+    double *d = h->uD, *lo = h->uL, *up = h->uU;
+    for (int k = 0; k < N; k++) {
+        const double *a = h->fdr + 3 * k, *b = h->fddr + 3 * k;
+        d[k] = 1.0 - w * (nu * (b[1] + h->or1[k] * a[1] - h->or2[k]) - h->CorFac * h->ekpump[k]);
+        lo[k] = -w * nu * (b[0] + h->or1[k] * a[0]);
+        up[k] = -w * nu * (b[2] + h->or1[k] * a[2]);
+    }
+    if (h->c.ktopv == 1) {
+        up[0] = up[0] + lo[0];
+        d[0] = d[0] - 2.0 * (h->r[1] - h->r[0]) * h->or1[0] * lo[0];
+    } else { d[0] = 1.0; lo[0] = 0.0; up[0] = 0.0; }
+    if (h->c.kbotv == 1) {
+        lo[N - 1] = lo[N - 1] + up[N - 1];
+        d[N - 1] = d[N - 1] + 2.0 * (h->r[N - 1] - h->r[N - 2]) * h->or1[N - 1] * up[N - 1];
+    } else { d[N - 1] = 1.0; lo[N - 1] = 0.0; up[N - 1] = 0.0; }
+    fd_lu3(N, d, lo, up);
+}
+
+/* get_psimat_Rdist (update_psi_fd.f90:608-770), one m /= 0 */
+static void fd_psi_mat(or_handle *h, int i)
+{
+    int N = h->N, m = h->idx2m[i];
+    size_t o = (size_t)i * N;
+    double w = h->ts.wimp_lin[0], nu = h->ViscFac, cf = h->CorFac, dm = (double)m, dm2 = dm * dm;
+    cplx *d = h->pD + o, *l1 = h->pL1 + o, *l2 = h->pL2 + o, *u1 = h->pU1 + o, *u2 = h->pU2 + o;
+    for (int k = 0; k < N; k++) {
+        const double *a = h->fdr + 3 * k, *b = h->fddr + 3 * k, *c3 = h->fdddr + 5 * k, *c4 = h->fddddr + 5 * k;
+        double o1 = h->or1[k], o2 = h->or2[k], o3 = h->or3[k], be = h->beta[k], db = h->dbeta[k];
+        double d2b = h->d2beta[k], d3b = h->d3beta[k], ek = h->ekpump[k], eu = h->ekp_up[k];
+        double A3 = 2.0 * o1 + be;
+        double A2 = 3.0 * db + 2.0 * be * o1 - o2 - 2.0 * dm2 * o2;
+        double A1 = 3.0 * d2b + 4.0 * db * o1 - be * o2 + o3 + dm2 * o2 * (2.0 * o1 - be);
+        double A0 = d3b + 2.0 * d2b * o1 - db * o2 + be * o3 - dm2 * o2 * (4.0 * o2 + be * o1 + db - dm2 * o2);
+        cplx P0 = db + be * o1 - dm2 * o2 + be * eu - I * dm * o1 * (h->ekp_us[k] + I * dm * h->ekp_dusdp[k]);
+        d[k] = -(b[1] + (o1 + be) * a[1] + (be * o1 + db - dm2 * o2)) +
+               w * (nu * (c4[2] + A3 * c3[2] + A2 * b[1] + A1 * a[1] + A0) -
+                    cf * ek * (b[1] + (be + o1 + eu) * a[1] + P0) - cf * be * I * dm * o1);
+        l1[k] = -(b[0] + (o1 + be) * a[0]) +
+                w * (nu * (c4[1] + A3 * c3[1] + A2 * b[0] + A1 * a[0]) - cf * ek * (b[0] + (be + o1 + eu) * a[0]));
+        u1[k] = -(b[2] + (o1 + be) * a[2]) +
+                w * (nu * (c4[3] + A3 * c3[3] + A2 * b[2] + A1 * a[2]) - cf * ek * (b[2] + (be + o1 + eu) * a[2]));
+        l2[k] = w * nu * (c4[0] + A3 * c3[0]);
+        u2[k] = w * nu * (c4[4] + A3 * c3[4]);
+    }
+    /* non-penetration rows and the second condition through the ghost point */
+    d[0] = 1.0; l1[0] = 0.0; l2[0] = 0.0; u1[0] = 0.0; u2[0] = 0.0;
+    if (h->c.ktopv == 1) {
+        double dr = h->r[1] - h->r[0], fac = (1.0 - 0.5 * h->or1[0] * dr) / (1.0 + 0.5 * h->or1[0] * dr);
+        d[1] = d[1] - fac * l2[1];
+        l1[1] = l1[1] + 2.0 / (1.0 + 0.5 * h->or1[0] * dr) * l2[1];
+    } else {
+        d[1] = d[1] + l2[1];
+    }
+    d[N - 1] = 1.0; l1[N - 1] = 0.0; l2[N - 1] = 0.0; u1[N - 1] = 0.0; u2[N - 1] = 0.0;
+    if (h->c.kbotv == 1) {
+        double dr = h->r[N - 1] - h->r[N - 2];
+        double fac = (1.0 + 0.5 * h->or1[N - 1] * dr) / (1.0 - 0.5 * h->or1[N - 1] * dr);
+        d[N - 2] = d[N - 2] - fac * u2[N - 2];
+        u1[N - 2] = u1[N - 2] + 2.0 * u2[N - 2] / (1.0 - 0.5 * h->or1[N - 1] * dr);
+    } else {
+        d[N - 2] = d[N - 2] + u2[N - 2];
+    }
+    fd_lu5(N, d, l1, l2, u1, u2);
+}
+
+static void fd_build(or_handle *h)
+{
+    int N = h->N;
+    for (int i = 0; i < h->nm; i++) {
+        if (h->l_heat)
+            fd_scalar_mat(h, i, h->TdiffFac, h->c.ktopt, h->c.kbott, h->tTd + (size_t)i * N, h->tTl + (size_t)i * N,
+                          h->tTu + (size_t)i * N);
+        if (h->l_chem)
+            fd_scalar_mat(h, i, h->XidiffFac, h->c.ktopxi, h->c.kbotxi, h->tXd + (size_t)i * N,
+                          h->tXl + (size_t)i * N, h->tXu + (size_t)i * N);
+        if (h->idx2m[i] != 0) fd_psi_mat(h, i);
+    }
+    fd_uphi_mat(h);
+}
+
+/* solver_up_3 / solver_dn_3 (and solver_single): x[k + 1], ghosts x[0], x[N + 1] */
+static void fd_solve3(int N, const double *d, const double *lo, const double *up, cplx *x)
+{
+    x[1] = x[1] - lo[0] * x[0];
+    for (int k = 1; k < N; k++) x[k + 1] = x[k + 1] - d[k - 1] * lo[k] * x[k];
+    for (int k = N - 1; k >= 0; k--) x[k + 1] = (x[k + 1] - up[k] * x[k + 2]) * d[k];
+}
+
+/* solver_up_5 / solver_dn_5: x[k + 2], ghosts two deep */
+static void fd_solve5(int N, const cplx *d, const cplx *l1, const cplx *l2, const cplx *u1, const cplx *u2, cplx *x)
+{
+    for (int k = 0; k < N; k++) x[k + 2] = d[k] * x[k + 2] - l1[k] * x[k + 1] - l2[k] * x[k];
+    for (int k = N - 1; k >= 0; k--) x[k + 2] = x[k + 2] - u1[k] * x[k + 3] - u2[k] * x[k + 4];
+}
+
+/* get_dr_Rloc (radial_derivatives.f90:669-760) of one radial column */
+static void fd_dr_col(or_handle *h, const cplx *f, cplx *df)
+{
+    int N = h->N;
+    for (int k = 0; k < N; k++) {
+        const double *a = h->fdr + 3 * k;
+        cplx fm = k > 0 ? f[k - 1] : 0.0, fp = k < N - 1 ? f[k + 1] : 0.0;
+        df[k] = a[0] * fm + a[1] * f[k] + a[2] * fp;
+    }
+    df[0] = 0.0;
+    for (int od = 0; od < 3; od++) df[0] += h->fdr_top[od] * f[od];
+    df[N - 1] = 0.0;
+    for (int od = 0; od < 3; od++) df[N - 1] += h->fdr_bot[od] * f[N - 1 - od];
+}
+
+/* finish_explicit_assembly_Rdist (mloop_fd.f90:78-110; update_temp_fd.f90:278-315,
+ * update_xi_fd.f90:280-290, update_psi_fd.f90:397-470); buoyancy is explicit */
+static void fd_finish_exp(or_handle *h, int is)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    cplx *w = malloc(sizeof(cplx) * N);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        size_t o = (size_t)i * N;
+        cplx *ep = h->dpsidt.expl + is * F + o;
+        if (m != 0) {
+            fd_dr_col(h, h->dVsOm + o, w);
+            for (int k = 0; k < N; k++) {
+                ep[k] = ep[k] - h->or1[k] * w[k];
+                if (!h->c.l_coriolis_imp) ep[k] = ep[k] + h->CorFac * h->beta[k] * h->us[o + k];
+                if (h->l_heat) ep[k] = ep[k] - h->BuoFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->temp[o + k];
+                if (h->l_chem) ep[k] = ep[k] - h->ChemFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->xi[o + k];
+            }
+        }
+        if (h->l_heat) {
+            cplx *et = h->dTdt.expl + is * F + o;
+            fd_dr_col(h, h->dVsT + o, w);
+            for (int k = 0; k < N; k++)
+                et[k] = et[k] - h->or1[k] * w[k] - h->us[o + k] * (h->dtcond[k] - h->tadvz * h->beta[k] * h->tcond[k]);
+        }
+        if (h->l_chem) {
+            cplx *ex = h->dxidt.expl + is * F + o;
+            fd_dr_col(h, h->dVsXi + o, w);
+            for (int k = 0; k < N; k++) ex[k] = ex[k] - h->or1[k] * w[k] - h->us[o + k] * h->dxicond[k];
+        }
+    }
+    free(w);
+}
+
+/* fill_ghosts_temp / fill_ghosts_xi (update_temp_fd.f90:117-169) */
+static void fd_fill_ghosts_scalar(or_handle *h, cplx *g, int ktop, int kbot)
+{
+    int N = h->N;
+    for (int i = 0; i < h->nm; i++) {
+        TG(g, i, -1) = ktop == 1 ? 2.0 * TG(g, i, 0) - TG(g, i, 1) : TG(g, i, 1);
+        TG(g, i, N) = kbot == 1 ? 2.0 * TG(g, i, N - 1) - TG(g, i, N - 2) : TG(g, i, N - 2);
+    }
+}
+
+/* fill_ghosts_psi (update_psi_fd.f90:140-215) */
+static void fd_fill_ghosts_psi(or_handle *h)
+{
+    int N = h->N;
+    cplx *u = h->u0g;
+    double dr = h->r[1] - h->r[0];
+    u[0] = h->c.ktopv == 1 ? u[2] - 2.0 * dr * h->or1[0] * u[1] : 2.0 * u[1] - u[2];
+    double db = h->r[N - 1] - h->r[N - 2];
+    u[N + 1] = h->c.kbotv == 1 ? u[N - 1] + 2.0 * db * h->or1[N - 1] * u[N] : 2.0 * u[N] - u[N - 1];
+    for (int i = 0; i < h->nm; i++) {
+        if (h->idx2m[i] == 0) continue;
+        if (h->c.ktopv == 1)
+            PG(h->pg, i, -1) = -(1.0 - 0.5 * h->or1[0] * dr) / (1.0 + 0.5 * h->or1[0] * dr) * PG(h->pg, i, 1);
+        else
+            PG(h->pg, i, -1) = PG(h->pg, i, 1);
+        PG(h->pg, i, -2) = 0.0;
+        if (h->c.kbotv == 1)
+            PG(h->pg, i, N) = -(1.0 + 0.5 * h->or1[N - 1] * db) / (1.0 - 0.5 * h->or1[N - 1] * db) * PG(h->pg, i, N - 2);
+        else
+            PG(h->pg, i, N) = PG(h->pg, i, N - 2);
+        PG(h->pg, i, N + 1) = 0.0;
+    }
+}
+
+/* get_temp_rhs_imp_ghost (update_temp_fd.f90:217-276) and its xi twin */
+static void fd_rhs_imp_scalar(or_handle *h, const cplx *g, cplx *dfld, tarray *a, int istage, int l_calc_lin,
+                              double diff)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    for (int i = 0; i < h->nm; i++) {
+        double dm2 = (double)h->idx2m[i] * (double)h->idx2m[i];
+        for (int k = 0; k < N; k++) {
+            const double *d1 = h->fdr + 3 * k, *d2 = h->fddr + 3 * k;
+            cplx fm = TG(g, i, k - 1), f0 = TG(g, i, k), fp = TG(g, i, k + 1);
+            cplx df = d1[0] * fm + d1[1] * f0 + d1[2] * fp, ddf = d2[0] * fm + d2[1] * f0 + d2[2] * fp;
+            size_t q = (size_t)i * N + k;
+            if (dfld) dfld[q] = df;
+            if (istage == 1) a->old[q] = f0;
+            if (l_calc_lin) a->impl[(istage - 1) * F + q] = diff * (ddf + h->or1[k] * df - dm2 * h->or2[k] * f0);
+        }
+    }
+}
+
+/* get_psi_rhs_imp_ghost (update_psi_fd.f90:247-395): old and implicit terms,
+ * and u_s, u_phi, omega from psi (m /= 0) and u_phi (m = 0) */
+static void fd_rhs_imp_psi(or_handle *h, tarray *a, int istage, int l_calc_lin)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    const cplx *u = h->u0g;
+    double nu = h->ViscFac, cf = h->CorFac;
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        double dm = (double)m, dm2 = dm * dm;
+        for (int k = 0; k < N; k++) {
+            size_t q = (size_t)i * N + k;
+            const double *d1 = h->fdr + 3 * k, *d2 = h->fddr + 3 * k;
+            double o1 = h->or1[k], o2 = h->or2[k], o3 = h->or3[k], be = h->beta[k], db = h->dbeta[k];
+            if (m == 0) {
+                cplx du = d1[0] * u[k] + d1[1] * u[k + 1] + d1[2] * u[k + 2];
+                cplx ddu = d2[0] * u[k] + d2[1] * u[k + 1] + d2[2] * u[k + 2];
+                if (istage == 1) a->old[q] = u[k + 1];
+                if (l_calc_lin)
+                    a->impl[(istage - 1) * F + q] = nu * (ddu + o1 * du - o2 * u[k + 1]) - cf * h->ekpump[k] * u[k + 1];
+                h->us[q] = 0.0;
+                h->up[q] = u[k + 1];
+                h->om[q] = du + o1 * u[k + 1];
+                continue;
+            }
+            const double *c3 = h->fdddr + 5 * k, *c4 = h->fddddr + 5 * k;
+            cplx f[5];
+            for (int j = 0; j < 5; j++) f[j] = PG(h->pg, i, k - 2 + j);
+            cplx dp = d1[0] * f[1] + d1[1] * f[2] + d1[2] * f[3];
+            cplx ddp = d2[0] * f[1] + d2[1] * f[2] + d2[2] * f[3];
+            cplx dddp = c3[0] * f[0] + c3[1] * f[1] + c3[2] * f[2] + c3[3] * f[3] + c3[4] * f[4];
+            cplx d4p = c4[0] * f[0] + c4[1] * f[1] + c4[2] * f[2] + c4[3] * f[3] + c4[4] * f[4];
+            cplx ps = f[2];
+            if (istage == 1) a->old[q] = -(ddp + (o1 + be) * dp + (o1 * be + db - dm2 * o2) * ps);
+            if (l_calc_lin) {
+                double d2b = h->d2beta[k], d3b = h->d3beta[k];
+                cplx v = -nu * (d4p + (be + 2.0 * o1) * dddp + (3.0 * db + 2.0 * be * o1 - o2 - 2.0 * dm2 * o2) * ddp +
+                                (3.0 * d2b + 4.0 * db * o1 - be * o2 + o3 + dm2 * o2 * (2.0 * o1 - be)) * dp +
+                                (d3b + 2.0 * d2b * o1 - db * o2 + be * o3 -
+                                 dm2 * o2 * (4.0 * o2 + be * o1 + db - dm2 * o2)) * ps);
+                if (h->c.l_coriolis_imp) v = v + cf * be * I * dm * o1 * ps;
+                if (h->c.l_ek_pump)
+                    v = v + cf * h->ekpump[k] *
+                                (ddp + (o1 + be + h->ekp_up[k]) * dp +
+                                 (db + be * o1 - dm2 * o2 + be * h->ekp_up[k] -
+                                  I * dm * o1 * (h->ekp_us[k] + I * dm * h->ekp_dusdp[k])) * ps);
+                a->impl[(istage - 1) * F + q] = v;
+            }
+            h->us[q] = I * dm * o1 * ps;
+            h->om[q] = -(ddp + (o1 + be) * dp + (be * o1 + db - dm2 * o2) * ps);
+            h->up[q] = -dp - be * ps;
+            h->psi[q] = ps;
+        }
+    }
+}
+
+/* mloop_Rdist (mloop_fd.f90:148-239) for one stage */
+static void fd_mloop(or_handle *h)
+{
+    tscheme *t = &h->ts;
+    int N = h->N, nm = h->nm;
+    size_t F = (size_t)nm * N;
+    cplx *rhs = malloc(sizeof(cplx) * F);
+    /* prepare_temp_fd / prepare_xi_fd: right-hand sides, boundary rows */
+    for (int f = 0; f < 2; f++) {
+        if ((f == 0 && !h->l_heat) || (f == 1 && !h->l_chem)) continue;
+        cplx *g = f == 0 ? h->tg : h->xg;
+        int ktop = f == 0 ? h->c.ktopt : h->c.ktopxi, kbot = f == 0 ? h->c.kbott : h->c.kbotxi;
+        ts_set_imex_rhs(h, t, f == 0 ? &h->dTdt : &h->dxidt, rhs);
+        for (int i = 0; i < nm; i++) {
+            for (int k = 0; k < N; k++) TG(g, i, k) = rhs[(size_t)i * N + k];
+            if (ktop == 1) TG(g, i, 0) = 0.0;
+            TG(g, i, -1) = 0.0;
+            if (kbot == 1) TG(g, i, N - 1) = 0.0;
+            TG(g, i, N) = 0.0;
+        }
+    }
+    /* prepare_psi_fd */
+    ts_set_imex_rhs(h, t, &h->dpsidt, rhs);
+    for (int i = 0; i < nm; i++) {
+        int m = h->idx2m[i];
+        for (int k = 0; k < N; k++) PG(h->pg, i, k) = rhs[(size_t)i * N + k];
+        if (m == 0) {
+            for (int k = 0; k < N; k++) h->u0g[k + 1] = rhs[(size_t)i * N + k];
+            if (h->c.ktopv != 1) h->u0g[1] = 0.0;
+            h->u0g[0] = 0.0;
+            if (h->c.kbotv != 1) h->u0g[N] = 0.0;
+            h->u0g[N + 1] = 0.0;
+        } else {
+            PG(h->pg, i, 0) = 0.0; PG(h->pg, i, -1) = 0.0; PG(h->pg, i, -2) = 0.0;
+            PG(h->pg, i, N - 1) = 0.0; PG(h->pg, i, N) = 0.0; PG(h->pg, i, N + 1) = 0.0;
+        }
+    }
+    free(rhs);
+    /* solves */
+    fd_solve3(N, h->uD, h->uL, h->uU, h->u0g);
+    for (int i = 0; i < nm; i++) {
+        size_t o = (size_t)i * N;
+        if (h->l_heat) fd_solve3(N, h->tTd + o, h->tTl + o, h->tTu + o, &TG(h->tg, i, -1));
+        if (h->l_chem) fd_solve3(N, h->tXd + o, h->tXl + o, h->tXu + o, &TG(h->xg, i, -1));
+        if (h->idx2m[i] != 0)
+            fd_solve5(N, h->pD + o, h->pL1 + o, h->pL2 + o, h->pU1 + o, h->pU2 + o, &PG(h->pg, i, -2));
+    }
+    /* ghost zones, then the arrays of the next stage */
+    if (h->l_heat) fd_fill_ghosts_scalar(h, h->tg, h->c.ktopt, h->c.kbott);
+    if (h->l_chem) fd_fill_ghosts_scalar(h, h->xg, h->c.ktopxi, h->c.kbotxi);
+    fd_fill_ghosts_psi(h);
+    int inext = t->istage == t->nstages ? 1 : t->istage + 1;
+    int lcalc = t->l_imp_calc_rhs[inext - 1];
+    if (h->l_heat) {
+        ts_rotate(h, t, &h->dTdt);
+        fd_rhs_imp_scalar(h, h->tg, h->dtemp, &h->dTdt, inext, lcalc, h->TdiffFac);
+        for (int i = 0; i < nm; i++)
+            for (int k = 0; k < N; k++) h->temp[(size_t)i * N + k] = TG(h->tg, i, k);
+    }
+    if (h->l_chem) {
+        ts_rotate(h, t, &h->dxidt);
+        fd_rhs_imp_scalar(h, h->xg, h->dxi, &h->dxidt, inext, lcalc, h->XidiffFac);
+        for (int i = 0; i < nm; i++)
+            for (int k = 0; k < N; k++) h->xi[(size_t)i * N + k] = TG(h->xg, i, k);
+    }
+    ts_rotate(h, t, &h->dpsidt);
+    fd_rhs_imp_psi(h, &h->dpsidt, inext, lcalc);
+}
+
+/* ghosted copies of the fields and the first implicit state: the FD branch
+ * of get_start_fields (init_fields.f90:129-186), also the start-up
+ * recomputation of start_from_another_scheme (step_time.f90:426-463) */
+static void fd_from_fields(or_handle *h, int with_om)
+{
+    int N = h->N, nm = h->nm;
+    if (h->l_heat) {
+        for (int i = 0; i < nm; i++)
+            for (int k = 0; k < N; k++) TG(h->tg, i, k) = h->temp[(size_t)i * N + k];
+        fd_fill_ghosts_scalar(h, h->tg, h->c.ktopt, h->c.kbott);
+        fd_rhs_imp_scalar(h, h->tg, h->dtemp, &h->dTdt, 1, 1, h->TdiffFac);
+    }
+    if (h->l_chem) {
+        for (int i = 0; i < nm; i++)
+            for (int k = 0; k < N; k++) TG(h->xg, i, k) = h->xi[(size_t)i * N + k];
+        fd_fill_ghosts_scalar(h, h->xg, h->c.ktopxi, h->c.kbotxi);
+        fd_rhs_imp_scalar(h, h->xg, h->dxi, &h->dxidt, 1, 1, h->XidiffFac);
+    }
+    /* psi is unknown: rebuild it from u_s */
+    for (int i = 0; i < nm; i++) {
+        int m = h->idx2m[i];
+        for (int k = 0; k < N; k++) {
+            size_t q = (size_t)i * N + k;
+            PG(h->pg, i, k) = m == 0 ? 0.0 : h->us[q] / (I * (double)m) * h->r[k];
+        }
+    }
+    int i0 = -1;
+    for (int i = 0; i < nm; i++)
+        if (h->idx2m[i] == 0) i0 = i;
+    for (int k = 0; k < N; k++) h->u0g[k + 1] = i0 >= 0 ? h->up[(size_t)i0 * N + k] : 0.0;
+    fd_fill_ghosts_psi(h);
+    fd_rhs_imp_psi(h, &h->dpsidt, 1, 1);
+    if (with_om) {
+        /* omega from u_phi and u_s (get_dr_Rloc of u_phi) */
+        cplx *w = malloc(sizeof(cplx) * N);
+        for (int i = 0; i < nm; i++) {
+            int m = h->idx2m[i];
+            size_t o = (size_t)i * N;
+            fd_dr_col(h, h->up + o, w);
+            for (int k = 0; k < N; k++)
+                h->om[o + k] = w[k] + h->or1[k] * h->up[o + k] - h->or1[k] * I * (double)m * h->us[o + k];
+        }
+        free(w);
+    }
+}
+
+/* rInt_R, finite-difference branch (integration.f90:60-96) */
+static double rint_fd(or_handle *h, const double *f)
+{
+    int N = h->N;
+    const double *r = h->r;
+    double s = 0.0;
+#define SIMPS(n)                                                                                     \
+    do {                                                                                             \
+        double h2 = r[(n) + 1] - r[(n)], h1 = r[(n)] - r[(n) - 1];                                   \
+        s += (h1 + h2) / 6.0 * (f[(n) - 1] * (2.0 * h1 - h2) / h1 + f[(n)] * (h1 + h2) * (h1 + h2) / (h1 * h2) + \
+                                f[(n) + 1] * (2.0 * h2 - h1) / h2);                                  \
+    } while (0)
+    if (N % 2 == 1) {
+        for (int n = 1; n < N - 1; n += 2) SIMPS(n);
+        return -s;
+    }
+    s = 0.5 * (r[1] - r[0]) * (f[1] + f[0]);
+    for (int n = 2; n < N - 1; n += 2) SIMPS(n);
+    s += 0.5 * (r[N - 1] - r[N - 2]) * (f[N - 1] + f[N - 2]);
+    for (int n = 1; n < N - 1; n += 2) SIMPS(n);
+#undef SIMPS
+    return -0.5 * s;
+}
+#undef TG
+#undef PG
+
 static int one_step(or_handle *h)
 {
     tscheme *t = &h->ts;
@@ -2650,7 +3257,10 @@ static int one_step(or_handle *h)
     t->istage = 1;
     for (int n_stage = 1; n_stage <= t->nstages; n_stage++) {
         int is = t->istage - 1;
-        if (t->l_exp_calc[n_stage - 1] && h->coll) {
+        if (t->l_exp_calc[n_stage - 1] && h->fd) {
+            radial_loop(h, is);
+            fd_finish_exp(h, is);
+        } else if (t->l_exp_calc[n_stage - 1] && h->coll) {
             radial_loop(h, is);
             coll_finish_exp(h, is);
         } else if (t->l_exp_calc[n_stage - 1]) {
@@ -2675,7 +3285,9 @@ static int one_step(or_handle *h)
          * multistep scheme other than CNAB2 recompute the implicit terms and
          * take the CNAB2 weights */
         if (t->family == 0 && t->scheme != PZ_CNAB2 && h->n_time_step + 1 <= t->nold - 1) {
-            if (h->coll) {
+            if (h->fd) {
+                fd_from_fields(h, 0);
+            } else if (h->coll) {
                 if (h->l_heat) coll_rhs_imp_scalar(h, h->temp, h->dtemp, &h->dTdt, 1, 1, 0, h->TdiffFac, h->hdifT);
                 if (h->l_chem) coll_rhs_imp_scalar(h, h->xi, h->dxi, &h->dxidt, 1, 1, 0, h->XidiffFac, h->hdifXi);
                 coll_rhs_imp_psi(h, &h->dpsidt, 1, 1, 0);
@@ -2688,10 +3300,14 @@ static int one_step(or_handle *h)
             ts_bridge(t, h->c.alpha);
         }
         if (lMat) {
-            int info = h->coll ? coll_build(h) : build_matrices(h, 1);
+            int info = 0;
+            if (h->fd) fd_build(h);
+            else info = h->coll ? coll_build(h) : build_matrices(h, 1);
             if (info) return info;
         }
-        if (h->coll) {
+        if (h->fd) {
+            fd_mloop(h);
+        } else if (h->coll) {
             if (h->l_heat) coll_update_scalar(h, 1);
             if (h->l_chem) coll_update_scalar(h, 0);
             coll_update_psi(h);
@@ -2882,6 +3498,12 @@ int or_start(or_handle *h)
     ts_set_weights(t, h->c.alpha, &dummy);
     /* dummy Courant limits before the first radial loop (step_time.f90:124-126) */
     for (int n = 0; n < N; n++) { h->dtr[n] = 1.0e10; h->dth[n] = 1.0e10; }
+    if (h->fd) {
+        fd_from_fields(h, 1);
+        h->started = 1;
+        h->lMatNext = 1;
+        return 0;
+    }
     for (int i = 0; i < h->nm; i++) {
         if (h->l_heat) get_dr_vec(h, h->temp + (size_t)i * N, h->dtemp + (size_t)i * N, 1);
         if (h->l_chem) get_dr_vec(h, h->xi + (size_t)i * N, h->dxi + (size_t)i * N, 1);
@@ -2937,6 +3559,7 @@ int or_time(or_handle *h, double *time, double *dt)
 /* rInt_R, Chebyshev branch (integration.f90:18-58) */
 static double rint_r(or_handle *h, const double *f)
 {
+    if (h->fd) return rint_fd(h, f);
     int N = h->N;
     double *f2 = dalloc(N);
     for (int n = 0; n < N; n++) f2[n] = f[n] / h->drx[n];

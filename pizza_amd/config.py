@@ -72,6 +72,8 @@ class PzConfig(ctypes.Structure):
         ("t_bot", ctypes.c_double * (3 * N_T_BOUNDS)),
         ("xi_top", ctypes.c_double * (3 * N_T_BOUNDS)),
         ("xi_bot", ctypes.c_double * (3 * N_T_BOUNDS)),
+        ("fd_stretch", ctypes.c_double), ("fd_ratio", ctypes.c_double),
+        ("l_finite_diff", ctypes.c_int), ("fd_pad", ctypes.c_int),
     ]
 
 
@@ -99,6 +101,8 @@ def default_config(**kw):
     c.l_ek_pump = 0
     c.l_temp_advz = 0
     c.tcond_fac = c.xicond_fac = 1.0
+    c.fd_stretch, c.fd_ratio = 0.3, 0.1   # namelists.f90:544-545
+    c.l_finite_diff = 0
     for k, v in kw.items():
         set_option(c, k, v)
     return c
@@ -115,6 +119,9 @@ def set_option(c, key, value):
     k = key.lower()
     if k == "time_scheme":
         c.time_scheme = scheme_id(value)
+    elif k == "radial_scheme":
+        # namelists.f90:369-376: anything but 'CHEB' selects finite differences
+        c.l_finite_diff = 0 if "CHEB" in str(value).upper() else 1
     elif k == "cheb_method":
         c.l_cheb_coll = 1 if "COLL" in str(value).upper() else 0
     elif k == "matrix_solve":

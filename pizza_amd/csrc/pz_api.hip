@@ -14,6 +14,7 @@
 #include "pz_host_ops.h"
 #include "pz_mphase.h"
 #include "pz_coll.h"
+#include "pz_fd.h"
 
 using pzh::ld;
 
@@ -186,7 +187,7 @@ struct pz_handle {
     hipGraphExec_t gexec = nullptr;
     bool started = false;
     bool use_graph = true;
-    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0;
+    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0;
     int inv_waves = 16;
     int mthreads = PZ_MPHASE_THREADS; // threads of k_mphase / k_mcoll
     PzDev *ddev = nullptr;
@@ -247,6 +248,9 @@ extern "C" void pz_config_default(pz_config *c)
     c->ktopt = c->kbott = c->ktopxi = c->kbotxi = 1;
     c->ktopv = c->kbotv = 2;
     c->tcond_fac = c->xicond_fac = 1.0;
+    c->fd_stretch = 0.3; // namelists.f90:544-545
+    c->fd_ratio = 0.1;
+    c->l_finite_diff = 0;
 }
 
 extern "C" const char *pz_last_error(const pz_handle *h) { return h ? h->err.c_str() : "null handle"; }
@@ -422,6 +426,103 @@ static std::vector<int> factor_radices(int n)
     return f;
 }
 
+// Fornberg's finite-difference weights (Math. Comp. 51 (1988) 699-706), as
+// in populate_fd_weights (finite_differences.f90:501-546): c[j * (m + 1) + k]
+// is the weight of x[j] in the k-th derivative at z.
+static void fd_weights(double z, const double *x, int nd, int m, double *c)
+{
+    const int w = m + 1;
+    double c1 = 1.0, c4 = x[0] - z;
+    for (int i = 0; i < (nd + 1) * w; i++) c[i] = 0.0;
+    c[0] = 1.0;
+    for (int i = 1; i <= nd; i++) {
+        const int mn = std::min(i, m);
+        double c2 = 1.0, c5 = c4;
+        c4 = x[i] - z;
+        for (int j = 0; j < i; j++) {
+            const double c3 = x[i] - x[j];
+            c2 *= c3;
+            if (j == i - 1) {
+                for (int k = mn; k >= 1; k--)
+                    c[i * w + k] = c1 * ((double)k * c[(i - 1) * w + k - 1] - c5 * c[(i - 1) * w + k]) / c2;
+                c[i * w] = -c1 * c5 * c[(i - 1) * w] / c2;
+            }
+            for (int k = mn; k >= 1; k--) c[j * w + k] = (c4 * c[j * w + k] - (double)k * c[j * w + k - 1]) / c3;
+            c[j * w] = c4 * c[j * w] / c3;
+        }
+        c1 = c2;
+    }
+}
+
+// get_FD_grid (finite_differences.f90:87-188) for an annulus, r[0] = r_cmb
+static std::vector<double> fd_grid(int N, double rcmb, double ricb, double ratio1, double ratio2)
+{
+    std::vector<double> r(N);
+    r[0] = rcmb;
+    double drb;
+    if (ratio2 == 1.0) {
+        drb = 1.0 / ((double)N - 1.0);
+        for (int k = 1; k < N; k++) r[k] = r[k - 1] - drb;
+    } else {
+        const int nbp = (int)((double)(N - 1) / (2.0 * (1.0 + ratio1)));
+        const int nbulk = N - 1 - 2 * nbp;
+        if (nbp < 1) throw std::invalid_argument("fd_stretch leaves no boundary points");
+        const double dra = std::exp(std::log(ratio2) / (double)nbp);
+        drb = 1.0;
+        for (int i = 0; i < nbp; i++) drb *= dra;
+        drb = 1.0 / ((double)nbulk + 2.0 * dra * ((1.0 - drb) / (1.0 - dra)));
+        for (int i = 0; i < nbp; i++) drb *= dra;
+        for (int k = 1; k <= nbp; k++) { r[k] = r[k - 1] - drb; drb /= dra; }
+        for (int k = 1; k <= nbulk; k++) r[k + nbp] = r[k + nbp - 1] - drb;
+        for (int k = 1; k <= nbp; k++) { drb *= dra; r[k + nbp + nbulk] = r[k + nbp + nbulk - 1] - drb; }
+    }
+    if (r[N - 1] != ricb) {
+        if (std::fabs(r[N - 1] - ricb) > drb) throw std::invalid_argument("wrong internal radius of the FD grid");
+        r[N - 1] = ricb;
+    }
+    return r;
+}
+
+// get_FD_coeffs, orders 2 and 2 (finite_differences.f90:190-446): the
+// stencils of the step, laid out as PzDev::fdc
+static std::vector<double> fd_coeffs(const std::vector<double> &r)
+{
+    const int N = (int)r.size();
+    std::vector<double> c(16 * (size_t)N + 6, 0.0);
+    double x[5], w[25];
+    for (int k = 0; k < N; k++) {
+        double *d1 = &c[3 * (size_t)k], *d2 = &c[3 * (size_t)N + 3 * k];
+        if (k == 0 || k == N - 1) {
+            const double drl = k == 0 ? r[1] - r[0] : r[N - 1] - r[N - 2];
+            d1[0] = -0.5 / drl; d1[1] = 0.0; d1[2] = 0.5 / drl;
+            d2[0] = 1.0 / drl / drl; d2[1] = -2.0 / drl / drl; d2[2] = 1.0 / drl / drl;
+        } else {
+            for (int od = 0; od < 3; od++) x[od] = r[k - 1 + od] - r[k];
+            fd_weights(0.0, x, 2, 2, w);
+            for (int od = 0; od < 3; od++) { d1[od] = w[od * 3 + 1]; d2[od] = w[od * 3 + 2]; }
+        }
+    }
+    for (int k = 1; k < N - 1; k++) {
+        for (int od = 0; od < 5; od++) {
+            if (k == 1 && od == 0) x[od] = 2.0 * (r[0] - r[1]);
+            else if (k == N - 2 && od == 4) x[od] = 2.0 * (r[N - 1] - r[N - 2]);
+            else x[od] = r[k - 2 + od] - r[k];
+        }
+        fd_weights(0.0, x, 4, 4, w);
+        for (int od = 0; od < 5; od++) {
+            c[6 * (size_t)N + 5 * k + od] = w[od * 5 + 3];
+            c[11 * (size_t)N + 5 * k + od] = w[od * 5 + 4];
+        }
+    }
+    for (int od = 0; od < 3; od++) x[od] = r[od] - r[0];
+    fd_weights(0.0, x, 2, 2, w);
+    for (int od = 0; od < 3; od++) c[16 * (size_t)N + od] = w[od * 3 + 1];
+    for (int od = 0; od < 3; od++) x[od] = r[N - 1 - od] - r[N - 1];
+    fd_weights(0.0, x, 2, 2, w);
+    for (int od = 0; od < 3; od++) c[16 * (size_t)N + 3 + od] = w[od * 3 + 1];
+    return c;
+}
+
 // Lets every kernel with dynamic LDS use the whole per-workgroup budget, once
 // per process: a per-run value would shrink the limit for a larger run
 // created earlier (runs of different sizes share the kernels).
@@ -433,7 +534,8 @@ static void allow_full_lds(int lim)
                              (const void *)k_start, (const void *)k_build_inv, (const void *)k_mcoll,
                              (const void *)k_start_coll, (const void *)k_build_lhs_batch,
                              (const void *)k_mphase_batch, (const void *)k_radial_loop_batch,
-                             (const void *)k_build_inv_batch, (const void *)k_mcoll_batch};
+                             (const void *)k_build_inv_batch, (const void *)k_mcoll_batch, (const void *)k_fd_build,
+                             (const void *)k_fd_mloop, (const void *)k_fd_start};
     for (const void *k : kernels) {
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, k));
@@ -524,6 +626,10 @@ static void setup(pz_handle *h)
         x[k] = std::cos(pi * (double)k / (double)(N - 1));
         r[k] = bma * x[k] + bpa;
         drx[k] = 2.0 / (h->r_cmb - h->r_icb);
+    }
+    d.fd = c.l_finite_diff ? 1 : 0;
+    if (d.fd) r = fd_grid(N, h->r_cmb, h->r_icb, c.fd_stretch, c.fd_ratio);
+    for (int k = 0; k < N; k++) {
         or1[k] = 1.0 / r[k];
         or2[k] = or1[k] * or1[k];
         rgrav[k] = c.g0 + c.g1 * r[k] / rc + c.g2 * rc * rc * or2[k];
@@ -540,11 +646,14 @@ static void setup(pz_handle *h)
         beta[n] = -r[n] / s2;
         ekpump[n] = 0.5 * ek_pump_fac * std::sqrt(c.ek * rc) / std::pow(s2, 0.75);
     }
-    std::vector<double> dbeta(N, 0.0); // radial.f90:222
+    std::vector<double> dbeta(N, 0.0), d2beta(N, 0.0), d3beta(N, 0.0), or3(N); // radial.f90:212-226
     for (int n = 1; n < N; n++) {
         double s2 = rc * rc - r[n] * r[n];
         dbeta[n] = -(rc * rc + r[n] * r[n]) / (s2 * s2);
+        d2beta[n] = -128.0 * r[n] * (3.0 * rc * rc + r[n] * r[n]) / std::pow(height[n], 6);
+        d3beta[n] = -1536.0 * (std::pow(rc, 4) + std::pow(r[n], 4) + 6.0 * r[n] * r[n] * rc * rc) / std::pow(height[n], 8);
     }
+    for (int n = 0; n < N; n++) or3[n] = or2[n] * or1[n];
     for (int n = 0; n < N; n++) {
         ekp_up[n] = -0.5 * beta[n];
         ekp_us[n] = -5.0 * rc * beta[n] * oheight[n];
@@ -599,6 +708,9 @@ static void setup(pz_handle *h)
         for (int i = 0; i < 18; i++) std::copy(rows[i]->begin(), rows[i]->end(), rc.begin() + (size_t)i * N);
         std::copy(drb.begin(), drb.end(), rc.begin() + (size_t)18 * N);
         std::copy(dbeta.begin(), dbeta.end(), rc.begin() + (size_t)21 * N);
+        std::copy(or3.begin(), or3.end(), rc.begin() + (size_t)22 * N);
+        std::copy(d2beta.begin(), d2beta.end(), rc.begin() + (size_t)23 * N);
+        std::copy(d3beta.begin(), d3beta.end(), rc.begin() + (size_t)24 * N);
         d.radc = h->upload(rc);
         set_radial_pointers(d, d.radc, (size_t)N);
     }
@@ -670,7 +782,14 @@ static void setup(pz_handle *h)
     d.T_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
     d.X_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
     d.P_inv = h->dalloc<cd>((size_t)nm * N * d.ldi);
-    d.coll = c.l_cheb_coll ? 1 : 0;
+    d.coll = (c.l_cheb_coll && !d.fd) ? 1 : 0;
+    if (d.fd) {
+        d.fdc = h->upload(fd_coeffs(h->r));
+        d.fT = h->dalloc<double>((size_t)nm * 3 * N);
+        d.fX = h->dalloc<double>((size_t)nm * 3 * N);
+        d.fP = h->dalloc<cd>((size_t)nm * 5 * N);
+        d.fU = h->dalloc<double>((size_t)3 * N);
+    }
     if (d.coll) {
         // get_der_mat, collocation branch (chebyshev.f90:296-321)
         std::vector<double> cm((size_t)N * N), cdm((size_t)N * N), cd2m((size_t)N * N);
@@ -696,11 +815,12 @@ static void setup(pz_handle *h)
         d.coll_work = h->dalloc<char>((size_t)nm * d.coll_ws);
     }
     d.ss = h->dalloc<PzScheme>(1);
-    if (!d.coll) build_operators(h);
+    if (!d.coll && !d.fd) build_operators(h);
     // LDS budgets
     h->lds_radial = (5 * (size_t)h->nm + 6 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
     h->mthreads = mloop_threads(N);
     h->lds_mphase = MpLayout(N, h->mthreads).total;
+    h->lds_fd = FdLayout(N).total;
     h->lds_start = 15 * (size_t)N * sizeof(cd);
     h->lds_start_coll = (13 * (size_t)N + 6 * (size_t)(2 * N - 2) + 3 * (size_t)N + 16) * sizeof(cd) +
                         3 * (size_t)N * sizeof(double);
@@ -719,7 +839,7 @@ static void setup(pz_handle *h)
     while (h->inv_waves > 1 && InvLayout(N, h->inv_waves).total > lim) h->inv_waves--;
     h->lds_inv = InvLayout(N, h->inv_waves).total;
     size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv,
-                            h->lds_start_coll});
+                            h->lds_start_coll, d.fd ? h->lds_fd : (size_t)0});
     if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
     allow_full_lds((int)lim);
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -992,7 +1112,9 @@ extern "C" int pz_start(pz_handle *h)
                         (size_t)sc.nexp, (size_t)sc.nold, (size_t)sc.nimp, (size_t)sc.nexp};
         for (int i = 0; i < 9; i++) HIPCHK(hipMemsetAsync(slots[i], 0, cnt[i] * F * sizeof(cd), h->stream));
         hipLaunchKernelGGL(k_set_weights, dim3(1), dim3(1), 0, h->stream, h->d);
-        if (h->d.coll)
+        if (h->d.fd)
+            hipLaunchKernelGGL(k_fd_start, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev);
+        else if (h->d.coll)
             hipLaunchKernelGGL(k_start_coll, dim3(h->nm), dim3(256), h->lds_start_coll, h->stream, h->d);
         else
             hipLaunchKernelGGL(k_start, dim3(h->nm), dim3(256), h->lds_start, h->stream, h->d);
@@ -1018,7 +1140,9 @@ static void launch_step(pz_handle *h)
                                stage == 1 ? 1 : 0);
         if (stage == 1) {
             hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev);
-            if (h->d.coll) {
+            if (h->d.fd) {
+                hipLaunchKernelGGL(k_fd_build, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 0);
+            } else if (h->d.coll) {
                 hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), 0, h->stream, (const PzDev *)h->ddev, 0);
             } else {
                 hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, (const PzDev *)h->ddev, 0);
@@ -1026,7 +1150,10 @@ static void launch_step(pz_handle *h)
                                    (const PzDev *)h->ddev, 0);
             }
         }
-        if (h->d.coll)
+        if (h->d.fd)
+            hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev,
+                               stage, exp_calc);
+        else if (h->d.coll)
             hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                (const PzDev *)h->ddev, stage, exp_calc);
         else
@@ -1085,9 +1212,33 @@ extern "C" int pz_time(pz_handle *h, double *time, double *dt)
     }
 }
 
+// rInt_R, finite-difference branch (integration.f90:60-96): Simpson's rule
+// on the non-uniform grid, trapezoids at both ends for an even n_r
+static double rint_fd(const pz_handle *h, const std::vector<double> &f)
+{
+    const int N = h->N;
+    const std::vector<double> &r = h->r;
+    auto simps = [&](int n) {
+        const double h2 = r[n + 1] - r[n], h1 = r[n] - r[n - 1];
+        return (h1 + h2) / 6.0 *
+               (f[n - 1] * (2.0 * h1 - h2) / h1 + f[n] * (h1 + h2) * (h1 + h2) / (h1 * h2) + f[n + 1] * (2.0 * h2 - h1) / h2);
+    };
+    double s = 0.0;
+    if (N % 2 == 1) {
+        for (int n = 1; n < N - 1; n += 2) s += simps(n);
+        return -s;
+    }
+    s = 0.5 * (r[1] - r[0]) * (f[1] + f[0]);
+    for (int n = 2; n < N - 1; n += 2) s += simps(n);
+    s += 0.5 * (r[N - 1] - r[N - 2]) * (f[N - 1] + f[N - 2]);
+    for (int n = 1; n < N - 1; n += 2) s += simps(n);
+    return -0.5 * s;
+}
+
 // rInt_R, Chebyshev branch (integration.f90:18-58)
 static double rint_r(const pz_handle *h, const std::vector<double> &f)
 {
+    if (h->d.fd) return rint_fd(h, f);
     const int N = h->N;
     std::vector<double> f2(N), c(N);
     for (int n = 0; n < N; n++) f2[n] = f[n] / h->drx[n];
@@ -1247,7 +1398,10 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, h->stream));
         for (int i = 0; i < reps; i++) {
-            if (which == 0 && h->d.coll)
+            if (which == 0 && h->d.fd)
+                hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 1,
+                                   1);
+            else if (which == 0 && h->d.coll)
                 hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else if (which == 0)
@@ -1358,6 +1512,7 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
             pz_handle *h = runs[r];
             if (!h) throw std::invalid_argument("null run handle");
             if (!h->started) throw std::invalid_argument("every run must be started (pz_start) first");
+            if (h->d.fd) throw std::invalid_argument("finite-difference runs are not batched into ensembles");
             if (h->sch.id != h0->sch.id || h->d.coll != h0->d.coll)
                 throw std::invalid_argument("the runs of an ensemble must share the time scheme and the radial method");
             e->runs.push_back(h);

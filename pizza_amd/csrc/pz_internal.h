@@ -65,6 +65,14 @@ struct PzDev {
     const double *drb;         // [3][N] bulk factors of the radial derivatives
     const double *dbeta;       // d beta / dr (collocation psi operator)
     const double *radc;        // block holding all radial rows above (PZ_NRADC x n_r)
+    // finite differences (radial_scheme = 'FD'): stencils
+    // [dr 3N | ddr 3N | dddr 5N | ddddr 5N | dr_top 3 | dr_bot 3], and the
+    // factorised systems: T and xi [m][d, low, up][N] (real), psi
+    // [m][d, low1, low2, up1, up2][N] (complex), u_phi(m = 0) [d, low, up][N]
+    int fd;
+    const double *fdc;
+    double *fT, *fX, *fU;
+    cd *fP;
     const double *tau;         // tau lines: [rMat(2,N) | drMat(2,N) | d2rMat(2,N)]
     const cd *twid;            // exp(-2 pi i k / n_phi)
     const int *idx2m;
@@ -124,7 +132,7 @@ struct PzDev {
 // rows of the radial block: x, r, drx, ddrx, or1, or2, rgrav, beta, ekpump,
 // ekp_up, ekp_us, ekp_dusdp, delxr2, delxh2, tcond, dtcond, xicond, dxicond,
 // drb (3 rows), dbeta
-#define PZ_NRADC 22
+#define PZ_NRADC 25
 template <class D>
 __host__ __device__ inline void set_radial_pointers(D &d, const double *rc, size_t N)
 {
@@ -178,6 +186,10 @@ struct RadView {
     __host__ __device__ double dxicond(int k) const { return row(17, k); }
     __host__ __device__ double drb(int k) const { return row(18, k); } // k in [0, 3N)
     __host__ __device__ double dbeta(int k) const { return row(21, k); }
+    // finite differences only: 1/r^3 and the higher derivatives of beta
+    __host__ __device__ double or3(int k) const { return row(22, k); }
+    __host__ __device__ double d2beta(int k) const { return row(23, k); }
+    __host__ __device__ double d3beta(int k) const { return row(24, k); }
 };
 
 // sizes of the bordered storage per m

@@ -206,6 +206,8 @@ contains
       cfg%n_r_max = n_r_max; cfg%n_cheb_max = n_cheb_max; cfg%m_max = m_max; cfg%minc = minc
       cfg%time_scheme = scheme_id(time_scheme)
       cfg%l_cheb_coll = flag(has(cheb_method, 'COLL'))
+      cfg%l_finite_diff = flag(.not. has(radial_scheme, 'CHEB'))
+      cfg%fd_stretch = fd_stretch; cfg%fd_ratio = fd_ratio
       cfg%l_direct_solve = flag(has(matrix_solve, 'DIRECT'))
       cfg%l_coriolis_imp = flag(has(corio_term, 'IMP'))
       cfg%l_buo_imp = flag(has(buo_term, 'IMP'))
@@ -232,7 +234,6 @@ contains
       if (h_temp /= 0.0_cp .or. h_xi /= 0.0_cp) call unsupported('volumetric heating h_temp / h_xi')
       if (l_var_m) call unsupported('l_var_m')
       if (l_newmap) call unsupported('l_newmap')
-      if (.not. has(radial_scheme, 'CHEB')) call unsupported('radial_scheme '//trim(radial_scheme))
       if (has(bc_method, 'GALERKIN')) call unsupported('bc_method galerkin')
       if (n_rings /= 0 .or. amp_forcing /= 0.0_cp) call unsupported('forcing')
       if (init_phi /= 0 .or. stef /= 0.0_cp) call unsupported('phase field')

@@ -30,6 +30,8 @@ module pizza_amd_gpu
       real(c_double) :: tcond_fac, xicond_fac
       real(c_double) :: t_top(3*PZ_N_T_BOUNDS), t_bot(3*PZ_N_T_BOUNDS)
       real(c_double) :: xi_top(3*PZ_N_T_BOUNDS), xi_bot(3*PZ_N_T_BOUNDS)
+      real(c_double) :: fd_stretch, fd_ratio
+      integer(c_int) :: l_finite_diff, fd_pad
    end type pz_config
 
    interface

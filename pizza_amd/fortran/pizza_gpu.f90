@@ -126,6 +126,10 @@ contains
       write(*, '(a, i0)') 'l_temp_advz ', c%l_temp_advz
       write(*, '(a, es24.16)') 'tcond_fac ', c%tcond_fac
       write(*, '(a, es24.16)') 'xicond_fac ', c%xicond_fac
+      write(*, '(a, es24.16)') 'fd_stretch ', c%fd_stretch
+      write(*, '(a, es24.16)') 'fd_ratio ', c%fd_ratio
+      write(*, '(a, i0)') 'l_finite_diff ', c%l_finite_diff
+      write(*, '(a, i0)') 'fd_pad ', c%fd_pad
       write(*, '(a, i0)') 'init_t ', init_t
       write(*, '(a, es24.16)') 'amp_t ', amp_t
       write(*, '(a, i0)') 'init_xi ', init_xi

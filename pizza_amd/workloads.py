@@ -61,11 +61,42 @@ def seeded_field(n_m, r, r_icb, amp, seed=SEED, stream=0):
     return amp * g[:, None] * prof[None, :]
 
 
+def fd_grid(n, r_cmb, r_icb, ratio1, ratio2):
+    """Stretched finite-difference radii (finite_differences.f90:87-188),
+    outer boundary first."""
+    r = np.zeros(n)
+    r[0] = r_cmb
+    if ratio2 == 1.0:
+        drb = 1.0 / (n - 1.0)
+        for k in range(1, n):
+            r[k] = r[k - 1] - drb
+    else:
+        nbp = int((n - 1) / (2.0 * (1.0 + ratio1)))
+        nbulk = n - 1 - 2 * nbp
+        dra = np.exp(np.log(ratio2) / nbp)
+        drb = dra ** nbp
+        drb = 1.0 / (nbulk + 2.0 * dra * ((1.0 - drb) / (1.0 - dra)))
+        drb *= dra ** nbp
+        for k in range(1, nbp + 1):
+            r[k] = r[k - 1] - drb
+            drb /= dra
+        for k in range(1, nbulk + 1):
+            r[k + nbp] = r[k + nbp - 1] - drb
+        for k in range(1, nbp + 1):
+            drb *= dra
+            r[k + nbp + nbulk] = r[k + nbp + nbulk - 1] - drb
+    r[n - 1] = r_icb
+    return r
+
+
 def radial_grid(cfg):
-    """Gauss-Lobatto radii r_k (chebyshev.f90:188-240), outer boundary first."""
+    """Radii r_k, outer boundary first: Gauss-Lobatto points
+    (chebyshev.f90:188-240) or the finite-difference grid."""
     n = cfg.n_r_max
     r_cmb = 1.0 / (1.0 - cfg.radratio)
     r_icb = r_cmb - 1.0
+    if getattr(cfg, "l_finite_diff", 0):
+        return fd_grid(n, r_cmb, r_icb, cfg.fd_stretch, cfg.fd_ratio), r_icb
     x = np.cos(np.pi * np.arange(n) / (n - 1))
     return 0.5 * (r_cmb - r_icb) * x + 0.5 * (r_cmb + r_icb), r_icb
 

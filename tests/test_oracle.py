@@ -229,3 +229,58 @@ def test_sparse_stencils(which, J, K, m):
         got = np.where(cols >= 0, got, 0.0)  # unfolded negative-column leftovers
         scale = max(np.max(np.abs(ref)), 1e-300)
         assert np.max(np.abs(got - ref)) / scale < 1e-12, (which, idx, got, ref)
+
+
+def _fd_cfg(n_r, fd, **kw):
+    from pizza_amd.config import default_config
+    base = dict(n_r_max=n_r, n_cheb_max=n_r, m_max=8, minc=1, time_scheme="CNAB2",
+                radial_scheme="FD" if fd else "CHEB", cheb_method="colloc", dtMax=1e-4,
+                ra=1e5, ek=1e-3, pr=1.0, radratio=0.35, buo_term="explicit")
+    base.update(kw)
+    return default_config(**base)
+
+
+def test_fd_grid_matches_stretched_grid():
+    """The oracle's FD radii equal the stretched grid restated on the host
+    side (finite_differences.f90:87-188) for stretched and uniform grids."""
+    from pizza_amd import workloads as W
+    for ratio in (0.1, 1.0):
+        cfg = _fd_cfg(129, True, fd_ratio=ratio)
+        r, _ = Oracle(cfg).grid()
+        assert np.array_equal(r, W.radial_grid(cfg)[0])
+        assert r[0] == 1.0 / 0.65 and r[-1] == 1.0 / 0.65 - 1.0
+        assert np.all(np.diff(r) < 0)
+
+
+def test_fd_converges_to_chebyshev_solution():
+    """Finite differences (second order) against the pinned Chebyshev
+    collocation method on the same physics after 10 steps: T, u_s and u_phi
+    differences shrink by about 4 per halving of the grid spacing, and omega
+    converges away from the outer wall (where beta is singular)."""
+    from numpy.polynomial import chebyshev as C
+    from pizza_amd import workloads as W
+
+    def run(cfg):
+        o = Oracle(cfg)
+        W.seed_run(o, cfg, amp_t=1e-3)
+        o.start()
+        o.step(10)
+        return o
+
+    oc = run(_fd_cfg(49, False))
+    rc, xc = oc.grid()
+    V = C.chebvander(xc, len(xc) - 1)
+    errs = {}
+    for n_r in (65, 129):
+        of = run(_fd_cfg(n_r, True))
+        rf, _ = of.grid()
+        xq = (2 * rf - rc[0] - rc[-1]) / (rc[0] - rc[-1])
+        for f in ("temp", "us", "up", "om"):
+            coef = np.linalg.solve(V, oc.get_field(f).T).T
+            ci = np.array([C.chebval(xq, c) for c in coef])
+            d = np.abs(of.get_field(f) - ci) / np.max(np.abs(ci))
+            errs[(n_r, f)] = d[:, 1:].max() if f == "om" else d.max()
+    for f in ("temp", "us", "up"):
+        assert errs[(65, f)] < 1e-2, (f, errs[(65, f)])
+        assert errs[(65, f)] / errs[(129, f)] > 3.0, (f, errs[(65, f)], errs[(129, f)])
+    assert errs[(129, "om")] < 2e-2 and errs[(129, "om")] < errs[(65, "om")]
