@@ -188,6 +188,7 @@ struct pz_handle {
     bool started = false;
     bool use_graph = true;
     size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0;
+    bool radial_big = false; // k_radial_loop_big for long azimuthal grids
     int inv_waves = 16;
     int mthreads = PZ_MPHASE_THREADS; // threads of k_mphase / k_mcoll
     PzDev *ddev = nullptr;
@@ -535,7 +536,7 @@ static void allow_full_lds(int lim)
                              (const void *)k_start_coll, (const void *)k_build_lhs_batch,
                              (const void *)k_mphase_batch, (const void *)k_radial_loop_batch,
                              (const void *)k_build_inv_batch, (const void *)k_mcoll_batch, (const void *)k_fd_build,
-                             (const void *)k_fd_mloop, (const void *)k_fd_start};
+                             (const void *)k_fd_mloop, (const void *)k_fd_start, (const void *)k_radial_loop_big};
     for (const void *k : kernels) {
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, k));
@@ -835,11 +836,18 @@ static void setup(pz_handle *h)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     size_t lim = prop.sharedMemPerBlock;
+    if (h->lds_radial > lim) {
+        h->radial_big = true;
+        h->lds_radial = 4 * (size_t)h->nphi * sizeof(cd) + 16 * sizeof(double);
+    }
     // waves of k_build_inv: as many right-hand sides as fit next to the factors
     while (h->inv_waves > 1 && InvLayout(N, h->inv_waves).total > lim) h->inv_waves--;
     h->lds_inv = InvLayout(N, h->inv_waves).total;
-    size_t need = std::max({h->lds_radial, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv,
-                            h->lds_start_coll, d.fd ? h->lds_fd : (size_t)0});
+    // the kernels of this run's radial method
+    size_t need = h->lds_radial;
+    if (d.fd) need = std::max(need, h->lds_fd);
+    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll});
+    else need = std::max({need, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
     if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
     allow_full_lds((int)lim);
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -861,7 +869,7 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
     else if (c.l_non_rot) why = "non-rotating runs are not implemented";
     else if (c.radratio <= 0.0) why = "full-disk geometry is not implemented";
     else if (c.n_r_max < 8 || c.n_r_max > 257) why = "n_r_max must be in [8, 257]";
-    else if (c.l_cheb_coll && c.n_r_max > 256) why = "the collocation method needs n_r_max <= 256";
+    else if (c.l_cheb_coll && !c.l_finite_diff && c.n_r_max > 256) why = "the collocation method needs n_r_max <= 256";
     else if (c.n_cheb_max < 1 || c.n_cheb_max > c.n_r_max) why = "n_cheb_max must be in [1, n_r_max]";
     else if (c.m_max < 3 || c.minc < 1 || c.m_max % c.minc) why = "invalid m_max/minc";
     else {
@@ -1136,7 +1144,8 @@ static void launch_step(pz_handle *h)
     for (int stage = 1; stage <= s.nstages; stage++) {
         const int exp_calc = s.l_exp_calc[stage - 1];
         if (exp_calc)
-            hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, (const PzDev *)h->ddev, stage - 1,
+            hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N), dim3(256), h->lds_radial,
+                               h->stream, (const PzDev *)h->ddev, stage - 1,
                                stage == 1 ? 1 : 0);
         if (stage == 1) {
             hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev);
@@ -1408,7 +1417,8 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
                 hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else
-                hipLaunchKernelGGL(k_radial_loop, dim3(h->N), dim3(256), h->lds_radial, h->stream, (const PzDev *)h->ddev, 0, 0);
+                hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N), dim3(256),
+                                   h->lds_radial, h->stream, (const PzDev *)h->ddev, 0, 0);
         }
         HIPCHK(hipEventRecord(e1, h->stream));
         HIPCHK(hipEventSynchronize(e1));
@@ -1513,6 +1523,7 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
             if (!h) throw std::invalid_argument("null run handle");
             if (!h->started) throw std::invalid_argument("every run must be started (pz_start) first");
             if (h->d.fd) throw std::invalid_argument("finite-difference runs are not batched into ensembles");
+            if (h->radial_big) throw std::invalid_argument("runs with long azimuthal grids are not batched into ensembles");
             if (h->sch.id != h0->sch.id || h->d.coll != h0->d.coll)
                 throw std::invalid_argument("the runs of an ensemble must share the time scheme and the radial method");
             e->runs.push_back(h);

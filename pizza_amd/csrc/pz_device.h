@@ -376,6 +376,141 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
     }
 }
 
+// The radial loop for long azimuthal grids (n_phi too large for the staged
+// variant above): the spectra are packed straight from global memory and
+// the real fields travel through the complex FFTs one pair at a time, so
+// LDS holds four columns of n_phi values (three fields and one scratch).
+__device__ __forceinline__ void radial_loop_big_body(const PzDev &p, int nr, int is, int courant)
+{
+    extern __shared__ double smem[];
+    const int n = p.nphi, nm = p.nm, N = p.N;
+    const size_t F = (size_t)nm * N;
+    cd *s0 = (cd *)smem, *s1 = s0 + n, *s2 = s1 + n, *sc = s2 + n;
+    double *red = (double *)(sc + n);
+    // Reynolds-stress m-sum (radial_loop.f90:267-278)
+    double part = 0.0;
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) {
+        const int m = p.idx2m[i];
+        const size_t q = (size_t)i * N + nr;
+        const cd usf = m == 0 ? cscale(0.5 * p.ek * p.CorFac * p.ekpump[nr], p.up[q]) : p.us[q];
+        part += cc22real(usf, p.om[q], m);
+    }
+    const double usom = block_reduce(part, red, 0);
+    // Hermitian spectra of z = a + i b for (us, up), (om, T), (xi, 0)
+    for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
+        const int tr = t / n, k = t - tr * n;
+        if (tr == 2 && !p.l_chem) break;
+        const cd *A = tr == 0 ? p.us : (tr == 1 ? p.om : p.xi);
+        const cd *B = tr == 0 ? p.up : (tr == 1 ? (p.l_heat ? p.temp : nullptr) : nullptr);
+        cd z = mk(0, 0);
+        if (k == 0) {
+            z = mk(A[nr].x, B ? B[nr].x : 0.0);
+        } else if (k < nm) {
+            const cd a = A[(size_t)k * N + nr], b = B ? B[(size_t)k * N + nr] : mk(0, 0);
+            z = mk(a.x - b.y, a.y + b.x);
+        } else if (n - k < nm) {
+            const int kk = n - k;
+            const cd a = cconj(A[(size_t)kk * N + nr]), b = B ? cconj(B[(size_t)kk * N + nr]) : mk(0, 0);
+            z = mk(a.x - b.y, a.y + b.x);
+        }
+        (tr == 0 ? s0 : (tr == 1 ? s1 : s2))[k] = z;
+    }
+    __syncthreads();
+    // one complex FFT at a time; the result lands in the column or in the
+    // scratch, whose roles are then exchanged
+    {
+        cd *g = fft_lds(p, s0, sc, n, 1, +1);
+        if (g != s0) { sc = s0; s0 = g; }
+        __syncthreads();
+        g = fft_lds(p, s1, sc, n, 1, +1);
+        if (g != s1) { sc = s1; s1 = g; }
+        if (p.l_chem) {
+            __syncthreads();
+            g = fft_lds(p, s2, sc, n, 1, +1);
+            if (g != s2) { sc = s2; s2 = g; }
+        }
+        __syncthreads();
+    }
+    if (courant) {
+        double vr = 0.0, vh = 0.0;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const cd z = s0[j];
+            vr = fmax(vr, z.x * z.x);
+            vh = fmax(vh, z.y * z.y);
+        }
+        vr = block_reduce(vr, red, 1);
+        vh = block_reduce(vh, red, 1);
+        if (threadIdx.x == 0) {
+            const double cf = p.courfac;
+            vr = cf * cf * vr;
+            vh = cf * cf * vh;
+            double dtrk = 1.0e10, dthk = 1.0e10;
+            if (vr != 0.0) dtrk = fmin(dtrk, sqrt(p.delxr2[nr] / vr));
+            if (vh != 0.0) dthk = fmin(dthk, sqrt(p.delxh2[nr] / vh));
+            p.dtr[nr] = dtrk;
+            p.dth[nr] = dthk;
+        }
+    }
+    // nonlinear products in place (radial_loop.f90:316-325): (up X + i r us X)
+    const double rr = p.r[nr];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const cd z0 = s0[j], z1 = s1[j];
+        const double us = z0.x, up = z0.y, om = z1.x, T = z1.y;
+        const double X = p.l_chem ? s2[j].x : 0.0;
+        s0[j] = mk(up * om, rr * us * om);
+        s1[j] = mk(up * T, rr * us * T);
+        if (p.l_chem) s2[j] = mk(up * X, rr * us * X);
+    }
+    __syncthreads();
+    {
+        cd *g = fft_lds(p, s0, sc, n, 1, -1);
+        if (g != s0) { sc = s0; s0 = g; }
+        if (p.l_heat) {
+            __syncthreads();
+            g = fft_lds(p, s1, sc, n, 1, -1);
+            if (g != s1) { sc = s1; s1 = g; }
+        }
+        if (p.l_chem) {
+            __syncthreads();
+            g = fft_lds(p, s2, sc, n, 1, -1);
+            if (g != s2) { sc = s2; s2 = g; }
+        }
+        __syncthreads();
+    }
+    // unpack, normalise and post-scale (radial_loop.f90:350-386)
+    const double or1 = p.or1[nr];
+    const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
+    for (int t = threadIdx.x; t < 3 * nm; t += blockDim.x) {
+        const int tr = t / nm, i = t - tr * nm;
+        if ((tr == 1 && !p.l_heat) || (tr == 2 && !p.l_chem)) continue;
+        const cd *Z = tr == 0 ? s0 : (tr == 1 ? s1 : s2);
+        const cd zk = Z[i], zc = cconj(Z[i == 0 ? 0 : n - i]);
+        cd A = mk(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+        cd B = mk(0.5 * (zk.y - zc.y), -0.5 * (zk.x - zc.x));
+        A = cdivr(A, (double)n);
+        B = cdivr(B, (double)n);
+        const int m = p.idx2m[i];
+        const size_t q = (size_t)i * N + nr;
+        const cd cim = mk(0.0, -or1 * (double)m);
+        if (tr == 0) {
+            p.P_expl[is * F + q] = (i == 0) ? mk(-usom, 0.0) : cmul(cim, A);
+            p.dVsOm[q] = B;
+        } else if (tr == 1) {
+            const cd v = cmul(cim, A);
+            p.T_expl[is * F + q] = mk(v.x - fbeta * B.x, v.y - fbeta * B.y);
+            p.dVsT[q] = B;
+        } else {
+            p.X_expl[is * F + q] = cmul(cim, A);
+            p.dVsXi[q] = B;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_radial_loop_big(const PzDev *__restrict__ pd, int is, int courant)
+{
+    radial_loop_big_body(*pd, blockIdx.x, is, courant);
+}
+
 __global__ void __launch_bounds__(256) k_radial_loop(const PzDev *__restrict__ pd, int is, int courant)
 {
     radial_loop_body(*pd, blockIdx.x, is, courant);

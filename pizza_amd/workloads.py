@@ -138,3 +138,12 @@ def seed_member(run, cfg, k, amp_t=1.0e-3, seed=SEED):
     r, r_icb = radial_grid(cfg)
     n_m = cfg.m_max // cfg.minc + 1
     run.set_field("temp", seeded_field(n_m, r, r_icb, amp_t, seed + 1000 + k, 0))
+
+
+def c5_fd(n_r=257, m_max=682, steps_scheme="SBDF3"):
+    """The large finite-difference case: n_r 257 on the stretched grid,
+    n_phi 2046 (m_max 682), Ekman 1e-6, Ra 1e9, Pr 0.1, SBDF3, dt 1e-7.
+    The reference's finite differences are second order."""
+    return default_config(
+        n_r_max=n_r, n_cheb_max=n_r, m_max=m_max, minc=1, time_scheme=steps_scheme,
+        radial_scheme="FD", dtMax=1.0e-7, ra=1.0e9, ek=1.0e-6, pr=0.1, radratio=0.35)

@@ -277,3 +277,20 @@ def test_finite_difference_parity(scheme):
             err = rel(g.get_field(f), o.get_field(f))
             assert err < max(1e-11 if i == 0 else 1e-10, 10 * floors[i][f]), (f, i, err, floors[i][f])
     np.testing.assert_allclose(g.ekin_3d(), o.ekin_3d(), rtol=1e-10, atol=1e-30)
+
+
+def test_long_azimuthal_grid_radial_loop():
+    """A grid whose radial loop does not fit the staged variant (n_phi 2046)
+    runs the one-pair-at-a-time variant: FD at n_r = 33, m_max = 682,
+    against the oracle after 2 steps."""
+    from pizza_amd import workloads as W
+    cfg = W.c5_fd(n_r=33, m_max=682)
+    print("case: FD n_r 33, m_max 682 (long azimuthal grid)", flush=True)
+    g, o = _pair(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg, amp_t=1e-3)
+        x.start()
+    g.step(2)
+    o.step(2)
+    for f in _fields(cfg):
+        assert rel(g.get_field(f), o.get_field(f)) < 1e-10, f
