@@ -42,6 +42,7 @@ def lib():
         L.or_ifft.argtypes = [vp, dp, dp, ip]
         L.or_grid.argtypes = [vp, dp, dp]
         L.or_stencil.argtypes = [vp, ip, ip, ip, ip, dp]
+        L.or_fd_stencils.argtypes = [vp, dp]
         L.or_scheme_weights.argtypes = [ip, ctypes.c_double, dp, ctypes.POINTER(ip), ctypes.POINTER(ip),
                                         dp, dp, dp, dp, dp]
         _LIB = L
@@ -122,6 +123,15 @@ class Oracle:
         out = np.zeros(4)
         lib().or_ekin_3d(self.h, _dp(out))
         return out
+
+    def fd_stencils(self):
+        """dr, ddr (N x 3), dddr, ddddr (N x 5), dr_top, dr_bot (3 each)."""
+        N = self.n_r
+        out = np.zeros(16 * N + 6)
+        if lib().or_fd_stencils(self.h, _dp(out)):
+            raise ValueError("not a finite-difference run")
+        return (out[:3 * N].reshape(N, 3), out[3 * N:6 * N].reshape(N, 3), out[6 * N:11 * N].reshape(N, 5),
+                out[11 * N:16 * N].reshape(N, 5), out[16 * N:16 * N + 3], out[16 * N + 3:])
 
     def grid(self):
         r, x = np.zeros(self.n_r), np.zeros(self.n_r)

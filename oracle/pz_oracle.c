@@ -3223,6 +3223,21 @@ static void fd_from_fields(or_handle *h, int with_om)
     }
 }
 
+/* test hook: the finite-difference stencils, [dr 3N | ddr 3N | dddr 5N |
+ * ddddr 5N | dr_top 3 | dr_bot 3]; -1 unless the run uses finite differences */
+int or_fd_stencils(or_handle *h, double *out)
+{
+    if (!h->fd) return -1;
+    int N = h->N;
+    memcpy(out, h->fdr, sizeof(double) * 3 * N);
+    memcpy(out + 3 * N, h->fddr, sizeof(double) * 3 * N);
+    memcpy(out + 6 * N, h->fdddr, sizeof(double) * 5 * N);
+    memcpy(out + 11 * N, h->fddddr, sizeof(double) * 5 * N);
+    memcpy(out + 16 * N, h->fdr_top, sizeof(double) * 3);
+    memcpy(out + 16 * N + 3, h->fdr_bot, sizeof(double) * 3);
+    return 0;
+}
+
 /* rInt_R, finite-difference branch (integration.f90:60-96) */
 static double rint_fd(or_handle *h, const double *f)
 {

@@ -45,6 +45,10 @@ int or_ifft(or_handle *h, const double *spec, double *grid, int n_rad);
 int or_scheme_weights(int scheme, double alpha, const double *dt, int *family, int *nstages,
                       double *wimp, double *wimp_lin, double *wexp, double *bimp, double *bexp);
 
+/* finite-difference stencils [dr 3N | ddr 3N | dddr 5N | ddddr 5N | dr_top 3 |
+ * dr_bot 3] (16 N + 6 values); -1 for a Chebyshev run */
+int or_fd_stencils(or_handle *h, double *out);
+
 /* radial grid and the Chebyshev x grid (n_r_max values each) */
 int or_grid(or_handle *h, double *r, double *x);
 

@@ -284,3 +284,30 @@ def test_fd_converges_to_chebyshev_solution():
         assert errs[(65, f)] < 1e-2, (f, errs[(65, f)])
         assert errs[(65, f)] / errs[(129, f)] > 3.0, (f, errs[(65, f)], errs[(129, f)])
     assert errs[(129, "om")] < 2e-2 and errs[(129, "om")] < errs[(65, "om")]
+
+
+def test_fd_stencils_exact_on_polynomials():
+    """Fornberg weights on the stretched grid (finite_differences.f90:501-546):
+    the 3-point first and second derivatives are exact for quadratics, the
+    5-point third and fourth derivatives for quartics, the one-sided wall
+    derivatives for quadratics; the wall rows are the symmetric formulas
+    with ghost points at the mirrored spacing."""
+    o = Oracle(_fd_cfg(65, True))
+    r, _ = o.grid()
+    N = len(r)
+    dr, ddr, d3, d4, top, bot = o.fd_stencils()
+    for p_ in range(3):
+        for k in range(1, N - 1):
+            f = (r[k - 1:k + 2] - r[k]) ** p_
+            assert abs(dr[k] @ f - (1.0 if p_ == 1 else 0.0)) < 1e-9
+            assert abs(ddr[k] @ f - (2.0 if p_ == 2 else 0.0)) < 1e-6
+        assert abs(top @ (r[:3] - r[0]) ** p_ - (1.0 if p_ == 1 else 0.0)) < 1e-9
+        assert abs(bot @ (r[[N - 1, N - 2, N - 3]] - r[-1]) ** p_ - (1.0 if p_ == 1 else 0.0)) < 1e-9
+    for p_ in range(5):
+        for k in range(2, N - 2):
+            f = (r[k - 2:k + 3] - r[k]) ** p_
+            assert abs(d3[k] @ f - (6.0 if p_ == 3 else 0.0)) < 1e-3 * max(1.0, np.abs(d3[k]).max() * 1e-9)
+            assert abs(d4[k] @ f - (24.0 if p_ == 4 else 0.0)) < 1e-3 * max(1.0, np.abs(d4[k]).max() * 1e-9)
+    h0 = r[1] - r[0]
+    assert np.allclose(dr[0], [-0.5 / h0, 0.0, 0.5 / h0]) and np.allclose(ddr[0], np.array([1, -2, 1]) / h0**2)
+    assert np.all(d3[0] == 0) and np.all(d4[-1] == 0)
