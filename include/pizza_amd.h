@@ -151,6 +151,13 @@ int pz_time(pz_handle *h, double *time, double *dt);
  */
 int pz_ekin_3d(pz_handle *h, double out[4]);
 
+/*
+ * Classical Nusselt numbers of the heat time series (outputs.f90:729-755):
+ * out[0] = NuTop, out[1] = NuBot, 1 + dT/dr of the m = 0 temperature
+ * perturbation over dT_cond/dr at each wall.  Synchronises.
+ */
+int pz_nusselt(pz_handle *h, double out[2]);
+
 /* The HIP stream (hipStream_t) the handle launches on. */
 void *pz_stream(pz_handle *h);
 

@@ -35,6 +35,7 @@ def lib():
         L.or_step.argtypes = [vp, ip]
         L.or_time.argtypes = [vp, dp, dp]
         L.or_ekin_3d.argtypes = [vp, dp]
+        L.or_nusselt.argtypes = [vp, dp]
         L.or_costf.argtypes = [vp, dp, ip]
         L.or_get_dr.argtypes = [vp, dp, dp, ip, ip]
         L.or_get_ddr.argtypes = [vp, dp, dp, dp, ip, ip]
@@ -118,6 +119,11 @@ class Oracle:
         t, dt = ctypes.c_double(), ctypes.c_double()
         lib().or_time(self.h, ctypes.byref(t), ctypes.byref(dt))
         return t.value, dt.value
+
+    def nusselt(self):
+        out = np.zeros(2)
+        lib().or_nusselt(self.h, _dp(out))
+        return out
 
     def ekin_3d(self):
         out = np.zeros(4)

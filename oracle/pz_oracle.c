@@ -3590,6 +3590,20 @@ static double rint_r(or_handle *h, const double *f)
 
 static double round_off(double v) { return fabs(v) < 10.0 * 2.220446049250313e-16 ? 0.0 : v; }
 
+/* classical Nusselt numbers (outputs.f90:729-755) */
+int or_nusselt(or_handle *h, double out[2])
+{
+    int N = h->N;
+    if (!h->l_heat) { out[0] = out[1] = 1.0; return 0; }
+    cplx *d = calloc(N, sizeof(cplx));
+    if (h->fd) memcpy(d, h->dtemp, sizeof(cplx) * N);
+    else get_dr_vec(h, h->temp, d, 1);
+    out[0] = round_off(fabs(h->dtcond[0]) > 10.0 * 2.220446049250313e-16 ? 1.0 + creal(d[0]) / h->dtcond[0] : 1.0);
+    out[1] = round_off(fabs(h->dtcond[N - 1]) > 10.0 * 2.220446049250313e-16 ? 1.0 + creal(d[N - 1]) / h->dtcond[N - 1] : 1.0);
+    free(d);
+    return 0;
+}
+
 /* 3-D kinetic energies (outputs.f90:458-571) */
 int or_ekin_3d(or_handle *h, double out[4])
 {

@@ -29,6 +29,7 @@ int or_start(or_handle *h);
 int or_step(or_handle *h, int nsteps);
 int or_time(or_handle *h, double *time, double *dt);
 int or_ekin_3d(or_handle *h, double out[4]);
+int or_nusselt(or_handle *h, double out[2]);
 
 /* unit operators on (n_m, n_r) complex arrays, m fastest */
 int or_costf(or_handle *h, double *f, int n_m);

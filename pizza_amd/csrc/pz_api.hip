@@ -189,6 +189,7 @@ struct pz_handle {
     bool use_graph = true;
     size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0;
     bool radial_big = false; // k_radial_loop_big for long azimuthal grids
+    double dtcond_top = 0.0, dtcond_bot = 0.0; // dT_cond/dr at the walls (Nusselt numbers)
     int inv_waves = 16;
     int mthreads = PZ_MPHASE_THREADS; // threads of k_mphase / k_mcoll
     PzDev *ddev = nullptr;
@@ -674,6 +675,8 @@ static void setup(pz_handle *h)
     }
     std::vector<double> tcond(N, 0.0), dtcond(N, 0.0), xicond(N, 0.0), dxicond(N, 0.0);
     if (h->l_heat) conducting_state(h, c.tcond_fac, c.ktopt, c.kbott, c.t_top, c.t_bot, tcond, dtcond);
+    h->dtcond_top = dtcond[0];
+    h->dtcond_bot = dtcond[N - 1];
     if (h->l_chem) conducting_state(h, c.xicond_fac, c.ktopxi, c.kbotxi, c.xi_top, c.xi_bot, xicond, dxicond);
     // transform tables
     const int L = 2 * N - 2;
@@ -1352,6 +1355,38 @@ extern "C" int pz_get_ddr(pz_handle *h, const double *f, double *df, double *ddf
 {
     if (!h || !f || !df || !ddf || n_m <= 0) return -1;
     return vec_op(h, f, df, ddf, n_m, 2, l_dct_in);
+}
+
+// Classical top and bottom Nusselt numbers of the heat time series
+// (outputs.f90:729-755): 1 + (dT/dr of the m = 0 perturbation) / (dT_cond/dr)
+// at each wall, from the radial derivative of the run's own method.
+extern "C" int pz_nusselt(pz_handle *h, double out[2])
+{
+    if (!h || !out) return -1;
+    const double tiny = 10.0 * 2.220446049250313e-16;
+    if (!h->l_heat) {
+        out[0] = out[1] = 1.0;
+        return 0;
+    }
+    const int N = h->N, nm = h->nm;
+    std::vector<double> fld(2 * (size_t)nm * N), col(2 * (size_t)N), dcol(2 * (size_t)N);
+    int rc;
+    if (h->d.fd) {
+        // the finite-difference step refreshes dT/dr (ghost-zone stencil) every stage
+        rc = pz_get_field(h, PZ_DTEMP, fld.data());
+        if (rc) return rc;
+        for (int k = 0; k < N; k++) dcol[2 * k] = fld[2 * ((size_t)k * nm)];
+    } else {
+        rc = pz_get_field(h, PZ_TEMP, fld.data());
+        if (rc) return rc;
+        for (int k = 0; k < N; k++) { col[2 * k] = fld[2 * ((size_t)k * nm)]; col[2 * k + 1] = fld[2 * ((size_t)k * nm) + 1]; }
+        rc = pz_get_dr(h, col.data(), dcol.data(), 1, 1);
+        if (rc) return rc;
+    }
+    auto round_off = [&](double v) { return std::fabs(v) < tiny ? 0.0 : v; };
+    out[0] = round_off(std::fabs(h->dtcond_top) > tiny ? 1.0 + dcol[0] / h->dtcond_top : 1.0);
+    out[1] = round_off(std::fabs(h->dtcond_bot) > tiny ? 1.0 + dcol[2 * (N - 1)] / h->dtcond_bot : 1.0);
+    return 0;
 }
 
 extern "C" int pz_fft(pz_handle *h, const double *grid, double *spec, int n_rad)

@@ -176,6 +176,11 @@ contains
          import :: c_int, c_ptr
          type(c_ptr), value :: e
       end function
+      integer(c_int) function pz_nusselt(h, nu) bind(c, name='pz_nusselt')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: nu(2)
+      end function
    end interface
       p = pz_last_error_c(h)
       n = int(c_strlen(p))

@@ -45,6 +45,7 @@ def lib():
         L.pz_step.argtypes = [vp, ip]
         L.pz_time.argtypes = [vp, dp, dp]
         L.pz_ekin_3d.argtypes = [vp, dp]
+        L.pz_nusselt.argtypes = [vp, dp]
         L.pz_stream.argtypes = [vp]
         L.pz_stream.restype = vp
         L.pz_costf.argtypes = [vp, dp, ip]
@@ -64,7 +65,7 @@ def lib():
 
 EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_dims",
            "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step",
-           "pz_time", "pz_ekin_3d", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
+           "pz_time", "pz_ekin_3d", "pz_nusselt", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
            "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
            "pz_ensemble_destroy", "pz_ensemble_last_error"]
 
@@ -132,6 +133,12 @@ class Run:
     def ekin_3d(self):
         out = np.zeros(4)
         self._check(lib().pz_ekin_3d(self.h, _dp(out)), "pz_ekin_3d")
+        return out
+
+    def nusselt(self):
+        """(NuTop, NuBot)."""
+        out = np.zeros(2)
+        self._check(lib().pz_nusselt(self.h, _dp(out)), "pz_nusselt")
         return out
 
     def stream(self):

@@ -132,6 +132,7 @@ def test_bpr353_thermochemical_parity(n_r, m_max, nsteps):
         assert err < max(TOLN, 10 * floors[1][f]), (f, err, floors[1][f])
     for f in ("temp_hat", "xi_hat", "psi_hat"):
         assert rel(g.get_field(f), o.get_field(f)) < TOLN, f
+    np.testing.assert_allclose(g.nusselt(), o.nusselt(), rtol=1e-10)
 
 
 def test_graph_replay_is_deterministic():
@@ -277,6 +278,7 @@ def test_finite_difference_parity(scheme):
             err = rel(g.get_field(f), o.get_field(f))
             assert err < max(1e-11 if i == 0 else 1e-10, 10 * floors[i][f]), (f, i, err, floors[i][f])
     np.testing.assert_allclose(g.ekin_3d(), o.ekin_3d(), rtol=1e-10, atol=1e-30)
+    np.testing.assert_allclose(g.nusselt(), o.nusselt(), rtol=1e-10)
 
 
 def test_long_azimuthal_grid_radial_loop():

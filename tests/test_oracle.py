@@ -311,3 +311,32 @@ def test_fd_stencils_exact_on_polynomials():
     h0 = r[1] - r[0]
     assert np.allclose(dr[0], [-0.5 / h0, 0.0, 0.5 / h0]) and np.allclose(ddr[0], np.array([1, -2, 1]) / h0**2)
     assert np.all(d3[0] == 0) and np.all(d4[-1] == 0)
+
+
+@pytest.mark.parametrize("method", ["integ", "coll", "fd"])
+def test_nusselt_closed_form(method):
+    """NuTop / NuBot (outputs.f90:729-755) of a conducting state plus the
+    axisymmetric perturbation eps sin(pi (r - r_icb)): 1 - eps pi / dTc/dr
+    at the outer wall and 1 + eps pi / dTc/dr at the inner one (exactly 1
+    without the perturbation)."""
+    from pizza_amd.config import default_config
+    n_r = 65 if method == "fd" else 33
+    cfg = default_config(n_r_max=n_r, n_cheb_max=n_r, m_max=8, time_scheme="CNAB2", ra=1e5, ek=1e-3,
+                         radratio=0.35, cheb_method="colloc" if method == "coll" else "integ",
+                         radial_scheme="FD" if method == "fd" else "CHEB")
+    o = Oracle(cfg)
+    r, _ = o.grid()
+    o.start()
+    assert np.array_equal(o.nusselt(), [1.0, 1.0])
+    eps = 1e-3
+    T = np.zeros((o.n_m, o.n_r), complex)
+    T[0] = eps * np.sin(np.pi * (r - r[-1]))
+    o2 = Oracle(cfg)
+    o2.set_field("temp", T)
+    o2.start()
+    # conducting state (radial.f90:284-397): dTc/dr = -r_icb r_cmb / r^2 / ln(r_cmb / r_icb) / r
+    rc, ri = r[0], r[-1]
+    dtc = np.array([-1.0 / (np.log(rc / ri) * rc), -1.0 / (np.log(rc / ri) * ri)])
+    expect = 1.0 + np.array([-eps * np.pi, eps * np.pi]) / dtc
+    tol = 3e-3 if method == "fd" else 1e-10
+    assert np.allclose(o2.nusselt(), expect, rtol=tol), (o2.nusselt(), expect)
