@@ -206,6 +206,32 @@ int pz_ensemble_step(pz_ensemble *e, int nsteps);
 int pz_ensemble_destroy(pz_ensemble *e);
 const char *pz_ensemble_last_error(const pz_ensemble *e);
 
+/*
+ * Discrete adjoint of the step.  The reference has no adjoint; the forward
+ * map differentiated is pz_step's (collocation method, multistep schemes,
+ * dt fixed over the window), its objectives are the diagnostics of
+ * outputs.f90:
+ *   PZ_OBJ_NUTOP  J = (1/K) sum_{n=1..K} NuTop(n), the outer-wall Nusselt
+ *                 number (outputs.f90:729-755) after each of the K steps;
+ *   PZ_OBJ_TEMP   J = sum Re(conj(W) T(K)) over the spectral temperature
+ *                 after the last step, W = weights (layout of pz_set_field).
+ * pz_adjoint_forward runs K steps from the state pz_start left, keeping a
+ * snapshot of the step state every ckpt_every steps, and returns J; the
+ * handle ends at the state after K steps.  pz_adjoint_backward replays each
+ * segment from its snapshot and runs the transposed step kernels backwards
+ * through it; it returns dJ times the gradient of J with respect to the
+ * temperature field given to pz_start (grad_temp, layout of pz_set_field)
+ * and to the boundary values of T in the implicit solves (grad_bc,
+ * complex(8) (n_m_max, 2): outer wall, inner wall; the fixed temperature or
+ * flux pattern of ktopt/kbott, with the conducting state held fixed), and
+ * restores the state after K steps.  Either output may be null.  Returns -7
+ * if dt (and so the matrices) changed inside the window.
+ */
+enum pz_objective { PZ_OBJ_NUTOP = 0, PZ_OBJ_TEMP = 2 };
+int pz_adjoint_forward(pz_handle *h, int objective, const double *weights, int nsteps, int ckpt_every,
+                       double *J);
+int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, double *grad_bc);
+
 #ifdef __cplusplus
 }
 #endif

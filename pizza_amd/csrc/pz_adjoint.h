@@ -1,0 +1,666 @@
+// Device code of libpizza_amd, part 5: the discrete adjoint of the IMEX
+// step for the collocation method and the multistep schemes.  The reference
+// has no adjoint; every kernel here is the exact transpose of the forward
+// kernel it names, so that <J dx, y> = <dx, J^T y> holds to rounding:
+//   k_adj_probe       the forward's radial operators as dense matrices
+//   k_mcoll_adj       transpose of k_mcoll (stage 1, multistep family)
+//   k_radial_adj      transpose of k_radial_loop, linearised at the
+//                     state the step started from
+//   k_start_coll_adj  transpose of k_start_coll with respect to T
+// Inner product: the real one, <a, b> = sum Re(conj(a) b) over every
+// complex entry, so complex-linear maps transpose to their conjugates.
+// Adjoint variables share the state layout (arena_layout), addressed
+// through a PzDev view of their own.  Included once by pz_api.hip.
+#pragma once
+#include "pz_coll.h"
+
+struct PzAdjDev {
+    // dense operators, entry (i, j) = component i of the image of the unit
+    // vector e_j, stored at [i N + j] (rows contiguous for the transposed
+    // products, one thread per output):
+    // C   costf (dct_fft), grid <-> Chebyshev coefficients
+    // G   grid values from coefficients as dr_fft returns them
+    // D1, D2  d/dr, d^2/dr^2 on the grid from coefficients (dr_fft)
+    // E1  m = 0: grid values of the dcheb derivative series from coefficients
+    // Uimp  m = 0: implicit term of u_phi from grid values (coll_uphi0_impl)
+    const double *C, *G, *D1, *D2, *E1, *Uimp;
+    const double *lnu; // d/dr at r_cmb of m = 0 grid values (Nusselt numbers)
+    // adjoints of the radial loop outputs T_expl(1), dVsT, xi, omega
+    cd *eT0, *aT0, *eX0, *aX0, *eP0, *aP0;
+    // accumulated adjoints of the boundary rows of the T and xi systems
+    cd *gtop, *gbot, *gtopxi, *gbotxi;
+    int *flag; // set to 1 when the window rebuilds its matrices
+    // copies of the inverses the window's steps ran with: _s those of the
+    // scheme's own weights, _b those of the start-up steps of a multistep
+    // scheme bridged with CNAB2 (their implicit weight differs); the live
+    // inverses may be rebuilt by a replay running beside the adjoint
+    const double *T_inv_s, *X_inv_s, *U_inv_s, *T_inv_b, *X_inv_b, *U_inv_b;
+    const cd *PC_inv_s, *PC_inv_b;
+};
+
+// out_f[j] (+)= sum_{i<n} M[j ld + i] y_f[i] for j < n: one wave per output
+// column, lanes over i, so the column reads are coalesced.  Writers of
+// output j are always the same wave, so successive calls may accumulate
+// without a barrier; callers still synchronise before reading out.
+template <int NF>
+__device__ void matT(int n, int ld, const double *__restrict__ M, VecSet<const cd> y, VecSet<cd> out, bool acc)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int j = w; j < n; j += nw) {
+        const double *col = M + (size_t)j * ld;
+        double sx[NF], sy[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) sx[f] = sy[f] = 0.0;
+        for (int i = lane; i < n; i += 64) {
+            const double a = col[i];
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const cd v = y[f][i];
+                sx[f] += a * v.x;
+                sy[f] += a * v.y;
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const double rx = wave_sum(sx[f]), ry = wave_sum(sy[f]);
+            if (lane == 0) out[f][j] = acc ? mk(out[f][j].x + rx, out[f][j].y + ry) : mk(rx, ry);
+        }
+    }
+}
+
+// out_f[j] (+)= sum_{i<n} M[i n + j] y_f[i] for j < n (the transpose of the
+// n x n matrix M stored by rows): one thread per output, so the reads of a
+// row are coalesced across the wave and y is a broadcast LDS read
+template <int NF>
+__device__ void matTr(int n, const double *__restrict__ M, VecSet<const cd> y, VecSet<cd> out, bool acc)
+{
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double sx[NF], sy[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) sx[f] = sy[f] = 0.0;
+#pragma unroll 8
+        for (int i = 0; i < n; i++) {
+            const double a = M[(size_t)i * n + j];
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const cd v = y[f][i];
+                sx[f] += a * v.x;
+                sy[f] += a * v.y;
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) out[f][j] = acc ? mk(out[f][j].x + sx[f], out[f][j].y + sy[f]) : mk(sx[f], sy[f]);
+    }
+}
+
+// out[j] = sum_{i<rows} conj(M[j ld + i]) y[i] for j < ncol (complex M)
+__device__ void matH(int rows, int ncol, int ld, const cd *__restrict__ M, const cd *y, cd *out)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int j = w; j < ncol; j += nw) {
+        const cd *col = M + (size_t)j * ld;
+        double sx = 0.0, sy = 0.0;
+        for (int i = lane; i < rows; i += 64) {
+            const cd a = col[i], v = y[i];
+            sx += a.x * v.x + a.y * v.y;
+            sy += a.x * v.y - a.y * v.x;
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        if (lane == 0) out[j] = mk(sx, sy);
+    }
+}
+
+// i s a for real s
+__device__ __forceinline__ cd ci(double s, cd a) { return mk(-s * a.y, s * a.x); }
+
+// ---------------------------------------------------------------------------
+// The forward's radial operators applied to the unit vectors: block j
+// writes column j of every matrix through the same device routines the
+// step uses, so the transposes below are those of the step's own
+// arithmetic.  Linear only without the rerror threshold (checked on the
+// host).  LDS: 6 N + 6 L + 16 complex and 3 N doubles.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_adj_probe(const PzDev p __attribute__((grid_constant)), double *C,
+                                                   double *G, double *D1, double *D2, double *E1, double *Uimp,
+                                                   double *lnu)
+{
+    extern __shared__ double smem[];
+    const int j = blockIdx.x, N = p.N, tid = threadIdx.x;
+    const RadView R(p.radc, N);
+    cd *e = (cd *)smem, *c = e + N, *d1 = e + 2 * N, *d2 = e + 3 * N, *g = e + 4 * N, *t = e + 5 * N;
+    cd *fbuf = e + 6 * N;
+    double *dt3 = (double *)(fbuf + 6 * p.L + 16);
+    // entry (k, j) at k N + j
+    for (int k = tid; k < N; k += blockDim.x) e[k] = mk(k == j ? 1.0 : 0.0, 0.0);
+    __syncthreads();
+    dct_fft(p, VecSet<const cd>(e), VecSet<cd>(c), 1, 0, p.cosT, p.sinT, fbuf);
+    __syncthreads();
+    dr_fft(p, R, VecSet<const cd>(e), VecSet<cd>(d1), VecSet<cd>(d2), 1, p.cosT, p.sinT, fbuf, g);
+    __syncthreads();
+    for (int k = tid; k < N; k += blockDim.x) {
+        C[(size_t)k * N + j] = c[k].x;
+        G[(size_t)k * N + j] = g[k].x;
+        D1[(size_t)k * N + j] = d1[k].x;
+        D2[(size_t)k * N + j] = d2[k].x;
+    }
+    __syncthreads();
+    // d/dr at the outer wall of the grid vector e_j: costf, then get_dr
+    dr_fft(p, R, VecSet<const cd>(c), VecSet<cd>(t), VecSet<cd>(), 1, p.cosT, p.sinT, fbuf);
+    __syncthreads();
+    if (tid == 0) lnu[j] = t[0].x;
+    // m = 0: dcheb series of the coefficient vector e_j, back to the grid
+    double *f = dt3, *df = dt3 + N;
+    for (int k = tid; k < N; k += blockDim.x) f[k] = k == j ? 1.0 : 0.0;
+    __syncthreads();
+    if (tid == 0) dcheb_real(p, f, df);
+    __syncthreads();
+    for (int k = tid; k < N; k += blockDim.x) d1[k] = mk(df[k], 0.0);
+    __syncthreads();
+    dct_fft(p, VecSet<const cd>(d1), VecSet<cd>(t), 1, 0, p.cosT, p.sinT, fbuf);
+    __syncthreads();
+    for (int k = tid; k < N; k += blockDim.x) E1[(size_t)k * N + j] = t[k].x;
+    __syncthreads();
+    // m = 0 implicit term of the grid vector e_j
+    coll_uphi0_impl(p, R, e, c, d1, dt3, p.cosT, p.sinT, fbuf, t, p.hdifV[0]);
+    __syncthreads();
+    for (int k = tid; k < N; k += blockDim.x) Uimp[(size_t)k * N + j] = t[k].x;
+}
+
+// Shifts the adjoints of one time array back through rotate_imex
+// (multistep_schemes.f90:462-498) for element k: slot o of the step's
+// input feeds slot o + 1 of its output; slot 0 of old and impl is
+// rewritten by the step (impl only when lcalc), slot 0 of expl always.
+__device__ __forceinline__ void unrotate(const PzDev &p, cd *old, cd *impl, cd *expl, size_t off, int k, int lcalc)
+{
+    const size_t F = (size_t)p.nm * p.N;
+    for (int o = 0; o + 1 < p.nold; o++) old[o * F + off + k] = old[(o + 1) * F + off + k];
+    old[(p.nold - 1) * F + off + k] = mk(0, 0);
+    if (!lcalc && p.nimp > 1) impl[off + k] = cadd(impl[off + k], impl[F + off + k]);
+    for (int o = lcalc ? 0 : 1; o + 1 < p.nimp; o++) impl[o * F + off + k] = impl[(o + 1) * F + off + k];
+    if (lcalc || p.nimp > 1) impl[(p.nimp - 1) * F + off + k] = mk(0, 0);
+    for (int o = 1; o + 1 < p.nexp; o++) expl[o * F + off + k] = expl[(o + 1) * F + off + k];
+    if (p.nexp > 1) expl[(p.nexp - 1) * F + off + k] = mk(0, 0);
+    expl[off + k] = mk(0, 0);
+}
+
+// the right-hand side rhs = sum wimp old + sum wimp_lin impl + sum wexp expl
+// (set_imex_rhs, multistep_schemes.f90:366-413) transposed, for the slots
+// other than the current explicit one
+__device__ __forceinline__ void rhs_to_slots(const PzDev &p, const PzScheme *s, cd *old, cd *impl, cd *expl,
+                                             size_t off, int k, cd rb)
+{
+    const size_t F = (size_t)p.nm * p.N;
+    for (int o = 0; o < p.nold; o++) old[o * F + off + k] = cadd(old[o * F + off + k], cscale(s->wimp[o], rb));
+    for (int o = 0; o < p.nimp; o++)
+        impl[o * F + off + k] = cadd(impl[o * F + off + k], cscale(s->wimp_lin[o + 1], rb));
+    for (int o = 1; o < p.nexp; o++) expl[o * F + off + k] = cadd(expl[o * F + off + k], cscale(s->wexp[o], rb));
+}
+
+#define ADJ_NVEC 34 // LDS vectors of k_mcoll_adj
+// ---------------------------------------------------------------------------
+// Transpose of mcoll_body (stage 1 of a multistep scheme) for one m: reads
+// the adjoints of the step's outputs from lam (in place, row m), writes the
+// adjoints of its inputs there, and the adjoints of the radial loop's
+// outputs to ad.  fw holds the scheme state the step ran with (weights,
+// step count); the m-loop is linear, so no field of fw is read.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(512) k_mcoll_adj(const PzDev *__restrict__ pfw, const PzDev *__restrict__ plam,
+                                                   const PzAdjDev *__restrict__ pad)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pfw;
+    const PzDev &L = *plam;
+    const PzAdjDev &A = *pad;
+    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi], tid = threadIdx.x, nb = blockDim.x;
+    const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
+    const PzScheme *ss = p.ss;
+    const RadView R(p.radc, N);
+    const bool hT = p.l_heat, hX = p.l_chem;
+    const int nc = p.ncheb, ldi = p.ldi;
+    const double dm = (double)m, dm2 = dm * dm;
+    const int lcalc = p.l_imp_calc_rhs[0] || (p.bridge && ss->n_steps + 1 <= p.nold - 1);
+    const double wcur = ss->wexp[0], wl = ss->wimp_lin[0];
+    // the CNAB2 start-up steps ran with their own matrices; the matrices
+    // may change only at the first step and at the end of the start-up
+    const bool brg = p.bridge && ss->n_steps <= p.nold - 1;
+    const double *Tinv = brg ? A.T_inv_b : A.T_inv_s, *Xinv = brg ? A.X_inv_b : A.X_inv_s;
+    const double *Uinv = brg ? A.U_inv_b : A.U_inv_s;
+    const cd *PCinv = brg ? A.PC_inv_b : A.PC_inv_s;
+    if (mi == 0 && tid == 0 &&
+        (ss->l_new_dt || (ss->lMat && ss->n_steps > 1 && !(p.bridge && ss->n_steps == p.nold))))
+        *A.flag = 1;
+
+    cd *V = (cd *)smem;
+    cd *gT = V, *iT = V + N, *d1T = V + 2 * N, *d2T = V + 3 * N, *xT = V + 4 * N, *rT = V + 5 * N, *eT = V + 6 * N;
+    cd *gX = V + 7 * N, *iX = V + 8 * N, *d1X = V + 9 * N, *d2X = V + 10 * N, *xX = V + 11 * N, *rX = V + 12 * N;
+    cd *eX = V + 13 * N;
+    cd *Lom = V + 14 * N, *Lpo = V + 15 * N, *Lps = V + 16 * N, *Lus = V + 17 * N, *Lup = V + 18 * N;
+    cd *iP = V + 19 * N, *d1w = V + 20 * N, *d2w = V + 21 * N, *dps = V + 22 * N, *Lw = V + 23 * N;
+    cd *zc = V + 24 * N; // 2 N: [omega; psi] coefficients
+    cd *rP = V + 26 * N, *eP = V + 27 * N;
+    cd *Lus0 = V + 28 * N, *LT0 = V + 29 * N, *LX0 = V + 30 * N; // adjoints of the step's input fields
+    cd *t1 = V + 31 * N, *t2 = V + 32 * N, *t3 = V + 33 * N;
+
+    // ---- adjoints of the step's outputs; time arrays shifted back -------
+    for (int k = tid; k < N; k += nb) {
+        if (hT) {
+            gT[k] = cadd(L.temp[off + k], L.T_old[off + k]);
+            iT[k] = lcalc ? L.T_impl[off + k] : mk(0, 0);
+            d1T[k] = lcalc ? L.dtemp[off + k] : mk(0, 0);
+            eT[k] = L.T_expl[off + k];
+            if (p.nexp > 1) eT[k] = cadd(eT[k], L.T_expl[F + off + k]);
+            unrotate(p, L.T_old, L.T_impl, L.T_expl, off, k, lcalc);
+        }
+        if (hX) {
+            gX[k] = cadd(L.xi[off + k], L.X_old[off + k]);
+            iX[k] = lcalc ? L.X_impl[off + k] : mk(0, 0);
+            d1X[k] = mk(0, 0);
+            eX[k] = L.X_expl[off + k];
+            if (p.nexp > 1) eX[k] = cadd(eX[k], L.X_expl[F + off + k]);
+            unrotate(p, L.X_old, L.X_impl, L.X_expl, off, k, lcalc);
+        }
+        Lom[k] = L.om[off + k];
+        Lpo[k] = L.P_old[off + k];
+        Lps[k] = L.psi[off + k];
+        Lus[k] = L.us[off + k];
+        Lup[k] = L.up[off + k];
+        iP[k] = lcalc ? L.P_impl[off + k] : mk(0, 0);
+        eP[k] = L.P_expl[off + k];
+        if (p.nexp > 1) eP[k] = cadd(eP[k], L.P_expl[F + off + k]);
+        unrotate(p, L.P_old, L.P_impl, L.P_expl, off, k, lcalc);
+        Lus0[k] = mk(0, 0);
+        LT0[k] = mk(0, 0);
+        LX0[k] = mk(0, 0);
+    }
+    __syncthreads();
+
+    // ---- psi / omega / u_phi (before T: the psi system reads the new T) --
+    if (m == 0) {
+        // u0 = up = Re(C f); om = drx Re(E1 f) + or1 u0; old = u0; impl = Uimp u0
+        for (int k = tid; k < N; k += nb) {
+            const double lo = Lom[k].x;
+            t1[k] = mk(Lup[k].x + Lpo[k].x + R.or1(k) * lo, 0.0); // adjoint of u0 (grid)
+            t2[k] = mk(R.drx(k) * lo, 0.0);                       // adjoint of E1 f
+            t3[k] = mk(iP[k].x, 0.0);
+        }
+        __syncthreads();
+        if (lcalc) {
+            matTr<1>(N, A.Uimp, VecSet<const cd>(t3), VecSet<cd>(t1), true);
+            __syncthreads();
+        }
+        matTr<1>(N, A.C, VecSet<const cd>(t1), VecSet<cd>(zc), false);
+        matTr<1>(N, A.E1, VecSet<const cd>(t2), VecSet<cd>(zc), true);
+        __syncthreads();
+        for (int k = nc + tid; k < N; k += nb) zc[k] = mk(0, 0);
+        __syncthreads();
+        matT<1>(N, ldi, Uinv, VecSet<const cd>(zc), VecSet<cd>(rP), false);
+        __syncthreads();
+        for (int k = tid; k < N; k += nb) {
+            const cd rb = (k == 0 || k == N - 1) ? mk(0, 0) : mk(rP[k].x, 0.0);
+            eP[k] = cadd(eP[k], cscale(wcur, rb));
+            rhs_to_slots(p, ss, L.P_old, L.P_impl, L.P_expl, off, k, rb);
+            A.eP0[off + k] = eP[k];
+            A.aP0[off + k] = mk(0, 0);
+        }
+    } else {
+        const double VH = p.ViscFac * p.hdifV[mi];
+        for (int k = tid; k < N; k += nb) {
+            const double o1 = R.or1(k);
+            cd lw = cadd(Lom[k], Lpo[k]), lus = Lus[k], lup = Lup[k];
+            cd lps = Lps[k];
+            cd a1 = mk(0, 0), a2 = mk(0, 0);
+            if (lcalc) {
+                const cd v = iP[k];
+                const double ekp = p.CorFac * R.ekpump(k);
+                a2 = cscale(VH, v);
+                a1 = cscale(VH * o1, v);
+                lw = cadd(lw, cscale(-VH * dm2 * R.or2(k) - ekp, v));
+                lup = cadd(lup, cscale(-ekp * R.ekp_up(k), v));
+                // us coefficient: -ekp (ekp_us + i m ekp_dusdp) (+ CorFac beta)
+                const double cr = -ekp * R.ekp_us(k) + (p.l_coriolis_imp ? p.CorFac * R.beta(k) : 0.0);
+                const double cim = -ekp * dm * R.ekp_dusdp(k);
+                lus = cadd(lus, mk(cr * v.x + cim * v.y, cr * v.y - cim * v.x));
+                if (p.l_buo_imp) {
+                    if (hT) gT[k] = cadd(gT[k], ci(dm * R.rgrav(k) * o1 * p.BuoFac, v));
+                    if (hX) gX[k] = cadd(gX[k], ci(dm * R.rgrav(k) * o1 * p.ChemFac, v));
+                }
+            }
+            // us = i m or1 psi, up = -dpsi - beta psi
+            lps = cadd(lps, ci(-dm * o1, lus));
+            lps = csub(lps, cscale(R.beta(k), lup));
+            Lw[k] = lw;
+            Lps[k] = lps;
+            dps[k] = mk(-lup.x, -lup.y);
+            d1w[k] = a1;
+            d2w[k] = a2;
+        }
+        __syncthreads();
+        cd *zo = zc, *zp = zc + N;
+        matTr<2>(N, A.D1, VecSet<const cd>(dps, d1w), VecSet<cd>(zp, zo), false);
+        __syncthreads();
+        matTr<1>(N, A.G, VecSet<const cd>(Lps), VecSet<cd>(zp), true);
+        if (lcalc) {
+            matTr<1>(N, A.G, VecSet<const cd>(Lw), VecSet<cd>(zo), true);
+            matTr<1>(N, A.D2, VecSet<const cd>(d2w), VecSet<cd>(zo), true);
+        } else {
+            matTr<1>(N, A.C, VecSet<const cd>(Lw), VecSet<cd>(zo), false);
+        }
+        __syncthreads();
+        for (int k = nc + tid; k < N; k += nb) { zo[k] = mk(0, 0); zp[k] = mk(0, 0); }
+        __syncthreads();
+        matH(2 * N, N, 2 * N, PCinv + off * 2 * N, zc, rP);
+        __syncthreads();
+        for (int k = tid; k < N; k += nb) {
+            const double o1 = R.or1(k);
+            const cd rb = (k == 0 || k == N - 1) ? mk(0, 0) : rP[k];
+            if (p.l_buo_imp) {
+                if (hT) gT[k] = cadd(gT[k], ci(wl * dm * p.BuoFac * R.rgrav(k) * o1, rb));
+                if (hX) gX[k] = cadd(gX[k], ci(wl * dm * p.ChemFac * R.rgrav(k) * o1, rb));
+            }
+            const cd e = cadd(eP[k], cscale(wcur, rb));
+            rhs_to_slots(p, ss, L.P_old, L.P_impl, L.P_expl, off, k, rb);
+            A.eP0[off + k] = e;
+            t1[k] = cscale(-o1, e);
+            if (!p.l_coriolis_imp) Lus0[k] = cadd(Lus0[k], cscale(p.CorFac * R.beta(k), e));
+            if (!p.l_buo_imp) {
+                if (hT) LT0[k] = cadd(LT0[k], ci(dm * p.BuoFac * R.rgrav(k) * o1, e));
+                if (hX) LX0[k] = cadd(LX0[k], ci(dm * p.ChemFac * R.rgrav(k) * o1, e));
+            }
+        }
+        __syncthreads();
+        matTr<1>(N, A.D1, VecSet<const cd>(t1), VecSet<cd>(t2), false);
+        __syncthreads();
+        matTr<1>(N, A.C, VecSet<const cd>(t2), VecSet<cd>(A.aP0 + off), false);
+    }
+    __syncthreads();
+
+    // ---- T and xi ----------------------------------------------------------
+    if (hT || hX) {
+        const double kT = p.TdiffFac * p.hdifT[mi], kX = p.XidiffFac * p.hdifXi[mi];
+        for (int k = tid; k < N; k += nb) {
+            const double o1 = R.or1(k), o2 = R.or2(k);
+            if (hT && lcalc) {
+                const cd v = iT[k];
+                d2T[k] = cscale(kT, v);
+                d1T[k] = cadd(d1T[k], cscale(kT * o1, v));
+                gT[k] = csub(gT[k], cscale(kT * dm2 * o2, v));
+            }
+            if (hX && lcalc) {
+                const cd v = iX[k];
+                d2X[k] = cscale(kX, v);
+                d1X[k] = cscale(kX * o1, v);
+                gX[k] = csub(gX[k], cscale(kX * dm2 * o2, v));
+            }
+        }
+        __syncthreads();
+        if (lcalc) {
+            if (hT && hX) {
+                matTr<2>(N, A.G, VecSet<const cd>(gT, gX), VecSet<cd>(xT, xX), false);
+                matTr<2>(N, A.D1, VecSet<const cd>(d1T, d1X), VecSet<cd>(xT, xX), true);
+                matTr<2>(N, A.D2, VecSet<const cd>(d2T, d2X), VecSet<cd>(xT, xX), true);
+            } else {
+                cd *g = hT ? gT : gX, *a = hT ? d1T : d1X, *b = hT ? d2T : d2X, *x = hT ? xT : xX;
+                matTr<1>(N, A.G, VecSet<const cd>(g), VecSet<cd>(x), false);
+                matTr<1>(N, A.D1, VecSet<const cd>(a), VecSet<cd>(x), true);
+                matTr<1>(N, A.D2, VecSet<const cd>(b), VecSet<cd>(x), true);
+            }
+        } else {
+            if (hT) matTr<1>(N, A.C, VecSet<const cd>(gT), VecSet<cd>(xT), false);
+            if (hX) matTr<1>(N, A.C, VecSet<const cd>(gX), VecSet<cd>(xX), false);
+        }
+        __syncthreads();
+        for (int k = nc + tid; k < N; k += nb) { xT[k] = mk(0, 0); xX[k] = mk(0, 0); }
+        __syncthreads();
+        if (hT) matT<1>(N, ldi, Tinv + off * ldi, VecSet<const cd>(xT), VecSet<cd>(rT), false);
+        if (hX) matT<1>(N, ldi, Xinv + off * ldi, VecSet<const cd>(xX), VecSet<cd>(rX), false);
+        __syncthreads();
+        if (tid == 0) {
+            if (hT) { A.gtop[mi] = cadd(A.gtop[mi], rT[0]); A.gbot[mi] = cadd(A.gbot[mi], rT[N - 1]); }
+            if (hX) { A.gtopxi[mi] = cadd(A.gtopxi[mi], rX[0]); A.gbotxi[mi] = cadd(A.gbotxi[mi], rX[N - 1]); }
+        }
+        for (int k = tid; k < N; k += nb) {
+            const double o1 = R.or1(k);
+            if (hT) {
+                const cd rb = (k == 0 || k == N - 1) ? mk(0, 0) : rT[k];
+                const cd e = cadd(eT[k], cscale(wcur, rb));
+                rhs_to_slots(p, ss, L.T_old, L.T_impl, L.T_expl, off, k, rb);
+                A.eT0[off + k] = e;
+                t1[k] = cscale(-o1, e);
+                Lus0[k] = csub(Lus0[k], cscale(R.dtcond(k) - p.tadvz * R.beta(k) * R.tcond(k), e));
+            }
+            if (hX) {
+                const cd rb = (k == 0 || k == N - 1) ? mk(0, 0) : rX[k];
+                const cd e = cadd(eX[k], cscale(wcur, rb));
+                rhs_to_slots(p, ss, L.X_old, L.X_impl, L.X_expl, off, k, rb);
+                A.eX0[off + k] = e;
+                t3[k] = cscale(-o1, e);
+                Lus0[k] = csub(Lus0[k], cscale(R.dxicond(k), e));
+            }
+        }
+        __syncthreads();
+        if (hT) matTr<1>(N, A.D1, VecSet<const cd>(t1), VecSet<cd>(t2), false);
+        if (hX) matTr<1>(N, A.D1, VecSet<const cd>(t3), VecSet<cd>(xX), false);
+        __syncthreads();
+        if (hT) matTr<1>(N, A.C, VecSet<const cd>(t2), VecSet<cd>(A.aT0 + off), false);
+        if (hX) matTr<1>(N, A.C, VecSet<const cd>(xX), VecSet<cd>(A.aX0 + off), false);
+    }
+    __syncthreads();
+    // ---- adjoints of the step's input fields (the radial loop adds its own)
+    for (int k = tid; k < N; k += nb) {
+        L.us[off + k] = Lus0[k];
+        L.up[off + k] = mk(0, 0);
+        L.om[off + k] = mk(0, 0);
+        L.psi[off + k] = mk(0, 0);
+        L.temp[off + k] = LT0[k];
+        L.xi[off + k] = LX0[k];
+        if (lcalc) L.dtemp[off + k] = mk(0, 0); // rewritten only with the implicit terms
+    }
+}
+
+// Hermitian spectrum of z = a + i b for the complex FFT of two real grid
+// columns (the packing of the radial loop): A, B hold m < nm, B may be null
+__device__ __forceinline__ cd herm_pack(const cd *A, const cd *B, int k, int n, int nm)
+{
+    cd z = mk(0, 0);
+    if (k == 0) {
+        z = mk(A[0].x, B ? B[0].x : 0.0);
+    } else if (k < nm) {
+        const cd a = A[k], b = B ? B[k] : mk(0, 0);
+        z = mk(a.x - b.y, a.y + b.x);
+    } else if (n - k < nm) {
+        const int kk = n - k;
+        const cd a = cconj(A[kk]), b = B ? cconj(B[kk]) : mk(0, 0);
+        z = mk(a.x - b.y, a.y + b.x);
+    }
+    return z;
+}
+
+// ---------------------------------------------------------------------------
+// Transpose of radial_loop_body, linearised at the state the step started
+// from (fw), for one radius: adjoints of the products from those of the
+// explicit terms and dVs fields (ad), back through the FFTs to the spectra;
+// adds to the adjoints of us, up, omega, T, xi in lam.
+// LDS: (11 nm + 9 n) complex + 16 doubles.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_radial_adj(const PzDev *__restrict__ pfw, const PzDev *__restrict__ plam,
+                                                    const PzAdjDev *__restrict__ pad)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pfw;
+    const PzDev &L = *plam;
+    const PzAdjDev &A = *pad;
+    const int nr = blockIdx.x, n = p.nphi, nm = p.nm, N = p.N, tid = threadIdx.x, nb = blockDim.x;
+    cd *spec = (cd *)smem;  // 5 nm: forward spectra us, up, om, T, xi
+    cd *sb = spec + 5 * nm; // 6 nm: adjoint spectra (A, B) of each product
+    cd *gf = sb + 6 * nm;   // 3 n: forward grid columns
+    cd *bufa = gf + 3 * n, *bufb = bufa + 3 * n;
+    const bool hT = p.l_heat, hX = p.l_chem;
+    const int nin = hX ? 3 : 2, nout = 1 + (hT ? 1 : 0) + (hX ? 1 : 0);
+    for (int i = tid; i < nm; i += nb) {
+        const size_t q = (size_t)i * N + nr;
+        spec[i] = p.us[q];
+        spec[nm + i] = p.up[q];
+        spec[2 * nm + i] = p.om[q];
+        spec[3 * nm + i] = hT ? p.temp[q] : mk(0, 0);
+        spec[4 * nm + i] = hX ? p.xi[q] : mk(0, 0);
+    }
+    const double or1 = p.or1[nr], rr = p.r[nr];
+    const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
+    // adjoint spectra of the products, halved above m = 0 (Hermitian halves)
+    for (int t = tid; t < nout * nm; t += nb) {
+        const int tr = t / nm, i = t - tr * nm;
+        const int m = p.idx2m[i];
+        const size_t q = (size_t)i * N + nr;
+        cd a, b;
+        if (tr == 0) {
+            a = m == 0 ? mk(0, 0) : ci(or1 * m, A.eP0[q]);
+            b = A.aP0[q];
+        } else if (tr == 1 && hT) {
+            const cd e = A.eT0[q];
+            a = ci(or1 * m, e);
+            b = csub(A.aT0[q], cscale(fbeta, e));
+        } else {
+            a = ci(or1 * m, A.eX0[q]);
+            b = A.aX0[q];
+        }
+        const double h = i == 0 ? 1.0 : 0.5;
+        sb[2 * tr * nm + i] = cscale(h, a);
+        sb[(2 * tr + 1) * nm + i] = cscale(h, b);
+    }
+    __syncthreads();
+    // forward grid columns (us, up), (om, T), (xi, 0)
+    for (int t = tid; t < nin * n; t += nb) {
+        const int tr = t / n, k = t - tr * n;
+        const cd *Bs = (2 * tr + 1) < 5 ? spec + (2 * tr + 1) * nm : nullptr;
+        bufa[t] = herm_pack(spec + 2 * tr * nm, Bs, k, n, nm);
+    }
+    __syncthreads();
+    cd *g = fft_lds(p, bufa, bufb, n, nin, +1);
+    for (int t = tid; t < nin * n; t += nb) gf[t] = g[t];
+    __syncthreads();
+    // adjoints of the product columns: (1/n) Re sum_m Abar_m e^{+i m phi}
+    for (int t = tid; t < nout * n; t += nb) {
+        const int tr = t / n, k = t - tr * n;
+        bufa[t] = herm_pack(sb + 2 * tr * nm, sb + (2 * tr + 1) * nm, k, n, nm);
+    }
+    __syncthreads();
+    cd *s = fft_lds(p, bufa, bufb, n, nout, +1);
+    cd *o = (s == bufa) ? bufb : bufa;
+    const double inv_n = 1.0 / (double)n;
+    const int tX = hT ? 2 : 1;
+    for (int j = tid; j < n; j += nb) {
+        const double us = gf[j].x, up = gf[j].y, om = gf[n + j].x, T = gf[n + j].y;
+        const double X = hX ? gf[2 * n + j].x : 0.0;
+        const double ao = s[j].x * inv_n, bo = s[j].y * inv_n;
+        const double aT = hT ? s[n + j].x * inv_n : 0.0, bT = hT ? s[n + j].y * inv_n : 0.0;
+        const double aX = hX ? s[tX * n + j].x * inv_n : 0.0, bX = hX ? s[tX * n + j].y * inv_n : 0.0;
+        const double lup = ao * om + aT * T + aX * X;
+        const double lus = rr * (bo * om + bT * T + bX * X);
+        const double lom = ao * up + bo * rr * us;
+        const double lT = aT * up + bT * rr * us;
+        const double lX = aX * up + bX * rr * us;
+        o[j] = mk(lus, lup);
+        o[n + j] = mk(lom, lT);
+        if (hX) o[2 * n + j] = mk(lX, 0.0);
+    }
+    __syncthreads();
+    cd *other = (o == bufa) ? bufb : bufa;
+    cd *Z = fft_lds(p, o, other, n, nin, -1);
+    // back to the spectra: 2 sum_j g_j e^{-i m phi_j} above m = 0, the real
+    // part at m = 0; plus the Reynolds-stress sum (radial_loop.f90:267-278)
+    const double lsom = -A.eP0[nr].x; // P_expl(m = 0) = -usom
+    const double cfac = 0.5 * p.ek * p.CorFac * p.ekpump[nr];
+    for (int t = tid; t < nin * nm; t += nb) {
+        const int tr = t / nm, i = t - tr * nm;
+        const cd zk = Z[tr * n + i], zc = cconj(Z[tr * n + (i == 0 ? 0 : n - i)]);
+        cd a = mk(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+        cd b = mk(0.5 * (zk.y - zc.y), -0.5 * (zk.x - zc.x));
+        if (i == 0) { a = mk(a.x, 0.0); b = mk(b.x, 0.0); }
+        else { a = cscale(2.0, a); b = cscale(2.0, b); }
+        const size_t q = (size_t)i * N + nr;
+        if (tr == 0) {
+            if (i == 0) b = mk(b.x + lsom * cfac * spec[2 * nm].x, b.y);
+            else a = cadd(a, cscale(2.0 * lsom, spec[2 * nm + i]));
+            L.us[q] = cadd(L.us[q], a);
+            L.up[q] = cadd(L.up[q], b);
+        } else if (tr == 1) {
+            if (i == 0) a = mk(a.x + lsom * cfac * spec[nm].x, a.y);
+            else a = cadd(a, cscale(2.0 * lsom, spec[i]));
+            L.om[q] = cadd(L.om[q], a);
+            if (hT) L.temp[q] = cadd(L.temp[q], b);
+        } else {
+            L.xi[q] = cadd(L.xi[q], a);
+        }
+    }
+}
+
+// adjoint seeds of the objectives at the state after a step
+// NuTop: lam(T, m = 0) += w lnu (real)
+__global__ void __launch_bounds__(256) k_adj_seed_nu(const PzDev *__restrict__ plam, const PzAdjDev *__restrict__ pad,
+                                                     double w)
+{
+    const PzDev &L = *plam;
+    for (int k = threadIdx.x; k < L.N; k += blockDim.x) L.temp[k] = mk(L.temp[k].x + w * pad->lnu[k], L.temp[k].y);
+}
+// weights on T: lam(T) += W (device array in the field layout)
+__global__ void __launch_bounds__(256) k_adj_seed_w(const PzDev *__restrict__ plam, const cd *__restrict__ W)
+{
+    const PzDev &L = *plam;
+    const size_t F = (size_t)L.nm * L.N;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < F; q += (size_t)gridDim.x * blockDim.x)
+        L.temp[q] = cadd(L.temp[q], W[q]);
+}
+// forward value of the objective, accumulated over the steps in acc[0]:
+// NuTop: sum_k lnu_k Re T(m = 0, r_k); weights: sum Re(conj(W) T)
+__global__ void __launch_bounds__(256) k_obj_acc(const PzDev *__restrict__ pd, const PzAdjDev *__restrict__ pad,
+                                                 const cd *__restrict__ W, double *acc)
+{
+    __shared__ double red[8];
+    const PzDev &p = *pd;
+    double v = 0.0;
+    if (W) {
+        const size_t F = (size_t)p.nm * p.N;
+        for (size_t q = threadIdx.x; q < F; q += blockDim.x) v += W[q].x * p.temp[q].x + W[q].y * p.temp[q].y;
+    } else {
+        for (int k = threadIdx.x; k < p.N; k += blockDim.x) v += pad->lnu[k] * p.temp[k].x;
+    }
+    v = block_reduce(v, red, 0);
+    if (threadIdx.x == 0) acc[0] += v;
+}
+
+// ---------------------------------------------------------------------------
+// Transpose of k_start_coll with respect to T: the initial T field feeds
+// temp, T_old(1), T_impl(1) and, with implicit buoyancy, P_impl(1).
+// grad = lam(temp) + lam(T_old) + L^T lam(T_impl) + buoyancy^T lam(P_impl).
+// LDS: 6 N complex.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_start_coll_adj(const PzDev *__restrict__ pd, const PzDev *__restrict__ plam,
+                                                        const PzAdjDev *__restrict__ pad, cd *grad)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pd;
+    const PzDev &L = *plam;
+    const PzAdjDev &A = *pad;
+    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi], tid = threadIdx.x, nb = blockDim.x;
+    const size_t off = (size_t)mi * N;
+    const RadView R(p.radc, N);
+    const double dm = (double)m, kT = p.TdiffFac * p.hdifT[mi];
+    cd *g = (cd *)smem, *a1 = g + N, *a2 = g + 2 * N, *c = g + 3 * N, *t = g + 4 * N;
+    for (int k = tid; k < N; k += nb) {
+        const cd v = L.T_impl[off + k];
+        cd gv = cadd(L.temp[off + k], L.T_old[off + k]);
+        gv = csub(gv, cscale(kT * dm * dm * R.or2(k), v));
+        if (p.l_buo_imp && m != 0) gv = cadd(gv, ci(dm * R.rgrav(k) * R.or1(k) * p.BuoFac, L.P_impl[off + k]));
+        g[k] = gv;
+        a1[k] = cscale(kT * R.or1(k), v);
+        a2[k] = cscale(kT, v);
+    }
+    __syncthreads();
+    matTr<1>(N, A.D1, VecSet<const cd>(a1), VecSet<cd>(c), false);
+    matTr<1>(N, A.D2, VecSet<const cd>(a2), VecSet<cd>(c), true);
+    __syncthreads();
+    matTr<1>(N, A.C, VecSet<const cd>(c), VecSet<cd>(t), false);
+    __syncthreads();
+    for (int k = tid; k < N; k += nb) grad[off + k] = cadd(g[k], t[k]);
+}

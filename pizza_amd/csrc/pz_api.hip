@@ -15,6 +15,7 @@
 #include "pz_mphase.h"
 #include "pz_coll.h"
 #include "pz_fd.h"
+#include "pz_adjoint.h"
 
 using pzh::ld;
 
@@ -168,6 +169,48 @@ bool make_scheme(int id, double courfac_nml, Scheme &s)
         if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Adjoint buffers of a handle (pz_adjoint_forward/backward): the probed
+// operators and scratch live as long as the handle; the snapshot blocks
+// are sized for the largest window so far.
+struct AdjCtx {
+    PzAdjDev ad{};
+    PzAdjDev *dad = nullptr; // device copy of ad
+    PzDev *dlam = nullptr;   // device view of the adjoint variables
+    char *lam = nullptr;     // adjoint variables (state layout)
+    double *acc = nullptr;   // objective sum
+    cd *W = nullptr, *grad = nullptr;
+    // window: nseg checkpoints every ck steps, ck + 1 segment snapshots
+    // and their ck forward views, the state at the end of the window
+    char *ckpt = nullptr, *seg = nullptr, *fin = nullptr;
+    PzDev *dviews = nullptr;
+    int cap_ckpt = 0, cap_ck = 0;
+    int objective = -1, K = 0, ck = 0, nseg = 0;
+    bool ready = false; // a forward sweep waits for its backward sweep
+    // copies of T_inv, X_inv, U_inv, PC_inv (steady, start-up)
+    std::vector<double *> mats_b, mats_s;
+    std::vector<size_t> mat_bytes;
+    // the transposed steps run on s2 beside the replay of the previous
+    // segment on the handle's stream; ev[0..1]: replay of buffer b done,
+    // ev[2..3]: adjoint of buffer b done
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev[4] = {};
+    void release()
+    {
+        for (void *q : {(void *)ckpt, (void *)seg, (void *)fin, (void *)dviews})
+            if (q) (void)hipFree(q);
+        ckpt = seg = fin = nullptr;
+        dviews = nullptr;
+        cap_ckpt = cap_ck = 0;
+    }
+    ~AdjCtx()
+    {
+        release();
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (s2) (void)hipStreamDestroy(s2);
+    }
+};
+
 } // namespace
 
 struct pz_handle {
@@ -190,6 +233,9 @@ struct pz_handle {
     size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0;
     bool radial_big = false; // k_radial_loop_big for long azimuthal grids
     double dtcond_top = 0.0, dtcond_bot = 0.0; // dT_cond/dr at the walls (Nusselt numbers)
+    char *arena = nullptr; // step state block (arena_layout)
+    size_t arena_bytes = 0;
+    AdjCtx *adj = nullptr; // adjoint buffers, made on first use
     int inv_waves = 16;
     int mthreads = PZ_MPHASE_THREADS; // threads of k_mphase / k_mcoll
     PzDev *ddev = nullptr;
@@ -211,6 +257,7 @@ struct pz_handle {
     }
     ~pz_handle()
     {
+        delete adj;
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void *p : allocs) (void)hipFree(p);
@@ -537,7 +584,9 @@ static void allow_full_lds(int lim)
                              (const void *)k_start_coll, (const void *)k_build_lhs_batch,
                              (const void *)k_mphase_batch, (const void *)k_radial_loop_batch,
                              (const void *)k_build_inv_batch, (const void *)k_mcoll_batch, (const void *)k_fd_build,
-                             (const void *)k_fd_mloop, (const void *)k_fd_start, (const void *)k_radial_loop_big};
+                             (const void *)k_fd_mloop, (const void *)k_fd_start, (const void *)k_radial_loop_big,
+                             (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
+                             (const void *)k_start_coll_adj};
     for (const void *k : kernels) {
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, k));
@@ -758,14 +807,13 @@ static void setup(pz_handle *h)
     d.botxi = h->upload(bc_pattern(h, c.t_bot, c.t_top, false));
     // fields and time arrays
     const size_t F = (size_t)h->nm * N;
-    d.us = h->dalloc<cd>(F); d.up = h->dalloc<cd>(F); d.om = h->dalloc<cd>(F); d.psi = h->dalloc<cd>(F);
-    d.temp = h->dalloc<cd>(F); d.xi = h->dalloc<cd>(F); d.temp_hat = h->dalloc<cd>(F);
-    d.xi_hat = h->dalloc<cd>(F); d.psi_hat = h->dalloc<cd>(F); d.dtemp = h->dalloc<cd>(F);
-    d.dVsT = h->dalloc<cd>(F); d.dVsXi = h->dalloc<cd>(F); d.dVsOm = h->dalloc<cd>(F);
+    // the state a step carries to the next (fields, time arrays, scheme
+    // state) lives in one block, so that a snapshot is one copy
     const Scheme &s = h->sch;
-    d.T_old = h->dalloc<cd>(F * s.nold); d.T_impl = h->dalloc<cd>(F * s.nimp); d.T_expl = h->dalloc<cd>(F * s.nexp);
-    d.X_old = h->dalloc<cd>(F * s.nold); d.X_impl = h->dalloc<cd>(F * s.nimp); d.X_expl = h->dalloc<cd>(F * s.nexp);
-    d.P_old = h->dalloc<cd>(F * s.nold); d.P_impl = h->dalloc<cd>(F * s.nimp); d.P_expl = h->dalloc<cd>(F * s.nexp);
+    h->arena_bytes = arena_layout(d, nullptr, F, s.nold, s.nimp, s.nexp);
+    h->arena = h->dalloc<char>(h->arena_bytes);
+    arena_layout(d, h->arena, F, s.nold, s.nimp, s.nexp);
+    d.dVsT = h->dalloc<cd>(F); d.dVsXi = h->dalloc<cd>(F); d.dVsOm = h->dalloc<cd>(F);
     d.dtr = h->dalloc<double>(N);
     d.dth = h->dalloc<double>(N);
     // factorised matrices
@@ -818,7 +866,6 @@ static void setup(pz_handle *h)
         d.coll_ws = (2 * (size_t)N * 2 * N * sizeof(double) + (size_t)2 * N * 4 * N * sizeof(cd) + 255) & ~(size_t)255;
         d.coll_work = h->dalloc<char>((size_t)nm * d.coll_ws);
     }
-    d.ss = h->dalloc<PzScheme>(1);
     if (!d.coll && !d.fd) build_operators(h);
     // LDS budgets
     h->lds_radial = (5 * (size_t)h->nm + 6 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
@@ -1174,22 +1221,28 @@ static void launch_step(pz_handle *h)
     }
 }
 
+// nsteps steps through the captured step graph (captured on first use)
+static void run_steps(pz_handle *h, int nsteps)
+{
+    if (h->use_graph && !h->gexec) {
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        launch_step(h);
+        HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
+        HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < nsteps; i++) {
+        if (h->gexec) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+        else launch_step(h);
+    }
+    HIPCHK(hipGetLastError());
+}
+
 extern "C" int pz_step(pz_handle *h, int nsteps)
 {
     if (!h) return -1;
     if (!h->started) return fail(h, -5, "pz_start has not been called");
     try {
-        if (h->use_graph && !h->gexec) {
-            HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            launch_step(h);
-            HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
-            HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
-        }
-        for (int i = 0; i < nsteps; i++) {
-            if (h->gexec) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
-            else launch_step(h);
-        }
-        HIPCHK(hipGetLastError());
+        run_steps(h, nsteps);
     } catch (std::exception &e) {
         return fail(h, -4, e.what());
     }
@@ -1386,6 +1439,278 @@ extern "C" int pz_nusselt(pz_handle *h, double out[2])
     auto round_off = [&](double v) { return std::fabs(v) < tiny ? 0.0 : v; };
     out[0] = round_off(std::fabs(h->dtcond_top) > tiny ? 1.0 + dcol[0] / h->dtcond_top : 1.0);
     out[1] = round_off(std::fabs(h->dtcond_bot) > tiny ? 1.0 + dcol[2 * (N - 1)] / h->dtcond_bot : 1.0);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the step (no reference counterpart).  The forward sweep keeps a
+// snapshot of the step state every ck steps; the backward sweep replays each
+// segment from its snapshot, keeping the state before every step of it,
+// and runs the transposed step kernels backwards through the segment.
+// ---------------------------------------------------------------------------
+static void adj_init(pz_handle *h)
+{
+    if (h->adj) return;
+    AdjCtx *a = new AdjCtx();
+    h->adj = a;
+    const int N = h->N, nm = h->nm;
+    const size_t F = (size_t)nm * N, NN = (size_t)N * N;
+    PzAdjDev &ad = a->ad;
+    double *C = h->dalloc<double>(NN), *G = h->dalloc<double>(NN), *D1 = h->dalloc<double>(NN);
+    double *D2 = h->dalloc<double>(NN), *E1 = h->dalloc<double>(NN), *U = h->dalloc<double>(NN);
+    double *lnu = h->dalloc<double>(N);
+    ad.C = C; ad.G = G; ad.D1 = D1; ad.D2 = D2; ad.E1 = E1; ad.Uimp = U; ad.lnu = lnu;
+    ad.eT0 = h->dalloc<cd>(F); ad.aT0 = h->dalloc<cd>(F); ad.eX0 = h->dalloc<cd>(F);
+    ad.aX0 = h->dalloc<cd>(F); ad.eP0 = h->dalloc<cd>(F); ad.aP0 = h->dalloc<cd>(F);
+    ad.gtop = h->dalloc<cd>(nm); ad.gbot = h->dalloc<cd>(nm);
+    ad.gtopxi = h->dalloc<cd>(nm); ad.gbotxi = h->dalloc<cd>(nm);
+    ad.flag = h->dalloc<int>(1);
+    {
+        // copies of the inverses: S = those of the scheme's own weights,
+        // B = those of the CNAB2 start-up steps of a bridged scheme
+        const size_t ti = (size_t)nm * N * h->d.ldi, pc = (size_t)nm * N * 2 * N, ui = (size_t)N * h->d.ldi;
+        a->mat_bytes = {ti * sizeof(double), ti * sizeof(double), ui * sizeof(double), pc * sizeof(cd)};
+        a->mats_s = {h->dalloc<double>(ti), h->dalloc<double>(ti), h->dalloc<double>(ui), (double *)h->dalloc<cd>(pc)};
+        a->mats_b = h->d.bridge ? std::vector<double *>{h->dalloc<double>(ti), h->dalloc<double>(ti),
+                                                        h->dalloc<double>(ui), (double *)h->dalloc<cd>(pc)}
+                                : a->mats_s;
+        ad.T_inv_s = a->mats_s[0]; ad.X_inv_s = a->mats_s[1]; ad.U_inv_s = a->mats_s[2];
+        ad.PC_inv_s = (const cd *)a->mats_s[3];
+        ad.T_inv_b = a->mats_b[0]; ad.X_inv_b = a->mats_b[1]; ad.U_inv_b = a->mats_b[2];
+        ad.PC_inv_b = (const cd *)a->mats_b[3];
+    }
+    HIPCHK(hipStreamCreate(&a->s2));
+    HIPCHK(hipEventCreateWithFlags(&a->ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&a->ev[1], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&a->ev[2], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&a->ev[3], hipEventDisableTiming));
+    a->dad = h->upload(std::vector<PzAdjDev>{ad});
+    a->lam = h->dalloc<char>(h->arena_bytes);
+    PzDev lv = h->d;
+    arena_layout(lv, a->lam, F, h->sch.nold, h->sch.nimp, h->sch.nexp);
+    lv.ss = h->d.ss;
+    a->dlam = h->upload(std::vector<PzDev>{lv});
+    a->acc = h->dalloc<double>(1);
+    a->W = h->dalloc<cd>(F);
+    a->grad = h->dalloc<cd>(F);
+    const size_t lds = (6 * (size_t)N + 6 * (size_t)h->d.L + 16) * sizeof(cd) + 3 * (size_t)N * sizeof(double);
+    hipLaunchKernelGGL(k_adj_probe, dim3(N), dim3(256), lds, h->stream, h->d, C, G, D1, D2, E1, U, lnu);
+    HIPCHK(hipGetLastError());
+}
+
+// snapshot blocks and views for a window of K steps, checkpoints every ck
+static void adj_window(pz_handle *h, int K, int ck)
+{
+    AdjCtx *a = h->adj;
+    const size_t A = h->arena_bytes, F = (size_t)h->nm * h->N;
+    const int nseg = (K + ck - 1) / ck;
+    if (nseg > a->cap_ckpt || ck > a->cap_ck) {
+        a->release();
+        HIPCHK(hipMalloc(&a->ckpt, A * nseg));
+        HIPCHK(hipMalloc(&a->seg, A * 2 * (ck + 1)));
+        HIPCHK(hipMalloc(&a->fin, A));
+        HIPCHK(hipMalloc(&a->dviews, sizeof(PzDev) * 2 * ck));
+        a->cap_ckpt = nseg;
+        a->cap_ck = ck;
+        // two segment buffers of ck + 1 snapshots; view j of buffer b: the
+        // state before step j of the segment, with the scheme state the
+        // step ran with (the one stored before step j + 1)
+        std::vector<PzDev> v(2 * ck, h->d);
+        for (int b = 0; b < 2; b++)
+            for (int j = 0; j < ck; j++) {
+                PzDev next = h->d;
+                char *base = a->seg + A * (b * (ck + 1) + j);
+                arena_layout(v[b * ck + j], base, F, h->sch.nold, h->sch.nimp, h->sch.nexp);
+                arena_layout(next, base + A, F, h->sch.nold, h->sch.nimp, h->sch.nexp);
+                v[b * ck + j].ss = next.ss;
+            }
+        HIPCHK(hipMemcpy(a->dviews, v.data(), sizeof(PzDev) * 2 * ck, hipMemcpyHostToDevice));
+    }
+    a->K = K;
+    a->ck = ck;
+    a->nseg = nseg;
+}
+
+// copies the live inverses to a set (to_live = false) or back
+static void adj_mats(pz_handle *h, const std::vector<double *> &set, bool to_live)
+{
+    double *live[4] = {h->d.T_inv, h->d.X_inv, h->d.U_inv, (double *)h->d.PC_inv};
+    for (int i = 0; i < 4; i++)
+        HIPCHK(hipMemcpyAsync(to_live ? live[i] : set[i], to_live ? set[i] : live[i], h->adj->mat_bytes[i],
+                              hipMemcpyDeviceToDevice, h->stream));
+}
+
+static int adj_check(pz_handle *h)
+{
+    if (!h->started) return fail(h, -5, "pz_start has not been called");
+    if (!h->d.coll || h->d.fd || h->sch.family != 0)
+        return fail(h, -2, "the adjoint covers the collocation method with a multistep time scheme");
+    if (h->c.l_rerror_fix) return fail(h, -2, "the adjoint needs l_rerror_fix = .false. (thresholds are not linear)");
+    if (!h->l_heat) return fail(h, -2, "the adjoint objectives need the temperature equation");
+    if (h->nphi < 2 * h->nm) return fail(h, -2, "the adjoint needs n_phi >= 2 n_m");
+    return 0;
+}
+
+extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *weights, int nsteps, int ckpt_every,
+                                  double *J)
+{
+    if (!h || !J || nsteps < 1 || ckpt_every < 1) return -1;
+    if (objective != PZ_OBJ_NUTOP && objective != PZ_OBJ_TEMP) return fail(h, -1, "unknown objective");
+    if (objective == PZ_OBJ_TEMP && !weights) return fail(h, -1, "the temperature objective needs weights");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    try {
+        PzScheme s0;
+        HIPCHK(hipMemcpyAsync(&s0, h->d.ss, sizeof(s0), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (s0.n_steps != 0) return fail(h, -5, "the adjoint window starts right after pz_start");
+        const size_t lds_radj = (11 * (size_t)h->nm + 9 * (size_t)h->nphi) * sizeof(cd);
+        const size_t lds_madj = ADJ_NVEC * (size_t)h->N * sizeof(cd);
+        int dev = 0, lim = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (lds_radj > (size_t)lim || lds_madj > (size_t)lim)
+            return fail(h, -2, "the adjoint kernels need more LDS than a workgroup has at this size");
+        allow_full_lds(lim);
+        adj_init(h);
+        adj_window(h, nsteps, ckpt_every);
+        AdjCtx *a = h->adj;
+        a->objective = objective;
+        a->ready = false;
+        const size_t A = h->arena_bytes;
+        HIPCHK(hipMemsetAsync(a->acc, 0, sizeof(double), h->stream));
+        if (objective == PZ_OBJ_TEMP) {
+            std::vector<double2> w;
+            to_device_layout(h, weights, w);
+            HIPCHK(hipMemcpyAsync(a->W, w.data(), w.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+        }
+        for (int sg = 0; sg < a->nseg; sg++) {
+            HIPCHK(hipMemcpyAsync(a->ckpt + A * sg, h->arena, A, hipMemcpyDeviceToDevice, h->stream));
+            const int len = std::min(a->ck, nsteps - sg * a->ck);
+            for (int i = 0; i < len; i++) {
+                run_steps(h, 1);
+                if (h->d.bridge && sg * a->ck + i == 0) adj_mats(h, a->mats_b, false);
+                if (objective == PZ_OBJ_NUTOP)
+                    hipLaunchKernelGGL(k_obj_acc, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev,
+                                       (const PzAdjDev *)a->dad, (const cd *)nullptr, a->acc);
+            }
+        }
+        if (objective == PZ_OBJ_TEMP)
+            hipLaunchKernelGGL(k_obj_acc, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev,
+                               (const PzAdjDev *)a->dad, (const cd *)a->W, a->acc);
+        HIPCHK(hipMemcpyAsync(a->fin, h->arena, A, hipMemcpyDeviceToDevice, h->stream));
+        adj_mats(h, a->mats_s, false);
+        HIPCHK(hipGetLastError());
+        double acc = 0.0;
+        HIPCHK(hipMemcpyAsync(&acc, a->acc, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        rc = check_state(h, nullptr);
+        if (rc) return rc;
+        const double tiny = 10.0 * 2.220446049250313e-16;
+        if (objective == PZ_OBJ_NUTOP)
+            *J = std::fabs(h->dtcond_top) > tiny ? 1.0 + acc / ((double)nsteps * h->dtcond_top) : 1.0;
+        else
+            *J = acc;
+        a->ready = true;
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, double *grad_bc)
+{
+    if (!h) return -1;
+    if (!h->adj || !h->adj->ready) return fail(h, -5, "pz_adjoint_backward needs a pz_adjoint_forward first");
+    try {
+        AdjCtx *a = h->adj;
+        const size_t A = h->arena_bytes, F = (size_t)h->nm * h->N;
+        const int K = a->K, ck = a->ck, nm = h->nm, N = h->N;
+        const size_t lds_radj = (11 * (size_t)nm + 9 * (size_t)h->nphi) * sizeof(cd);
+        const size_t lds_madj = ADJ_NVEC * (size_t)N * sizeof(cd);
+        const double tiny = 10.0 * 2.220446049250313e-16;
+        double wnu = 0.0;
+        if (a->objective == PZ_OBJ_NUTOP && std::fabs(h->dtcond_top) > tiny) wnu = dJ / ((double)K * h->dtcond_top);
+        HIPCHK(hipMemsetAsync(a->lam, 0, A, h->stream));
+        for (cd *q : {a->ad.gtop, a->ad.gbot, a->ad.gtopxi, a->ad.gbotxi})
+            HIPCHK(hipMemsetAsync(q, 0, nm * sizeof(cd), h->stream));
+        HIPCHK(hipMemsetAsync(a->ad.flag, 0, sizeof(int), h->stream));
+        if (a->objective == PZ_OBJ_TEMP && dJ != 1.0) {
+            // dJ W as the seed
+            std::vector<double2> w(F);
+            HIPCHK(hipMemcpyAsync(w.data(), a->W, F * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            for (auto &v : w) { v.x *= dJ; v.y *= dJ; }
+            HIPCHK(hipMemcpyAsync(a->grad, w.data(), F * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+        } else if (a->objective == PZ_OBJ_TEMP) {
+            HIPCHK(hipMemcpyAsync(a->grad, a->W, F * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+        }
+        // replay of segment sg into buffer sg % 2 on the handle's stream
+        auto replay = [&](int sg) {
+            const int len = std::min(ck, K - sg * ck), c = sg * ck, b = sg % 2;
+            char *seg = a->seg + A * (size_t)(b * (ck + 1));
+            HIPCHK(hipMemcpyAsync(h->arena, a->ckpt + A * sg, A, hipMemcpyDeviceToDevice, h->stream));
+            // a replay from inside the start-up needs its matrices (the
+            // first step and the end of the start-up rebuild them)
+            if (h->d.bridge && c >= 1 && c <= h->sch.nold - 2) adj_mats(h, a->mats_b, true);
+            if (h->d.bridge && c >= h->sch.nold) adj_mats(h, a->mats_s, true);
+            for (int j = 0; j < len; j++) {
+                HIPCHK(hipMemcpyAsync(seg + A * j, h->arena, A, hipMemcpyDeviceToDevice, h->stream));
+                run_steps(h, 1);
+            }
+            HIPCHK(hipMemcpyAsync(seg + A * len, h->arena, A, hipMemcpyDeviceToDevice, h->stream));
+            HIPCHK(hipEventRecord(a->ev[b], h->stream));
+        };
+        // transposed steps of segment sg on s2, last step first
+        auto adjoint = [&](int sg) {
+            const int len = std::min(ck, K - sg * ck), b = sg % 2;
+            HIPCHK(hipStreamWaitEvent(a->s2, a->ev[b], 0));
+            for (int j = len - 1; j >= 0; j--) {
+                const int n = sg * ck + j + 1; // the state after step n seeds the objective
+                const PzDev *fw = a->dviews + b * ck + j;
+                if (a->objective == PZ_OBJ_NUTOP)
+                    hipLaunchKernelGGL(k_adj_seed_nu, dim3(1), dim3(256), 0, a->s2, (const PzDev *)a->dlam,
+                                       (const PzAdjDev *)a->dad, wnu);
+                else if (n == K)
+                    hipLaunchKernelGGL(k_adj_seed_w, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, a->s2,
+                                       (const PzDev *)a->dlam, (const cd *)a->grad);
+                hipLaunchKernelGGL(k_mcoll_adj, dim3(nm), dim3(512), lds_madj, a->s2, fw, (const PzDev *)a->dlam,
+                                   (const PzAdjDev *)a->dad);
+                hipLaunchKernelGGL(k_radial_adj, dim3(N), dim3(256), lds_radj, a->s2, fw, (const PzDev *)a->dlam,
+                                   (const PzAdjDev *)a->dad);
+            }
+            HIPCHK(hipEventRecord(a->ev[2 + b], a->s2));
+        };
+        // the adjoint variables and seeds are set on the handle's stream
+        HIPCHK(hipEventRecord(a->ev[2], h->stream));
+        HIPCHK(hipStreamWaitEvent(a->s2, a->ev[2], 0));
+        replay(a->nseg - 1);
+        for (int sg = a->nseg - 1; sg >= 0; sg--) {
+            if (sg >= 1) {
+                // buffer (sg - 1) % 2 was last read by the adjoint of sg + 1
+                if (sg + 1 < a->nseg) HIPCHK(hipStreamWaitEvent(h->stream, a->ev[2 + (sg + 1) % 2], 0));
+                replay(sg - 1);
+            }
+            adjoint(sg);
+        }
+        HIPCHK(hipStreamWaitEvent(h->stream, a->ev[2], 0));
+        hipLaunchKernelGGL(k_start_coll_adj, dim3(nm), dim3(256), 6 * (size_t)N * sizeof(cd), h->stream,
+                           (const PzDev *)h->ddev, (const PzDev *)a->dlam, (const PzAdjDev *)a->dad, a->grad);
+        HIPCHK(hipMemcpyAsync(h->arena, a->fin, A, hipMemcpyDeviceToDevice, h->stream));
+        adj_mats(h, a->mats_s, true);
+        HIPCHK(hipGetLastError());
+        int flag = 0;
+        std::vector<double2> g(F), bc(2 * (size_t)nm);
+        HIPCHK(hipMemcpyAsync(&flag, a->ad.flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(g.data(), a->grad, F * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(bc.data(), a->ad.gtop, nm * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(bc.data() + nm, a->ad.gbot, nm * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (flag) return fail(h, -7, "dt or the implicit matrices changed inside the adjoint window");
+        if (grad_temp) from_device_layout(h, g, grad_temp);
+        if (grad_bc) std::memcpy(grad_bc, bc.data(), bc.size() * sizeof(double2));
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
     return 0;
 }
 

@@ -140,6 +140,31 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
     for (int s = 0; s < nrad; s++) {
         const int R = radices[s];
         const int nb = n / R;
+        if (R != 2 && R != 3 && R != 4 && R != 8) {
+            // other radices: a direct DFT of length R per output, one output
+            // per item so that all n outputs of a stage run in parallel; the
+            // stage twiddle and the DFT kernel combine into one table entry,
+            // exp(dir 2 pi i (r tstep + ((r k) mod R) n/R) / n)
+            const int q = n / R;
+            for (int t = threadIdx.x; t < n * ntr; t += blockDim.x) {
+                const int tr = t / n, rem = t - tr * n, k = rem / nb, j = rem - k * nb;
+                const cd *x = a + tr * n;
+                const int jm = j % Ns, tstep = (n / (Ns * R)) * jm;
+                cd acc = mk(0.0, 0.0);
+                for (int r = 0; r < R; r++) {
+                    int e = r * tstep + ((r * k) % R) * q;
+                    if (e >= n) e -= n;
+                    acc = cadd(acc, cmul(x[j + r * nb], tw(e)));
+                }
+                b[tr * n + (j / Ns) * Ns * R + jm + k * Ns] = acc;
+            }
+            __syncthreads();
+            cd *t = a;
+            a = b;
+            b = t;
+            Ns *= R;
+            continue;
+        }
         const int tot = nb * ntr;
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
             int tr = t / nb, j = t - tr * nb;
@@ -201,17 +226,6 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
                 y[idxD] = cadd(v0, t1);
                 y[idxD + Ns] = cadd(m0, t2);
                 y[idxD + 2 * Ns] = csub(m0, t2);
-            } else {
-                // generic radix: direct DFT of length R, inputs re-read from LDS
-                int q = n / R;
-                for (int k = 0; k < R; k++) {
-                    cd acc = mk(0.0, 0.0);
-                    for (int r = 0; r < R; r++) {
-                        cd v = cmul(x[j + r * nb], tw(r * tstep));
-                        acc = cadd(acc, cmul(v, tw(((r * k) % R) * q)));
-                    }
-                    y[idxD + k * Ns] = acc;
-                }
             }
         }
         __syncthreads();
@@ -335,7 +349,7 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
     }
     // nonlinear products (radial_loop.f90:316-325): (upX + i r us X)
     const double rr = p.r[nr];
-    const int nout = p.l_chem ? 3 : (p.l_heat ? 2 : 1);
+    const int nout = 1 + (p.l_heat ? 1 : 0) + (p.l_chem ? 1 : 0); // omega, then T and xi as present
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         cd z0 = g[j], z1 = g[n + j];
         double us = z0.x, up = z0.y, om = z1.x, T = z1.y;

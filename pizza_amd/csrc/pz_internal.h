@@ -129,6 +129,28 @@ struct PzDev {
     PzScheme *ss;
 };
 
+// Carves the step state out of one block at base (base = nullptr: only
+// sizes it) and returns its size in bytes: the fields, the time-scheme
+// slots and the scheme state, each at a 256-byte boundary.  Every block of
+// this size (snapshots, adjoint variables) shares the layout.
+inline size_t arena_layout(PzDev &d, char *base, size_t F, int nold, int nimp, int nexp)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *q = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    const size_t fb = F * sizeof(cd);
+    cd **one[] = {&d.us, &d.up, &d.om, &d.psi, &d.temp, &d.xi, &d.temp_hat, &d.xi_hat, &d.psi_hat, &d.dtemp};
+    for (cd **q : one) *q = (cd *)take(fb);
+    cd **slot[] = {&d.T_old, &d.T_impl, &d.T_expl, &d.X_old, &d.X_impl, &d.X_expl, &d.P_old, &d.P_impl, &d.P_expl};
+    const int cnt[] = {nold, nimp, nexp, nold, nimp, nexp, nold, nimp, nexp};
+    for (int i = 0; i < 9; i++) *slot[i] = (cd *)take(fb * cnt[i]);
+    d.ss = (PzScheme *)take(sizeof(PzScheme));
+    return off;
+}
+
 // rows of the radial block: x, r, drx, ddrx, or1, or2, rgrav, beta, ekpump,
 // ekp_up, ekp_us, ekp_dusdp, delxr2, delxh2, tcond, dtcond, xicond, dxicond,
 // drb (3 rows), dbeta

@@ -59,6 +59,8 @@ def lib():
         L.pz_ensemble_destroy.argtypes = [vp]
         L.pz_ensemble_last_error.argtypes = [vp]
         L.pz_ensemble_last_error.restype = ctypes.c_char_p
+        L.pz_adjoint_forward.argtypes = [vp, ip, dp, ip, ip, dp]
+        L.pz_adjoint_backward.argtypes = [vp, ctypes.c_double, dp, dp]
         _LIB = L
     return _LIB
 
@@ -67,7 +69,9 @@ EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_
            "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step",
            "pz_time", "pz_ekin_3d", "pz_nusselt", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
            "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
-           "pz_ensemble_destroy", "pz_ensemble_last_error"]
+           "pz_ensemble_destroy", "pz_ensemble_last_error", "pz_adjoint_forward", "pz_adjoint_backward"]
+
+OBJECTIVES = {"nutop": 0, "temp": 2}
 
 
 def _dp(a):
@@ -143,6 +147,30 @@ class Run:
 
     def stream(self):
         return lib().pz_stream(self.h)
+
+    def adjoint_forward(self, objective, nsteps, ckpt_every=20, weights=None):
+        """Runs nsteps steps from the started state keeping snapshots every
+        ckpt_every steps; returns the objective J ('nutop': time-averaged
+        outer-wall Nusselt number; 'temp': sum Re(conj(weights) T) after the
+        last step, weights complex (n_m, n_r))."""
+        J = ctypes.c_double()
+        wp = None
+        if weights is not None:
+            self._wbuf = np.ascontiguousarray(np.asarray(weights, dtype=np.complex128).T)
+            wp = _dp(self._wbuf.view(np.float64))
+        self._check(lib().pz_adjoint_forward(self.h, OBJECTIVES[objective], wp, nsteps, ckpt_every,
+                                             ctypes.byref(J)), "pz_adjoint_forward")
+        return J.value
+
+    def adjoint_backward(self, dJ=1.0):
+        """dJ times the gradient of the last forward window's objective:
+        (d/dT of the temperature given to start(), complex (n_m, n_r);
+        d/d of the boundary values of T, complex (n_m, 2): outer, inner)."""
+        g = np.zeros((self.n_r, self.n_m), dtype=np.complex128)
+        bc = np.zeros((2, self.n_m), dtype=np.complex128)
+        self._check(lib().pz_adjoint_backward(self.h, float(dJ), _dp(g.view(np.float64)), _dp(bc.view(np.float64))),
+                    "pz_adjoint_backward")
+        return g.T.copy(), bc.T.copy()
 
     def time_kernel(self, which, reps):
         ms = ctypes.c_double()

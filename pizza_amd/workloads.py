@@ -140,6 +140,16 @@ def seed_member(run, cfg, k, amp_t=1.0e-3, seed=SEED):
     run.set_field("temp", seeded_field(n_m, r, r_icb, amp_t, seed + 1000 + k, 0))
 
 
+def c4_adjoint(n_r=65, m_max=85):
+    """The adjoint case: collocation, n_r 65, n_phi 256 (m_max 85), Ekman
+    1e-4, Ra 1e6, Pr 1, CNAB2, dtMax = Ek / 10; objective: the outer-wall
+    Nusselt number averaged over 400 steps, checkpoints every 20 steps."""
+    return default_config(
+        n_r_max=n_r, n_cheb_max=n_r, m_max=m_max, minc=1, time_scheme="CNAB2",
+        cheb_method="colloc", matrix_solve="direct", dtMax=1.0e-5,
+        ra=1.0e6, ek=1.0e-4, pr=1.0, radratio=0.35)
+
+
 def c5_fd(n_r=257, m_max=682, steps_scheme="SBDF3"):
     """The large finite-difference case: n_r 257 on the stretched grid,
     n_phi 2046 (m_max 682), Ekman 1e-6, Ra 1e9, Pr 0.1, SBDF3, dt 1e-7.

@@ -1,0 +1,194 @@
+"""The discrete adjoint of the step (pz_adjoint_forward/backward).
+
+The reference has no adjoint, so these tests pin it by properties:
+- over K = 2 steps the objective <W, T(2)> is a quadratic polynomial of the
+  initial temperature (the step is affine in T once the velocity is fixed,
+  and quadratic through the advection products), so the central difference
+  (J(T0 + h d) - J(T0 - h d)) / 2h equals <grad, d> up to rounding for any h:
+  a dot-product test of every transposed kernel (m-loop, radial loop,
+  start), tolerance 1e-9 relative;
+- the gradient does not depend on the checkpoint interval (the replayed
+  segments repeat the forward bit for bit);
+- the time-averaged Nusselt gradient over a longer window matches central
+  differences (step 1e-6 of the seeded amplitude) of the GPU forward and of
+  the oracle's forward, tolerance 1e-6 relative;
+- the torch.autograd binding returns the same gradient.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(cfg, seed=7, amp_u=1.0e-3):
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Run
+    run = Run(cfg)
+    r, r_icb = W.radial_grid(cfg)
+    rng = np.random.default_rng(seed)
+    prof = np.sin(np.pi * (r - r_icb))
+    n_m = run.n_m
+
+    def field(a):
+        g = rng.standard_normal(n_m) + 1j * rng.standard_normal(n_m)
+        return a * g[:, None] * prof[None, :]
+
+    T0 = field(1.0e-3)
+    us, up = field(amp_u), field(amp_u)
+    return run, T0, us, up, field
+
+
+def _J(run, T0, us, up, objective, K, W=None, ckpt=20):
+    run.set_field("us", us)
+    run.set_field("up", up)
+    run.set_field("temp", T0)
+    run.start()
+    return run.adjoint_forward(objective, K, ckpt, W)
+
+
+def _dot(a, b):
+    return float(np.sum(a.real * b.real + a.imag * b.imag))
+
+
+@pytest.mark.parametrize("scheme,amp_u", [("CNAB2", 1.0e-3), ("CNAB2", 5.0e-2), ("SBDF2", 1.0e-3)])
+def test_two_step_dot_product(scheme, amp_u):
+    from pizza_amd import workloads as W
+    from pizza_amd.config import scheme_id
+    cfg = W.c1_collocation()
+    cfg.time_scheme = scheme_id(scheme)
+    print(f"case: adjoint dot product {scheme} amp_u={amp_u}", flush=True)
+    run, T0, us, up, field = _setup(cfg, amp_u=amp_u)
+    rng = np.random.default_rng(11)
+    Wt = rng.standard_normal((run.n_m, run.n_r)) + 1j * rng.standard_normal((run.n_m, run.n_r))
+    J0 = _J(run, T0, us, up, "temp", 2, Wt)
+    g, bc = run.adjoint_backward()
+    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
+    for k in range(3):
+        d = field(1.0e-3)
+        h = 1.0
+        jp = _J(run, T0 + h * d, us, up, "temp", 2, Wt)
+        jm = _J(run, T0 - h * d, us, up, "temp", 2, Wt)
+        fd = (jp - jm) / (2 * h)
+        ad = _dot(g, d)
+        assert abs(fd - ad) <= 1e-9 * max(abs(fd), abs(ad)), (k, fd, ad, J0)
+
+
+def test_checkpoint_interval_invariance():
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    print("case: adjoint checkpoint intervals", flush=True)
+    run, T0, us, up, _ = _setup(cfg)
+    grads = []
+    for ck in (1, 3, 7):
+        J = _J(run, T0, us, up, "nutop", 7, ckpt=ck)
+        g, bc = run.adjoint_backward()
+        grads.append((J, g, bc))
+    for J, g, bc in grads[1:]:
+        assert J == grads[0][0]
+        assert np.array_equal(g, grads[0][1])
+        assert np.array_equal(bc, grads[0][2])
+
+
+def test_nusselt_objective_matches_nusselt_series():
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    run, T0, us, up, _ = _setup(cfg)
+    J = _J(run, T0, us, up, "nutop", 5, ckpt=2)
+    run.set_field("us", us)
+    run.set_field("up", up)
+    run.set_field("temp", T0)
+    run.start()
+    nu = []
+    for _ in range(5):
+        run.step(1)
+        nu.append(run.nusselt()[0])
+    assert abs(J - np.mean(nu)) < 1e-12
+
+
+def test_nusselt_gradient_vs_central_differences():
+    """dJ/dT0 of the Nusselt number averaged over 40 steps against central
+    differences of the GPU forward and of the oracle's forward."""
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    K = 40
+    print("case: adjoint Nusselt vs central differences", flush=True)
+    run, T0, us, up, field = _setup(cfg, amp_u=0.0)
+    zero = np.zeros_like(T0)
+    _J(run, T0, zero, zero, "nutop", K, ckpt=10)
+    g, _ = run.adjoint_backward()
+
+    def j_oracle(T):
+        o = Oracle(cfg)
+        o.set_field("temp", T)
+        o.start()
+        nu = []
+        for _ in range(K):
+            o.step(1)
+            nu.append(o.nusselt()[0])
+        return float(np.mean(nu))
+
+    for k in range(4):
+        d = field(1.0e-3)
+        h = 1.0e-6
+        fd_gpu = (_J(run, T0 + h * d, zero, zero, "nutop", K) - _J(run, T0 - h * d, zero, zero, "nutop", K)) / (2 * h)
+        ad = _dot(g, d)
+        assert abs(fd_gpu - ad) <= 1e-6 * abs(ad), (k, fd_gpu, ad)
+        if k < 2:
+            fd_or = (j_oracle(T0 + h * d) - j_oracle(T0 - h * d)) / (2 * h)
+            assert abs(fd_or - ad) <= 1e-6 * abs(ad), (k, fd_or, ad)
+
+
+def test_torch_autograd_binding():
+    import torch
+    from pizza_amd import workloads as W
+    from pizza_amd.autograd import window_objective
+    cfg = W.c1_collocation()
+    run, T0, us, up, _ = _setup(cfg, amp_u=0.0)
+    t = torch.tensor(T0, dtype=torch.complex128, requires_grad=True)
+    J = window_objective(t, run, "nutop", nsteps=6, ckpt_every=3)
+    J.backward()
+    run2, *_ = _setup(cfg, amp_u=0.0)
+    J2 = _J(run2, T0, np.zeros_like(T0), np.zeros_like(T0), "nutop", 6, ckpt=3)
+    g2, _ = run2.adjoint_backward()
+    assert float(J.detach()) == J2
+    assert rel(t.grad.numpy(), g2) < 1e-14
+
+
+@pytest.mark.parametrize("ktopt", [1, 2])
+def test_boundary_pattern_gradient(ktopt):
+    """Gradient with respect to the outer-wall pattern of T (fixed
+    temperature or fixed flux, horizontal.f90:149-173) over two steps:
+    exact central differences of the pattern coefficients (m = 2, 5)."""
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Run
+    print(f"case: adjoint boundary pattern ktopt={ktopt}", flush=True)
+    base = [(2, 0.05, 0.02), (5, 0.01, -0.03)]
+
+    def make(pattern):
+        cfg = W.c1_collocation()
+        cfg.ktopt = ktopt
+        for n, (m, tr, ti) in enumerate(pattern):
+            cfg.t_top[3 * n], cfg.t_top[3 * n + 1], cfg.t_top[3 * n + 2] = m, tr, ti
+        return cfg
+
+    run, T0, us, up, _ = _setup(make(base))
+    rng = np.random.default_rng(5)
+    Wt = rng.standard_normal((run.n_m, run.n_r)) + 1j * rng.standard_normal((run.n_m, run.n_r))
+    _J(run, T0, us, up, "temp", 2, Wt)
+    _, gbc = run.adjoint_backward()
+    for n, (m, tr, ti) in enumerate(base):
+        for part in (1, 2):
+            h = 0.01
+            js = []
+            for sgn in (1, -1):
+                pat = [list(e) for e in base]
+                pat[n][part] += sgn * h
+                r2 = Run(make(pat))
+                js.append(_J(r2, T0, us, up, "temp", 2, Wt))
+                r2.close()
+            fd = (js[0] - js[1]) / (2 * h)
+            ad = gbc[m, 0].real if part == 1 else gbc[m, 0].imag
+            assert abs(fd - ad) <= 1e-9 * max(abs(fd), 1e-30), (m, part, fd, ad)
