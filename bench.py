@@ -10,7 +10,8 @@ state resident in HBM.
 Prints one JSON line: steps/s, the roofline figure of the dominant kernel
 (the m-loop stage kernel k_mphase, timed with HIP events on the library
 stream), the ensemble throughput factor of the uneven 64-run collocation
-ensemble (batched launches versus the plain step run by run) and a CPU
+ensemble (batched launches versus the plain step run by run), the cost of
+the adjoint (forward and backward sweeps over the plain forward) and a CPU
 baseline of the oracle (single-thread C restatement) timed on a bounded
 sample of the same workload.
 """
@@ -153,6 +154,49 @@ def ensemble_bench(nruns=64, steps=100, warmup=3):
                         "Ra log-uniform [1e5,1e8], E in {1e-4,1e-5}, CNAB2, seeded T perturbation 1e-3"}
 
 
+def adjoint_bench(steps=400, ckpt=20, reps=2):
+    """Adjoint cost: the collocation CNAB2 case (n_r 65, n_phi 255, E 1e-4,
+    Ra 1e6), gradient of the outer-wall Nusselt number averaged over
+    `steps` steps with respect to the initial temperature, snapshots every
+    `ckpt` steps.  cost = (forward sweep + backward sweep) / plain forward
+    (pz_step over the same steps from the same state); best of `reps`."""
+    import numpy as np
+    cfg = W.c4_adjoint()
+    run = Run(cfg)
+    W.seed_member(run, cfg, 0)
+    T0 = run.get_field("temp")
+    z = np.zeros_like(T0)
+
+    def reset():
+        run.set_field("us", z)
+        run.set_field("up", z)
+        run.set_field("temp", T0)
+        run.start()
+
+    reset()
+    run.step(5)
+    run.time()
+    t_fwd = t_adj = float("inf")
+    for _ in range(reps):
+        reset()
+        t0 = time.perf_counter()
+        run.step(steps)
+        run.time()
+        t_fwd = min(t_fwd, time.perf_counter() - t0)
+    J = None
+    for _ in range(reps):
+        reset()
+        t0 = time.perf_counter()
+        J = run.adjoint_forward("nutop", steps, ckpt)
+        g, _ = run.adjoint_backward()
+        t_adj = min(t_adj, time.perf_counter() - t0)
+    run.close()
+    return {"workload": "colloc-CNAB2 n_r=65 n_phi=255 E=1e-4 Ra=1e6 Pr=1 dt=1e-5, d<NuTop>/dT0",
+            "steps": steps, "checkpoint_every": ckpt, "forward_ms": 1e3 * t_fwd,
+            "forward_plus_adjoint_ms": 1e3 * t_adj, "cost": t_adj / t_fwd, "objective": J,
+            "grad_max_abs": float(np.abs(g).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +207,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true", help="measure only the ensemble factor")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true", help="measure only the adjoint cost")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="run only the short sample profiled by the counter runs")
     args = ap.parse_args()
@@ -176,6 +222,9 @@ def main():
         return
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(steps=args.ensemble_steps)}))
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench()}))
         return
 
     cfg = W.c2_production()
@@ -227,6 +276,8 @@ def main():
     }
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(steps=args.ensemble_steps)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(line))

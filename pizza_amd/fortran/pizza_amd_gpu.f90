@@ -15,6 +15,7 @@ module pizza_amd_gpu
    integer(c_int), parameter :: PZ_ARS222 = 101, PZ_ARS443 = 102, PZ_BPR353 = 103, PZ_DP1_242 = 104, &
                                 PZ_DP2_242 = 105, PZ_LZ232 = 106, PZ_CK232 = 107, PZ_LZ453 = 108, PZ_PC2 = 109
    ! fields (fields.f90)
+   integer(c_int), parameter :: PZ_OBJ_NUTOP = 0, PZ_OBJ_TEMP = 2
    integer(c_int), parameter :: PZ_US = 0, PZ_UP = 1, PZ_OM = 2, PZ_PSI = 3, PZ_TEMP = 4, PZ_XI = 5, &
                                 PZ_TEMP_HAT = 6, PZ_XI_HAT = 7, PZ_PSI_HAT = 8, PZ_DTEMP = 9
 
@@ -180,6 +181,18 @@ contains
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: h
          real(c_double), intent(out) :: nu(2)
+      end function
+      integer(c_int) function pz_adjoint_forward(h, objective, weights, nsteps, ckpt_every, J) &
+            bind(c, name='pz_adjoint_forward')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, weights
+         integer(c_int), value :: objective, nsteps, ckpt_every
+         real(c_double), intent(out) :: J
+      end function
+      integer(c_int) function pz_adjoint_backward(h, dJ, grad_temp, grad_bc) bind(c, name='pz_adjoint_backward')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, grad_temp, grad_bc
+         real(c_double), value :: dJ
       end function
    end interface
       p = pz_last_error_c(h)
