@@ -41,7 +41,23 @@ enum pz_time_scheme {
     PZ_LZ232 = 106,
     PZ_CK232 = 107,
     PZ_LZ453 = 108,
-    PZ_PC2 = 109
+    PZ_PC2 = 109,
+    /* DIRK family with an assembly stage (dirk_schemes.f90:63-118, 183-230;
+       tables :299-392, 427-441, 526-711): the last stage only evaluates the
+       explicit terms and the step ends with assemble_stage (mloop.f90:105-160) */
+    PZ_SSP222 = 110,
+    PZ_LSDIRK222 = 111,
+    PZ_ARS232 = 112,
+    PZ_ARS233 = 113,
+    PZ_SSP322 = 114,
+    PZ_SSP332 = 115,
+    PZ_SSP433 = 116,
+    PZ_ARS343 = 117,
+    PZ_MARS343 = 118,
+    PZ_KC343 = 119,
+    PZ_KC564 = 120,
+    PZ_KC785 = 121,
+    PZ_DBM453 = 122
 };
 
 /* Field identifiers for pz_set_field / pz_get_field (fields.f90). */
@@ -205,6 +221,20 @@ int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble **e);
 int pz_ensemble_step(pz_ensemble *e, int nsteps);
 int pz_ensemble_destroy(pz_ensemble *e);
 const char *pz_ensemble_last_error(const pz_ensemble *e);
+/*
+ * The collocation m-loop shape of the classes with n_r > 65, picked by the
+ * ensemble's autotuner on its first pz_ensemble_step: it times a few steps
+ * of the whole ensemble under each policy from the same member states
+ * (restored after every trial) and keeps the fastest.  Policies: 0 the
+ * single-run shape (512 threads, radial rows in LDS), 1 256 threads
+ * (n_r <= 128), 2 radial rows read through L2, 3 both.  Policies 0 and 2
+ * give each member's pz_step results bit for bit; 1 and 3 sum the dense
+ * products in another order (rounding-level differences).  The environment
+ * variable PZ_ENSEMBLE_POLICY=k fixes the policy.  trial_ms (4 values, may
+ * be null) receives the timed step of each policy tried.  Returns 1 before
+ * the first step.
+ */
+int pz_ensemble_policy(const pz_ensemble *e, int *policy, double *trial_ms);
 
 /*
  * Discrete adjoint of the step.  The reference has no adjoint; the forward

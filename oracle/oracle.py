@@ -36,6 +36,10 @@ def lib():
         L.or_time.argtypes = [vp, dp, dp]
         L.or_ekin_3d.argtypes = [vp, dp]
         L.or_nusselt.argtypes = [vp, dp]
+        L.or_robin.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.or_ass_solve.argtypes = [vp, ip, dp, dp]
+        L.or_radial.argtypes = [vp, dp]
+        L.or_dirk_tables.argtypes = [ip, ctypes.POINTER(ip), ctypes.POINTER(ip), dp, dp, dp, dp, dp]
         L.or_costf.argtypes = [vp, dp, ip]
         L.or_get_dr.argtypes = [vp, dp, dp, ip, ip]
         L.or_get_ddr.argtypes = [vp, dp, dp, dp, ip, ip]
@@ -125,6 +129,28 @@ class Oracle:
         lib().or_nusselt(self.h, _dp(out))
         return out
 
+    def robin(self, at, bt, rtop, ab, bb, rbot, f):
+        """robin_bc on a complex column f (the boundary values are replaced)."""
+        g = np.ascontiguousarray(np.asarray(f, dtype=np.complex128).copy())
+        rt = np.array([complex(rtop).real, complex(rtop).imag])
+        rb = np.array([complex(rbot).real, complex(rbot).imag])
+        self._check(lib().or_robin(self.h, at, bt, _dp(rt), ab, bb, _dp(rb), _dp(g.view(np.float64))), "or_robin")
+        return g
+
+    def ass_solve(self, mi, omega):
+        """psi on the grid from omega through the assembly matrix of mode mi."""
+        w = np.ascontiguousarray(np.asarray(omega, dtype=np.complex128))
+        out = np.zeros(self.n_r, dtype=np.complex128)
+        self._check(lib().or_ass_solve(self.h, mi, _dp(w.view(np.float64)), _dp(out.view(np.float64))),
+                    "or_ass_solve")
+        return out
+
+    def radial(self):
+        """r, 1/r, 1/r^2, beta, dbeta/dr on the grid."""
+        out = np.zeros(5 * self.n_r)
+        lib().or_radial(self.h, _dp(out))
+        return out.reshape(5, self.n_r)
+
     def ekin_3d(self):
         out = np.zeros(4)
         lib().or_ekin_3d(self.h, _dp(out))
@@ -188,3 +214,14 @@ class Oracle:
         rc = lib().or_stencil(self.h, which, m, idx, ku, _dp(out))
         self._check(rc, "stencil")
         return out
+
+def dirk_tables(scheme):
+    """(nstages, l_assembly, wimp_lin0, A_imp, A_exp, b_imp, b_exp) unscaled."""
+    ns, la, w0 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    bi, be, bai, bae = np.zeros(64), np.zeros(64), np.zeros(8), np.zeros(8)
+    rc = lib().or_dirk_tables(scheme, ctypes.byref(ns), ctypes.byref(la), ctypes.byref(w0), _dp(bi), _dp(be),
+                              _dp(bai), _dp(bae))
+    if rc:
+        raise ValueError(f"scheme {scheme} is not a DIRK scheme")
+    n = ns.value
+    return n, la.value, w0.value, bi.reshape(8, 8)[:n, :n], be.reshape(8, 8)[:n, :n], bai[:n], bae[:n]

@@ -129,6 +129,8 @@ typedef struct {          /* time_schemes.f90:10-40 + children */
     double dt[8], wimp[8], wimp_lin[8], wexp[8];
     double bimp[8][8], bexp[8][8];   /* scaled Butcher tables (row, col) 0-based */
     double bimp0[8][8], bexp0[8][8]; /* unscaled */
+    double bass_imp[8], bass_exp[8];   /* assembly weights (dt-scaled) */
+    double bass_imp0[8], bass_exp0[8];
     double wimp_lin0;
     int l_exp_calc[8], l_imp_calc_rhs[8];
 } tscheme;
@@ -182,6 +184,9 @@ struct or_handle {
     double *dbeta;
     double *cM, *cdM, *cd2M; /* T_n(x_k), d/dr, d2/dr2 at point k: [k * N + n] */
     double *cTmat, *cXmat, *cTfac, *cXfac, *cUmat;
+    double *dr_top, *dr_bot; /* boundary-derivative rows (chebyshev.f90:338-360) */
+    cplx *cAss;              /* LU of the psi assembly matrices, [m][N][N] */
+    int *cAssPiv, lAss;
     int *cTpiv, *cXpiv, *cUpiv, *cPpiv;
     cplx *cPmat;
     double *cPfac;
@@ -1213,6 +1218,21 @@ static void ts_butcher(tscheme *t, int nst, const double *bi, const double *be)
     for (int s = 0; s < nst; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
 }
 
+/* tables of a DIRK scheme with an assembly stage: nst x nst Butcher tables
+ * and the assembly rows (dirk_schemes.f90:239-243: sizet = nstages) */
+static void ts_butcher_ass(tscheme *t, int nst, const double *bi, const double *be, const double *bai,
+                           const double *bae)
+{
+    t->nimp = t->nexp = t->nstages = nst;
+    for (int i = 0; i < nst; i++) {
+        for (int j = 0; j < nst; j++) { t->bimp0[i][j] = bi[i * nst + j]; t->bexp0[i][j] = be[i * nst + j]; }
+        t->bass_imp0[i] = bai[i];
+        t->bass_exp0[i] = bae[i];
+    }
+    for (int s = 0; s < nst; s++) { t->l_exp_calc[s] = 1; t->l_imp_calc_rhs[s] = 1; }
+    t->l_assembly = 1;
+}
+
 /* initialize (multistep_schemes.f90:35-125; dirk_schemes.f90:34-263) and
  * the DIRK tables of set_weights (dirk_schemes.f90:286-525) */
 static int ts_init(tscheme *t, int scheme, double courfac_nml)
@@ -1315,10 +1335,190 @@ static int ts_init(tscheme *t, int scheme, double courfac_nml)
             t->wimp_lin0 = 0.5; cf = 0.8;
             break;
         }
+        case PZ_SSP222: { /* :299-311 */
+            g = 1.0 - 1.0 / r2;
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - 2 * g, g}, be[9] = {0, 0, 0, 0, 0, 0, 0, 1, 0};
+            double ba[3] = {0, 0.5, 0.5};
+            ts_butcher_ass(t, 3, bi, be, ba, ba);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_LSDIRK222: { /* :312-326 */
+            g = 1.0 - 1.0 / r2;
+            double chat = 1.0 / (2.0 * g);
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - g, g}, be[9] = {0, 0, 0, 0, 0, 0, 0, chat, 0};
+            double ba[3] = {0, 1 - g, g};
+            ts_butcher_ass(t, 3, bi, be, ba, ba);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_ARS232: { /* :427-441 */
+            g = 0.5 * (2.0 - r2);
+            double del = -2.0 * r2 / 3.0;
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - g, g}, be[9] = {0, 0, 0, g, 0, 0, del, 1 - del, 0};
+            double ba[3] = {0, 1 - g, g};
+            ts_butcher_ass(t, 3, bi, be, ba, ba);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_ARS233: { /* :327-341 */
+            g = (3.0 + sqrt(3.0)) / 6.0;
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - 2 * g, g}, be[9] = {0, 0, 0, g, 0, 0, g - 1, 2 * (1 - g), 0};
+            double ba[3] = {0, 0.5, 0.5};
+            ts_butcher_ass(t, 3, bi, be, ba, ba);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_SSP322: { /* :342-356 */
+            double bi[16] = {0, 0, 0, 0, 0, 0.5, 0, 0, 0, -0.5, 0.5, 0, 0, 0, 0.5, 0.5};
+            double be[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0};
+            double ba[4] = {0, 0, 0.5, 0.5};
+            ts_butcher_ass(t, 4, bi, be, ba, ba);
+            t->wimp_lin0 = 0.5; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_SSP332: { /* :357-372 */
+            g = 1.0 - 1.0 / r2;
+            double bi[16] = {0, 0, 0, 0, 0, g, 0, 0, 0, 1 - 2 * g, g, 0, 0, 0.5 - g, 0, g};
+            double be[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0.25, 0.25, 0};
+            double ba[4] = {0, 1.0 / 6.0, 1.0 / 6.0, 2.0 * (1.0 / 3.0)};
+            ts_butcher_ass(t, 4, bi, be, ba, ba);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_SSP433: { /* :373-392; alpha, beta, eta are default-real
+                             (single precision) literals in the reference */
+            const double al = (double)0.24169426078821f, be_ = (double)0.06042356519705f;
+            const double et = (double)0.12915286960590f;
+            double bi[25] = {0, 0, 0, 0, 0, 0, al, 0, 0, 0, 0, -al, al, 0, 0,
+                             0, 0, 1 - al, al, 0, 0, be_, et, 0.5 - be_ - et - al, al};
+            double be[25] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                             0, 0, 1, 0, 0, 0, 0, 0.25, 0.25, 0};
+            double ba[5] = {0, 0, 1.0 / 6.0, 1.0 / 6.0, 2.0 * (1.0 / 3.0)};
+            ts_butcher_ass(t, 5, bi, be, ba, ba);
+            t->wimp_lin0 = al; t->l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_ARS343: { /* :526-545 */
+            g = 0.435866521508459;
+            const double b1 = -1.5 * g * g + 4.0 * g - 0.25, b2 = 1.5 * g * g - 5.0 * g + 1.25;
+            double bi[16] = {0, 0, 0, 0, 0, g, 0, 0, 0, 0.5 * (1 - g), g, 0, 0, b1, b2, g};
+            double be[16] = {0, 0, 0, 0, g, 0, 0, 0, 0.3212788860286278, 0.3966543747256017, 0, 0,
+                             -0.1058582960718797, 0.5529291480359398, 0.5529291480359398, 0};
+            double ba[4] = {0, b1, b2, g};
+            ts_butcher_ass(t, 4, bi, be, ba, ba);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 0.7;
+            break;
+        }
+        case PZ_MARS343: { /* :571-590 */
+            g = 0.435866521508458;
+            const double b1 = 1.20849664917601, b2 = -0.64436317068446;
+            double bi[16] = {0, 0, 0, 0, 0, g, 0, 0, 0, 0.5 * (1 - g), g, 0, 0, b1, b2, g};
+            double be[16] = {0, 0, 0, 0, 0.877173304301691, 0, 0, 0, 0.535396540307354, 0.182536720446875, 0, 0,
+                             0.63041255815287, -0.83193390106308, 1.20152134291021, 0};
+            double ba[4] = {0, b1, b2, g};
+            ts_butcher_ass(t, 4, bi, be, ba, ba);
+            t->wimp_lin0 = g; t->l_imp_calc_rhs[0] = 0; cf = 0.5;
+            break;
+        }
+        case PZ_KC343: { /* :546-570 */
+            g = 1767732205903.0 / 4055673282236.0;
+            const double a30 = 1471266399579.0 / 7840856788654.0, a31 = -4482444167858.0 / 7529755066697.0;
+            const double a32 = 11266239266428.0 / 11593286722821.0;
+            double bi[16] = {0, 0, 0, 0, g, g, 0, 0, 2746238789719.0 / 10658868560708.0,
+                             -640167445237.0 / 6845629431997.0, g, 0, a30, a31, a32, g};
+            double be[16] = {0, 0, 0, 0, 1767732205903.0 / 2027836641118.0, 0, 0, 0,
+                             5535828885825.0 / 10492691773637.0, 788022342437.0 / 10882634858940.0, 0, 0,
+                             6485989280629.0 / 16251701735622.0, -4246266847089.0 / 9704473918619.0,
+                             10755448449292.0 / 10357097424841.0, 0};
+            double ba[4] = {a30, a31, a32, g};
+            ts_butcher_ass(t, 4, bi, be, ba, ba);
+            t->wimp_lin0 = g; cf = 0.5;
+            break;
+        }
+        case PZ_KC564: { /* :591-625 */
+            double bi[36] = {0, 0, 0, 0, 0, 0,
+                             0.25, 0.25, 0, 0, 0, 0,
+                             8611.0 / 62500.0, -1743.0 / 31250.0, 0.25, 0, 0, 0,
+                             5012029.0 / 34652500.0, -654441.0 / 2922500.0, 174375.0 / 388108.0, 0.25, 0, 0,
+                             15267082809.0 / 155376265600.0, -71443401.0 / 120774400.0, 730878875.0 / 902184768.0,
+                             2285395.0 / 8070912.0, 0.25, 0,
+                             82889.0 / 524892.0, 0, 15625.0 / 83664.0, 69875.0 / 102672.0, -2260.0 / 8211.0, 0.25};
+            double be[36] = {0, 0, 0, 0, 0, 0,
+                             0.5, 0, 0, 0, 0, 0,
+                             13861.0 / 62500.0, 6889.0 / 62500.0, 0, 0, 0, 0,
+                             -116923316275.0 / 2393684061468.0, -2731218467317.0 / 15368042101831.0,
+                             9408046702089.0 / 11113171139209.0, 0, 0, 0,
+                             -451086348788.0 / 2902428689909.0, -2682348792572.0 / 7519795681897.0,
+                             12662868775082.0 / 11960479115383.0, 3355817975965.0 / 11060851509271.0, 0, 0,
+                             647845179188.0 / 3216320057751.0, 73281519250.0 / 8382639484533.0,
+                             552539513391.0 / 3454668386233.0, 3354512671639.0 / 8306763924573.0,
+                             4040.0 / 17871.0, 0};
+            double ba[6] = {82889.0 / 524892.0, 0, 15625.0 / 83664.0, 69875.0 / 102672.0, -2260.0 / 8211.0, 0.25};
+            ts_butcher_ass(t, 6, bi, be, ba, ba);
+            t->wimp_lin0 = 0.25; cf = 0.5;
+            break;
+        }
+        case PZ_DBM453: { /* :626-651 */
+            const double w = 0.32591194130117247;
+            double bi[25] = {0, 0, 0, 0, 0,
+                             -0.22284985318525410, w, 0, 0, 0,
+                             -0.46801347074080545, 0.86349284225716961, w, 0, 0,
+                             -0.46509906651927421, 0.81063103116959553, 0.61036726756832357, w, 0,
+                             0.87795339639076675, -0.72692641526151547, 0.75204137157372720, -0.22898029400415088, w};
+            double be[25] = {0, 0, 0, 0, 0,
+                             0.10306208811591838, 0, 0, 0, 0,
+                             -0.94124866143519894, 1.6626399742527356, 0, 0, 0,
+                             -1.3670975201437765, 1.3815852911016873, 1.2673234025619065, 0, 0,
+                             -0.81287582068772448, 0.81223739060505738, 0.90644429603699305, 0.094194134045674111, 0};
+            double ba[5] = {0.87795339639076672, -0.72692641526151549, 0.7520413715737272, -0.22898029400415090, w};
+            ts_butcher_ass(t, 5, bi, be, ba, ba);
+            t->wimp_lin0 = w; cf = 0.5;
+            break;
+        }
+        case PZ_KC785: { /* :652-711 */
+            const double w = 2.0 / 9.0;
+            const double c2 = 2366667076620.0 / 8822750406821.0, c3 = -257962897183.0 / 4451812247028.0;
+            const double c4 = -486229321650.0 / 11227943450093.0, c5 = 621307788657.0 / 4714163060173.0;
+            const double c6 = 2036305566805.0 / 6583108094622.0;
+            const double b2 = 3517720773327.0 / 20256071687669.0, b3 = 4569610470461.0 / 17934693873752.0;
+            const double b4 = 2819471173109.0 / 11655438449929.0, b5 = 3296210113763.0 / 10722700128969.0;
+            const double b6 = -1142099968913.0 / 5710983926999.0;
+            double bi[64] = {0, 0, 0, 0, 0, 0, 0, 0,
+                             w, w, 0, 0, 0, 0, 0, 0,
+                             c2, c2, w, 0, 0, 0, 0, 0,
+                             c3, c3, 128530224461.0 / 14379561246022.0, w, 0, 0, 0, 0,
+                             c4, c4, -225633144460.0 / 6633558740617.0, 1741320951451.0 / 6824444397158.0, w, 0, 0, 0,
+                             c5, c5, -125196015625.0 / 3866852212004.0, 940440206406.0 / 7593089888465.0,
+                             961109811699.0 / 6734810228204.0, w, 0, 0,
+                             c6, c6, -3039402635899.0 / 4450598839912.0, -1829510709469.0 / 31102090912115.0,
+                             -286320471013.0 / 6931253422520.0, 8651533662697.0 / 9642993110008.0, w, 0,
+                             0, 0, b2, b3, b4, b5, b6, w};
+            const double e7 = 2193717860234.0 / 3570523412979.0;
+            double be[64] = {0, 0, 0, 0, 0, 0, 0, 0,
+                             4.0 / 9.0, 0, 0, 0, 0, 0, 0, 0,
+                             1.0 / 9.0, 1183333538310.0 / 1827251437969.0, 0, 0, 0, 0, 0, 0,
+                             895379019517.0 / 9750411845327.0, 477606656805.0 / 13473228687314.0,
+                             -112564739183.0 / 9373365219272.0, 0, 0, 0, 0, 0,
+                             -4458043123994.0 / 13015289567637.0, -2500665203865.0 / 9342069639922.0,
+                             983347055801.0 / 8893519644487.0, 2185051477207.0 / 2551468980502.0, 0, 0, 0, 0,
+                             -167316361917.0 / 17121522574472.0, 1605541814917.0 / 7619724128744.0,
+                             991021770328.0 / 13052792161721.0, 2342280609577.0 / 11279663441611.0,
+                             3012424348531.0 / 12792462456678.0, 0, 0, 0,
+                             6680998715867.0 / 14310383562358.0, 5029118570809.0 / 3897454228471.0,
+                             2415062538259.0 / 6382199904604.0, -3924368632305.0 / 6964820224454.0,
+                             -4331110370267.0 / 15021686902756.0, -3944303808049.0 / 11994238218192.0, 0, 0,
+                             e7, e7, 5952760925747.0 / 18750164281544.0, -4412967128996.0 / 6196664114337.0,
+                             4151782504231.0 / 36106512998704.0, 572599549169.0 / 6265429158920.0,
+                             -457874356192.0 / 11306498036315.0, 0};
+            double ba[8] = {0, 0, b2, b3, b4, b5, b6, w};
+            ts_butcher_ass(t, 8, bi, be, ba, ba);
+            t->wimp_lin0 = w; cf = 0.5;
+            break;
+        }
         default:
             return -1;
         }
-        t->l_assembly = 0;
     }
     t->courfac = fabs(courfac_nml) >= 1.0e3 ? cf : courfac_nml;
     return 0;
@@ -1424,11 +1624,14 @@ static void ts_set_weights(tscheme *t, double alpha, int *lMatNext)
         if (t->wimp_lin[0] != wimp_old) *lMatNext = 1;
     } else {
         t->wimp_lin[0] = t->dt[0] * t->wimp_lin0;
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < 8; i++) {
             for (int j = 0; j < 8; j++) {
                 t->bimp[i][j] = t->dt[0] * t->bimp0[i][j];
                 t->bexp[i][j] = t->dt[0] * t->bexp0[i][j];
             }
+            t->bass_imp[i] = t->dt[0] * t->bass_imp0[i];
+            t->bass_exp[i] = t->dt[0] * t->bass_exp0[i];
+        }
     }
 }
 
@@ -1442,6 +1645,25 @@ static void ts_bridge(tscheme *t, double alpha)
     t->scheme = save;
     for (int i = 1; i < 8; i++) t->wimp[i] = 0.0;
     for (int i = 2; i < 8; i++) { t->wimp_lin[i] = 0.0; t->wexp[i] = 0.0; }
+}
+
+/* the unscaled tables of a DIRK scheme: nstages, whether it has an
+ * assembly stage, the Butcher tables (8 x 8 row-major) and the assembly
+ * rows (8); tests check them against the order conditions */
+int or_dirk_tables(int scheme, int *nstages, int *l_assembly, double *wimp_lin0, double *bimp, double *bexp,
+                   double *bass_imp, double *bass_exp)
+{
+    tscheme t;
+    if (ts_init(&t, scheme, 1.0e4) || t.family != 1) return -1;
+    *nstages = t.nstages;
+    *l_assembly = t.l_assembly;
+    *wimp_lin0 = t.wimp_lin0;
+    for (int i = 0; i < 8; i++) {
+        for (int j = 0; j < 8; j++) { bimp[i * 8 + j] = t.bimp0[i][j]; bexp[i * 8 + j] = t.bexp0[i][j]; }
+        bass_imp[i] = t.bass_imp0[i];
+        bass_exp[i] = t.bass_exp0[i];
+    }
+    return 0;
 }
 
 int or_scheme_weights(int scheme, double alpha, const double *dt, int *family, int *nstages,
@@ -1595,6 +1817,8 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->fd = cfg->l_finite_diff != 0;
     if (h->fd) h->coll = 0;
     if (ts_init(&h->ts, cfg->time_scheme, cfg->courfac)) { free(h); return -14; }
+    /* the assembly stage is restated for the collocation method only */
+    if (h->ts.l_assembly && !h->coll) { free(h); return -15; }
     /* namelists.f90:398-526 */
     h->l_heat = cfg->ra != 0.0;
     h->l_chem = cfg->raxi != 0.0;
@@ -2254,6 +2478,50 @@ static void coll_der_mat(or_handle *h)
         }
     }
 }
+/* the boundary first-derivative rows of the collocation method
+ * (chebyshev.f90:338-360): the flipped-row form at r_cmb, its mirror at
+ * r_icb, both times the mapping */
+static void coll_dr_rows(or_handle *h)
+{
+    int N = h->N;
+    h->dr_top = dalloc(N);
+    h->dr_bot = dalloc(N);
+    h->dr_top[0] = (2.0 * (N - 1) * (N - 1) + 1.0) / 6.0;
+    for (int k = 2; k <= N; k++) {
+        double s = sin(0.5 * (k - 1) * M_PI / (N - 1));
+        double diff = 2.0 * s * s;
+        h->dr_top[k - 1] = 2.0 * ((k % 2 == 0) ? -1.0 : 1.0) / diff;
+    }
+    h->dr_top[N - 1] *= 0.5;
+    for (int k = 0; k < N; k++) h->dr_bot[k] = -h->dr_top[N - 1 - k];
+    for (int k = 0; k < N; k++) h->dr_top[k] *= h->drx[0];
+    double db = h->drx[N - 1];
+    for (int k = 0; k < N; k++) h->dr_bot[k] *= db;
+}
+
+/* robin_bc (chebyshev.f90:385-445): the two boundary values of f from
+ * atop df/dr + btop f = rhs_top, abot df/dr + bbot f = rhs_bot */
+static void robin_bc(or_handle *h, double atop, double btop, cplx rhs_top, double abot, double bbot, cplx rhs_bot,
+                     cplx *f)
+{
+    int N = h->N;
+    const double *dt = h->dr_top, *db = h->dr_bot;
+    double str = 0, sti = 0, sbr = 0, sbi = 0;
+    for (int k = 1; k < N - 1; k++) {
+        str += dt[k] * creal(f[k]);
+        sti += dt[k] * cimag(f[k]);
+        sbr += db[k] * creal(f[k]);
+        sbi += db[k] * cimag(f[k]);
+    }
+    double den = abot * atop * db[0] * dt[N - 1] - (atop * dt[0] + btop) * (abot * db[N - 1] + bbot);
+    double vbr = (abot * db[0] * (creal(rhs_top) - atop * str) - (atop * dt[0] + btop) * (creal(rhs_bot) - abot * sbr)) / den;
+    double vbi = (abot * db[0] * (cimag(rhs_top) - atop * sti) - (atop * dt[0] + btop) * (cimag(rhs_bot) - abot * sbi)) / den;
+    double vtr = (creal(rhs_top) - atop * (str + dt[N - 1] * vbr)) / (atop * dt[0] + btop);
+    double vti = (cimag(rhs_top) - atop * (sti + dt[N - 1] * vbi)) / (atop * dt[0] + btop);
+    f[0] = vtr + I * vti;
+    f[N - 1] = vbr + I * vbi;
+}
+
 #define CM(k, n) h->cM[(size_t)(k) * N + (n)]
 #define CDM(k, n) h->cdM[(size_t)(k) * N + (n)]
 #define CD2M(k, n) h->cd2M[(size_t)(k) * N + (n)]
@@ -2400,6 +2668,7 @@ static void coll_alloc(or_handle *h)
         h->dbeta[n] = -(h->r_cmb * h->r_cmb + h->r[n] * h->r[n]) / (s2 * s2); /* radial.f90:222 */
     }
     coll_der_mat(h);
+    coll_dr_rows(h);
     h->cTmat = calloc(nm * N * N, sizeof(double)); h->cXmat = calloc(nm * N * N, sizeof(double));
     h->cTfac = dalloc(nm * N); h->cXfac = dalloc(nm * N);
     h->cTpiv = calloc(nm * N, sizeof(int)); h->cXpiv = calloc(nm * N, sizeof(int));
@@ -2664,6 +2933,149 @@ static void coll_update_psi(or_handle *h)
     free(rhs);
     free(u0);
     free(om0);
+}
+
+/* assemble_imex (dirk_schemes.f90:882-921): old + sum_s b_exp(s) expl(s)
+ * + sum_s b_imp(s) impl(s) over all stages */
+static void ts_assemble(or_handle *h, const tscheme *t, const tarray *a, cplx *rhs)
+{
+    size_t F = (size_t)h->nm * h->N;
+    for (size_t i = 0; i < F; i++) rhs[i] = a->old[i];
+    for (int s = 0; s < t->nstages; s++)
+        for (size_t i = 0; i < F; i++) rhs[i] += t->bass_exp[s] * a->expl[s * F + i];
+    for (int s = 0; s < t->nstages; s++)
+        for (size_t i = 0; i < F; i++) rhs[i] += t->bass_imp[s] * a->impl[s * F + i];
+}
+
+/* assemble_temp_coll / assemble_xi_coll (update_temp_coll.f90:238-307):
+ * interior grid values from the assembled right-hand side (real at m = 0),
+ * boundary values by robin_bc, then the first-stage implicit terms */
+static void coll_assemble_scalar(or_handle *h, int is_temp)
+{
+    int N = h->N;
+    tscheme *t = &h->ts;
+    tarray *a = is_temp ? &h->dTdt : &h->dxidt;
+    cplx *fld = is_temp ? h->temp : h->xi, *dfld = is_temp ? h->dtemp : h->dxi;
+    cplx *top = is_temp ? h->topt : h->topxi, *bot = is_temp ? h->bott : h->botxi;
+    int ktop = is_temp ? h->c.ktopt : h->c.ktopxi, kbot = is_temp ? h->c.kbott : h->c.kbotxi;
+    ts_assemble(h, t, a, h->work);
+    /* Dirichlet: (a, b) = (0, 1); Neumann: (1, 0) */
+    double at = ktop == 1 ? 0.0 : 1.0, bt = ktop == 1 ? 1.0 : 0.0;
+    double ab = kbot == 1 ? 0.0 : 1.0, bb = kbot == 1 ? 1.0 : 0.0;
+    for (int i = 0; i < h->nm; i++) {
+        cplx *f = fld + (size_t)i * N, *w = h->work + (size_t)i * N;
+        for (int k = 1; k < N - 1; k++) f[k] = h->idx2m[i] == 0 ? creal(w[k]) : w[k];
+        robin_bc(h, at, bt, top[i], ab, bb, bot[i], f);
+    }
+    double diff = is_temp ? h->TdiffFac : h->XidiffFac;
+    double *hdif = is_temp ? h->hdifT : h->hdifXi;
+    coll_rhs_imp_scalar(h, fld, dfld, a, 1, t->l_imp_calc_rhs[0], 0, diff, hdif);
+}
+
+/* get_assMat (update_psi_coll_smat.f90:839-917): psi from omega with the
+ * non-penetration and no-slip / free-slip rows, boundary_fac columns, LU */
+static int coll_ass_mat(or_handle *h)
+{
+    int N = h->N, nc = h->ncheb;
+    double rn = h->rnorm;
+    h->cAss = calloc((size_t)h->nm * N * N, sizeof(cplx));
+    h->cAssPiv = calloc((size_t)h->nm * N, sizeof(int));
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        if (m == 0) continue;
+        double dm2 = (double)m * (double)m;
+        cplx *A = h->cAss + (size_t)i * N * N;
+        for (int c = 0; c < N; c++) {
+            int in = c < nc;
+            A[0 + c * N] = in ? rn * CM(0, c) : 0.0;
+            A[(N - 1) + c * N] = in ? rn * CM(N - 1, c) : 0.0;
+            A[1 + c * N] = !in ? 0.0 : (h->c.ktopv == 1 ? rn * (CD2M(0, c) - h->or1[0] * CDM(0, c)) : rn * CDM(0, c));
+            A[(N - 2) + c * N] = !in ? 0.0
+                                     : (h->c.kbotv == 1 ? rn * (CD2M(N - 1, c) - h->or1[N - 1] * CDM(N - 1, c))
+                                                        : rn * CDM(N - 1, c));
+            for (int r = 2; r < N - 2; r++)
+                A[r + c * N] = -rn * (CD2M(r, c) + (h->or1[r] + h->beta[r]) * CDM(r, c) +
+                                      (h->or1[r] * h->beta[r] + h->dbeta[r] - dm2 * h->or2[r]) * CM(r, c));
+        }
+        for (int r = 0; r < N; r++) {
+            A[r] *= h->boundary_fac;
+            A[r + (size_t)(N - 1) * N] *= h->boundary_fac;
+        }
+        int info = 0;
+        getrf_c(N, A, h->cAssPiv + (size_t)i * N, &info);
+        if (info) return info;
+    }
+    h->lAss = 1;
+    return 0;
+}
+
+/* assemble_psi_coll_smat (update_psi_coll_smat.f90:529-672) */
+static int coll_assemble_psi(or_handle *h)
+{
+    int N = h->N;
+    size_t F = (size_t)h->nm * N;
+    tscheme *t = &h->ts;
+    tarray *a = &h->dpsidt;
+    if (!h->lAss) {
+        int info = coll_ass_mat(h);
+        if (info) return info;
+    }
+    ts_assemble(h, t, a, h->work);
+    double *u0 = dalloc(N), *om0 = dalloc(N);
+    cplx *rhs = calloc(N, sizeof(cplx)), *dpsi = calloc(F, sizeof(cplx)), *d2psi = calloc(F, sizeof(cplx));
+    int nm0 = -1;
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        cplx *w = h->work + (size_t)i * N;
+        if (m == 0) {
+            nm0 = i;
+            cplx *up0 = h->up + (size_t)i * N;
+            for (int k = 1; k < N - 1; k++) up0[k] = creal(w[k]);
+            int kt = h->c.ktopv, kb = h->c.kbotv;
+            if (kt != 1 && kb != 1) {
+                up0[0] = 0.0;
+                up0[N - 1] = 0.0;
+            } else if (kt == 1 && kb != 1) {
+                robin_bc(h, 1.0, -h->or1[0], 0.0, 0.0, 1.0, 0.0, up0);
+            } else if (kb == 1 && kt != 1) {
+                robin_bc(h, 0.0, 1.0, 0.0, 1.0, -h->or1[N - 1], 0.0, up0);
+            } else {
+                robin_bc(h, 1.0, -h->or1[0], 0.0, 1.0, -h->or1[N - 1], 0.0, up0);
+            }
+            continue;
+        }
+        rhs[0] = rhs[1] = rhs[N - 2] = rhs[N - 1] = 0.0;
+        for (int k = 2; k < N - 2; k++) rhs[k] = w[k];
+        getrs_c(N, h->cAss + (size_t)i * N * N, h->cAssPiv + (size_t)i * N, rhs);
+        for (int k = 0; k < N; k++) h->psi[(size_t)i * N + k] = k < h->ncheb ? rhs[k] : 0.0;
+    }
+    if (nm0 >= 0) {
+        for (int k = 0; k < N; k++) u0[k] = creal(h->up[(size_t)nm0 * N + k]);
+        get_dr_real(h, u0, om0, 1);
+        for (int k = 0; k < N; k++) h->psi[(size_t)nm0 * N + k] = 0.0;
+    }
+    for (int i = 0; i < h->nm; i++)
+        get_ddr_vec(h, h->psi + (size_t)i * N, dpsi + (size_t)i * N, d2psi + (size_t)i * N, 0);
+    costf_field(h, h->psi);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        double dm2 = (double)m * (double)m;
+        for (int k = 0; k < N; k++) {
+            size_t q = (size_t)i * N + k;
+            if (m == 0) {
+                h->us[q] = 0.0;
+                h->om[q] = om0[k] + h->or1[k] * u0[k];
+            } else {
+                h->us[q] = I * (double)m * h->or1[k] * h->psi[q];
+                h->up[q] = -dpsi[q] - h->beta[k] * h->psi[q];
+                h->om[q] = -(d2psi[q] + (h->or1[k] + h->beta[k]) * dpsi[q] +
+                             (h->or1[k] * h->beta[k] + h->dbeta[k] - dm2 * h->or2[k]) * h->psi[q]);
+            }
+        }
+    }
+    coll_rhs_imp_psi(h, a, 1, t->l_imp_calc_rhs[0], 0);
+    free(u0); free(om0); free(rhs); free(dpsi); free(d2psi);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -3314,6 +3726,7 @@ static int one_step(or_handle *h)
             }
             ts_bridge(t, h->c.alpha);
         }
+        if (t->l_assembly && t->istage == t->nstages) break; /* step_time.f90:313 */
         if (lMat) {
             int info = 0;
             if (h->fd) fd_build(h);
@@ -3332,6 +3745,12 @@ static int one_step(or_handle *h)
             update_psi(h);
         }
         t->istage++;
+    }
+    if (t->l_assembly) { /* assemble_stage (step_time.f90:353-368, mloop.f90:105-160) */
+        if (h->l_heat) coll_assemble_scalar(h, 1);
+        if (h->l_chem) coll_assemble_scalar(h, 0);
+        int info = coll_assemble_psi(h);
+        if (info) return info;
     }
     h->n_time_step++;
     return 0;
@@ -3589,6 +4008,46 @@ static double rint_r(or_handle *h, const double *f)
 }
 
 static double round_off(double v) { return fabs(v) < 10.0 * 2.220446049250313e-16 ? 0.0 : v; }
+
+/* test hooks of the assembly stage: robin_bc on one complex column (f
+ * interleaved re, im, n_r values), and psi on the grid from omega through
+ * the assembly matrix of mode index mi; radial rows r, or1, or2, beta, dbeta */
+int or_robin(or_handle *h, double at, double bt, const double *rtop, double ab, double bb, const double *rbot,
+             double *f)
+{
+    if (!h->coll) return -1;
+    robin_bc(h, at, bt, rtop[0] + I * rtop[1], ab, bb, rbot[0] + I * rbot[1], (cplx *)f);
+    return 0;
+}
+
+int or_ass_solve(or_handle *h, int mi, const double *omega, double *psi)
+{
+    int N = h->N;
+    if (!h->coll || mi <= 0 || mi >= h->nm) return -1;
+    if (!h->lAss && coll_ass_mat(h)) return -2;
+    cplx *rhs = calloc(N, sizeof(cplx));
+    const cplx *w = (const cplx *)omega;
+    for (int k = 2; k < N - 2; k++) rhs[k] = w[k];
+    getrs_c(N, h->cAss + (size_t)mi * N * N, h->cAssPiv + (size_t)mi * N, rhs);
+    for (int k = h->ncheb; k < N; k++) rhs[k] = 0.0;
+    costf_vec(h, rhs);
+    memcpy(psi, rhs, sizeof(cplx) * N);
+    free(rhs);
+    return 0;
+}
+
+int or_radial(or_handle *h, double *out)
+{
+    int N = h->N;
+    for (int k = 0; k < N; k++) {
+        out[k] = h->r[k];
+        out[N + k] = h->or1[k];
+        out[2 * N + k] = h->or2[k];
+        out[3 * N + k] = h->beta[k];
+        out[4 * N + k] = h->dbeta[k];
+    }
+    return 0;
+}
 
 /* classical Nusselt numbers (outputs.f90:729-755) */
 int or_nusselt(or_handle *h, double out[2])

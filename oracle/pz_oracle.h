@@ -30,6 +30,12 @@ int or_step(or_handle *h, int nsteps);
 int or_time(or_handle *h, double *time, double *dt);
 int or_ekin_3d(or_handle *h, double out[4]);
 int or_nusselt(or_handle *h, double out[2]);
+int or_robin(or_handle *h, double at, double bt, const double *rtop, double ab, double bb, const double *rbot,
+             double *f);
+int or_ass_solve(or_handle *h, int mi, const double *omega, double *psi);
+int or_radial(or_handle *h, double *out);
+int or_dirk_tables(int scheme, int *nstages, int *l_assembly, double *wimp_lin0, double *bimp, double *bexp,
+                   double *bass_imp, double *bass_exp);
 
 /* unit operators on (n_m, n_r) complex arrays, m fastest */
 int or_costf(or_handle *h, double *f, int n_m);

@@ -12,15 +12,20 @@ N_T_BOUNDS = 32  # namelists.f90:23
 
 CNAB2, CNLF, MODCNAB, SBDF2, SBDF3, SBDF4, TVB33 = 1, 2, 3, 4, 5, 6, 7
 ARS222, ARS443, BPR353, DP1_242, DP2_242, LZ232, CK232, LZ453, PC2 = range(101, 110)
+# DIRK schemes with an assembly stage
+(SSP222, LSDIRK222, ARS232, ARS233, SSP322, SSP332, SSP433, ARS343, MARS343, KC343, KC564, KC785,
+ DBM453) = range(110, 123)
+ASSEMBLY_SCHEMES = (SSP222, LSDIRK222, ARS232, ARS233, SSP322, SSP332, SSP433, ARS343, MARS343, KC343,
+                    KC564, KC785, DBM453)
 
 # The DIRK names, in the order initialize() of dirk_schemes.f90:55-230 tests
-# them; None marks schemes with an assembly stage, which are not implemented.
-DIRK_ORDER = [("ARS222", ARS222), ("SSP222", None), ("LSDIRK222", None), ("ARS232", None),
-              ("ARS233", None), ("SSP322", None), ("SSP332", None), ("SSP433", None),
+# them.
+DIRK_ORDER = [("ARS222", ARS222), ("SSP222", SSP222), ("LSDIRK222", LSDIRK222), ("ARS232", ARS232),
+              ("ARS233", ARS233), ("SSP322", SSP322), ("SSP332", SSP332), ("SSP433", SSP433),
               ("DP1-242", DP1_242), ("DP2-242", DP2_242), ("LZ232", LZ232), ("CK232", CK232),
               ("ARS443", ARS443), ("LZ453", LZ453), ("BPR353", BPR353), ("PC2", PC2),
-              ("ARS343", None), ("MARS343", None), ("KC343", None), ("KC564", None),
-              ("KC785", None), ("DBM453", None)]
+              ("ARS343", ARS343), ("MARS343", MARS343), ("KC343", KC343), ("KC564", KC564),
+              ("KC785", KC785), ("DBM453", DBM453)]
 # The multistep names in the order of multistep_schemes.f90:56-104
 MULTISTEP_ORDER = [("CNAB2", CNAB2), ("MODCNAB", MODCNAB), ("CNLF", CNLF), ("BDF2AB2", SBDF2),
                    ("SBDF2", SBDF2), ("BDF3AB3", SBDF3), ("SBDF3", SBDF3), ("TVB33", TVB33),
@@ -35,8 +40,6 @@ def scheme_id(name):
     order = DIRK_ORDER if any(n in name for n, _ in DIRK_ORDER) else MULTISTEP_ORDER
     for n, v in order:
         if n in name:
-            if v is None:
-                raise ValueError(f"time scheme {n} (with an assembly stage) is not implemented")
             return v
     raise ValueError(f"time scheme {name!r} was not found")
 

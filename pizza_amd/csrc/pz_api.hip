@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -15,6 +17,7 @@
 #include "pz_mphase.h"
 #include "pz_coll.h"
 #include "pz_fd.h"
+#include "pz_assembly.h"
 #include "pz_adjoint.h"
 
 using pzh::ld;
@@ -26,6 +29,8 @@ struct Scheme {
     double courfac = 0, wimp_lin0 = 0;
     double bimp0[PZ_MAX_STAGES][PZ_MAX_STAGES] = {}, bexp0[PZ_MAX_STAGES][PZ_MAX_STAGES] = {};
     int l_exp_calc[8] = {}, l_imp_calc_rhs[8] = {};
+    int l_assembly = 0;
+    double bass_imp0[PZ_MAX_STAGES] = {}, bass_exp0[PZ_MAX_STAGES] = {};
 };
 
 // Butcher tables of a DIRK scheme without an assembly stage: n = nstages + 1
@@ -37,6 +42,20 @@ void set_butcher(Scheme &s, int nst, const double *bi, const double *be)
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++) { s.bimp0[i][j] = bi[i * n + j]; s.bexp0[i][j] = be[i * n + j]; }
     for (int k = 0; k < nst; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
+}
+
+// Tables of a DIRK scheme with an assembly stage: nst x nst Butcher tables
+// (dirk_schemes.f90:239-243) and the assembly weights.
+void set_butcher_ass(Scheme &s, int nst, const double *bi, const double *be, const double *bai, const double *bae)
+{
+    s.nstages = s.nimp = s.nexp = nst;
+    for (int i = 0; i < nst; i++) {
+        for (int j = 0; j < nst; j++) { s.bimp0[i][j] = bi[i * nst + j]; s.bexp0[i][j] = be[i * nst + j]; }
+        s.bass_imp0[i] = bai[i];
+        s.bass_exp0[i] = bae[i];
+    }
+    for (int k = 0; k < nst; k++) s.l_exp_calc[k] = s.l_imp_calc_rhs[k] = 1;
+    s.l_assembly = 1;
 }
 
 // select_time_scheme / initialize of multistep_schemes.f90:40-135 and
@@ -153,6 +172,187 @@ bool make_scheme(int id, double courfac_nml, Scheme &s)
             set_butcher(s, 3, bi, be);
             s.wimp_lin0 = 0.5;
             cf = 0.8;
+            break;
+        }
+        case PZ_SSP222: { /* :299-311 */
+            const double g = 1.0 - 1.0 / r2;
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - 2 * g, g}, be[9] = {0, 0, 0, 0, 0, 0, 0, 1, 0};
+            double ba[3] = {0, 0.5, 0.5};
+            set_butcher_ass(s, 3, bi, be, ba, ba);
+            s.wimp_lin0 = g; s.l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_LSDIRK222: { /* :312-326 */
+            const double g = 1.0 - 1.0 / r2;
+            double chat = 1.0 / (2.0 * g);
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - g, g}, be[9] = {0, 0, 0, 0, 0, 0, 0, chat, 0};
+            double ba[3] = {0, 1 - g, g};
+            set_butcher_ass(s, 3, bi, be, ba, ba);
+            s.wimp_lin0 = g; s.l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_ARS232: { /* :427-441 */
+            const double g = 0.5 * (2.0 - r2);
+            double del = -2.0 * r2 / 3.0;
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - g, g}, be[9] = {0, 0, 0, g, 0, 0, del, 1 - del, 0};
+            double ba[3] = {0, 1 - g, g};
+            set_butcher_ass(s, 3, bi, be, ba, ba);
+            s.wimp_lin0 = g; s.l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_ARS233: { /* :327-341 */
+            const double g = (3.0 + sqrt(3.0)) / 6.0;
+            double bi[9] = {0, 0, 0, 0, g, 0, 0, 1 - 2 * g, g}, be[9] = {0, 0, 0, g, 0, 0, g - 1, 2 * (1 - g), 0};
+            double ba[3] = {0, 0.5, 0.5};
+            set_butcher_ass(s, 3, bi, be, ba, ba);
+            s.wimp_lin0 = g; s.l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_SSP322: { /* :342-356 */
+            double bi[16] = {0, 0, 0, 0, 0, 0.5, 0, 0, 0, -0.5, 0.5, 0, 0, 0, 0.5, 0.5};
+            double be[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0};
+            double ba[4] = {0, 0, 0.5, 0.5};
+            set_butcher_ass(s, 4, bi, be, ba, ba);
+            s.wimp_lin0 = 0.5; s.l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_SSP332: { /* :357-372 */
+            const double g = 1.0 - 1.0 / r2;
+            double bi[16] = {0, 0, 0, 0, 0, g, 0, 0, 0, 1 - 2 * g, g, 0, 0, 0.5 - g, 0, g};
+            double be[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0.25, 0.25, 0};
+            double ba[4] = {0, 1.0 / 6.0, 1.0 / 6.0, 2.0 * (1.0 / 3.0)};
+            set_butcher_ass(s, 4, bi, be, ba, ba);
+            s.wimp_lin0 = g; s.l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_SSP433: { /* :373-392; alpha, beta, eta are default-real
+                             (single precision) literals in the reference */
+            const double al = (double)0.24169426078821f, be_ = (double)0.06042356519705f;
+            const double et = (double)0.12915286960590f;
+            double bi[25] = {0, 0, 0, 0, 0, 0, al, 0, 0, 0, 0, -al, al, 0, 0,
+                             0, 0, 1 - al, al, 0, 0, be_, et, 0.5 - be_ - et - al, al};
+            double be[25] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                             0, 0, 1, 0, 0, 0, 0, 0.25, 0.25, 0};
+            double ba[5] = {0, 0, 1.0 / 6.0, 1.0 / 6.0, 2.0 * (1.0 / 3.0)};
+            set_butcher_ass(s, 5, bi, be, ba, ba);
+            s.wimp_lin0 = al; s.l_imp_calc_rhs[0] = 0; cf = 1.35;
+            break;
+        }
+        case PZ_ARS343: { /* :526-545 */
+            const double g = 0.435866521508459;
+            const double b1 = -1.5 * g * g + 4.0 * g - 0.25, b2 = 1.5 * g * g - 5.0 * g + 1.25;
+            double bi[16] = {0, 0, 0, 0, 0, g, 0, 0, 0, 0.5 * (1 - g), g, 0, 0, b1, b2, g};
+            double be[16] = {0, 0, 0, 0, g, 0, 0, 0, 0.3212788860286278, 0.3966543747256017, 0, 0,
+                             -0.1058582960718797, 0.5529291480359398, 0.5529291480359398, 0};
+            double ba[4] = {0, b1, b2, g};
+            set_butcher_ass(s, 4, bi, be, ba, ba);
+            s.wimp_lin0 = g; s.l_imp_calc_rhs[0] = 0; cf = 0.7;
+            break;
+        }
+        case PZ_MARS343: { /* :571-590 */
+            const double g = 0.435866521508458;
+            const double b1 = 1.20849664917601, b2 = -0.64436317068446;
+            double bi[16] = {0, 0, 0, 0, 0, g, 0, 0, 0, 0.5 * (1 - g), g, 0, 0, b1, b2, g};
+            double be[16] = {0, 0, 0, 0, 0.877173304301691, 0, 0, 0, 0.535396540307354, 0.182536720446875, 0, 0,
+                             0.63041255815287, -0.83193390106308, 1.20152134291021, 0};
+            double ba[4] = {0, b1, b2, g};
+            set_butcher_ass(s, 4, bi, be, ba, ba);
+            s.wimp_lin0 = g; s.l_imp_calc_rhs[0] = 0; cf = 0.5;
+            break;
+        }
+        case PZ_KC343: { /* :546-570 */
+            const double g = 1767732205903.0 / 4055673282236.0;
+            const double a30 = 1471266399579.0 / 7840856788654.0, a31 = -4482444167858.0 / 7529755066697.0;
+            const double a32 = 11266239266428.0 / 11593286722821.0;
+            double bi[16] = {0, 0, 0, 0, g, g, 0, 0, 2746238789719.0 / 10658868560708.0,
+                             -640167445237.0 / 6845629431997.0, g, 0, a30, a31, a32, g};
+            double be[16] = {0, 0, 0, 0, 1767732205903.0 / 2027836641118.0, 0, 0, 0,
+                             5535828885825.0 / 10492691773637.0, 788022342437.0 / 10882634858940.0, 0, 0,
+                             6485989280629.0 / 16251701735622.0, -4246266847089.0 / 9704473918619.0,
+                             10755448449292.0 / 10357097424841.0, 0};
+            double ba[4] = {a30, a31, a32, g};
+            set_butcher_ass(s, 4, bi, be, ba, ba);
+            s.wimp_lin0 = g; cf = 0.5;
+            break;
+        }
+        case PZ_KC564: { /* :591-625 */
+            double bi[36] = {0, 0, 0, 0, 0, 0,
+                             0.25, 0.25, 0, 0, 0, 0,
+                             8611.0 / 62500.0, -1743.0 / 31250.0, 0.25, 0, 0, 0,
+                             5012029.0 / 34652500.0, -654441.0 / 2922500.0, 174375.0 / 388108.0, 0.25, 0, 0,
+                             15267082809.0 / 155376265600.0, -71443401.0 / 120774400.0, 730878875.0 / 902184768.0,
+                             2285395.0 / 8070912.0, 0.25, 0,
+                             82889.0 / 524892.0, 0, 15625.0 / 83664.0, 69875.0 / 102672.0, -2260.0 / 8211.0, 0.25};
+            double be[36] = {0, 0, 0, 0, 0, 0,
+                             0.5, 0, 0, 0, 0, 0,
+                             13861.0 / 62500.0, 6889.0 / 62500.0, 0, 0, 0, 0,
+                             -116923316275.0 / 2393684061468.0, -2731218467317.0 / 15368042101831.0,
+                             9408046702089.0 / 11113171139209.0, 0, 0, 0,
+                             -451086348788.0 / 2902428689909.0, -2682348792572.0 / 7519795681897.0,
+                             12662868775082.0 / 11960479115383.0, 3355817975965.0 / 11060851509271.0, 0, 0,
+                             647845179188.0 / 3216320057751.0, 73281519250.0 / 8382639484533.0,
+                             552539513391.0 / 3454668386233.0, 3354512671639.0 / 8306763924573.0,
+                             4040.0 / 17871.0, 0};
+            double ba[6] = {82889.0 / 524892.0, 0, 15625.0 / 83664.0, 69875.0 / 102672.0, -2260.0 / 8211.0, 0.25};
+            set_butcher_ass(s, 6, bi, be, ba, ba);
+            s.wimp_lin0 = 0.25; cf = 0.5;
+            break;
+        }
+        case PZ_DBM453: { /* :626-651 */
+            const double w = 0.32591194130117247;
+            double bi[25] = {0, 0, 0, 0, 0,
+                             -0.22284985318525410, w, 0, 0, 0,
+                             -0.46801347074080545, 0.86349284225716961, w, 0, 0,
+                             -0.46509906651927421, 0.81063103116959553, 0.61036726756832357, w, 0,
+                             0.87795339639076675, -0.72692641526151547, 0.75204137157372720, -0.22898029400415088, w};
+            double be[25] = {0, 0, 0, 0, 0,
+                             0.10306208811591838, 0, 0, 0, 0,
+                             -0.94124866143519894, 1.6626399742527356, 0, 0, 0,
+                             -1.3670975201437765, 1.3815852911016873, 1.2673234025619065, 0, 0,
+                             -0.81287582068772448, 0.81223739060505738, 0.90644429603699305, 0.094194134045674111, 0};
+            double ba[5] = {0.87795339639076672, -0.72692641526151549, 0.7520413715737272, -0.22898029400415090, w};
+            set_butcher_ass(s, 5, bi, be, ba, ba);
+            s.wimp_lin0 = w; cf = 0.5;
+            break;
+        }
+        case PZ_KC785: { /* :652-711 */
+            const double w = 2.0 / 9.0;
+            const double c2 = 2366667076620.0 / 8822750406821.0, c3 = -257962897183.0 / 4451812247028.0;
+            const double c4 = -486229321650.0 / 11227943450093.0, c5 = 621307788657.0 / 4714163060173.0;
+            const double c6 = 2036305566805.0 / 6583108094622.0;
+            const double b2 = 3517720773327.0 / 20256071687669.0, b3 = 4569610470461.0 / 17934693873752.0;
+            const double b4 = 2819471173109.0 / 11655438449929.0, b5 = 3296210113763.0 / 10722700128969.0;
+            const double b6 = -1142099968913.0 / 5710983926999.0;
+            double bi[64] = {0, 0, 0, 0, 0, 0, 0, 0,
+                             w, w, 0, 0, 0, 0, 0, 0,
+                             c2, c2, w, 0, 0, 0, 0, 0,
+                             c3, c3, 128530224461.0 / 14379561246022.0, w, 0, 0, 0, 0,
+                             c4, c4, -225633144460.0 / 6633558740617.0, 1741320951451.0 / 6824444397158.0, w, 0, 0, 0,
+                             c5, c5, -125196015625.0 / 3866852212004.0, 940440206406.0 / 7593089888465.0,
+                             961109811699.0 / 6734810228204.0, w, 0, 0,
+                             c6, c6, -3039402635899.0 / 4450598839912.0, -1829510709469.0 / 31102090912115.0,
+                             -286320471013.0 / 6931253422520.0, 8651533662697.0 / 9642993110008.0, w, 0,
+                             0, 0, b2, b3, b4, b5, b6, w};
+            const double e7 = 2193717860234.0 / 3570523412979.0;
+            double be[64] = {0, 0, 0, 0, 0, 0, 0, 0,
+                             4.0 / 9.0, 0, 0, 0, 0, 0, 0, 0,
+                             1.0 / 9.0, 1183333538310.0 / 1827251437969.0, 0, 0, 0, 0, 0, 0,
+                             895379019517.0 / 9750411845327.0, 477606656805.0 / 13473228687314.0,
+                             -112564739183.0 / 9373365219272.0, 0, 0, 0, 0, 0,
+                             -4458043123994.0 / 13015289567637.0, -2500665203865.0 / 9342069639922.0,
+                             983347055801.0 / 8893519644487.0, 2185051477207.0 / 2551468980502.0, 0, 0, 0, 0,
+                             -167316361917.0 / 17121522574472.0, 1605541814917.0 / 7619724128744.0,
+                             991021770328.0 / 13052792161721.0, 2342280609577.0 / 11279663441611.0,
+                             3012424348531.0 / 12792462456678.0, 0, 0, 0,
+                             6680998715867.0 / 14310383562358.0, 5029118570809.0 / 3897454228471.0,
+                             2415062538259.0 / 6382199904604.0, -3924368632305.0 / 6964820224454.0,
+                             -4331110370267.0 / 15021686902756.0, -3944303808049.0 / 11994238218192.0, 0, 0,
+                             e7, e7, 5952760925747.0 / 18750164281544.0, -4412967128996.0 / 6196664114337.0,
+                             4151782504231.0 / 36106512998704.0, 572599549169.0 / 6265429158920.0,
+                             -457874356192.0 / 11306498036315.0, 0};
+            double ba[8] = {0, 0, b2, b3, b4, b5, b6, w};
+            set_butcher_ass(s, 8, bi, be, ba, ba);
+            s.wimp_lin0 = w; cf = 0.5;
             break;
         }
         default:
@@ -583,10 +783,10 @@ static void allow_full_lds(int lim)
                              (const void *)k_start, (const void *)k_build_inv, (const void *)k_mcoll,
                              (const void *)k_start_coll, (const void *)k_build_lhs_batch,
                              (const void *)k_mphase_batch, (const void *)k_radial_loop_batch,
-                             (const void *)k_build_inv_batch, (const void *)k_mcoll_batch, (const void *)k_fd_build,
+                             (const void *)k_build_inv_batch, (const void *)k_mcoll_batch<true>, (const void *)k_mcoll_batch<false>, (const void *)k_fd_build,
                              (const void *)k_fd_mloop, (const void *)k_fd_start, (const void *)k_radial_loop_big,
                              (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
-                             (const void *)k_start_coll_adj};
+                             (const void *)k_start_coll_adj, (const void *)k_assemble_coll};
     for (const void *k : kernels) {
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, k));
@@ -630,6 +830,9 @@ static void setup(pz_handle *h)
     for (int k = 0; k < 8; k++) { d.l_exp_calc[k] = h->sch.l_exp_calc[k]; d.l_imp_calc_rhs[k] = h->sch.l_imp_calc_rhs[k]; }
     std::memcpy(d.bimp0, h->sch.bimp0, sizeof(d.bimp0));
     std::memcpy(d.bexp0, h->sch.bexp0, sizeof(d.bexp0));
+    d.l_assembly = h->sch.l_assembly;
+    std::memcpy(d.bass_imp0, h->sch.bass_imp0, sizeof(d.bass_imp0));
+    std::memcpy(d.bass_exp0, h->sch.bass_exp0, sizeof(d.bass_exp0));
     d.wimp_lin0 = h->sch.wimp_lin0;
     d.courfac = h->sch.courfac;
     d.l_heat = h->l_heat;
@@ -865,6 +1068,23 @@ static void setup(pz_handle *h)
         // Gauss-Jordan blocks per m: T and xi (N x 2N real), psi (2N x 4N complex)
         d.coll_ws = (2 * (size_t)N * 2 * N * sizeof(double) + (size_t)2 * N * 4 * N * sizeof(cd) + 255) & ~(size_t)255;
         d.coll_work = h->dalloc<char>((size_t)nm * d.coll_ws);
+        if (h->sch.l_assembly) {
+            // boundary first-derivative rows for robin_bc (chebyshev.f90:338-360):
+            // the flipped-row form at r_cmb, its mirror at r_icb, times the map
+            std::vector<double> top(N), bot(N);
+            top[0] = (2.0 * (N - 1) * (N - 1) + 1.0) / 6.0;
+            for (int k = 2; k <= N; k++) {
+                const double sn = std::sin(0.5 * (k - 1) * M_PI / (N - 1));
+                top[k - 1] = 2.0 * ((k % 2 == 0) ? -1.0 : 1.0) / (2.0 * sn * sn);
+            }
+            top[N - 1] *= 0.5;
+            for (int k = 0; k < N; k++) bot[k] = -top[N - 1 - k];
+            const double dt0 = h->drx[0], db0 = h->drx[N - 1];
+            for (int k = 0; k < N; k++) { top[k] *= dt0; bot[k] *= db0; }
+            d.dr_top = h->upload(top);
+            d.dr_bot = h->upload(bot);
+            d.A_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
+        }
     }
     if (!d.coll && !d.fd) build_operators(h);
     // LDS budgets
@@ -925,6 +1145,8 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
     else {
         Scheme probe;
         if (!make_scheme(c.time_scheme, c.courfac, probe)) why = "time scheme not implemented";
+        else if (probe.l_assembly && (!c.l_cheb_coll || c.l_finite_diff))
+            why = "DIRK schemes with an assembly stage are implemented for the collocation method only";
     }
     if (!why.empty()) {
         int rc = fail(h, -2, why);
@@ -1170,12 +1392,22 @@ extern "C" int pz_start(pz_handle *h)
                         (size_t)sc.nexp, (size_t)sc.nold, (size_t)sc.nimp, (size_t)sc.nexp};
         for (int i = 0; i < 9; i++) HIPCHK(hipMemsetAsync(slots[i], 0, cnt[i] * F * sizeof(cd), h->stream));
         hipLaunchKernelGGL(k_set_weights, dim3(1), dim3(1), 0, h->stream, h->d);
-        if (h->d.fd)
+        // exactly one start kernel per radial method: k_start reads the banded
+        // operators that only the integration method builds
+        if (h->d.fd) {
             hipLaunchKernelGGL(k_fd_start, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev);
-        else if (h->d.coll)
+        } else if (h->d.coll) {
             hipLaunchKernelGGL(k_start_coll, dim3(h->nm), dim3(256), h->lds_start_coll, h->stream, h->d);
-        else
+            if (h->d.l_assembly) {
+                if (!h->d.A_inv || !h->d.dr_top || !h->d.dr_bot)
+                    throw std::logic_error("assembly stage without its matrices");
+                hipLaunchKernelGGL(k_build_ass, dim3(h->nm), dim3(1024), 0, h->stream, h->d);
+            }
+        } else {
+            if (!h->d.T_RHSE2 || !h->d.T_RHSI || !h->d.P_RHSI)
+                throw std::logic_error("integration start without its banded operators");
             hipLaunchKernelGGL(k_start, dim3(h->nm), dim3(256), h->lds_start, h->stream, h->d);
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));
         if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -1209,6 +1441,7 @@ static void launch_step(pz_handle *h)
                                    (const PzDev *)h->ddev, 0);
             }
         }
+        if (s.l_assembly && stage == s.nstages) break; // the last stage only evaluates the explicit terms
         if (h->d.fd)
             hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev,
                                stage, exp_calc);
@@ -1219,6 +1452,10 @@ static void launch_step(pz_handle *h)
             hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                (const PzDev *)h->ddev, stage, exp_calc);
     }
+    // assemble_stage (step_time.f90:353-368)
+    if (s.l_assembly)
+        hipLaunchKernelGGL(k_assemble_coll, dim3(h->nm), dim3(ASM_THREADS), asm_lds_bytes(h->N), h->stream,
+                           (const PzDev *)h->ddev);
 }
 
 // nsteps steps through the captured step graph (captured on first use)
@@ -1807,6 +2044,8 @@ struct EnsClass {
     int nb_r = 0, nb_c = 0, nb_3 = 0, nb_m = 0;
     size_t lds_r = 0, lds_m = 0, lds_b = 0, lds_i = 0;
     int inv_threads = 64, mthreads = PZ_MPHASE_THREADS;
+    int rad_lds = 1; // collocation m-loop: radial rows staged in LDS
+    int n = 0;       // n_r of the class
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     hipGraph_t graph = nullptr;
@@ -1818,6 +2057,8 @@ struct pz_ensemble {
     std::vector<EnsClass> cls;
     Scheme sch;
     int coll = 0;
+    int tuned = 0, policy = 0; // autotuned m-loop shape of the large classes
+    double policy_ms[4] = {};
     std::vector<void *> allocs;
     std::string err;
     ~pz_ensemble()
@@ -1862,8 +2103,8 @@ static void launch_class_step(const pz_ensemble *e, const EnsClass &c)
             }
         }
         if (e->coll)
-            hipLaunchKernelGGL(k_mcoll_batch, dim3(c.nb_m), dim3(c.mthreads), c.lds_m, c.stream,
-                               PzBatch{runs, c.map_m, (int)c.members.size()}, stage, exp_calc);
+            hipLaunchKernelGGL(c.rad_lds ? k_mcoll_batch<true> : k_mcoll_batch<false>, dim3(c.nb_m), dim3(c.mthreads),
+                               c.lds_m, c.stream, PzBatch{runs, c.map_m, (int)c.members.size()}, stage, exp_calc);
         else
             hipLaunchKernelGGL(k_mphase_batch, dim3(c.nb_m), dim3(c.mthreads), c.lds_m, c.stream,
                                PzBatch{runs, c.map_m, (int)c.members.size()}, stage, exp_calc);
@@ -1884,6 +2125,8 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
             if (!h->started) throw std::invalid_argument("every run must be started (pz_start) first");
             if (h->d.fd) throw std::invalid_argument("finite-difference runs are not batched into ensembles");
             if (h->radial_big) throw std::invalid_argument("runs with long azimuthal grids are not batched into ensembles");
+            if (h->sch.l_assembly)
+                throw std::invalid_argument("DIRK schemes with an assembly stage are not batched into ensembles");
             if (h->sch.id != h0->sch.id || h->d.coll != h0->d.coll)
                 throw std::invalid_argument("the runs of an ensemble must share the time scheme and the radial method");
             e->runs.push_back(h);
@@ -1914,6 +2157,7 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
             }
             c.lds_i = InvLayout(n, nw).total;
             c.mthreads = mloop_threads(n);
+            c.n = n;
             c.inv_threads = 64 * nw;
             c.druns = e->upload(dr);
             c.map_r = e->upload(mr);
@@ -1937,12 +2181,104 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
     return 0;
 }
 
+// m-loop shape of the collocation classes with n_r > 65 under a policy:
+// 0 the single-run shape (512 threads, radial rows in LDS), 1 256 threads
+// (n_r <= 128 only), 2 radial rows read through L2, 3 both (less LDS: more
+// workgroups per CU)
+static void ens_policy(pz_ensemble *e, int policy)
+{
+    for (EnsClass &c : e->cls) {
+        const bool big = c.n > 65;
+        // the coupled omega-psi product needs 2 n_r <= threads
+        c.mthreads = (big && (policy & 1) && 2 * c.n <= 256) ? 256 : mloop_threads(c.n);
+        c.rad_lds = (big && (policy & 2)) ? 0 : 1;
+        c.lds_m = MpLayout(c.n, c.mthreads, c.rad_lds != 0).total;
+        if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
+        if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
+    }
+}
+
+static void ens_capture(pz_ensemble *e)
+{
+    for (EnsClass &c : e->cls)
+        if (!c.gexec) {
+            HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            launch_class_step(e, c);
+            HIPCHK(hipStreamEndCapture(c.stream, &c.graph));
+            HIPCHK(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
+        }
+}
+
+// The autotuner: times a few ensemble steps under each policy from the same
+// member states (snapshots of the step state blocks, restored after every
+// trial) and keeps the fastest.  The trials may rebuild matrices for
+// another dt, so the members' matrices are rebuilt for their restored
+// state at the end; the results of the following steps are those of the
+// plain step, bit for bit.
+static void ens_autotune(pz_ensemble *e)
+{
+    size_t total = 0;
+    for (pz_handle *h : e->runs) total += h->arena_bytes;
+    char *snap = nullptr;
+    HIPCHK(hipMalloc(&snap, total));
+    auto copy = [&](bool save) {
+        size_t off = 0;
+        for (pz_handle *h : e->runs) {
+            HIPCHK(hipMemcpyAsync(save ? snap + off : h->arena, save ? h->arena : snap + off, h->arena_bytes,
+                                  hipMemcpyDeviceToDevice, h->stream));
+            off += h->arena_bytes;
+        }
+        for (pz_handle *h : e->runs) HIPCHK(hipStreamSynchronize(h->stream));
+    };
+    copy(true);
+    bool any_big = false;
+    for (EnsClass &c : e->cls) any_big |= c.n > 65;
+    const int npol = any_big ? 4 : 1, reps = 3;
+    double best = 1e300;
+    int best_p = 0;
+    for (int pol = 0; pol < npol; pol++) {
+        ens_policy(e, pol);
+        ens_capture(e);
+        for (EnsClass &c : e->cls) HIPCHK(hipGraphLaunch(c.gexec, c.stream)); // warm-up (and first builds)
+        for (EnsClass &c : e->cls) HIPCHK(hipStreamSynchronize(c.stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < reps; i++)
+            for (EnsClass &c : e->cls) HIPCHK(hipGraphLaunch(c.gexec, c.stream));
+        for (EnsClass &c : e->cls) HIPCHK(hipStreamSynchronize(c.stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+        e->policy_ms[pol] = ms;
+        if (ms < best) { best = ms; best_p = pol; }
+        copy(false);
+    }
+    HIPCHK(hipFree(snap));
+    // matrices for the restored states
+    for (EnsClass &c : e->cls)
+        hipLaunchKernelGGL(k_build_coll_batch, dim3(std::min(c.nb_3, 512)), dim3(1024), 0, c.stream,
+                           PzBatch{c.druns, c.map_3, (int)c.members.size()}, c.nb_3, 1);
+    for (EnsClass &c : e->cls) HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipGetLastError());
+    ens_policy(e, best_p);
+    e->policy = best_p;
+    e->tuned = 1;
+}
+
 extern "C" int pz_ensemble_step(pz_ensemble *e, int nsteps)
 {
     if (!e) return -1;
     try {
         // work already queued on the members comes first
         for (pz_handle *h : e->runs) HIPCHK(hipStreamSynchronize(h->stream));
+        if (!e->tuned) {
+            // PZ_ENSEMBLE_POLICY=k fixes the policy (no tuning)
+            const char *fixed = std::getenv("PZ_ENSEMBLE_POLICY");
+            if (fixed && e->coll) {
+                e->policy = std::atoi(fixed) & 3;
+                ens_policy(e, e->policy);
+            } else if (e->coll) {
+                ens_autotune(e);
+            }
+            e->tuned = 1;
+        }
         for (EnsClass &c : e->cls)
             if (!c.gexec) {
                 HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
@@ -1976,3 +2312,12 @@ extern "C" int pz_ensemble_destroy(pz_ensemble *e)
 }
 
 extern "C" const char *pz_ensemble_last_error(const pz_ensemble *e) { return e ? e->err.c_str() : "null ensemble"; }
+
+extern "C" int pz_ensemble_policy(const pz_ensemble *e, int *policy, double *trial_ms)
+{
+    if (!e || !policy) return -1;
+    *policy = e->policy;
+    if (trial_ms)
+        for (int i = 0; i < 4; i++) trial_ms[i] = e->policy_ms[i];
+    return e->tuned ? 0 : 1;
+}

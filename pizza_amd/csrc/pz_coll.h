@@ -318,6 +318,7 @@ __device__ void coll_uphi0_impl(const PzDev &p, const RadView &R, const cd *u0, 
 // exp_calc, update_temp_co / update_xi_co (update_temp_coll.f90:66-150) and
 // update_om_coll_smat (update_psi_coll_smat.f90:88-321) with their implicit
 // terms (get_*_rhs_imp_coll*).
+template <bool RADLDS>
 __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, int exp_calc)
 {
     extern __shared__ double smem[];
@@ -326,7 +327,7 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
     const int tid = threadIdx.x;
-    const MpLayout lay(N, blockDim.x);
+    const MpLayout lay(N, blockDim.x, RADLDS);
     char *lb = (char *)smem;
     cd *V = (cd *)(lb + lay.vec);
     cd *aT = V, *cT = V + N, *eT = V + 2 * N;
@@ -342,8 +343,8 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
     const double dm = (double)m, dm2 = dm * dm;
     lds_copy(cosT, p.cosT, sizeof(double) * p.L);
     lds_copy(sinT, p.sinT, sizeof(double) * p.L);
-    lds_copy(lb + lay.rad, p.radc, sizeof(double) * PZ_NRADC * N);
-    const RadView R((const double *)(lb + lay.rad), N);
+    if (RADLDS) lds_copy(lb + lay.rad, p.radc, sizeof(double) * PZ_NRADC * N);
+    const RadView R(RADLDS ? (const double *)(lb + lay.rad) : p.radc, N);
 
     // ---------------- global inputs -------------------------------------
     for (int k = tid; k < N; k += blockDim.x) {
@@ -585,12 +586,14 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
 
 __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll(const PzDev *__restrict__ pd, int istage, int exp_calc)
 {
-    mcoll_body(*pd, blockIdx.x, istage, exp_calc);
+    mcoll_body<true>(*pd, blockIdx.x, istage, exp_calc);
 }
+// batched; RADLDS picks the radial rows' place per size class
+template <bool RADLDS>
 __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll_batch(PzBatch b, int istage, int exp_calc)
 {
     const int2 e = b.map[blockIdx.x];
-    mcoll_body(*b.runs[e.x], e.y, istage, exp_calc);
+    mcoll_body<RADLDS>(*b.runs[e.x], e.y, istage, exp_calc);
 }
 
 // get_start_fields, collocation branch (init_fields.f90:191-244): dT/dr,

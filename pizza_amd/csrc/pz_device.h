@@ -656,11 +656,14 @@ __device__ void scheme_weights(const PzDev &p, PzScheme *s, int *lMatNext)
         if (s->wimp_lin[0] != wimp_old) *lMatNext = 1;
     } else {
         s->wimp_lin[0] = s->dt[0] * p.wimp_lin0;
-        for (int i = 0; i < PZ_MAX_STAGES; i++)
+        for (int i = 0; i < PZ_MAX_STAGES; i++) {
             for (int j = 0; j < PZ_MAX_STAGES; j++) {
                 s->bimp[i][j] = s->dt[0] * p.bimp0[i][j];
                 s->bexp[i][j] = s->dt[0] * p.bexp0[i][j];
             }
+            s->bass_imp[i] = s->dt[0] * p.bass_imp0[i];
+            s->bass_exp[i] = s->dt[0] * p.bass_exp0[i];
+        }
     }
 }
 
@@ -1059,8 +1062,8 @@ __device__ void dr_many(const PzDev &p, const RadView &R, VecSet<const cd> c, Ve
 // even extension y_j = x_j, y_{L-j} = x_j: Y_k = x_0 + (-1)^k x_{N-1}
 // + 2 sum_{j=1}^{N-2} x_j cos(pi j k/(N-1)), the DCT-I of dct_fftw.f90.
 // buf: LDS scratch of 2 nf L complex values; no trailing barrier.
-__device__ void dct_fft(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int nf, int dealias,
-                        const double *cosT, const double *sinT, cd *buf)
+__device__ __forceinline__ void dct_fft(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int nf, int dealias,
+                                        const double *cosT, const double *sinT, cd *buf)
 {
     if (p.nradL == 0) {
         dct_many(p, in, out, nf, dealias, cosT);
@@ -1089,8 +1092,9 @@ __device__ void dct_fft(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int
 // same FFT pass also returns costf(c) (even extension of c itself).
 // buf: LDS scratch of 2 ntr L + 4 nf complex values (ntr = nf, + nf with
 // the second derivative, + 1 with dct_out); no trailing barrier.
-__device__ void dr_fft(const PzDev &p, const RadView &R, VecSet<const cd> c, VecSet<cd> df, VecSet<cd> ddf, int nf,
-                       const double *cosT, const double *sinT, cd *buf, cd *dct_out = nullptr)
+__device__ __forceinline__ void dr_fft(const PzDev &p, const RadView &R, VecSet<const cd> c, VecSet<cd> df,
+                                       VecSet<cd> ddf, int nf, const double *cosT, const double *sinT, cd *buf,
+                                       cd *dct_out = nullptr)
 {
     if (p.nradL == 0) {
         dr_many(p, R, c, df, ddf, nf, cosT, sinT);

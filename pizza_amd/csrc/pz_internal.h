@@ -12,7 +12,7 @@
 typedef double2 cd;
 
 #define PZ_MAX_RADICES 24
-#define PZ_MAX_STAGES 6
+#define PZ_MAX_STAGES 8
 
 // Time-scheme state that changes during the run.  It lives in device memory
 // and is updated by k_courant, so a captured step replays without the host.
@@ -23,6 +23,7 @@ struct PzScheme {
     double wexp[8];
     double bimp[PZ_MAX_STAGES][PZ_MAX_STAGES]; // dt-scaled Butcher tables
     double bexp[PZ_MAX_STAGES][PZ_MAX_STAGES];
+    double bass_imp[PZ_MAX_STAGES], bass_exp[PZ_MAX_STAGES]; // dt-scaled assembly weights
     double time;
     double dt_r, dt_h; // last Courant limits
     int lMat;          // matrices must be rebuilt in this step
@@ -57,6 +58,13 @@ struct PzDev {
     double CorFac, BuoFac, TdiffFac, ChemFac, XidiffFac, ViscFac, tadvz, r_cmb, ek;
     double alpha, dtMax, dtMin, dt_fac, courfac, wimp_lin0;
     double bimp0[PZ_MAX_STAGES][PZ_MAX_STAGES], bexp0[PZ_MAX_STAGES][PZ_MAX_STAGES];
+    // DIRK schemes with an assembly stage: the assembly weights, the
+    // collocation boundary-derivative rows (robin_bc) and the inverses of
+    // the psi assembly matrices [m][col][ldi]
+    int l_assembly;
+    double bass_imp0[PZ_MAX_STAGES], bass_exp0[PZ_MAX_STAGES];
+    const double *dr_top, *dr_bot;
+    double *A_inv;
 
     // radial functions (n_r) and transform tables
     const double *r, *or1, *or2, *rgrav, *beta, *ekpump, *ekp_up, *ekp_us, *ekp_dusdp;

@@ -93,7 +93,9 @@ __host__ __device__ inline int mloop_threads(int N) { return N <= 65 ? 256 : PZ_
 // reduction scratch of the dense solves (16-byte aligned blocks).
 struct MpLayout {
     size_t vec, dtmp, cosT, sinT, rad, red, total;
-    __host__ __device__ MpLayout(int N, int threads)
+    // rad_lds = false: the radial rows stay in global memory (read through
+    // L2), which frees PZ_NRADC n_r doubles of LDS per workgroup
+    __host__ __device__ MpLayout(int N, int threads, bool rad_lds = true)
     {
         size_t o = 0;
         auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
@@ -102,7 +104,7 @@ struct MpLayout {
         dtmp = take(sizeof(double) * 2 * N);
         cosT = take(sizeof(double) * L);
         sinT = take(sizeof(double) * L);
-        rad = take(sizeof(double) * PZ_NRADC * N);
+        rad = rad_lds ? take(sizeof(double) * PZ_NRADC * N) : 0;
         // FFT scratch of the Chebyshev transforms (6 L + 12), shared with the
         // reduction scratch of the dense solves (2 matrices x 2 rows x threads)
         red = take(sizeof(cd) * (6 * L + 16 > 4 * threads ? 6 * L + 16 : 4 * threads));

@@ -159,14 +159,15 @@ contains
    ! The scheme a time_scheme string selects: the family test of
    ! select_time_scheme.f90:15-49, then the first name that initialize()
    ! matches (dirk_schemes.f90:55-230, multistep_schemes.f90:56-104).
-   ! Schemes with an assembly stage stop the run.
    integer(c_int) function scheme_id(name)
       character(len=*), intent(in) :: name
       character(len=10), parameter :: dirk(22) = [character(len=10) :: 'ARS222', 'SSP222', 'LSDIRK222', &
          'ARS232', 'ARS233', 'SSP322', 'SSP332', 'SSP433', 'DP1-242', 'DP2-242', 'LZ232', 'CK232', &
          'ARS443', 'LZ453', 'BPR353', 'PC2', 'ARS343', 'MARS343', 'KC343', 'KC564', 'KC785', 'DBM453']
-      integer(c_int), parameter :: dirk_id(22) = [PZ_ARS222, -1, -1, -1, -1, -1, -1, -1, PZ_DP1_242, &
-         PZ_DP2_242, PZ_LZ232, PZ_CK232, PZ_ARS443, PZ_LZ453, PZ_BPR353, PZ_PC2, -1, -1, -1, -1, -1, -1]
+      integer(c_int), parameter :: dirk_id(22) = [PZ_ARS222, PZ_SSP222, PZ_LSDIRK222, PZ_ARS232, &
+         PZ_ARS233, PZ_SSP322, PZ_SSP332, PZ_SSP433, PZ_DP1_242, PZ_DP2_242, PZ_LZ232, PZ_CK232, &
+         PZ_ARS443, PZ_LZ453, PZ_BPR353, PZ_PC2, PZ_ARS343, PZ_MARS343, PZ_KC343, PZ_KC564, PZ_KC785, &
+         PZ_DBM453]
       character(len=10), parameter :: ms(10) = [character(len=10) :: 'CNAB2', 'MODCNAB', 'CNLF', &
          'BDF2AB2', 'SBDF2', 'BDF3AB3', 'SBDF3', 'TVB33', 'BDF4AB4', 'SBDF4']
       integer(c_int), parameter :: ms_id(10) = [PZ_CNAB2, PZ_MODCNAB, PZ_CNLF, PZ_SBDF2, PZ_SBDF2, &

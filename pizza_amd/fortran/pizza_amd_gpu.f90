@@ -14,6 +14,9 @@ module pizza_amd_gpu
                                 PZ_SBDF4 = 6, PZ_TVB33 = 7
    integer(c_int), parameter :: PZ_ARS222 = 101, PZ_ARS443 = 102, PZ_BPR353 = 103, PZ_DP1_242 = 104, &
                                 PZ_DP2_242 = 105, PZ_LZ232 = 106, PZ_CK232 = 107, PZ_LZ453 = 108, PZ_PC2 = 109
+   integer(c_int), parameter :: PZ_SSP222 = 110, PZ_LSDIRK222 = 111, PZ_ARS232 = 112, PZ_ARS233 = 113, &
+                                PZ_SSP322 = 114, PZ_SSP332 = 115, PZ_SSP433 = 116, PZ_ARS343 = 117, &
+                                PZ_MARS343 = 118, PZ_KC343 = 119, PZ_KC564 = 120, PZ_KC785 = 121, PZ_DBM453 = 122
    ! fields (fields.f90)
    integer(c_int), parameter :: PZ_OBJ_NUTOP = 0, PZ_OBJ_TEMP = 2
    integer(c_int), parameter :: PZ_US = 0, PZ_UP = 1, PZ_OM = 2, PZ_PSI = 3, PZ_TEMP = 4, PZ_XI = 5, &

@@ -59,6 +59,7 @@ def lib():
         L.pz_ensemble_destroy.argtypes = [vp]
         L.pz_ensemble_last_error.argtypes = [vp]
         L.pz_ensemble_last_error.restype = ctypes.c_char_p
+        L.pz_ensemble_policy.argtypes = [vp, ctypes.POINTER(ip), dp]
         L.pz_adjoint_forward.argtypes = [vp, ip, dp, ip, ip, dp]
         L.pz_adjoint_backward.argtypes = [vp, ctypes.c_double, dp, dp]
         _LIB = L
@@ -69,7 +70,7 @@ EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_
            "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step",
            "pz_time", "pz_ekin_3d", "pz_nusselt", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
            "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
-           "pz_ensemble_destroy", "pz_ensemble_last_error", "pz_adjoint_forward", "pz_adjoint_backward"]
+           "pz_ensemble_destroy", "pz_ensemble_last_error", "pz_ensemble_policy", "pz_adjoint_forward", "pz_adjoint_backward"]
 
 OBJECTIVES = {"nutop": 0, "temp": 2}
 
@@ -234,6 +235,13 @@ class Ensemble:
         if rc:
             msg = lib().pz_ensemble_last_error(self.e)
             raise PizzaError(f"pz_ensemble_step failed ({rc}): {msg.decode() if msg else ''}")
+
+    def policy(self):
+        """(policy picked by the autotuner, trial step times in ms)."""
+        pol = ctypes.c_int()
+        ms = np.zeros(4)
+        lib().pz_ensemble_policy(self.e, ctypes.byref(pol), _dp(ms))
+        return pol.value, ms
 
     def close(self):
         if getattr(self, "e", None):

@@ -21,7 +21,8 @@ def _members(cfgs, seed_fn):
     return runs
 
 
-def _check_equal(cfgs, seed_fn, nsteps):
+def _check_equal(cfgs, seed_fn, nsteps, tol=0.0):
+    """tol = 0: bitwise; else the largest relative difference per field."""
     from pizza_amd.runtime import Ensemble
     print(f"case: ensemble of {len(cfgs)} runs, {nsteps} steps", flush=True)
     alone = _members(cfgs, seed_fn)
@@ -31,14 +32,20 @@ def _check_equal(cfgs, seed_fn, nsteps):
     ens = Ensemble(batch)
     ens.step(nsteps // 2)
     ens.step(nsteps - nsteps // 2)
+    pol = ens.policy()
     ens.close()
     for k, (a, b) in enumerate(zip(alone, batch)):
         assert a.time() == b.time(), k
         for f in FIELDS:
-            assert np.array_equal(a.get_field(f), b.get_field(f)), (k, f)
+            fa, fb = a.get_field(f), b.get_field(f)
+            if tol == 0.0:
+                assert np.array_equal(fa, fb), (k, f, pol)
+            else:
+                assert np.abs(fa - fb).max() <= tol * max(np.abs(fa).max(), 1e-300), (k, f, pol)
+    return pol
 
 
-def test_uneven_collocation_ensemble_equals_single_runs():
+def _c3_pick():
     from pizza_amd import workloads as W
     cfgs = W.c3_ensemble(nruns=64)
     pick = []
@@ -46,7 +53,27 @@ def test_uneven_collocation_ensemble_equals_single_runs():
         if c.n_r_max not in [p.n_r_max for p in pick] or (c.m_max == 170 and all(p.m_max != 170 for p in pick)):
             pick.append(c)
     assert len({c.n_r_max for c in pick}) == 4
-    _check_equal(pick, W.seed_member, 6)
+    return pick
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2, 3])
+def test_uneven_collocation_ensemble_equals_single_runs(policy, monkeypatch):
+    """Every m-loop policy of the autotuner: policies 0 and 2 (radial rows in
+    LDS or read through L2) run the single-run arithmetic bit for bit;
+    1 and 3 (256 threads) sum the dense products in another order."""
+    from pizza_amd import workloads as W
+    monkeypatch.setenv("PZ_ENSEMBLE_POLICY", str(policy))
+    pol = _check_equal(_c3_pick(), W.seed_member, 6, tol=0.0 if policy in (0, 2) else 1e-12)
+    assert pol[0] == policy
+
+
+def test_autotuned_ensemble_matches_single_runs(monkeypatch):
+    """The autotuner times every policy from the same member states and
+    restores them: the tuned ensemble still matches the single runs."""
+    from pizza_amd import workloads as W
+    monkeypatch.delenv("PZ_ENSEMBLE_POLICY", raising=False)
+    pol, ms = _check_equal(_c3_pick(), W.seed_member, 6, tol=1e-12)
+    assert np.all(ms > 0) and ms[pol] == ms.min()
 
 
 def test_integration_ensemble_equals_single_runs():

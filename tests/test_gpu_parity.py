@@ -224,6 +224,47 @@ def test_time_scheme_parity(scheme, method):
         assert err < max(1e-10, 10 * floors[0][f]), (f, err, floors[0][f])
 
 
+ASSEMBLY = ["SSP222", "LSDIRK222", "ARS232", "ARS233", "SSP322", "SSP332", "SSP433", "ARS343", "MARS343",
+            "KC343", "KC564", "KC785", "DBM453"]
+
+
+@pytest.mark.parametrize("scheme", ASSEMBLY)
+def test_assembly_scheme_parity(scheme):
+    """DIRK schemes with an assembly stage (collocation): robin_bc boundary
+    values, psi from the assembled omega and the first-stage implicit terms,
+    fields against the oracle after 1 and 12 steps of the n_r = 33
+    thermochemical physics."""
+    from pizza_amd import config as C
+    from pizza_amd import workloads as W
+    cfg = W.small_bpr353()
+    cfg.time_scheme = getattr(C, scheme)
+    cfg.l_cheb_coll = 1
+    cfg.dtMax = 2.0e-6
+    print(f"case: assembly scheme {scheme}", flush=True)
+    floors = _floor(cfg, [1, 12], lambda x: W.seed_run(x, cfg))
+    g, o = _pair(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg)
+        x.start()
+    for i, n in enumerate((1, 11)):
+        g.step(n)
+        o.step(n)
+        assert abs(g.time()[0] - o.time()[0]) < 1e-15
+        for f in _fields(cfg):
+            err = rel(g.get_field(f), o.get_field(f))
+            assert err < max(1e-11 if i == 0 else 1e-10, 10 * floors[i][f]), (f, i, err, floors[i][f])
+
+
+def test_assembly_schemes_rejected_off_collocation():
+    from pizza_amd import config as C
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import PizzaError, Run
+    cfg = W.small_bpr353()
+    cfg.time_scheme = C.ARS343
+    with pytest.raises(PizzaError):
+        Run(cfg)
+
+
 @pytest.mark.parametrize("case", ["c1", "bpr353_xi"])
 def test_collocation_parity(case):
     """Collocation method: fields against the oracle after 1 and 20 steps,
