@@ -38,6 +38,7 @@ def lib():
         L.or_nusselt.argtypes = [vp, dp]
         L.or_robin.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp, ctypes.c_double, ctypes.c_double, dp, dp]
         L.or_ass_solve.argtypes = [vp, ip, dp, dp]
+        L.or_fd_ell_solve.argtypes = [vp, ip, dp, dp]
         L.or_radial.argtypes = [vp, dp]
         L.or_dirk_tables.argtypes = [ip, ctypes.POINTER(ip), ctypes.POINTER(ip), dp, dp, dp, dp, dp]
         L.or_costf.argtypes = [vp, dp, ip]
@@ -143,6 +144,15 @@ class Oracle:
         out = np.zeros(self.n_r, dtype=np.complex128)
         self._check(lib().or_ass_solve(self.h, mi, _dp(w.view(np.float64)), _dp(out.view(np.float64))),
                     "or_ass_solve")
+        return out
+
+    def fd_ell_solve(self, mi, omega):
+        """psi from omega by the finite-difference assembly solve of mode mi
+        (psi = 0 at both walls; omega's wall rows are ignored)."""
+        w = np.ascontiguousarray(np.asarray(omega, dtype=np.complex128))
+        out = np.zeros(self.n_r, dtype=np.complex128)
+        self._check(lib().or_fd_ell_solve(self.h, mi, _dp(w.view(np.float64)), _dp(out.view(np.float64))),
+                    "or_fd_ell_solve")
         return out
 
     def radial(self):

@@ -1817,8 +1817,8 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->fd = cfg->l_finite_diff != 0;
     if (h->fd) h->coll = 0;
     if (ts_init(&h->ts, cfg->time_scheme, cfg->courfac)) { free(h); return -14; }
-    /* the assembly stage is restated for the collocation method only */
-    if (h->ts.l_assembly && !h->coll) { free(h); return -15; }
+    /* the assembly stage is restated for collocation and finite differences */
+    if (h->ts.l_assembly && !h->coll && !h->fd) { free(h); return -15; }
     /* namelists.f90:398-526 */
     h->l_heat = cfg->ra != 0.0;
     h->l_chem = cfg->raxi != 0.0;
@@ -3673,6 +3673,86 @@ static double rint_fd(or_handle *h, const double *f)
 #undef SIMPS
     return -0.5 * s;
 }
+
+/* assemble_temp_Rdist / assemble_xi_Rdist (update_temp_fd.f90:317-385,
+ * update_xi_fd.f90:293-361): the assembled right-hand side is the new field
+ * (real at m = 0), zero on a fixed-value wall; ghosts by fill_ghosts_*,
+ * then the first-stage implicit terms */
+static void fd_assemble_scalar(or_handle *h, int is_temp)
+{
+    int N = h->N;
+    tscheme *t = &h->ts;
+    tarray *a = is_temp ? &h->dTdt : &h->dxidt;
+    cplx *g = is_temp ? h->tg : h->xg, *fld = is_temp ? h->temp : h->xi;
+    int ktop = is_temp ? h->c.ktopt : h->c.ktopxi, kbot = is_temp ? h->c.kbott : h->c.kbotxi;
+    ts_assemble(h, t, a, h->work);
+    for (int i = 0; i < h->nm; i++) {
+        for (int k = 0; k < N; k++) {
+            cplx w = h->work[(size_t)i * N + k];
+            TG(g, i, k) = h->idx2m[i] == 0 ? creal(w) : w;
+        }
+        if (ktop == 1) TG(g, i, 0) = 0.0;
+        if (kbot == 1) TG(g, i, N - 1) = 0.0;
+    }
+    fd_fill_ghosts_scalar(h, g, ktop, kbot);
+    fd_rhs_imp_scalar(h, g, is_temp ? h->dtemp : h->dxi, a, 1, t->l_imp_calc_rhs[0],
+                      is_temp ? h->TdiffFac : h->XidiffFac);
+    for (int i = 0; i < h->nm; i++)
+        for (int k = 0; k < N; k++) fld[(size_t)i * N + k] = TG(g, i, k);
+}
+
+/* -L psi = omega for one m /= 0 with psi = 0 at both walls: the
+ * tridiagonal matrix of get_elliptic_mat_Rdist (update_psi_fd.f90:838-888)
+ * factorised by prepare_mat_3 and solved by solver_up_3 / solver_dn_3.
+ * x: N + 2 values, x[k + 1] at point k; on entry the omega rows
+ * (x[1] and x[N] are set to zero), on return psi */
+static void fd_ell_solve(or_handle *h, int m, cplx *x)
+{
+    int N = h->N;
+    double *d = dalloc(N), *lo = dalloc(N), *up = dalloc(N);
+    double dm2 = (double)m * (double)m;
+    for (int k = 0; k < N; k++) {
+        const double *a1 = h->fdr + 3 * k, *a2 = h->fddr + 3 * k;
+        double c1 = h->beta[k] + h->or1[k];
+        d[k] = -(a2[1] + c1 * a1[1] + (h->dbeta[k] + h->beta[k] * h->or1[k] - dm2 * h->or2[k]));
+        up[k] = -(a2[2] + c1 * a1[2]);
+        lo[k] = -(a2[0] + c1 * a1[0]);
+    }
+    d[0] = 1.0; up[0] = 0.0; lo[0] = 0.0;          /* non-penetration rows */
+    d[N - 1] = 1.0; up[N - 1] = 0.0; lo[N - 1] = 0.0;
+    fd_lu3(N, d, lo, up);
+    x[0] = 0.0; x[1] = 0.0; x[N] = 0.0; x[N + 1] = 0.0;
+    fd_solve3(N, d, lo, up, x);
+    free(d); free(lo); free(up);
+}
+
+/* assemble_psi_Rloc (update_psi_fd.f90:472-606): u_phi(m = 0) is the
+ * assembled right-hand side (zero on a no-slip wall); for m /= 0 psi solves
+ * -L psi = omega (fd_ell_solve); ghosts by fill_ghosts_psi, then u_s,
+ * u_phi, omega and the first-stage implicit terms */
+static void fd_assemble_psi(or_handle *h)
+{
+    int N = h->N;
+    tscheme *t = &h->ts;
+    ts_assemble(h, t, &h->dpsidt, h->work);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        const cplx *w = h->work + (size_t)i * N;
+        if (m == 0) {
+            for (int k = 0; k < N; k++) h->u0g[k + 1] = w[k];
+            if (h->c.ktopv != 1) h->u0g[1] = 0.0;
+            if (h->c.kbotv != 1) h->u0g[N] = 0.0;
+            h->u0g[0] = 0.0;
+            h->u0g[N + 1] = 0.0;
+            continue;
+        }
+        for (int k = -2; k <= N + 1; k++) PG(h->pg, i, k) = 0.0;
+        for (int k = 0; k < N; k++) PG(h->pg, i, k) = w[k];
+        fd_ell_solve(h, m, &PG(h->pg, i, -1));
+    }
+    fd_fill_ghosts_psi(h);
+    fd_rhs_imp_psi(h, &h->dpsidt, 1, t->l_imp_calc_rhs[0]);
+}
 #undef TG
 #undef PG
 
@@ -3746,7 +3826,11 @@ static int one_step(or_handle *h)
         }
         t->istage++;
     }
-    if (t->l_assembly) { /* assemble_stage (step_time.f90:353-368, mloop.f90:105-160) */
+    if (t->l_assembly && h->fd) { /* assemble_stage_Rdist (step_time.f90:353-362, mloop_fd.f90:241-274) */
+        if (h->l_heat) fd_assemble_scalar(h, 1);
+        if (h->l_chem) fd_assemble_scalar(h, 0);
+        fd_assemble_psi(h);
+    } else if (t->l_assembly) { /* assemble_stage (step_time.f90:363-368, mloop.f90:105-160) */
         if (h->l_heat) coll_assemble_scalar(h, 1);
         if (h->l_chem) coll_assemble_scalar(h, 0);
         int info = coll_assemble_psi(h);
@@ -4033,6 +4117,18 @@ int or_ass_solve(or_handle *h, int mi, const double *omega, double *psi)
     costf_vec(h, rhs);
     memcpy(psi, rhs, sizeof(cplx) * N);
     free(rhs);
+    return 0;
+}
+
+int or_fd_ell_solve(or_handle *h, int mi, const double *omega, double *psi)
+{
+    int N = h->N;
+    if (!h->fd || mi <= 0 || mi >= h->nm) return -1;
+    cplx *x = calloc(N + 2, sizeof(cplx));
+    memcpy(x + 1, omega, sizeof(cplx) * N);
+    fd_ell_solve(h, h->idx2m[mi], x);
+    memcpy(psi, x + 1, sizeof(cplx) * N);
+    free(x);
     return 0;
 }
 

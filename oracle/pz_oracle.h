@@ -33,6 +33,8 @@ int or_nusselt(or_handle *h, double out[2]);
 int or_robin(or_handle *h, double at, double bt, const double *rtop, double ab, double bb, const double *rbot,
              double *f);
 int or_ass_solve(or_handle *h, int mi, const double *omega, double *psi);
+/* -L psi = omega of the finite-difference assembly stage, psi = 0 at the walls */
+int or_fd_ell_solve(or_handle *h, int mi, const double *omega, double *psi);
 int or_radial(or_handle *h, double *out);
 int or_dirk_tables(int scheme, int *nstages, int *l_assembly, double *wimp_lin0, double *bimp, double *bexp,
                    double *bass_imp, double *bass_exp);

@@ -786,7 +786,8 @@ static void allow_full_lds(int lim)
                              (const void *)k_build_inv_batch, (const void *)k_mcoll_batch<true>, (const void *)k_mcoll_batch<false>, (const void *)k_fd_build,
                              (const void *)k_fd_mloop, (const void *)k_fd_start, (const void *)k_radial_loop_big,
                              (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
-                             (const void *)k_start_coll_adj, (const void *)k_assemble_coll};
+                             (const void *)k_start_coll_adj, (const void *)k_assemble_coll,
+                             (const void *)k_assemble_fd};
     for (const void *k : kernels) {
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, k));
@@ -1145,8 +1146,9 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
     else {
         Scheme probe;
         if (!make_scheme(c.time_scheme, c.courfac, probe)) why = "time scheme not implemented";
-        else if (probe.l_assembly && (!c.l_cheb_coll || c.l_finite_diff))
-            why = "DIRK schemes with an assembly stage are implemented for the collocation method only";
+        else if (probe.l_assembly && !c.l_cheb_coll && !c.l_finite_diff)
+            why = "DIRK schemes with an assembly stage are implemented for the collocation and finite-difference "
+                  "methods only";
     }
     if (!why.empty()) {
         int rc = fail(h, -2, why);
@@ -1453,7 +1455,9 @@ static void launch_step(pz_handle *h)
                                (const PzDev *)h->ddev, stage, exp_calc);
     }
     // assemble_stage (step_time.f90:353-368)
-    if (s.l_assembly)
+    if (s.l_assembly && h->d.fd)
+        hipLaunchKernelGGL(k_assemble_fd, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev);
+    else if (s.l_assembly)
         hipLaunchKernelGGL(k_assemble_coll, dim3(h->nm), dim3(ASM_THREADS), asm_lds_bytes(h->N), h->stream,
                            (const PzDev *)h->ddev);
 }

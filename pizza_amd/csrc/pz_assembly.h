@@ -348,3 +348,115 @@ __global__ void __launch_bounds__(ASM_THREADS) k_assemble_coll(const PzDev *__re
         p.P_impl[off + k] = v;
     }
 }
+
+// ---------------------------------------------------------------------------
+// The assembly stage of the finite-difference method for one m per
+// workgroup (assemble_stage_Rdist, mloop_fd.f90:241-274):
+// assemble_temp_Rdist / assemble_xi_Rdist (update_temp_fd.f90:317-385,
+// update_xi_fd.f90:293-361) take the assembled right-hand side as the new
+// field (real at m = 0, zero on a fixed-value wall); assemble_psi_Rloc
+// (update_psi_fd.f90:472-606) takes it as u_phi for m = 0 (zero on a
+// no-slip wall) and solves -L psi = omega with psi = 0 at both walls for
+// m /= 0 (get_elliptic_mat_Rdist, :838-888: tridiagonal, factorised here
+// by one thread).  Ghost zones and the first-stage implicit terms follow as
+// in the m-loop.  LDS: FdLayout(N).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_assemble_fd(const PzDev *__restrict__ pd)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pd;
+    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi], tid = threadIdx.x;
+    const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
+    const PzScheme *ss = p.ss;
+    const FdLayout lay(N);
+    char *lb = (char *)smem;
+    cd *gT = (cd *)(lb + lay.gT), *gX = (cd *)(lb + lay.gX), *gP = (cd *)(lb + lay.gP);
+    cd *V = (cd *)(lb + lay.vec);
+    cd *aT = V, *aX = V + N, *aP = V + 2 * N, *usv = V + 3 * N, *Tv = V + 4 * N, *Xv = V + 5 * N;
+    double *eD = (double *)(lb + lay.mT), *eL = eD + N, *eU = eD + 2 * N;
+    const bool hT = p.l_heat, hX = p.l_chem;
+    const RadView R(p.radc, N);
+    const FdView S(p.fdc, N);
+
+    // explicit terms of the last stage (finish_explicit_assembly_Rdist,
+    // mloop_fd.f90:78-110), which the skipped m-loop of that stage would do
+    if (p.l_exp_calc[p.nstages - 1]) {
+        const size_t qe = (size_t)(p.nstages - 1) * F + off;
+        for (int k = tid; k < N; k += blockDim.x) {
+            if (hT) { aT[k] = p.dVsT[off + k]; Tv[k] = p.temp[off + k]; }
+            if (hX) { aX[k] = p.dVsXi[off + k]; Xv[k] = p.xi[off + k]; }
+            aP[k] = p.dVsOm[off + k];
+            usv[k] = p.us[off + k];
+        }
+        __syncthreads();
+        for (int k = tid; k < N; k += blockDim.x) {
+            const double o1 = R.or1(k);
+            const cd us = usv[k];
+            if (hT)
+                p.T_expl[qe + k] = csub(csub(p.T_expl[qe + k], cscale(o1, fd_dr_at(S, aT, k, N))),
+                                        cscale(R.dtcond(k) - p.tadvz * R.beta(k) * R.tcond(k), us));
+            if (hX)
+                p.X_expl[qe + k] = csub(csub(p.X_expl[qe + k], cscale(o1, fd_dr_at(S, aX, k, N))),
+                                        cscale(R.dxicond(k), us));
+            if (m != 0) {
+                cd v = csub(p.P_expl[qe + k], cscale(o1, fd_dr_at(S, aP, k, N)));
+                if (!p.l_coriolis_imp) v = cadd(v, cscale(p.CorFac * R.beta(k), us));
+                if (hT) v = csub(v, cmul(mk(0.0, p.BuoFac * R.rgrav(k) * o1 * (double)m), Tv[k]));
+                if (hX) v = csub(v, cmul(mk(0.0, p.ChemFac * R.rgrav(k) * o1 * (double)m), Xv[k]));
+                p.P_expl[qe + k] = v;
+            }
+        }
+        __syncthreads();
+    }
+
+    // assembled fields and right-hand sides with their wall rows
+    const cd z = mk(0.0, 0.0);
+    const double dm2 = (double)m * (double)m;
+    for (int k = tid; k < N; k += blockDim.x) {
+        const size_t q = off + k;
+        if (hT) {
+            cd v = assemble_elem(p, ss, p.T_old, p.T_impl, p.T_expl, q);
+            if (m == 0) v.y = 0.0;
+            if ((k == 0 && p.ktopt == 1) || (k == N - 1 && p.kbott == 1)) v = z;
+            gT[k + 1] = v;
+        }
+        if (hX) {
+            cd v = assemble_elem(p, ss, p.X_old, p.X_impl, p.X_expl, q);
+            if (m == 0) v.y = 0.0;
+            if ((k == 0 && p.ktopxi == 1) || (k == N - 1 && p.kbotxi == 1)) v = z;
+            gX[k + 1] = v;
+        }
+        cd v = assemble_elem(p, ss, p.P_old, p.P_impl, p.P_expl, q);
+        if (m == 0) {
+            if ((k == 0 && p.ktopv != 1) || (k == N - 1 && p.kbotv != 1)) v = z;
+            gP[k + 1] = v;
+        } else {
+            if (k == 0 || k == N - 1) v = z;
+            gP[k + 2] = v;
+            // the elliptic matrix row k (identity rows at the walls)
+            const double c1 = R.beta(k) + R.or1(k);
+            const bool wall = k == 0 || k == N - 1;
+            eD[k] = wall ? 1.0 : -(S.ddr(k, 1) + c1 * S.dr(k, 1) + (R.dbeta(k) + R.beta(k) * R.or1(k) - dm2 * R.or2(k)));
+            eU[k] = wall ? 0.0 : -(S.ddr(k, 2) + c1 * S.dr(k, 2));
+            eL[k] = wall ? 0.0 : -(S.ddr(k, 0) + c1 * S.dr(k, 0));
+        }
+    }
+    if (tid == 0) {
+        gT[0] = z; gT[N + 1] = z; gX[0] = z; gX[N + 1] = z;
+        if (m == 0) { gP[0] = z; gP[N + 1] = z; }
+        else { gP[0] = z; gP[1] = z; gP[N + 2] = z; gP[N + 3] = z; }
+    }
+    __syncthreads();
+    if (tid == 0 && m != 0) {
+        fd_lu3(N, eD, eL, eU);
+        fd_tri_solve(N, eD, eL, eU, gP + 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (hT) fd_ghost_scalar(gT, N, p.ktopt, p.kbott);
+        if (hX) fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi);
+        fd_ghost_psi(p, R, m, gP);
+    }
+    __syncthreads();
+    fd_implicit(p, R, S, mi, m, gT, gX, gP, 1, p.l_imp_calc_rhs[0], nullptr);
+}

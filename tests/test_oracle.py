@@ -286,6 +286,40 @@ def test_fd_converges_to_chebyshev_solution():
     assert errs[(129, "om")] < 2e-2 and errs[(129, "om")] < errs[(65, "om")]
 
 
+@pytest.mark.parametrize("scheme", ["ARS343", "KC564"])
+def test_fd_assembly_converges_to_chebyshev_assembly(scheme):
+    """The assembly stage on finite differences (assemble_stage_Rdist,
+    mloop_fd.f90:241-274) against the same scheme on the collocation method,
+    whose assembly is pinned separately, after 10 steps of the same dt: the
+    time errors agree to leading order, so T, u_s and u_phi differences
+    shrink by about 4 per halving of the grid spacing."""
+    from numpy.polynomial import chebyshev as C
+    from pizza_amd import workloads as W
+
+    def run(cfg):
+        o = Oracle(cfg)
+        W.seed_run(o, cfg, amp_t=1e-3)
+        o.start()
+        o.step(10)
+        return o
+
+    oc = run(_fd_cfg(49, False, time_scheme=scheme))
+    rc, xc = oc.grid()
+    V = C.chebvander(xc, len(xc) - 1)
+    errs = {}
+    for n_r in (65, 129):
+        of = run(_fd_cfg(n_r, True, time_scheme=scheme))
+        rf, _ = of.grid()
+        xq = (2 * rf - rc[0] - rc[-1]) / (rc[0] - rc[-1])
+        for f in ("temp", "us", "up"):
+            coef = np.linalg.solve(V, oc.get_field(f).T).T
+            ci = np.array([C.chebval(xq, c) for c in coef])
+            errs[(n_r, f)] = (np.abs(of.get_field(f) - ci) / np.max(np.abs(ci))).max()
+    for f in ("temp", "us", "up"):
+        assert errs[(65, f)] < 1e-2, (f, errs[(65, f)])
+        assert errs[(65, f)] / errs[(129, f)] > 3.0, (f, errs[(65, f)], errs[(129, f)])
+
+
 def test_fd_stencils_exact_on_polynomials():
     """Fornberg weights on the stretched grid (finite_differences.f90:501-546):
     the 3-point first and second derivatives are exact for quadratics, the

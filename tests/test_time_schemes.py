@@ -107,7 +107,7 @@ def test_scheme_names_follow_the_reference_matching():
     assert C.scheme_id("dp1-242") == C.DP1_242
     assert C.scheme_id("ARS343") == C.ARS343
     assert C.scheme_id("kc785") == C.KC785
-    # initialize() tests 'ARS343' before 'MARS343' (dirk_schemes.f90:183-198)
+    # initialize() tests 'ARS343' before 'MARS343' (dirk_schemes.f90:183-192)
     # and 'ARS343' is a substring of 'MARS343': the name selects ARS343
     assert C.scheme_id("MARS343") == C.ARS343
 
@@ -212,3 +212,27 @@ def test_assembly_matrix_recovers_psi():
         om[[0, 1, N - 2, N - 1]] = 0.0
         got = o.ass_solve(mi, om)
         assert np.abs(got - psi).max() < 1e-9 * np.abs(psi).max(), (mi, np.abs(got - psi).max())
+
+
+def test_fd_elliptic_solve_recovers_quadratic_psi():
+    """get_elliptic_mat_Rdist (update_psi_fd.f90:838-888), the psi solve of
+    the finite-difference assembly stage: the three-point stencils are exact
+    for quadratics on any grid, so psi = c (r - r_o)(r - r_i), zero at both
+    walls, comes back from omega = -(psi'' + (beta + 1/r) psi' + (beta' +
+    beta/r - m^2/r^2) psi) on the stretched grid to rounding."""
+    from oracle.oracle import Oracle
+    from pizza_amd.config import default_config
+    cfg = default_config(n_r_max=33, n_cheb_max=33, m_max=8, time_scheme="ARS343", radial_scheme="FD",
+                         fd_ratio=0.1, ra=1e5, ek=1e-3, pr=1.0, radratio=0.35, buo_term="explicit")
+    o = Oracle(cfg)
+    r, or1, or2, beta, dbeta = o.radial()
+    assert np.ptp(np.diff(r)) > 1e-3 * abs(r[1] - r[0])          # the grid is stretched
+    c = 0.7 - 1.3j
+    psi = c * (r - r[0]) * (r - r[-1])
+    p1 = c * (2 * r - r[0] - r[-1])
+    for mi in (1, 3, 8):
+        om = -(2 * c + (beta + or1) * p1 + (dbeta + beta * or1 - mi**2 * or2) * psi)
+        om[[0, -1]] = 5.0 + 2.0j                                  # ignored: psi = 0 rows
+        got = o.fd_ell_solve(mi, om)
+        assert got[0] == 0 and got[-1] == 0
+        assert np.abs(got - psi).max() < 1e-12 * np.abs(psi).max(), (mi, np.abs(got - psi).max())
