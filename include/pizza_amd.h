@@ -243,6 +243,9 @@ int pz_ensemble_policy(const pz_ensemble *e, int *policy, double *trial_ms);
  * outputs.f90:
  *   PZ_OBJ_NUTOP  J = (1/K) sum_{n=1..K} NuTop(n), the outer-wall Nusselt
  *                 number (outputs.f90:729-755) after each of the K steps;
+ *   PZ_OBJ_EKIN   J = (1/K) sum_{n=1..K} E(n), the kinetic energy
+ *                 us2_3D + up2_3D + uz2_3D of the e_kin_3D series
+ *                 (outputs.f90:552-570, without round_off) after each step;
  *   PZ_OBJ_TEMP   J = sum Re(conj(W) T(K)) over the spectral temperature
  *                 after the last step, W = weights (layout of pz_set_field).
  * pz_adjoint_forward runs K steps from the state pz_start left, keeping a
@@ -257,7 +260,7 @@ int pz_ensemble_policy(const pz_ensemble *e, int *policy, double *trial_ms);
  * restores the state after K steps.  Either output may be null.  Returns -7
  * if dt (and so the matrices) changed inside the window.
  */
-enum pz_objective { PZ_OBJ_NUTOP = 0, PZ_OBJ_TEMP = 2 };
+enum pz_objective { PZ_OBJ_NUTOP = 0, PZ_OBJ_EKIN = 1, PZ_OBJ_TEMP = 2 };
 int pz_adjoint_forward(pz_handle *h, int objective, const double *weights, int nsteps, int ckpt_every,
                        double *J);
 int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, double *grad_bc);

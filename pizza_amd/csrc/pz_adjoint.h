@@ -25,6 +25,9 @@ struct PzAdjDev {
     // Uimp  m = 0: implicit term of u_phi from grid values (coll_uphi0_impl)
     const double *C, *G, *D1, *D2, *E1, *Uimp;
     const double *lnu; // d/dr at r_cmb of m = 0 grid values (Nusselt numbers)
+    // kinetic energy weights per radius: pi times the rInt_R quadrature
+    // weight times r h (u_phi) and r h + beta^2 r h^3 / 12 (u_s)
+    const double *kus, *kup;
     // adjoints of the radial loop outputs T_expl(1), dVsT, xi, omega
     cd *eT0, *aT0, *eX0, *aX0, *eP0, *aP0;
     // accumulated adjoints of the boundary rows of the T and xi systems
@@ -627,6 +630,44 @@ __global__ void __launch_bounds__(256) k_obj_acc(const PzDev *__restrict__ pd, c
     }
     v = block_reduce(v, red, 0);
     if (threadIdx.x == 0) acc[0] += v;
+}
+
+// kinetic energy of a state (outputs.f90:552-570), us2_3D + up2_3D + uz2_3D
+// = sum over m and r of c_m (kus_r |u_s|^2 + kup_r |u_phi|^2), c_0 = 1 on
+// the real parts only, c_m = 2 for m >= 1; accumulated in acc[0]
+__global__ void __launch_bounds__(256) k_obj_ke(const PzDev *__restrict__ pd, const PzAdjDev *__restrict__ pad,
+                                                double *acc)
+{
+    __shared__ double red[8];
+    const PzDev &p = *pd;
+    const size_t F = (size_t)p.nm * p.N;
+    double v = 0.0;
+    for (size_t q = threadIdx.x; q < F; q += blockDim.x) {
+        const int i = (int)(q / p.N), k = (int)(q % p.N);
+        const cd u = p.us[q], w = p.up[q];
+        if (p.idx2m[i] == 0) v += pad->kus[k] * u.x * u.x + pad->kup[k] * w.x * w.x;
+        else v += 2.0 * (pad->kus[k] * (u.x * u.x + u.y * u.y) + pad->kup[k] * (w.x * w.x + w.y * w.y));
+    }
+    v = block_reduce(v, red, 0);
+    if (threadIdx.x == 0) acc[0] += v;
+}
+// w times the derivative of that energy at the state (us, up):
+// lam(u_s) += 2 w c_m kus_r u_s, lam(u_phi) += 2 w c_m kup_r u_phi (no
+// imaginary parts at m = 0)
+__global__ void __launch_bounds__(256) k_adj_seed_ke(const cd *__restrict__ us, const cd *__restrict__ up,
+                                                     const PzDev *__restrict__ plam,
+                                                     const PzAdjDev *__restrict__ pad, double w)
+{
+    const PzDev &L = *plam;
+    const size_t F = (size_t)L.nm * L.N;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < F; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q / L.N), k = (int)(q % L.N);
+        const bool m0 = L.idx2m[i] == 0;
+        const double c = m0 ? 2.0 * w : 4.0 * w, a = c * pad->kus[k], b = c * pad->kup[k];
+        const cd ls = L.us[q], lp = L.up[q], u = us[q], v = up[q];
+        L.us[q] = mk(ls.x + a * u.x, m0 ? ls.y : ls.y + a * u.y);
+        L.up[q] = mk(lp.x + b * v.x, m0 ? lp.y : lp.y + b * v.y);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -383,7 +383,7 @@ struct AdjCtx {
     // and their ck forward views, the state at the end of the window
     char *ckpt = nullptr, *seg = nullptr, *fin = nullptr;
     PzDev *dviews = nullptr;
-    int cap_ckpt = 0, cap_ck = 0;
+    int cap_ckpt = 0, cap_ck = 0, views_ck = 0; // views_ck: the ck the views were built for
     int objective = -1, K = 0, ck = 0, nseg = 0;
     bool ready = false; // a forward sweep waits for its backward sweep
     // copies of T_inv, X_inv, U_inv, PC_inv (steady, start-up)
@@ -1701,6 +1701,21 @@ static void adj_init(pz_handle *h)
     double *D2 = h->dalloc<double>(NN), *E1 = h->dalloc<double>(NN), *U = h->dalloc<double>(NN);
     double *lnu = h->dalloc<double>(N);
     ad.C = C; ad.G = G; ad.D1 = D1; ad.D2 = D2; ad.E1 = E1; ad.Uimp = U; ad.lnu = lnu;
+    {
+        // kinetic energy weights (outputs.f90:552-570): rInt_R is linear, so
+        // its quadrature weights are its values on the unit vectors
+        const double pi = 4.0 * std::atan(1.0);
+        std::vector<double> kus(N), kup(N), e(N, 0.0);
+        for (int k = 0; k < N; k++) {
+            e[k] = 1.0;
+            const double q = pi * rint_r(h, e), rh = h->r[k] * h->height[k];
+            e[k] = 0.0;
+            kup[k] = q * rh;
+            kus[k] = q * (rh + h->beta[k] * h->beta[k] * h->r[k] * std::pow(h->height[k], 3) / 12.0);
+        }
+        ad.kus = h->upload(kus);
+        ad.kup = h->upload(kup);
+    }
     ad.eT0 = h->dalloc<cd>(F); ad.aT0 = h->dalloc<cd>(F); ad.eX0 = h->dalloc<cd>(F);
     ad.aX0 = h->dalloc<cd>(F); ad.eP0 = h->dalloc<cd>(F); ad.aP0 = h->dalloc<cd>(F);
     ad.gtop = h->dalloc<cd>(nm); ad.gbot = h->dalloc<cd>(nm);
@@ -1753,9 +1768,13 @@ static void adj_window(pz_handle *h, int K, int ck)
         HIPCHK(hipMalloc(&a->dviews, sizeof(PzDev) * 2 * ck));
         a->cap_ckpt = nseg;
         a->cap_ck = ck;
+        a->views_ck = 0;
+    }
+    if (a->views_ck != ck) {
         // two segment buffers of ck + 1 snapshots; view j of buffer b: the
         // state before step j of the segment, with the scheme state the
-        // step ran with (the one stored before step j + 1)
+        // step ran with (the one stored before step j + 1).  The layout
+        // follows ck, so the views are rebuilt whenever ck changes.
         std::vector<PzDev> v(2 * ck, h->d);
         for (int b = 0; b < 2; b++)
             for (int j = 0; j < ck; j++) {
@@ -1766,6 +1785,7 @@ static void adj_window(pz_handle *h, int K, int ck)
                 v[b * ck + j].ss = next.ss;
             }
         HIPCHK(hipMemcpy(a->dviews, v.data(), sizeof(PzDev) * 2 * ck, hipMemcpyHostToDevice));
+        a->views_ck = ck;
     }
     a->K = K;
     a->ck = ck;
@@ -1796,7 +1816,8 @@ extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *wei
                                   double *J)
 {
     if (!h || !J || nsteps < 1 || ckpt_every < 1) return -1;
-    if (objective != PZ_OBJ_NUTOP && objective != PZ_OBJ_TEMP) return fail(h, -1, "unknown objective");
+    if (objective != PZ_OBJ_NUTOP && objective != PZ_OBJ_EKIN && objective != PZ_OBJ_TEMP)
+        return fail(h, -1, "unknown objective");
     if (objective == PZ_OBJ_TEMP && !weights) return fail(h, -1, "the temperature objective needs weights");
     int rc = adj_check(h);
     if (rc) return rc;
@@ -1834,6 +1855,9 @@ extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *wei
                 if (objective == PZ_OBJ_NUTOP)
                     hipLaunchKernelGGL(k_obj_acc, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev,
                                        (const PzAdjDev *)a->dad, (const cd *)nullptr, a->acc);
+                else if (objective == PZ_OBJ_EKIN)
+                    hipLaunchKernelGGL(k_obj_ke, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev,
+                                       (const PzAdjDev *)a->dad, a->acc);
             }
         }
         if (objective == PZ_OBJ_TEMP)
@@ -1849,6 +1873,8 @@ extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *wei
         const double tiny = 10.0 * 2.220446049250313e-16;
         if (objective == PZ_OBJ_NUTOP)
             *J = std::fabs(h->dtcond_top) > tiny ? 1.0 + acc / ((double)nsteps * h->dtcond_top) : 1.0;
+        else if (objective == PZ_OBJ_EKIN)
+            *J = acc / (double)nsteps;
         else
             *J = acc;
         a->ready = true;
@@ -1871,6 +1897,9 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
         const double tiny = 10.0 * 2.220446049250313e-16;
         double wnu = 0.0;
         if (a->objective == PZ_OBJ_NUTOP && std::fabs(h->dtcond_top) > tiny) wnu = dJ / ((double)K * h->dtcond_top);
+        if (a->objective == PZ_OBJ_EKIN) wnu = dJ / (double)K;
+        // u_s and u_phi of a snapshot sit at fixed offsets of its block
+        const size_t off_us = (size_t)((char *)h->d.us - h->arena), off_up = (size_t)((char *)h->d.up - h->arena);
         HIPCHK(hipMemsetAsync(a->lam, 0, A, h->stream));
         for (cd *q : {a->ad.gtop, a->ad.gbot, a->ad.gtopxi, a->ad.gbotxi})
             HIPCHK(hipMemsetAsync(q, 0, nm * sizeof(cd), h->stream));
@@ -1908,10 +1937,16 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
             for (int j = len - 1; j >= 0; j--) {
                 const int n = sg * ck + j + 1; // the state after step n seeds the objective
                 const PzDev *fw = a->dviews + b * ck + j;
-                if (a->objective == PZ_OBJ_NUTOP)
+                if (a->objective == PZ_OBJ_NUTOP) {
                     hipLaunchKernelGGL(k_adj_seed_nu, dim3(1), dim3(256), 0, a->s2, (const PzDev *)a->dlam,
                                        (const PzAdjDev *)a->dad, wnu);
-                else if (n == K)
+                } else if (a->objective == PZ_OBJ_EKIN) {
+                    // the state after step n: snapshot j + 1 of the segment buffer
+                    const char *after = a->seg + A * (size_t)(b * (ck + 1) + j + 1);
+                    hipLaunchKernelGGL(k_adj_seed_ke, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, a->s2,
+                                       (const cd *)(after + off_us), (const cd *)(after + off_up),
+                                       (const PzDev *)a->dlam, (const PzAdjDev *)a->dad, wnu);
+                } else if (n == K)
                     hipLaunchKernelGGL(k_adj_seed_w, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, a->s2,
                                        (const PzDev *)a->dlam, (const cd *)a->grad);
                 hipLaunchKernelGGL(k_mcoll_adj, dim3(nm), dim3(512), lds_madj, a->s2, fw, (const PzDev *)a->dlam,

@@ -18,7 +18,7 @@ module pizza_amd_gpu
                                 PZ_SSP322 = 114, PZ_SSP332 = 115, PZ_SSP433 = 116, PZ_ARS343 = 117, &
                                 PZ_MARS343 = 118, PZ_KC343 = 119, PZ_KC564 = 120, PZ_KC785 = 121, PZ_DBM453 = 122
    ! fields (fields.f90)
-   integer(c_int), parameter :: PZ_OBJ_NUTOP = 0, PZ_OBJ_TEMP = 2
+   integer(c_int), parameter :: PZ_OBJ_NUTOP = 0, PZ_OBJ_EKIN = 1, PZ_OBJ_TEMP = 2
    integer(c_int), parameter :: PZ_US = 0, PZ_UP = 1, PZ_OM = 2, PZ_PSI = 3, PZ_TEMP = 4, PZ_XI = 5, &
                                 PZ_TEMP_HAT = 6, PZ_XI_HAT = 7, PZ_PSI_HAT = 8, PZ_DTEMP = 9
 

@@ -72,7 +72,7 @@ EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_
            "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
            "pz_ensemble_destroy", "pz_ensemble_last_error", "pz_ensemble_policy", "pz_adjoint_forward", "pz_adjoint_backward"]
 
-OBJECTIVES = {"nutop": 0, "temp": 2}
+OBJECTIVES = {"nutop": 0, "ekin": 1, "temp": 2}
 
 
 def _dp(a):
@@ -152,7 +152,8 @@ class Run:
     def adjoint_forward(self, objective, nsteps, ckpt_every=20, weights=None):
         """Runs nsteps steps from the started state keeping snapshots every
         ckpt_every steps; returns the objective J ('nutop': time-averaged
-        outer-wall Nusselt number; 'temp': sum Re(conj(weights) T) after the
+        outer-wall Nusselt number; 'ekin': time-averaged kinetic energy
+        us2_3D + up2_3D + uz2_3D; 'temp': sum Re(conj(weights) T) after the
         last step, weights complex (n_m, n_r))."""
         J = ctypes.c_double()
         wp = None

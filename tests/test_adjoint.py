@@ -12,6 +12,10 @@ The reference has no adjoint, so these tests pin it by properties:
 - the time-averaged Nusselt gradient over a longer window matches central
   differences (step 1e-6 of the seeded amplitude) of the GPU forward and of
   the oracle's forward, tolerance 1e-6 relative;
+- the time-averaged kinetic energy ('ekin'): an exact one-step dot-product
+  test (the energy is quadratic in the initial temperature over one step),
+  central differences over 40 steps against the GPU and oracle forwards,
+  and the gradient with respect to an outer-wall heat-flux pattern;
 - the torch.autograd binding returns the same gradient.
 """
 import numpy as np
@@ -76,12 +80,14 @@ def test_two_step_dot_product(scheme, amp_u):
 
 
 def test_checkpoint_interval_invariance():
+    """Intervals 1, 7, 3 on one handle: the last window reuses the larger
+    buffers of the second with a smaller interval."""
     from pizza_amd import workloads as W
     cfg = W.c1_collocation()
     print("case: adjoint checkpoint intervals", flush=True)
     run, T0, us, up, _ = _setup(cfg)
     grads = []
-    for ck in (1, 3, 7):
+    for ck in (1, 7, 3):
         J = _J(run, T0, us, up, "nutop", 7, ckpt=ck)
         g, bc = run.adjoint_backward()
         grads.append((J, g, bc))
@@ -192,3 +198,117 @@ def test_boundary_pattern_gradient(ktopt):
             fd = (js[0] - js[1]) / (2 * h)
             ad = gbc[m, 0].real if part == 1 else gbc[m, 0].imag
             assert abs(fd - ad) <= 1e-9 * max(abs(fd), 1e-30), (m, part, fd, ad)
+
+
+def _ekin(run):
+    e = run.ekin_3d()
+    return e[0] + e[1] + e[2]
+
+
+def test_ekin_objective_matches_ekin_series():
+    """'ekin' is the mean over the window of us2_3D + up2_3D + uz2_3D of the
+    e_kin_3D series (outputs.f90:552-570) after each step."""
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    run, T0, us, up, _ = _setup(cfg, amp_u=5.0e-2)
+    J = _J(run, T0, us, up, "ekin", 5, ckpt=2)
+    run.set_field("us", us)
+    run.set_field("up", up)
+    run.set_field("temp", T0)
+    run.start()
+    e = []
+    for _ in range(5):
+        run.step(1)
+        e.append(_ekin(run))
+    assert abs(J - np.mean(e)) < 1e-12 * abs(J), (J, np.mean(e))
+
+
+@pytest.mark.parametrize("amp_u", [0.0, 5.0e-2])
+def test_ekin_one_step_dot_product(amp_u):
+    """Over one step the velocity is affine in the initial temperature (the
+    advection products pair it with the fixed initial velocity), so the
+    kinetic energy is quadratic in it and central differences equal
+    <grad, d> up to rounding for any step size."""
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    print(f"case: adjoint ekin one step amp_u={amp_u}", flush=True)
+    run, T0, us, up, field = _setup(cfg, amp_u=amp_u)
+    _J(run, T0, us, up, "ekin", 1)
+    g, _ = run.adjoint_backward()
+    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
+    for k in range(3):
+        d = field(1.0e-3)
+        jp = _J(run, T0 + d, us, up, "ekin", 1)
+        jm = _J(run, T0 - d, us, up, "ekin", 1)
+        fd = (jp - jm) / 2.0
+        ad = _dot(g, d)
+        assert abs(fd - ad) <= 1e-9 * max(abs(fd), abs(ad)), (k, fd, ad)
+
+
+def test_ekin_gradient_vs_central_differences():
+    """dJ/dT0 of the kinetic energy averaged over 40 steps against central
+    differences of the GPU forward and of the oracle's forward."""
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    K = 40
+    print("case: adjoint ekin vs central differences", flush=True)
+    run, T0, us, up, field = _setup(cfg, amp_u=0.0)
+    zero = np.zeros_like(T0)
+    _J(run, T0, zero, zero, "ekin", K, ckpt=10)
+    g, _ = run.adjoint_backward()
+
+    def j_oracle(T):
+        o = Oracle(cfg)
+        o.set_field("temp", T)
+        o.start()
+        e = []
+        for _ in range(K):
+            o.step(1)
+            v = o.ekin_3d()
+            e.append(v[0] + v[1] + v[2])
+        return float(np.mean(e))
+
+    for k in range(4):
+        d = field(1.0e-3)
+        h = 1.0e-5
+        fd_gpu = (_J(run, T0 + h * d, zero, zero, "ekin", K) - _J(run, T0 - h * d, zero, zero, "ekin", K)) / (2 * h)
+        ad = _dot(g, d)
+        assert abs(fd_gpu - ad) <= 1e-6 * abs(ad), (k, fd_gpu, ad)
+        if k < 2:
+            fd_or = (j_oracle(T0 + h * d) - j_oracle(T0 - h * d)) / (2 * h)
+            assert abs(fd_or - ad) <= 1e-6 * abs(ad), (k, fd_or, ad)
+
+
+def test_ekin_flux_pattern_gradient():
+    """Gradient of the kinetic energy over two steps with respect to an
+    outer-wall heat-flux pattern (ktopt = 2, horizontal.f90:149-173, modes
+    m = 2 and 5) against central differences of the pattern coefficients."""
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Run
+    print("case: adjoint ekin flux pattern", flush=True)
+    base = [(2, 0.05, 0.02), (5, 0.01, -0.03)]
+
+    def make(pattern):
+        cfg = W.c1_collocation()
+        cfg.ktopt = 2
+        for n, (m, tr, ti) in enumerate(pattern):
+            cfg.t_top[3 * n], cfg.t_top[3 * n + 1], cfg.t_top[3 * n + 2] = m, tr, ti
+        return cfg
+
+    run, T0, us, up, _ = _setup(make(base), amp_u=5.0e-2)
+    _J(run, T0, us, up, "ekin", 2)
+    _, gbc = run.adjoint_backward()
+    for n, (m, tr, ti) in enumerate(base):
+        for part in (1, 2):
+            h = 1.0e-4
+            js = []
+            for sgn in (1, -1):
+                pat = [list(e) for e in base]
+                pat[n][part] += sgn * h
+                r2 = Run(make(pat))
+                js.append(_J(r2, T0, us, up, "ekin", 2))
+                r2.close()
+            fd = (js[0] - js[1]) / (2 * h)
+            ad = gbc[m, 0].real if part == 1 else gbc[m, 0].imag
+            assert abs(fd - ad) <= 1e-6 * max(abs(fd), abs(ad)), (m, part, fd, ad)
