@@ -209,8 +209,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true", help="measure only the ensemble factor")
     ap.add_argument("--ensemble-steps", type=int, default=100)
-    ap.add_argument("--adjoint", action="store_true",
-                    help="also measure the adjoint cost (off: its GPU run is suspended, see the design notes)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost")
     ap.add_argument("--adjoint-only", action="store_true", help="measure only the adjoint cost")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="run only the short sample profiled by the counter runs")
@@ -279,7 +278,7 @@ def main():
     }
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(steps=args.ensemble_steps)
-    if args.adjoint:
+    if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)

@@ -227,7 +227,8 @@ const char *pz_ensemble_last_error(const pz_ensemble *e);
  * of the whole ensemble under each policy from the same member states
  * (restored after every trial) and keeps the fastest.  Policies: 0 the
  * single-run shape (512 threads, radial rows in LDS), 1 256 threads
- * (n_r <= 128), 2 radial rows read through L2, 3 both.  Policies 0 and 2
+ * (n_r <= 128), 2 radial rows and transform tables read through L2, 3
+ * both.  Policies 0 and 2
  * give each member's pz_step results bit for bit; 1 and 3 sum the dense
  * products in another order (rounding-level differences).  The environment
  * variable PZ_ENSEMBLE_POLICY=k fixes the policy.  trial_ms (4 values, may

@@ -2222,8 +2222,8 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
 
 // m-loop shape of the collocation classes with n_r > 65 under a policy:
 // 0 the single-run shape (512 threads, radial rows in LDS), 1 256 threads
-// (n_r <= 128 only), 2 radial rows read through L2, 3 both (less LDS: more
-// workgroups per CU)
+// (n_r <= 128 only), 2 radial rows and transform tables read through L2, 3
+// both (less LDS: more workgroups per CU)
 static void ens_policy(pz_ensemble *e, int policy)
 {
     for (EnsClass &c : e->cls) {

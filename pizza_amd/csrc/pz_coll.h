@@ -336,14 +336,20 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
     cd *psiv = V + 9 * N, *omv = V + 10 * N, *upv = V + 11 * N, *usv = V + 12 * N;
     cd *Tv = V + 13 * N, *Xv = V + 14 * N, *bv = V + 15 * N, *b2 = V + 16 * N, *wv = V + 17 * N;
     cd *PT = V + 18 * N, *QT = V + 19 * N, *PX = V + 20 * N, *QX = V + 21 * N, *PP = V + 22 * N, *QP = V + 23 * N;
-    double *cosT = (double *)(lb + lay.cosT), *sinT = (double *)(lb + lay.sinT);
     cd *red = (cd *)(lb + lay.red), *fbuf = red;
     const bool hT = p.l_heat, hX = p.l_chem;
     const int ldi = p.ldi, nc = p.ncheb;
     const double dm = (double)m, dm2 = dm * dm;
-    lds_copy(cosT, p.cosT, sizeof(double) * p.L);
-    lds_copy(sinT, p.sinT, sizeof(double) * p.L);
-    if (RADLDS) lds_copy(lb + lay.rad, p.radc, sizeof(double) * PZ_NRADC * N);
+    // transform tables and radial rows: LDS copies, or global (L2) reads
+    const double *cosT = p.cosT, *sinT = p.sinT;
+    if (RADLDS) {
+        double *c = (double *)(lb + lay.cosT), *s = (double *)(lb + lay.sinT);
+        lds_copy(c, p.cosT, sizeof(double) * p.L);
+        lds_copy(s, p.sinT, sizeof(double) * p.L);
+        cosT = c;
+        sinT = s;
+        lds_copy(lb + lay.rad, p.radc, sizeof(double) * PZ_NRADC * N);
+    }
     const RadView R(RADLDS ? (const double *)(lb + lay.rad) : p.radc, N);
 
     // ---------------- global inputs -------------------------------------

@@ -1606,8 +1606,17 @@ __device__ __forceinline__ void lds_copy(void *dst, const void *src, size_t byte
 // groups, each thread keeps 16 column loads in flight, and the partial sums
 // are combined in a fixed order (bitwise reproducible).  x and y may alias;
 // every thread of the block must call it.  red: LDS scratch of
-// 2 * blockDim.x * NMAT complex values.
+// matvec_scratch(nrow, ncol, blockDim.x, RPT, NMAT) complex values.
 // ---------------------------------------------------------------------------
+// complex values of scratch for nmat matrices of nrow x ncol with `threads`
+// threads, rpt rows per thread: (CG - 1) partial sums of R row units each
+// (at most 2 threads nmat)
+__host__ __device__ inline size_t matvec_scratch(int nrow, int ncol, int threads, int rpt, int nmat)
+{
+    const int R = (nrow + rpt - 1) / rpt;
+    const int CG = threads / R < ncol ? threads / R : ncol;
+    return CG > 1 ? (size_t)nmat * (size_t)(CG - 1) * (size_t)R * (size_t)rpt : 0;
+}
 template <class E, int NMAT>
 __device__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y,
                                 cd *red)
@@ -1617,7 +1626,7 @@ __device__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv
     const int R = (nrow + RPT - 1) / RPT;
     const int CG = min((int)blockDim.x / R, N);
     const int t = threadIdx.x, cg = t / R, ru = t - cg * R;
-    const size_t RS = 2 * (size_t)blockDim.x; // scratch per matrix
+    const size_t RS = matvec_scratch(nrow, ncol, blockDim.x, RPT, 1); // scratch per matrix
     cd acc[NMAT][RPT];
 #pragma unroll
     for (int a = 0; a < NMAT; a++)

@@ -93,8 +93,9 @@ __host__ __device__ inline int mloop_threads(int N) { return N <= 65 ? 256 : PZ_
 // reduction scratch of the dense solves (16-byte aligned blocks).
 struct MpLayout {
     size_t vec, dtmp, cosT, sinT, rad, red, total;
-    // rad_lds = false: the radial rows stay in global memory (read through
-    // L2), which frees PZ_NRADC n_r doubles of LDS per workgroup
+    // rad_lds = false: the radial rows and the transform tables stay in
+    // global memory (read through L2), which frees PZ_NRADC n_r + 2 L
+    // doubles of LDS per workgroup (collocation m-loop only)
     __host__ __device__ MpLayout(int N, int threads, bool rad_lds = true)
     {
         size_t o = 0;
@@ -102,12 +103,18 @@ struct MpLayout {
         const int L = 2 * N - 2;
         vec = take(sizeof(cd) * MPHASE_NVEC * N);
         dtmp = take(sizeof(double) * 2 * N);
-        cosT = take(sizeof(double) * L);
-        sinT = take(sizeof(double) * L);
+        cosT = rad_lds ? take(sizeof(double) * L) : 0;
+        sinT = rad_lds ? take(sizeof(double) * L) : 0;
         rad = rad_lds ? take(sizeof(double) * PZ_NRADC * N) : 0;
         // FFT scratch of the Chebyshev transforms (6 L + 12), shared with the
-        // reduction scratch of the dense solves (2 matrices x 2 rows x threads)
-        red = take(sizeof(cd) * (6 * L + 16 > 4 * threads ? 6 * L + 16 : 4 * threads));
+        // reduction scratch of the dense solves: T and xi together (real),
+        // psi (complex, n_r rows on the integration method, 2 n_r on the
+        // collocation method)
+        size_t mv = matvec_scratch(N, N, threads, 2, 2);
+        const size_t m1 = matvec_scratch(N, N, threads, 1, 1), m2 = matvec_scratch(2 * N, N, threads, 1, 1);
+        if (m1 > mv) mv = m1;
+        if (m2 > mv) mv = m2;
+        red = take(sizeof(cd) * ((size_t)(6 * L + 16) > mv ? (size_t)(6 * L + 16) : mv));
         total = o;
     }
 };
