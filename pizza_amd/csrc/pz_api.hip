@@ -431,6 +431,7 @@ struct pz_handle {
     bool started = false;
     bool use_graph = true;
     size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0;
+    size_t lds_gj = 0; // k_build_coll: blocked Gauss-Jordan panels
     bool radial_big = false; // k_radial_loop_big for long azimuthal grids
     double dtcond_top = 0.0, dtcond_bot = 0.0; // dT_cond/dr at the walls (Nusselt numbers)
     char *arena = nullptr; // step state block (arena_layout)
@@ -787,7 +788,8 @@ static void allow_full_lds(int lim)
                              (const void *)k_fd_mloop, (const void *)k_fd_start, (const void *)k_radial_loop_big,
                              (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
                              (const void *)k_start_coll_adj, (const void *)k_assemble_coll,
-                             (const void *)k_assemble_fd};
+                             (const void *)k_assemble_fd, (const void *)k_build_coll,
+                             (const void *)k_build_coll_batch};
     for (const void *k : kernels) {
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, k));
@@ -1093,6 +1095,7 @@ static void setup(pz_handle *h)
     h->mthreads = mloop_threads(N);
     h->lds_mphase = MpLayout(N, h->mthreads).total;
     h->lds_fd = FdLayout(N).total;
+    h->lds_gj = gj_lds_bytes<cd>(2 * N);
     h->lds_start = 15 * (size_t)N * sizeof(cd);
     h->lds_start_coll = (13 * (size_t)N + 6 * (size_t)(2 * N - 2) + 3 * (size_t)N + 16) * sizeof(cd) +
                         3 * (size_t)N * sizeof(double);
@@ -1117,7 +1120,7 @@ static void setup(pz_handle *h)
     // the kernels of this run's radial method
     size_t need = h->lds_radial;
     if (d.fd) need = std::max(need, h->lds_fd);
-    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll});
+    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + 16384}); // + static LDS
     else need = std::max({need, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
     if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
     allow_full_lds((int)lim);
@@ -1436,7 +1439,8 @@ static void launch_step(pz_handle *h)
             if (h->d.fd) {
                 hipLaunchKernelGGL(k_fd_build, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 0);
             } else if (h->d.coll) {
-                hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), 0, h->stream, (const PzDev *)h->ddev, 0);
+                hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), h->lds_gj, h->stream, (const PzDev *)h->ddev,
+                                   0);
             } else {
                 hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, (const PzDev *)h->ddev, 0);
                 hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream,
@@ -2081,7 +2085,7 @@ struct EnsClass {
     const PzDev **druns = nullptr;
     int2 *map_r = nullptr, *map_c = nullptr, *map_3 = nullptr, *map_m = nullptr;
     int nb_r = 0, nb_c = 0, nb_3 = 0, nb_m = 0;
-    size_t lds_r = 0, lds_m = 0, lds_b = 0, lds_i = 0;
+    size_t lds_r = 0, lds_m = 0, lds_b = 0, lds_i = 0, lds_gj = 0;
     int inv_threads = 64, mthreads = PZ_MPHASE_THREADS;
     int rad_lds = 1; // collocation m-loop: radial rows staged in LDS
     int n = 0;       // n_r of the class
@@ -2132,7 +2136,7 @@ static void launch_class_step(const pz_ensemble *e, const EnsClass &c)
         if (stage == 1) {
             hipLaunchKernelGGL(k_courant_batch, dim3(c.nb_c), dim3(256), 0, c.stream, PzBatch{runs, c.map_c, (int)c.members.size()});
             if (e->coll) {
-                hipLaunchKernelGGL(k_build_coll_batch, dim3(std::min(c.nb_3, 512)), dim3(1024), 0, c.stream,
+                hipLaunchKernelGGL(k_build_coll_batch, dim3(std::min(c.nb_3, 512)), dim3(1024), c.lds_gj, c.stream,
                                    PzBatch{runs, c.map_3, (int)c.members.size()}, c.nb_3, 0);
             } else {
                 hipLaunchKernelGGL(k_build_lhs_batch, dim3(std::min(c.nb_3, 2048)), dim3(64), c.lds_b, c.stream,
@@ -2197,6 +2201,7 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
             c.lds_i = InvLayout(n, nw).total;
             c.mthreads = mloop_threads(n);
             c.n = n;
+            c.lds_gj = gj_lds_bytes<cd>(2 * n);
             c.inv_threads = 64 * nw;
             c.druns = e->upload(dr);
             c.map_r = e->upload(mr);
@@ -2292,7 +2297,7 @@ static void ens_autotune(pz_ensemble *e)
     HIPCHK(hipFree(snap));
     // matrices for the restored states
     for (EnsClass &c : e->cls)
-        hipLaunchKernelGGL(k_build_coll_batch, dim3(std::min(c.nb_3, 512)), dim3(1024), 0, c.stream,
+        hipLaunchKernelGGL(k_build_coll_batch, dim3(std::min(c.nb_3, 512)), dim3(1024), c.lds_gj, c.stream,
                            PzBatch{c.druns, c.map_3, (int)c.members.size()}, c.nb_3, 1);
     for (EnsClass &c : e->cls) HIPCHK(hipStreamSynchronize(c.stream));
     HIPCHK(hipGetLastError());

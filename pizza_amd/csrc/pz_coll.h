@@ -80,6 +80,147 @@ __device__ int gj_invert(M *W, int n, M *f)
     return s_info;
 }
 
+#define GJ_B 8   // pivot columns per panel
+#define GJ_CH 64 // columns per chunk of the panel update
+// dynamic LDS bytes of gj_invert_blk for n x n matrices of element M
+template <class M>
+__host__ __device__ inline size_t gj_lds_bytes(int n)
+{
+    return sizeof(M) * (2 * (size_t)n * GJ_B + (size_t)GJ_B * GJ_CH + 2 * GJ_B);
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Gauss-Jordan inversion with partial pivoting of the left half of
+// the row-major n x 2n block W = [A | I] by one workgroup, forming only the
+// first nr columns of A^{-1} in W's right half.  Each panel of GJ_B pivot
+// columns is eliminated in LDS with the pivots and multipliers of the
+// unblocked elimination (gj_invert).  The panel's row swaps, then its
+// composed elimination T = I + U e_K^T, go to the rest of W in one pass:
+// U = C (I - Lc)^{-1}, where column q of C holds the multipliers of pivot
+// q (c_i = -f_i / p for i /= k, 1/p - 1 at the pivot row) and Lc is the
+// strictly lower part of C's pivot rows.  W is read and written n / GJ_B
+// times instead of n times.  lds: gj_lds_bytes<M>(n) bytes.  Returns the
+// LAPACK-style info (k + 1 of the first zero pivot, 0 if none).
+// ---------------------------------------------------------------------------
+template <class M>
+__device__ int gj_invert_blk(M *W, int n, int nr, M *lds)
+{
+    __shared__ int s_p, s_info, s_piv[GJ_B];
+    __shared__ double s_v[32];
+    __shared__ int s_i[32];
+    M *P = lds, *C = P + (size_t)n * GJ_B, *Wk = C + (size_t)n * GJ_B, *prow = Wk + GJ_B * GJ_CH,
+      *coef = prow + GJ_B;
+    const int tid = threadIdx.x, nt = blockDim.x, w2 = 2 * n;
+    if (tid == 0) s_info = 0;
+    for (int t = tid; t < n * nr; t += nt) {
+        const int i = t / nr, j = t - i * nr;
+        W[(size_t)i * w2 + n + j] = i == j ? mk_elem<M>(1.0, 0.0) : mzero<M>();
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < n; k0 += GJ_B) {
+        const int b = min(GJ_B, n - k0);
+        for (int t = tid; t < n * GJ_B; t += nt) {
+            const int i = t / GJ_B, c = t - i * GJ_B;
+            P[t] = c < b ? W[(size_t)i * w2 + k0 + c] : mzero<M>();
+            C[t] = mzero<M>();
+        }
+        __syncthreads();
+        // ---- the panel, pivot by pivot, in LDS
+        for (int kk = 0; kk < b; kk++) {
+            const int k = k0 + kk;
+            double bv = -1.0;
+            int bi = n;
+            for (int i = k + tid; i < n; i += nt) {
+                const double v = mabs1(P[(size_t)i * GJ_B + kk]);
+                if (v > bv) { bv = v; bi = i; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(bv, o, 64);
+                const int oi = __shfl_xor(bi, o, 64);
+                if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+            }
+            if ((tid & 63) == 0) { s_v[tid >> 6] = bv; s_i[tid >> 6] = bi; }
+            __syncthreads();
+            if (tid == 0) {
+                double v = s_v[0];
+                int ii = s_i[0];
+                for (int q = 1; q < (nt + 63) / 64; q++)
+                    if (s_v[q] > v || (s_v[q] == v && s_i[q] < ii)) { v = s_v[q]; ii = s_i[q]; }
+                s_p = ii;
+                s_piv[kk] = ii;
+                if (!(v > 0.0) && s_info == 0) s_info = k + 1;
+            }
+            __syncthreads();
+            if (s_info != 0) break;
+            const int p = s_p;
+            if (p != k && tid < GJ_B) {
+                const M a = P[(size_t)k * GJ_B + tid], c = C[(size_t)k * GJ_B + tid];
+                P[(size_t)k * GJ_B + tid] = P[(size_t)p * GJ_B + tid];
+                C[(size_t)k * GJ_B + tid] = C[(size_t)p * GJ_B + tid];
+                P[(size_t)p * GJ_B + tid] = a;
+                C[(size_t)p * GJ_B + tid] = c;
+            }
+            __syncthreads();
+            const M r = mrecip(P[(size_t)k * GJ_B + kk]);
+            if (tid < GJ_B) prow[tid] = P[(size_t)k * GJ_B + tid];
+            for (int i = tid; i < n; i += nt)
+                C[(size_t)i * GJ_B + kk] =
+                    i == k ? msub(r, mk_elem<M>(1.0, 0.0)) : mneg(mmm(P[(size_t)i * GJ_B + kk], r));
+            __syncthreads();
+            for (int t = tid; t < n * GJ_B; t += nt) {
+                const int i = t / GJ_B, c = t - i * GJ_B;
+                if (c >= kk && c < b) P[t] = madd(P[t], mmm(C[(size_t)i * GJ_B + kk], prow[c]));
+            }
+            __syncthreads();
+        }
+        if (s_info != 0) break;
+        // ---- the other columns J = [k0 + b, n) and [n, n + nr): swaps in order
+        const int nl = n - k0 - b, nJ = nl + nr;
+        for (int kk = 0; kk < b; kk++) {
+            const int k = k0 + kk, p = s_piv[kk];
+            if (p != k)
+                for (int jj = tid; jj < nJ; jj += nt) {
+                    const int j = jj < nl ? k0 + b + jj : n + (jj - nl);
+                    const M a = W[(size_t)k * w2 + j];
+                    W[(size_t)k * w2 + j] = W[(size_t)p * w2 + j];
+                    W[(size_t)p * w2 + j] = a;
+                }
+            __syncthreads();
+        }
+        // U = C (I - Lc)^{-1}, in place, last column first
+        for (int kk = b - 1; kk >= 0; kk--) {
+            if (tid < GJ_B) coef[tid] = (tid > kk && tid < b) ? C[(size_t)(k0 + tid) * GJ_B + kk] : mzero<M>();
+            __syncthreads();
+            for (int i = tid; i < n; i += nt) {
+                M u = C[(size_t)i * GJ_B + kk];
+                for (int q = kk + 1; q < b; q++) u = madd(u, mmm(C[(size_t)i * GJ_B + q], coef[q]));
+                C[(size_t)i * GJ_B + kk] = u;
+            }
+            __syncthreads();
+        }
+        // W[:, J] += U W[K, J], GJ_CH columns at a time
+        for (int j0 = 0; j0 < nJ; j0 += GJ_CH) {
+            const int cw = min(GJ_CH, nJ - j0);
+            for (int t = tid; t < b * cw; t += nt) {
+                const int q = t / cw, jc = t - q * cw, jj = j0 + jc;
+                const int j = jj < nl ? k0 + b + jj : n + (jj - nl);
+                Wk[q * GJ_CH + jc] = W[(size_t)(k0 + q) * w2 + j];
+            }
+            __syncthreads();
+            for (int t = tid; t < n * cw; t += nt) {
+                const int i = t / cw, jc = t - i * cw, jj = j0 + jc;
+                const int j = jj < nl ? k0 + b + jj : n + (jj - nl);
+                M v = W[(size_t)i * w2 + j];
+                for (int q = 0; q < b; q++) v = madd(v, mmm(C[(size_t)i * GJ_B + q], Wk[q * GJ_CH + jc]));
+                W[(size_t)i * w2 + j] = v;
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    return s_info;
+}
+
 #define PZ_CM(k, n) p.cM[(size_t)(k) * N + (n)]
 #define PZ_CDM(k, n) p.cdM[(size_t)(k) * N + (n)]
 #define PZ_CD2M(k, n) p.cd2M[(size_t)(k) * N + (n)]
@@ -194,11 +335,13 @@ __device__ void coll_assemble(const PzDev &p, const RadView &R, int fld, int m, 
 
 // grid (n_m, 3): field 0 = T, 1 = xi, 2 = psi (m /= 0) or u_phi (m = 0).
 // Forms the inverses, with the reference's scalings folded in, whenever the
-// matrices change (lMat) or when forced.
+// matrices change (lMat) or when forced; only the first n_r columns of the
+// psi inverse are formed, the ones the m-loop applies.  Dynamic LDS:
+// gj_lds_bytes<cd>(2 n_r).
 __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld, int force)
 {
-    __shared__ cd fcol[2 * 257];
     __shared__ double facs[4 * 257];
+    extern __shared__ double2 gj_smem[]; // gj_lds_bytes<cd>(2 n_r) bytes
     if (!force && p.ss->lMat == 0) return;
     const int N = p.N, m = p.idx2m[mi], tid = threadIdx.x;
     if ((fld == 0 && !p.l_heat) || (fld == 1 && !p.l_chem)) return;
@@ -209,7 +352,7 @@ __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld,
     if (fld < 2 || m == 0) {
         double *W = (double *)(ws + (fld == 2 ? 2 * regT : fld * regT));
         coll_assemble<double>(p, R, fld, m, fld == 0 ? p.hdifT[mi] : fld == 1 ? p.hdifXi[mi] : p.hdifV[mi], W, facs);
-        const int info = gj_invert<double>(W, N, (double *)fcol);
+        const int info = gj_invert_blk<double>(W, N, N, (double *)gj_smem);
         if (tid == 0) p.info[fld * p.nm + mi] = info;
         if (info) return;
         // INV = A_s^{-1} diag(fac) (T, xi); A^{-1} (u_phi); column-major [col][ldi]
@@ -227,7 +370,7 @@ __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld,
         cd *W = (cd *)(ws + 2 * regT);
         const int n = 2 * N;
         coll_assemble<cd>(p, R, fld, m, p.hdifV[mi], W, facs);
-        const int info = gj_invert<cd>(W, n, fcol);
+        const int info = gj_invert_blk<cd>(W, n, N, (cd *)gj_smem);
         if (tid == 0) p.info[fld * p.nm + mi] = info;
         if (info) return;
         // INV = diag(fac_col) A_s^{-1} diag(fac_row); only the first N
