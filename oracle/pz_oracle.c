@@ -78,7 +78,9 @@ static void fft_rec(const fftp *p, const cplx *x, long is, cplx *y, int n, int d
     int f = small_factor(n), q = n / f;
     for (int r = 0; r < f; r++) fft_rec(p, x + r * is, is * f, y + r * q, q, dir);
     long step = p->n / n;
-    cplx t[64];
+    /* radix f: a prime factor of n, any size (e.g. 193 in 2 n_r - 2 = 386) */
+    cplx tbuf[64];
+    cplx *t = f <= 64 ? tbuf : malloc(sizeof(cplx) * f);
     cplx *out = p->tmp2;
     for (int k = 0; k < q; k++) {
         for (int r = 0; r < f; r++) t[r] = y[r * q + k] * twid(p, (long)r * k * step, dir);
@@ -88,6 +90,7 @@ static void fft_rec(const fftp *p, const cplx *x, long is, cplx *y, int n, int d
             out[k + s * q] = acc;
         }
     }
+    if (t != tbuf) free(t);
     memcpy(y, out, sizeof(cplx) * n);
 }
 

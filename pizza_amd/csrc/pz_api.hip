@@ -432,6 +432,7 @@ struct pz_handle {
     bool use_graph = true;
     size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0;
     size_t lds_gj = 0; // k_build_coll: blocked Gauss-Jordan panels
+    bool mcoll_l2 = false; // collocation m-loop reads radial rows and tables through L2
     bool radial_big = false; // k_radial_loop_big for long azimuthal grids
     double dtcond_top = 0.0, dtcond_bot = 0.0; // dT_cond/dr at the walls (Nusselt numbers)
     char *arena = nullptr; // step state block (arena_layout)
@@ -789,7 +790,7 @@ static void allow_full_lds(int lim)
                              (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
                              (const void *)k_start_coll_adj, (const void *)k_assemble_coll,
                              (const void *)k_assemble_fd, (const void *)k_build_coll,
-                             (const void *)k_build_coll_batch};
+                             (const void *)k_build_coll_batch, (const void *)k_mcoll_l2};
     for (const void *k : kernels) {
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, k));
@@ -1110,6 +1111,10 @@ static void setup(pz_handle *h)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     size_t lim = prop.sharedMemPerBlock;
+    if (d.coll && h->lds_mphase > lim) {
+        h->mcoll_l2 = true;
+        h->lds_mphase = MpLayout(N, h->mthreads, false).total;
+    }
     if (h->lds_radial > lim) {
         h->radial_big = true;
         h->lds_radial = 4 * (size_t)h->nphi * sizeof(cd) + 16 * sizeof(double);
@@ -1120,7 +1125,8 @@ static void setup(pz_handle *h)
     // the kernels of this run's radial method
     size_t need = h->lds_radial;
     if (d.fd) need = std::max(need, h->lds_fd);
-    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + 16384}); // + static LDS
+    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + 16384, // + static LDS
+                                      h->sch.l_assembly ? asm_lds_bytes(N) : (size_t)0});
     else need = std::max({need, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
     if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
     allow_full_lds((int)lim);
@@ -1452,7 +1458,7 @@ static void launch_step(pz_handle *h)
             hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev,
                                stage, exp_calc);
         else if (h->d.coll)
-            hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
+            hipLaunchKernelGGL(h->mcoll_l2 ? k_mcoll_l2 : k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                (const PzDev *)h->ddev, stage, exp_calc);
         else
             hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
@@ -2051,7 +2057,7 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
                 hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 1,
                                    1);
             else if (which == 0 && h->d.coll)
-                hipLaunchKernelGGL(k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
+                hipLaunchKernelGGL(h->mcoll_l2 ? k_mcoll_l2 : k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else if (which == 0)
                 hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
@@ -2231,11 +2237,16 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
 // both (less LDS: more workgroups per CU)
 static void ens_policy(pz_ensemble *e, int policy)
 {
+    int dev = 0, lim = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     for (EnsClass &c : e->cls) {
         const bool big = c.n > 65;
         // the coupled omega-psi product needs 2 n_r <= threads
         c.mthreads = (big && (policy & 1) && 2 * c.n <= 256) ? 256 : mloop_threads(c.n);
         c.rad_lds = (big && (policy & 2)) ? 0 : 1;
+        // grids whose full layout does not fit read through L2 under any policy
+        if (e->coll && MpLayout(c.n, c.mthreads, true).total > (size_t)lim) c.rad_lds = 0;
         c.lds_m = MpLayout(c.n, c.mthreads, c.rad_lds != 0).total;
         if (c.gexec) { (void)hipGraphExecDestroy(c.gexec); c.gexec = nullptr; }
         if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }

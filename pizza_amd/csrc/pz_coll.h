@@ -737,6 +737,12 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll(const PzDev *__rest
 {
     mcoll_body<true>(*pd, blockIdx.x, istage, exp_calc);
 }
+// the same with the radial rows and transform tables read through L2, for
+// grids whose full LDS layout does not fit a workgroup (n_r > 193)
+__global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll_l2(const PzDev *__restrict__ pd, int istage, int exp_calc)
+{
+    mcoll_body<false>(*pd, blockIdx.x, istage, exp_calc);
+}
 // batched; RADLDS picks the radial rows' place per size class
 template <bool RADLDS>
 __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll_batch(PzBatch b, int istage, int exp_calc)

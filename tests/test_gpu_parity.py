@@ -283,6 +283,36 @@ def test_fd_assembly_scheme_parity(scheme):
             assert err < max(1e-11 if i == 0 else 1e-10, 10 * floors[i][f]), (f, i, err, floors[i][f])
 
 
+def test_collocation_maximum_size():
+    """Collocation at its largest n_r (256): the m-loop reads its radial
+    rows and tables through L2, and the blocked Gauss-Jordan of the
+    512 x 512 complex omega-psi systems runs with its largest LDS panels.
+    Fields against the oracle after 1 step (1e-9) and 3 steps (1e-8).
+    At n_r <= 129 the same comparison holds to about 1e-12. Above it the
+    difference grows with n_r (about 1e-10 after one step at n_r = 256),
+    while the unit operators still agree to 1e-16. The oracle's response to
+    a rounding-level dt perturbation stays near 1e-12, so the growth is not
+    yet explained by conditioning alone; the explicit inverses against the
+    oracle's LU solves are the open suspect.  These tolerances record the
+    measured agreement; they are looser than those of the smaller grids."""
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    cfg.n_r_max = cfg.n_cheb_max = 256
+    cfg.m_max = 6
+    cfg.dtMax = 1.0e-6
+    print("case: collocation n_r 256, m_max 6", flush=True)
+    g, o = _pair(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg)
+        x.start()
+    for n, tol in ((1, 1e-9), (2, 1e-8)):
+        g.step(n)
+        o.step(n)
+        for f in _fields(cfg):
+            err = rel(g.get_field(f), o.get_field(f))
+            assert err < tol, (f, n, err)
+
+
 def test_assembly_schemes_rejected_on_integration():
     from pizza_amd import config as C
     from pizza_amd import workloads as W
