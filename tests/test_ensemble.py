@@ -1,7 +1,8 @@
 """Ensembles (pz_ensemble_*): uneven runs advanced together with one batched
 launch per kernel and stage.  Each member must equal the same run advanced
 alone through pz_step — the batched kernels run the very same per-block
-code — so the comparison is bitwise."""
+code — so the comparison is bitwise; and the members match the oracle
+after 500 steps to 1e-8."""
 import numpy as np
 import pytest
 
@@ -91,3 +92,34 @@ def test_ensemble_rejects_mixed_methods():
         r.start()
     with pytest.raises(PizzaError):
         Ensemble([a, b])
+
+
+def test_ensemble_members_match_oracle_over_500_steps():
+    """Each member of an uneven collocation ensemble against the oracle after
+    500 steps, 1e-8 relative per field (one member changes dt under the
+    Courant rule, so its matrices are rebuilt inside the batched step)."""
+    from helpers import rel
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Ensemble
+    cfgs = []
+    for n_r, m_max, ra in ((33, 32, 1e5), (33, 16, 3e5), (49, 21, 1e5)):
+        c = W.c1_collocation()
+        c.n_r_max = c.n_cheb_max = n_r
+        c.m_max = m_max
+        c.ra = ra
+        cfgs.append(c)
+    print("case: ensemble of 3 runs against the oracle, 500 steps", flush=True)
+    runs = _members(cfgs, W.seed_member)
+    ens = Ensemble(runs)
+    ens.step(500)
+    ens.close()
+    for k, (c, r) in enumerate(zip(cfgs, runs)):
+        o = Oracle(c)
+        W.seed_member(o, c, k)
+        o.start()
+        o.step(500)
+        assert abs(r.time()[0] - o.time()[0]) < 1e-12 and r.time()[1] == o.time()[1], k
+        for f in FIELDS:
+            err = rel(r.get_field(f), o.get_field(f))
+            assert err < 1e-8, (k, f, err)
