@@ -199,6 +199,25 @@ def adjoint_bench(steps=400, ckpt=20, reps=2):
             "grad_max_abs": float(np.abs(g).max())}
 
 
+def fd_bench(steps=50, warmup=5):
+    """The finite-difference workload: n_r 257 on the stretched grid,
+    n_phi 2046, E 1e-6, Ra 1e9, Pr 0.1, SBDF3, dt 1e-7, seeded T
+    perturbation 1e-3; steps/s of pz_step over `steps` steps."""
+    cfg = W.c5_fd()
+    run = Run(cfg)
+    W.seed_run(run, cfg, amp_t=1e-3)
+    run.start()
+    run.step(warmup)
+    run.time()
+    t0 = time.perf_counter()
+    run.step(steps)
+    run.time()
+    elapsed = time.perf_counter() - t0
+    run.close()
+    return {"workload": "FD-SBDF3 n_r=257 n_phi=2046 E=1e-6 Ra=1e9 Pr=0.1 dt=1e-7", "steps": steps,
+            "steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +230,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost")
     ap.add_argument("--adjoint-only", action="store_true", help="measure only the adjoint cost")
+    ap.add_argument("--fd-only", action="store_true",
+                    help="measure only the finite-difference workload (n_r 257, n_phi 2046, SBDF3)")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="run only the short sample profiled by the counter runs")
     args = ap.parse_args()
@@ -227,6 +248,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench()}))
+        return
+    if args.fd_only:
+        print(json.dumps({"fd": fd_bench()}))
         return
 
     cfg = W.c2_production()
