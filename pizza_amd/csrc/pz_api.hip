@@ -1096,7 +1096,6 @@ static void setup(pz_handle *h)
     h->mthreads = mloop_threads(N);
     h->lds_mphase = MpLayout(N, h->mthreads).total;
     h->lds_fd = FdLayout(N).total;
-    h->lds_gj = gj_lds_bytes<cd>(2 * N);
     h->lds_start = 15 * (size_t)N * sizeof(cd);
     h->lds_start_coll = (13 * (size_t)N + 6 * (size_t)(2 * N - 2) + 3 * (size_t)N + 16) * sizeof(cd) +
                         3 * (size_t)N * sizeof(double);
@@ -1111,6 +1110,9 @@ static void setup(pz_handle *h)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     size_t lim = prop.sharedMemPerBlock;
+    // rebuild panels of 16 pivots where they fit next to the static LDS, else 8
+    d.gj_b = gj_lds_bytes<cd>(2 * N, 16) + GJ_STATIC_LDS <= lim ? 16 : 8;
+    h->lds_gj = gj_lds_bytes<cd>(2 * N, d.gj_b);
     if (d.coll && h->lds_mphase > lim) {
         h->mcoll_l2 = true;
         h->lds_mphase = MpLayout(N, h->mthreads, false).total;
@@ -1125,7 +1127,7 @@ static void setup(pz_handle *h)
     // the kernels of this run's radial method
     size_t need = h->lds_radial;
     if (d.fd) need = std::max(need, h->lds_fd);
-    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + 16384, // + static LDS
+    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + GJ_STATIC_LDS,
                                       h->sch.l_assembly ? asm_lds_bytes(N) : (size_t)0});
     else need = std::max({need, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
     if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
@@ -2202,12 +2204,12 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
                 c.lds_r = std::max(c.lds_r, h->lds_radial);
                 c.lds_m = std::max(c.lds_m, h->lds_mphase);
                 c.lds_b = std::max(c.lds_b, h->lds_build);
+                c.lds_gj = std::max(c.lds_gj, h->lds_gj);
                 nw = std::min(nw, h->inv_waves);
             }
             c.lds_i = InvLayout(n, nw).total;
             c.mthreads = mloop_threads(n);
             c.n = n;
-            c.lds_gj = gj_lds_bytes<cd>(2 * n);
             c.inv_threads = 64 * nw;
             c.druns = e->upload(dr);
             c.map_r = e->upload(mr);

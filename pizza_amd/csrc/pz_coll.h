@@ -80,36 +80,40 @@ __device__ int gj_invert(M *W, int n, M *f)
     return s_info;
 }
 
-#define GJ_B 8   // pivot columns per panel
-#define GJ_CH 64 // columns per chunk of the panel update
-// dynamic LDS bytes of gj_invert_blk for n x n matrices of element M
+#define GJ_BMAX 16 // widest panel (pivot columns per panel)
+#define GJ_CH 64   // columns per chunk of the panel update
+// static LDS of k_build_coll / k_build_coll_batch is below this (about 9.1 KB)
+#define GJ_STATIC_LDS 10240
+// dynamic LDS bytes of gj_invert_blk for n x n matrices of element M and
+// panels of B pivot columns
 template <class M>
-__host__ __device__ inline size_t gj_lds_bytes(int n)
+__host__ __device__ inline size_t gj_lds_bytes(int n, int B)
 {
-    return sizeof(M) * (2 * (size_t)n * GJ_B + (size_t)GJ_B * GJ_CH + 2 * GJ_B);
+    return sizeof(M) * (2 * (size_t)n * B + (size_t)B * GJ_CH + 2 * (size_t)B);
 }
 
 // ---------------------------------------------------------------------------
 // Blocked Gauss-Jordan inversion with partial pivoting of the left half of
 // the row-major n x 2n block W = [A | I] by one workgroup, forming only the
-// first nr columns of A^{-1} in W's right half.  Each panel of GJ_B pivot
+// first nr columns of A^{-1} in W's right half.  Each panel of B pivot
 // columns is eliminated in LDS with the pivots and multipliers of the
 // unblocked elimination (gj_invert).  The panel's row swaps, then its
 // composed elimination T = I + U e_K^T, go to the rest of W in one pass:
 // U = C (I - Lc)^{-1}, where column q of C holds the multipliers of pivot
 // q (c_i = -f_i / p for i /= k, 1/p - 1 at the pivot row) and Lc is the
-// strictly lower part of C's pivot rows.  W is read and written n / GJ_B
-// times instead of n times.  lds: gj_lds_bytes<M>(n) bytes.  Returns the
-// LAPACK-style info (k + 1 of the first zero pivot, 0 if none).
+// strictly lower part of C's pivot rows.  W is read and written n / B
+// times instead of n times; B <= GJ_BMAX.  lds: gj_lds_bytes<M>(n, B)
+// bytes.  Returns the LAPACK-style info (k + 1 of the first zero pivot, 0
+// if none).
 // ---------------------------------------------------------------------------
 template <class M>
-__device__ int gj_invert_blk(M *W, int n, int nr, M *lds)
+__device__ int gj_invert_blk(M *W, int n, int nr, M *lds, int B)
 {
-    __shared__ int s_p, s_info, s_piv[GJ_B];
+    __shared__ int s_p, s_info, s_piv[GJ_BMAX];
     __shared__ double s_v[32];
     __shared__ int s_i[32];
-    M *P = lds, *C = P + (size_t)n * GJ_B, *Wk = C + (size_t)n * GJ_B, *prow = Wk + GJ_B * GJ_CH,
-      *coef = prow + GJ_B;
+    M *P = lds, *C = P + (size_t)n * B, *Wk = C + (size_t)n * B, *prow = Wk + B * GJ_CH,
+      *coef = prow + B;
     const int tid = threadIdx.x, nt = blockDim.x, w2 = 2 * n;
     if (tid == 0) s_info = 0;
     for (int t = tid; t < n * nr; t += nt) {
@@ -117,10 +121,10 @@ __device__ int gj_invert_blk(M *W, int n, int nr, M *lds)
         W[(size_t)i * w2 + n + j] = i == j ? mk_elem<M>(1.0, 0.0) : mzero<M>();
     }
     __syncthreads();
-    for (int k0 = 0; k0 < n; k0 += GJ_B) {
-        const int b = min(GJ_B, n - k0);
-        for (int t = tid; t < n * GJ_B; t += nt) {
-            const int i = t / GJ_B, c = t - i * GJ_B;
+    for (int k0 = 0; k0 < n; k0 += B) {
+        const int b = min(B, n - k0);
+        for (int t = tid; t < n * B; t += nt) {
+            const int i = t / B, c = t - i * B;
             P[t] = c < b ? W[(size_t)i * w2 + k0 + c] : mzero<M>();
             C[t] = mzero<M>();
         }
@@ -131,7 +135,7 @@ __device__ int gj_invert_blk(M *W, int n, int nr, M *lds)
             double bv = -1.0;
             int bi = n;
             for (int i = k + tid; i < n; i += nt) {
-                const double v = mabs1(P[(size_t)i * GJ_B + kk]);
+                const double v = mabs1(P[(size_t)i * B + kk]);
                 if (v > bv) { bv = v; bi = i; }
             }
             for (int o = 32; o > 0; o >>= 1) {
@@ -153,23 +157,23 @@ __device__ int gj_invert_blk(M *W, int n, int nr, M *lds)
             __syncthreads();
             if (s_info != 0) break;
             const int p = s_p;
-            if (p != k && tid < GJ_B) {
-                const M a = P[(size_t)k * GJ_B + tid], c = C[(size_t)k * GJ_B + tid];
-                P[(size_t)k * GJ_B + tid] = P[(size_t)p * GJ_B + tid];
-                C[(size_t)k * GJ_B + tid] = C[(size_t)p * GJ_B + tid];
-                P[(size_t)p * GJ_B + tid] = a;
-                C[(size_t)p * GJ_B + tid] = c;
+            if (p != k && tid < B) {
+                const M a = P[(size_t)k * B + tid], c = C[(size_t)k * B + tid];
+                P[(size_t)k * B + tid] = P[(size_t)p * B + tid];
+                C[(size_t)k * B + tid] = C[(size_t)p * B + tid];
+                P[(size_t)p * B + tid] = a;
+                C[(size_t)p * B + tid] = c;
             }
             __syncthreads();
-            const M r = mrecip(P[(size_t)k * GJ_B + kk]);
-            if (tid < GJ_B) prow[tid] = P[(size_t)k * GJ_B + tid];
+            const M r = mrecip(P[(size_t)k * B + kk]);
+            if (tid < B) prow[tid] = P[(size_t)k * B + tid];
             for (int i = tid; i < n; i += nt)
-                C[(size_t)i * GJ_B + kk] =
-                    i == k ? msub(r, mk_elem<M>(1.0, 0.0)) : mneg(mmm(P[(size_t)i * GJ_B + kk], r));
+                C[(size_t)i * B + kk] =
+                    i == k ? msub(r, mk_elem<M>(1.0, 0.0)) : mneg(mmm(P[(size_t)i * B + kk], r));
             __syncthreads();
-            for (int t = tid; t < n * GJ_B; t += nt) {
-                const int i = t / GJ_B, c = t - i * GJ_B;
-                if (c >= kk && c < b) P[t] = madd(P[t], mmm(C[(size_t)i * GJ_B + kk], prow[c]));
+            for (int t = tid; t < n * B; t += nt) {
+                const int i = t / B, c = t - i * B;
+                if (c >= kk && c < b) P[t] = madd(P[t], mmm(C[(size_t)i * B + kk], prow[c]));
             }
             __syncthreads();
         }
@@ -189,12 +193,12 @@ __device__ int gj_invert_blk(M *W, int n, int nr, M *lds)
         }
         // U = C (I - Lc)^{-1}, in place, last column first
         for (int kk = b - 1; kk >= 0; kk--) {
-            if (tid < GJ_B) coef[tid] = (tid > kk && tid < b) ? C[(size_t)(k0 + tid) * GJ_B + kk] : mzero<M>();
+            if (tid < B) coef[tid] = (tid > kk && tid < b) ? C[(size_t)(k0 + tid) * B + kk] : mzero<M>();
             __syncthreads();
             for (int i = tid; i < n; i += nt) {
-                M u = C[(size_t)i * GJ_B + kk];
-                for (int q = kk + 1; q < b; q++) u = madd(u, mmm(C[(size_t)i * GJ_B + q], coef[q]));
-                C[(size_t)i * GJ_B + kk] = u;
+                M u = C[(size_t)i * B + kk];
+                for (int q = kk + 1; q < b; q++) u = madd(u, mmm(C[(size_t)i * B + q], coef[q]));
+                C[(size_t)i * B + kk] = u;
             }
             __syncthreads();
         }
@@ -211,7 +215,7 @@ __device__ int gj_invert_blk(M *W, int n, int nr, M *lds)
                 const int i = t / cw, jc = t - i * cw, jj = j0 + jc;
                 const int j = jj < nl ? k0 + b + jj : n + (jj - nl);
                 M v = W[(size_t)i * w2 + j];
-                for (int q = 0; q < b; q++) v = madd(v, mmm(C[(size_t)i * GJ_B + q], Wk[q * GJ_CH + jc]));
+                for (int q = 0; q < b; q++) v = madd(v, mmm(C[(size_t)i * B + q], Wk[q * GJ_CH + jc]));
                 W[(size_t)i * w2 + j] = v;
             }
             __syncthreads();
@@ -337,11 +341,11 @@ __device__ void coll_assemble(const PzDev &p, const RadView &R, int fld, int m, 
 // Forms the inverses, with the reference's scalings folded in, whenever the
 // matrices change (lMat) or when forced; only the first n_r columns of the
 // psi inverse are formed, the ones the m-loop applies.  Dynamic LDS:
-// gj_lds_bytes<cd>(2 n_r).
+// gj_lds_bytes<cd>(2 n_r, p.gj_b).
 __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld, int force)
 {
     __shared__ double facs[4 * 257];
-    extern __shared__ double2 gj_smem[]; // gj_lds_bytes<cd>(2 n_r) bytes
+    extern __shared__ double2 gj_smem[]; // gj_lds_bytes<cd>(2 n_r, p.gj_b) bytes
     if (!force && p.ss->lMat == 0) return;
     const int N = p.N, m = p.idx2m[mi], tid = threadIdx.x;
     if ((fld == 0 && !p.l_heat) || (fld == 1 && !p.l_chem)) return;
@@ -352,7 +356,7 @@ __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld,
     if (fld < 2 || m == 0) {
         double *W = (double *)(ws + (fld == 2 ? 2 * regT : fld * regT));
         coll_assemble<double>(p, R, fld, m, fld == 0 ? p.hdifT[mi] : fld == 1 ? p.hdifXi[mi] : p.hdifV[mi], W, facs);
-        const int info = gj_invert_blk<double>(W, N, N, (double *)gj_smem);
+        const int info = gj_invert_blk<double>(W, N, N, (double *)gj_smem, p.gj_b);
         if (tid == 0) p.info[fld * p.nm + mi] = info;
         if (info) return;
         // INV = A_s^{-1} diag(fac) (T, xi); A^{-1} (u_phi); column-major [col][ldi]
@@ -370,7 +374,7 @@ __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld,
         cd *W = (cd *)(ws + 2 * regT);
         const int n = 2 * N;
         coll_assemble<cd>(p, R, fld, m, p.hdifV[mi], W, facs);
-        const int info = gj_invert_blk<cd>(W, n, N, (cd *)gj_smem);
+        const int info = gj_invert_blk<cd>(W, n, N, (cd *)gj_smem, p.gj_b);
         if (tid == 0) p.info[fld * p.nm + mi] = info;
         if (info) return;
         // INV = diag(fac_col) A_s^{-1} diag(fac_row); only the first N

@@ -62,6 +62,7 @@ struct PzDev {
     // collocation boundary-derivative rows (robin_bc) and the inverses of
     // the psi assembly matrices [m][col][ldi]
     int l_assembly;
+    int gj_b; // pivot columns per panel of the collocation rebuilds (8 or 16)
     double bass_imp0[PZ_MAX_STAGES], bass_exp0[PZ_MAX_STAGES];
     const double *dr_top, *dr_bot;
     double *A_inv;
