@@ -196,10 +196,16 @@ int pz_fft(pz_handle *h, const double *grid, double *spec, int n_rad);
 int pz_ifft(pz_handle *h, const double *spec, double *grid, int n_rad);
 
 /*
- * Kernel timing for the roofline report: launches the m-loop update of the
- * psi equation (stage 1 weights, no state change outside scratch) reps
- * times between HIP events on the handle's stream; returns the mean
- * duration in ms.  No reference counterpart (measurement only).
+ * Kernel timing for the roofline report: launches one stage kernel reps
+ * times between HIP events on the handle's stream and returns the mean
+ * duration in ms.  which = 0: the m-loop kernel of the run's radial method
+ * for stage 1 with its explicit part (it advances the fields and the
+ * time-scheme slots); which = 1: the radial loop.  The step state (fields,
+ * time-scheme slots and scheme state) is copied before the launches and
+ * restored after them, so the run continues from where it was; the
+ * per-stage advection terms that the next radial loop rewrites are left as
+ * the launches wrote them.  Allocates a copy of the step state.  No
+ * reference counterpart (measurement only).
  */
 int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms);
 

@@ -40,6 +40,7 @@ def lib():
         L.or_ass_solve.argtypes = [vp, ip, dp, dp]
         L.or_fd_ell_solve.argtypes = [vp, ip, dp, dp]
         L.or_radial.argtypes = [vp, dp]
+        L.or_cond.argtypes = [vp, dp]
         L.or_dirk_tables.argtypes = [ip, ctypes.POINTER(ip), ctypes.POINTER(ip), dp, dp, dp, dp, dp]
         L.or_costf.argtypes = [vp, dp, ip]
         L.or_get_dr.argtypes = [vp, dp, dp, ip, ip]
@@ -155,6 +156,12 @@ class Oracle:
                     "or_fd_ell_solve")
         return out
 
+    def cond(self):
+        """tcond, dtcond, xicond, dxicond on the grid."""
+        out = np.zeros(4 * self.n_r)
+        lib().or_cond(self.h, _dp(out))
+        return out.reshape(4, self.n_r)
+
     def radial(self):
         """r, 1/r, 1/r^2, beta, dbeta/dr on the grid."""
         out = np.zeros(5 * self.n_r)
@@ -224,6 +231,22 @@ class Oracle:
         rc = lib().or_stencil(self.h, which, m, idx, ku, _dp(out))
         self._check(rc, "stencil")
         return out
+
+class planted:
+    """Context manager that plants mistake k in the oracle (the or_plant
+    switch of pz_oracle.c) for the runs created inside it."""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        ctypes.c_int.in_dll(lib(), "or_plant").value = self.k
+        return self
+
+    def __exit__(self, *exc):
+        ctypes.c_int.in_dll(lib(), "or_plant").value = 0
+        return False
+
 
 def dirk_tables(scheme):
     """(nstages, l_assembly, wimp_lin0, A_imp, A_exp, b_imp, b_exp) unscaled."""

@@ -20,6 +20,43 @@ typedef long double ld;
 
 #define PI_F (4.0 * atan(1.0)) /* constants.f90:5 */
 
+/* Scratch buffers owned by the calling thread: the per-radius and per-m
+ * loops of the step run on OpenMP threads, and every transform below takes
+ * its work arrays from here instead of from the handle.  Buffer k grows on
+ * demand; a thread uses one handle at a time. */
+static cplx *tl_cbuf(int k, size_t n)
+{
+    static _Thread_local cplx *buf[6];
+    static _Thread_local size_t cap[6];
+    if (cap[k] < n) {
+        free(buf[k]);
+        buf[k] = malloc(n * sizeof(cplx));
+        cap[k] = n;
+    }
+    return buf[k];
+}
+/* Planted mistakes for the tests that show a check can fail (0 = none):
+ * 1 flips the sign of the integration method's explicit Ekman-pumping term
+ *   (update_psi_integ_smat.f90:895-913),
+ * 2 drops its factor r for m /= 0,
+ * 3 drops its u_s part (ekp_dusdp, ekp_us),
+ * 4 flips the sign of the compositional buoyancy (ChemFac),
+ * 5 doubles the compositional diffusivity (XidiffFac).
+ * Read when a run is created (4, 5) or in the explicit finish (1-3). */
+int or_plant = 0;
+
+static double *tl_dbuf(size_t n)
+{
+    static _Thread_local double *buf;
+    static _Thread_local size_t cap;
+    if (cap < n) {
+        free(buf);
+        buf = malloc(n * sizeof(double));
+        cap = n;
+    }
+    return buf;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Mixed-radix complex FFT (unnormalised; dir=-1 forward, +1 backward).       */
 /* Stands in for the FFTW plans of fft_fftw.f90:21-35 and dct_fftw.f90:77-84. */
@@ -27,8 +64,6 @@ typedef long double ld;
 typedef struct {
     int n;
     cplx *w;    /* w[k] = exp(-2 pi i k / n) */
-    cplx *tmp;  /* scratch n */
-    cplx *tmp2; /* scratch n */
 } fftp;
 
 static fftp *fft_new(int n)
@@ -36,8 +71,6 @@ static fftp *fft_new(int n)
     fftp *p = calloc(1, sizeof(fftp));
     p->n = n;
     p->w = malloc(sizeof(cplx) * n);
-    p->tmp = malloc(sizeof(cplx) * n);
-    p->tmp2 = malloc(sizeof(cplx) * n);
     for (int k = 0; k < n; k++) {
         ld ang = 2.0L * 3.14159265358979323846264338327950288L * (ld)k / (ld)n;
         p->w[k] = (double)cosl(ang) - I * (double)sinl(ang);
@@ -49,8 +82,6 @@ static void fft_free(fftp *p)
 {
     if (!p) return;
     free(p->w);
-    free(p->tmp);
-    free(p->tmp2);
     free(p);
 }
 
@@ -81,7 +112,7 @@ static void fft_rec(const fftp *p, const cplx *x, long is, cplx *y, int n, int d
     /* radix f: a prime factor of n, any size (e.g. 193 in 2 n_r - 2 = 386) */
     cplx tbuf[64];
     cplx *t = f <= 64 ? tbuf : malloc(sizeof(cplx) * f);
-    cplx *out = p->tmp2;
+    cplx *out = tl_cbuf(1, 0);
     for (int k = 0; k < q; k++) {
         for (int r = 0; r < f; r++) t[r] = y[r * q + k] * twid(p, (long)r * k * step, dir);
         for (int s = 0; s < f; s++) {
@@ -96,8 +127,10 @@ static void fft_rec(const fftp *p, const cplx *x, long is, cplx *y, int n, int d
 
 static void fft_exec(fftp *p, const cplx *in, cplx *out, int dir)
 {
-    memcpy(p->tmp, in, sizeof(cplx) * p->n);
-    fft_rec(p, p->tmp, 1, out, p->n, dir);
+    cplx *x = tl_cbuf(0, p->n);
+    (void)tl_cbuf(1, p->n); /* sized before the recursion reuses it */
+    memcpy(x, in, sizeof(cplx) * p->n);
+    fft_rec(p, x, 1, out, p->n, dir);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -194,8 +227,7 @@ struct or_handle {
     cplx *cPmat;
     double *cPfac;
     /* scratch */
-    cplx *w1, *w2, *w3;
-    double *gr[16];
+    int wbig; /* length of the per-thread transform buffers (tl_cbuf) */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -248,7 +280,7 @@ void or_config_default(pz_config *c)
 static void phi_fft(or_handle *h, const double *grid, cplx *spec, int m_R)
 {
     int n = h->nphi;
-    cplx *a = h->w1, *b = h->w2;
+    cplx *a = tl_cbuf(2, h->wbig), *b = tl_cbuf(3, h->wbig);
     for (int j = 0; j < n; j++) a[j] = grid[j];
     fft_exec(h->fphi, a, b, -1);
     for (int nm = 0; nm < h->nm; nm++) {
@@ -261,7 +293,7 @@ static void phi_fft(or_handle *h, const double *grid, cplx *spec, int m_R)
 static void phi_ifft(or_handle *h, const cplx *spec, double *grid, int m_R)
 {
     int n = h->nphi;
-    cplx *a = h->w1, *b = h->w2;
+    cplx *a = tl_cbuf(2, h->wbig), *b = tl_cbuf(3, h->wbig);
     for (int k = 0; k < n; k++) a[k] = 0;
     for (int nm = 0; nm < h->nm; nm++) {
         cplx v = (h->idx2m[nm] <= m_R) ? spec[nm] : 0;
@@ -285,7 +317,7 @@ static void phi_ifft(or_handle *h, const cplx *spec, double *grid, int m_R)
 static void costf_vec(or_handle *h, cplx *f)
 {
     int N = h->N, L = 2 * N - 2;
-    cplx *a = h->w1, *b = h->w2;
+    cplx *a = tl_cbuf(2, h->wbig), *b = tl_cbuf(3, h->wbig);
     for (int j = 0; j < N; j++) a[j] = f[j];
     for (int j = N; j < L; j++) a[j] = f[L - j];
     fft_exec(h->fdct, a, b, -1);
@@ -296,7 +328,7 @@ static void costf_vec(or_handle *h, cplx *f)
 static void costf_real(or_handle *h, double *f)
 {
     int N = h->N, L = 2 * N - 2;
-    cplx *a = h->w1, *b = h->w2;
+    cplx *a = tl_cbuf(2, h->wbig), *b = tl_cbuf(3, h->wbig);
     for (int j = 0; j < N; j++) a[j] = f[j];
     for (int j = N; j < L; j++) a[j] = f[L - j];
     fft_exec(h->fdct, a, b, -1);
@@ -305,6 +337,7 @@ static void costf_real(or_handle *h, double *f)
 
 static void costf_field(or_handle *h, cplx *F)
 {
+#pragma omp parallel for schedule(static)
     for (int nm = 0; nm < h->nm; nm++) costf_vec(h, F + (size_t)nm * h->N);
 }
 
@@ -313,7 +346,7 @@ static void costf_field(or_handle *h, cplx *F)
 static void get_dr_vec(or_handle *h, const cplx *f, cplx *df, int l_dct_in)
 {
     int N = h->N, L = 2 * N - 2;
-    cplx *a = h->w1, *o = h->w3, *b = h->w2;
+    cplx *a = tl_cbuf(2, h->wbig), *o = tl_cbuf(4, h->wbig), *b = tl_cbuf(3, h->wbig);
     if (l_dct_in) {
         for (int j = 0; j < N; j++) a[j] = f[j];
         for (int j = N; j < L; j++) a[j] = f[L - j];
@@ -341,7 +374,7 @@ static void get_dr_vec(or_handle *h, const cplx *f, cplx *df, int l_dct_in)
 static void get_ddr_vec(or_handle *h, const cplx *f, cplx *df, cplx *ddf, int l_dct_in)
 {
     int N = h->N, L = 2 * N - 2;
-    cplx *a = h->w1, *t = h->w3, *b = h->w2;
+    cplx *a = tl_cbuf(2, h->wbig), *t = tl_cbuf(4, h->wbig), *b = tl_cbuf(3, h->wbig);
     cplx *w2 = malloc(sizeof(cplx) * L), *w1 = malloc(sizeof(cplx) * L);
     if (l_dct_in) {
         for (int j = 0; j < N; j++) a[j] = f[j];
@@ -1844,6 +1877,8 @@ int or_create(const pz_config *cfg, or_handle **out)
         h->XidiffFac = h->l_chem ? 1.0 / cfg->sc : 0.0;
         h->ViscFac = 1.0;
     }
+    if (or_plant == 4) h->ChemFac = -h->ChemFac;
+    if (or_plant == 5) h->XidiffFac = 2.0 * h->XidiffFac;
     /* chebyshev.f90:55-60, dct_fftw.f90:124 */
     h->rnorm = sqrt(2.0 / (double)(N - 1));
     h->boundary_fac = 0.5;
@@ -1861,9 +1896,7 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->fdct = fft_new(L);
     h->fphi = fft_new(h->nphi);
     int big = L > h->nphi ? L : h->nphi;
-    h->w1 = calloc(big, sizeof(cplx));
-    h->w2 = calloc(big, sizeof(cplx));
-    h->w3 = calloc(big, sizeof(cplx));
+    h->wbig = big;
     /* grid: chebyshev.f90:188-240 */
     double *arr[] = {0};
     (void)arr;
@@ -1940,7 +1973,6 @@ int or_create(const pz_config *cfg, or_handle **out)
     tarray_alloc(h, &h->dpsidt, &h->ts);
     h->dtr = dalloc(N);
     h->dth = dalloc(N);
-    for (int i = 0; i < 16; i++) h->gr[i] = dalloc(h->nphi);
     /* matrices */
     if (h->fd) {
         fd_alloc(h);
@@ -2091,12 +2123,16 @@ static void radial_loop(or_handle *h, int is /* 0-based stage */)
 {
     int N = h->N, nm = h->nm, np = h->nphi;
     size_t F = (size_t)nm * N;
-    double *usg = h->gr[0], *upg = h->gr[1], *omg = h->gr[2], *tg = h->gr[3], *xg = h->gr[4];
-    double *usT = h->gr[5], *upT = h->gr[6], *usX = h->gr[7], *upX = h->gr[8], *usO = h->gr[9],
-           *upO = h->gr[10];
-    cplx *sp = malloc(sizeof(cplx) * nm), *o = malloc(sizeof(cplx) * nm);
     cplx *dpsidt = h->dpsidt.expl + is * F, *dTdt = h->dTdt.expl + is * F, *dxidt = h->dxidt.expl + is * F;
+    /* the radii are independent: one OpenMP thread per radius, each with its
+     * own grid and spectrum buffers */
+#pragma omp parallel for schedule(static)
     for (int n_r = 0; n_r < N; n_r++) {
+        double *gb = tl_dbuf(11 * (size_t)np);
+        double *usg = gb, *upg = gb + np, *omg = gb + 2 * np, *tg = gb + 3 * np, *xg = gb + 4 * np;
+        double *usT = gb + 5 * np, *upT = gb + 6 * np, *usX = gb + 7 * np, *upX = gb + 8 * np, *usO = gb + 9 * np,
+               *upO = gb + 10 * np;
+        cplx *sp = tl_cbuf(5, 2 * (size_t)nm), *o = sp + nm;
         double usom = 0;
         for (int i = 0; i < nm; i++) {
             int m = h->idx2m[i];
@@ -2152,8 +2188,6 @@ static void radial_loop(or_handle *h, int is /* 0-based stage */)
             else dpsidt[q] = cim * dpsidt[q];
         }
     }
-    free(sp);
-    free(o);
 }
 
 static void dealias(or_handle *h, cplx *F)
@@ -2170,6 +2204,7 @@ static void finish_exp_scalar(or_handle *h, cplx *dVs, cplx *exp_last, const dou
     int N = h->N;
     costf_field(h, dVs);
     dealias(h, dVs);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) get_dr_vec(h, dVs + (size_t)i * N, h->work + (size_t)i * N, 0);
     for (int n_r = 0; n_r < N; n_r++) {
         double h2 = h->r_cmb * h->r_cmb - h->r[n_r] * h->r[n_r];
@@ -2200,6 +2235,7 @@ static void finish_exp_psi(or_handle *h, cplx *dpsi)
     int N = h->N;
     costf_field(h, h->dVsOm);
     dealias(h, h->dVsOm);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsOm + (size_t)i * N, h->work + (size_t)i * N, 0);
     for (int n_r = 0; n_r < N; n_r++)
         for (int i = 0; i < h->nm; i++) {
@@ -2215,16 +2251,18 @@ static void finish_exp_psi(or_handle *h, cplx *dpsi)
             }
         }
     if (h->c.l_ek_pump) {
+        const double sgn = or_plant == 1 ? -1.0 : 1.0, rfac = or_plant == 2 ? 0.0 : 1.0;
+        const double usfac = or_plant == 3 ? 0.0 : 1.0;
         for (int n_r = 0; n_r < N; n_r++) {
-            double ekp_fac = h->CorFac * h->ekpump[n_r];
+            double ekp_fac = sgn * h->CorFac * h->ekpump[n_r];
             for (int i = 0; i < h->nm; i++) {
                 int m = h->idx2m[i];
                 size_t q = (size_t)i * N + n_r;
                 if (m == 0) dpsi[q] = dpsi[q] - ekp_fac * h->up[q];
                 else
-                    dpsi[q] = dpsi[q] - ekp_fac * h->r[n_r] *
+                    dpsi[q] = dpsi[q] - ekp_fac * (rfac * h->r[n_r] + (1.0 - rfac)) *
                                             (h->om[q] + h->ekp_up[n_r] * h->up[q] +
-                                             (I * (double)m * h->ekp_dusdp[n_r] + h->ekp_us[n_r]) * h->us[q]);
+                                             usfac * (I * (double)m * h->ekp_dusdp[n_r] + h->ekp_us[n_r]) * h->us[q]);
             }
         }
     }
@@ -2408,6 +2446,7 @@ static int update_psi(or_handle *h)
         if (h->idx2m[i] != 0) memcpy(h->psi + (size_t)i * N, h->psi_hat + (size_t)i * N, sizeof(cplx) * N);
     get_dr_real(h, uphi0, om0, 0);
     costf_real(h, uphi0);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++)
         get_ddr_vec(h, h->psi + (size_t)i * N, h->work + (size_t)i * N, h->om + (size_t)i * N, 0);
     costf_field(h, h->psi);
@@ -2641,24 +2680,29 @@ static int coll_uphi_mat(or_handle *h, double *A, int *piv)
 static int coll_build(or_handle *h)
 {
     int N = h->N, N2 = 2 * N;
+    /* the modes factorise independently; the first failing mode (in m
+     * order) reports its info, as the serial loop would */
+    int *infos = calloc(h->nm, sizeof(int));
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) {
-        int m = h->idx2m[i], info;
-        if (h->l_heat) {
+        int m = h->idx2m[i], info = 0;
+        if (h->l_heat)
             info = coll_scalar_mat(h, m, h->TdiffFac * h->hdifT[i], h->c.ktopt, h->c.kbott, h->cTmat + (size_t)i * N * N,
                                    h->cTfac + (size_t)i * N, h->cTpiv + (size_t)i * N);
-            if (info) return info;
-        }
-        if (h->l_chem) {
+        if (!info && h->l_chem)
             info = coll_scalar_mat(h, m, h->XidiffFac * h->hdifXi[i], h->c.ktopxi, h->c.kbotxi,
                                    h->cXmat + (size_t)i * N * N, h->cXfac + (size_t)i * N, h->cXpiv + (size_t)i * N);
-            if (info) return info;
+        if (!info) {
+            if (m == 0) info = coll_uphi_mat(h, h->cUmat, h->cUpiv);
+            else info = coll_psi_mat(h, m, h->cPmat + (size_t)i * N2 * N2, h->cPfac + (size_t)i * 2 * N2,
+                                     h->cPpiv + (size_t)i * N2);
         }
-        if (m == 0) info = coll_uphi_mat(h, h->cUmat, h->cUpiv);
-        else info = coll_psi_mat(h, m, h->cPmat + (size_t)i * N2 * N2, h->cPfac + (size_t)i * 2 * N2,
-                                 h->cPpiv + (size_t)i * N2);
-        if (info) return info;
+        infos[i] = info;
     }
-    return 0;
+    int info = 0;
+    for (int i = 0; i < h->nm && !info; i++) info = infos[i];
+    free(infos);
+    return info;
 }
 
 static void coll_alloc(or_handle *h)
@@ -2721,6 +2765,7 @@ static void coll_finish_exp(or_handle *h, int is)
     size_t F = (size_t)h->nm * N;
     if (h->l_heat) {
         cplx *e = h->dTdt.expl + is * F;
+    #pragma omp parallel for schedule(static)
         for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsT + (size_t)i * N, h->work + (size_t)i * N, 1);
         for (int i = 0; i < h->nm; i++)
             for (int k = 0; k < N; k++) {
@@ -2730,6 +2775,7 @@ static void coll_finish_exp(or_handle *h, int is)
     }
     if (h->l_chem) {
         cplx *e = h->dxidt.expl + is * F;
+    #pragma omp parallel for schedule(static)
         for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsXi + (size_t)i * N, h->work + (size_t)i * N, 1);
         for (int i = 0; i < h->nm; i++)
             for (int k = 0; k < N; k++) {
@@ -2738,6 +2784,7 @@ static void coll_finish_exp(or_handle *h, int is)
             }
     }
     cplx *e = h->dpsidt.expl + is * F;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->dVsOm + (size_t)i * N, h->work + (size_t)i * N, 1);
     for (int i = 0; i < h->nm; i++) {
         int m = h->idx2m[i];
@@ -2761,9 +2808,11 @@ static void coll_rhs_imp_scalar(or_handle *h, cplx *fld, cplx *dfld, tarray *a, 
 {
     int N = h->N;
     size_t F = (size_t)h->nm * N;
-    if (l_calc_lin)
+    if (l_calc_lin) {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < h->nm; i++)
             get_ddr_vec(h, fld + (size_t)i * N, dfld + (size_t)i * N, h->work + (size_t)i * N, !in_cheb);
+    }
     if (in_cheb) costf_field(h, fld);
     if (istage == 1) memcpy(a->old, fld, sizeof(cplx) * F);
     if (l_calc_lin)
@@ -2789,9 +2838,11 @@ static void coll_rhs_imp_psi(or_handle *h, tarray *a, int istage, int l_calc_lin
     if (nm0 >= 0)
         for (int k = 0; k < N; k++) om0[k] = creal(h->om[(size_t)nm0 * N + k]);
     cplx *dom = calloc(F, sizeof(cplx));
-    if (l_calc_lin)
+    if (l_calc_lin) {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < h->nm; i++)
             get_ddr_vec(h, h->om + (size_t)i * N, dom + (size_t)i * N, h->work + (size_t)i * N, !in_cheb);
+    }
     if (in_cheb) costf_field(h, h->om);
     if (nm0 >= 0)
         for (int k = 0; k < N; k++) h->om[(size_t)nm0 * N + k] = om0[k];
@@ -2844,8 +2895,9 @@ static void coll_update_scalar(or_handle *h, int is_temp)
     double *Am = is_temp ? h->cTmat : h->cXmat, *fac = is_temp ? h->cTfac : h->cXfac;
     int *piv = is_temp ? h->cTpiv : h->cXpiv;
     ts_set_imex_rhs(h, t, a, h->work);
-    double *rr = dalloc(N), *ri = dalloc(N);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) {
+        double *rr = tl_dbuf(2 * (size_t)N), *ri = rr + N;
         cplx *w = h->work + (size_t)i * N;
         w[0] = top[i];
         w[N - 1] = bot[i];
@@ -2858,8 +2910,6 @@ static void coll_update_scalar(or_handle *h, int is_temp)
         getrs_r(N, Am + (size_t)i * N * N, piv + (size_t)i * N, ri);
         for (int k = 0; k < N; k++) fld[(size_t)i * N + k] = k < h->ncheb ? rr[k] + I * ri[k] : 0.0;
     }
-    free(rr);
-    free(ri);
     ts_rotate(h, t, a);
     double diff = is_temp ? h->TdiffFac : h->XidiffFac;
     double *hdif = is_temp ? h->hdifT : h->hdifXi;
@@ -2876,13 +2926,16 @@ static void coll_update_psi(or_handle *h)
     double wimp = t->wimp_lin[0];
     ts_set_imex_rhs(h, t, a, h->work);
     double *u0 = dalloc(N), *om0 = dalloc(N);
-    cplx *rhs = calloc(N2, sizeof(cplx));
     int nm0 = -1;
+    for (int i = 0; i < h->nm; i++)
+        if (h->idx2m[i] == 0) nm0 = i;
+    /* one mode per OpenMP thread; only the m = 0 iteration writes u0 */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) {
         int m = h->idx2m[i];
         cplx *w = h->work + (size_t)i * N;
+        cplx *rhs = tl_cbuf(5, N2);
         if (m == 0) {
-            nm0 = i;
             u0[0] = 0.0;
             u0[N - 1] = 0.0;
             for (int k = 1; k < N - 1; k++) u0[k] = creal(w[k]);
@@ -2914,6 +2967,7 @@ static void coll_update_psi(or_handle *h)
         costf_real(h, u0);
         for (int k = 0; k < N; k++) h->psi[(size_t)nm0 * N + k] = 0.0;
     }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->psi + (size_t)i * N, h->work + (size_t)i * N, 0);
     costf_field(h, h->psi);
     for (int i = 0; i < h->nm; i++) {
@@ -2933,7 +2987,6 @@ static void coll_update_psi(or_handle *h)
     ts_rotate(h, t, a);
     if (t->istage == t->nstages) coll_rhs_imp_psi(h, a, 1, t->l_imp_calc_rhs[0], 1);
     else coll_rhs_imp_psi(h, a, t->istage + 1, t->l_imp_calc_rhs[t->istage], 1);
-    free(rhs);
     free(u0);
     free(om0);
 }
@@ -3335,6 +3388,7 @@ static void fd_psi_mat(or_handle *h, int i)
 static void fd_build(or_handle *h)
 {
     int N = h->N;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) {
         if (h->l_heat)
             fd_scalar_mat(h, i, h->TdiffFac, h->c.ktopt, h->c.kbott, h->tTd + (size_t)i * N, h->tTl + (size_t)i * N,
@@ -3383,8 +3437,9 @@ static void fd_finish_exp(or_handle *h, int is)
 {
     int N = h->N;
     size_t F = (size_t)h->nm * N;
-    cplx *w = malloc(sizeof(cplx) * N);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < h->nm; i++) {
+        cplx *w = tl_cbuf(5, N);
         int m = h->idx2m[i];
         size_t o = (size_t)i * N;
         cplx *ep = h->dpsidt.expl + is * F + o;
@@ -3409,7 +3464,6 @@ static void fd_finish_exp(or_handle *h, int is)
             for (int k = 0; k < N; k++) ex[k] = ex[k] - h->or1[k] * w[k] - h->us[o + k] * h->dxicond[k];
         }
     }
-    free(w);
 }
 
 /* fill_ghosts_temp / fill_ghosts_xi (update_temp_fd.f90:117-169) */
@@ -3563,6 +3617,7 @@ static void fd_mloop(or_handle *h)
     free(rhs);
     /* solves */
     fd_solve3(N, h->uD, h->uL, h->uU, h->u0g);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nm; i++) {
         size_t o = (size_t)i * N;
         if (h->l_heat) fd_solve3(N, h->tTd + o, h->tTl + o, h->tTu + o, &TG(h->tg, i, -1));
@@ -4132,6 +4187,18 @@ int or_fd_ell_solve(or_handle *h, int mi, const double *omega, double *psi)
     fd_ell_solve(h, h->idx2m[mi], x);
     memcpy(psi, x + 1, sizeof(cplx) * N);
     free(x);
+    return 0;
+}
+
+int or_cond(or_handle *h, double *out)
+{
+    int N = h->N;
+    for (int n = 0; n < N; n++) {
+        out[n] = h->l_heat ? h->tcond[n] : 0.0;
+        out[N + n] = h->l_heat ? h->dtcond[n] : 0.0;
+        out[2 * N + n] = h->l_chem ? h->xicond[n] : 0.0;
+        out[3 * N + n] = h->l_chem ? h->dxicond[n] : 0.0;
+    }
     return 0;
 }
 

@@ -36,6 +36,10 @@ int or_ass_solve(or_handle *h, int mi, const double *omega, double *psi);
 /* -L psi = omega of the finite-difference assembly stage, psi = 0 at the walls */
 int or_fd_ell_solve(or_handle *h, int mi, const double *omega, double *psi);
 int or_radial(or_handle *h, double *out);
+/* conducting profiles [tcond | dtcond | xicond | dxicond] (4 n_r values) */
+int or_cond(or_handle *h, double *out);
+/* planted mistake of the failure-demonstrating tests (0 = none) */
+extern int or_plant;
 int or_dirk_tables(int scheme, int *nstages, int *l_assembly, double *wimp_lin0, double *bimp, double *bexp,
                    double *bass_imp, double *bass_exp);
 

@@ -774,6 +774,20 @@ static std::vector<double> fd_coeffs(const std::vector<double> &r)
     return c;
 }
 
+// Static LDS of the collocation rebuild kernels (the larger of the single-run
+// and batched variants), read from the compiled kernels once per process.
+static size_t gj_static_lds()
+{
+    static size_t bytes = 0;
+    if (!bytes) {
+        hipFuncAttributes a, b;
+        HIPCHK(hipFuncGetAttributes(&a, (const void *)k_build_coll));
+        HIPCHK(hipFuncGetAttributes(&b, (const void *)k_build_coll_batch));
+        bytes = std::max<size_t>({a.sharedSizeBytes, b.sharedSizeBytes, 1});
+    }
+    return bytes;
+}
+
 // Lets every kernel with dynamic LDS use the whole per-workgroup budget, once
 // per process: a per-run value would shrink the limit for a larger run
 // created earlier (runs of different sizes share the kernels).
@@ -1110,8 +1124,10 @@ static void setup(pz_handle *h)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     size_t lim = prop.sharedMemPerBlock;
-    // rebuild panels of 16 pivots where they fit next to the static LDS, else 8
-    d.gj_b = gj_lds_bytes<cd>(2 * N, 16) + GJ_STATIC_LDS <= lim ? 16 : 8;
+    // rebuild panels of 16 pivots where they fit next to the larger static LDS
+    // of the single-run and batched rebuild kernels, else 8
+    const size_t gj_static = gj_static_lds();
+    d.gj_b = gj_lds_bytes<cd>(2 * N, 16) + gj_static <= lim ? 16 : 8;
     h->lds_gj = gj_lds_bytes<cd>(2 * N, d.gj_b);
     if (d.coll && h->lds_mphase > lim) {
         h->mcoll_l2 = true;
@@ -1127,7 +1143,7 @@ static void setup(pz_handle *h)
     // the kernels of this run's radial method
     size_t need = h->lds_radial;
     if (d.fd) need = std::max(need, h->lds_fd);
-    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + GJ_STATIC_LDS,
+    else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + gj_static,
                                       h->sch.l_assembly ? asm_lds_bytes(N) : (size_t)0});
     else need = std::max({need, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
     if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
@@ -2050,6 +2066,11 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
     if (!h || !avg_ms || reps <= 0) return -1;
     if (!h->started) return fail(h, -5, "pz_start has not been called");
     try {
+        // the m-loop kernels advance the fields and the time-scheme slots:
+        // keep a copy of the step state and put it back after the timing
+        char *keep = nullptr;
+        HIPCHK(hipMalloc(&keep, h->arena_bytes));
+        HIPCHK(hipMemcpyAsync(keep, h->arena, h->arena_bytes, hipMemcpyDeviceToDevice, h->stream));
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -2075,6 +2096,9 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
         *avg_ms = (double)ms / reps;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        HIPCHK(hipMemcpyAsync(h->arena, keep, h->arena_bytes, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipFree(keep));
     } catch (std::exception &e) {
         return fail(h, -4, e.what());
     }

@@ -82,8 +82,6 @@ __device__ int gj_invert(M *W, int n, M *f)
 
 #define GJ_BMAX 16 // widest panel (pivot columns per panel)
 #define GJ_CH 64   // columns per chunk of the panel update
-// static LDS of k_build_coll / k_build_coll_batch is below this (about 9.1 KB)
-#define GJ_STATIC_LDS 10240
 // dynamic LDS bytes of gj_invert_blk for n x n matrices of element M and
 // panels of B pivot columns
 template <class M>
