@@ -158,6 +158,90 @@ int pz_start(pz_handle *h);
  */
 int pz_step(pz_handle *h, int nsteps);
 
+/*
+ * The step one procedure of time_loop at a time (step_time.f90:215-370), for
+ * a driver that keeps the reference's stage loop:
+ *   for stage = 1..nstages (pz_nstages):
+ *       pz_stage_explicit(h, stage)        radial_loop with its transposes
+ *                                          (radial_loop.f90:231-394,
+ *                                          step_time.f90:222-251)
+ *       if stage == 1: pz_courant(h, ...)  dt_courant, dt roll, set_weights,
+ *                                          rebuild flag (courant.f90:78-167,
+ *                                          step_time.f90:256-304)
+ *       pz_stage_implicit(h, stage, lMat)  finish_explicit_assembly and mloop
+ *                                          (mloop.f90:36-103, 169-229)
+ * is pz_step(h, 1) launch for launch (same kernels, same order), so the
+ * results are bit for bit those of pz_step.
+ * pz_stage_explicit runs the radial loop of the stage when the scheme
+ * evaluates explicit terms there (l_exp_calc), else nothing.
+ * pz_courant synchronises only when dtr or dth is not NULL; they receive
+ * the smallest radial and horizontal Courant limits of the step.
+ * pz_stage_implicit: on stage 1 it first rebuilds the matrices when lMat is
+ * nonzero or dt changed (the rebuild of step_time.f90:277-304); the explicit
+ * finish of the stage is fused into the per-m kernel of mloop; on the last
+ * stage of a DIRK scheme with an assembly stage it runs assemble_stage
+ * (mloop.f90:105-160) instead of mloop.  All three are asynchronous on the
+ * handle's stream; stage must lie in 1..nstages.
+ */
+int pz_nstages(const pz_handle *h);
+int pz_stage_explicit(pz_handle *h, int stage);
+int pz_courant(pz_handle *h, double *dtr, double *dth);
+int pz_stage_implicit(pz_handle *h, int stage, int lMat);
+
+/*
+ * Checkpoint files of checkpoints.f90.
+ * pz_write_checkpoint writes a version-7 file of the run's state
+ * (write_checkpoint_mloc, checkpoints.f90:33-183): the header (time, scheme
+ * family and sizes, dt array, physical parameters, grid, radial scheme and
+ * radii), then u_s, u_phi and the psi history, T and its history, xi and its
+ * history (expl(2:nexp), impl(2:nimp), old(2:nold) of a multistep scheme).
+ * Synchronises; needs a started run.
+ * pz_read_checkpoint reads a version 1-7 file into the run
+ * (read_checkpoint, checkpoints.f90:185-707): the fields and the history
+ * slots the run's scheme keeps, mapped onto the run's grid (same grid: a
+ * copy; Chebyshev to Chebyshev: through the coefficients; to or from finite
+ * differences: cubic interpolation between the nearest stored radii), modes
+ * above the stored m_max start from zero, the time and dt array of the file,
+ * and the restart rules of the reference (first-order Adams-Bashforth first
+ * step when ek or the radial method changed, CNAB2 start-up steps when the
+ * stored history is shorter than the scheme needs).  pz_start must follow:
+ * it then performs the restart branch of get_start_fields
+ * (init_fields.f90:67-95), keeping the stored history.  pz_init_fields may
+ * be called in between to add the namelist's perturbation, as the
+ * reference does after reading.
+ * pz_read_checkpoint_header reads only the header of a file (host code, no
+ * device): r (r_len values, may be NULL) receives the stored radii,
+ * n_arrays the number of (n_m_max, n_r_max) complex arrays that follow the
+ * header_bytes of the header (phase-field arrays included).
+ */
+typedef struct pz_checkpoint_info {
+    int version;
+    double time;
+    int l_cheb_coll;
+    char family[11];   /* "MULTISTEP" or "DIRK" (versions >= 5) */
+    int nexp, nimp, nold, n_dt;
+    double dt[16];
+    double ra, pr, raxi, sc, ek, stef, radratio;
+    int n_r_max, m_max, minc;
+    char rscheme[73];  /* "cheb" or "fd" */
+    int n_max, order_boundary;
+    double alph1, alph2;
+    int l_heat, l_chem, l_phase_field;
+    long long header_bytes;
+    int n_arrays;
+} pz_checkpoint_info;
+int pz_write_checkpoint(pz_handle *h, const char *path);
+int pz_read_checkpoint(pz_handle *h, const char *path);
+int pz_read_checkpoint_header(const char *path, pz_checkpoint_info *info, double *r, int r_len);
+
+/*
+ * A frame file of one field (write_snapshot_mloc, output_frames.f90:96-167,
+ * version 2): time, ra, ek, pr, radratio, sc, raxi, n_r_max, n_m_max, m_max,
+ * minc, n_phi_max, r, tcond, xicond, idx2m, then the field complex(8)
+ * (n_m_max, n_r_max).  Synchronises.
+ */
+int pz_write_frame(pz_handle *h, int field, const char *path);
+
 /* Synchronises and returns the simulation time and current dt(1). */
 int pz_time(pz_handle *h, double *time, double *dt);
 

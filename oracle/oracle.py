@@ -41,6 +41,8 @@ def lib():
         L.or_fd_ell_solve.argtypes = [vp, ip, dp, dp]
         L.or_radial.argtypes = [vp, dp]
         L.or_cond.argtypes = [vp, dp]
+        L.or_read_checkpoint.argtypes = [vp, ctypes.c_char_p]
+        L.or_write_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.or_dirk_tables.argtypes = [ip, ctypes.POINTER(ip), ctypes.POINTER(ip), dp, dp, dp, dp, dp]
         L.or_costf.argtypes = [vp, dp, ip]
         L.or_get_dr.argtypes = [vp, dp, dp, ip, ip]
@@ -155,6 +157,15 @@ class Oracle:
         self._check(lib().or_fd_ell_solve(self.h, mi, _dp(w.view(np.float64)), _dp(out.view(np.float64))),
                     "or_fd_ell_solve")
         return out
+
+    def write_checkpoint(self, path):
+        """Version-7 checkpoint of the run (checkpoints.f90:33-183)."""
+        self._check(lib().or_write_checkpoint(self.h, str(path).encode()), "write_checkpoint")
+
+    def read_checkpoint(self, path):
+        """Reads a checkpoint into this run (checkpoints.f90:185-707); call
+        start() afterwards for the restart branch of get_start_fields."""
+        self._check(lib().or_read_checkpoint(self.h, str(path).encode()), "read_checkpoint")
 
     def cond(self):
         """tcond, dtcond, xicond, dxicond on the grid."""

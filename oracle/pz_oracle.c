@@ -204,6 +204,10 @@ struct or_handle {
     tscheme ts;
     double time;
     int lMatNext, n_time_step, started;
+    /* restart (read_checkpoint, checkpoints.f90:185-707): l_bridge_step and
+     * l_AB1 of namelists.f90:364-365, the time and dt array of the file */
+    int l_bridge, l_ab1, restart;
+    double rst_time, rst_dt[8];
     double *dtr, *dth;
     /* finite differences (radial_scheme = 'FD', finite_differences.f90) */
     int fd;
@@ -1841,6 +1845,7 @@ int or_create(const pz_config *cfg, or_handle **out)
     if (cfg->radratio == 0.0) return -13;
     or_handle *h = calloc(1, sizeof(or_handle));
     h->c = *cfg;
+    h->l_bridge = 1; /* namelists.f90:365: a fresh start bridges with CNAB2 */
     int N = h->N = cfg->n_r_max;
     h->mmax = cfg->m_max;
     h->minc = cfg->minc;
@@ -3849,7 +3854,7 @@ static int one_step(or_handle *h)
          * (l_bridge_step, namelists.f90:365): the first nold-1 steps of a
          * multistep scheme other than CNAB2 recompute the implicit terms and
          * take the CNAB2 weights */
-        if (t->family == 0 && t->scheme != PZ_CNAB2 && h->n_time_step + 1 <= t->nold - 1) {
+        if (h->l_bridge && t->family == 0 && t->scheme != PZ_CNAB2 && h->n_time_step + 1 <= t->nold - 1) {
             if (h->fd) {
                 fd_from_fields(h, 0);
             } else if (h->coll) {
@@ -3863,6 +3868,14 @@ static int one_step(or_handle *h)
                 rhs_imp_psi(h, h->psi_hat, h->up, h->temp, h->psi, &h->dpsidt, 1, 1);
             }
             ts_bridge(t, h->c.alpha);
+        }
+        /* start_with_ab1 (multistep_schemes.f90:526-534) on the first step
+         * after a restart that changed ek or the radial method
+         * (step_time.f90:511-514) */
+        if (h->l_ab1 && h->n_time_step == 0 && t->family == 0) {
+            t->wexp[0] = t->dt[0];
+            for (int o = 1; o < t->nexp; o++) t->wexp[o] = 0.0;
+            h->l_ab1 = 0;
         }
         if (t->l_assembly && t->istage == t->nstages) break; /* step_time.f90:313 */
         if (lMat) {
@@ -4067,9 +4080,21 @@ int or_start(or_handle *h)
     int N = h->N;
     size_t F = (size_t)h->nm * N;
     tscheme *t = &h->ts;
-    h->time = 0.0;
+    h->time = h->restart ? h->rst_time : 0.0;
     h->n_time_step = 0;
-    for (int i = 0; i < 8; i++) t->dt[i] = h->c.dtMax;
+    for (int i = 0; i < 8; i++) t->dt[i] = h->restart ? h->rst_dt[i] : h->c.dtMax;
+    if (h->restart && !h->fd && !h->coll) {
+        /* get_start_fields after read_checkpoint (init_fields.f90:67-95): the
+         * integration method rebuilds psi(m /= 0) from the stored u_s */
+        for (int i = 0; i < h->nm; i++) {
+            int m = h->idx2m[i];
+            for (int k = 0; k < N; k++) {
+                double h2 = h->r_cmb * h->r_cmb - h->r[k] * h->r[k];
+                h->psi[(size_t)i * N + k] = (m > 0 && k > 0) ? -I * h->r[k] / (double)m / h2 * h->us[(size_t)i * N + k] : 0.0;
+            }
+        }
+    }
+    h->restart = 0;
     int dummy = 0;
     ts_set_weights(t, h->c.alpha, &dummy);
     /* dummy Courant limits before the first radial loop (step_time.f90:124-126) */
@@ -4113,6 +4138,271 @@ int or_start(or_handle *h)
     h->started = 1;
     h->lMatNext = 1;
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Checkpoints (checkpoints.f90): version 7 writer and the reader with its    */
+/* grid, truncation and scheme remap                                          */
+/* ------------------------------------------------------------------------ */
+
+/* costf1 of a real vector of length n (dct_fftw.f90:231-246) as the direct
+ * cosine sum of the DCT-I, sqrt(0.5/(n-1)) scaling */
+static void ck_costf(int n, double *f)
+{
+    double *g = malloc(sizeof(double) * n);
+    const ld pi = 3.14159265358979323846264338327950288L;
+    for (int k = 0; k < n; k++) {
+        ld acc = (ld)f[0] + ((k & 1) ? -(ld)f[n - 1] : (ld)f[n - 1]);
+        for (int j = 1; j < n - 1; j++) acc += 2.0L * (ld)f[j] * cosl(pi * (ld)(((long)j * k) % (2 * (n - 1))) / (ld)(n - 1));
+        g[k] = (double)(acc * sqrtl(0.5L / (ld)(n - 1)));
+    }
+    memcpy(f, g, sizeof(double) * n);
+    free(g);
+}
+
+/* the cubic through four points at xn (polynomial_interpolation,
+ * useful.f90:99-179, there by Neville's recursion), in Lagrange form */
+static cplx ck_cubic(const double *x, const cplx *y, double xn)
+{
+    cplx out = 0;
+    for (int a = 0; a < 4; a++) {
+        double w = 1.0;
+        for (int b = 0; b < 4; b++)
+            if (b != a) w *= (xn - x[b]) / (x[a] - x[b]);
+        out += w * y[a];
+    }
+    return out;
+}
+
+typedef struct {
+    int n_r, n_m, m_max, minc, cheb, order_boundary;
+    double *r;
+} ck_grid;
+
+/* map_field / map_field_r (checkpoints.f90:709-866) of one field stored
+ * (n_m_old, n_r_old), m fastest, into dst [m][r] of this run */
+static void ck_map(or_handle *h, const ck_grid *g, const cplx *old, cplx *dst, int lBc, int l_phys)
+{
+    int N = h->N, cheb_new = !h->fd, ob_new = h->fd ? 2 : 0;
+    int same = g->n_r == N && g->order_boundary == ob_new && g->cheb == cheb_new;
+    int nmax = N > g->n_r ? N : g->n_r;
+    cplx *col = malloc(sizeof(cplx) * nmax);
+    double *re = malloc(sizeof(double) * nmax), *im = malloc(sizeof(double) * nmax);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        int io = (m <= g->m_max && m % g->minc == 0) ? m / g->minc : -1;
+        cplx *d = dst + (size_t)i * N;
+        if (io < 0) {
+            for (int k = 0; k < N; k++) d[k] = 0;
+            continue;
+        }
+        for (int k = 0; k < g->n_r; k++) col[k] = old[(size_t)k * g->n_m + io];
+        if (same) {
+            for (int k = 0; k < N; k++) d[k] = col[k];
+            continue;
+        }
+        if (lBc && l_phys) col[0] = 2.0 * col[1] - col[2];
+        if (cheb_new && g->cheb && g->order_boundary == ob_new) {
+            for (int k = 0; k < g->n_r; k++) { re[k] = creal(col[k]); im[k] = cimag(col[k]); }
+            if (l_phys) { ck_costf(g->n_r, re); ck_costf(g->n_r, im); }
+            for (int k = g->n_r; k < N; k++) re[k] = im[k] = 0.0;
+            if (l_phys) { ck_costf(N, re); ck_costf(N, im); }
+            double fac = sqrt(2.0 / (double)(g->n_r - 1)) / h->rnorm;
+            for (int k = 0; k < N; k++) d[k] = fac * (re[k] + I * im[k]);
+        } else {
+            for (int k = 0; k < N; k++) {
+                int j = 0;
+                for (int q = 1; q < g->n_r; q++)
+                    if (fabs(g->r[q] - h->r[k]) < fabs(g->r[j] - h->r[k])) j = q;
+                if (j < 2) j = 2;
+                if (j == g->n_r - 1) j = g->n_r - 2;
+                d[k] = ck_cubic(g->r + j - 2, col + j - 2, h->r[k]);
+            }
+        }
+    }
+    free(col);
+    free(re);
+    free(im);
+}
+
+static int ck_read(FILE *f, void *p, size_t n) { return fread(p, 1, n, f) == n ? 0 : -1; }
+
+/* read_checkpoint (checkpoints.f90:185-707) into this run; or_start then
+ * does the restart branch of get_start_fields */
+int or_read_checkpoint(or_handle *h, const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    int rc = -2, version = 0, l_coll_old = 1, nexp_old = 2, nimp_old = 2, nold_old = 2, fam_dirk = 0;
+    int itmp[3], lh = 1, lc = 0, lp = 0;
+    double time = 0, dt_old[16] = {0}, par[7] = {0};
+    char fam[11] = {0}, ver[73] = {0};
+    ck_grid g = {0};
+    cplx *buf = NULL, *tmp = NULL;
+    if (ck_read(f, &version, 4)) goto out;
+    if (version == 1) {
+        double d2[3];
+        if (ck_read(f, d2, 24)) goto out;
+        time = d2[0]; dt_old[1] = d2[1]; dt_old[0] = d2[2];
+        nimp_old = nold_old = nexp_old = 2;
+    } else if (version == 2) {
+        if (ck_read(f, &time, 8) || ck_read(f, itmp, 8)) goto out;
+        nexp_old = itmp[0]; nold_old = itmp[1]; nimp_old = nold_old;
+        if (nexp_old > 16 || ck_read(f, dt_old, 8 * (size_t)nexp_old)) goto out;
+    } else if (version == 3) {
+        if (ck_read(f, &time, 8) || ck_read(f, &l_coll_old, 4) || ck_read(f, itmp, 8)) goto out;
+        nexp_old = itmp[0]; nold_old = itmp[1]; nimp_old = nold_old;
+        if (nexp_old > 16 || ck_read(f, dt_old, 8 * (size_t)nexp_old)) goto out;
+    } else if (version == 4) {
+        if (ck_read(f, &time, 8) || ck_read(f, &l_coll_old, 4) || ck_read(f, itmp, 12)) goto out;
+        nexp_old = itmp[0]; nimp_old = itmp[1]; nold_old = itmp[2];
+        if (nexp_old > 16 || ck_read(f, dt_old, 8 * (size_t)nexp_old)) goto out;
+    } else if (version >= 5 && version <= 7) {
+        if (ck_read(f, &time, 8) || ck_read(f, &l_coll_old, 4) || ck_read(f, fam, 10) || ck_read(f, itmp, 12)) goto out;
+        nexp_old = itmp[0]; nimp_old = itmp[1]; nold_old = itmp[2];
+        fam_dirk = strncmp(fam, "DIRK", 4) == 0;
+        if (fam_dirk) { if (ck_read(f, dt_old, 8)) goto out; }
+        else if (nexp_old > 16 || ck_read(f, dt_old, 8 * (size_t)nexp_old)) goto out;
+    } else {
+        goto out;
+    }
+    /* the dt array of the new scheme (checkpoints.f90:278-282, 357-366) */
+    {
+        tscheme *t = &h->ts;
+        int nnew = t->family == 0 ? t->nexp : 1;
+        int nold_arr = fam_dirk ? (nnew > 1 ? nnew : 1) : (nexp_old > nnew ? nexp_old : nnew);
+        if (fam_dirk) for (int o = 1; o < nold_arr; o++) dt_old[o] = dt_old[0];
+        else for (int o = nexp_old - 1; o < nnew; o++) dt_old[o] = dt_old[nexp_old - 1];
+        for (int o = 0; o < 8; o++) h->rst_dt[o] = o < nold_arr ? dt_old[o] : dt_old[nold_arr - 1];
+        if (t->family == 0) for (int o = nnew; o < 8; o++) h->rst_dt[o] = h->rst_dt[nnew - 1];
+    }
+    if (ck_read(f, par, version >= 7 ? 56 : 48)) goto out;
+    if (version < 7) { par[6] = par[5]; par[5] = 0.0; } /* no stef */
+    if (ck_read(f, itmp, 12)) goto out;
+    g.n_r = itmp[0]; g.m_max = itmp[1]; g.minc = itmp[2];
+    if (g.n_r < 4 || g.n_r > 100000 || g.m_max < 0 || g.minc < 1) goto out;
+    g.n_m = g.m_max / g.minc + 1;
+    {
+        int nn[2];
+        double al[2];
+        if (ck_read(f, ver, 72) || ck_read(f, nn, 8) || ck_read(f, al, 16)) goto out;
+        g.cheb = strncmp(ver, "cheb", 4) == 0;
+        g.order_boundary = nn[1];
+    }
+    g.r = malloc(sizeof(double) * g.n_r);
+    if (ck_read(f, g.r, 8 * (size_t)g.n_r) || ck_read(f, &lh, 4) || ck_read(f, &lc, 4)) goto out;
+    if (version >= 7 && ck_read(f, &lp, 4)) goto out;
+    if (version <= 5) { nimp_old--; nold_old--; }
+    {
+        /* restart flags (checkpoints.f90:398-434) */
+        tscheme *t = &h->ts;
+        double ek_old = par[4];
+        h->l_ab1 = ek_old != h->c.ek;
+        if (t->family == 0) {
+            if (fam_dirk) h->l_bridge = 1;
+            else h->l_bridge = (t->nold > nold_old || t->nimp > nimp_old);
+            if ((l_coll_old != 0) != (h->c.l_cheb_coll != 0)) { h->l_ab1 = 1; h->l_bridge = 1; }
+            if (version < 4 && nold_old > 2) { h->l_ab1 = 1; h->l_bridge = 1; }
+        } else {
+            h->l_bridge = 0;
+        }
+    }
+    {
+        size_t Fo = (size_t)g.n_m * g.n_r, F = (size_t)h->nm * h->N;
+        buf = malloc(sizeof(cplx) * Fo);
+        tmp = malloc(sizeof(cplx) * F);
+        tscheme *t = &h->ts;
+#define CK_FIELD(dst, lBc, phys)                                               \
+        do {                                                                   \
+            if (ck_read(f, buf, sizeof(cplx) * Fo)) goto out;                  \
+            cplx *dd_ = (dst);                                                 \
+            ck_map(h, &g, buf, tmp, lBc, phys);                                \
+            if (dd_) memcpy(dd_, tmp, sizeof(cplx) * F);                       \
+        } while (0)
+#define CK_HIST(a)                                                                                 \
+        do {                                                                                       \
+            if (!fam_dirk) {                                                                       \
+                for (int o = 2; o <= nexp_old; o++) CK_FIELD(o <= t->nexp && t->family == 0 ? (a)->expl + (o - 1) * F : NULL, 1, l_coll_old); \
+                for (int o = 2; o <= nimp_old; o++) CK_FIELD(o <= t->nimp && t->family == 0 ? (a)->impl + (o - 1) * F : NULL, 1, l_coll_old); \
+                if (version > 3)                                                                   \
+                    for (int o = 2; o <= nold_old; o++) CK_FIELD(o <= t->nold && t->family == 0 ? (a)->old + (o - 1) * F : NULL, 0, l_coll_old); \
+            }                                                                                      \
+        } while (0)
+        CK_FIELD(h->us, 0, 1);
+        CK_FIELD(h->up, 0, 1);
+        CK_HIST(&h->dpsidt);
+        if (lh) { CK_FIELD(h->temp, 0, 1); CK_HIST(&h->dTdt); }
+        if (lc) { CK_FIELD(h->xi, 0, 1); CK_HIST(&h->dxidt); }
+#undef CK_HIST
+#undef CK_FIELD
+    }
+    h->rst_time = time;
+    h->restart = 1;
+    rc = 0;
+out:
+    fclose(f);
+    free(g.r);
+    free(buf);
+    free(tmp);
+    return rc;
+}
+
+/* write_checkpoint_mloc (checkpoints.f90:33-183), version 7 */
+int or_write_checkpoint(or_handle *h, const char *path)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    tscheme *t = &h->ts;
+    int N = h->N, i4;
+    size_t F = (size_t)h->nm * N;
+    i4 = 7; fwrite(&i4, 4, 1, f);
+    fwrite(&h->time, 8, 1, f);
+    i4 = h->c.l_cheb_coll ? 1 : 0; fwrite(&i4, 4, 1, f);
+    fwrite(t->family == 0 ? "MULTISTEP " : "DIRK      ", 1, 10, f);
+    int nen[3] = {t->nexp, t->nimp, t->nold};
+    fwrite(nen, 4, 3, f);
+    fwrite(t->dt, 8, t->family == 0 ? (size_t)t->nexp : 1, f);
+    double par[7] = {h->c.ra, h->c.pr, h->c.raxi, h->c.sc, h->c.ek, 0.0, h->c.radratio};
+    fwrite(par, 8, 7, f);
+    int dims[3] = {N, h->mmax, h->minc};
+    fwrite(dims, 4, 3, f);
+    char ver[72];
+    memset(ver, ' ', 72);
+    memcpy(ver, h->fd ? "fd" : "cheb", h->fd ? 2 : 4);
+    fwrite(ver, 1, 72, f);
+    int nn[2] = {h->fd ? N : h->ncheb, h->fd ? 2 : 0};
+    fwrite(nn, 4, 2, f);
+    double al[2] = {0.8, 0.0}; /* alph1, alph2 defaults (namelists.f90:556-557) */
+    fwrite(al, 8, 2, f);
+    fwrite(h->r, 8, N, f);
+    int fl[3] = {h->l_heat, h->l_chem, 0};
+    fwrite(fl, 4, 3, f);
+    cplx *out = malloc(sizeof(cplx) * F);
+#define CK_PUT(src)                                                                 \
+    do {                                                                            \
+        for (int i = 0; i < h->nm; i++)                                             \
+            for (int k = 0; k < N; k++) out[(size_t)k * h->nm + i] = (src)[(size_t)i * N + k]; \
+        fwrite(out, sizeof(cplx), F, f);                                            \
+    } while (0)
+#define CK_PUT_HIST(a)                                                              \
+    do {                                                                            \
+        if (t->family == 0) {                                                       \
+            for (int o = 1; o < t->nexp; o++) CK_PUT((a)->expl + o * F);            \
+            for (int o = 1; o < t->nimp; o++) CK_PUT((a)->impl + o * F);            \
+            for (int o = 1; o < t->nold; o++) CK_PUT((a)->old + o * F);             \
+        }                                                                           \
+    } while (0)
+    CK_PUT(h->us);
+    CK_PUT(h->up);
+    CK_PUT_HIST(&h->dpsidt);
+    if (h->l_heat) { CK_PUT(h->temp); CK_PUT_HIST(&h->dTdt); }
+    if (h->l_chem) { CK_PUT(h->xi); CK_PUT_HIST(&h->dxidt); }
+#undef CK_PUT_HIST
+#undef CK_PUT
+    free(out);
+    int bad = ferror(f);
+    fclose(f);
+    return bad ? -2 : 0;
 }
 
 int or_step(or_handle *h, int nsteps)

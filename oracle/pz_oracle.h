@@ -36,6 +36,12 @@ int or_ass_solve(or_handle *h, int mi, const double *omega, double *psi);
 /* -L psi = omega of the finite-difference assembly stage, psi = 0 at the walls */
 int or_fd_ell_solve(or_handle *h, int mi, const double *omega, double *psi);
 int or_radial(or_handle *h, double *out);
+/* checkpoint files of checkpoints.f90: the version-7 writer, and the
+ * reader (versions 1-7) that maps the stored grid, truncation and scheme
+ * onto this run; or_start afterwards performs the restart branch of
+ * get_start_fields */
+int or_write_checkpoint(or_handle *h, const char *path);
+int or_read_checkpoint(or_handle *h, const char *path);
 /* conducting profiles [tcond | dtcond | xicond | dxicond] (4 n_r values) */
 int or_cond(or_handle *h, double *out);
 /* planted mistake of the failure-demonstrating tests (0 = none) */

@@ -19,6 +19,7 @@
 #include "pz_fd.h"
 #include "pz_assembly.h"
 #include "pz_adjoint.h"
+#include "pz_checkpoint.h"
 
 using pzh::ld;
 
@@ -441,6 +442,14 @@ struct pz_handle {
     int inv_waves = 16;
     int mthreads = PZ_MPHASE_THREADS; // threads of k_mphase / k_mcoll
     PzDev *ddev = nullptr;
+    // restart state of pz_read_checkpoint, consumed by the next pz_start
+    struct {
+        bool pending = false;
+        double time = 0.0, dt[8] = {};
+        int ab1 = 0, bridge = 0;
+    } rst;
+    bool restarted = false; // the run started from a checkpoint
+    int bridge_default = 0; // l_bridge_step of a fresh start
     std::string err;
 
     template <class T> T *dalloc(size_t n)
@@ -841,6 +850,7 @@ static void setup(pz_handle *h)
     // a fresh start sets l_bridge_step (namelists.f90:364-365): the first
     // nold-1 steps of a multistep scheme other than CNAB2 are CNAB2 steps
     d.bridge = h->sch.family == 0 && h->sch.id != PZ_CNAB2;
+    h->bridge_default = d.bridge;
     d.nstages = h->sch.nstages;
     d.nexp = h->sch.nexp;
     d.nimp = h->sch.nimp;
@@ -1282,6 +1292,251 @@ extern "C" int pz_get_field(pz_handle *h, int f, double *host)
 }
 
 // Gaspari-Cohn pieces (useful.f90:295-341)
+// ---------------------------------------------------------------------------
+// Checkpoint and frame files (checkpoints.f90, output_frames.f90)
+// ---------------------------------------------------------------------------
+static std::vector<double2> dev_to_host(pz_handle *h, const cd *p, size_t n)
+{
+    std::vector<double2> v(n);
+    HIPCHK(hipMemcpyAsync(v.data(), p, n * sizeof(cd), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return v;
+}
+
+// device [m][r] -> stored f(n_m, n_r), m fastest
+static void put_field(pzck::Writer &w, const pz_handle *h, const std::vector<double2> &v)
+{
+    std::vector<double2> o(v.size());
+    for (int i = 0; i < h->nm; i++)
+        for (int k = 0; k < h->N; k++) o[(size_t)k * h->nm + i] = v[(size_t)i * h->N + k];
+    w.raw(o.data(), o.size() * sizeof(double2));
+}
+
+extern "C" int pz_write_checkpoint(pz_handle *h, const char *path)
+{
+    if (!h || !path) return -1;
+    if (!h->started) return fail(h, -5, "pz_start has not been called");
+    try {
+        PzScheme s;
+        HIPCHK(hipMemcpyAsync(&s, h->d.ss, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const Scheme &sc = h->sch;
+        const size_t F = (size_t)h->nm * h->N;
+        const bool fd = h->d.fd;
+        pzck::Writer w(path);
+        // header (checkpoints.f90:99-134)
+        w.put<int>(7);
+        w.put<double>(s.time);
+        w.put<int>(h->c.l_cheb_coll ? 1 : 0);
+        w.text(sc.family == 0 ? "MULTISTEP" : "DIRK", 10);
+        w.put<int>(sc.nexp);
+        w.put<int>(sc.nimp);
+        w.put<int>(sc.nold);
+        for (int i = 0; i < (sc.family == 0 ? sc.nexp : 1); i++) w.put<double>(s.dt[i]);
+        for (double v : {h->c.ra, h->c.pr, h->c.raxi, h->c.sc, h->c.ek, 0.0 /* stef */, h->c.radratio}) w.put<double>(v);
+        w.put<int>(h->N);
+        w.put<int>(h->mmax);
+        w.put<int>(h->minc);
+        w.text(fd ? "fd" : "cheb", 72);
+        w.put<int>(fd ? h->N : h->ncheb); // rscheme%n_max
+        w.put<int>(fd ? 2 : 0);           // rscheme%order_boundary (no mapping)
+        w.put<double>(0.8);               // alph1, alph2: namelist defaults (namelists.f90:556-557)
+        w.put<double>(0.0);
+        w.raw(h->r.data(), sizeof(double) * h->N);
+        w.put<int>(h->l_heat ? 1 : 0);
+        w.put<int>(h->l_chem ? 1 : 0);
+        w.put<int>(0); // no phase field
+        // fields and their history (write_one_field_mpi, checkpoints.f90:868-924)
+        auto history = [&](cd *old, cd *impl, cd *expl) {
+            if (sc.family != 0) return;
+            for (int o = 1; o < sc.nexp; o++) put_field(w, h, dev_to_host(h, expl + o * F, F));
+            for (int o = 1; o < sc.nimp; o++) put_field(w, h, dev_to_host(h, impl + o * F, F));
+            for (int o = 1; o < sc.nold; o++) put_field(w, h, dev_to_host(h, old + o * F, F));
+        };
+        put_field(w, h, dev_to_host(h, h->d.us, F));
+        put_field(w, h, dev_to_host(h, h->d.up, F));
+        history(h->d.P_old, h->d.P_impl, h->d.P_expl);
+        if (h->l_heat) {
+            put_field(w, h, dev_to_host(h, h->d.temp, F));
+            history(h->d.T_old, h->d.T_impl, h->d.T_expl);
+        }
+        if (h->l_chem) {
+            put_field(w, h, dev_to_host(h, h->d.xi, F));
+            history(h->d.X_old, h->d.X_impl, h->d.X_expl);
+        }
+        w.close();
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_read_checkpoint(pz_handle *h, const char *path)
+{
+    if (!h || !path) return -1;
+    try {
+        pzck::Reader in(path);
+        const pzck::Header hd = pzck::read_header(in);
+        const Scheme &sc = h->sch;
+        const size_t F = (size_t)h->nm * h->N, Fo = (size_t)hd.n_m() * hd.n_r;
+        // the dt array of this run's scheme (checkpoints.f90:278-282, 357-366)
+        const int nnew = sc.family == 0 ? sc.nexp : 1;
+        std::vector<double> dto;
+        if (hd.dirk()) dto.assign(std::max(1, nnew), hd.dt[0]);
+        else {
+            dto.assign(std::max((int)hd.dt.size(), nnew), 0.0);
+            for (size_t i = 0; i < hd.dt.size(); i++) dto[i] = hd.dt[i];
+            for (int o = (int)hd.dt.size() - 1; o < nnew; o++) dto[o] = hd.dt.back();
+        }
+        for (int o = 0; o < 8; o++) h->rst.dt[o] = o < (int)dto.size() ? dto[o] : dto.back();
+        if (sc.family == 0)
+            for (int o = nnew; o < 8; o++) h->rst.dt[o] = h->rst.dt[nnew - 1];
+        // restart flags (checkpoints.f90:398-434)
+        int nimp_old = hd.nimp, nold_old = hd.nold;
+        if (hd.version <= 5) { nimp_old--; nold_old--; }
+        h->rst.ab1 = hd.ek != h->c.ek;
+        if (sc.family == 0) {
+            h->rst.bridge = hd.dirk() ? 1 : (sc.nold > nold_old || sc.nimp > nimp_old);
+            if (hd.l_coll != (h->c.l_cheb_coll != 0)) h->rst.ab1 = h->rst.bridge = 1;
+            if (hd.version < 4 && nold_old > 2) h->rst.ab1 = h->rst.bridge = 1;
+        } else {
+            h->rst.bridge = 0;
+        }
+        pzck::Target t;
+        t.N = h->N;
+        t.nm = h->nm;
+        t.idx2m = h->idx2m.data();
+        t.r = h->r.data();
+        t.cheb = !h->d.fd;
+        t.order_boundary = h->d.fd ? 2 : 0;
+        t.rnorm = h->d.fd ? 1.0 : h->rnorm;
+        std::vector<double2> raw(Fo), mapped;
+        auto next = [&](cd *dst, bool lBc, bool phys) {
+            in.raw(raw.data(), Fo * sizeof(double2));
+            if (!dst) return;
+            pzck::map_field(hd, raw, t, lBc, phys, mapped);
+            HIPCHK(hipMemcpyAsync(dst, mapped.data(), F * sizeof(cd), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        };
+        auto history = [&](cd *old, cd *impl, cd *expl) {
+            if (hd.dirk()) return;
+            const bool ms = sc.family == 0;
+            for (int o = 2; o <= hd.nexp; o++) next(ms && o <= sc.nexp ? expl + (o - 1) * F : nullptr, true, hd.l_coll);
+            for (int o = 2; o <= nimp_old; o++) next(ms && o <= sc.nimp ? impl + (o - 1) * F : nullptr, true, hd.l_coll);
+            if (hd.version > 3)
+                for (int o = 2; o <= nold_old; o++) next(ms && o <= sc.nold ? old + (o - 1) * F : nullptr, false, hd.l_coll);
+        };
+        next(h->d.us, false, true);
+        next(h->d.up, false, true);
+        history(h->d.P_old, h->d.P_impl, h->d.P_expl);
+        if (hd.l_heat) {
+            next(h->d.temp, false, true);
+            history(h->d.T_old, h->d.T_impl, h->d.T_expl);
+        }
+        if (hd.l_chem) {
+            next(h->d.xi, false, true);
+            history(h->d.X_old, h->d.X_impl, h->d.X_expl);
+        }
+        if (!h->d.coll && !h->d.fd) {
+            // the integration method rebuilds psi(m /= 0) from the stored u_s
+            // (init_fields.f90:67-95); the finite-difference start does the
+            // same in its start kernel, collocation does not need psi
+            std::vector<double2> us = dev_to_host(h, h->d.us, F), psi(F, double2{0.0, 0.0});
+            const double rc2 = h->r_cmb * h->r_cmb;
+            for (int i = 0; i < h->nm; i++) {
+                const int m = h->idx2m[i];
+                if (m == 0) continue;
+                for (int k = 1; k < h->N; k++) {
+                    const double f = h->r[k] / (double)m / (rc2 - h->r[k] * h->r[k]);
+                    const double2 u = us[(size_t)i * h->N + k];
+                    psi[(size_t)i * h->N + k] = double2{f * u.y, -f * u.x}; // -i f u
+                }
+            }
+            HIPCHK(hipMemcpyAsync(h->d.psi, psi.data(), F * sizeof(cd), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+        h->rst.time = hd.time;
+        h->rst.pending = true;
+        h->started = false;
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_read_checkpoint_header(const char *path, pz_checkpoint_info *info, double *r, int r_len)
+{
+    if (!path || !info) return -1;
+    try {
+        pzck::Reader in(path);
+        const pzck::Header hd = pzck::read_header(in);
+        std::memset(info, 0, sizeof(*info));
+        info->version = hd.version;
+        info->time = hd.time;
+        info->l_cheb_coll = hd.l_coll;
+        std::snprintf(info->family, sizeof(info->family), "%s", hd.family.substr(0, hd.family.find(' ')).c_str());
+        info->nexp = hd.nexp;
+        info->nimp = hd.nimp;
+        info->nold = hd.nold;
+        info->n_dt = (int)std::min<size_t>(hd.dt.size(), 16);
+        for (int i = 0; i < info->n_dt; i++) info->dt[i] = hd.dt[i];
+        info->ra = hd.ra;
+        info->pr = hd.pr;
+        info->raxi = hd.raxi;
+        info->sc = hd.sc;
+        info->ek = hd.ek;
+        info->stef = hd.stef;
+        info->radratio = hd.radratio;
+        info->n_r_max = hd.n_r;
+        info->m_max = hd.m_max;
+        info->minc = hd.minc;
+        std::snprintf(info->rscheme, sizeof(info->rscheme), "%s", hd.rscheme.substr(0, hd.rscheme.find(' ')).c_str());
+        info->n_max = hd.n_max;
+        info->order_boundary = hd.order_boundary;
+        info->alph1 = hd.alph1;
+        info->alph2 = hd.alph2;
+        info->l_heat = hd.l_heat;
+        info->l_chem = hd.l_chem;
+        info->l_phase_field = hd.l_phase;
+        info->header_bytes = hd.header_bytes;
+        info->n_arrays = hd.n_arrays();
+        if (r) for (int i = 0; i < std::min(r_len, hd.n_r); i++) r[i] = hd.r[i];
+    } catch (std::exception &) {
+        return -4;
+    }
+    return 0;
+}
+
+// write_snapshot_mloc (output_frames.f90:96-167), version 2
+extern "C" int pz_write_frame(pz_handle *h, int field, const char *path)
+{
+    if (!h || !path) return -1;
+    cd *p = field_ptr(h, field);
+    if (!p) return fail(h, -1, "unknown field");
+    try {
+        PzScheme s;
+        HIPCHK(hipMemcpyAsync(&s, h->d.ss, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+        std::vector<double2> v = dev_to_host(h, p, (size_t)h->nm * h->N);
+        std::vector<double> tc(h->N), xc(h->N);
+        HIPCHK(hipMemcpy(tc.data(), h->d.tcond, sizeof(double) * h->N, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(xc.data(), h->d.xicond, sizeof(double) * h->N, hipMemcpyDeviceToHost));
+        pzck::Writer w(path);
+        w.put<int>(2);
+        for (double x : {h->started ? s.time : 0.0, h->c.ra, h->c.ek, h->c.pr, h->c.radratio, h->c.sc, h->c.raxi})
+            w.put<double>(x);
+        for (int x : {h->N, h->nm, h->mmax, h->minc, h->nphi}) w.put<int>(x);
+        w.raw(h->r.data(), sizeof(double) * h->N);
+        w.raw(tc.data(), sizeof(double) * h->N);
+        w.raw(xc.data(), sizeof(double) * h->N);
+        w.raw(h->idx2m.data(), sizeof(int) * h->nm);
+        put_field(w, h, v);
+        w.close();
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
 static double gc_center(double c, double L)
 {
     const double pi = 4.0 * std::atan(1.0);
@@ -1410,16 +1665,30 @@ extern "C" int pz_start(pz_handle *h)
     try {
         PzScheme s;
         std::memset(&s, 0, sizeof(s));
-        for (int i = 0; i < 8; i++) s.dt[i] = h->c.dtMax; // init_fields.f90:111
-        s.lMatNext = 1;                                     // step_time.f90:115
+        const bool rst = h->rst.pending;
+        // a fresh start: dt(:) = dtMax (init_fields.f90:111); a restart: the
+        // time and dt array of the checkpoint (checkpoints.f90:357-366)
+        for (int i = 0; i < 8; i++) s.dt[i] = rst ? h->rst.dt[i] : h->c.dtMax;
+        s.time = rst ? h->rst.time : 0.0;
+        s.ab1 = rst ? h->rst.ab1 : 0;
+        s.lMatNext = 1; // step_time.f90:115
         HIPCHK(hipMemcpyAsync(h->d.ss, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+        // l_bridge_step: always on a fresh start, as read_checkpoint decided on
+        // a restart (the kernels read it through the descriptor)
+        const int bridge = rst ? (h->bridge_default ? h->rst.bridge : 0) : h->bridge_default;
+        if (bridge != h->d.bridge) {
+            h->d.bridge = bridge;
+            HIPCHK(hipMemcpyAsync(h->ddev, &h->d, sizeof(PzDev), hipMemcpyHostToDevice, h->stream));
+        }
         const size_t F = (size_t)h->nm * h->N;
         const Scheme &sc = h->sch;
         cd *slots[] = {h->d.T_old, h->d.T_impl, h->d.T_expl, h->d.X_old, h->d.X_impl, h->d.X_expl,
                        h->d.P_old, h->d.P_impl, h->d.P_expl};
         size_t cnt[] = {(size_t)sc.nold, (size_t)sc.nimp, (size_t)sc.nexp, (size_t)sc.nold, (size_t)sc.nimp,
                         (size_t)sc.nexp, (size_t)sc.nold, (size_t)sc.nimp, (size_t)sc.nexp};
-        for (int i = 0; i < 9; i++) HIPCHK(hipMemsetAsync(slots[i], 0, cnt[i] * F * sizeof(cd), h->stream));
+        // a restart keeps the history slots 2.. that the checkpoint filled
+        for (int i = 0; i < 9; i++)
+            HIPCHK(hipMemsetAsync(slots[i], 0, (rst ? 1 : cnt[i]) * F * sizeof(cd), h->stream));
         hipLaunchKernelGGL(k_set_weights, dim3(1), dim3(1), 0, h->stream, h->d);
         // exactly one start kernel per radial method: k_start reads the banded
         // operators that only the integration method builds
@@ -1442,52 +1711,89 @@ extern "C" int pz_start(pz_handle *h)
         if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
         if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
         h->started = true;
+        h->restarted = rst;
+        h->rst.pending = false;
     } catch (std::exception &e) {
         return fail(h, -4, e.what());
     }
     return 0;
 }
 
-// the stage loop of time_loop (step_time.f90:215-348) for one step
-static void launch_step(pz_handle *h)
+// The pieces of one stage of time_loop (step_time.f90:215-370).
+// Explicit part: transposes + radial_loop (step_time.f90:222-251), when the
+// scheme evaluates the explicit terms at this stage.
+static void launch_explicit(pz_handle *h, int stage)
+{
+    if (h->sch.l_exp_calc[stage - 1])
+        hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N), dim3(256), h->lds_radial,
+                           h->stream, (const PzDev *)h->ddev, stage - 1, stage == 1 ? 1 : 0);
+}
+
+// Stage 1: dt_courant, the dt roll, set_weights and the rebuild flag
+// (step_time.f90:256-304, courant.f90:78-167)
+static void launch_courant(pz_handle *h)
+{
+    hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev);
+}
+
+__global__ void k_force_lmat(PzScheme *s)
+{
+    if (threadIdx.x == 0) s->lMat = 1;
+}
+
+// Stage 1: the matrix rebuild of the run's radial method; the kernels exit
+// at once unless the rebuild flag is set
+static void launch_builds(pz_handle *h)
+{
+    if (h->d.fd) {
+        hipLaunchKernelGGL(k_fd_build, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 0);
+    } else if (h->d.coll) {
+        hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), h->lds_gj, h->stream, (const PzDev *)h->ddev, 0);
+    } else {
+        hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, (const PzDev *)h->ddev, 0);
+        hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream,
+                           (const PzDev *)h->ddev, 0);
+    }
+}
+
+// Implicit part: finish_explicit_assembly (fused into the per-m kernel,
+// mloop.f90:169-229) and mloop (mloop.f90:36-103) of this stage; on the
+// last stage of a scheme with an assembly stage, assemble_stage instead
+// (step_time.f90:313, 353-368)
+static void launch_implicit(pz_handle *h, int stage)
 {
     const Scheme &s = h->sch;
-    for (int stage = 1; stage <= s.nstages; stage++) {
-        const int exp_calc = s.l_exp_calc[stage - 1];
-        if (exp_calc)
-            hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N), dim3(256), h->lds_radial,
-                               h->stream, (const PzDev *)h->ddev, stage - 1,
-                               stage == 1 ? 1 : 0);
-        if (stage == 1) {
-            hipLaunchKernelGGL(k_courant, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev);
-            if (h->d.fd) {
-                hipLaunchKernelGGL(k_fd_build, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 0);
-            } else if (h->d.coll) {
-                hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), h->lds_gj, h->stream, (const PzDev *)h->ddev,
-                                   0);
-            } else {
-                hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, (const PzDev *)h->ddev, 0);
-                hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream,
-                                   (const PzDev *)h->ddev, 0);
-            }
-        }
-        if (s.l_assembly && stage == s.nstages) break; // the last stage only evaluates the explicit terms
+    const int exp_calc = s.l_exp_calc[stage - 1];
+    if (s.l_assembly && stage == s.nstages) {
         if (h->d.fd)
-            hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev,
-                               stage, exp_calc);
-        else if (h->d.coll)
-            hipLaunchKernelGGL(h->mcoll_l2 ? k_mcoll_l2 : k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
-                               (const PzDev *)h->ddev, stage, exp_calc);
+            hipLaunchKernelGGL(k_assemble_fd, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev);
         else
-            hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
-                               (const PzDev *)h->ddev, stage, exp_calc);
+            hipLaunchKernelGGL(k_assemble_coll, dim3(h->nm), dim3(ASM_THREADS), asm_lds_bytes(h->N), h->stream,
+                               (const PzDev *)h->ddev);
+        return;
     }
-    // assemble_stage (step_time.f90:353-368)
-    if (s.l_assembly && h->d.fd)
-        hipLaunchKernelGGL(k_assemble_fd, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev);
-    else if (s.l_assembly)
-        hipLaunchKernelGGL(k_assemble_coll, dim3(h->nm), dim3(ASM_THREADS), asm_lds_bytes(h->N), h->stream,
-                           (const PzDev *)h->ddev);
+    if (h->d.fd)
+        hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, stage,
+                           exp_calc);
+    else if (h->d.coll)
+        hipLaunchKernelGGL(h->mcoll_l2 ? k_mcoll_l2 : k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
+                           (const PzDev *)h->ddev, stage, exp_calc);
+    else
+        hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream, (const PzDev *)h->ddev,
+                           stage, exp_calc);
+}
+
+// the stage loop of time_loop (step_time.f90:215-370) for one step
+static void launch_step(pz_handle *h)
+{
+    for (int stage = 1; stage <= h->sch.nstages; stage++) {
+        launch_explicit(h, stage);
+        if (stage == 1) {
+            launch_courant(h);
+            launch_builds(h);
+        }
+        launch_implicit(h, stage);
+    }
 }
 
 // nsteps steps through the captured step graph (captured on first use)
@@ -1512,6 +1818,67 @@ extern "C" int pz_step(pz_handle *h, int nsteps)
     if (!h->started) return fail(h, -5, "pz_start has not been called");
     try {
         run_steps(h, nsteps);
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+// ---- the stage-granular boundary: time_loop's procedures one by one ----
+static int stage_check(pz_handle *h, int stage)
+{
+    if (!h) return -1;
+    if (!h->started) return fail(h, -5, "pz_start has not been called");
+    if (stage < 1 || stage > h->sch.nstages) return fail(h, -1, "stage out of range");
+    return 0;
+}
+
+extern "C" int pz_nstages(const pz_handle *h) { return h ? h->sch.nstages : -1; }
+
+extern "C" int pz_stage_explicit(pz_handle *h, int stage)
+{
+    int rc = stage_check(h, stage);
+    if (rc) return rc;
+    try {
+        launch_explicit(h, stage);
+        HIPCHK(hipGetLastError());
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_courant(pz_handle *h, double *dtr, double *dth)
+{
+    int rc = stage_check(h, 1);
+    if (rc) return rc;
+    try {
+        launch_courant(h);
+        HIPCHK(hipGetLastError());
+        if (dtr || dth) {
+            PzScheme s;
+            HIPCHK(hipMemcpyAsync(&s, h->d.ss, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (dtr) *dtr = s.dt_r;
+            if (dth) *dth = s.dt_h;
+        }
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_stage_implicit(pz_handle *h, int stage, int lMat)
+{
+    int rc = stage_check(h, stage);
+    if (rc) return rc;
+    try {
+        if (stage == 1) {
+            if (lMat) hipLaunchKernelGGL(k_force_lmat, dim3(1), dim3(64), 0, h->stream, h->d.ss);
+            launch_builds(h);
+        }
+        launch_implicit(h, stage);
+        HIPCHK(hipGetLastError());
     } catch (std::exception &e) {
         return fail(h, -4, e.what());
     }

@@ -720,6 +720,13 @@ __device__ __forceinline__ void courant_body(const PzDev &p)
             s->wexp[i] = i < 2 ? we[i] : 0.0;
         }
     }
+    if (s->ab1 && s->n_steps == 0 && p.family == 0) {
+        // start_with_ab1 (multistep_schemes.f90:526-534): the first step
+        // after a restart that changed ek or the radial method
+        s->wexp[0] = s->dt[0];
+        for (int i = 1; i < 8; i++) s->wexp[i] = 0.0;
+        s->ab1 = 0;
+    }
     int lMat = lMatNext;
     if (l_new_dt || lMat) lMat = 1;
     s->lMat = lMat;

@@ -31,7 +31,8 @@ struct PzScheme {
     int l_new_dt;
     int error;         // 1: dt fell below dtMin
     int n_steps;
-    int pad[3];
+    int ab1;  // the first step after a restart uses first-order Adams-Bashforth
+    int pad[2];
 };
 
 struct PzDev;

@@ -228,7 +228,6 @@ contains
    ! options the GPU path does not implement stop the run here, as the
    ! reference's abortRun would for an invalid combination
    subroutine check_supported()
-      if (l_start_file) call unsupported('l_start_file (checkpoint input)')
       if (hdif_temp /= 0.0_cp .or. hdif_vel /= 0.0_cp .or. hdif_comp /= 0.0_cp) &
          call unsupported('hyperdiffusion (&hdif)')
       if (l_temp_3D .or. l_xi_3D) call unsupported('l_temp_3D / l_xi_3D')

@@ -99,6 +99,47 @@ module pizza_amd_gpu
          integer(c_int), value :: nsteps
       end function pz_step
 
+      integer(c_int) function pz_nstages(h) bind(c, name='pz_nstages')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function pz_nstages
+
+      integer(c_int) function pz_stage_explicit(h, stage) bind(c, name='pz_stage_explicit')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: stage
+      end function pz_stage_explicit
+
+      integer(c_int) function pz_courant(h, dtr, dth) bind(c, name='pz_courant')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, dtr, dth
+      end function pz_courant
+
+      integer(c_int) function pz_stage_implicit(h, stage, lMat) bind(c, name='pz_stage_implicit')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: stage, lMat
+      end function pz_stage_implicit
+
+      integer(c_int) function pz_write_checkpoint(h, path) bind(c, name='pz_write_checkpoint')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: h
+         character(kind=c_char), intent(in) :: path(*)
+      end function pz_write_checkpoint
+
+      integer(c_int) function pz_read_checkpoint(h, path) bind(c, name='pz_read_checkpoint')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: h
+         character(kind=c_char), intent(in) :: path(*)
+      end function pz_read_checkpoint
+
+      integer(c_int) function pz_write_frame(h, field, path) bind(c, name='pz_write_frame')
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: h
+         integer(c_int), value :: field
+         character(kind=c_char), intent(in) :: path(*)
+      end function pz_write_frame
+
       integer(c_int) function pz_time(h, time, dt) bind(c, name='pz_time')
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: h

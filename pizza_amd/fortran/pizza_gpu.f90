@@ -2,32 +2,43 @@
 ! main program (pizza.f90:63-199) and time_loop (step_time.f90:96-405) for
 ! the path the library implements.  It reads the reference's input
 ! namelist, creates the run, applies the namelist's initial conditions
-! (init_fields.f90:260-509), steps through libpizza_amd and writes the
-! e_kin_3D.TAG time series in the reference's format (outputs.f90:686-687)
-! on every log step (l_correct_step, step_time.f90:145).
+! (init_fields.f90:260-509) or reads its start_file checkpoint
+! (read_checkpoint, checkpoints.f90:185-707), steps through libpizza_amd and
+! writes the e_kin_3D.TAG time series in the reference's format
+! (outputs.f90:686-687) on every log step, the frame files
+! frame_<field>_<n>.TAG on the frame steps (outputs.f90:293-312) and the
+! checkpoint_end.TAG file at the end of the run (step_time.f90:190-193,
+! checkpoints.f90:33-183), as time_loop does (l_correct_step,
+! step_time.f90:145-160).
 !
-! usage: pizza_gpu input.nml [-n n_time_steps] [--print-config]
+! usage: pizza_gpu input.nml [-n n_time_steps] [--print-config] [--stages]
+!
+! --stages steps through the reference's stage-loop procedures of module
+! step_stages (radial_loop, dt_courant, finish_explicit_assembly, mloop)
+! one step at a time instead of the captured pz_step.
 program pizza_gpu
    use iso_c_binding
    use pizza_amd_gpu
+   use step_stages
    use namelists
    implicit none
 
    character(len=512) :: fname, arg
    type(pz_config) :: cfg
    type(c_ptr) :: h
-   integer :: nargs, i, n_steps, n_done, chunk, u
-   logical :: print_only
+   integer :: nargs, i, n_steps, n_done, chunk, u, n_frame_out
+   logical :: print_only, by_stages
    real(c_double) :: time, dt, e(4)
 
    nargs = command_argument_count()
    if (nargs < 1) then
-      write(*, '(a)') 'usage: pizza_gpu input.nml [-n n_time_steps] [--print-config]'
+      write(*, '(a)') 'usage: pizza_gpu input.nml [-n n_time_steps] [--print-config] [--stages]'
       error stop 1
    end if
    call get_command_argument(1, fname)
    n_steps = -1
    print_only = .false.
+   by_stages = .false.
    i = 2
    do while (i <= nargs)
       call get_command_argument(i, arg)
@@ -37,6 +48,8 @@ program pizza_gpu
          read(arg, *) n_steps
       else if (trim(arg) == '--print-config') then
          print_only = .true.
+      else if (trim(arg) == '--stages') then
+         by_stages = .true.
       else
          write(*, '(a)') 'pizza_gpu: unknown argument '//trim(arg)
          error stop 1
@@ -59,25 +72,90 @@ contains
    subroutine run()
 
       call check(pz_create(cfg, h), 'pz_create')
+      ! get_start_fields (init_fields.f90:58-112): the checkpoint first, then
+      ! the namelist's perturbations on top of it
+      if (l_start_file) call check(pz_read_checkpoint(h, cstr(start_file)), 'pz_read_checkpoint')
       call check(pz_init_fields(h, int(init_t, c_int), real(amp_t, c_double), int(init_xi, c_int), &
                                 real(amp_xi, c_double), int(init_u, c_int), real(amp_u, c_double)), 'pz_init_fields')
       call check(pz_start(h), 'pz_start')
 
       open(newunit=u, file='e_kin_3D.'//trim(tag), status='replace', action='write')
       n_done = 0
+      n_frame_out = 0
       do
-         call check(pz_time(h, time, dt), 'pz_time')
-         call check(pz_ekin_3d(h, e), 'pz_ekin_3d')
-         write(u, '(1P, es20.12, 4es16.8)') time, e(1), e(2), e(3), e(4)
+         if (l_correct_step(n_done, n_steps, n_log_step, 0) .or. n_done == 0) then
+            call check(pz_time(h, time, dt), 'pz_time')
+            call check(pz_ekin_3d(h, e), 'pz_ekin_3d')
+            write(u, '(1P, es20.12, 4es16.8)') time, e(1), e(2), e(3), e(4)
+         end if
+         if (l_correct_step(n_done, n_steps, n_frame_step, n_frames)) call write_frames()
          if (n_done >= n_steps) exit
-         chunk = min(max(n_log_step, 1), n_steps - n_done)
-         call check(pz_step(h, int(chunk, c_int)), 'pz_step')
+         chunk = next_output(n_done) - n_done
+         if (by_stages) then
+            do i = 1, chunk
+               call time_loop_step(h)
+            end do
+         else
+            call check(pz_step(h, int(chunk, c_int)), 'pz_step')
+         end if
          n_done = n_done + chunk
       end do
       close(u)
+      call check(pz_write_checkpoint(h, cstr('checkpoint_end.'//trim(tag))), 'pz_write_checkpoint')
       call check(pz_destroy(h), 'pz_destroy')
 
    end subroutine run
+
+   ! the next step count after n at which time_loop writes an output
+   integer function next_output(n)
+      integer, intent(in) :: n
+      integer :: k
+      do k = n + 1, n_steps
+         if (l_correct_step(k, n_steps, n_log_step, 0) .or. l_correct_step(k, n_steps, n_frame_step, n_frames)) exit
+      end do
+      next_output = min(k, n_steps)
+   end function next_output
+
+   ! frame files of outputs.f90:293-312
+   subroutine write_frames()
+      character(len=32) :: num
+      n_frame_out = n_frame_out + 1
+      write(num, '(i0)') n_frame_out
+      if (ra /= 0.0_c_double) call frame(PZ_TEMP, 'temp', num)
+      if (raxi /= 0.0_c_double) call frame(PZ_XI, 'xi', num)
+      call frame(PZ_US, 'us', num)
+      call frame(PZ_UP, 'up', num)
+      call frame(PZ_OM, 'om', num)
+   end subroutine write_frames
+
+   subroutine frame(field, name, num)
+      integer(c_int), intent(in) :: field
+      character(len=*), intent(in) :: name, num
+      call check(pz_write_frame(h, field, cstr('frame_'//name//'_'//trim(num)//'.'//trim(tag))), 'pz_write_frame')
+   end subroutine frame
+
+   ! l_correct_step (useful.f90:181-222) with the step offset of time_loop:
+   ! true after n of n_max steps when the action of n_step steps (or of
+   ! n_intervals evenly spaced ones) falls there, and at the last step
+   logical function l_correct_step(n, n_max, n_step, n_intervals)
+      integer, intent(in) :: n, n_max, n_step, n_intervals
+      integer :: n_offset, n_every
+      l_correct_step = .false.
+      if (n_intervals /= 0) then
+         n_every = max(n_max / n_intervals, 1)
+         n_offset = n_max - n_every * n_intervals
+         if (n > n_offset .and. mod(n - n_offset, n_every) == 0) l_correct_step = .true.
+      else if (n_step /= 0) then
+         if (n == n_max .or. mod(n, n_step) == 0) l_correct_step = .true.
+      end if
+   end function l_correct_step
+
+   ! a Fortran string as a NUL-terminated C string
+   function cstr(f) result(c)
+      character(len=*), intent(in) :: f
+      character(kind=c_char, len=:), allocatable :: c
+      c = trim(f)//c_null_char
+   end function cstr
 
    subroutine check(ierr, what)
       integer(c_int), intent(in) :: ierr

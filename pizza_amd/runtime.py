@@ -23,6 +23,20 @@ class PizzaError(RuntimeError):
     pass
 
 
+class PzCheckpointInfo(ctypes.Structure):
+    """struct pz_checkpoint_info (include/pizza_amd.h)."""
+    _fields_ = [("version", ctypes.c_int), ("time", ctypes.c_double), ("l_cheb_coll", ctypes.c_int),
+                ("family", ctypes.c_char * 11), ("nexp", ctypes.c_int), ("nimp", ctypes.c_int),
+                ("nold", ctypes.c_int), ("n_dt", ctypes.c_int), ("dt", ctypes.c_double * 16),
+                ("ra", ctypes.c_double), ("pr", ctypes.c_double), ("raxi", ctypes.c_double),
+                ("sc", ctypes.c_double), ("ek", ctypes.c_double), ("stef", ctypes.c_double),
+                ("radratio", ctypes.c_double), ("n_r_max", ctypes.c_int), ("m_max", ctypes.c_int),
+                ("minc", ctypes.c_int), ("rscheme", ctypes.c_char * 73), ("n_max", ctypes.c_int),
+                ("order_boundary", ctypes.c_int), ("alph1", ctypes.c_double), ("alph2", ctypes.c_double),
+                ("l_heat", ctypes.c_int), ("l_chem", ctypes.c_int), ("l_phase_field", ctypes.c_int),
+                ("header_bytes", ctypes.c_longlong), ("n_arrays", ctypes.c_int)]
+
+
 def lib():
     """Loads libpizza_amd.so; raises if it has not been built."""
     global _LIB
@@ -44,6 +58,10 @@ def lib():
         L.pz_start.argtypes = [vp]
         L.pz_step.argtypes = [vp, ip]
         L.pz_time.argtypes = [vp, dp, dp]
+        L.pz_nstages.argtypes = [vp]
+        L.pz_stage_explicit.argtypes = [vp, ip]
+        L.pz_courant.argtypes = [vp, dp, dp]
+        L.pz_stage_implicit.argtypes = [vp, ip, ip]
         L.pz_ekin_3d.argtypes = [vp, dp]
         L.pz_nusselt.argtypes = [vp, dp]
         L.pz_stream.argtypes = [vp]
@@ -53,6 +71,10 @@ def lib():
         L.pz_get_ddr.argtypes = [vp, dp, dp, dp, ip, ip]
         L.pz_fft.argtypes = [vp, dp, dp, ip]
         L.pz_ifft.argtypes = [vp, dp, dp, ip]
+        L.pz_write_checkpoint.argtypes = [vp, ctypes.c_char_p]
+        L.pz_read_checkpoint.argtypes = [vp, ctypes.c_char_p]
+        L.pz_read_checkpoint_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(PzCheckpointInfo), dp, ip]
+        L.pz_write_frame.argtypes = [vp, ip, ctypes.c_char_p]
         L.pz_time_kernel.argtypes = [vp, ip, ip, dp]
         L.pz_ensemble_create.argtypes = [ctypes.POINTER(vp), ip, ctypes.POINTER(vp)]
         L.pz_ensemble_step.argtypes = [vp, ip]
@@ -67,7 +89,9 @@ def lib():
 
 
 EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_dims",
-           "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step",
+           "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step", "pz_nstages",
+           "pz_stage_explicit", "pz_courant", "pz_stage_implicit", "pz_write_checkpoint",
+           "pz_read_checkpoint", "pz_read_checkpoint_header", "pz_write_frame",
            "pz_time", "pz_ekin_3d", "pz_nusselt", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
            "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
            "pz_ensemble_destroy", "pz_ensemble_last_error", "pz_ensemble_policy", "pz_adjoint_forward", "pz_adjoint_backward"]
@@ -77,6 +101,16 @@ OBJECTIVES = {"nutop": 0, "ekin": 1, "temp": 2}
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def checkpoint_header(path):
+    """(PzCheckpointInfo, stored radii) of a checkpoint file (host only)."""
+    info = PzCheckpointInfo()
+    r = np.zeros(100000)
+    rc = lib().pz_read_checkpoint_header(str(path).encode(), ctypes.byref(info), _dp(r), len(r))
+    if rc:
+        raise PizzaError(f"pz_read_checkpoint_header failed ({rc}) on {path}")
+    return info, r[:info.n_r_max].copy()
 
 
 class Run:
@@ -129,6 +163,45 @@ class Run:
 
     def step(self, n=1):
         self._check(lib().pz_step(self.h, n), "pz_step")
+
+    # time_loop's procedures one at a time (pz_step(h, 1) launch for launch)
+    def nstages(self):
+        return lib().pz_nstages(self.h)
+
+    def stage_explicit(self, stage):
+        self._check(lib().pz_stage_explicit(self.h, stage), "pz_stage_explicit")
+
+    def courant(self, want=False):
+        """Stage-1 dt_courant; returns (dt_r, dt_h) when want is true."""
+        if not want:
+            self._check(lib().pz_courant(self.h, None, None), "pz_courant")
+            return None
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().pz_courant(self.h, ctypes.byref(a), ctypes.byref(b)), "pz_courant")
+        return a.value, b.value
+
+    def stage_implicit(self, stage, lMat=0):
+        self._check(lib().pz_stage_implicit(self.h, stage, int(lMat)), "pz_stage_implicit")
+
+    def step_by_stages(self):
+        """One step through the stage-granular entry points."""
+        for s in range(1, self.nstages() + 1):
+            self.stage_explicit(s)
+            if s == 1:
+                self.courant()
+            self.stage_implicit(s)
+
+    def write_checkpoint(self, path):
+        """Version-7 checkpoint file (checkpoints.f90:33-183)."""
+        self._check(lib().pz_write_checkpoint(self.h, str(path).encode()), "pz_write_checkpoint")
+
+    def read_checkpoint(self, path):
+        """Reads a checkpoint into the run; start() must follow."""
+        self._check(lib().pz_read_checkpoint(self.h, str(path).encode()), "pz_read_checkpoint")
+
+    def write_frame(self, name, path):
+        """Frame file of one field (output_frames.f90:96-167)."""
+        self._check(lib().pz_write_frame(self.h, FIELDS[name], str(path).encode()), "pz_write_frame")
 
     def time(self):
         t, dt = ctypes.c_double(), ctypes.c_double()

@@ -1,0 +1,308 @@
+"""Checkpoint and frame files (checkpoints.f90, output_frames.f90).
+
+Pinned by the reference's own data:
+- the header of samples/PhaseField/checkpoint_end.ini, a version-7 file the
+  reference wrote, parses to the sample's namelist values and its size is
+  the header plus the arrays the header announces;
+- the reference's RotE1e3EkPump sample is a chain of four runs, each
+  restarting from the checkpoint the previous one wrote:
+  collocation n_r 32 CNAB2 -> finite differences n_r 64 CNAB2 ->
+  collocation n_r 33 BPR353 -> finite differences n_r 64 ARS343.  The
+  restarts map the fields between the Chebyshev and the stretched
+  finite-difference grids and carry the multistep history.  The e_kin_3D
+  rows of all four segments reproduce samples/RotE1e3EkPump/reference.out
+  at the reference test's rtol 1e-8, on the oracle and through the GPU
+  library and its Fortran driver;
+- the RotInteg sample's continue run restarts its integration run (n_r 49,
+  n_cheb 32, m_max 32, CNAB2) on n_r 97, n_cheb 64, m_max 64 with SBDF3:
+  the first row after the restart (the Chebyshev-to-Chebyshev map with more
+  modes) matches reference.out at rtol 1e-8.  That run solves psi by the
+  influence-matrix method, which this path replaces by the direct solve; the
+  later rows then differ by the two discretisations (measured 2e-6 to 4e-6).
+
+Round trips: a run restarted from its own checkpoint continues like the run
+that was not stopped.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, ROOT, rel
+
+SEGMENTS = [("rotekpump_input_start.nml", 1000), ("rotekpump_input_FD.nml", 500),
+            ("rotekpump_input_cheb.nml", 500), ("rotekpump_input_final.nml", 500)]
+
+
+def test_phasefield_checkpoint_header():
+    from pizza_amd.runtime import checkpoint_header
+    path = os.path.join(GOLDEN, "phasefield_checkpoint_end.ini")
+    info, r = checkpoint_header(path)
+    assert info.version == 7 and info.family == b"DIRK"
+    # BPR353 (dirk_schemes.f90:78-83): nexp = nimp = 4 slots, nold = 1
+    assert (info.nexp, info.nimp, info.nold, info.n_dt) == (4, 4, 1, 1)
+    assert info.dt[0] == 4.0e-6
+    # &grid / &phys_param of samples/PhaseField/input.nml
+    assert (info.n_r_max, info.m_max, info.minc) == (65, 96, 1)
+    assert (info.ra, info.pr, info.stef, info.radratio) == (1.0e6, 1.0, 1.0, 0.35)
+    assert info.rscheme == b"cheb" and info.n_max == 65 and info.order_boundary == 1  # l_newmap
+    assert (info.alph1, info.alph2) == (10.0, -0.037)
+    assert info.l_heat and not info.l_chem and info.l_phase_field and info.l_cheb_coll
+    # the time of the file is the first row of the sample's reference.out
+    assert abs(info.time - 5.723938498250e1) < 1e-10
+    r_cmb = 1.0 / (1.0 - 0.35)
+    assert abs(r[0] - r_cmb) < 1e-14 and abs(r[-1] - (r_cmb - 1.0)) < 1e-14 and np.all(np.diff(r) < 0)
+    # u_s, u_phi, T, phase: DIRK checkpoints carry no history
+    assert info.n_arrays == 4
+    n_m = info.m_max // info.minc + 1
+    assert os.path.getsize(path) == info.header_bytes + info.n_arrays * n_m * info.n_r_max * 16
+
+
+def _rows(run, n, nlog=100):
+    out = []
+    for k in range(n // nlog + 1):
+        if k:
+            run.step(nlog)
+        t, _ = run.time()
+        out.append(np.r_[t, run.ekin_3d()])
+    return np.array(out)
+
+
+def _chain(make, tmp_path):
+    from pizza_amd.config import config_from_namelist
+    ref = np.loadtxt(os.path.join(GOLDEN, "rotekpump_e_kin_3D.txt"))
+    row, ck, got = 0, None, []
+    for name, n in SEGMENTS:
+        cfg, start, _ = config_from_namelist(os.path.join(GOLDEN, name))
+        run = make(cfg)
+        if ck:
+            run.read_checkpoint(ck)
+        else:
+            run.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+        run.start()
+        rows = _rows(run, n)
+        got.append(rows)
+        ck = tmp_path / f"checkpoint_end.{name}"
+        run.write_checkpoint(ck)
+        row += len(rows)
+    got = np.concatenate(got)
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-20)
+
+
+def test_rotekpump_restart_chain_oracle(tmp_path):
+    from oracle.oracle import Oracle
+    _chain(Oracle, tmp_path)
+
+
+def test_rotinteg_continue_first_row_oracle(tmp_path):
+    from oracle.oracle import Oracle
+    from pizza_amd.config import config_from_namelist
+    ref = np.loadtxt(os.path.join(GOLDEN, "rotinteg_e_kin_3D.txt"))
+    cfg, start, out = config_from_namelist(os.path.join(GOLDEN, "rotinteg_input_start.nml"))
+    o = Oracle(cfg)
+    o.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
+    o.start()
+    o.step(4000)
+    o.write_checkpoint(tmp_path / "checkpoint_end.start")
+    cfg2, _, _ = config_from_namelist(os.path.join(GOLDEN, "rotinteg_input_continue.nml"))
+    assert cfg2.l_direct_solve == 0 and cfg2.n_r_max == 97 and cfg2.m_max == 64
+    cfg2.l_direct_solve = 1  # influence matrix -> direct solve
+    o2 = Oracle(cfg2)
+    o2.read_checkpoint(tmp_path / "checkpoint_end.start")
+    o2.start()
+    t, _ = o2.time()
+    np.testing.assert_allclose(np.r_[t, o2.ekin_3d()], ref[201], rtol=1e-8, atol=1e-20)
+    o2.step(20)
+    t, _ = o2.time()
+    np.testing.assert_allclose(np.r_[t, o2.ekin_3d()], ref[202], rtol=1e-5, atol=1e-20)
+
+
+def _roundtrip_cases():
+    from pizza_amd import config as C
+    from pizza_amd import workloads as W
+    c1 = W.c1_collocation()
+    c2 = W.small_bpr353()
+    c3 = W.small_bpr353(65, 32)
+    c3.l_finite_diff = 1
+    c3.time_scheme = C.SBDF3
+    c3.dtMax = 2.0e-6
+    c4 = W.small_bpr353()
+    c4.l_cheb_coll = 1
+    c4.time_scheme = C.SBDF2
+    # (name, config, seed amplitude, round-trip tolerance)
+    return [("collocation CNAB2", c1, 1e-3, 1e-10), ("integration BPR353", c2, 1e-4, None),
+            ("finite differences SBDF3", c3, 1e-4, 1e-6), ("collocation SBDF2", c4, 1e-4, 1e-6)]
+
+
+@pytest.mark.parametrize("k", [0, 2, 3])
+def test_restart_round_trip_oracle(k, tmp_path):
+    """Stopped after 6 steps, written, read into a new run and continued for
+    6 more: u_s, u_phi, T and xi stay those of the run that went on.  The
+    restart rebuilds omega from u_phi and u_s (init_fields.f90:196-205,
+    174-183) where the running step had it from its own solve: on the walls
+    the two differ outright (the wall rows of the omega-psi system hold the
+    boundary conditions, not omega), and in the interior by the
+    truncation error of differentiating the interpolant of u_phi: the solve
+    enforces omega = -Laplacian(psi) with beta terms at the collocation
+    points, while u_phi = -dpsi/dr - beta psi is not a polynomial of degree
+    n_r - 1.  At Ekman 1e-3 (the first case) that is 7e-12 of the fields
+    after the 6 further steps; at Ekman 1e-5 the interior omega of the
+    restart differs by 1.1e-7 and the fields by 2e-7 (collocation SBDF2),
+    1e-7 (finite differences SBDF3, two second-order stencils).  omega
+    itself is not compared.  The integration method is left out: its restart
+    rebuilds psi from u_s with psi = 0 on the outer wall
+    (init_fields.f90:74-95), where the stored streamfunction is not zero,
+    so a restarted run departs from the uninterrupted one by design of the
+    reference (the interior psi comes back to 2e-16)."""
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    name, cfg, amp, tol = _roundtrip_cases()[k]
+    a = Oracle(cfg)
+    W.seed_run(a, cfg, amp_t=amp)
+    a.start()
+    a.step(6)
+    a.write_checkpoint(tmp_path / "ck")
+    b = Oracle(cfg)
+    b.read_checkpoint(tmp_path / "ck")
+    b.start()
+    assert a.time() == b.time()
+    a.step(6)
+    b.step(6)
+    assert a.time() == b.time()
+    for f in ("us", "up", "temp", "xi"):
+        assert rel(b.get_field(f), a.get_field(f)) < tol, (name, f)
+
+
+def test_integration_restart_rebuilds_psi_with_zero_wall_value(tmp_path):
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    cfg = W.small_bpr353()
+    a = Oracle(cfg)
+    W.seed_run(a, cfg)
+    a.start()
+    a.step(6)
+    a.write_checkpoint(tmp_path / "ck")
+    b = Oracle(cfg)
+    b.read_checkpoint(tmp_path / "ck")
+    b.start()
+    pa, pb = a.get_field("psi"), b.get_field("psi")
+    assert np.abs(pa[:, 0]).max() > 1e-6 and np.abs(pb[:, 0]).max() == 0.0
+    assert rel(pb[:, 1:], pa[:, 1:]) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# GPU: the library's reader and writer
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_rotekpump_restart_chain_gpu(tmp_path):
+    from pizza_amd.runtime import Run
+    print("case: RotE1e3EkPump restart chain through the library", flush=True)
+    _chain(Run, tmp_path)
+
+
+@pytest.mark.gpu
+def test_rotekpump_sample_through_fortran_driver(tmp_path):
+    """The reference's unitTest of the sample, with the GPU driver in place of
+    pizza.exe: the four input files run one after the other in one
+    directory (each reads the previous checkpoint_end file), and the
+    concatenated e_kin_3D files match reference.out (rtol 1e-8)."""
+    import shutil
+    driver = os.path.join(ROOT, "pizza_amd", "pizza_gpu")
+    names = {"rotekpump_input_start.nml": "input.nml", "rotekpump_input_FD.nml": "input_FD.nml",
+             "rotekpump_input_cheb.nml": "input_cheb.nml", "rotekpump_input_final.nml": "input_final.nml"}
+    for src, dst in names.items():
+        shutil.copy(os.path.join(GOLDEN, src), tmp_path / dst)
+    for dst in names.values():
+        print(f"case: pizza_gpu {dst}", flush=True)
+        out = subprocess.run([driver, str(tmp_path / dst)], cwd=tmp_path, capture_output=True, text=True,
+                             timeout=300)
+        assert out.returncode == 0, out.stdout + out.stderr
+    got = np.concatenate([np.loadtxt(tmp_path / f"e_kin_3D.{t}") for t in ("start", "restart", "restart1", "final")])
+    ref = np.loadtxt(os.path.join(GOLDEN, "rotekpump_e_kin_3D.txt"))
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-20)
+    # n_frames = 1: one frame set at the end of each run
+    for t in ("start", "restart", "restart1", "final"):
+        for f in ("temp", "us", "up", "om"):
+            assert (tmp_path / f"frame_{f}_1.{t}").exists(), (f, t)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", range(4))
+def test_restart_round_trip_gpu_and_oracle(k, tmp_path):
+    """The library writes a checkpoint mid-run; the library and the oracle
+    both restart from it and continue: the GPU restart stays with the
+    uninterrupted GPU run (the tolerances of the oracle round trip), the two
+    restarts agree with each other (1e-9), and a checkpoint the oracle writes
+    reads back into the library to rounding."""
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Run
+    name, cfg, amp, tol = _roundtrip_cases()[k]
+    print(f"case: restart round trip {name}", flush=True)
+    a = Run(cfg)
+    W.seed_run(a, cfg, amp_t=amp)
+    a.start()
+    a.step(6)
+    a.write_checkpoint(tmp_path / "gpu.ck")
+    b, o = Run(cfg), Oracle(cfg)
+    for x in (b, o):
+        x.read_checkpoint(tmp_path / "gpu.ck")
+        x.start()
+    a.step(6)
+    b.step(6)
+    o.step(6)
+    assert a.time() == b.time() and abs(o.time()[0] - a.time()[0]) < 1e-15
+    for f in ("us", "up", "om", "temp", "xi"):
+        if tol is not None and f != "om":
+            assert rel(b.get_field(f), a.get_field(f)) < tol, (name, f)
+        assert rel(b.get_field(f), o.get_field(f)) < 1e-9, (name, f)
+    o.write_checkpoint(tmp_path / "oracle.ck")
+    c = Run(cfg)
+    c.read_checkpoint(tmp_path / "oracle.ck")
+    c.start()
+    for f in ("us", "up", "temp", "xi"):
+        assert rel(c.get_field(f), o.get_field(f)) < 1e-14, (name, f)
+
+
+def _read_frame(path):
+    """write_snapshot_mloc layout (output_frames.f90:96-167)."""
+    b = open(path, "rb").read()
+    o = 0
+
+    def take(fmt, n):
+        nonlocal o
+        a = np.frombuffer(b, dtype=fmt, count=n, offset=o)
+        o += a.nbytes
+        return a
+    version = take("<i4", 1)[0]
+    head = take("<f8", 7)
+    n_r, n_m, m_max, minc, n_phi = take("<i4", 5)
+    r = take("<f8", n_r)
+    tcond = take("<f8", n_r)
+    xicond = take("<f8", n_r)
+    idx2m = take("<i4", n_m)
+    f = take("<c16", n_m * n_r).reshape(n_r, n_m).T
+    assert o == len(b)
+    return version, head, (n_r, n_m, m_max, minc, n_phi), r, tcond, idx2m, f
+
+
+@pytest.mark.gpu
+def test_frame_file(tmp_path):
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Run
+    cfg = W.small_bpr353()
+    run = Run(cfg)
+    W.seed_run(run, cfg)
+    run.start()
+    run.step(3)
+    run.write_frame("temp", tmp_path / "frame_temp_1.t")
+    version, head, dims, r, tcond, idx2m, f = _read_frame(tmp_path / "frame_temp_1.t")
+    assert version == 2 and head[0] == run.time()[0] and head[1] == cfg.ra and head[2] == cfg.ek
+    assert dims == (run.n_r, run.n_m, cfg.m_max, cfg.minc, run.n_phi)
+    assert np.array_equal(idx2m, np.arange(run.n_m))
+    assert np.array_equal(f, run.get_field("temp"))
+    r0, _ = W.radial_grid(cfg)
+    assert np.allclose(r, r0, rtol=0, atol=1e-14)

@@ -88,11 +88,13 @@ def test_driver_reads_production_namelist(tmp_path):
 
 
 def test_driver_stops_on_unsupported_options(tmp_path):
-    path = tmp_path / "rst.nml"
-    path.write_text(open(NML).read().replace("l_start_file=.false.", "l_start_file=.true."))
+    path = tmp_path / "map.nml"
+    text = open(NML).read()
+    assert "l_newmap   =.false." in text
+    path.write_text(text.replace("l_newmap   =.false.", "l_newmap   =.true."))
     out = subprocess.run([_driver(), str(path)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert out.returncode == 2
-    assert "unsupported option: l_start_file" in out.stdout
+    assert "unsupported option: l_newmap" in out.stdout
 
 
 @pytest.mark.gpu
