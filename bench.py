@@ -12,8 +12,9 @@ Prints one JSON line: steps/s, the roofline figure of the dominant kernel
 stream), the ensemble throughput factor of the uneven 64-run collocation
 ensemble (batched launches versus the plain step run by run), the cost of
 the adjoint (forward and backward sweeps over the plain forward) and a CPU
-baseline of the oracle (single-thread C restatement) timed on a bounded
-sample of the same workload.
+baseline of the oracle (the C restatement, on all host cores and on one)
+timed on a bounded sample of the same workload, and the steps/s of the
+finite-difference workload (n_r 257, n_phi 2046, SBDF3).
 """
 import argparse
 import json
@@ -92,21 +93,45 @@ def pmc_sample():
     print(json.dumps({"pmc_sample": "k_mphase x5 (stage 1, C2)", "copy_bytes": COPY_BYTES, "copies": 3}))
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cfg, seconds=15.0):
-    """Oracle (single-thread C restatement) steps/s on a bounded sample."""
-    from oracle.oracle import Oracle
-    o = Oracle(cfg)
-    W.seed_run(o, cfg)
-    o.start()
-    o.step(1)  # includes the first matrix factorisation
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        o.step(1)
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the same workload (n_r=97, n_phi=384, BPR353) "
-                      f"after one untimed step, {dt:.1f} s on one host core"}
+    """Oracle (the C restatement of the reference's step, OpenMP over the
+    radii of the radial loop and the modes of the m-loop) steps/s on a
+    bounded sample of the same workload, on all the host threads OpenMP is
+    given (OMP_NUM_THREADS, else every core), and beside it on one thread."""
+    from oracle.oracle import Oracle, threads
+
+    def rate(nthreads, secs):
+        threads(nthreads)
+        o = Oracle(cfg)
+        W.seed_run(o, cfg)
+        o.start()
+        o.step(1)  # includes the first matrix factorisation
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < secs:
+            o.step(1)
+            n += 1
+        return n, time.perf_counter() - t0
+
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    n, dt = rate(all_threads, seconds)
+    used = threads()
+    n1, dt1 = rate(1, seconds / 2)
+    threads(all_threads)
+    return {"value": n / dt, "unit": "steps/s", "cores": used, "kind": "port",
+            "nproc": os.cpu_count(), "cpu": _cpu_model(),
+            "single_thread_value": n1 / dt1,
+            "sample": f"{n} steps of the same workload (n_r=97, n_phi=384, BPR353) after one untimed step, "
+                      f"{dt:.1f} s on {used} OpenMP threads; one thread: {n1} steps in {dt1:.1f} s"}
 
 
 def ensemble_bench(nruns=64, steps=100, warmup=3):
@@ -304,6 +329,7 @@ def main():
         line["ensemble"] = ensemble_bench(steps=args.ensemble_steps)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
+    line["fd"] = fd_bench()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(line))

@@ -41,6 +41,7 @@ def lib():
         L.or_fd_ell_solve.argtypes = [vp, ip, dp, dp]
         L.or_radial.argtypes = [vp, dp]
         L.or_cond.argtypes = [vp, dp]
+        L.or_threads.argtypes = [ip]
         L.or_read_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.or_write_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.or_dirk_tables.argtypes = [ip, ctypes.POINTER(ip), ctypes.POINTER(ip), dp, dp, dp, dp, dp]
@@ -242,6 +243,11 @@ class Oracle:
         rc = lib().or_stencil(self.h, which, m, idx, ku, _dp(out))
         self._check(rc, "stencil")
         return out
+
+def threads(n=0):
+    """OpenMP threads of the oracle's loops (n > 0 sets them); the count in use."""
+    return lib().or_threads(n)
+
 
 class planted:
     """Context manager that plants mistake k in the oracle (the or_plant

@@ -19,6 +19,9 @@ typedef double complex cplx;
 typedef long double ld;
 
 #define PI_F (4.0 * atan(1.0)) /* constants.f90:5 */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* Scratch buffers owned by the calling thread: the per-radius and per-m
  * loops of the step run on OpenMP threads, and every transform below takes
@@ -4478,6 +4481,19 @@ int or_fd_ell_solve(or_handle *h, int mi, const double *omega, double *psi)
     memcpy(psi, x + 1, sizeof(cplx) * N);
     free(x);
     return 0;
+}
+
+/* OpenMP threads of the oracle's loops: n > 0 sets them; returns the
+ * number in use (1 without OpenMP) */
+int or_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 int or_cond(or_handle *h, double *out)

@@ -44,6 +44,8 @@ int or_write_checkpoint(or_handle *h, const char *path);
 int or_read_checkpoint(or_handle *h, const char *path);
 /* conducting profiles [tcond | dtcond | xicond | dxicond] (4 n_r values) */
 int or_cond(or_handle *h, double *out);
+/* OpenMP threads of the oracle: n > 0 sets them; returns the count in use */
+int or_threads(int n);
 /* planted mistake of the failure-demonstrating tests (0 = none) */
 extern int or_plant;
 int or_dirk_tables(int scheme, int *nstages, int *l_assembly, double *wimp_lin0, double *bimp, double *bexp,

@@ -179,7 +179,8 @@ def read_namelist(path):
         for m in re.finditer(r"(\w+)\s*=\s*((?:\"[^\"]*\"|'[^']*'|[^=])*?)(?=,?\s*\w+\s*=|,?\s*$)",
                              body, re.S):
             raw = m.group(2).strip().rstrip(",")
-            parts = [p for p in re.split(r",\s*", raw) if p.strip()]
+            # list items are separated by commas or blanks (Fortran list-directed input)
+            parts = re.findall(r"\"[^\"]*\"|'[^']*'|[^,\s]+", raw)
             vals = [_parse_value(p) for p in parts]
             entries[m.group(1).lower()] = vals[0] if len(vals) == 1 else vals
         groups[g.group(1).lower()] = entries

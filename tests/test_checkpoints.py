@@ -4,15 +4,7 @@ Pinned by the reference's own data:
 - the header of samples/PhaseField/checkpoint_end.ini, a version-7 file the
   reference wrote, parses to the sample's namelist values and its size is
   the header plus the arrays the header announces;
-- the reference's RotE1e3EkPump sample is a chain of four runs, each
-  restarting from the checkpoint the previous one wrote:
-  collocation n_r 32 CNAB2 -> finite differences n_r 64 CNAB2 ->
-  collocation n_r 33 BPR353 -> finite differences n_r 64 ARS343.  The
-  restarts map the fields between the Chebyshev and the stretched
-  finite-difference grids and carry the multistep history.  The e_kin_3D
-  rows of all four segments reproduce samples/RotE1e3EkPump/reference.out
-  at the reference test's rtol 1e-8, on the oracle and through the GPU
-  library and its Fortran driver;
+- the restart chains of the reference's samples (tests/test_samples.py);
 - the RotInteg sample's continue run restarts its integration run (n_r 49,
   n_cheb 32, m_max 32, CNAB2) on n_r 97, n_cheb 64, m_max 64 with SBDF3:
   the first row after the restart (the Chebyshev-to-Chebyshev map with more
@@ -24,16 +16,11 @@ Round trips: a run restarted from its own checkpoint continues like the run
 that was not stopped.
 """
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
-from helpers import GOLDEN, ROOT, rel
-
-SEGMENTS = [("rotekpump_input_start.nml", 1000), ("rotekpump_input_FD.nml", 500),
-            ("rotekpump_input_cheb.nml", 500), ("rotekpump_input_final.nml", 500)]
-
+from helpers import GOLDEN, rel
 
 def test_phasefield_checkpoint_header():
     from pizza_amd.runtime import checkpoint_header
@@ -57,43 +44,6 @@ def test_phasefield_checkpoint_header():
     assert info.n_arrays == 4
     n_m = info.m_max // info.minc + 1
     assert os.path.getsize(path) == info.header_bytes + info.n_arrays * n_m * info.n_r_max * 16
-
-
-def _rows(run, n, nlog=100):
-    out = []
-    for k in range(n // nlog + 1):
-        if k:
-            run.step(nlog)
-        t, _ = run.time()
-        out.append(np.r_[t, run.ekin_3d()])
-    return np.array(out)
-
-
-def _chain(make, tmp_path):
-    from pizza_amd.config import config_from_namelist
-    ref = np.loadtxt(os.path.join(GOLDEN, "rotekpump_e_kin_3D.txt"))
-    row, ck, got = 0, None, []
-    for name, n in SEGMENTS:
-        cfg, start, _ = config_from_namelist(os.path.join(GOLDEN, name))
-        run = make(cfg)
-        if ck:
-            run.read_checkpoint(ck)
-        else:
-            run.init_fields(init_t=start["init_t"], amp_t=start["amp_t"])
-        run.start()
-        rows = _rows(run, n)
-        got.append(rows)
-        ck = tmp_path / f"checkpoint_end.{name}"
-        run.write_checkpoint(ck)
-        row += len(rows)
-    got = np.concatenate(got)
-    assert got.shape == ref.shape
-    np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-20)
-
-
-def test_rotekpump_restart_chain_oracle(tmp_path):
-    from oracle.oracle import Oracle
-    _chain(Oracle, tmp_path)
 
 
 def test_rotinteg_continue_first_row_oracle(tmp_path):
@@ -195,40 +145,6 @@ def test_integration_restart_rebuilds_psi_with_zero_wall_value(tmp_path):
 # ---------------------------------------------------------------------------
 # GPU: the library's reader and writer
 # ---------------------------------------------------------------------------
-@pytest.mark.gpu
-def test_rotekpump_restart_chain_gpu(tmp_path):
-    from pizza_amd.runtime import Run
-    print("case: RotE1e3EkPump restart chain through the library", flush=True)
-    _chain(Run, tmp_path)
-
-
-@pytest.mark.gpu
-def test_rotekpump_sample_through_fortran_driver(tmp_path):
-    """The reference's unitTest of the sample, with the GPU driver in place of
-    pizza.exe: the four input files run one after the other in one
-    directory (each reads the previous checkpoint_end file), and the
-    concatenated e_kin_3D files match reference.out (rtol 1e-8)."""
-    import shutil
-    driver = os.path.join(ROOT, "pizza_amd", "pizza_gpu")
-    names = {"rotekpump_input_start.nml": "input.nml", "rotekpump_input_FD.nml": "input_FD.nml",
-             "rotekpump_input_cheb.nml": "input_cheb.nml", "rotekpump_input_final.nml": "input_final.nml"}
-    for src, dst in names.items():
-        shutil.copy(os.path.join(GOLDEN, src), tmp_path / dst)
-    for dst in names.values():
-        print(f"case: pizza_gpu {dst}", flush=True)
-        out = subprocess.run([driver, str(tmp_path / dst)], cwd=tmp_path, capture_output=True, text=True,
-                             timeout=300)
-        assert out.returncode == 0, out.stdout + out.stderr
-    got = np.concatenate([np.loadtxt(tmp_path / f"e_kin_3D.{t}") for t in ("start", "restart", "restart1", "final")])
-    ref = np.loadtxt(os.path.join(GOLDEN, "rotekpump_e_kin_3D.txt"))
-    assert got.shape == ref.shape
-    np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-20)
-    # n_frames = 1: one frame set at the end of each run
-    for t in ("start", "restart", "restart1", "final"):
-        for f in ("temp", "us", "up", "om"):
-            assert (tmp_path / f"frame_{f}_1.{t}").exists(), (f, t)
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("k", range(4))
 def test_restart_round_trip_gpu_and_oracle(k, tmp_path):
