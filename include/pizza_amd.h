@@ -252,6 +252,43 @@ int pz_time(pz_handle *h, double *time, double *dt);
 int pz_ekin_3d(pz_handle *h, double out[4]);
 
 /*
+ * The log-step time series of get_time_series (outputs.f90:418-830), one
+ * row of each file of the reference, from the current state.  Synchronises.
+ *   ekin_2d     e_kin_2D: us2_2D, up2_2D, up2_axi_2D
+ *   ekin_3d     e_kin_3D: us2_3D, up2_3D, uz2_3D, up2_axi_3D
+ *   power_2d/3d power_2D/3D: buoyancy power, chemical power, viscous
+ *               dissipation, zonal and total Ekman-pumping dissipation
+ *   reynolds_2d/3d reynolds_2D/3D: Re, zonal Re, fluctuating Re (the
+ *               volume outside the tangent cylinder of the sphere container)
+ *   lengths     length_scales: dlus_peak, dlekin_peak, dlvort_peak,
+ *               dlekin_int (get_lengthscales, outputs.f90:897-962), dl_diss
+ *   heat        heat: NuTop, NuBot, Nu_vol, Nu_int, tTop, tBot, beta_t
+ *   chem        chem: ShTop, ShBot, Sh_vol, xiTop, xiBot, beta_xi
+ * The radial derivatives of T and xi are those of the run's method (get_dr
+ * on Chebyshev grids, the ghost-zone stencil of the finite-difference step).
+ */
+typedef struct pz_series {
+    double time;
+    double ekin_2d[3], ekin_3d[4];
+    double power_2d[5], power_3d[5];
+    double reynolds_2d[3], reynolds_3d[3];
+    double lengths[5];
+    double heat[7];
+    double chem[6];
+} pz_series;
+int pz_time_series(pz_handle *h, pz_series *out);
+
+/*
+ * The m-spectra of calculate_spectra (spectra.f90:104-197) at the current
+ * state: per m (n_m_max values each, any may be NULL) the radial integrals
+ * of pi r h times |u_s|^2, |u_phi|^2, |omega|^2, |T|^2 and the xi entry,
+ * which the reference takes from the temperature profile
+ * (spectra.f90:188; from |xi|^2 when there is no temperature).
+ * Synchronises.
+ */
+int pz_spectra(pz_handle *h, double *us2_m, double *up2_m, double *enst_m, double *temp2_m, double *xi2_m);
+
+/*
  * Classical Nusselt numbers of the heat time series (outputs.f90:729-755):
  * out[0] = NuTop, out[1] = NuBot, 1 + dT/dr of the m = 0 temperature
  * perturbation over dT_cond/dr at each wall.  Synchronises.

@@ -42,6 +42,7 @@ def lib():
         L.or_radial.argtypes = [vp, dp]
         L.or_cond.argtypes = [vp, dp]
         L.or_threads.argtypes = [ip]
+        L.or_time_series.argtypes = [vp, dp, dp]
         L.or_read_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.or_write_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.or_dirk_tables.argtypes = [ip, ctypes.POINTER(ip), ctypes.POINTER(ip), dp, dp, dp, dp, dp]
@@ -167,6 +168,12 @@ class Oracle:
         """Reads a checkpoint into this run (checkpoints.f90:185-707); call
         start() afterwards for the restart branch of get_start_fields."""
         self._check(lib().or_read_checkpoint(self.h, str(path).encode()), "read_checkpoint")
+
+    def time_series(self):
+        """(42 time-series values, 5 x n_m spectra) of get_time_series."""
+        out, spec = np.zeros(42), np.zeros(5 * self.n_m)
+        lib().or_time_series(self.h, _dp(out), _dp(spec))
+        return out, spec.reshape(5, self.n_m)
 
     def cond(self):
         """tcond, dtcond, xicond, dxicond on the grid."""

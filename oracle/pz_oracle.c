@@ -4560,6 +4560,190 @@ int or_ekin_3d(or_handle *h, double out[4])
     return 0;
 }
 
+/* simps (integration.f90:102-150) on abscissae x */
+static double simps_x(int n, const double *f, const double *x)
+{
+    double s = 0.0;
+#define SX(k)                                                                                                \
+    do {                                                                                                     \
+        double h2 = x[(k) + 1] - x[(k)], h1 = x[(k)] - x[(k) - 1];                                           \
+        s += (h1 + h2) / 6.0 *                                                                               \
+             (f[(k) - 1] * (2.0 * h1 - h2) / h1 + f[(k)] * (h1 + h2) * (h1 + h2) / (h1 * h2) + f[(k) + 1] * (2.0 * h2 - h1) / h2); \
+    } while (0)
+    if (n % 2 == 1) {
+        for (int k = 1; k < n - 1; k += 2) SX(k);
+        return -s;
+    }
+    s = 0.5 * (x[1] - x[0]) * (f[1] + f[0]);
+    for (int k = 2; k < n - 1; k += 2) SX(k);
+    s += 0.5 * (x[n - 1] - x[n - 2]) * (f[n - 1] + f[n - 2]);
+    for (int k = 1; k < n - 1; k += 2) SX(k);
+#undef SX
+    return -0.5 * s;
+}
+
+/* get_time_series (outputs.f90:418-830) with get_lengthscales (:897-962)
+ * and calculate_spectra (spectra.f90:104-197): out = [time, e_kin_2D(3),
+ * e_kin_3D(4), power_2D(5), power_3D(5), reynolds_2D(3), reynolds_3D(3),
+ * length_scales(5), heat(7), chem(6)] (42 values); spec = us2_m, up2_m,
+ * enst_m, temp2_m, xi2_m (5 n_m values, may be NULL) */
+int or_time_series(or_handle *h, double *out, double *spec)
+{
+    int N = h->N, nm = h->nm;
+    size_t F = (size_t)nm * N;
+    double pi = PI_F, tiny = 10.0 * 2.220446049250313e-16;
+    cplx *dT = calloc(F, sizeof(cplx)), *dX = calloc(F, sizeof(cplx));
+    /* radial derivatives as the reference takes them */
+    for (int i = 0; i < nm; i++) {
+        const cplx *t = h->temp + (size_t)i * N, *x = h->xi + (size_t)i * N;
+        if (h->fd) {
+            for (int f = 0; f < 2; f++) {
+                if ((f == 0 && !h->l_heat) || (f == 1 && !h->l_chem)) continue;
+                const cplx *c = f == 0 ? t : x;
+                int ktop = f == 0 ? h->c.ktopt : h->c.ktopxi, kbot = f == 0 ? h->c.kbott : h->c.kbotxi;
+                cplx *d = (f == 0 ? dT : dX) + (size_t)i * N;
+                for (int k = 0; k < N; k++) {
+                    cplx fm = k > 0 ? c[k - 1] : (ktop == 1 ? 2.0 * c[0] - c[1] : c[1]);
+                    cplx fp = k < N - 1 ? c[k + 1] : (kbot == 1 ? 2.0 * c[N - 1] - c[N - 2] : c[N - 2]);
+                    const double *a = h->fdr + 3 * k;
+                    d[k] = a[0] * fm + a[1] * c[k] + a[2] * fp;
+                }
+            }
+        } else {
+            if (h->l_heat) get_dr_vec(h, t, dT + (size_t)i * N, 1);
+            if (h->l_chem) get_dr_vec(h, x, dX + (size_t)i * N, 1);
+        }
+    }
+    double *P = calloc(9 * (size_t)N, sizeof(double));
+    double *us2 = P, *up2 = P + N, *upa = P + 2 * N, *en = P + 3 * N, *buo = P + 4 * N, *chp = P + 5 * N;
+    double *nuv = P + 6 * N, *shv = P + 7 * N, *flux = P + 8 * N;
+    for (int n = 0; n < N; n++) {
+        for (int i = 0; i < nm; i++) {
+            int m = h->idx2m[i];
+            size_t q = (size_t)i * N + n;
+            us2[n] += cc2real(h->us[q], m);
+            up2[n] += cc2real(h->up[q], m);
+            en[n] += cc2real(h->om[q], m);
+            if (h->l_heat) {
+                buo[n] += cc22real(h->us[q], h->temp[q], m);
+                nuv[n] += cc2real(dT[q], m) + (double)m * (double)m * h->or2[n] * cc2real(h->temp[q], m);
+                flux[n] += cc22real(h->us[q], h->temp[q], m);
+            }
+            if (h->l_chem) {
+                chp[n] += cc22real(h->us[q], h->xi[q], m);
+                shv[n] += cc2real(dX[q], m) + (double)m * (double)m * h->or2[n] * cc2real(h->xi[q], m);
+            }
+            if (m == 0) {
+                upa[n] += cc2real(h->up[q], m);
+                if (h->l_heat) flux[n] += -h->TdiffFac * creal(dT[q]) - h->TdiffFac * h->dtcond[n];
+            }
+        }
+        if (h->l_heat) nuv[n] *= h->r[n] * h->height[n];
+        if (h->l_chem) shv[n] *= h->r[n] * h->height[n];
+    }
+    double *tmp = dalloc(N);
+#define INTEG(expr) (({ for (int k = 0; k < N; k++) tmp[k] = (expr); }), rint_r(h, tmp))
+    const double *r = h->r, *hg = h->height;
+    double us2_2D = round_off(pi * INTEG(us2[k] * r[k])), up2_2D = round_off(pi * INTEG(up2[k] * r[k]));
+    double upa_2D = round_off(pi * INTEG(upa[k] * r[k]));
+    double us2_3D = round_off(pi * INTEG(us2[k] * r[k] * hg[k])), up2_3D = round_off(pi * INTEG(up2[k] * r[k] * hg[k]));
+    double uz2_3D = round_off(pi * INTEG(us2[k] * h->beta[k] * h->beta[k] * r[k] * pow(hg[k], 3) / 12.0));
+    double upa_3D = round_off(pi * INTEG(upa[k] * r[k] * hg[k]));
+    double surf = pi * (h->r_cmb * h->r_cmb - h->r_icb * h->r_icb);
+    double vol_otc = 4.0 / 3.0 * pi * pow(h->r_cmb * h->r_cmb - h->r_icb * h->r_icb, 1.5);
+    double visc_2D = round_off(2.0 * pi * INTEG(en[k] * r[k]));
+    if (h->c.kbotv == 1) visc_2D += 4.0 * pi * up2[N - 1];
+    if (h->c.ktopv == 1) visc_2D -= 4.0 * pi * up2[0];
+    double pow_2D = h->l_heat ? round_off(2.0 * pi * INTEG(h->BuoFac * buo[k] * h->rgrav[k] * r[k])) : 0.0;
+    double chem_2D = h->l_chem ? round_off(2.0 * pi * INTEG(h->ChemFac * chp[k] * h->rgrav[k] * r[k])) : 0.0;
+    double visc_3D = round_off(2.0 * pi * INTEG(en[k] * r[k] * hg[k]));
+    if (h->c.kbotv == 1) visc_3D -= 4.0 * h->or1[N - 1] * up2[N - 1];
+    if (h->c.ktopv == 1) visc_3D -= 4.0 * h->or1[0] * up2[0];
+    double pow_3D = h->l_heat ? round_off(2.0 * pi * INTEG(h->BuoFac * buo[k] * h->rgrav[k] * r[k] * hg[k])) : 0.0;
+    double chem_3D = h->l_chem ? round_off(2.0 * pi * INTEG(h->ChemFac * chp[k] * h->rgrav[k] * r[k] * hg[k])) : 0.0;
+    double ptot_2D = round_off(2.0 * pi * INTEG(h->CorFac * (us2[k] + up2[k]) * h->ekpump[k] * r[k]));
+    double pzon_2D = round_off(2.0 * pi * INTEG(h->CorFac * upa[k] * h->ekpump[k] * r[k]));
+    double ptot_3D = round_off(2.0 * pi * INTEG(h->CorFac * (us2[k] + up2[k]) * h->ekpump[k] * hg[k] * r[k]));
+    double pzon_3D = round_off(2.0 * pi * INTEG(h->CorFac * upa[k] * h->ekpump[k] * hg[k] * r[k]));
+    /* m-spectra */
+    double *S = calloc(5 * (size_t)nm, sizeof(double));
+    for (int i = 0; i < nm; i++) {
+        int m = h->idx2m[i];
+        for (int j = 0; j < 5; j++) {
+            for (int k = 0; k < N; k++) {
+                size_t q = (size_t)i * N + k;
+                cplx v = j == 0 ? h->us[q] : j == 1 ? h->up[q] : j == 2 ? h->om[q] : j == 3 ? (h->l_heat ? h->temp[q] : 0) : (h->l_chem ? h->xi[q] : 0);
+                tmp[k] = pi * cc2real(v, m) * r[k] * hg[k];
+            }
+            S[(size_t)j * nm + i] = rint_r(h, tmp);
+        }
+        if (h->l_heat) S[4 * (size_t)nm + i] = S[3 * (size_t)nm + i]; /* spectra.f90:188 */
+    }
+    double L[5] = {0};
+    {
+        int bu = -1, be = -1, bv = -1;
+        for (int i = 0; i < nm; i++) {
+            if (h->idx2m[i] == 0) continue;
+            if (bu < 0 || S[i] > S[bu]) bu = i;
+            if (be < 0 || S[i] + S[nm + i] > S[be] + S[nm + be]) be = i;
+            if (bv < 0 || S[2 * nm + i] > S[2 * nm + bv]) bv = i;
+        }
+        int lu = bu < 0 ? 0 : h->idx2m[bu], le = be < 0 ? 0 : h->idx2m[be], lv = bv < 0 ? 0 : h->idx2m[bv];
+        L[0] = lu > 0 ? pi / lu : 0.0;
+        L[1] = le > 0 ? pi / le : 0.0;
+        L[2] = lv > 0 ? pi / lv : 0.0;
+        double E = 0, Em = 0;
+        for (int i = 0; i < nm; i++) { E += S[i] + S[nm + i]; Em += (double)h->idx2m[i] * (S[i] + S[nm + i]); }
+        L[3] = fabs(E) > tiny ? pi * E / Em : 0.0;
+        L[4] = fabs(visc_3D) > tiny ? sqrt(2.0 * (us2_3D + up2_3D) * h->ViscFac / visc_3D) : 0.0;
+    }
+    double heat[7] = {0}, chem[6] = {0};
+    int mid = N / 2 - 1;
+    if (h->l_heat) {
+        heat[4] = round_off(creal(h->temp[0]) + h->tcond[0]);
+        heat[5] = round_off(creal(h->temp[N - 1]) + h->tcond[N - 1]);
+        heat[0] = round_off(fabs(h->dtcond[0]) > tiny ? 1.0 + creal(dT[0]) / h->dtcond[0] : 1.0);
+        heat[1] = round_off(fabs(h->dtcond[N - 1]) > tiny ? 1.0 + creal(dT[N - 1]) / h->dtcond[N - 1] : 1.0);
+        double nv = rint_r(h, nuv);
+        double nc = INTEG(h->dtcond[k] * h->dtcond[k] * hg[k] * r[k]);
+        heat[2] = round_off(1.0 + nv / nc);
+        double *th = dalloc(N), *f1 = dalloc(N), *f2 = dalloc(N);
+        for (int k = 0; k < N; k++) {
+            double sr = r[k] / h->r_cmb;
+            f1[k] = -h->TdiffFac * h->dtcond[k] * sr * sr;
+            th[k] = asin(sr);
+            f2[k] = flux[k] * sr * sr;
+        }
+        heat[3] = round_off(simps_x(N, f2, th) / simps_x(N, f1, th));
+        heat[6] = round_off(h->dtcond[mid] + creal(dT[mid]));
+        free(th); free(f1); free(f2);
+    }
+    if (h->l_chem) {
+        chem[3] = round_off(creal(h->xi[0]) + h->xicond[0]);
+        chem[4] = round_off(creal(h->xi[N - 1]) + h->xicond[N - 1]);
+        chem[0] = round_off(1.0 + creal(dX[0]) / h->dxicond[0]);
+        chem[1] = round_off(1.0 + creal(dX[N - 1]) / h->dxicond[N - 1]);
+        double sv = rint_r(h, shv);
+        double sc = INTEG(h->dxicond[k] * h->dxicond[k] * hg[k] * r[k]);
+        chem[2] = round_off(1.0 + sv / sc);
+        chem[5] = round_off(h->dxicond[mid] + creal(dX[mid]));
+    }
+#undef INTEG
+    double vals[42] = {h->time, us2_2D, up2_2D, upa_2D, us2_3D, up2_3D, uz2_3D, upa_3D,
+                       pow_2D, chem_2D, visc_2D, pzon_2D, ptot_2D, pow_3D, chem_3D, visc_3D, pzon_3D, ptot_3D,
+                       sqrt(2.0 * (us2_2D + up2_2D) / surf), sqrt(2.0 * upa_2D / surf),
+                       sqrt(2.0 * (us2_2D + up2_2D - upa_2D) / surf),
+                       sqrt(2.0 * (us2_3D + up2_3D + uz2_3D) / vol_otc), sqrt(2.0 * upa_3D / vol_otc),
+                       sqrt(2.0 * (us2_3D + up2_3D + uz2_3D - upa_3D) / vol_otc),
+                       L[0], L[1], L[2], L[3], L[4],
+                       heat[0], heat[1], heat[2], heat[3], heat[4], heat[5], heat[6],
+                       chem[0], chem[1], chem[2], chem[3], chem[4], chem[5]};
+    memcpy(out, vals, sizeof(vals));
+    if (spec) memcpy(spec, S, sizeof(double) * 5 * nm);
+    free(S); free(P); free(tmp); free(dT); free(dX);
+    return 0;
+}
+
 /* ---- unit operators ---- */
 static void host_to_vecs(or_handle *h, const double *host, int n_m, cplx *v)
 {

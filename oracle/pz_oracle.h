@@ -44,6 +44,11 @@ int or_write_checkpoint(or_handle *h, const char *path);
 int or_read_checkpoint(or_handle *h, const char *path);
 /* conducting profiles [tcond | dtcond | xicond | dxicond] (4 n_r values) */
 int or_cond(or_handle *h, double *out);
+/* the log-step time series of get_time_series (outputs.f90:418-830): 42
+ * values [time, e_kin_2D(3), e_kin_3D(4), power_2D(5), power_3D(5),
+ * reynolds_2D(3), reynolds_3D(3), length_scales(5), heat(7), chem(6)], and
+ * the m-spectra of calculate_spectra (5 n_m values, may be NULL) */
+int or_time_series(or_handle *h, double *out, double *spec);
 /* OpenMP threads of the oracle: n > 0 sets them; returns the count in use */
 int or_threads(int n);
 /* planted mistake of the failure-demonstrating tests (0 = none) */

@@ -449,6 +449,8 @@ struct pz_handle {
         int ab1 = 0, bridge = 0;
     } rst;
     bool restarted = false; // the run started from a checkpoint
+    std::vector<double> rintw; // rInt_R weights on this grid (time series)
+    std::vector<double> spec;  // m-spectra of the last pz_time_series
     int bridge_default = 0; // l_bridge_step of a fresh start
     std::string err;
 
@@ -1982,6 +1984,260 @@ extern "C" int pz_ekin_3d(pz_handle *h, double out[4])
     } catch (std::exception &e) {
         return fail(h, -4, e.what());
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Time series and spectra (get_time_series, outputs.f90:418-830;
+// get_lengthscales, :897-962; calculate_spectra, spectra.f90:104-197)
+// ---------------------------------------------------------------------------
+// rInt_R is linear in its integrand: its weights on this run's grid, built
+// once from the unit vectors
+static const std::vector<double> &rint_weights(pz_handle *h)
+{
+    if ((int)h->rintw.size() != h->N) {
+        std::vector<double> e(h->N, 0.0);
+        h->rintw.assign(h->N, 0.0);
+        for (int k = 0; k < h->N; k++) {
+            e[k] = 1.0;
+            h->rintw[k] = rint_r(h, e);
+            e[k] = 0.0;
+        }
+    }
+    return h->rintw;
+}
+
+static double rint_w(pz_handle *h, const std::vector<double> &f)
+{
+    const std::vector<double> &w = rint_weights(h);
+    double s = 0.0;
+    for (int k = 0; k < h->N; k++) s += w[k] * f[k];
+    return s;
+}
+
+// simps (integration.f90:102-150) on abscissae x
+static double simps_x(const std::vector<double> &f, const std::vector<double> &x)
+{
+    const int n = (int)f.size();
+    auto piece = [&](int k) {
+        const double h2 = x[k + 1] - x[k], h1 = x[k] - x[k - 1];
+        return (h1 + h2) / 6.0 * (f[k - 1] * (2.0 * h1 - h2) / h1 + f[k] * (h1 + h2) * (h1 + h2) / (h1 * h2) +
+                                  f[k + 1] * (2.0 * h2 - h1) / h2);
+    };
+    double s = 0.0;
+    if (n % 2 == 1) {
+        for (int k = 1; k < n - 1; k += 2) s += piece(k);
+        return -s;
+    }
+    s = 0.5 * (x[1] - x[0]) * (f[1] + f[0]);
+    for (int k = 2; k < n - 1; k += 2) s += piece(k);
+    s += 0.5 * (x[n - 1] - x[n - 2]) * (f[n - 1] + f[n - 2]);
+    for (int k = 1; k < n - 1; k += 2) s += piece(k);
+    return -0.5 * s;
+}
+
+// the radial derivatives of T and xi as the time series take them: the
+// finite-difference ghost-zone stencil of the step, or get_dr of the
+// Chebyshev methods
+static void series_derivatives(pz_handle *h, cd *dT, cd *dX)
+{
+    const size_t F = (size_t)h->nm * h->N;
+    if (h->d.fd) {
+        const int blocks = (int)((F + 255) / 256);
+        if (h->l_heat)
+            hipLaunchKernelGGL(k_fd_dr_ghost, dim3(blocks), dim3(256), 0, h->stream, h->d, (const cd *)h->d.temp, dT,
+                               h->c.ktopt, h->c.kbott);
+        if (h->l_chem)
+            hipLaunchKernelGGL(k_fd_dr_ghost, dim3(blocks), dim3(256), 0, h->stream, h->d, (const cd *)h->d.xi, dX,
+                               h->c.ktopxi, h->c.kbotxi);
+    } else {
+        const size_t lds = (4 * h->N + 4 * (size_t)h->d.L + 16) * sizeof(cd);
+        if (h->l_heat)
+            hipLaunchKernelGGL(k_dr_vecs, dim3(h->nm), dim3(256), lds, h->stream, h->d, (const cd *)h->d.temp, dT,
+                               (cd *)nullptr, 1);
+        if (h->l_chem)
+            hipLaunchKernelGGL(k_dr_vecs, dim3(h->nm), dim3(256), lds, h->stream, h->d, (const cd *)h->d.xi, dX,
+                               (cd *)nullptr, 1);
+    }
+}
+
+extern "C" int pz_time_series(pz_handle *h, pz_series *ts)
+{
+    if (!h || !ts) return -1;
+    if (!h->started) return fail(h, -5, "pz_start has not been called");
+    try {
+        const int N = h->N, nm = h->nm;
+        const size_t F = (size_t)nm * N;
+        std::memset(ts, 0, sizeof(*ts));
+        cd *dd = nullptr;
+        double *dprof = nullptr, *dspec = nullptr;
+        HIPCHK(hipMallocAsync((void **)&dd, 2 * F * sizeof(cd), h->stream));
+        HIPCHK(hipMallocAsync((void **)&dprof, PZ_NPROF * (size_t)N * sizeof(double), h->stream));
+        HIPCHK(hipMallocAsync((void **)&dspec, 5 * F * sizeof(double), h->stream));
+        HIPCHK(hipMemsetAsync(dd, 0, 2 * F * sizeof(cd), h->stream));
+        series_derivatives(h, dd, dd + F);
+        hipLaunchKernelGGL(k_series_profiles, dim3((N + 63) / 64), dim3(64), 0, h->stream, h->d, (const cd *)dd,
+                           (const cd *)(dd + F), dprof);
+        hipLaunchKernelGGL(k_spectra_mr, dim3((int)((F + 255) / 256)), dim3(256), 0, h->stream, h->d, dspec);
+        HIPCHK(hipGetLastError());
+        std::vector<double> prof(PZ_NPROF * (size_t)N), spec(5 * F);
+        std::vector<double2> dT(F), dX(F), T(F), X(F);
+        PzScheme s;
+        HIPCHK(hipMemcpyAsync(prof.data(), dprof, prof.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(spec.data(), dspec, spec.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(dT.data(), dd, F * sizeof(cd), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(dX.data(), dd + F, F * sizeof(cd), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(T.data(), h->d.temp, F * sizeof(cd), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(X.data(), h->d.xi, F * sizeof(cd), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(&s, h->d.ss, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipFreeAsync(dd, h->stream));
+        HIPCHK(hipFreeAsync(dprof, h->stream));
+        HIPCHK(hipFreeAsync(dspec, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+
+        const double pi = 4.0 * std::atan(1.0), tiny = 10.0 * 2.220446049250313e-16;
+        auto ro = [&](double v) { return std::fabs(v) < tiny ? 0.0 : v; };
+        const std::vector<double> &r = h->r, &hg = h->height, &beta = h->beta;
+        auto row = [&](int j) { return std::vector<double>(prof.begin() + (size_t)j * N, prof.begin() + (size_t)(j + 1) * N); };
+        const std::vector<double> us2 = row(0), up2 = row(1), upa = row(2), enst = row(3), buo = row(4), chem = row(5);
+        std::vector<double> nuv = row(6), shv = row(7), flux = row(8);
+        std::vector<double> rc(N), t(N);
+        // radial functions not kept on the host
+        HIPCHK(hipMemcpy(rc.data(), h->d.rgrav, N * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<double> ekp(N), tcond(N), dtcond(N), xicond(N), dxicond(N);
+        HIPCHK(hipMemcpy(ekp.data(), h->d.ekpump, N * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(tcond.data(), h->d.tcond, N * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dtcond.data(), h->d.dtcond, N * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(xicond.data(), h->d.xicond, N * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dxicond.data(), h->d.dxicond, N * sizeof(double), hipMemcpyDeviceToHost));
+        const double CorFac = h->d.CorFac, BuoFac = h->d.BuoFac, ChemFac = h->d.ChemFac, TdiffFac = h->d.TdiffFac;
+        const double ViscFac = h->d.ViscFac;
+        auto integ = [&](auto fn) {
+            for (int k = 0; k < N; k++) t[k] = fn(k);
+            return rint_w(h, t);
+        };
+        ts->time = s.time;
+        // kinetic energies (outputs.f90:521-569)
+        const double us2_2D = ro(pi * integ([&](int k) { return us2[k] * r[k]; }));
+        const double up2_2D = ro(pi * integ([&](int k) { return up2[k] * r[k]; }));
+        const double upa_2D = ro(pi * integ([&](int k) { return upa[k] * r[k]; }));
+        const double us2_3D = ro(pi * integ([&](int k) { return us2[k] * r[k] * hg[k]; }));
+        const double up2_3D = ro(pi * integ([&](int k) { return up2[k] * r[k] * hg[k]; }));
+        const double uz2_3D = ro(pi * integ([&](int k) { return us2[k] * beta[k] * beta[k] * r[k] * std::pow(hg[k], 3) / 12.0; }));
+        const double upa_3D = ro(pi * integ([&](int k) { return upa[k] * r[k] * hg[k]; }));
+        ts->ekin_2d[0] = us2_2D; ts->ekin_2d[1] = up2_2D; ts->ekin_2d[2] = upa_2D;
+        ts->ekin_3d[0] = us2_3D; ts->ekin_3d[1] = up2_3D; ts->ekin_3d[2] = uz2_3D; ts->ekin_3d[3] = upa_3D;
+        // Reynolds numbers (:572-584); the sphere container (pre_calculations.f90:50-59)
+        const double surf = pi * (h->r_cmb * h->r_cmb - h->r_icb * h->r_icb);
+        const double vol_otc = 4.0 / 3.0 * pi * std::pow(h->r_cmb * h->r_cmb - h->r_icb * h->r_icb, 1.5);
+        ts->reynolds_2d[0] = std::sqrt(2.0 * (us2_2D + up2_2D) / surf);
+        ts->reynolds_2d[1] = std::sqrt(2.0 * upa_2D / surf);
+        ts->reynolds_2d[2] = std::sqrt(2.0 * (us2_2D + up2_2D - upa_2D) / surf);
+        ts->reynolds_3d[0] = std::sqrt(2.0 * (us2_3D + up2_3D + uz2_3D) / vol_otc);
+        ts->reynolds_3d[1] = std::sqrt(2.0 * upa_3D / vol_otc);
+        ts->reynolds_3d[2] = std::sqrt(2.0 * (us2_3D + up2_3D + uz2_3D - upa_3D) / vol_otc);
+        // power budget (:586-670)
+        double visc_2D = ro(2.0 * pi * integ([&](int k) { return enst[k] * r[k]; }));
+        if (h->c.kbotv == 1) visc_2D += 4.0 * pi * up2[N - 1];
+        if (h->c.ktopv == 1) visc_2D -= 4.0 * pi * up2[0];
+        const double pow_2D = h->l_heat ? ro(2.0 * pi * integ([&](int k) { return BuoFac * buo[k] * rc[k] * r[k]; })) : 0.0;
+        const double chem_2D = h->l_chem ? ro(2.0 * pi * integ([&](int k) { return ChemFac * chem[k] * rc[k] * r[k]; })) : 0.0;
+        double visc_3D = ro(2.0 * pi * integ([&](int k) { return enst[k] * r[k] * hg[k]; }));
+        if (h->c.kbotv == 1) visc_3D -= 4.0 * h->or1[N - 1] * up2[N - 1];
+        if (h->c.ktopv == 1) visc_3D -= 4.0 * h->or1[0] * up2[0];
+        const double pow_3D = h->l_heat ? ro(2.0 * pi * integ([&](int k) { return BuoFac * buo[k] * rc[k] * r[k] * hg[k]; })) : 0.0;
+        const double chem_3D = h->l_chem ? ro(2.0 * pi * integ([&](int k) { return ChemFac * chem[k] * rc[k] * r[k] * hg[k]; })) : 0.0;
+        const double ptot_2D = ro(2.0 * pi * integ([&](int k) { return CorFac * (us2[k] + up2[k]) * ekp[k] * r[k]; }));
+        const double pzon_2D = ro(2.0 * pi * integ([&](int k) { return CorFac * upa[k] * ekp[k] * r[k]; }));
+        const double ptot_3D = ro(2.0 * pi * integ([&](int k) { return CorFac * (us2[k] + up2[k]) * ekp[k] * hg[k] * r[k]; }));
+        const double pzon_3D = ro(2.0 * pi * integ([&](int k) { return CorFac * upa[k] * ekp[k] * hg[k] * r[k]; }));
+        ts->power_2d[0] = pow_2D; ts->power_2d[1] = chem_2D; ts->power_2d[2] = visc_2D;
+        ts->power_2d[3] = pzon_2D; ts->power_2d[4] = ptot_2D;
+        ts->power_3d[0] = pow_3D; ts->power_3d[1] = chem_3D; ts->power_3d[2] = visc_3D;
+        ts->power_3d[3] = pzon_3D; ts->power_3d[4] = ptot_3D;
+        // length scales (get_lengthscales, :897-962; dl_diss, :694-701)
+        std::vector<double> us2m(nm), up2m(nm), enm(nm), tm(nm), xm(nm);
+        for (int i = 0; i < nm; i++) {
+            double a[5] = {0, 0, 0, 0, 0};
+            for (int j = 0; j < 5; j++)
+                for (int k = 0; k < N; k++)
+                    a[j] += rint_weights(h)[k] * (pi * spec[(size_t)j * F + (size_t)i * N + k] * r[k] * hg[k]);
+            us2m[i] = a[0]; up2m[i] = a[1]; enm[i] = a[2]; tm[i] = a[3]; xm[i] = a[4];
+        }
+        auto peak = [&](const std::vector<double> &v) {
+            int best = -1;
+            for (int i = 0; i < nm; i++)
+                if (h->idx2m[i] != 0 && (best < 0 || v[i] > v[best])) best = i;
+            return best < 0 ? 0 : h->idx2m[best];
+        };
+        std::vector<double> ek(nm);
+        for (int i = 0; i < nm; i++) ek[i] = us2m[i] + up2m[i];
+        const int lus = peak(us2m), lek = peak(ek), lvo = peak(enm);
+        ts->lengths[0] = lus > 0 ? pi / lus : 0.0;
+        ts->lengths[1] = lek > 0 ? pi / lek : 0.0;
+        ts->lengths[2] = lvo > 0 ? pi / lvo : 0.0;
+        double E = 0.0, Em = 0.0;
+        for (int i = 0; i < nm; i++) { E += ek[i]; Em += (double)h->idx2m[i] * ek[i]; }
+        ts->lengths[3] = std::fabs(E) > tiny ? pi * E / Em : 0.0;
+        ts->lengths[4] = std::fabs(visc_3D) > tiny ? std::sqrt(2.0 * (us2_3D + up2_3D) * ViscFac / visc_3D) : 0.0;
+        // heat and composition (:720-828), m = 0 at index 0
+        const int i0 = 0, mid = N / 2 - 1; // int(n_r_max/2), 1-based
+        if (h->l_heat) {
+            ts->heat[4] = ro(T[(size_t)i0 * N].x + tcond[0]);
+            ts->heat[5] = ro(T[(size_t)i0 * N + N - 1].x + tcond[N - 1]);
+            ts->heat[0] = ro(std::fabs(dtcond[0]) > tiny ? 1.0 + dT[(size_t)i0 * N].x / dtcond[0] : 1.0);
+            ts->heat[1] = ro(std::fabs(dtcond[N - 1]) > tiny ? 1.0 + dT[(size_t)i0 * N + N - 1].x / dtcond[N - 1] : 1.0);
+            for (int k = 0; k < N; k++) nuv[k] *= r[k] * hg[k];
+            const double nv = rint_w(h, nuv);
+            const double nc = integ([&](int k) { return dtcond[k] * dtcond[k] * hg[k] * r[k]; });
+            ts->heat[2] = ro(1.0 + nv / nc);
+            std::vector<double> th(N), f1(N), f2(N);
+            for (int k = 0; k < N; k++) {
+                const double sr = r[k] / h->r_cmb;
+                f1[k] = -TdiffFac * dtcond[k] * sr * sr;
+                th[k] = std::asin(sr);
+                f2[k] = flux[k] * sr * sr;
+            }
+            ts->heat[3] = ro(simps_x(f2, th) / simps_x(f1, th));
+            ts->heat[6] = ro(dtcond[mid] + dT[(size_t)i0 * N + mid].x);
+        }
+        if (h->l_chem) {
+            ts->chem[3] = ro(X[(size_t)i0 * N].x + xicond[0]);
+            ts->chem[4] = ro(X[(size_t)i0 * N + N - 1].x + xicond[N - 1]);
+            ts->chem[0] = ro(1.0 + dX[(size_t)i0 * N].x / dxicond[0]);
+            ts->chem[1] = ro(1.0 + dX[(size_t)i0 * N + N - 1].x / dxicond[N - 1]);
+            for (int k = 0; k < N; k++) shv[k] *= r[k] * hg[k];
+            const double sv = rint_w(h, shv);
+            const double sc = integ([&](int k) { return dxicond[k] * dxicond[k] * hg[k] * r[k]; });
+            ts->chem[2] = ro(1.0 + sv / sc);
+            ts->chem[5] = ro(dxicond[mid] + dX[(size_t)i0 * N + mid].x);
+        }
+        // the m-spectra of the same call (calculate_spectra): us2, up2,
+        // enstrophy, T^2 and xi^2 per m, kept for pz_spectra
+        h->spec.assign(5 * (size_t)nm, 0.0);
+        for (int i = 0; i < nm; i++) {
+            h->spec[i] = us2m[i];
+            h->spec[nm + i] = up2m[i];
+            h->spec[2 * nm + i] = enm[i];
+            h->spec[3 * nm + i] = tm[i];
+            // spectra.f90:188 integrates the temperature profile for xi2_m
+            h->spec[4 * nm + i] = h->l_heat ? tm[i] : xm[i];
+        }
+    } catch (std::exception &e) {
+        return fail(h, -4, e.what());
+    }
+    return 0;
+}
+
+extern "C" int pz_spectra(pz_handle *h, double *us2_m, double *up2_m, double *enst_m, double *temp2_m, double *xi2_m)
+{
+    if (!h) return -1;
+    pz_series ts;
+    const int rc = pz_time_series(h, &ts);
+    if (rc) return rc;
+    double *outs[5] = {us2_m, up2_m, enst_m, temp2_m, xi2_m};
+    for (int j = 0; j < 5; j++)
+        if (outs[j]) std::memcpy(outs[j], h->spec.data() + (size_t)j * h->nm, sizeof(double) * h->nm);
     return 0;
 }
 

@@ -512,3 +512,84 @@ __global__ void __launch_bounds__(256) k_fd_start(const PzDev *__restrict__ pd)
         p.om[off + k] = csub(cadd(fd_dr_at(S, upv, k, N), cscale(R.or1(k), upv[k])),
                              cmul(mk(0.0, R.or1(k) * (double)m), usv[k]));
 }
+
+// ---------------------------------------------------------------------------
+// Diagnostics of the time series (outputs.f90:418-830) and the m-spectra
+// (spectra.f90:104-197)
+// ---------------------------------------------------------------------------
+// The radial derivative of a finite-difference field as the step computes
+// it (get_temp_rhs_imp_ghost, update_temp_fd.f90:217-276): the ghost point
+// of each wall from its boundary condition (fill_ghosts_temp,
+// update_temp_fd.f90:117-169: fixed value 2 f_0 - f_1, fixed flux f_1),
+// then the centred three-point stencil at every point.  One thread per
+// (m, r).
+__global__ void k_fd_dr_ghost(const PzDev p __attribute__((grid_constant)), const cd *f, cd *df, int ktop, int kbot)
+{
+    const int N = p.N;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)p.nm * N) return;
+    const int k = (int)(q % N);
+    const cd *c = f + (q - k);
+    const FdView S(p.fdc, N);
+    const cd fm = k > 0 ? c[k - 1] : (ktop == 1 ? csub(cscale(2.0, c[0]), c[1]) : c[1]);
+    const cd fp = k < N - 1 ? c[k + 1] : (kbot == 1 ? csub(cscale(2.0, c[N - 1]), c[N - 2]) : c[N - 2]);
+    df[q] = cadd3(cscale(S.dr(k, 0), fm), cscale(S.dr(k, 1), c[k]), cscale(S.dr(k, 2), fp));
+}
+
+__device__ __forceinline__ double cc2r(cd a, int m) { return m == 0 ? a.x * a.x : 2.0 * (a.x * a.x + a.y * a.y); }
+__device__ __forceinline__ double cc22r(cd a, cd b, int m)
+{
+    return m == 0 ? a.x * b.x : 2.0 * (a.x * b.x + a.y * b.y);
+}
+
+// Radial profiles of get_time_series (outputs.f90:456-502), one thread per
+// radius: out[j * N + r] for j = us2, up2, up2_axi, enstrophy, buoyancy
+// power, chemical power, volume Nusselt and Sherwood integrands (before
+// the r h factor), radial heat flux.
+#define PZ_NPROF 9
+__global__ void k_series_profiles(const PzDev p __attribute__((grid_constant)), const cd *dT, const cd *dX, double *out)
+{
+    const int nr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nr >= p.N) return;
+    const int N = p.N;
+    double v[PZ_NPROF] = {};
+    const double o2 = p.or2[nr];
+    for (int i = 0; i < p.nm; i++) {
+        const int m = p.idx2m[i];
+        const size_t q = (size_t)i * N + nr;
+        const cd us = p.us[q];
+        v[0] += cc2r(us, m);
+        v[1] += cc2r(p.up[q], m);
+        v[2] += m == 0 ? cc2r(p.up[q], m) : 0.0;
+        v[3] += cc2r(p.om[q], m);
+        if (p.l_heat) {
+            const cd t = p.temp[q];
+            v[4] += cc22r(us, t, m);
+            v[6] += cc2r(dT[q], m) + (double)m * (double)m * o2 * cc2r(t, m);
+            v[8] += cc22r(us, t, m);
+            if (m == 0) v[8] += -p.TdiffFac * dT[q].x - p.TdiffFac * p.dtcond[nr];
+        }
+        if (p.l_chem) {
+            const cd x = p.xi[q];
+            v[5] += cc22r(us, x, m);
+            v[7] += cc2r(dX[q], m) + (double)m * (double)m * o2 * cc2r(x, m);
+        }
+    }
+    for (int j = 0; j < PZ_NPROF; j++) out[(size_t)j * N + nr] = v[j];
+}
+
+// Per (m, r) squares of calculate_spectra (spectra.f90:131-153) before the
+// pi r h factor and the radial integral: out[(j * nm + i) * N + r] for
+// j = us, up, omega, T, xi.
+__global__ void k_spectra_mr(const PzDev p __attribute__((grid_constant)), double *out)
+{
+    const size_t F = (size_t)p.nm * p.N;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= F) return;
+    const int m = p.idx2m[q / p.N];
+    out[q] = cc2r(p.us[q], m);
+    out[F + q] = cc2r(p.up[q], m);
+    out[2 * F + q] = cc2r(p.om[q], m);
+    out[3 * F + q] = p.l_heat ? cc2r(p.temp[q], m) : 0.0;
+    out[4 * F + q] = p.l_chem ? cc2r(p.xi[q], m) : 0.0;
+}

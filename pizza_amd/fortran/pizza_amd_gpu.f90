@@ -38,7 +38,20 @@ module pizza_amd_gpu
       integer(c_int) :: l_finite_diff, fd_pad
    end type pz_config
 
+   ! one log-step row of every time-series file, layout of struct pz_series
+   type, bind(c) :: pz_series
+      real(c_double) :: time
+      real(c_double) :: ekin_2d(3), ekin_3d(4), power_2d(5), power_3d(5)
+      real(c_double) :: reynolds_2d(3), reynolds_3d(3), lengths(5), heat(7), chem(6)
+   end type pz_series
+
    interface
+      integer(c_int) function pz_time_series(h, ts) bind(c, name='pz_time_series')
+         import :: c_int, c_ptr, pz_series
+         type(c_ptr), value :: h
+         type(pz_series), intent(out) :: ts
+      end function pz_time_series
+
       subroutine pz_config_default(cfg) bind(c, name='pz_config_default')
          import :: pz_config
          type(pz_config), intent(out) :: cfg

@@ -4,8 +4,10 @@
 ! namelist, creates the run, applies the namelist's initial conditions
 ! (init_fields.f90:260-509) or reads its start_file checkpoint
 ! (read_checkpoint, checkpoints.f90:185-707), steps through libpizza_amd and
-! writes the e_kin_3D.TAG time series in the reference's format
-! (outputs.f90:686-687) on every log step, the frame files
+! writes the time series of get_time_series in the reference's files and
+! formats (outputs.f90:672-717, 775-777, 824-826: e_kin_2D, e_kin_3D,
+! power_2D, power_3D, reynolds_2D, reynolds_3D, length_scales, heat, chem)
+! on every log step, the frame files
 ! frame_<field>_<n>.TAG on the frame steps (outputs.f90:293-312) and the
 ! checkpoint_end.TAG file at the end of the run (step_time.f90:190-193,
 ! checkpoints.f90:33-183), as time_loop does (l_correct_step,
@@ -27,6 +29,8 @@ program pizza_gpu
    type(pz_config) :: cfg
    type(c_ptr) :: h
    integer :: nargs, i, n_steps, n_done, chunk, u, n_frame_out
+   integer :: uf(8)
+   type(pz_series) :: ts
    logical :: print_only, by_stages
    real(c_double) :: time, dt, e(4)
 
@@ -80,6 +84,7 @@ contains
       call check(pz_start(h), 'pz_start')
 
       open(newunit=u, file='e_kin_3D.'//trim(tag), status='replace', action='write')
+      call open_series()
       n_done = 0
       n_frame_out = 0
       do
@@ -87,6 +92,7 @@ contains
             call check(pz_time(h, time, dt), 'pz_time')
             call check(pz_ekin_3d(h, e), 'pz_ekin_3d')
             write(u, '(1P, es20.12, 4es16.8)') time, e(1), e(2), e(3), e(4)
+            call write_series()
          end if
          if (l_correct_step(n_done, n_steps, n_frame_step, n_frames)) call write_frames()
          if (n_done >= n_steps) exit
@@ -101,10 +107,40 @@ contains
          n_done = n_done + chunk
       end do
       close(u)
+      do i = 1, size(uf)
+         if (uf(i) /= 0) close(uf(i))
+      end do
       call check(pz_write_checkpoint(h, cstr('checkpoint_end.'//trim(tag))), 'pz_write_checkpoint')
       call check(pz_destroy(h), 'pz_destroy')
 
    end subroutine run
+
+   ! the time-series files of outputs.f90:110-139 (heat and chem only with
+   ! the corresponding field)
+   subroutine open_series()
+      character(len=16), parameter :: names(8) = [character(len=16) :: 'e_kin_2D', 'power_2D', 'power_3D', &
+         'reynolds_2D', 'reynolds_3D', 'length_scales', 'heat', 'chem']
+      integer :: k
+      uf = 0
+      do k = 1, size(names)
+         if (k == 7 .and. ra == 0.0_c_double) cycle
+         if (k == 8 .and. raxi == 0.0_c_double) cycle
+         open(newunit=uf(k), file=trim(names(k))//'.'//trim(tag), status='replace', action='write')
+      end do
+   end subroutine open_series
+
+   ! one row of each (outputs.f90:672-717, 775-777, 824-826)
+   subroutine write_series()
+      call check(pz_time_series(h, ts), 'pz_time_series')
+      write(uf(1), '(1P, es20.12, 3es16.8)') ts%time, ts%ekin_2d
+      write(uf(2), '(1P, es20.12, 5es16.8)') ts%time, ts%power_2d
+      write(uf(3), '(1P, es20.12, 5es16.8)') ts%time, ts%power_3d
+      write(uf(4), '(1P, es20.12, 3es16.8)') ts%time, ts%reynolds_2d
+      write(uf(5), '(1P, es20.12, 3es16.8)') ts%time, ts%reynolds_3d
+      write(uf(6), '(1P, es20.12, 5es16.8)') ts%time, ts%lengths
+      if (uf(7) /= 0) write(uf(7), '(1P, ES20.12, 7ES16.8)') ts%time, ts%heat
+      if (uf(8) /= 0) write(uf(8), '(1P, ES20.12, 6ES16.8)') ts%time, ts%chem
+   end subroutine write_series
 
    ! the next step count after n at which time_loop writes an output
    integer function next_output(n)

@@ -23,6 +23,14 @@ class PizzaError(RuntimeError):
     pass
 
 
+class PzSeries(ctypes.Structure):
+    """struct pz_series (include/pizza_amd.h)."""
+    _fields_ = [("time", ctypes.c_double), ("ekin_2d", ctypes.c_double * 3), ("ekin_3d", ctypes.c_double * 4),
+                ("power_2d", ctypes.c_double * 5), ("power_3d", ctypes.c_double * 5),
+                ("reynolds_2d", ctypes.c_double * 3), ("reynolds_3d", ctypes.c_double * 3),
+                ("lengths", ctypes.c_double * 5), ("heat", ctypes.c_double * 7), ("chem", ctypes.c_double * 6)]
+
+
 class PzCheckpointInfo(ctypes.Structure):
     """struct pz_checkpoint_info (include/pizza_amd.h)."""
     _fields_ = [("version", ctypes.c_int), ("time", ctypes.c_double), ("l_cheb_coll", ctypes.c_int),
@@ -75,6 +83,8 @@ def lib():
         L.pz_read_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.pz_read_checkpoint_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(PzCheckpointInfo), dp, ip]
         L.pz_write_frame.argtypes = [vp, ip, ctypes.c_char_p]
+        L.pz_time_series.argtypes = [vp, ctypes.POINTER(PzSeries)]
+        L.pz_spectra.argtypes = [vp, dp, dp, dp, dp, dp]
         L.pz_time_kernel.argtypes = [vp, ip, ip, dp]
         L.pz_ensemble_create.argtypes = [ctypes.POINTER(vp), ip, ctypes.POINTER(vp)]
         L.pz_ensemble_step.argtypes = [vp, ip]
@@ -91,7 +101,7 @@ def lib():
 EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_dims",
            "pz_set_field", "pz_get_field", "pz_init_fields", "pz_start", "pz_step", "pz_nstages",
            "pz_stage_explicit", "pz_courant", "pz_stage_implicit", "pz_write_checkpoint",
-           "pz_read_checkpoint", "pz_read_checkpoint_header", "pz_write_frame",
+           "pz_read_checkpoint", "pz_read_checkpoint_header", "pz_write_frame", "pz_time_series", "pz_spectra",
            "pz_time", "pz_ekin_3d", "pz_nusselt", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
            "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
            "pz_ensemble_destroy", "pz_ensemble_last_error", "pz_ensemble_policy", "pz_adjoint_forward", "pz_adjoint_backward"]
@@ -198,6 +208,22 @@ class Run:
     def read_checkpoint(self, path):
         """Reads a checkpoint into the run; start() must follow."""
         self._check(lib().pz_read_checkpoint(self.h, str(path).encode()), "pz_read_checkpoint")
+
+    def time_series(self):
+        """The 42 log-step values of get_time_series in the layout of
+        or_time_series: [time, e_kin_2D(3), e_kin_3D(4), power_2D(5),
+        power_3D(5), reynolds_2D(3), reynolds_3D(3), length_scales(5),
+        heat(7), chem(6)]."""
+        ts = PzSeries()
+        self._check(lib().pz_time_series(self.h, ctypes.byref(ts)), "pz_time_series")
+        return np.r_[ts.time, list(ts.ekin_2d), list(ts.ekin_3d), list(ts.power_2d), list(ts.power_3d),
+                     list(ts.reynolds_2d), list(ts.reynolds_3d), list(ts.lengths), list(ts.heat), list(ts.chem)]
+
+    def spectra(self):
+        """(5, n_m): us2_m, up2_m, enst_m, temp2_m, xi2_m of calculate_spectra."""
+        out = np.zeros((5, self.n_m))
+        self._check(lib().pz_spectra(self.h, *[_dp(out[j]) for j in range(5)]), "pz_spectra")
+        return out
 
     def write_frame(self, name, path):
         """Frame file of one field (output_frames.f90:96-167)."""

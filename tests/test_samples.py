@@ -73,3 +73,9 @@ def test_sample_fortran_driver(name, tmp_path):
     np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-20)
     for t in tags:
         assert (tmp_path / f"frame_us_1.{t}").exists() and (tmp_path / f"checkpoint_end.{t}").exists()
+        # the other time series share the log steps of e_kin_3D
+        n = len(np.loadtxt(tmp_path / f"e_kin_3D.{t}"))
+        for series in ("e_kin_2D", "power_2D", "power_3D", "reynolds_2D", "reynolds_3D", "length_scales", "heat"):
+            rows = np.loadtxt(tmp_path / f"{series}.{t}", ndmin=2)
+            assert rows.shape[0] == n, (series, t)
+            assert np.array_equal(rows[:, 0], np.loadtxt(tmp_path / f"e_kin_3D.{t}")[:, 0])

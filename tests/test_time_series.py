@@ -1,0 +1,115 @@
+"""The log-step diagnostics of get_time_series (outputs.f90:418-830):
+kinetic energies, power budget, Reynolds numbers, length scales
+(get_lengthscales, outputs.f90:897-962), heat and composition series, and
+the m-spectra of calculate_spectra (spectra.f90:104-197).
+
+The oracle is pinned by:
+- its e_kin_3D entries, which are the golden-pinned e_kin_3D series (the
+  reference's samples check exactly these);
+- closed forms: the conducting state has every Nusselt and Sherwood number
+  1, tTop = 0, tBot = 1 and beta_t = dT_cond/dr at mid-shell; a velocity of
+  one mode m = 3 with u_s = a sin(pi (r - r_i)) has us2_2D = 2 pi a^2
+  int sin^2 r dr (checked against scipy's quadrature), all three peak length
+  scales and the integral length scale pi / 3, and Reynolds numbers
+  sqrt(2 E / area);
+- the reference's xi spectrum, which integrates the temperature profile
+  (spectra.f90:188).
+The GPU library matches the oracle after a few steps of each radial method.
+"""
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+from helpers import rel
+
+PI = np.pi
+SL = {"time": 0, "ekin_2d": slice(1, 4), "ekin_3d": slice(4, 8), "power_2d": slice(8, 13),
+      "power_3d": slice(13, 18), "rey_2d": slice(18, 21), "rey_3d": slice(21, 24), "lengths": slice(24, 29),
+      "heat": slice(29, 36), "chem": slice(36, 42)}
+
+
+def _cfg(method, n_r=33, **kw):
+    from pizza_amd.config import default_config
+    c = default_config(n_r_max=n_r, n_cheb_max=n_r, m_max=16, minc=1, time_scheme="CNAB2",
+                       cheb_method="colloc" if method == "coll" else "integ", dtMax=1e-5, ra=1e5, ek=1e-3,
+                       raxi=3e4, sc=3.0, radratio=0.35, l_ek_pump=True, **kw)
+    if method == "fd":
+        c.l_finite_diff = 1
+    return c
+
+
+@pytest.mark.parametrize("method", ["coll", "integ", "fd"])
+def test_conducting_state(method):
+    from oracle.oracle import Oracle
+    o = Oracle(_cfg(method))
+    o.start()
+    ts, _ = o.time_series()
+    heat, chem = ts[SL["heat"]], ts[SL["chem"]]
+    assert np.allclose(heat[:4], 1.0, atol=1e-12) and np.allclose(chem[:3], 1.0, atol=1e-12)
+    assert abs(heat[4]) < 1e-14 and abs(heat[5] - 1.0) < 1e-12 and abs(chem[4] - 1.0) < 1e-12
+    r, _ = o.grid()
+    mid = len(r) // 2 - 1
+    assert abs(heat[6] - 1.0 / (r[mid] * np.log(0.35))) < 1e-12   # dtcond of radial.f90:365-366
+    assert np.all(ts[1:24] == 0.0)
+
+
+@pytest.mark.parametrize("method", ["coll", "fd"])
+def test_single_mode_closed_forms(method):
+    from oracle.oracle import Oracle
+    n_r = 65 if method == "coll" else 257
+    o = Oracle(_cfg(method, n_r=n_r))
+    r, _ = o.grid()
+    ri, ro = r[-1], r[0]
+    a = 0.3
+    us = np.zeros((o.n_m, o.n_r), complex)
+    us[3] = a * np.sin(PI * (r - ri))
+    o.set_field("us", us)
+    ts, spec = o.time_series()
+    exact = 2 * PI * a * a * quad(lambda x: np.sin(PI * (x - ri)) ** 2 * x, ri, ro, epsabs=1e-14)[0]
+    tol = 1e-12 if method == "coll" else 1e-4   # Simpson on the stretched grid
+    assert abs(ts[SL["ekin_2d"]][0] - exact) < tol * exact
+    surf = PI * (ro * ro - ri * ri)
+    assert abs(ts[SL["rey_2d"]][0] - np.sqrt(2 * ts[1] / surf)) < 1e-14
+    assert np.allclose(ts[SL["lengths"]][[0, 1, 3]], PI / 3, rtol=1e-14)
+    # e_kin_3D is the golden-pinned series
+    e3 = o.ekin_3d()
+    assert np.array_equal(ts[SL["ekin_3d"]], e3)
+    # spectrum: all the energy at m = 3, and its sum is us2_3D
+    assert np.argmax(spec[0]) == 3 and abs(spec[0].sum() - e3[0]) < 1e-13 * e3[0]
+
+
+def test_xi_spectrum_is_the_temperature_profile():
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    cfg = _cfg("coll")
+    o = Oracle(cfg)
+    W.seed_run(o, cfg, amp_t=1e-3, amp_xi=3e-3)
+    o.start()
+    o.step(2)
+    _, spec = o.time_series()
+    assert np.array_equal(spec[4], spec[3]) and spec[3].max() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["coll", "integ", "fd"])
+def test_time_series_gpu_matches_oracle(method):
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    from pizza_amd.runtime import Run
+    cfg = _cfg(method, n_r=33 if method != "fd" else 65)
+    print(f"case: time series {method}", flush=True)
+    g, o = Run(cfg), Oracle(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg, amp_t=1e-3, amp_xi=1e-3)
+        r, ri = W.radial_grid(cfg)
+        up = np.zeros((x.n_m, x.n_r), complex)
+        up[0] = 1e-2 * np.sin(PI * (r - ri)) ** 2
+        up[2] = 3e-3 * np.sin(PI * (r - ri)) ** 2
+        x.set_field("up", up)
+        x.start()
+        x.step(5)
+    a, (b, sb) = g.time_series(), o.time_series()
+    scale = np.maximum(np.abs(b), 1e-14)
+    bad = np.abs(a - b) > 1e-9 * scale
+    assert not bad.any(), (np.nonzero(bad), a[bad], b[bad])
+    assert rel(g.spectra(), sb) < 1e-10
