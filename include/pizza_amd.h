@@ -317,6 +317,23 @@ int pz_fft(pz_handle *h, const double *grid, double *spec, int n_rad);
 int pz_ifft(pz_handle *h, const double *spec, double *grid, int n_rad);
 
 /*
+ * pz_get_dr / pz_get_ddr on DEVICE buffers, without allocating: no
+ * counterpart in the reference; for callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  d_f, d_df and d_ddf are device arrays complex(8)
+ * (n_m, n_r_max) in the layout of pz_get_ddr (m fastest); d_ddf = NULL
+ * computes the first derivative only (pz_get_dr).  The layout changes are
+ * staged through a static device array of 3 x 262144 complex values that
+ * each GPU holds until the process ends, so n_m * n_r_max <= 262144
+ * (larger sizes return -2).  Asynchronous on the handle's stream
+ * (pz_stream); the caller synchronises before reading d_df.  Every call on
+ * one GPU shares that GPU's staging array: calls that overlap on one GPU
+ * (handles on different streams) race.
+ */
+int pz_get_ddr_noalloc(pz_handle *h, const double *d_f, double *d_df, double *d_ddf, int n_m,
+                       int l_dct_in);
+
+/*
  * Kernel timing for the roofline report: launches one stage kernel reps
  * times between HIP events on the handle's stream and returns the mean
  * duration in ms.  which = 0: the m-loop kernel of the run's radial method

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -22,6 +23,53 @@
 #include "pz_checkpoint.h"
 
 using pzh::ld;
+
+// Staging memory of pz_get_ddr_noalloc: three buffers of PZ_SCRATCH_ELEMS
+// complex values (12 MiB) that every GPU running the library holds until the
+// process ends.  Host code never passes the array's name to a kernel: it
+// takes the device address of this GPU's copy from scratch_address().
+#define PZ_SCRATCH_ELEMS (1 << 18)
+static __device__ __align__(256) double2 scratch[3 * PZ_SCRATCH_ELEMS];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy at its own address).
+static double2 *scratch_address()
+{
+    static std::mutex mu;
+    static double2 *addr[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) throw std::runtime_error("hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[dev] = (double2 *)p;
+    }
+    return addr[dev];
+}
+
+// Layout changes between the reference's f(n_m, n_r) (m fastest) and the
+// M-layout [m][r] of the transform kernels.
+__global__ void __launch_bounds__(256) k_from_ref_layout(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                          int n_m, int N)
+{
+    const size_t S = (size_t)n_m * N;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < S; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = q / N, k = q % N;
+        dst[q] = src[k * n_m + i];
+    }
+}
+__global__ void __launch_bounds__(256) k_to_ref_layout(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                        int n_m, int N)
+{
+    const size_t S = (size_t)n_m * N;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < S; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = q / n_m, i = q % n_m;
+        dst[q] = src[i * N + k];
+    }
+}
 
 namespace {
 
@@ -442,6 +490,7 @@ struct pz_handle {
     int inv_waves = 16;
     int mthreads = PZ_MPHASE_THREADS; // threads of k_mphase / k_mcoll
     PzDev *ddev = nullptr;
+    double2 *scratch = nullptr; // this device's copy of the staging array
     // restart state of pz_read_checkpoint, consumed by the next pz_start
     struct {
         bool pending = false;
@@ -1161,6 +1210,7 @@ static void setup(pz_handle *h)
     if (need > lim) throw std::invalid_argument("problem size exceeds the LDS budget of one workgroup");
     allow_full_lds((int)lim);
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->scratch = scratch_address();
     // device copy of the (now final) descriptor for kernels that read it
     // through a pointer
     h->ddev = h->dalloc<PzDev>(1);
@@ -2300,6 +2350,30 @@ extern "C" int pz_get_ddr(pz_handle *h, const double *f, double *df, double *ddf
 {
     if (!h || !f || !df || !ddf || n_m <= 0) return -1;
     return vec_op(h, f, df, ddf, n_m, 2, l_dct_in);
+}
+
+// get_dr / get_ddr on device buffers in the reference layout, staged through
+// the static scratch array: no allocation, asynchronous on the handle's
+// stream.  The transform kernel is pz_get_ddr's.
+extern "C" int pz_get_ddr_noalloc(pz_handle *h, const double *d_f, double *d_df, double *d_ddf, int n_m,
+                                  int l_dct_in)
+{
+    if (!h || !d_f || !d_df || n_m <= 0) return -1;
+    const int N = h->N;
+    const size_t S = (size_t)n_m * N;
+    if (S > PZ_SCRATCH_ELEMS)
+        return fail(h, -2, "pz_get_ddr_noalloc: n_m * n_r_max exceeds the staging limit of 262144 complex values");
+    if (!h->scratch) return fail(h, -4, "pz_get_ddr_noalloc: no staging address on this device");
+    double2 *a = h->scratch, *b = a + PZ_SCRATCH_ELEMS, *c = b + PZ_SCRATCH_ELEMS;
+    const int blocks = (int)std::min<size_t>((S + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_from_ref_layout, dim3(blocks), dim3(256), 0, h->stream, (const double2 *)d_f, a, n_m, N);
+    hipLaunchKernelGGL(k_dr_vecs, dim3(n_m), dim3(256), (4 * N + 4 * (size_t)h->d.L + 16) * sizeof(cd), h->stream, h->d,
+                       a, b, d_ddf ? c : nullptr, l_dct_in);
+    hipLaunchKernelGGL(k_to_ref_layout, dim3(blocks), dim3(256), 0, h->stream, b, (double2 *)d_df, n_m, N);
+    if (d_ddf) hipLaunchKernelGGL(k_to_ref_layout, dim3(blocks), dim3(256), 0, h->stream, c, (double2 *)d_ddf, n_m, N);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(h, -4, std::string("pz_get_ddr_noalloc: ") + hipGetErrorString(e));
+    return 0;
 }
 
 // Classical top and bottom Nusselt numbers of the heat time series

@@ -79,6 +79,7 @@ def lib():
         L.pz_get_ddr.argtypes = [vp, dp, dp, dp, ip, ip]
         L.pz_fft.argtypes = [vp, dp, dp, ip]
         L.pz_ifft.argtypes = [vp, dp, dp, ip]
+        L.pz_get_ddr_noalloc.argtypes = [vp, vp, vp, vp, ip, ip]
         L.pz_write_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.pz_read_checkpoint.argtypes = [vp, ctypes.c_char_p]
         L.pz_read_checkpoint_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(PzCheckpointInfo), dp, ip]
@@ -103,7 +104,7 @@ EXPORTS = ["pz_config_default", "pz_create", "pz_destroy", "pz_last_error", "pz_
            "pz_stage_explicit", "pz_courant", "pz_stage_implicit", "pz_write_checkpoint",
            "pz_read_checkpoint", "pz_read_checkpoint_header", "pz_write_frame", "pz_time_series", "pz_spectra",
            "pz_time", "pz_ekin_3d", "pz_nusselt", "pz_stream", "pz_costf", "pz_get_dr", "pz_get_ddr",
-           "pz_fft", "pz_ifft", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
+           "pz_fft", "pz_ifft", "pz_get_ddr_noalloc", "pz_time_kernel", "pz_ensemble_create", "pz_ensemble_step",
            "pz_ensemble_destroy", "pz_ensemble_last_error", "pz_ensemble_policy", "pz_adjoint_forward", "pz_adjoint_backward"]
 
 OBJECTIVES = {"nutop": 0, "ekin": 1, "temp": 2}
