@@ -1167,7 +1167,7 @@ static void setup(pz_handle *h)
     }
     if (!d.coll && !d.fd) build_operators(h);
     // LDS budgets
-    h->lds_radial = (5 * (size_t)h->nm + 6 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
+    h->lds_radial = (5 * (size_t)h->nm + 7 * (size_t)h->nphi) * sizeof(cd) + 16 * sizeof(double);
     h->mthreads = mloop_threads(N);
     h->lds_mphase = MpLayout(N, h->mthreads).total;
     h->lds_fd = FdLayout(N).total;
@@ -1777,7 +1777,8 @@ extern "C" int pz_start(pz_handle *h)
 static void launch_explicit(pz_handle *h, int stage)
 {
     if (h->sch.l_exp_calc[stage - 1])
-        hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N), dim3(256), h->lds_radial,
+        hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N),
+                           dim3(h->radial_big ? PZ_RADIAL_BIG_THREADS : 256), h->lds_radial,
                            h->stream, (const PzDev *)h->ddev, stage - 1, stage == 1 ? 1 : 0);
 }
 
@@ -2783,7 +2784,8 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
                 hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else
-                hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N), dim3(256),
+                hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N),
+                                   dim3(h->radial_big ? PZ_RADIAL_BIG_THREADS : 256),
                                    h->lds_radial, h->stream, (const PzDev *)h->ddev, 0, 0);
         }
         HIPCHK(hipEventRecord(e1, h->stream));

@@ -145,16 +145,23 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
             // per item so that all n outputs of a stage run in parallel; the
             // stage twiddle and the DFT kernel combine into one table entry,
             // exp(dir 2 pi i (r tstep + ((r k) mod R) n/R) / n)
+            // The exponent is kept as two running residues (r tstep mod n
+            // and r k mod R) so the tap loop has no integer division.
             const int q = n / R;
             for (int t = threadIdx.x; t < n * ntr; t += blockDim.x) {
                 const int tr = t / n, rem = t - tr * n, k = rem / nb, j = rem - k * nb;
                 const cd *x = a + tr * n;
                 const int jm = j % Ns, tstep = (n / (Ns * R)) * jm;
                 cd acc = mk(0.0, 0.0);
+                int e1 = 0, rk = 0;
                 for (int r = 0; r < R; r++) {
-                    int e = r * tstep + ((r * k) % R) * q;
+                    int e = e1 + rk * q;
                     if (e >= n) e -= n;
                     acc = cadd(acc, cmul(x[j + r * nb], tw(e)));
+                    e1 += tstep;
+                    if (e1 >= n) e1 -= n;
+                    rk += k;
+                    if (rk >= R) rk -= R;
                 }
                 b[tr * n + (j / Ns) * Ns * R + jm + k * Ns] = acc;
             }
@@ -285,9 +292,11 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
     cd *spec = (cd *)smem;           // 5 x nm
     cd *bufa = spec + 5 * nm;        // 3 x n
     cd *bufb = bufa + 3 * n;         // 3 x n
-    double *red = (double *)(bufb + 3 * n);
+    cd *twl = bufb + 3 * n;          // n twiddles exp(-2 pi i k / n)
+    double *red = (double *)(twl + n);
     const int nin = (p.l_heat || p.l_chem) ? (p.l_chem ? 3 : 2) : 2;
 
+    for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = p.twid[k];
     for (int i = threadIdx.x; i < nm; i += blockDim.x) {
         size_t q = (size_t)i * N + nr;
         spec[i] = p.us[q];
@@ -324,7 +333,8 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
         bufa[t] = z;
     }
     __syncthreads();
-    cd *g = fft_lds(p, bufa, bufb, n, nin, +1);
+    cd *g = fft_stockham(bufa, bufb, n, nin, +1, p.radices, p.nrad,
+                         [&](int k) { const cd w = twl[k]; return mk(w.x, -w.y); });
     cd *o = (g == bufa) ? bufb : bufa;
     // Courant (courant.f90:41-76)
     if (courant) {
@@ -362,7 +372,7 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
     }
     __syncthreads();
     cd *other = (o == bufa) ? bufb : bufa;
-    cd *Z = fft_lds(p, o, other, n, nout, -1);
+    cd *Z = fft_stockham(o, other, n, nout, -1, p.radices, p.nrad, [&](int k) { return twl[k]; });
     // unpack, normalise and post-scale (radial_loop.f90:350-386)
     const double or1 = p.or1[nr];
     const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
@@ -401,6 +411,16 @@ __device__ __forceinline__ void radial_loop_big_body(const PzDev &p, int nr, int
     const size_t F = (size_t)nm * N;
     cd *s0 = (cd *)smem, *s1 = s0 + n, *s2 = s1 + n, *sc = s2 + n;
     double *red = (double *)(sc + n);
+    // without composition the third column holds the twiddle table
+    const bool twl_lds = !p.l_chem;
+    cd *twl = s2;
+    if (twl_lds)
+        for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = p.twid[k];
+    auto fwd = [&](int k) { return twl_lds ? twl[k] : p.twid[k]; };
+    auto bwd = [&](int k) {
+        const cd w = twl_lds ? twl[k] : p.twid[k];
+        return mk(w.x, -w.y);
+    };
     // Reynolds-stress m-sum (radial_loop.f90:267-278)
     double part = 0.0;
     for (int i = threadIdx.x; i < nm; i += blockDim.x) {
@@ -433,14 +453,14 @@ __device__ __forceinline__ void radial_loop_big_body(const PzDev &p, int nr, int
     // one complex FFT at a time; the result lands in the column or in the
     // scratch, whose roles are then exchanged
     {
-        cd *g = fft_lds(p, s0, sc, n, 1, +1);
+        cd *g = fft_stockham(s0, sc, n, 1, +1, p.radices, p.nrad, bwd);
         if (g != s0) { sc = s0; s0 = g; }
         __syncthreads();
-        g = fft_lds(p, s1, sc, n, 1, +1);
+        g = fft_stockham(s1, sc, n, 1, +1, p.radices, p.nrad, bwd);
         if (g != s1) { sc = s1; s1 = g; }
         if (p.l_chem) {
             __syncthreads();
-            g = fft_lds(p, s2, sc, n, 1, +1);
+            g = fft_stockham(s2, sc, n, 1, +1, p.radices, p.nrad, bwd);
             if (g != s2) { sc = s2; s2 = g; }
         }
         __syncthreads();
@@ -477,16 +497,16 @@ __device__ __forceinline__ void radial_loop_big_body(const PzDev &p, int nr, int
     }
     __syncthreads();
     {
-        cd *g = fft_lds(p, s0, sc, n, 1, -1);
+        cd *g = fft_stockham(s0, sc, n, 1, -1, p.radices, p.nrad, fwd);
         if (g != s0) { sc = s0; s0 = g; }
         if (p.l_heat) {
             __syncthreads();
-            g = fft_lds(p, s1, sc, n, 1, -1);
+            g = fft_stockham(s1, sc, n, 1, -1, p.radices, p.nrad, fwd);
             if (g != s1) { sc = s1; s1 = g; }
         }
         if (p.l_chem) {
             __syncthreads();
-            g = fft_lds(p, s2, sc, n, 1, -1);
+            g = fft_stockham(s2, sc, n, 1, -1, p.radices, p.nrad, fwd);
             if (g != s2) { sc = s2; s2 = g; }
         }
         __syncthreads();
@@ -520,7 +540,8 @@ __device__ __forceinline__ void radial_loop_big_body(const PzDev &p, int nr, int
     }
 }
 
-__global__ void __launch_bounds__(256) k_radial_loop_big(const PzDev *__restrict__ pd, int is, int courant)
+#define PZ_RADIAL_BIG_THREADS 1024
+__global__ void __launch_bounds__(PZ_RADIAL_BIG_THREADS) k_radial_loop_big(const PzDev *__restrict__ pd, int is, int courant)
 {
     radial_loop_big_body(*pd, blockIdx.x, is, courant);
 }
