@@ -124,10 +124,11 @@ __device__ __forceinline__ cd ci(double s, cd a) { return mk(-s * a.y, s * a.x);
 // arithmetic.  Linear only without the rerror threshold (checked on the
 // host).  LDS: 6 N + 6 L + 16 complex and 3 N doubles.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_probe(const PzDev p __attribute__((grid_constant)), double *C,
+__global__ void __launch_bounds__(256) k_adj_probe(const PzDev *__restrict__ pd, double *C,
                                                    double *G, double *D1, double *D2, double *E1, double *Uimp,
                                                    double *lnu)
 {
+    const PzDev &p = *pd;
     extern __shared__ double smem[];
     const int j = blockIdx.x, N = p.N, tid = threadIdx.x;
     const RadView R(p.radc, N);

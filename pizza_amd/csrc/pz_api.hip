@@ -1747,7 +1747,8 @@ extern "C" int pz_start(pz_handle *h)
         if (h->d.fd) {
             hipLaunchKernelGGL(k_fd_start, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev);
         } else if (h->d.coll) {
-            hipLaunchKernelGGL(k_start_coll, dim3(h->nm), dim3(256), h->lds_start_coll, h->stream, h->d);
+            hipLaunchKernelGGL(k_start_coll, dim3(h->nm), dim3(256), h->lds_start_coll, h->stream,
+                               (const PzDev *)h->ddev);
             if (h->d.l_assembly) {
                 if (!h->d.A_inv || !h->d.dr_top || !h->d.dr_bot)
                     throw std::logic_error("assembly stage without its matrices");
@@ -2476,7 +2477,8 @@ static void adj_init(pz_handle *h)
     a->W = h->dalloc<cd>(F);
     a->grad = h->dalloc<cd>(F);
     const size_t lds = (6 * (size_t)N + 6 * (size_t)h->d.L + 16) * sizeof(cd) + 3 * (size_t)N * sizeof(double);
-    hipLaunchKernelGGL(k_adj_probe, dim3(N), dim3(256), lds, h->stream, h->d, C, G, D1, D2, E1, U, lnu);
+    hipLaunchKernelGGL(k_adj_probe, dim3(N), dim3(256), lds, h->stream, (const PzDev *)h->ddev, C, G, D1, D2, E1, U,
+                       lnu);
     HIPCHK(hipGetLastError());
 }
 

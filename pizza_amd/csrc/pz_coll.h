@@ -404,28 +404,47 @@ __global__ void __launch_bounds__(1024) k_build_coll_batch(PzBatch b, int nb, in
 
 // get_ddcheb_real_1d on Chebyshev coefficients f (radial_derivatives.f90:
 // 255-311), one thread; df, ddf are coefficients
-__device__ void ddcheb_real(const PzDev &p, const double *f, double *df, double *ddf)
+__device__ __forceinline__ void ddcheb_real(const PzDev &p, const double *__restrict__ f, double *__restrict__ df,
+                                            double *__restrict__ ddf)
 {
     const int N = p.N;
-    double mx = 0.0;
-    for (int j = 0; j < N; j++) mx = fmax(mx, fabs(f[j]));
-    const double thr = mx * p.thr;
+    // four independent maxima so that the loads overlap
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    int j = 0;
+    for (; j + 3 < N; j += 4) {
+        m0 = fmax(m0, fabs(f[j]));
+        m1 = fmax(m1, fabs(f[j + 1]));
+        m2 = fmax(m2, fabs(f[j + 2]));
+        m3 = fmax(m3, fabs(f[j + 3]));
+    }
+    for (; j < N; j++) m0 = fmax(m0, fabs(f[j]));
+    const double thr = fmax(fmax(m0, m1), fmax(m2, m3)) * p.thr;
+    // both recurrences carried in registers: d1 = df(n), d2 = df(n+1),
+    // e1 = ddf(n), e2 = ddf(n+1)
     df[N - 1] = 0.0;
     ddf[N - 1] = 0.0;
     const int nc = N - 1;
-    df[nc - 1] = fabs(f[nc]) >= thr ? (double)nc * f[nc] : 0.0;
+    double d1 = fabs(f[nc]) >= thr ? (double)nc * f[nc] : 0.0, d2 = 0.0, e1 = 0.0, e2 = 0.0;
+    df[nc - 1] = d1;
     ddf[nc - 1] = 0.0;
+#pragma unroll 4
     for (int n = N - 2; n >= 1; n--) {
-        const double fac = (double)(2 * n);
-        df[n - 1] = fabs(f[n]) >= thr ? df[n + 1] + fac * f[n] : df[n + 1];
-        ddf[n - 1] = fabs(df[n]) >= thr ? ddf[n + 1] + fac * df[n] : ddf[n + 1];
+        const double fac = (double)(2 * n), fn = f[n];
+        const double dv = fabs(fn) >= thr ? d2 + fac * fn : d2;
+        const double ev = fabs(d1) >= thr ? e2 + fac * d1 : e2;
+        df[n - 1] = dv;
+        ddf[n - 1] = ev;
+        d2 = d1;
+        d1 = dv;
+        e2 = e1;
+        e1 = ev;
     }
 }
 
 // m = 0 implicit term of get_psi_rhs_imp_coll_smat (update_psi_coll_smat.f90:
 // 454-466): u0 holds u_phi(m = 0) on the grid (real parts); writes impl.
 // Uses cA, cB (LDS vectors) and dt3 (3 N doubles) as scratch.
-__device__ void coll_uphi0_impl(const PzDev &p, const RadView &R, const cd *u0, cd *cA, cd *cB, double *dt3,
+__device__ __forceinline__ void coll_uphi0_impl(const PzDev &p, const RadView &R, const cd *u0, cd *cA, cd *cB, double *dt3,
                                 const double *cosT, const double *sinT, cd *fbuf, cd *impl_out, double hdif)
 {
     const int N = p.N, tid = threadIdx.x;
@@ -498,21 +517,26 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
     const RadView R(RADLDS ? (const double *)(lb + lay.rad) : p.radc, N);
 
     // ---------------- global inputs -------------------------------------
-    for (int k = tid; k < N; k += blockDim.x) {
-        if (hT) eT[k] = p.T_expl[is * F + off + k];
-        if (hX) eX[k] = p.X_expl[is * F + off + k];
-        eP[k] = p.P_expl[is * F + off + k];
-        usv[k] = p.us[off + k];
-        Tv[k] = hT ? p.temp[off + k] : mk(0, 0);
-        Xv[k] = hX ? p.xi[off + k] : mk(0, 0);
-        if (exp_calc) {
-            if (hT) aT[k] = p.dVsT[off + k];
-            if (hX) aX[k] = p.dVsXi[off + k];
-            aP[k] = p.dVsOm[off + k];
+    // three groups of N threads: T, xi, psi (three times as many loads in
+    // flight)
+    for (int t = tid; t < 3 * N; t += blockDim.x) {
+        const int g = t / N, k = t - g * N;
+        if (g == 0) {
+            if (hT) eT[k] = p.T_expl[is * F + off + k];
+            Tv[k] = hT ? p.temp[off + k] : mk(0, 0);
+            if (exp_calc && hT) aT[k] = p.dVsT[off + k];
+            if (hT) imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, PT[k], QT[k]);
+        } else if (g == 1) {
+            if (hX) eX[k] = p.X_expl[is * F + off + k];
+            Xv[k] = hX ? p.xi[off + k] : mk(0, 0);
+            if (exp_calc && hX) aX[k] = p.dVsXi[off + k];
+            if (hX) imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, PX[k], QX[k]);
+        } else {
+            eP[k] = p.P_expl[is * F + off + k];
+            usv[k] = p.us[off + k];
+            if (exp_calc) aP[k] = p.dVsOm[off + k];
+            imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, PP[k], QP[k]);
         }
-        if (hT) imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, PT[k], QT[k]);
-        if (hX) imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, PX[k], QX[k]);
-        imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, PP[k], QP[k]);
     }
     const double wcur = imex_weight(p, ss, istage);
 
@@ -755,8 +779,11 @@ __global__ void __launch_bounds__(PZ_MPHASE_THREADS) k_mcoll_batch(PzBatch b, in
 
 // get_start_fields, collocation branch (init_fields.f90:191-244): dT/dr,
 // omega from u_phi and u_s, and the first implicit state from grid values.
-__global__ void __launch_bounds__(256) k_start_coll(const PzDev p __attribute__((grid_constant)))
+// the descriptor comes through a pointer: the out-of-line transforms take
+// its address, which a by-value kernel argument would copy to scratch
+__global__ void __launch_bounds__(256) k_start_coll(const PzDev *__restrict__ pd)
 {
+    const PzDev &p = *pd;
     extern __shared__ double smem[];
     const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi], tid = threadIdx.x;
     const size_t off = (size_t)mi * N;

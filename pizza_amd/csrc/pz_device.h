@@ -142,26 +142,22 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
         const int nb = n / R;
         if (R != 2 && R != 3 && R != 4 && R != 8) {
             // other radices: a direct DFT of length R per output, one output
-            // per item so that all n outputs of a stage run in parallel; the
-            // stage twiddle and the DFT kernel combine into one table entry,
-            // exp(dir 2 pi i (r tstep + ((r k) mod R) n/R) / n)
-            // The exponent is kept as two running residues (r tstep mod n
-            // and r k mod R) so the tap loop has no integer division.
+            // per item so that all n outputs of a stage run in parallel.  The
+            // stage twiddle and the DFT kernel of tap r combine into
+            // exp(dir 2 pi i (r tstep + r k n/R) / n) = z^r with
+            // z = exp(dir 2 pi i (tstep + k n/R) / n), tstep + k n/R < n, so
+            // the output is the polynomial sum_r x_r z^r, evaluated by
+            // Horner's rule: one table entry and R complex FMAs per output.
             const int q = n / R;
             for (int t = threadIdx.x; t < n * ntr; t += blockDim.x) {
                 const int tr = t / n, rem = t - tr * n, k = rem / nb, j = rem - k * nb;
                 const cd *x = a + tr * n;
                 const int jm = j % Ns, tstep = (n / (Ns * R)) * jm;
-                cd acc = mk(0.0, 0.0);
-                int e1 = 0, rk = 0;
-                for (int r = 0; r < R; r++) {
-                    int e = e1 + rk * q;
-                    if (e >= n) e -= n;
-                    acc = cadd(acc, cmul(x[j + r * nb], tw(e)));
-                    e1 += tstep;
-                    if (e1 >= n) e1 -= n;
-                    rk += k;
-                    if (rk >= R) rk -= R;
+                const cd z = tw(tstep + k * q);
+                cd acc = x[j + (R - 1) * nb];
+                for (int r = R - 2; r >= 0; r--) {
+                    const cd v = x[j + r * nb];
+                    acc = mk(fma(acc.x, z.x, fma(-acc.y, z.y, v.x)), fma(acc.x, z.y, fma(acc.y, z.x, v.y)));
                 }
                 b[tr * n + (j / Ns) * Ns * R + jm + k * Ns] = acc;
             }
@@ -962,6 +958,98 @@ __device__ void wave_bordered_solve(int N, int nt, int kl, int ku, int ld, const
 // ---------------------------------------------------------------------------
 // Chebyshev transforms on LDS vectors, whole workgroup (dct_fftw.f90)
 // ---------------------------------------------------------------------------
+// Forward FFTs of length L = 64 P (P <= 4) held in registers, one wave per
+// transform (waves loop over the transforms): lane l holds x[j1 + P l],
+// j1 < P.  With j = j1 + P j2 and k = k2 + 64 k1,
+//   X[k] = sum_j1 w_P^(j1 k1) w_L^(j1 k2) sum_j2 w_64^(j2 k2) x[j1 + P j2]:
+// the inner 64-point DFTs run across the lanes as six radix-2
+// decimation-in-frequency stages with xor-shuffles (lane l then holds
+// k2 = bitreverse(l)), the outer P-point DFT in registers.  No barrier:
+// ld(tr, j) supplies the input, st(tr, k, X) takes each output once;
+// tw(q) = exp(-2 pi i q / L).  All twiddles of a lane are read before the
+// first stage, so their latencies overlap.
+// x of lane l ^ H within the wave, for a compile-time H: DPP quad
+// permutations for H = 1, 2, swizzles for H = 4, 8, 16, a permute for 32
+template <int H>
+__device__ __forceinline__ int xor_lane(int x)
+{
+    if constexpr (H == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false); // quad_perm(1,0,3,2)
+    else if constexpr (H == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false); // quad_perm(2,3,0,1)
+    else if constexpr (H <= 16) return __builtin_amdgcn_ds_swizzle(x, 0x1F | (H << 10));  // bit mode, xor H
+    else return __builtin_amdgcn_ds_bpermute(((int)(threadIdx.x & 63) ^ H) << 2, x);
+}
+template <int H>
+__device__ __forceinline__ double xor_lane_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = xor_lane<H>((int)b), hi = xor_lane<H>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// one radix-2 decimation-in-frequency stage of span H across the lanes
+template <int P, int H>
+__device__ __forceinline__ void dif_stage(cd (&v)[P], cd w, int lane)
+{
+    const bool up = (lane & H) != 0;
+#pragma unroll
+    for (int j1 = 0; j1 < P; j1++) {
+        const cd o = mk(xor_lane_d<H>(v[j1].x), xor_lane_d<H>(v[j1].y));
+        v[j1] = up ? cmul(csub(o, v[j1]), w) : cadd(v[j1], o);
+    }
+}
+
+template <int P, class LD, class ST, class TW>
+__device__ __forceinline__ void fft_wave(int ntr, LD ld, ST st, TW tw)
+{
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int k2 = (int)(__brev((unsigned)lane) >> 26);
+    cd ws[6], wk[P > 1 ? P - 1 : 1], wp[P > 1 ? P : 1];
+#pragma unroll
+    for (int s = 0; s < 6; s++) ws[s] = tw((lane & ((1 << s) - 1)) * (32 >> s) * P); // W_2h^t, h = 2^s
+#pragma unroll
+    for (int j1 = 1; j1 < P; j1++) wk[j1 - 1] = tw(j1 * k2);
+#pragma unroll
+    for (int q = 0; q < P; q++) wp[P > 1 ? q : 0] = tw(q * 64); // w_P^q
+    for (int tr = threadIdx.x >> 6; tr < ntr; tr += nw) {
+        cd v[P];
+#pragma unroll
+        for (int j1 = 0; j1 < P; j1++) v[j1] = ld(tr, j1 + P * lane);
+        dif_stage<P, 32>(v, ws[5], lane);
+        dif_stage<P, 16>(v, ws[4], lane);
+        dif_stage<P, 8>(v, ws[3], lane);
+        dif_stage<P, 4>(v, ws[2], lane);
+        dif_stage<P, 2>(v, ws[1], lane);
+        dif_stage<P, 1>(v, ws[0], lane);
+#pragma unroll
+        for (int j1 = 1; j1 < P; j1++) v[j1] = cmul(v[j1], wk[j1 - 1]);
+#pragma unroll
+        for (int k1 = 0; k1 < P; k1++) {
+            cd acc = v[0];
+#pragma unroll
+            for (int j1 = 1; j1 < P; j1++) acc = cadd(acc, cmul(v[j1], wp[(j1 * k1) % P]));
+            st(tr, k2 + 64 * k1, acc);
+        }
+    }
+}
+
+template <class LD, class ST, class TW>
+__device__ __forceinline__ void fft_wave_p(int P, int ntr, LD ld, ST st, TW tw)
+{
+    switch (P) {
+    case 1: fft_wave<1>(ntr, ld, st, tw); break;
+    case 2: fft_wave<2>(ntr, ld, st, tw); break;
+    case 3: fft_wave<3>(ntr, ld, st, tw); break;
+    default: fft_wave<4>(ntr, ld, st, tw); break;
+    }
+}
+
+// P of the register FFTs for this run's Chebyshev length, 0 when L is not
+// 64, 128, 192 or 256 (n_r 33, 65, 97, 129)
+__device__ __forceinline__ int wave_fft_p(const PzDev &p)
+{
+    return (p.L & 63) == 0 && p.L <= 256 ? p.L >> 6 : 0;
+}
+
 // costf (DCT-I with cheb_fac) of nf vectors: out[f][k]
 __device__ void dct_many(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int nf, int dealias,
                          const double *cosT = nullptr)
@@ -1098,6 +1186,27 @@ __device__ __forceinline__ void dct_fft(const PzDev &p, VecSet<const cd> in, Vec
         return;
     }
     const int N = p.N, L = p.L;
+    if (const int P = wave_fft_p(p)) {
+        // even extension in, costf out, straight from and to the vectors
+        // the vector pointers as plain values (a VecSet inside the closures
+        // would be kept in scratch memory)
+        const double cf = p.cheb_fac;
+        const int nc = dealias ? p.ncheb : N;
+        const cd *i0 = in.v0, *i1 = in.v1, *i2 = in.v2, *i3 = in.v3;
+        cd *o0 = out.v0, *o1 = out.v1, *o2 = out.v2, *o3 = out.v3;
+        fft_wave_p(
+            P, nf,
+            [=](int f, int j) __attribute__((always_inline)) {
+                const cd *x = f == 0 ? i0 : f == 1 ? i1 : f == 2 ? i2 : i3;
+                return lds_load(x, j < N ? j : L - j);
+            },
+            [=](int f, int k, cd z) __attribute__((always_inline)) {
+                cd *y = f == 0 ? o0 : f == 1 ? o1 : f == 2 ? o2 : o3;
+                if (k < N) lds_store(y, k, k >= nc ? mk(0.0, 0.0) : mk(cf * z.x, cf * z.y));
+            },
+            [=](int q) __attribute__((always_inline)) { return mk(cosT[q], -sinT[q]); });
+        return;
+    }
     cd *A = buf, *B = buf + (size_t)nf * L;
     for (int t = threadIdx.x; t < nf * L; t += blockDim.x) {
         const int f = t / L, j = t - f * L;
@@ -1132,22 +1241,38 @@ __device__ __forceinline__ void dr_fft(const PzDev &p, const RadView &R, VecSet<
     const bool second = ddf.v0 != nullptr, with_dct = dct_out != nullptr;
     const int N = p.N, L = p.L, nder = second ? 2 * nf : nf, ntr = nder + (with_dct ? 1 : 0);
     cd *A = buf, *B = buf + (size_t)ntr * L, *bnd = buf + (size_t)2 * ntr * L;
-    for (int t = threadIdx.x; t < ntr * L; t += blockDim.x) {
-        const int tr = t / L, j = t - tr * L;
+    // input j of transform tr: the odd extension of k w_k (tr < nf), the
+    // even extension of k^2 w_k (nf <= tr < nder), c itself (dct_out)
+    const cd *c0 = c.v0, *c1 = c.v1, *c2 = c.v2, *c3 = c.v3;
+    auto in_at = [=](int tr, int j) __attribute__((always_inline)) {
         const int f = tr < nf ? tr : (tr < nder ? tr - nf : 0);
         const int k = j < N ? j : L - j;
         cd y = mk(0.0, 0.0);
         if (tr >= nder) {
-            y = lds_load(c[0], k);
+            y = lds_load(c0, k);
         } else if (k != 0 && k != N - 1) {
-            const cd w = lds_load(c[f], k);
+            const cd w = lds_load(f == 0 ? c0 : f == 1 ? c1 : f == 2 ? c2 : c3, k);
             const double sc = tr < nf ? (j < N ? (double)k : -(double)k) : (double)k * (double)k;
             y = mk(sc * w.x, sc * w.y);
         }
-        A[t] = y;
+        return y;
+    };
+    const int P = wave_fft_p(p);
+    if (P) {
+        // transforms straight into A (natural order), one wave each
+        fft_wave_p(
+            P, ntr, in_at, [=](int tr, int k, cd z) __attribute__((always_inline)) { A[(size_t)tr * L + k] = z; },
+            [=](int q) __attribute__((always_inline)) { return mk(cosT[q], -sinT[q]); });
+    } else {
+        for (int t = threadIdx.x; t < ntr * L; t += blockDim.x) {
+            const int tr = t / L, j = t - tr * L;
+            A[t] = in_at(tr, j);
+        }
     }
     {
-        const int seg = threadIdx.x >> 5, hl = threadIdx.x & 31;
+        // half-waves counted from the last wave, away from the waves that
+        // run the register FFTs
+        const int seg = (int)((blockDim.x - 1 - threadIdx.x) >> 5), hl = threadIdx.x & 31;
         if (seg < 2 * nf) {
             const int f = seg >> 1, side = seg & 1; // side 0: x = 1 (j = 0), 1: x = -1 (j = N-1)
             const cd *w = c[f];
@@ -1181,7 +1306,7 @@ __device__ __forceinline__ void dr_fft(const PzDev &p, const RadView &R, VecSet<
         }
     }
     __syncthreads();
-    const cd *Z = fft_cheb(p, A, B, ntr, -1, cosT, sinT);
+    const cd *Z = P ? A : fft_cheb(p, A, B, ntr, -1, cosT, sinT);
     for (int t = threadIdx.x; t < nf * N; t += blockDim.x) {
         const int f = t / N, j = t - f * N;
         cd d1, d2;
@@ -1210,7 +1335,7 @@ __device__ __forceinline__ void dr_fft(const PzDev &p, const RadView &R, VecSet<
 // issued together), and two xor-shuffles combine the partial sums in a
 // fixed order.
 template <int K>
-__device__ void bandmv(const PzDev &p, const double *band, const cd *x, cd *y, int nzero)
+__device__ __forceinline__ void bandmv(const PzDev &p, const double *band, const cd *x, cd *y, int nzero)
 {
     constexpr int W = 2 * K + 1, G = 4, PER = (W + G - 1) / G;
     const int N = p.N, sub = threadIdx.x & (G - 1);
@@ -1241,7 +1366,7 @@ __device__ void bandmv(const PzDev &p, const double *band, const cd *x, cd *y, i
     }
 }
 template <int K>
-__device__ void bandmv_c(const PzDev &p, const cd *band, const cd *x, cd *y, int nzero)
+__device__ __forceinline__ void bandmv_c(const PzDev &p, const cd *band, const cd *x, cd *y, int nzero)
 {
     constexpr int W = 2 * K + 1, G = 4, PER = (W + G - 1) / G;
     const int N = p.N, sub = threadIdx.x & (G - 1);
@@ -1646,7 +1771,7 @@ __host__ __device__ inline size_t matvec_scratch(int nrow, int ncol, int threads
     return CG > 1 ? (size_t)nmat * (size_t)(CG - 1) * (size_t)R * (size_t)rpt : 0;
 }
 template <class E, int NMAT>
-__device__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y,
+__device__ __forceinline__ void block_matvec_rc(int nrow, int ncol, int ldi, VecSet<const E> inv, VecSet<cd> x, VecSet<cd> y,
                                 cd *red)
 {
     constexpr int RPT = sizeof(E) == 8 ? 2 : 1, U = 16 / NMAT;

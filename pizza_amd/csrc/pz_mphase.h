@@ -41,7 +41,7 @@ __device__ __forceinline__ double imex_weight(const PzDev &p, const PzScheme *s,
 }
 
 // rotate_imex for one m row (multistep_schemes.f90:462-498); DIRK: no-op
-__device__ void rotate_row(const PzDev &p, cd *old, cd *impl, cd *expl, size_t off)
+__device__ __forceinline__ void rotate_row(const PzDev &p, cd *old, cd *impl, cd *expl, size_t off)
 {
     if (p.family != 0) return;
     const size_t F = (size_t)p.nm * p.N;
@@ -70,16 +70,34 @@ __device__ __forceinline__ cd buoyancy(const PzDev &p, const RadView &R, int m, 
 
 // get_dcheb_real_1d with the rerror threshold (radial_derivatives.f90:143-185),
 // one thread: Chebyshev coefficients of d/dx from those of f
-__device__ void dcheb_real(const PzDev &p, const double *f, double *df)
+__device__ __forceinline__ void dcheb_real(const PzDev &p, const double *__restrict__ f, double *__restrict__ df)
 {
     const int N = p.N;
-    double mx = 0.0;
-    for (int j = 0; j < N; j++) mx = fmax(mx, fabs(f[j]));
-    const double thr = mx * p.thr;
+    // four independent maxima so that the loads overlap
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    int j = 0;
+    for (; j + 3 < N; j += 4) {
+        m0 = fmax(m0, fabs(f[j]));
+        m1 = fmax(m1, fabs(f[j + 1]));
+        m2 = fmax(m2, fabs(f[j + 2]));
+        m3 = fmax(m3, fabs(f[j + 3]));
+    }
+    for (; j < N; j++) m0 = fmax(m0, fabs(f[j]));
+    const double thr = fmax(fmax(m0, m1), fmax(m2, m3)) * p.thr;
+    // the recurrence df(n-1) = df(n+1) + 2 n f(n) carried in registers:
+    // up1 = df(n), up2 = df(n+1)
     df[N - 1] = 0.0;
     const int nc = N - 1;
-    df[nc - 1] = fabs(f[nc]) >= thr ? (double)nc * f[nc] : 0.0;
-    for (int n = N - 2; n >= 1; n--) df[n - 1] = fabs(f[n]) >= thr ? df[n + 1] + (double)(2 * n) * f[n] : 0.0;
+    double up1 = fabs(f[nc]) >= thr ? (double)nc * f[nc] : 0.0, up2 = 0.0;
+    df[nc - 1] = up1;
+#pragma unroll 4
+    for (int n = N - 2; n >= 1; n--) {
+        const double fn = f[n];
+        const double v = fabs(fn) >= thr ? up2 + (double)(2 * n) * fn : 0.0;
+        df[n - 1] = v;
+        up2 = up1;
+        up1 = v;
+    }
 }
 
 #define MPHASE_NVEC 24
@@ -96,7 +114,7 @@ struct MpLayout {
     // rad_lds = false: the radial rows and the transform tables stay in
     // global memory (read through L2), which frees PZ_NRADC n_r + 2 L
     // doubles of LDS per workgroup (collocation m-loop only)
-    __host__ __device__ MpLayout(int N, int threads, bool rad_lds = true)
+    __host__ __device__ __forceinline__ MpLayout(int N, int threads, bool rad_lds = true)
     {
         size_t o = 0;
         auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
@@ -156,30 +174,42 @@ __device__ __forceinline__ void mphase_body(const PzDev &p, int mi, int istage, 
     // ---------------- global inputs of this m in one phase ----------------
     // explicit-assembly inputs (or the current explicit slot) and the
     // right-hand-side terms that do not depend on this stage's explicit part
-    for (int k = tid; k < N; k += blockDim.x) {
+    // four groups of N threads: T, xi, psi's right-hand side, the velocity
+    // fields (so that four times as many loads are in flight)
+    for (int t = tid; t < 4 * N; t += blockDim.x) {
+        const int g = t / N, k = t - g * N;
         const size_t q = off + k, qe = is * F + q;
-        if (hT) lds_store(eT, k, gld(p.T_expl, qe));
-        if (hX) lds_store(eX, k, gld(p.X_expl, qe));
-        lds_store(eP, k, gld(p.P_expl, qe));
-        if (exp_calc) {
-            if (hT) lds_store(aT, k, gld(p.dVsT, q));
-            if (hX) lds_store(aX, k, gld(p.dVsXi, q));
-            lds_store(aP, k, gld(p.dVsOm, q));
+        cd a, b;
+        if (g == 0) {
+            if (hT) {
+                lds_store(eT, k, gld(p.T_expl, qe));
+                if (exp_calc) lds_store(aT, k, gld(p.dVsT, q));
+                imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, a, b);
+                lds_store(PT, k, a);
+                lds_store(QT, k, b);
+            }
+            if (exp_calc && !p.l_buo_imp) lds_store(Tv, k, hT ? gld(p.temp, q) : mk(0, 0));
+        } else if (g == 1) {
+            if (hX) {
+                lds_store(eX, k, gld(p.X_expl, qe));
+                if (exp_calc) lds_store(aX, k, gld(p.dVsXi, q));
+                imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, a, b);
+                lds_store(PX, k, a);
+                lds_store(QX, k, b);
+            }
+            if (exp_calc && !p.l_buo_imp) lds_store(Xv, k, hX ? gld(p.xi, q) : mk(0, 0));
+        } else if (g == 2) {
+            lds_store(eP, k, gld(p.P_expl, qe));
+            if (exp_calc) lds_store(aP, k, gld(p.dVsOm, q));
+            imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, a, b);
+            lds_store(PP, k, a);
+            lds_store(QP, k, b);
+        } else if (exp_calc) {
             lds_store(psiv, k, gld(p.psi, q));
             lds_store(omv, k, gld(p.om, q));
             lds_store(upv, k, gld(p.up, q));
             lds_store(usv, k, gld(p.us, q));
-            if (!p.l_buo_imp) {
-                lds_store(Tv, k, hT ? gld(p.temp, q) : mk(0, 0));
-                lds_store(Xv, k, hX ? gld(p.xi, q) : mk(0, 0));
-            }
         }
-        cd a, b;
-        if (hT) { imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, a, b); lds_store(PT, k, a); lds_store(QT, k, b); }
-        if (hX) { imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, a, b); lds_store(PX, k, a); lds_store(QX, k, b); }
-        imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, a, b);
-        lds_store(PP, k, a);
-        lds_store(QP, k, b);
     }
     const double wcur = imex_weight(p, ss, istage);
 

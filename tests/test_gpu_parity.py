@@ -53,6 +53,25 @@ def test_unit_operators():
     assert rel(g.fft(g.ifft(sp)), sp) < 1e-14            # round trip
 
 
+@pytest.mark.parametrize("m_max", [85, 170, 682])
+def test_fft_prime_radices(m_max):
+    """Azimuthal transforms whose n_phi has prime factors above 8, where
+    the stages evaluate a direct DFT per output by Horner's rule: n_phi 255
+    (3 5 17), 510 (2 3 5 17) and 2046 (2 3 11 31), against the oracle's
+    recursive FFT to 1e-14."""
+    from pizza_amd import workloads as W
+    cfg = W.c1_collocation()
+    cfg.m_max = m_max
+    g, o = _pair(cfg)
+    assert g.n_phi == 3 * m_max
+    rng = np.random.default_rng(m_max)
+    grid = rng.standard_normal((g.n_phi, 6))
+    assert rel(g.fft(grid), o.fft(grid)) < 1e-14
+    sp = o.fft(grid)
+    assert rel(g.ifft(sp), o.ifft(sp)) < 1e-14
+    assert rel(g.fft(g.ifft(sp)), sp) < 1e-14
+
+
 def test_rotinteg_parity_and_golden():
     """Reference sample RotInteg (integration, CNAB2): fields vs the oracle
     after 1 and 100 steps, and the e_kin_3D series vs the reference's
