@@ -1844,7 +1844,9 @@ int or_create(const pz_config *cfg, or_handle **out)
 {
     *out = NULL;
     if (!cfg->l_direct_solve) return -11;
-    if (cfg->l_non_rot) return -12;
+    /* non-rotating runs change the integration operators (h^2 = 1); they
+       are restated for collocation and finite differences */
+    if (cfg->l_non_rot && !cfg->l_cheb_coll && !cfg->l_finite_diff) return -12;
     if (cfg->radratio == 0.0) return -13;
     or_handle *h = calloc(1, sizeof(or_handle));
     h->c = *cfg;
@@ -1869,8 +1871,17 @@ int or_create(const pz_config *cfg, or_handle **out)
     h->tadvz = cfg->l_temp_advz ? 1.0 : 0.0;
     h->r_cmb = 1.0 / (1.0 - cfg->radratio);
     h->r_icb = h->r_cmb - 1.0;
-    h->ek = cfg->ek;
-    if (cfg->time_scale_rot) {
+    /* namelists.f90:312-315: ek = -1 flags a non-rotating run */
+    h->ek = cfg->l_non_rot ? -1.0 : cfg->ek;
+    if (cfg->l_non_rot) {
+        /* namelists.f90:473-489 */
+        h->CorFac = 0.0;
+        h->BuoFac = h->l_heat ? cfg->ra / cfg->pr : 0.0;
+        h->TdiffFac = h->l_heat ? 1.0 / cfg->pr : 0.0;
+        h->ChemFac = h->l_chem ? cfg->raxi / cfg->sc : 0.0;
+        h->XidiffFac = h->l_chem ? 1.0 / cfg->sc : 0.0;
+        h->ViscFac = 1.0;
+    } else if (cfg->time_scale_rot) {
         h->CorFac = 2.0;
         h->BuoFac = h->l_heat ? cfg->ra * cfg->ek * cfg->ek / cfg->pr : 0.0;
         h->TdiffFac = h->l_heat ? cfg->ek / cfg->pr : 0.0;
@@ -1963,6 +1974,12 @@ int or_create(const pz_config *cfg, or_handle **out)
         h->ekp_us[n] = -5.0 * rc * h->beta[n] * h->oheight[n];
         h->ekp_dusdp[n] = h->beta[n];
     }
+    if (cfg->l_non_rot)
+        /* radial.f90:197-207 */
+        for (int n = 0; n < N; n++) {
+            h->beta[n] = h->ekpump[n] = h->ekp_up[n] = h->ekp_us[n] = h->ekp_dusdp[n] = 0.0;
+            h->height[n] = h->oheight[n] = 1.0;
+        }
     /* horizontal.f90:69-173 (no hyperdiffusion) */
     h->hdifT = dalloc(h->nm); h->hdifXi = dalloc(h->nm); h->hdifV = dalloc(h->nm);
     for (int i = 0; i < h->nm; i++) h->hdifT[i] = h->hdifXi[i] = h->hdifV[i] = 1.0;
@@ -2718,7 +2735,7 @@ static void coll_alloc(or_handle *h)
     int N = h->N, N2 = 2 * N;
     size_t nm = h->nm;
     h->dbeta = dalloc(N);
-    for (int n = 1; n < N; n++) {
+    for (int n = 1; n < N && !h->c.l_non_rot; n++) {
         double s2 = h->r_cmb * h->r_cmb - h->r[n] * h->r[n];
         h->dbeta[n] = -(h->r_cmb * h->r_cmb + h->r[n] * h->r[n]) / (s2 * s2); /* radial.f90:222 */
     }
@@ -3258,7 +3275,7 @@ static void fd_alloc(or_handle *h)
     h->or3 = dalloc(N); h->dbeta = dalloc(N); h->d2beta = dalloc(N); h->d3beta = dalloc(N);
     for (int n = 0; n < N; n++) h->or3[n] = h->or2[n] * h->or1[n];
     /* radial.f90:212-226 (sphere container; zero at r_cmb) */
-    for (int n = 1; n < N; n++) {
+    for (int n = 1; n < N && !h->c.l_non_rot; n++) {
         double r = h->r[n], s2 = rc * rc - r * r, hh = h->height[n];
         h->dbeta[n] = -(rc * rc + r * r) / (s2 * s2);
         h->d2beta[n] = -128.0 * r * (3.0 * rc * rc + r * r) / pow(hh, 6);
@@ -4300,7 +4317,7 @@ int or_read_checkpoint(or_handle *h, const char *path)
         /* restart flags (checkpoints.f90:398-434) */
         tscheme *t = &h->ts;
         double ek_old = par[4];
-        h->l_ab1 = ek_old != h->c.ek;
+        h->l_ab1 = ek_old != h->ek;
         if (t->family == 0) {
             if (fam_dirk) h->l_bridge = 1;
             else h->l_bridge = (t->nold > nold_old || t->nimp > nimp_old);
@@ -4365,7 +4382,7 @@ int or_write_checkpoint(or_handle *h, const char *path)
     int nen[3] = {t->nexp, t->nimp, t->nold};
     fwrite(nen, 4, 3, f);
     fwrite(t->dt, 8, t->family == 0 ? (size_t)t->nexp : 1, f);
-    double par[7] = {h->c.ra, h->c.pr, h->c.raxi, h->c.sc, h->c.ek, 0.0, h->c.radratio};
+    double par[7] = {h->c.ra, h->c.pr, h->c.raxi, h->c.sc, h->ek, 0.0, h->c.radratio};
     fwrite(par, 8, 7, f);
     int dims[3] = {N, h->mmax, h->minc};
     fwrite(dims, 4, 3, f);
@@ -4695,7 +4712,11 @@ int or_time_series(or_handle *h, double *out, double *spec)
         double E = 0, Em = 0;
         for (int i = 0; i < nm; i++) { E += S[i] + S[nm + i]; Em += (double)h->idx2m[i] * (S[i] + S[nm + i]); }
         L[3] = fabs(E) > tiny ? pi * E / Em : 0.0;
-        L[4] = fabs(visc_3D) > tiny ? sqrt(2.0 * (us2_3D + up2_3D) * h->ViscFac / visc_3D) : 0.0;
+        /* outputs.f90:697-709: the 2D dissipation length of a non-rotating run */
+        if (h->c.l_non_rot)
+            L[4] = fabs(visc_2D) > tiny ? sqrt(2.0 * (us2_2D + up2_2D) * h->ViscFac / visc_2D) : 0.0;
+        else
+            L[4] = fabs(visc_3D) > tiny ? sqrt(2.0 * (us2_3D + up2_3D) * h->ViscFac / visc_3D) : 0.0;
     }
     double heat[7] = {0}, chem[6] = {0};
     int mid = N / 2 - 1;
@@ -4714,7 +4735,13 @@ int or_time_series(or_handle *h, double *out, double *spec)
             th[k] = asin(sr);
             f2[k] = flux[k] * sr * sr;
         }
-        heat[3] = round_off(simps_x(N, f2, th) / simps_x(N, f1, th));
+        if (h->c.l_non_rot) {
+            /* outputs.f90:763-767 */
+            double fac = INTEG(-h->TdiffFac * h->dtcond[k] * r[k]);
+            heat[3] = round_off(INTEG(flux[k] * r[k] / fac));
+        } else {
+            heat[3] = round_off(simps_x(N, f2, th) / simps_x(N, f1, th));
+        }
         heat[6] = round_off(h->dtcond[mid] + creal(dT[mid]));
         free(th); free(f1); free(f2);
     }
@@ -4738,6 +4765,11 @@ int or_time_series(or_handle *h, double *out, double *spec)
                        L[0], L[1], L[2], L[3], L[4],
                        heat[0], heat[1], heat[2], heat[3], heat[4], heat[5], heat[6],
                        chem[0], chem[1], chem[2], chem[3], chem[4], chem[5]};
+    if (h->c.l_non_rot) {
+        /* a non-rotating run writes no 3D series (outputs.f90:102-121, 552, 580, 620, 685) */
+        static const int z3[] = {4, 5, 6, 7, 13, 14, 15, 16, 17, 21, 22, 23};
+        for (size_t q = 0; q < sizeof(z3) / sizeof(z3[0]); q++) vals[z3[q]] = 0.0;
+    }
     memcpy(out, vals, sizeof(vals));
     if (spec) memcpy(spec, S, sizeof(double) * 5 * nm);
     free(S); free(P); free(tmp); free(dT); free(dX);

@@ -937,9 +937,18 @@ static void setup(pz_handle *h)
         d.XidiffFac = h->l_chem ? 1.0 / c.sc : 0.0;
         d.ViscFac = 1.0;
     }
+    if (c.l_non_rot) {
+        // namelists.f90:473-489
+        d.CorFac = 0.0;
+        d.BuoFac = h->l_heat ? c.ra / c.pr : 0.0;
+        d.TdiffFac = h->l_heat ? 1.0 / c.pr : 0.0;
+        d.ChemFac = h->l_chem ? c.raxi / c.sc : 0.0;
+        d.XidiffFac = h->l_chem ? 1.0 / c.sc : 0.0;
+        d.ViscFac = 1.0;
+    }
     d.tadvz = c.l_temp_advz ? 1.0 : 0.0;
     d.r_cmb = h->r_cmb;
-    d.ek = c.ek;
+    d.ek = c.l_non_rot ? -1.0 : c.ek; // namelists.f90:312-315: ek = -1 flags a non-rotating run
     d.alpha = c.alpha;
     d.dtMax = c.dtMax;
     d.dtMin = c.dtMax / 1.0e6; // pre_calculations.f90:30
@@ -992,6 +1001,13 @@ static void setup(pz_handle *h)
         ekp_us[n] = -5.0 * rc * beta[n] * oheight[n];
         ekp_dusdp[n] = beta[n];
     }
+    if (c.l_non_rot)
+        // radial.f90:197-207
+        for (int n = 0; n < N; n++) {
+            beta[n] = dbeta[n] = d2beta[n] = d3beta[n] = 0.0;
+            ekpump[n] = ekp_up[n] = ekp_us[n] = ekp_dusdp[n] = 0.0;
+            height[n] = oheight[n] = 1.0;
+        }
     h->height = height;
     h->beta = beta;
     const int mR = c.m_max;
@@ -1226,7 +1242,8 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
     const pz_config &c = *cfg;
     std::string why;
     if (!c.l_direct_solve) why = "only matrix_solve='direct' is implemented";
-    else if (c.l_non_rot) why = "non-rotating runs are not implemented";
+    else if (c.l_non_rot && !c.l_cheb_coll && !c.l_finite_diff)
+        why = "non-rotating runs are implemented for the collocation and finite-difference methods";
     else if (c.radratio <= 0.0) why = "full-disk geometry is not implemented";
     else if (c.n_r_max < 8 || c.n_r_max > 257) why = "n_r_max must be in [8, 257]";
     else if (c.l_cheb_coll && !c.l_finite_diff && c.n_r_max > 256) why = "the collocation method needs n_r_max <= 256";
@@ -1385,7 +1402,7 @@ extern "C" int pz_write_checkpoint(pz_handle *h, const char *path)
         w.put<int>(sc.nimp);
         w.put<int>(sc.nold);
         for (int i = 0; i < (sc.family == 0 ? sc.nexp : 1); i++) w.put<double>(s.dt[i]);
-        for (double v : {h->c.ra, h->c.pr, h->c.raxi, h->c.sc, h->c.ek, 0.0 /* stef */, h->c.radratio}) w.put<double>(v);
+        for (double v : {h->c.ra, h->c.pr, h->c.raxi, h->c.sc, h->d.ek, 0.0 /* stef */, h->c.radratio}) w.put<double>(v);
         w.put<int>(h->N);
         w.put<int>(h->mmax);
         w.put<int>(h->minc);
@@ -1446,7 +1463,7 @@ extern "C" int pz_read_checkpoint(pz_handle *h, const char *path)
         // restart flags (checkpoints.f90:398-434)
         int nimp_old = hd.nimp, nold_old = hd.nold;
         if (hd.version <= 5) { nimp_old--; nold_old--; }
-        h->rst.ab1 = hd.ek != h->c.ek;
+        h->rst.ab1 = hd.ek != h->d.ek;
         if (sc.family == 0) {
             h->rst.bridge = hd.dirk() ? 1 : (sc.nold > nold_old || sc.nimp > nimp_old);
             if (hd.l_coll != (h->c.l_cheb_coll != 0)) h->rst.ab1 = h->rst.bridge = 1;
@@ -1574,7 +1591,7 @@ extern "C" int pz_write_frame(pz_handle *h, int field, const char *path)
         HIPCHK(hipMemcpy(xc.data(), h->d.xicond, sizeof(double) * h->N, hipMemcpyDeviceToHost));
         pzck::Writer w(path);
         w.put<int>(2);
-        for (double x : {h->started ? s.time : 0.0, h->c.ra, h->c.ek, h->c.pr, h->c.radratio, h->c.sc, h->c.raxi})
+        for (double x : {h->started ? s.time : 0.0, h->c.ra, h->d.ek, h->c.pr, h->c.radratio, h->c.sc, h->c.raxi})
             w.put<double>(x);
         for (int x : {h->N, h->nm, h->mmax, h->minc, h->nphi}) w.put<int>(x);
         w.raw(h->r.data(), sizeof(double) * h->N);
@@ -2231,7 +2248,11 @@ extern "C" int pz_time_series(pz_handle *h, pz_series *ts)
         double E = 0.0, Em = 0.0;
         for (int i = 0; i < nm; i++) { E += ek[i]; Em += (double)h->idx2m[i] * ek[i]; }
         ts->lengths[3] = std::fabs(E) > tiny ? pi * E / Em : 0.0;
-        ts->lengths[4] = std::fabs(visc_3D) > tiny ? std::sqrt(2.0 * (us2_3D + up2_3D) * ViscFac / visc_3D) : 0.0;
+        // a non-rotating run takes the 2D dissipation length (:697-709)
+        if (h->c.l_non_rot)
+            ts->lengths[4] = std::fabs(visc_2D) > tiny ? std::sqrt(2.0 * (us2_2D + up2_2D) * ViscFac / visc_2D) : 0.0;
+        else
+            ts->lengths[4] = std::fabs(visc_3D) > tiny ? std::sqrt(2.0 * (us2_3D + up2_3D) * ViscFac / visc_3D) : 0.0;
         // heat and composition (:720-828), m = 0 at index 0
         const int i0 = 0, mid = N / 2 - 1; // int(n_r_max/2), 1-based
         if (h->l_heat) {
@@ -2250,7 +2271,13 @@ extern "C" int pz_time_series(pz_handle *h, pz_series *ts)
                 th[k] = std::asin(sr);
                 f2[k] = flux[k] * sr * sr;
             }
-            ts->heat[3] = ro(simps_x(f2, th) / simps_x(f1, th));
+            if (h->c.l_non_rot) {
+                // :763-767
+                const double fac = integ([&](int k) { return -TdiffFac * dtcond[k] * r[k]; });
+                ts->heat[3] = ro(integ([&](int k) { return flux[k] * r[k] / fac; }));
+            } else {
+                ts->heat[3] = ro(simps_x(f2, th) / simps_x(f1, th));
+            }
             ts->heat[6] = ro(dtcond[mid] + dT[(size_t)i0 * N + mid].x);
         }
         if (h->l_chem) {
@@ -2263,6 +2290,12 @@ extern "C" int pz_time_series(pz_handle *h, pz_series *ts)
             const double sc = integ([&](int k) { return dxicond[k] * dxicond[k] * hg[k] * r[k]; });
             ts->chem[2] = ro(1.0 + sv / sc);
             ts->chem[5] = ro(dxicond[mid] + dX[(size_t)i0 * N + mid].x);
+        }
+        if (h->c.l_non_rot) {
+            // a non-rotating run writes no 3D series (outputs.f90:102-121)
+            for (double &v : ts->ekin_3d) v = 0.0;
+            for (double &v : ts->power_3d) v = 0.0;
+            for (double &v : ts->reynolds_3d) v = 0.0;
         }
         // the m-spectra of the same call (calculate_spectra): us2, up2,
         // enstrophy, T^2 and xi^2 per m, kept for pz_spectra

@@ -539,10 +539,12 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
         }
     }
     const double wcur = imex_weight(p, ss, istage);
+    // the groups above wrote vectors that other threads read next, with or
+    // without the explicit part
+    __syncthreads();
 
     // ---------------- explicit part --------------------------------------
     if (exp_calc) {
-        __syncthreads();
         {
             VecSet<const cd> din;
             VecSet<cd> dout;

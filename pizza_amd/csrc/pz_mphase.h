@@ -212,10 +212,12 @@ __device__ __forceinline__ void mphase_body(const PzDev &p, int mi, int istage, 
         }
     }
     const double wcur = imex_weight(p, ss, istage);
+    // the groups above wrote vectors that other threads read next, with or
+    // without the explicit part
+    __syncthreads();
 
     // ---------------- explicit assembly -----------------------------------
     if (exp_calc) {
-        __syncthreads();
         VecSet<const cd> din;
         VecSet<cd> dout;
         if (hT) { din.push(aT); dout.push(cT); }

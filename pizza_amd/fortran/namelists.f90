@@ -72,7 +72,7 @@ module namelists
    &                       l_2D_spectra, l_2D_SD, l_corr,               &
    &                       l_energy_trans, m_max_transfer
 
-   public :: read_namelists, fill_config, check_supported
+   public :: read_namelists, fill_config, check_supported, has
 
 contains
 

@@ -5,9 +5,10 @@
 ! (init_fields.f90:260-509) or reads its start_file checkpoint
 ! (read_checkpoint, checkpoints.f90:185-707), steps through libpizza_amd and
 ! writes the time series of get_time_series in the reference's files and
-! formats (outputs.f90:672-717, 775-777, 824-826: e_kin_2D, e_kin_3D,
-! power_2D, power_3D, reynolds_2D, reynolds_3D, length_scales, heat, chem)
-! on every log step, the frame files
+! formats (outputs.f90:79-121, 672-717, 775-777, 824-826: e_kin, e_kin_3D,
+! power, power_3D, reynolds or rossby, reynolds_3D or rossby_3D,
+! length_scales, heat, composition; no 3D file in a non-rotating run) on
+! every log step, the frame files
 ! frame_<field>_<n>.TAG on the frame steps (outputs.f90:293-312) and the
 ! checkpoint_end.TAG file at the end of the run (step_time.f90:190-193,
 ! checkpoints.f90:33-183), as time_loop does (l_correct_step,
@@ -83,15 +84,19 @@ contains
                                 real(amp_xi, c_double), int(init_u, c_int), real(amp_u, c_double)), 'pz_init_fields')
       call check(pz_start(h), 'pz_start')
 
-      open(newunit=u, file='e_kin_3D.'//trim(tag), status='replace', action='write')
+      ! a non-rotating run writes no 3D series (outputs.f90:102-121)
+      u = 0
+      if (.not. l_non_rot) open(newunit=u, file='e_kin_3D.'//trim(tag), status='replace', action='write')
       call open_series()
       n_done = 0
       n_frame_out = 0
       do
          if (l_correct_step(n_done, n_steps, n_log_step, 0) .or. n_done == 0) then
             call check(pz_time(h, time, dt), 'pz_time')
-            call check(pz_ekin_3d(h, e), 'pz_ekin_3d')
-            write(u, '(1P, es20.12, 4es16.8)') time, e(1), e(2), e(3), e(4)
+            if (u /= 0) then
+               call check(pz_ekin_3d(h, e), 'pz_ekin_3d')
+               write(u, '(1P, es20.12, 4es16.8)') time, e(1), e(2), e(3), e(4)
+            end if
             call write_series()
          end if
          if (l_correct_step(n_done, n_steps, n_frame_step, n_frames)) call write_frames()
@@ -106,7 +111,7 @@ contains
          end if
          n_done = n_done + chunk
       end do
-      close(u)
+      if (u /= 0) close(u)
       do i = 1, size(uf)
          if (uf(i) /= 0) close(uf(i))
       end do
@@ -115,16 +120,23 @@ contains
 
    end subroutine run
 
-   ! the time-series files of outputs.f90:110-139 (heat and chem only with
-   ! the corresponding field)
+   ! the time-series files under the names of outputs.f90:79-121: rossby
+   ! instead of reynolds with the rotational time scale, heat and composition
+   ! only with the corresponding field, the 3D files only when rotating
    subroutine open_series()
-      character(len=16), parameter :: names(8) = [character(len=16) :: 'e_kin_2D', 'power_2D', 'power_3D', &
-         'reynolds_2D', 'reynolds_3D', 'length_scales', 'heat', 'chem']
+      character(len=16) :: names(8)
       integer :: k
+      names = [character(len=16) :: 'e_kin', 'power', 'power_3D', 'reynolds', 'reynolds_3D', &
+               'length_scales', 'heat', 'composition']
+      if (has(time_scale, 'ROT')) then
+         names(4) = 'rossby'
+         names(5) = 'rossby_3D'
+      end if
       uf = 0
       do k = 1, size(names)
          if (k == 7 .and. ra == 0.0_c_double) cycle
          if (k == 8 .and. raxi == 0.0_c_double) cycle
+         if ((k == 3 .or. k == 5) .and. l_non_rot) cycle
          open(newunit=uf(k), file=trim(names(k))//'.'//trim(tag), status='replace', action='write')
       end do
    end subroutine open_series
@@ -134,9 +146,9 @@ contains
       call check(pz_time_series(h, ts), 'pz_time_series')
       write(uf(1), '(1P, es20.12, 3es16.8)') ts%time, ts%ekin_2d
       write(uf(2), '(1P, es20.12, 5es16.8)') ts%time, ts%power_2d
-      write(uf(3), '(1P, es20.12, 5es16.8)') ts%time, ts%power_3d
+      if (uf(3) /= 0) write(uf(3), '(1P, es20.12, 5es16.8)') ts%time, ts%power_3d
       write(uf(4), '(1P, es20.12, 3es16.8)') ts%time, ts%reynolds_2d
-      write(uf(5), '(1P, es20.12, 3es16.8)') ts%time, ts%reynolds_3d
+      if (uf(5) /= 0) write(uf(5), '(1P, es20.12, 3es16.8)') ts%time, ts%reynolds_3d
       write(uf(6), '(1P, es20.12, 5es16.8)') ts%time, ts%lengths
       if (uf(7) /= 0) write(uf(7), '(1P, ES20.12, 7ES16.8)') ts%time, ts%heat
       if (uf(8) /= 0) write(uf(8), '(1P, ES20.12, 6ES16.8)') ts%time, ts%chem

@@ -14,7 +14,10 @@ reference.out at the reference tests' rtol 1e-8
   every step logged: restarts that keep, extend and shorten the multistep
   history;
 - TestRestart: a restart from n_r 32, m_max 32 onto n_r 36, m_max 64;
-- InhomogeneousHeatFlux: fixed-flux walls with an m = 2 outer flux pattern.
+- InhomogeneousHeatFlux: fixed-flux walls with an m = 2 outer flux pattern;
+- NonRotRa2e3: a non-rotating collocation run (CorFac = 0, no beta, no
+  pumping), compared on its 2D kinetic energies (e_kin.TAG), the file its
+  unitTest reads.
 
 On the CPU through the oracle; on the GPU through the library and, for
 the longest chains, its Fortran driver.
@@ -26,7 +29,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from helpers import GOLDEN, ROOT, SAMPLES, replay_sample
+from helpers import EKIN_2D, GOLDEN, ROOT, SAMPLES, replay_sample
 
 NAMES = list(SAMPLES)
 
@@ -50,11 +53,12 @@ def test_sample_gpu(name, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["RotE1e3EkPump", "DIRKSchemes"])
+@pytest.mark.parametrize("name", ["RotE1e3EkPump", "DIRKSchemes", "NonRotRa2e3"])
 def test_sample_fortran_driver(name, tmp_path):
     """pizza_gpu in place of pizza.exe: the input files run one after the
     other in one directory, each reading the checkpoint_end file the
-    previous one wrote; the concatenated e_kin_3D files match reference.out."""
+    previous one wrote; the concatenated e_kin_3D files (e_kin for a
+    non-rotating sample, which writes no 3D file) match reference.out."""
     driver = os.path.join(ROOT, "pizza_amd", "pizza_gpu")
     files, compared = SAMPLES[name]
     base = os.path.join(GOLDEN, "samples", name)
@@ -67,15 +71,22 @@ def test_sample_fortran_driver(name, tmp_path):
         assert out.returncode == 0, out.stdout + out.stderr
         from pizza_amd.config import config_from_namelist
         tags.append(config_from_namelist(str(tmp_path / f))[2]["tag"])
-    got = np.concatenate([np.loadtxt(tmp_path / f"e_kin_3D.{t}") for t in tags])
+    kin = "e_kin" if name in EKIN_2D else "e_kin_3D"
+    got = np.concatenate([np.loadtxt(tmp_path / f"{kin}.{t}") for t in tags])
     ref = np.loadtxt(os.path.join(base, "reference.out"))
     assert got.shape == ref.shape
     np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-20)
     for t in tags:
         assert (tmp_path / f"frame_us_1.{t}").exists() and (tmp_path / f"checkpoint_end.{t}").exists()
         # the other time series share the log steps of e_kin_3D
-        n = len(np.loadtxt(tmp_path / f"e_kin_3D.{t}"))
-        for series in ("e_kin_2D", "power_2D", "power_3D", "reynolds_2D", "reynolds_3D", "length_scales", "heat"):
-            rows = np.loadtxt(tmp_path / f"{series}.{t}", ndmin=2)
-            assert rows.shape[0] == n, (series, t)
-            assert np.array_equal(rows[:, 0], np.loadtxt(tmp_path / f"e_kin_3D.{t}")[:, 0])
+        n = len(np.loadtxt(tmp_path / f"{kin}.{t}"))
+        # the reference's file names (outputs.f90:79-121)
+        series = ["e_kin", "power", "reynolds", "length_scales", "heat"]
+        if name not in EKIN_2D:
+            series += ["power_3D", "reynolds_3D"]
+        else:
+            assert not (tmp_path / f"e_kin_3D.{t}").exists()
+        for sname in series:
+            rows = np.loadtxt(tmp_path / f"{sname}.{t}", ndmin=2)
+            assert rows.shape[0] == n, (sname, t)
+            assert np.array_equal(rows[:, 0], np.loadtxt(tmp_path / f"{kin}.{t}")[:, 0])

@@ -90,14 +90,36 @@ def test_xi_spectrum_is_the_temperature_profile():
     assert np.array_equal(spec[4], spec[3]) and spec[3].max() > 0
 
 
+def test_non_rotating_series_have_no_3d_part():
+    """A non-rotating run (namelists.f90:312-315, 473-489; radial.f90:
+    197-207) writes no 3D series: its 3D entries are zero, its dissipation
+    length is the 2D one (outputs.f90:697-709) and its surface Nusselt
+    number the plain radial integral (:763-767)."""
+    from oracle.oracle import Oracle
+    from pizza_amd import workloads as W
+    cfg = _cfg("coll", l_non_rot=True)
+    o = Oracle(cfg)
+    W.seed_run(o, cfg, amp_t=1e-3, amp_xi=1e-3)
+    o.start()
+    o.step(3)
+    ts, _ = o.time_series()
+    for k in ("ekin_3d", "power_3d", "rey_3d"):
+        assert not ts[SL[k]].any(), k
+    us2, up2 = ts[SL["ekin_2d"]][:2]
+    visc = ts[SL["power_2d"]][2]
+    assert abs(ts[SL["lengths"]][4] - np.sqrt(2.0 * (us2 + up2) / visc)) < 1e-12 * ts[SL["lengths"]][4]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("method", ["coll", "integ", "fd"])
+@pytest.mark.parametrize("method", ["coll", "integ", "fd", "coll-nonrot", "fd-nonrot"])
 def test_time_series_gpu_matches_oracle(method):
     from oracle.oracle import Oracle
     from pizza_amd import workloads as W
     from pizza_amd.runtime import Run
-    cfg = _cfg(method, n_r=33 if method != "fd" else 65)
-    print(f"case: time series {method}", flush=True)
+    nonrot = method.endswith("-nonrot")
+    method = method.split("-")[0]
+    cfg = _cfg(method, n_r=33 if method != "fd" else 65, l_non_rot=nonrot)
+    print(f"case: time series {method} non-rotating {nonrot}", flush=True)
     g, o = Run(cfg), Oracle(cfg)
     for x in (g, o):
         W.seed_run(x, cfg, amp_t=1e-3, amp_xi=1e-3)
