@@ -44,8 +44,10 @@ static cplx *tl_cbuf(int k, size_t n)
  * 2 drops its factor r for m /= 0,
  * 3 drops its u_s part (ekp_dusdp, ekp_us),
  * 4 flips the sign of the compositional buoyancy (ChemFac),
- * 5 doubles the compositional diffusivity (XidiffFac).
- * Read when a run is created (4, 5) or in the explicit finish (1-3). */
+ * 5 doubles the compositional diffusivity (XidiffFac),
+ * 6 flips the sign of the collocation influence-matrix method's explicit
+ *   Ekman-pumping term (update_psi_coll_dmat.f90:527-549).
+ * Read when a run is created (4, 5) or in the explicit finish (1-3, 6). */
 int or_plant = 0;
 
 static double *tl_dbuf(size_t n)
@@ -233,6 +235,12 @@ struct or_handle {
     int *cTpiv, *cXpiv, *cUpiv, *cPpiv;
     cplx *cPmat;
     double *cPfac;
+    /* influence-matrix psi solve (update_psi_coll_dmat.f90): per m the
+     * omega and psi matrices with their row/column scalings, the two
+     * homogeneous solutions of each (Chebyshev space) and the 2 x 2
+     * influence matrix */
+    double *dOmat, *dOfac, *dPmat, *dPfac, *dom2, *dom3, *dpsi2, *dpsi3, *dinfl;
+    int *dOpiv, *dPpiv;
     /* scratch */
     int wbig; /* length of the per-thread transform buffers (tl_cbuf) */
 };
@@ -1843,13 +1851,17 @@ static void fd_alloc(or_handle *h);
 int or_create(const pz_config *cfg, or_handle **out)
 {
     *out = NULL;
-    if (!cfg->l_direct_solve) return -11;
+    /* the influence-matrix psi solve is restated for collocation */
+    if (!cfg->l_direct_solve && (!cfg->l_cheb_coll || cfg->l_finite_diff)) return -11;
     /* non-rotating runs change the integration operators (h^2 = 1); they
        are restated for collocation and finite differences */
     if (cfg->l_non_rot && !cfg->l_cheb_coll && !cfg->l_finite_diff) return -12;
     if (cfg->radratio == 0.0) return -13;
     or_handle *h = calloc(1, sizeof(or_handle));
     h->c = *cfg;
+    /* namelists.f90:448-457: the influence-matrix method treats Coriolis
+       explicitly */
+    if (!cfg->l_direct_solve && !cfg->l_non_rot) h->c.l_coriolis_imp = 0;
     h->l_bridge = 1; /* namelists.f90:365: a fresh start bridges with CNAB2 */
     int N = h->N = cfg->n_r_max;
     h->mmax = cfg->m_max;
@@ -1865,6 +1877,8 @@ int or_create(const pz_config *cfg, or_handle **out)
     if (ts_init(&h->ts, cfg->time_scheme, cfg->courfac)) { free(h); return -14; }
     /* the assembly stage is restated for collocation and finite differences */
     if (h->ts.l_assembly && !h->coll && !h->fd) { free(h); return -15; }
+    /* the assembly stage of the influence-matrix method is not restated */
+    if (h->ts.l_assembly && !cfg->l_direct_solve) { free(h); return -15; }
     /* namelists.f90:398-526 */
     h->l_heat = cfg->ra != 0.0;
     h->l_chem = cfg->raxi != 0.0;
@@ -2701,6 +2715,133 @@ static int coll_uphi_mat(or_handle *h, double *A, int *piv)
     return info;
 }
 
+/* row then column max-scaling of a column-major n x n matrix
+ * (update_psi_coll_dmat.f90:880-896): fac[0..n) rows, fac[n..2n) columns */
+static void scale_rows_cols(int n, double *A, double *fac)
+{
+    for (int r = 0; r < n; r++) {
+        double mx = 0.0;
+        for (int c = 0; c < n; c++) mx = fmax(mx, fabs(A[r + (size_t)c * n]));
+        fac[r] = 1.0 / mx;
+        for (int c = 0; c < n; c++) A[r + (size_t)c * n] *= fac[r];
+    }
+    for (int c = 0; c < n; c++) {
+        double mx = 0.0;
+        for (int r = 0; r < n; r++) mx = fmax(mx, fabs(A[r + (size_t)c * n]));
+        fac[n + c] = 1.0 / mx;
+        for (int r = 0; r < n; r++) A[r + (size_t)c * n] *= fac[n + c];
+    }
+}
+
+/* get_omMat of the influence-matrix method (update_psi_coll_dmat.f90:
+ * 829-911): omega = 0 rows at both walls, the implicit viscous operator
+ * inside, boundary_fac columns, row and column scalings, dgetrf */
+static int coll_om_mat_dmat(or_handle *h, int i, double *A, double *fac, int *piv)
+{
+    int N = h->N, nc = h->ncheb, m = h->idx2m[i];
+    double wimp = h->ts.wimp_lin[0], rn = h->rnorm, dm2 = (double)m * (double)m;
+    for (int c = 0; c < N; c++) {
+        A[0 + (size_t)c * N] = c < nc ? rn * CM(0, c) : 0.0;
+        A[(N - 1) + (size_t)c * N] = c < nc ? rn * CM(N - 1, c) : 0.0;
+        for (int r = 1; r < N - 1; r++)
+            A[r + (size_t)c * N] = rn * (CM(r, c) - wimp * h->ViscFac * h->hdifV[i] *
+                                                        (CD2M(r, c) + h->or1[r] * CDM(r, c) - dm2 * h->or2[r] * CM(r, c)));
+    }
+    for (int r = 0; r < N; r++) {
+        A[r] *= h->boundary_fac;
+        A[r + (size_t)(N - 1) * N] *= h->boundary_fac;
+    }
+    scale_rows_cols(N, A, fac);
+    int info;
+    getrf_r(N, A, piv, &info);
+    return info;
+}
+
+/* get_psiMat of the influence-matrix method (update_psi_coll_dmat.f90:
+ * 913-983): psi = 0 rows at both walls, the stream-function operator
+ * (with beta) inside */
+static int coll_psi_mat_dmat(or_handle *h, int i, double *A, double *fac, int *piv)
+{
+    int N = h->N, nc = h->ncheb, m = h->idx2m[i];
+    double rn = h->rnorm, dm2 = (double)m * (double)m;
+    for (int c = 0; c < N; c++) {
+        A[0 + (size_t)c * N] = c < nc ? rn * CM(0, c) : 0.0;
+        A[(N - 1) + (size_t)c * N] = c < nc ? rn * CM(N - 1, c) : 0.0;
+        for (int r = 1; r < N - 1; r++)
+            A[r + (size_t)c * N] = rn * (CD2M(r, c) + (h->or1[r] + h->beta[r]) * CDM(r, c) +
+                                         (h->or1[r] * h->beta[r] + h->dbeta[r] - dm2 * h->or2[r]) * CM(r, c));
+    }
+    for (int r = 0; r < N; r++) {
+        A[r] *= h->boundary_fac;
+        A[r + (size_t)(N - 1) * N] *= h->boundary_fac;
+    }
+    scale_rows_cols(N, A, fac);
+    int info;
+    getrf_r(N, A, piv, &info);
+    return info;
+}
+
+/* get_bc_influence_matrix (update_psi_coll_dmat.f90:1302-1452): the wall
+ * values of the velocity condition (du_phi/dr - u_phi/r = 0 stress-free,
+ * u_phi = 0 no-slip, in dpsi/dr form) of Chebyshev coefficients c */
+static void coll_bc_dmat(or_handle *h, const cplx *c, cplx *top, cplx *bot)
+{
+    int N = h->N;
+    cplx t = 0.0, b = 0.0;
+    for (int n = 0; n < N; n++) {
+        double fac = (n == 0 || n == N - 1) ? h->boundary_fac : 1.0;
+        t += fac * (h->c.ktopv == 1 ? CD2M(0, n) - h->or1[0] * CDM(0, n) : CDM(0, n)) * c[n];
+        b += fac * (h->c.kbotv == 1 ? CD2M(N - 1, n) - h->or1[N - 1] * CDM(N - 1, n) : CDM(N - 1, n)) * c[n];
+    }
+    *top = h->rnorm * t;
+    *bot = h->rnorm * b;
+}
+
+/* one mode's influence-matrix set-up (update_psi_coll_dmat.f90:182-205,
+ * 296-312, 344-347, 1136-1254): omega_2 (omega = 1 at the inner wall),
+ * omega_3 (= 1 at the outer wall), psi_2,3 from L_psi psi = -omega_2,3 and
+ * the inverse of the matrix of their wall velocity conditions */
+static int coll_dmat_mode(or_handle *h, int i)
+{
+    int N = h->N, nc = h->ncheb;
+    double *Om = h->dOmat + (size_t)i * N * N, *of = h->dOfac + (size_t)i * 2 * N;
+    double *Pm = h->dPmat + (size_t)i * N * N, *pf = h->dPfac + (size_t)i * 2 * N;
+    int *op = h->dOpiv + (size_t)i * N, *pp = h->dPpiv + (size_t)i * N;
+    int info = coll_om_mat_dmat(h, i, Om, of, op);
+    if (!info) info = coll_psi_mat_dmat(h, i, Pm, pf, pp);
+    if (info) return info;
+    double *rhs = tl_dbuf(2 * (size_t)N), *w = rhs + N;
+    cplx bt[2], bb[2];
+    for (int set = 0; set < 2; set++) {
+        double *om = (set == 0 ? h->dom2 : h->dom3) + (size_t)i * N;
+        double *ps = (set == 0 ? h->dpsi2 : h->dpsi3) + (size_t)i * N;
+        memset(rhs, 0, sizeof(double) * N);
+        if (set == 0) rhs[N - 1] = of[N - 1];
+        else rhs[0] = of[0];
+        getrs_r(N, Om, op, rhs);
+        for (int k = 0; k < N; k++) om[k] = k < nc ? rhs[k] * of[N + k] : 0.0;
+        /* get_intermediate_psi: omega to the grid, -omega inside */
+        memcpy(w, om, sizeof(double) * N);
+        costf_real(h, w);
+        rhs[0] = rhs[N - 1] = 0.0;
+        for (int k = 1; k < N - 1; k++) rhs[k] = -w[k];
+        for (int k = 0; k < N; k++) rhs[k] *= pf[k];
+        getrs_r(N, Pm, pp, rhs);
+        for (int k = 0; k < N; k++) ps[k] = k < nc ? rhs[k] * pf[N + k] : 0.0;
+        cplx *tmp = tl_cbuf(5, N);
+        for (int k = 0; k < N; k++) tmp[k] = ps[k];
+        coll_bc_dmat(h, tmp, &bt[set], &bb[set]);
+    }
+    double t2 = creal(bt[0]), b2 = creal(bb[0]), t3 = creal(bt[1]), b3 = creal(bb[1]);
+    double det = t2 * b3 - t3 * b2;
+    double *inf = h->dinfl + (size_t)i * 4;
+    inf[0] = b3 / det;  /* (1,1) */
+    inf[1] = -t3 / det; /* (1,2) */
+    inf[2] = -b2 / det; /* (2,1) */
+    inf[3] = t2 / det;  /* (2,2) */
+    return 0;
+}
+
 /* the per-m LU factorisations of one lMat rebuild */
 static int coll_build(or_handle *h)
 {
@@ -2719,6 +2860,7 @@ static int coll_build(or_handle *h)
                                    h->cXmat + (size_t)i * N * N, h->cXfac + (size_t)i * N, h->cXpiv + (size_t)i * N);
         if (!info) {
             if (m == 0) info = coll_uphi_mat(h, h->cUmat, h->cUpiv);
+            else if (!h->c.l_direct_solve) info = coll_dmat_mode(h, i);
             else info = coll_psi_mat(h, m, h->cPmat + (size_t)i * N2 * N2, h->cPfac + (size_t)i * 2 * N2,
                                      h->cPpiv + (size_t)i * N2);
         }
@@ -2745,9 +2887,18 @@ static void coll_alloc(or_handle *h)
     h->cTfac = dalloc(nm * N); h->cXfac = dalloc(nm * N);
     h->cTpiv = calloc(nm * N, sizeof(int)); h->cXpiv = calloc(nm * N, sizeof(int));
     h->cUmat = dalloc(N * N); h->cUpiv = calloc(N, sizeof(int));
-    h->cPmat = calloc(nm * N2 * N2, sizeof(cplx));
-    h->cPfac = dalloc(nm * 2 * N2);
-    h->cPpiv = calloc(nm * N2, sizeof(int));
+    if (h->c.l_direct_solve) {
+        h->cPmat = calloc(nm * N2 * N2, sizeof(cplx));
+        h->cPfac = dalloc(nm * 2 * N2);
+        h->cPpiv = calloc(nm * N2, sizeof(int));
+    } else {
+        h->dOmat = dalloc(nm * N * N); h->dPmat = dalloc(nm * N * N);
+        h->dOfac = dalloc(nm * 2 * N); h->dPfac = dalloc(nm * 2 * N);
+        h->dOpiv = calloc(nm * N, sizeof(int)); h->dPpiv = calloc(nm * N, sizeof(int));
+        h->dom2 = dalloc(nm * N); h->dom3 = dalloc(nm * N);
+        h->dpsi2 = dalloc(nm * N); h->dpsi3 = dalloc(nm * N);
+        h->dinfl = dalloc(nm * 4);
+    }
 }
 
 /* get_ddr_real_1d with get_ddcheb_real_1d (radial_derivatives.f90:255-311,
@@ -2822,6 +2973,11 @@ static void coll_finish_exp(or_handle *h, int is)
                 if (h->l_heat) e[q] -= h->BuoFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->temp[q];
                 if (h->l_chem) e[q] -= h->ChemFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->xi[q];
             }
+            /* finish_exp_psi_coll_dmat (update_psi_coll_dmat.f90:527-549):
+               the influence-matrix method treats pumping explicitly */
+            if (!h->c.l_direct_solve && h->c.l_ek_pump)
+                e[q] -= (or_plant == 6 ? -1.0 : 1.0) * h->CorFac * h->ekpump[k] *
+                        (h->om[q] + h->ekp_up[k] * h->up[q] + (h->ekp_dusdp[k] * I * (double)m + h->ekp_us[k]) * h->us[q]);
         }
     }
 }
@@ -2890,10 +3046,13 @@ static void coll_rhs_imp_psi(or_handle *h, tarray *a, int istage, int l_calc_lin
                     v = h->ViscFac * h->hdifV[i] * (d2u0[k] + h->or1[k] * du0[k] - h->or2[k] * u0[k]) -
                         h->CorFac * h->ekpump[k] * u0[k];
                 } else {
-                    v = h->ViscFac * h->hdifV[i] * (h->work[q] + h->or1[k] * dom[q] - dm2 * h->or2[k] * h->om[q]) -
-                        h->CorFac * h->ekpump[k] *
-                            (h->om[q] + h->ekp_up[k] * h->up[q] +
-                             (h->ekp_dusdp[k] * I * (double)m + h->ekp_us[k]) * h->us[q]);
+                    v = h->ViscFac * h->hdifV[i] * (h->work[q] + h->or1[k] * dom[q] - dm2 * h->or2[k] * h->om[q]);
+                    /* get_psi_rhs_imp_coll_dmat (update_psi_coll_dmat.f90:641-667)
+                       has no pumping term */
+                    if (h->c.l_direct_solve)
+                        v -= h->CorFac * h->ekpump[k] *
+                             (h->om[q] + h->ekp_up[k] * h->up[q] +
+                              (h->ekp_dusdp[k] * I * (double)m + h->ekp_us[k]) * h->us[q]);
                     if (h->c.l_coriolis_imp) v += h->CorFac * h->beta[k] * h->us[q];
                     if (h->c.l_buo_imp) {
                         if (h->l_heat) v -= h->rgrav[k] * h->or1[k] * h->BuoFac * I * (double)m * h->temp[q];
@@ -2942,9 +3101,109 @@ static void coll_update_scalar(or_handle *h, int is_temp)
     else coll_rhs_imp_scalar(h, fld, dfld, a, t->istage + 1, t->l_imp_calc_rhs[t->istage], 1, diff, hdif);
 }
 
+/* update_om_coll_dmat (update_psi_coll_dmat.f90:126-466): omega from its
+ * own equation with omega = 0 at the walls, psi from -L_psi psi = omega with
+ * psi = 0 at the walls, then the multiples c1, c2 of the homogeneous pairs
+ * (omega_2,3, psi_2,3) that make psi meet the velocity conditions */
+static void coll_update_psi_dmat(or_handle *h)
+{
+    int N = h->N, nc = h->ncheb;
+    tscheme *t = &h->ts;
+    tarray *a = &h->dpsidt;
+    double wimp = t->wimp_lin[0];
+    ts_set_imex_rhs(h, t, a, h->work);
+    double *u0 = dalloc(N), *om0 = dalloc(N);
+    int nm0 = -1;
+    for (int i = 0; i < h->nm; i++)
+        if (h->idx2m[i] == 0) nm0 = i;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        cplx *w = h->work + (size_t)i * N;
+        if (m == 0) {
+            u0[0] = 0.0;
+            u0[N - 1] = 0.0;
+            for (int k = 1; k < N - 1; k++) u0[k] = creal(w[k]);
+            getrs_r(N, h->cUmat, h->cUpiv, u0);
+            for (int k = nc; k < N; k++) u0[k] = 0.0;
+            continue;
+        }
+        const double *of = h->dOfac + (size_t)i * 2 * N, *pf = h->dPfac + (size_t)i * 2 * N;
+        double *rr = tl_dbuf(2 * (size_t)N), *ri = rr + N;
+        cplx *om = h->om + (size_t)i * N, *ps = h->psi + (size_t)i * N, *og = tl_cbuf(5, N);
+        /* omega (Chebyshev coefficients) */
+        rr[0] = ri[0] = rr[N - 1] = ri[N - 1] = 0.0;
+        for (int k = 1; k < N - 1; k++) {
+            size_t q = (size_t)i * N + k;
+            cplx v = w[k];
+            if (h->c.l_buo_imp && (h->l_heat || h->l_chem)) {
+                if (h->l_heat) v -= wimp * h->BuoFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->temp[q];
+                if (h->l_chem) v -= wimp * h->ChemFac * h->rgrav[k] * h->or1[k] * I * (double)m * h->xi[q];
+            }
+            rr[k] = creal(v);
+            ri[k] = cimag(v);
+        }
+        for (int k = 0; k < N; k++) { rr[k] *= of[k]; ri[k] *= of[k]; }
+        getrs_r(N, h->dOmat + (size_t)i * N * N, h->dOpiv + (size_t)i * N, rr);
+        getrs_r(N, h->dOmat + (size_t)i * N * N, h->dOpiv + (size_t)i * N, ri);
+        for (int k = 0; k < N; k++) om[k] = k < nc ? (rr[k] + I * ri[k]) * of[N + k] : 0.0;
+        /* omega on the grid, then psi_1 */
+        memcpy(og, om, sizeof(cplx) * N);
+        costf_vec(h, og);
+        rr[0] = ri[0] = rr[N - 1] = ri[N - 1] = 0.0;
+        for (int k = 1; k < N - 1; k++) { rr[k] = -creal(og[k]); ri[k] = -cimag(og[k]); }
+        for (int k = 0; k < N; k++) { rr[k] *= pf[k]; ri[k] *= pf[k]; }
+        getrs_r(N, h->dPmat + (size_t)i * N * N, h->dPpiv + (size_t)i * N, rr);
+        getrs_r(N, h->dPmat + (size_t)i * N * N, h->dPpiv + (size_t)i * N, ri);
+        for (int k = 0; k < N; k++) ps[k] = k < nc ? (rr[k] + I * ri[k]) * pf[N + k] : 0.0;
+        /* solve_influence_matrix (:1256-1300) */
+        cplx bt, bb;
+        coll_bc_dmat(h, ps, &bt, &bb);
+        const double *inf = h->dinfl + (size_t)i * 4;
+        cplx c1 = -inf[0] * bt - inf[1] * bb, c2 = -inf[2] * bt - inf[3] * bb;
+        const double *p2 = h->dpsi2 + (size_t)i * N, *p3 = h->dpsi3 + (size_t)i * N;
+        const double *o2 = h->dom2 + (size_t)i * N, *o3 = h->dom3 + (size_t)i * N;
+        for (int k = 0; k < N; k++) {
+            ps[k] = ps[k] + c1 * p2[k] + c2 * p3[k];
+            om[k] = om[k] + c1 * o2[k] + c2 * o3[k];
+        }
+    }
+    if (nm0 >= 0) {
+        get_dr_real(h, u0, om0, 0);
+        costf_real(h, u0);
+        for (int k = 0; k < N; k++) h->psi[(size_t)nm0 * N + k] = 0.0;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < h->nm; i++) get_dr_vec(h, h->psi + (size_t)i * N, h->work + (size_t)i * N, 0);
+    costf_field(h, h->psi);
+    for (int i = 0; i < h->nm; i++) {
+        int m = h->idx2m[i];
+        for (int k = 0; k < N; k++) {
+            size_t q = (size_t)i * N + k;
+            if (m == 0) {
+                h->us[q] = 0.0;
+                h->up[q] = u0[k];
+                h->om[q] = om0[k] + h->or1[k] * u0[k];
+            } else {
+                h->us[q] = I * (double)m * h->or1[k] * h->psi[q];
+                h->up[q] = -h->work[q] - h->beta[k] * h->psi[q];
+            }
+        }
+    }
+    ts_rotate(h, t, a);
+    if (t->istage == t->nstages) coll_rhs_imp_psi(h, a, 1, t->l_imp_calc_rhs[0], 1);
+    else coll_rhs_imp_psi(h, a, t->istage + 1, t->l_imp_calc_rhs[t->istage], 1);
+    free(u0);
+    free(om0);
+}
+
 /* update_om_coll_smat (update_psi_coll_smat.f90:88-321) */
 static void coll_update_psi(or_handle *h)
 {
+    if (!h->c.l_direct_solve) {
+        coll_update_psi_dmat(h);
+        return;
+    }
     int N = h->N, N2 = 2 * N;
     tscheme *t = &h->ts;
     tarray *a = &h->dpsidt;

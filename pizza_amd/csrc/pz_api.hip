@@ -916,7 +916,10 @@ static void setup(pz_handle *h)
     d.courfac = h->sch.courfac;
     d.l_heat = h->l_heat;
     d.l_chem = h->l_chem;
-    d.l_coriolis_imp = c.l_coriolis_imp;
+    // namelists.f90:448-457: the influence-matrix method treats Coriolis
+    // explicitly
+    d.l_coriolis_imp = (!c.l_direct_solve && !c.l_non_rot) ? 0 : c.l_coriolis_imp;
+    h->c.l_coriolis_imp = d.l_coriolis_imp;
     d.l_buo_imp = c.l_buo_imp;
     d.l_ek_pump = c.l_ek_pump;
     d.ktopt = c.ktopt; d.kbott = c.kbott; d.ktopxi = c.ktopxi; d.kbotxi = c.kbotxi;
@@ -1127,7 +1130,7 @@ static void setup(pz_handle *h)
     d.P_A1 = h->dalloc<cd>(nm * 16); d.P_A2 = h->dalloc<cd>((size_t)nm * PZ_P_NT * N);
     d.P_A3 = h->dalloc<cd>((size_t)nm * PZ_P_NT * N); d.P_A4 = h->dalloc<cd>((size_t)nm * PZ_P_LD * N);
     d.P_p1 = h->dalloc<int>(nm * 4); d.P_p4 = h->dalloc<int>((size_t)nm * N); d.P_fac = h->dalloc<double>(F);
-    d.info = h->dalloc<int>(3 * nm);
+    d.info = h->dalloc<int>(4 * nm);
     d.ldi = (N + 1) & ~1;
     d.T_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
     d.X_inv = h->dalloc<double>((size_t)nm * N * d.ldi);
@@ -1158,6 +1161,25 @@ static void setup(pz_handle *h)
         d.cM = h->upload(cm);
         d.cdM = h->upload(cdm);
         d.cd2M = h->upload(cd2m);
+        if (!c.l_direct_solve) {
+            // influence-matrix method (update_psi_coll_dmat.f90): the
+            // inverses and homogeneous solutions per m, and the wall rows of
+            // the velocity condition, rnorm and boundary_fac folded in
+            // (get_bc_influence_matrix, :1302-1376)
+            d.dmat = 1;
+            d.OmI = h->dalloc<double>((size_t)nm * N * d.ldi);
+            d.PsI = h->dalloc<double>((size_t)nm * N * d.ldi);
+            d.dhom = h->dalloc<double>((size_t)nm * 4 * N);
+            d.dinfl = h->dalloc<double>((size_t)nm * 4);
+            std::vector<double> bc(2 * (size_t)N);
+            for (int n = 0; n < N; n++) {
+                const double fac = (n == 0 || n == N - 1) ? 0.5 : 1.0, rn = std::sqrt(2.0 / (double)(N - 1));
+                const size_t t = n, b = (size_t)(N - 1) * N + n;
+                bc[n] = rn * fac * (c.ktopv == 1 ? cd2m[t] - h->or1[0] * cdm[t] : cdm[t]);
+                bc[N + n] = rn * fac * (c.kbotv == 1 ? cd2m[b] - h->or1[N - 1] * cdm[b] : cdm[b]);
+            }
+            d.bcrow = h->upload(bc);
+        }
         d.PC_inv = h->dalloc<cd>((size_t)nm * N * 2 * N);
         d.U_inv = h->dalloc<double>((size_t)N * d.ldi);
         // Gauss-Jordan blocks per m: T and xi (N x 2N real), psi (2N x 4N complex)
@@ -1241,7 +1263,8 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
     h->c = *cfg;
     const pz_config &c = *cfg;
     std::string why;
-    if (!c.l_direct_solve) why = "only matrix_solve='direct' is implemented";
+    if (!c.l_direct_solve && (!c.l_cheb_coll || c.l_finite_diff))
+        why = "matrix_solve='influence' is implemented for the collocation method";
     else if (c.l_non_rot && !c.l_cheb_coll && !c.l_finite_diff)
         why = "non-rotating runs are implemented for the collocation and finite-difference methods";
     else if (c.radratio <= 0.0) why = "full-disk geometry is not implemented";
@@ -1255,6 +1278,8 @@ extern "C" int pz_create(const pz_config *cfg, pz_handle **out)
         else if (probe.l_assembly && !c.l_cheb_coll && !c.l_finite_diff)
             why = "DIRK schemes with an assembly stage are implemented for the collocation and finite-difference "
                   "methods only";
+        else if (probe.l_assembly && !c.l_direct_solve)
+            why = "DIRK schemes with an assembly stage are implemented for matrix_solve='direct' only";
     }
     if (!why.empty()) {
         int rc = fail(h, -2, why);
@@ -1819,7 +1844,11 @@ static void launch_builds(pz_handle *h)
     if (h->d.fd) {
         hipLaunchKernelGGL(k_fd_build, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 0);
     } else if (h->d.coll) {
-        hipLaunchKernelGGL(k_build_coll, dim3(h->nm, 3), dim3(1024), h->lds_gj, h->stream, (const PzDev *)h->ddev, 0);
+        hipLaunchKernelGGL(k_build_coll, dim3(h->nm, h->d.dmat ? 4 : 3), dim3(1024), h->lds_gj, h->stream,
+                           (const PzDev *)h->ddev, 0);
+        if (h->d.dmat)
+            hipLaunchKernelGGL(k_dmat_homog, dim3(h->nm), dim3(256), dmat_homog_lds(h->N, h->d.L), h->stream,
+                               (const PzDev *)h->ddev, 0);
     } else {
         hipLaunchKernelGGL(k_build_lhs, dim3(3, h->nm), dim3(64), h->lds_build, h->stream, (const PzDev *)h->ddev, 0);
         hipLaunchKernelGGL(k_build_inv, dim3(h->nm, 3), dim3(64 * h->inv_waves), h->lds_inv, h->stream,
@@ -1960,7 +1989,7 @@ static int check_state(pz_handle *h, PzScheme *out)
 {
     PzScheme s;
     HIPCHK(hipMemcpyAsync(&s, h->d.ss, sizeof(s), hipMemcpyDeviceToHost, h->stream));
-    std::vector<int> info(3 * h->nm);
+    std::vector<int> info(4 * h->nm);
     HIPCHK(hipMemcpyAsync(info.data(), h->d.info, info.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (out) *out = s;
@@ -2570,6 +2599,7 @@ static int adj_check(pz_handle *h)
     if (h->c.l_rerror_fix) return fail(h, -2, "the adjoint needs l_rerror_fix = .false. (thresholds are not linear)");
     if (!h->l_heat) return fail(h, -2, "the adjoint objectives need the temperature equation");
     if (h->nphi < 2 * h->nm) return fail(h, -2, "the adjoint needs n_phi >= 2 n_m");
+    if (h->d.dmat) return fail(h, -2, "the adjoint covers matrix_solve='direct'");
     return 0;
 }
 
@@ -2934,6 +2964,7 @@ extern "C" int pz_ensemble_create(pz_handle *const *runs, int nruns, pz_ensemble
             if (!h->started) throw std::invalid_argument("every run must be started (pz_start) first");
             if (h->d.fd) throw std::invalid_argument("finite-difference runs are not batched into ensembles");
             if (h->radial_big) throw std::invalid_argument("runs with long azimuthal grids are not batched into ensembles");
+            if (h->d.dmat) throw std::invalid_argument("runs with matrix_solve='influence' are not batched into ensembles");
             if (h->sch.l_assembly)
                 throw std::invalid_argument("DIRK schemes with an assembly stage are not batched into ensembles");
             if (h->sch.id != h0->sch.id || h->d.coll != h0->d.coll)

@@ -335,7 +335,59 @@ __device__ void coll_assemble(const PzDev &p, const RadView &R, int fld, int m, 
     }
 }
 
-// grid (n_m, 3): field 0 = T, 1 = xi, 2 = psi (m /= 0) or u_phi (m = 0).
+// get_omMat / get_psiMat of the influence-matrix method
+// (update_psi_coll_dmat.f90:829-983) into the left half of W (row-major,
+// 2N wide): which = 0 the omega operator (identity minus the implicit
+// viscous term), 1 the stream-function operator with beta; zero rows at
+// both walls, boundary_fac on the first and last columns, then the row
+// (fac[0..N)) and column (fac[N..2N)) max-scalings
+__device__ void coll_assemble_dmat(const PzDev &p, const RadView &R, int which, int m, double hdif, double *W,
+                                   double *fac)
+{
+    const int N = p.N, nc = p.ncheb, tid = threadIdx.x, nt = blockDim.x, w2 = 2 * N;
+    const double wimp = p.ss->wimp_lin[0], rn = p.rnorm, dm2 = (double)m * (double)m;
+    for (int t = tid; t < N * N; t += nt) {
+        const int r = t / N, c = t - r * N;
+        double a;
+        if (r == 0 || r == N - 1)
+            a = c < nc ? rn * PZ_CM(r, c) : 0.0;
+        else if (which == 0)
+            a = rn * (PZ_CM(r, c) -
+                      wimp * p.ViscFac * hdif * (PZ_CD2M(r, c) + R.or1(r) * PZ_CDM(r, c) - dm2 * R.or2(r) * PZ_CM(r, c)));
+        else
+            a = rn * (PZ_CD2M(r, c) + (R.or1(r) + R.beta(r)) * PZ_CDM(r, c) +
+                      (R.or1(r) * R.beta(r) + R.dbeta(r) - dm2 * R.or2(r)) * PZ_CM(r, c));
+        if (c == 0 || c == N - 1) a *= p.boundary_fac;
+        W[(size_t)r * w2 + c] = a;
+    }
+    __syncthreads();
+    for (int r = tid; r < N; r += nt) {
+        double mx = 0.0;
+        for (int c = 0; c < N; c++) mx = fmax(mx, fabs(W[(size_t)r * w2 + c]));
+        fac[r] = 1.0 / mx;
+    }
+    __syncthreads();
+    for (int t = tid; t < N * N; t += nt) {
+        const int r = t / N, c = t - r * N;
+        W[(size_t)r * w2 + c] *= fac[r];
+    }
+    __syncthreads();
+    for (int c = tid; c < N; c += nt) {
+        double mx = 0.0;
+        for (int r = 0; r < N; r++) mx = fmax(mx, fabs(W[(size_t)r * w2 + c]));
+        fac[N + c] = 1.0 / mx;
+    }
+    __syncthreads();
+    for (int t = tid; t < N * N; t += nt) {
+        const int r = t / N, c = t - r * N;
+        W[(size_t)r * w2 + c] *= fac[N + c];
+    }
+    __syncthreads();
+}
+
+// grid (n_m, 3): field 0 = T, 1 = xi, 2 = psi (m /= 0) or u_phi (m = 0);
+// with the influence-matrix method (n_m, 4): 2 = omega (m /= 0) or u_phi,
+// 3 = psi (m /= 0).
 // Forms the inverses, with the reference's scalings folded in, whenever the
 // matrices change (lMat) or when forced; only the first n_r columns of the
 // psi inverse are formed, the ones the m-loop applies.  Dynamic LDS:
@@ -351,6 +403,22 @@ __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld,
     char *ws = p.coll_work + (size_t)mi * p.coll_ws;
     const size_t regT = (size_t)N * 2 * N * sizeof(double);
     const int ldi = p.ldi;
+    if (p.dmat && fld >= 2 && m != 0) {
+        // influence-matrix method: field 2 the omega operator, 3 the psi
+        // operator; INV = diag(fac_col) A_s^{-1} diag(fac_row), real
+        double *W = (double *)(ws + fld * regT);
+        coll_assemble_dmat(p, R, fld - 2, m, p.hdifV[mi], W, facs);
+        const int info = gj_invert_blk<double>(W, N, N, (double *)gj_smem, p.gj_b);
+        if (tid == 0) p.info[fld * p.nm + mi] = info;
+        if (info) return;
+        double *out = (fld == 2 ? p.OmI : p.PsI) + (size_t)mi * N * ldi;
+        for (int t = tid; t < N * ldi; t += blockDim.x) {
+            const int c = t / ldi, r = t - c * ldi;
+            out[(size_t)c * ldi + r] = r < N ? facs[N + r] * W[(size_t)r * 2 * N + N + c] * facs[c] : 0.0;
+        }
+        return;
+    }
+    if (fld == 3) return;
     if (fld < 2 || m == 0) {
         double *W = (double *)(ws + (fld == 2 ? 2 * regT : fld * regT));
         coll_assemble<double>(p, R, fld, m, fld == 0 ? p.hdifT[mi] : fld == 1 ? p.hdifXi[mi] : p.hdifV[mi], W, facs);
@@ -382,6 +450,74 @@ __device__ __forceinline__ void build_coll_body(const PzDev &p, int mi, int fld,
             const int c = t / n, r = t - c * n;
             out[(size_t)c * n + r] = cscale(facs[n + r] * facs[c], W[(size_t)r * 2 * n + n + c]);
         }
+    }
+}
+
+// dynamic LDS of k_dmat_homog: omega_2,3 and their grid values (4 N
+// complex), the transform scratch (4 L + 16 complex), the psi right-hand
+// sides (2 N doubles)
+__host__ __device__ inline size_t dmat_homog_lds(int N, int L)
+{
+    return (4 * (size_t)N + 4 * (size_t)L + 16) * sizeof(cd) + 2 * (size_t)N * sizeof(double);
+}
+
+// The influence-matrix set-up of one m, after the inverses are built
+// (update_psi_coll_dmat.f90:182-205, 296-312, 344-347, 1136-1254):
+// omega_2 = INV_om e_(N-1) (omega = 1 at the inner wall), omega_3 =
+// INV_om e_0 (outer wall), psi_2,3 = INV_psi (0, -omega_2,3 on the grid, 0),
+// and the inverse of the 2 x 2 matrix of the wall conditions of psi_2,3.
+// One workgroup per m; m = 0 has no influence matrix.
+__global__ void __launch_bounds__(256) k_dmat_homog(const PzDev *__restrict__ pd, int force)
+{
+    const PzDev &p = *pd;
+    if (!force && p.ss->lMat == 0) return;
+    const int mi = blockIdx.x;
+    if (p.idx2m[mi] == 0) return;
+    extern __shared__ double smem[];
+    const int N = p.N, nc = p.ncheb, tid = threadIdx.x, ldi = p.ldi;
+    cd *o = (cd *)smem, *g = o + 2 * N, *buf = g + 2 * N;
+    double *rhs = (double *)(buf + 4 * p.L + 16);
+    const double *OmI = p.OmI + (size_t)mi * N * ldi, *PsI = p.PsI + (size_t)mi * N * ldi;
+    double *hom = p.dhom + (size_t)mi * 4 * N;
+    for (int k = tid; k < N; k += blockDim.x) {
+        const double a = k < nc ? OmI[(size_t)(N - 1) * ldi + k] : 0.0, b = k < nc ? OmI[k] : 0.0;
+        o[k] = mk(a, 0.0);
+        o[N + k] = mk(b, 0.0);
+        hom[k] = a;
+        hom[N + k] = b;
+    }
+    __syncthreads();
+    dct_fft(p, VecSet<const cd>(o, o + N), VecSet<cd>(g, g + N), 2, 0, p.cosT, p.sinT, buf);
+    __syncthreads();
+    for (int t = tid; t < 2 * N; t += blockDim.x) {
+        const int k = t % N;
+        rhs[t] = (k == 0 || k == N - 1) ? 0.0 : -g[t].x;
+    }
+    __syncthreads();
+    // psi_2,3 = INV_psi rhs, one row per thread (columns of INV_psi are
+    // contiguous, so the rows of a wave read consecutive addresses)
+    for (int t = tid; t < 2 * N; t += blockDim.x) {
+        const int set = t / N, r = t - set * N;
+        double v = 0.0;
+        if (r < nc)
+            for (int c = 0; c < N; c++) v += PsI[(size_t)c * ldi + r] * rhs[(size_t)set * N + c];
+        hom[(2 + set) * (size_t)N + r] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double t2 = 0, b2 = 0, t3 = 0, b3 = 0;
+        for (int k = 0; k < N; k++) {
+            t2 += p.bcrow[k] * hom[2 * (size_t)N + k];
+            b2 += p.bcrow[N + k] * hom[2 * (size_t)N + k];
+            t3 += p.bcrow[k] * hom[3 * (size_t)N + k];
+            b3 += p.bcrow[N + k] * hom[3 * (size_t)N + k];
+        }
+        const double det = t2 * b3 - t3 * b2;
+        double *inf = p.dinfl + (size_t)mi * 4;
+        inf[0] = b3 / det;
+        inf[1] = -t3 / det;
+        inf[2] = -b2 / det;
+        inf[3] = t2 / det;
     }
 }
 
@@ -535,6 +671,10 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
             eP[k] = p.P_expl[is * F + off + k];
             usv[k] = p.us[off + k];
             if (exp_calc) aP[k] = p.dVsOm[off + k];
+            if (exp_calc && p.dmat && p.l_ek_pump) {
+                omv[k] = p.om[off + k];
+                upv[k] = p.up[off + k];
+            }
             imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, PP[k], QP[k]);
         }
     }
@@ -580,6 +720,13 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
                 if (!p.l_buo_imp) {
                     if (hT) e = csub(e, cmul(mk(0.0, p.BuoFac * R.rgrav(k) * o1 * dm), Tv[k]));
                     if (hX) e = csub(e, cmul(mk(0.0, p.ChemFac * R.rgrav(k) * o1 * dm), Xv[k]));
+                }
+                if (p.dmat && p.l_ek_pump) {
+                    // finish_exp_psi_coll_dmat (update_psi_coll_dmat.f90:527-549):
+                    // pumping explicit with the influence-matrix method
+                    cd pump = cadd(omv[k], cscale(R.ekp_up(k), upv[k]));
+                    pump = cadd(pump, cmul(mk(R.ekp_us(k), dm * R.ekp_dusdp(k)), usv[k]));
+                    e = csub(e, cscale(p.CorFac * R.ekpump(k), pump));
                 }
                 eP[k] = e;
                 p.P_expl[is * F + off + k] = e;
@@ -717,11 +864,49 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
         aP[k] = (k == 0 || k == N - 1) ? mk(0, 0) : r;
     }
     __syncthreads();
-    // [omega; psi] coefficients (2N values in aP, cP)
-    block_matvec_rc<cd, 1>(2 * N, N, 2 * N, VecSet<const cd>(p.PC_inv + off * 2 * N), VecSet<cd>(aP),
-                           VecSet<cd>(aP), red);
-    for (int k = nc + tid; k < N; k += blockDim.x) { aP[k] = mk(0, 0); cP[k] = mk(0, 0); }
-    __syncthreads();
+    if (p.dmat) {
+        // update_om_coll_dmat (update_psi_coll_dmat.f90:244-453): omega_1
+        // from its own equation (omega = 0 at the walls) ...
+        block_matvec<double, 1>(N, ldi, VecSet<const double>(p.OmI + off * ldi), VecSet<cd>(aP), VecSet<cd>(aP), red);
+        for (int k = nc + tid; k < N; k += blockDim.x) aP[k] = mk(0, 0);
+        __syncthreads();
+        dct_fft(p, VecSet<const cd>(aP), VecSet<cd>(eP), 1, 0, cosT, sinT, fbuf);
+        __syncthreads();
+        // ... psi_1 from L_psi psi = -omega_1 (psi = 0 at the walls) ...
+        for (int k = tid; k < N; k += blockDim.x) cP[k] = (k == 0 || k == N - 1) ? mk(0, 0) : mk(-eP[k].x, -eP[k].y);
+        __syncthreads();
+        block_matvec<double, 1>(N, ldi, VecSet<const double>(p.PsI + off * ldi), VecSet<cd>(cP), VecSet<cd>(cP), red);
+        for (int k = nc + tid; k < N; k += blockDim.x) cP[k] = mk(0, 0);
+        __syncthreads();
+        // ... and the multiples of the homogeneous pairs that meet the
+        // velocity conditions (solve_influence_matrix, :1256-1300)
+        double tx = 0.0, ty = 0.0, bx = 0.0, by = 0.0;
+        for (int k = tid; k < N; k += blockDim.x) {
+            const double ct = p.bcrow[k], cb = p.bcrow[N + k];
+            tx += ct * cP[k].x;
+            ty += ct * cP[k].y;
+            bx += cb * cP[k].x;
+            by += cb * cP[k].y;
+        }
+        tx = block_reduce(tx, (double *)red, 0);
+        ty = block_reduce(ty, (double *)red, 0);
+        bx = block_reduce(bx, (double *)red, 0);
+        by = block_reduce(by, (double *)red, 0);
+        const double *inf = p.dinfl + (size_t)mi * 4, *hom = p.dhom + (size_t)mi * 4 * N;
+        const cd c1 = mk(-inf[0] * tx - inf[1] * bx, -inf[0] * ty - inf[1] * by);
+        const cd c2 = mk(-inf[2] * tx - inf[3] * bx, -inf[2] * ty - inf[3] * by);
+        for (int k = tid; k < N; k += blockDim.x) {
+            aP[k] = cadd(cadd(aP[k], cscale(hom[k], c1)), cscale(hom[N + k], c2));
+            cP[k] = cadd(cadd(cP[k], cscale(hom[2 * N + k], c1)), cscale(hom[3 * N + k], c2));
+        }
+        __syncthreads();
+    } else {
+        // [omega; psi] coefficients (2N values in aP, cP)
+        block_matvec_rc<cd, 1>(2 * N, N, 2 * N, VecSet<const cd>(p.PC_inv + off * 2 * N), VecSet<cd>(aP),
+                               VecSet<cd>(aP), red);
+        for (int k = nc + tid; k < N; k += blockDim.x) { aP[k] = mk(0, 0); cP[k] = mk(0, 0); }
+        __syncthreads();
+    }
     // d psi / dr and psi on the grid
     dr_fft(p, R, VecSet<const cd>(cP), VecSet<cd>(eP), VecSet<cd>(), 1, cosT, sinT, fbuf, psiv);
     __syncthreads();
@@ -748,9 +933,12 @@ __device__ __forceinline__ void mcoll_body(const PzDev &p, int mi, int istage, i
             const double o1 = R.or1(k), ekp = p.CorFac * R.ekpump(k);
             cd v = cscale(p.ViscFac * p.hdifV[mi],
                           csub(cadd(b2[k], cscale(o1, bv[k])), cscale(dm2 * R.or2(k), omv[k])));
-            cd pump = cadd(omv[k], cscale(R.ekp_up(k), upv[k]));
-            pump = cadd(pump, cmul(mk(R.ekp_us(k), dm * R.ekp_dusdp(k)), usv[k]));
-            v = csub(v, cscale(ekp, pump));
+            if (!p.dmat) {
+                // get_psi_rhs_imp_coll_dmat has no pumping term (update_psi_coll_dmat.f90:641-667)
+                cd pump = cadd(omv[k], cscale(R.ekp_up(k), upv[k]));
+                pump = cadd(pump, cmul(mk(R.ekp_us(k), dm * R.ekp_dusdp(k)), usv[k]));
+                v = csub(v, cscale(ekp, pump));
+            }
             if (p.l_coriolis_imp) v = cadd(v, cscale(p.CorFac * R.beta(k), usv[k]));
             if (p.l_buo_imp) {
                 if (hT) v = csub(v, cmul(mk(0.0, R.rgrav(k) * o1 * p.BuoFac * dm), Tv[k]));
@@ -856,9 +1044,11 @@ __global__ void __launch_bounds__(256) k_start_coll(const PzDev *__restrict__ pd
         p.P_old[off + k] = omv[k];
         const double o1 = R.or1(k), ekp = p.CorFac * R.ekpump(k);
         cd v = cscale(p.ViscFac * p.hdifV[mi], csub(cadd(d2[k], cscale(o1, d1[k])), cscale(dm2 * R.or2(k), omv[k])));
-        cd pump = cadd(omv[k], cscale(R.ekp_up(k), upv[k]));
-        pump = cadd(pump, cmul(mk(R.ekp_us(k), dm * R.ekp_dusdp(k)), usv[k]));
-        v = csub(v, cscale(ekp, pump));
+        if (!p.dmat) { // the influence-matrix method has no implicit pumping term
+            cd pump = cadd(omv[k], cscale(R.ekp_up(k), upv[k]));
+            pump = cadd(pump, cmul(mk(R.ekp_us(k), dm * R.ekp_dusdp(k)), usv[k]));
+            v = csub(v, cscale(ekp, pump));
+        }
         if (p.l_coriolis_imp) v = cadd(v, cscale(p.CorFac * R.beta(k), usv[k]));
         if (p.l_buo_imp) {
             if (hT) v = csub(v, cmul(mk(0.0, R.rgrav(k) * o1 * p.BuoFac * dm), Tv[k]));

@@ -136,6 +136,16 @@ struct PzDev {
     char *coll_work;
     size_t coll_ws;
 
+    // influence-matrix psi solve of the collocation method (matrix_solve =
+    // 'influence', update_psi_coll_dmat.f90): per m the scaled inverses of
+    // the omega and psi operators [m][col][ldi], the homogeneous solutions
+    // [m][4][n_r] (omega_2, omega_3, psi_2, psi_3; Chebyshev space), the
+    // 2 x 2 influence matrices [m][4], and the wall rows of the velocity
+    // condition [2][n_r] (outer, inner)
+    int dmat;
+    double *OmI, *PsI, *dhom, *dinfl;
+    const double *bcrow;
+
     PzScheme *ss;
 };
 

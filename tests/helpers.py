@@ -59,12 +59,13 @@ SAMPLES = {
     "TestRestart": (["inputStart.nml", "input.nml"], "input.nml"),
     "InhomogeneousHeatFlux": (["input.nml"], None),
     "NonRotRa2e3": (["input.nml"], None),
+    "TimeStepChange": (["input.nml"], None),
 }
 
 # Samples whose unitTest compares the 2D kinetic energies (e_kin.TAG:
 # time, us2_2D, up2_2D, up2_axi_2D, outputs.f90:677) instead of e_kin_3D:
 # the non-rotating runs, which write no 3D series
-EKIN_2D = {"NonRotRa2e3"}
+EKIN_2D = {"NonRotRa2e3", "TimeStepChange"}
 
 
 def _ekin_2d(run):

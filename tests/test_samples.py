@@ -17,7 +17,10 @@ reference.out at the reference tests' rtol 1e-8
 - InhomogeneousHeatFlux: fixed-flux walls with an m = 2 outer flux pattern;
 - NonRotRa2e3: a non-rotating collocation run (CorFac = 0, no beta, no
   pumping), compared on its 2D kinetic energies (e_kin.TAG), the file its
-  unitTest reads.
+  unitTest reads;
+- TimeStepChange: a non-rotating collocation run with the influence-matrix
+  psi solve (matrix_solve = 'influence'), SBDF3, 2000 steps through
+  Courant-driven dt changes, on e_kin.TAG.
 
 On the CPU through the oracle; on the GPU through the library and, for
 the longest chains, its Fortran driver.
@@ -53,7 +56,7 @@ def test_sample_gpu(name, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["RotE1e3EkPump", "DIRKSchemes", "NonRotRa2e3"])
+@pytest.mark.parametrize("name", ["RotE1e3EkPump", "DIRKSchemes", "NonRotRa2e3", "TimeStepChange"])
 def test_sample_fortran_driver(name, tmp_path):
     """pizza_gpu in place of pizza.exe: the input files run one after the
     other in one directory, each reading the checkpoint_end file the
