@@ -479,7 +479,8 @@ struct pz_handle {
     hipGraphExec_t gexec = nullptr;
     bool started = false;
     bool use_graph = true;
-    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0;
+    size_t lds_radial = 0, lds_mphase = 0, lds_build = 0, lds_start = 0, lds_inv = 0, lds_start_coll = 0, lds_fd = 0,
+           lds_fd_loop = 0;
     size_t lds_gj = 0; // k_build_coll: blocked Gauss-Jordan panels
     bool mcoll_l2 = false; // collocation m-loop reads radial rows and tables through L2
     bool radial_big = false; // k_radial_loop_big for long azimuthal grids
@@ -1209,6 +1210,7 @@ static void setup(pz_handle *h)
     h->mthreads = mloop_threads(N);
     h->lds_mphase = MpLayout(N, h->mthreads).total;
     h->lds_fd = FdLayout(N).total;
+    h->lds_fd_loop = FdLoopLayout(N).total;
     h->lds_start = 15 * (size_t)N * sizeof(cd);
     h->lds_start_coll = (13 * (size_t)N + 6 * (size_t)(2 * N - 2) + 3 * (size_t)N + 16) * sizeof(cd) +
                         3 * (size_t)N * sizeof(double);
@@ -1241,7 +1243,7 @@ static void setup(pz_handle *h)
     h->lds_inv = InvLayout(N, h->inv_waves).total;
     // the kernels of this run's radial method
     size_t need = h->lds_radial;
-    if (d.fd) need = std::max(need, h->lds_fd);
+    if (d.fd) need = std::max({need, h->lds_fd, h->lds_fd_loop});
     else if (d.coll) need = std::max({need, h->lds_mphase, h->lds_start_coll, h->lds_gj + gj_static,
                                       h->sch.l_assembly ? asm_lds_bytes(N) : (size_t)0});
     else need = std::max({need, h->lds_mphase, h->lds_build, h->lds_start, h->lds_inv});
@@ -1873,7 +1875,7 @@ static void launch_implicit(pz_handle *h, int stage)
         return;
     }
     if (h->d.fd)
-        hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, stage,
+        hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd_loop, h->stream, (const PzDev *)h->ddev, stage,
                            exp_calc);
     else if (h->d.coll)
         hipLaunchKernelGGL(h->mcoll_l2 ? k_mcoll_l2 : k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
@@ -2840,7 +2842,7 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
         HIPCHK(hipEventRecord(e0, h->stream));
         for (int i = 0; i < reps; i++) {
             if (which == 0 && h->d.fd)
-                hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd, h->stream, (const PzDev *)h->ddev, 1,
+                hipLaunchKernelGGL(k_fd_mloop, dim3(h->nm), dim3(256), h->lds_fd_loop, h->stream, (const PzDev *)h->ddev, 1,
                                    1);
             else if (which == 0 && h->d.coll)
                 hipLaunchKernelGGL(h->mcoll_l2 ? k_mcoll_l2 : k_mcoll, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,

@@ -358,8 +358,8 @@ __global__ void __launch_bounds__(ASM_THREADS) k_assemble_coll(const PzDev *__re
 // (update_psi_fd.f90:472-606) takes it as u_phi for m = 0 (zero on a
 // no-slip wall) and solves -L psi = omega with psi = 0 at both walls for
 // m /= 0 (get_elliptic_mat_Rdist, :838-888: tridiagonal, factorised here
-// by one thread).  Ghost zones and the first-stage implicit terms follow as
-// in the m-loop.  LDS: FdLayout(N).
+// by one thread and solved by one wave).  Ghost zones and the first-stage
+// implicit terms follow as in the m-loop.  LDS: FdLayout(N).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_assemble_fd(const PzDev *__restrict__ pd)
 {
@@ -447,10 +447,9 @@ __global__ void __launch_bounds__(256) k_assemble_fd(const PzDev *__restrict__ p
         else { gP[0] = z; gP[1] = z; gP[N + 2] = z; gP[N + 3] = z; }
     }
     __syncthreads();
-    if (tid == 0 && m != 0) {
-        fd_lu3(N, eD, eL, eU);
-        fd_tri_solve(N, eD, eL, eU, gP + 1);
-    }
+    if (tid == 0 && m != 0) fd_lu3(N, eD, eL, eU);
+    __syncthreads();
+    if (tid < 64 && m != 0) fd_tri_solve_wave(N, eD, eL, eU, gP + 1);
     __syncthreads();
     if (tid == 0) {
         if (hT) fd_ghost_scalar(gT, N, p.ktopt, p.kbott);

@@ -32,41 +32,159 @@ __device__ __forceinline__ cd fd_dr_at(const FdView &S, const cd *f, int k, int 
     return cadd3(cscale(S.dr(k, 0), f[k - 1]), cscale(S.dr(k, 1), f[k]), cscale(S.dr(k, 2), f[k + 1]));
 }
 
-// solver_up_3 / solver_dn_3 (parallel_solver.f90:367-537), one thread:
-// x[k + 1] for k in [0, N), ghosts x[0] and x[N + 1]; d holds 1/pivot
-__device__ void fd_tri_solve(int N, const double *d, const double *lo, const double *up, cd *x)
+// Forward and back substitution of the factorised systems (solver_up_3 /
+// solver_dn_3 and solver_up_5 / solver_dn_5, parallel_solver.f90:367-593)
+// by one whole wave (every lane calls them): lane l owns rows [l C, l C + C),
+// C = ceil(N / 64).  Each sweep is a linear recurrence of first
+// (tridiagonal) or second (pentadiagonal) order; every lane reduces its rows
+// to one affine map of the state entering them, an inclusive scan over the
+// lanes composes the maps (lower lanes first going down the column, higher
+// lanes first coming back), and each lane re-runs its rows from the state
+// that the scan hands it.  The factors and the row arithmetic are the
+// reference's; only the order in which the chunks' contributions are
+// summed differs from its serial loops.
+__device__ __forceinline__ double lane_up(double v, int s) { return __shfl_up(v, s, 64); }
+__device__ __forceinline__ double lane_dn(double v, int s) { return __shfl_down(v, s, 64); }
+__device__ __forceinline__ cd lane_up(cd v, int s) { return mk(lane_up(v.x, s), lane_up(v.y, s)); }
+__device__ __forceinline__ cd lane_dn(cd v, int s) { return mk(lane_dn(v.x, s), lane_dn(v.y, s)); }
+__device__ __forceinline__ cd cfma(cd a, cd b, cd c) { return cadd(cmul(a, b), c); }
+__device__ __forceinline__ cd cneg(cd a) { return mk(-a.x, -a.y); }
+
+// y -> a y + g, composed over lanes: UP takes lanes below first
+template <bool UP> __device__ __forceinline__ void lane_scan1(double &a, cd &g)
 {
-    cd prev = csub(x[1], cscale(lo[0], x[0]));
-    x[1] = prev;
-    for (int k = 1; k < N; k++) {
-        prev = csub(x[k + 1], cscale(d[k - 1] * lo[k], prev));
-        x[k + 1] = prev;
-    }
-    cd next = x[N + 1];
-    for (int k = N - 1; k >= 0; k--) {
-        next = cscale(d[k], csub(x[k + 1], cscale(up[k], next)));
-        x[k + 1] = next;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const double ap = UP ? lane_up(a, s) : lane_dn(a, s);
+        const cd gp = UP ? lane_up(g, s) : lane_dn(g, s);
+        if (UP ? lane >= s : lane + s < 64) {
+            g = cadd(cscale(a, gp), g);
+            a *= ap;
+        }
     }
 }
 
-// solver_up_5 / solver_dn_5 (parallel_solver.f90:426-593), one thread:
-// x[k + 2], ghosts two deep on each side
-__device__ void fd_penta_solve(int N, const cd *d, const cd *l1, const cd *l2, const cd *u1, const cd *u2, cd *x)
+// (p, q) -> M (p, q) + g with M = [[m00, m01], [m10, m11]]
+struct LaneAff2 {
+    cd m00, m01, m10, m11, g0, g1;
+};
+__device__ __forceinline__ LaneAff2 lane_shift(const LaneAff2 &t, int s, bool up)
 {
-    cd xm2 = x[0], xm1 = x[1];
-    for (int k = 0; k < N; k++) {
-        const cd v = csub(csub(cmul(d[k], x[k + 2]), cmul(l1[k], xm1)), cmul(l2[k], xm2));
-        x[k + 2] = v;
-        xm2 = xm1;
-        xm1 = v;
+    LaneAff2 o;
+    o.m00 = up ? lane_up(t.m00, s) : lane_dn(t.m00, s);
+    o.m01 = up ? lane_up(t.m01, s) : lane_dn(t.m01, s);
+    o.m10 = up ? lane_up(t.m10, s) : lane_dn(t.m10, s);
+    o.m11 = up ? lane_up(t.m11, s) : lane_dn(t.m11, s);
+    o.g0 = up ? lane_up(t.g0, s) : lane_dn(t.g0, s);
+    o.g1 = up ? lane_up(t.g1, s) : lane_dn(t.g1, s);
+    return o;
+}
+template <bool UP> __device__ __forceinline__ void lane_scan2(LaneAff2 &t)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const LaneAff2 o = lane_shift(t, s, UP);
+        if (UP ? lane >= s : lane + s < 64) {
+            // t after o: M = Mt Mo, g = Mt go + gt
+            const cd g0 = cfma(t.m00, o.g0, cfma(t.m01, o.g1, t.g0));
+            const cd g1 = cfma(t.m10, o.g0, cfma(t.m11, o.g1, t.g1));
+            const cd n00 = cfma(t.m00, o.m00, cmul(t.m01, o.m10)), n01 = cfma(t.m00, o.m01, cmul(t.m01, o.m11));
+            const cd n10 = cfma(t.m10, o.m00, cmul(t.m11, o.m10)), n11 = cfma(t.m10, o.m01, cmul(t.m11, o.m11));
+            t.m00 = n00; t.m01 = n01; t.m10 = n10; t.m11 = n11; t.g0 = g0; t.g1 = g1;
+        }
     }
-    cd xp1 = x[N + 2], xp2 = x[N + 3];
-    for (int k = N - 1; k >= 0; k--) {
-        const cd v = csub(csub(x[k + 2], cmul(u1[k], xp1)), cmul(u2[k], xp2));
-        x[k + 2] = v;
-        xp2 = xp1;
-        xp1 = v;
+}
+
+// tridiagonal: x[k + 1] for k in [0, N), ghosts x[0] and x[N + 1]; d holds
+// 1/pivot
+__device__ __forceinline__ void fd_tri_solve_wave(int N, const double *__restrict__ d, const double *__restrict__ lo,
+                                                  const double *__restrict__ up, cd *x)
+{
+    const int lane = threadIdx.x & 63, C = (N + 63) >> 6;
+    const int k0 = min(N, lane * C), k1 = min(N, k0 + C);
+    // forward: y_k = x[k + 1] - c_k y_{k - 1}, c_0 = lo_0, c_k = d_{k-1} lo_k, y_{-1} = x[0]
+    double a = 1.0;
+    cd g = mk(0.0, 0.0);
+    for (int k = k0; k < k1; k++) {
+        const double c = k == 0 ? lo[0] : d[k - 1] * lo[k];
+        g = csub(x[k + 1], cscale(c, g));
+        a = -c * a;
     }
+    lane_scan1<true>(a, g);
+    const cd x0 = x[0];
+    cd y = lane_up(cadd(cscale(a, x0), g), 1);
+    if (lane == 0) y = x0;
+    for (int k = k0; k < k1; k++) {
+        const double c = k == 0 ? lo[0] : d[k - 1] * lo[k];
+        y = csub(x[k + 1], cscale(c, y));
+        x[k + 1] = y;
+    }
+    // backward: z_k = d_k (y_k - up_k z_{k + 1}), z_N = x[N + 1]
+    a = 1.0;
+    g = mk(0.0, 0.0);
+    for (int k = k1 - 1; k >= k0; k--) {
+        g = cscale(d[k], csub(x[k + 1], cscale(up[k], g)));
+        a = -(d[k] * up[k]) * a;
+    }
+    lane_scan1<false>(a, g);
+    const cd xn = x[N + 1];
+    cd zv = lane_dn(cadd(cscale(a, xn), g), 1);
+    if (lane == 63) zv = xn;
+    for (int k = k1 - 1; k >= k0; k--) {
+        zv = cscale(d[k], csub(x[k + 1], cscale(up[k], zv)));
+        x[k + 1] = zv;
+    }
+}
+
+// one sweep v_k = b_k - c1_k p - c2_k q of the pentadiagonal solve, the
+// state (p, q) entering row k being (v_{k-1}, v_{k-2}) going down (FWD) or
+// (v_{k+1}, v_{k+2}) coming back; b_k = d_k x[k + 2] going down, x[k + 2]
+// coming back
+template <bool FWD>
+__device__ __forceinline__ void fd_penta_sweep_wave(int N, const cd *__restrict__ dg, const cd *__restrict__ c1,
+                                                    const cd *__restrict__ c2, cd *x, cd p_in, cd q_in)
+{
+    const int lane = threadIdx.x & 63, C = (N + 63) >> 6;
+    const int k0 = min(N, lane * C), k1 = min(N, k0 + C);
+    const int n = k1 - k0;
+    const cd z = mk(0.0, 0.0), one = mk(1.0, 0.0);
+    LaneAff2 t{one, z, z, one, z, z};
+    for (int i = 0; i < n; i++) {
+        const int k = FWD ? k0 + i : k1 - 1 - i;
+        const cd a1 = c1[k], a2 = c2[k];
+        const cd b = FWD ? cmul(dg[k], x[k + 2]) : x[k + 2];
+        const cd n0 = csub(csub(b, cmul(a1, t.g0)), cmul(a2, t.g1));
+        t.g1 = t.g0;
+        t.g0 = n0;
+        const cd r00 = cneg(cfma(a1, t.m00, cmul(a2, t.m10))), r01 = cneg(cfma(a1, t.m01, cmul(a2, t.m11)));
+        t.m10 = t.m00;
+        t.m11 = t.m01;
+        t.m00 = r00;
+        t.m01 = r01;
+    }
+    lane_scan2<FWD>(t);
+    cd p = cfma(t.m00, p_in, cfma(t.m01, q_in, t.g0)), q = cfma(t.m10, p_in, cfma(t.m11, q_in, t.g1));
+    p = FWD ? lane_up(p, 1) : lane_dn(p, 1);
+    q = FWD ? lane_up(q, 1) : lane_dn(q, 1);
+    if (lane == (FWD ? 0 : 63)) { p = p_in; q = q_in; }
+    for (int i = 0; i < n; i++) {
+        const int k = FWD ? k0 + i : k1 - 1 - i;
+        const cd b = FWD ? cmul(dg[k], x[k + 2]) : x[k + 2];
+        const cd v = csub(csub(b, cmul(c1[k], p)), cmul(c2[k], q));
+        x[k + 2] = v;
+        q = p;
+        p = v;
+    }
+}
+
+// pentadiagonal: x[k + 2], ghosts two deep on each side
+__device__ __forceinline__ void fd_penta_solve_wave(int N, const cd *d, const cd *l1, const cd *l2, const cd *u1,
+                                                    const cd *u2, cd *x)
+{
+    fd_penta_sweep_wave<true>(N, d, l1, l2, x, x[1], x[0]);
+    fd_penta_sweep_wave<false>(N, d, u1, u2, x, x[N + 2], x[N + 3]);
 }
 
 // prepare_mat_3 (parallel_solver.f90:135-187), one thread
@@ -103,9 +221,10 @@ __device__ void fd_lu5(int N, cd *d, cd *l1, cd *l2, cd *u1, cd *u2)
     }
 }
 
-// LDS layout of the finite-difference kernels (16-byte blocks): ghosted
-// columns gT, gX (N + 2), gP (N + 4; u_phi of m = 0 at offset 1), fifteen
-// N-vectors, then the factorised systems of this m.
+// LDS layout of the start, rebuild and assembly kernels of the
+// finite-difference method (16-byte blocks): ghosted columns gT, gX
+// (N + 2), gP (N + 4; u_phi of m = 0 at offset 1), six N-vectors, then the
+// factorised systems of this m.
 struct FdLayout {
     size_t gT, gX, gP, vec, mT, mX, mP, total;
     __host__ __device__ FdLayout(int N)
@@ -115,7 +234,7 @@ struct FdLayout {
         gT = take(sizeof(cd) * (N + 2));
         gX = take(sizeof(cd) * (N + 2));
         gP = take(sizeof(cd) * (N + 4));
-        vec = take(sizeof(cd) * 15 * (size_t)N);
+        vec = take(sizeof(cd) * 6 * (size_t)N);
         mT = take(sizeof(double) * 3 * (size_t)N);
         mX = take(sizeof(double) * 3 * (size_t)N);
         mP = take(sizeof(cd) * 5 * (size_t)N);
@@ -346,9 +465,82 @@ __global__ void __launch_bounds__(256) k_fd_build(const PzDev *__restrict__ pd, 
     fd_build_body(*pd, blockIdx.x, force);
 }
 
+// LDS of the m-loop kernel (16-byte blocks): the ghosted columns gT, gX, gP
+// of the right-hand sides (then of the solutions) and the factorised
+// systems of this m; small enough for three workgroups per CU at n_r 257.
+struct FdLoopLayout {
+    size_t gT, gX, gP, mT, mX, mP, total;
+    __host__ __device__ FdLoopLayout(int N)
+    {
+        size_t o = 0;
+        auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
+        gT = take(sizeof(cd) * (N + 2));
+        gX = take(sizeof(cd) * (N + 2));
+        gP = take(sizeof(cd) * (N + 4));
+        mT = take(sizeof(double) * 3 * (size_t)N);
+        mX = take(sizeof(double) * 3 * (size_t)N);
+        mP = take(sizeof(cd) * 5 * (size_t)N);
+        total = o;
+    }
+};
+
+// imex_split of a multistep scheme at one point, the stage's right-hand
+// side (P + w e) + Q, and rotate_imex (multistep_schemes.f90:462-498) of
+// the point: every slot moves up by one from the values the split read,
+// slot 0 of the explicit terms being e.  Multistep schemes keep at most
+// four slots of each kind.
+__device__ __forceinline__ cd fd_rhs_multistep(const PzDev &p, const PzScheme *s, cd *old, cd *impl, cd *expl,
+                                               size_t q, double w, cd e)
+{
+    const size_t F = (size_t)p.nm * p.N;
+    const cd z = mk(0.0, 0.0);
+    cd vo[4], vi[4], ve[4];
+#pragma unroll
+    for (int o = 0; o < 4; o++) {
+        vo[o] = o < p.nold ? gld(old, o * F + q) : z;
+        vi[o] = o < p.nimp ? gld(impl, o * F + q) : z;
+        ve[o] = o >= 1 && o < p.nexp ? gld(expl, o * F + q) : z;
+    }
+    cd r = cscale(s->wimp[0], vo[0]);
+#pragma unroll
+    for (int o = 1; o < 4; o++)
+        if (o < p.nold) r = cadd(r, cscale(s->wimp[o], vo[o]));
+#pragma unroll
+    for (int o = 0; o < 4; o++)
+        if (o < p.nimp) r = cadd(r, cscale(s->wimp_lin[o + 1], vi[o]));
+    cd Q = z;
+#pragma unroll
+    for (int o = 1; o < 4; o++)
+        if (o < p.nexp) {
+            const cd t = cscale(s->wexp[o], ve[o]);
+            Q = o == 1 ? t : cadd(Q, t);
+        }
+#pragma unroll
+    for (int o = 3; o >= 1; o--) {
+        if (o < p.nexp) expl[o * F + q] = o == 1 ? e : ve[o - 1];
+        if (o < p.nold) old[o * F + q] = vo[o - 1];
+        if (o < p.nimp) impl[o * F + q] = vi[o - 1];
+    }
+    return cadd(cadd(r, cscale(w, e)), Q);
+}
+
+// The right-hand side of one field at one point: multistep (with the slot
+// rotation) or DIRK (imex_split)
+__device__ __forceinline__ cd fd_rhs_point(const PzDev &p, const PzScheme *s, int istage, cd *old, cd *impl, cd *expl,
+                                           size_t off, int k, double w, cd e)
+{
+    if (p.family == 0) return fd_rhs_multistep(p, s, old, impl, expl, off + k, w, e);
+    cd a, b;
+    imex_split(p, s, istage, old, impl, expl, off, k, a, b);
+    return cadd(cadd(a, cscale(w, e)), b);
+}
+
 // One stage of mloop_Rdist (mloop_fd.f90:148-239) for one m, with
 // finish_explicit_assembly_Rdist (mloop_fd.f90:78-110) when exp_calc.
 // Buoyancy is explicit with finite differences (namelists.f90:459-466).
+// One pass over the points forms the explicit terms (the radial stencils
+// read their neighbours from memory), the right-hand sides and the slot
+// rotation; one wave per field then solves its system.
 __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage, int exp_calc)
 {
     extern __shared__ double smem[];
@@ -356,13 +548,9 @@ __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage
     const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
     const int is = istage - 1;
     const PzScheme *ss = p.ss;
-    const FdLayout lay(N);
+    const FdLoopLayout lay(N);
     char *lb = (char *)smem;
     cd *gT = (cd *)(lb + lay.gT), *gX = (cd *)(lb + lay.gX), *gP = (cd *)(lb + lay.gP);
-    cd *V = (cd *)(lb + lay.vec);
-    cd *PT = V, *QT = V + N, *PX = V + 2 * N, *QX = V + 3 * N, *PP = V + 4 * N, *QP = V + 5 * N;
-    cd *eT = V + 6 * N, *eX = V + 7 * N, *eP = V + 8 * N, *aT = V + 9 * N, *aX = V + 10 * N, *aP = V + 11 * N;
-    cd *usv = V + 12 * N, *Tv = V + 13 * N, *Xv = V + 14 * N;
     double *mT = (double *)(lb + lay.mT), *mX = (double *)(lb + lay.mX);
     cd *mP = (cd *)(lb + lay.mP);
     const bool hT = p.l_heat, hX = p.l_chem;
@@ -372,66 +560,43 @@ __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage
     if (hX) lds_copy(mX, p.fX + off * 3, sizeof(double) * 3 * N);
     if (m == 0) lds_copy(mP, p.fU, sizeof(double) * 3 * N);
     else lds_copy(mP, p.fP + off * 5, sizeof(cd) * 5 * N);
+    const double wcur = imex_weight(p, ss, istage);
+    const cd z = mk(0.0, 0.0);
+    // explicit terms, right-hand sides with their boundary rows (prepare_*_fd)
     for (int k = tid; k < N; k += blockDim.x) {
         const size_t q = off + k, qe = is * F + q;
-        if (hT) eT[k] = gld(p.T_expl, qe);
-        if (hX) eX[k] = gld(p.X_expl, qe);
-        eP[k] = gld(p.P_expl, qe);
-        if (exp_calc) {
-            if (hT) { aT[k] = gld(p.dVsT, q); Tv[k] = gld(p.temp, q); }
-            if (hX) { aX[k] = gld(p.dVsXi, q); Xv[k] = gld(p.xi, q); }
-            aP[k] = gld(p.dVsOm, q);
-            usv[k] = gld(p.us, q);
-        }
-        cd a, b;
-        if (hT) { imex_split(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, a, b); PT[k] = a; QT[k] = b; }
-        if (hX) { imex_split(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, a, b); PX[k] = a; QX[k] = b; }
-        imex_split(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, a, b);
-        PP[k] = a;
-        QP[k] = b;
-    }
-    const double wcur = imex_weight(p, ss, istage);
-    __syncthreads();
-    if (exp_calc) {
-        for (int k = tid; k < N; k += blockDim.x) {
-            const double o1 = R.or1(k);
-            const cd us = usv[k];
-            if (hT) {
-                const cd v = csub(csub(eT[k], cscale(o1, fd_dr_at(S, aT, k, N))),
-                                  cscale(R.dtcond(k) - p.tadvz * R.beta(k) * R.tcond(k), us));
-                eT[k] = v;
-                p.T_expl[is * F + off + k] = v;
-            }
-            if (hX) {
-                const cd v = csub(csub(eX[k], cscale(o1, fd_dr_at(S, aX, k, N))), cscale(R.dxicond(k), us));
-                eX[k] = v;
-                p.X_expl[is * F + off + k] = v;
-            }
-            if (m != 0) {
-                cd v = csub(eP[k], cscale(o1, fd_dr_at(S, aP, k, N)));
-                if (!p.l_coriolis_imp) v = cadd(v, cscale(p.CorFac * R.beta(k), us));
-                if (hT) v = csub(v, cmul(mk(0.0, p.BuoFac * R.rgrav(k) * o1 * (double)m), Tv[k]));
-                if (hX) v = csub(v, cmul(mk(0.0, p.ChemFac * R.rgrav(k) * o1 * (double)m), Xv[k]));
-                eP[k] = v;
-                p.P_expl[is * F + off + k] = v;
-            }
-        }
-        __syncthreads();
-    }
-    // right-hand sides with their boundary rows (prepare_*_fd)
-    const cd z = mk(0.0, 0.0);
-    for (int k = tid; k < N; k += blockDim.x) {
+        const double o1 = R.or1(k);
+        const cd us = exp_calc ? gld(p.us, q) : z;
         if (hT) {
-            cd v = cadd(cadd(PT[k], cscale(wcur, eT[k])), QT[k]);
+            cd e = gld(p.T_expl, qe);
+            if (exp_calc) {
+                e = csub(csub(e, cscale(o1, fd_dr_at(S, p.dVsT + off, k, N))),
+                         cscale(R.dtcond(k) - p.tadvz * R.beta(k) * R.tcond(k), us));
+                p.T_expl[qe] = e;
+            }
+            cd v = fd_rhs_point(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, wcur, e);
             if ((k == 0 && p.ktopt == 1) || (k == N - 1 && p.kbott == 1)) v = z;
             gT[k + 1] = v;
         }
         if (hX) {
-            cd v = cadd(cadd(PX[k], cscale(wcur, eX[k])), QX[k]);
+            cd e = gld(p.X_expl, qe);
+            if (exp_calc) {
+                e = csub(csub(e, cscale(o1, fd_dr_at(S, p.dVsXi + off, k, N))), cscale(R.dxicond(k), us));
+                p.X_expl[qe] = e;
+            }
+            cd v = fd_rhs_point(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, wcur, e);
             if ((k == 0 && p.ktopxi == 1) || (k == N - 1 && p.kbotxi == 1)) v = z;
             gX[k + 1] = v;
         }
-        cd v = cadd(cadd(PP[k], cscale(wcur, eP[k])), QP[k]);
+        cd e = gld(p.P_expl, qe);
+        if (exp_calc && m != 0) {
+            e = csub(e, cscale(o1, fd_dr_at(S, p.dVsOm + off, k, N)));
+            if (!p.l_coriolis_imp) e = cadd(e, cscale(p.CorFac * R.beta(k), us));
+            if (hT) e = csub(e, cmul(mk(0.0, p.BuoFac * R.rgrav(k) * o1 * (double)m), gld(p.temp, q)));
+            if (hX) e = csub(e, cmul(mk(0.0, p.ChemFac * R.rgrav(k) * o1 * (double)m), gld(p.xi, q)));
+            p.P_expl[qe] = e;
+        }
+        cd v = fd_rhs_point(p, ss, istage, p.P_old, p.P_impl, p.P_expl, off, k, wcur, e);
         if (m == 0) {
             if ((k == 0 && p.ktopv != 1) || (k == N - 1 && p.kbotv != 1)) v = z;
             gP[k + 1] = v;
@@ -446,12 +611,13 @@ __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage
         else { gP[0] = z; gP[1] = z; gP[N + 2] = z; gP[N + 3] = z; }
     }
     __syncthreads();
-    // the serial solves, one wave each
-    if (tid == 0 && hT) fd_tri_solve(N, mT, mT + N, mT + 2 * N, gT);
-    if (tid == 64 && hX) fd_tri_solve(N, mX, mX + N, mX + 2 * N, gX);
-    if (tid == 128) {
-        if (m == 0) fd_tri_solve(N, (const double *)mP, (const double *)mP + N, (const double *)mP + 2 * N, gP);
-        else fd_penta_solve(N, mP, mP + N, mP + 2 * N, mP + 3 * N, mP + 4 * N, gP);
+    // the solves, one wave each (blocks of 256 threads)
+    const int wave = tid >> 6;
+    if (wave == 0 && hT) fd_tri_solve_wave(N, mT, mT + N, mT + 2 * N, gT);
+    if (wave == 1 && hX) fd_tri_solve_wave(N, mX, mX + N, mX + 2 * N, gX);
+    if (wave == 2) {
+        if (m == 0) fd_tri_solve_wave(N, (const double *)mP, (const double *)mP + N, (const double *)mP + 2 * N, gP);
+        else fd_penta_solve_wave(N, mP, mP + N, mP + 2 * N, mP + 3 * N, mP + 4 * N, gP);
     }
     __syncthreads();
     if (tid == 0) {
@@ -459,9 +625,6 @@ __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage
         if (hX) fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi);
         fd_ghost_psi(p, R, m, gP);
     }
-    if (hT) rotate_row(p, p.T_old, p.T_impl, p.T_expl, off);
-    if (hX) rotate_row(p, p.X_old, p.X_impl, p.X_expl, off);
-    rotate_row(p, p.P_old, p.P_impl, p.P_expl, off);
     __syncthreads();
     const int inext = (istage == p.nstages) ? 1 : istage + 1;
     // implicit terms for the next stage; also before a CNAB2 start-up step
