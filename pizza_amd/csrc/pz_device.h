@@ -255,6 +255,118 @@ __device__ cd *fft_cheb(const PzDev &p, cd *a, cd *b, int ntr, int dir, const do
                         [&](int k) { return mk(cosT[k], sd * sinT[k]); });
 }
 
+// ---------------------------------------------------------------------------
+// FFTs of length L = 64 P held in registers, one wave per transform
+// (waves loop over the transforms): lane l holds x[j1 + P l],
+// j1 < P.  With j = j1 + P j2 and k = k2 + 64 k1,
+//   X[k] = sum_j1 w_P^(j1 k1) w_L^(j1 k2) sum_j2 w_64^(j2 k2) x[j1 + P j2]:
+// the inner 64-point DFTs run across the lanes as six radix-2
+// decimation-in-frequency stages with xor-shuffles (lane l then holds
+// k2 = bitreverse(l)), the outer P-point DFT in registers.  No barrier:
+// ld(tr, j) supplies the input, st(tr, k, X) takes each output once;
+// tw(q) = exp(-2 pi i q / L) for the forward transform, its conjugate for
+// the backward one.  All twiddles of a lane are read before the first
+// stage, so their latencies overlap.
+// x of lane l ^ H within the wave, for a compile-time H: DPP quad
+// permutations for H = 1, 2, swizzles for H = 4, 8, 16, a permute for 32
+template <int H>
+__device__ __forceinline__ int xor_lane(int x)
+{
+    if constexpr (H == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false); // quad_perm(1,0,3,2)
+    else if constexpr (H == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false); // quad_perm(2,3,0,1)
+    else if constexpr (H <= 16) return __builtin_amdgcn_ds_swizzle(x, 0x1F | (H << 10));  // bit mode, xor H
+    else return __builtin_amdgcn_ds_bpermute(((int)(threadIdx.x & 63) ^ H) << 2, x);
+}
+template <int H>
+__device__ __forceinline__ double xor_lane_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = xor_lane<H>((int)b), hi = xor_lane<H>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// one radix-2 decimation-in-frequency stage of span H across the lanes
+template <int P, int H>
+__device__ __forceinline__ void dif_stage(cd (&v)[P], cd w, int lane)
+{
+    const bool up = (lane & H) != 0;
+#pragma unroll
+    for (int j1 = 0; j1 < P; j1++) {
+        const cd o = mk(xor_lane_d<H>(v[j1].x), xor_lane_d<H>(v[j1].y));
+        v[j1] = up ? cmul(csub(o, v[j1]), w) : cadd(v[j1], o);
+    }
+}
+
+template <int P, class LD, class ST, class TW>
+__device__ __forceinline__ void fft_wave(int ntr, LD ld, ST st, TW tw)
+{
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int k2 = (int)(__brev((unsigned)lane) >> 26);
+    cd ws[6], wk[P > 1 ? P - 1 : 1], wp[P > 1 ? P : 1];
+#pragma unroll
+    for (int s = 0; s < 6; s++) ws[s] = tw((lane & ((1 << s) - 1)) * (32 >> s) * P); // W_2h^t, h = 2^s
+#pragma unroll
+    for (int j1 = 1; j1 < P; j1++) wk[j1 - 1] = tw(j1 * k2);
+#pragma unroll
+    for (int q = 0; q < P; q++) wp[P > 1 ? q : 0] = tw(q * 64); // w_P^q
+    for (int tr = threadIdx.x >> 6; tr < ntr; tr += nw) {
+        cd v[P];
+#pragma unroll
+        for (int j1 = 0; j1 < P; j1++) v[j1] = ld(tr, j1 + P * lane);
+        dif_stage<P, 32>(v, ws[5], lane);
+        dif_stage<P, 16>(v, ws[4], lane);
+        dif_stage<P, 8>(v, ws[3], lane);
+        dif_stage<P, 4>(v, ws[2], lane);
+        dif_stage<P, 2>(v, ws[1], lane);
+        dif_stage<P, 1>(v, ws[0], lane);
+#pragma unroll
+        for (int j1 = 1; j1 < P; j1++) v[j1] = cmul(v[j1], wk[j1 - 1]);
+#pragma unroll
+        for (int k1 = 0; k1 < P; k1++) {
+            cd acc = v[0];
+#pragma unroll
+            for (int j1 = 1; j1 < P; j1++) acc = cadd(acc, cmul(v[j1], wp[(j1 * k1) % P]));
+            st(tr, k2 + 64 * k1, acc);
+        }
+    }
+}
+
+template <class LD, class ST, class TW>
+__device__ __forceinline__ void fft_wave_p(int P, int ntr, LD ld, ST st, TW tw)
+{
+    switch (P) {
+    case 1: fft_wave<1>(ntr, ld, st, tw); break;
+    case 2: fft_wave<2>(ntr, ld, st, tw); break;
+    case 3: fft_wave<3>(ntr, ld, st, tw); break;
+    default: fft_wave<4>(ntr, ld, st, tw); break;
+    }
+}
+
+// The azimuthal transforms of an n_phi = 64 P grid, P in {2, 3, 4, 6, 8}
+template <class LD, class ST, class TW>
+__device__ __forceinline__ void fft_wave_phi(int P, int ntr, LD ld, ST st, TW tw)
+{
+    switch (P) {
+    case 2: fft_wave<2>(ntr, ld, st, tw); break;
+    case 3: fft_wave<3>(ntr, ld, st, tw); break;
+    case 4: fft_wave<4>(ntr, ld, st, tw); break;
+    case 6: fft_wave<6>(ntr, ld, st, tw); break;
+    default: fft_wave<8>(ntr, ld, st, tw); break;
+    }
+}
+__device__ __forceinline__ int wave_fft_p_phi(int n)
+{
+    const int P = n >> 6;
+    return (n & 63) == 0 && (P == 2 || P == 3 || P == 4 || P == 6 || P == 8) ? P : 0;
+}
+
+// P of the register FFTs for this run's Chebyshev length, 0 when L is not
+// 64, 128, 192 or 256 (n_r 33, 65, 97, 129)
+__device__ __forceinline__ int wave_fft_p(const PzDev &p)
+{
+    return (p.L & 63) == 0 && p.L <= 256 ? p.L >> 6 : 0;
+}
+
 __device__ __forceinline__ double cc22real(cd a, cd b, int m)
 {
     return m == 0 ? a.x * b.x : 2.0 * (a.x * b.x + a.y * b.y);
@@ -329,8 +441,20 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
         bufa[t] = z;
     }
     __syncthreads();
-    cd *g = fft_stockham(bufa, bufb, n, nin, +1, p.radices, p.nrad,
+    // register FFTs (one wave per transform) where n_phi = 64 P, else LDS
+    // Stockham FFTs
+    const int Pw = wave_fft_p_phi(n);
+    cd *g;
+    if (Pw) {
+        fft_wave_phi(Pw, nin, [=](int tr, int i) { return bufa[tr * n + i]; },
+                     [=](int tr, int k, cd v) { bufb[tr * n + k] = v; },
+                     [=](int k) { const cd w = twl[k]; return mk(w.x, -w.y); });
+        __syncthreads();
+        g = bufb;
+    } else {
+        g = fft_stockham(bufa, bufb, n, nin, +1, p.radices, p.nrad,
                          [&](int k) { const cd w = twl[k]; return mk(w.x, -w.y); });
+    }
     cd *o = (g == bufa) ? bufb : bufa;
     // Courant (courant.f90:41-76)
     if (courant) {
@@ -368,7 +492,15 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
     }
     __syncthreads();
     cd *other = (o == bufa) ? bufb : bufa;
-    cd *Z = fft_stockham(o, other, n, nout, -1, p.radices, p.nrad, [&](int k) { return twl[k]; });
+    cd *Z;
+    if (Pw) {
+        fft_wave_phi(Pw, nout, [=](int tr, int i) { return o[tr * n + i]; },
+                     [=](int tr, int k, cd v) { other[tr * n + k] = v; }, [=](int k) { return twl[k]; });
+        __syncthreads();
+        Z = other;
+    } else {
+        Z = fft_stockham(o, other, n, nout, -1, p.radices, p.nrad, [&](int k) { return twl[k]; });
+    }
     // unpack, normalise and post-scale (radial_loop.f90:350-386)
     const double or1 = p.or1[nr];
     const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
@@ -957,99 +1089,6 @@ __device__ void wave_bordered_solve(int N, int nt, int kl, int ku, int ld, const
 
 // ---------------------------------------------------------------------------
 // Chebyshev transforms on LDS vectors, whole workgroup (dct_fftw.f90)
-// ---------------------------------------------------------------------------
-// Forward FFTs of length L = 64 P (P <= 4) held in registers, one wave per
-// transform (waves loop over the transforms): lane l holds x[j1 + P l],
-// j1 < P.  With j = j1 + P j2 and k = k2 + 64 k1,
-//   X[k] = sum_j1 w_P^(j1 k1) w_L^(j1 k2) sum_j2 w_64^(j2 k2) x[j1 + P j2]:
-// the inner 64-point DFTs run across the lanes as six radix-2
-// decimation-in-frequency stages with xor-shuffles (lane l then holds
-// k2 = bitreverse(l)), the outer P-point DFT in registers.  No barrier:
-// ld(tr, j) supplies the input, st(tr, k, X) takes each output once;
-// tw(q) = exp(-2 pi i q / L).  All twiddles of a lane are read before the
-// first stage, so their latencies overlap.
-// x of lane l ^ H within the wave, for a compile-time H: DPP quad
-// permutations for H = 1, 2, swizzles for H = 4, 8, 16, a permute for 32
-template <int H>
-__device__ __forceinline__ int xor_lane(int x)
-{
-    if constexpr (H == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false); // quad_perm(1,0,3,2)
-    else if constexpr (H == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false); // quad_perm(2,3,0,1)
-    else if constexpr (H <= 16) return __builtin_amdgcn_ds_swizzle(x, 0x1F | (H << 10));  // bit mode, xor H
-    else return __builtin_amdgcn_ds_bpermute(((int)(threadIdx.x & 63) ^ H) << 2, x);
-}
-template <int H>
-__device__ __forceinline__ double xor_lane_d(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = xor_lane<H>((int)b), hi = xor_lane<H>((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// one radix-2 decimation-in-frequency stage of span H across the lanes
-template <int P, int H>
-__device__ __forceinline__ void dif_stage(cd (&v)[P], cd w, int lane)
-{
-    const bool up = (lane & H) != 0;
-#pragma unroll
-    for (int j1 = 0; j1 < P; j1++) {
-        const cd o = mk(xor_lane_d<H>(v[j1].x), xor_lane_d<H>(v[j1].y));
-        v[j1] = up ? cmul(csub(o, v[j1]), w) : cadd(v[j1], o);
-    }
-}
-
-template <int P, class LD, class ST, class TW>
-__device__ __forceinline__ void fft_wave(int ntr, LD ld, ST st, TW tw)
-{
-    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    const int k2 = (int)(__brev((unsigned)lane) >> 26);
-    cd ws[6], wk[P > 1 ? P - 1 : 1], wp[P > 1 ? P : 1];
-#pragma unroll
-    for (int s = 0; s < 6; s++) ws[s] = tw((lane & ((1 << s) - 1)) * (32 >> s) * P); // W_2h^t, h = 2^s
-#pragma unroll
-    for (int j1 = 1; j1 < P; j1++) wk[j1 - 1] = tw(j1 * k2);
-#pragma unroll
-    for (int q = 0; q < P; q++) wp[P > 1 ? q : 0] = tw(q * 64); // w_P^q
-    for (int tr = threadIdx.x >> 6; tr < ntr; tr += nw) {
-        cd v[P];
-#pragma unroll
-        for (int j1 = 0; j1 < P; j1++) v[j1] = ld(tr, j1 + P * lane);
-        dif_stage<P, 32>(v, ws[5], lane);
-        dif_stage<P, 16>(v, ws[4], lane);
-        dif_stage<P, 8>(v, ws[3], lane);
-        dif_stage<P, 4>(v, ws[2], lane);
-        dif_stage<P, 2>(v, ws[1], lane);
-        dif_stage<P, 1>(v, ws[0], lane);
-#pragma unroll
-        for (int j1 = 1; j1 < P; j1++) v[j1] = cmul(v[j1], wk[j1 - 1]);
-#pragma unroll
-        for (int k1 = 0; k1 < P; k1++) {
-            cd acc = v[0];
-#pragma unroll
-            for (int j1 = 1; j1 < P; j1++) acc = cadd(acc, cmul(v[j1], wp[(j1 * k1) % P]));
-            st(tr, k2 + 64 * k1, acc);
-        }
-    }
-}
-
-template <class LD, class ST, class TW>
-__device__ __forceinline__ void fft_wave_p(int P, int ntr, LD ld, ST st, TW tw)
-{
-    switch (P) {
-    case 1: fft_wave<1>(ntr, ld, st, tw); break;
-    case 2: fft_wave<2>(ntr, ld, st, tw); break;
-    case 3: fft_wave<3>(ntr, ld, st, tw); break;
-    default: fft_wave<4>(ntr, ld, st, tw); break;
-    }
-}
-
-// P of the register FFTs for this run's Chebyshev length, 0 when L is not
-// 64, 128, 192 or 256 (n_r 33, 65, 97, 129)
-__device__ __forceinline__ int wave_fft_p(const PzDev &p)
-{
-    return (p.L & 63) == 0 && p.L <= 256 ? p.L >> 6 : 0;
-}
-
 // costf (DCT-I with cheb_fac) of nf vectors: out[f][k]
 __device__ void dct_many(const PzDev &p, VecSet<const cd> in, VecSet<cd> out, int nf, int dealias,
                          const double *cosT = nullptr)
