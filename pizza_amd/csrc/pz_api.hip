@@ -484,6 +484,7 @@ struct pz_handle {
     size_t lds_gj = 0; // k_build_coll: blocked Gauss-Jordan panels
     bool mcoll_l2 = false; // collocation m-loop reads radial rows and tables through L2
     bool radial_big = false; // k_radial_loop_big for long azimuthal grids
+    bool radial_inplace = false; // k_gather_r + k_radial_loop_inplace instead
     double dtcond_top = 0.0, dtcond_bot = 0.0; // dT_cond/dr at the walls (Nusselt numbers)
     char *arena = nullptr; // step state block (arena_layout)
     size_t arena_bytes = 0;
@@ -862,6 +863,7 @@ static void allow_full_lds(int lim)
                              (const void *)k_mphase_batch, (const void *)k_radial_loop_batch,
                              (const void *)k_build_inv_batch, (const void *)k_mcoll_batch<true>, (const void *)k_mcoll_batch<false>, (const void *)k_fd_build,
                              (const void *)k_fd_mloop, (const void *)k_fd_start, (const void *)k_radial_loop_big,
+                             (const void *)k_radial_loop_inplace,
                              (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
                              (const void *)k_start_coll_adj, (const void *)k_assemble_coll,
                              (const void *)k_assemble_fd, (const void *)k_build_coll,
@@ -1237,6 +1239,14 @@ static void setup(pz_handle *h)
     if (h->lds_radial > lim) {
         h->radial_big = true;
         h->lds_radial = 4 * (size_t)h->nphi * sizeof(cd) + 16 * sizeof(double);
+        // without composition: two columns and in-place transforms where the
+        // plan allows (k_radial_loop_inplace), the spectra by radius from
+        // k_gather_r
+        if (!d.l_chem && fft_inplace_fits(h->nphi, 2, d.radices, d.nrad, PZ_RADIAL_INPLACE_THREADS)) {
+            h->radial_inplace = true;
+            h->lds_radial = (2 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64) * sizeof(cd) + 16 * sizeof(double);
+            d.rrow = h->dalloc<cd>(4 * (size_t)h->nm * N);
+        }
     }
     // waves of k_build_inv: as many right-hand sides as fit next to the factors
     while (h->inv_waves > 1 && InvLayout(N, h->inv_waves).total > lim) h->inv_waves--;
@@ -1819,12 +1829,22 @@ extern "C" int pz_start(pz_handle *h)
 // The pieces of one stage of time_loop (step_time.f90:215-370).
 // Explicit part: transposes + radial_loop (step_time.f90:222-251), when the
 // scheme evaluates the explicit terms at this stage.
+static void launch_radial(pz_handle *h, int is, int courant)
+{
+    if (h->radial_inplace) {
+        hipLaunchKernelGGL(k_gather_r, dim3((h->nm + PZ_GATHER_TM - 1) / PZ_GATHER_TM, (h->N + PZ_GATHER_TR - 1) / PZ_GATHER_TR),
+                           dim3(256), 0, h->stream, (const PzDev *)h->ddev);
+        hipLaunchKernelGGL(k_radial_loop_inplace, dim3(h->N), dim3(PZ_RADIAL_INPLACE_THREADS), h->lds_radial, h->stream,
+                           (const PzDev *)h->ddev, is, courant);
+    } else {
+        hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N),
+                           dim3(h->radial_big ? PZ_RADIAL_BIG_THREADS : 256), h->lds_radial, h->stream,
+                           (const PzDev *)h->ddev, is, courant);
+    }
+}
 static void launch_explicit(pz_handle *h, int stage)
 {
-    if (h->sch.l_exp_calc[stage - 1])
-        hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N),
-                           dim3(h->radial_big ? PZ_RADIAL_BIG_THREADS : 256), h->lds_radial,
-                           h->stream, (const PzDev *)h->ddev, stage - 1, stage == 1 ? 1 : 0);
+    if (h->sch.l_exp_calc[stage - 1]) launch_radial(h, stage - 1, stage == 1 ? 1 : 0);
 }
 
 // Stage 1: dt_courant, the dt roll, set_weights and the rebuild flag
@@ -2851,9 +2871,7 @@ extern "C" int pz_time_kernel(pz_handle *h, int which, int reps, double *avg_ms)
                 hipLaunchKernelGGL(k_mphase, dim3(h->nm), dim3(h->mthreads), h->lds_mphase, h->stream,
                                    (const PzDev *)h->ddev, 1, 1);
             else
-                hipLaunchKernelGGL(h->radial_big ? k_radial_loop_big : k_radial_loop, dim3(h->N),
-                                   dim3(h->radial_big ? PZ_RADIAL_BIG_THREADS : 256),
-                                   h->lds_radial, h->stream, (const PzDev *)h->ddev, 0, 0);
+                launch_radial(h, 0, 0);
         }
         HIPCHK(hipEventRecord(e1, h->stream));
         HIPCHK(hipEventSynchronize(e1));

@@ -133,6 +133,69 @@ __device__ __forceinline__ cd twd(const PzDev &p, int k, int dir)
     return dir < 0 ? w : mk(w.x, -w.y);
 }
 
+// Radix-R butterfly of a Stockham stage (R = 2, 3, 4, 8): the inputs
+// x[j + r nb] times the stage twiddles tw(r tstep), outputs v[r] (to be
+// stored R outputs apart by Ns)
+template <int R, class TW>
+__device__ __forceinline__ void stockham_bfly(const cd *x, int j, int nb, int tstep, int dir, TW tw, cd (&v)[R])
+{
+    if constexpr (R == 2) {
+        const cd v0 = x[j], v1 = cmul(x[j + nb], tw(tstep));
+        v[0] = cadd(v0, v1);
+        v[1] = csub(v0, v1);
+    } else if constexpr (R == 4) {
+        const cd v0 = x[j];
+        const cd v1 = cmul(x[j + nb], tw(tstep));
+        const cd v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
+        const cd v3 = cmul(x[j + 3 * nb], tw(3 * tstep));
+        const cd a0 = cadd(v0, v2), b0 = csub(v0, v2), c0 = cadd(v1, v3);
+        const cd d0 = cimul((double)dir, csub(v1, v3));
+        v[0] = cadd(a0, c0);
+        v[1] = cadd(b0, d0);
+        v[2] = csub(a0, c0);
+        v[3] = csub(b0, d0);
+    } else if constexpr (R == 8) {
+        // radix 8 as 2 x 4: a_r = v_r + v_{r+4} gives the even outputs,
+        // (v_r - v_{r+4}) w8^r the odd ones, w8 = exp(dir i pi/4)
+        const cd v0 = x[j], v1 = cmul(x[j + nb], tw(tstep)), v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
+        const cd v3 = cmul(x[j + 3 * nb], tw(3 * tstep)), v4 = cmul(x[j + 4 * nb], tw(4 * tstep));
+        const cd v5 = cmul(x[j + 5 * nb], tw(5 * tstep)), v6 = cmul(x[j + 6 * nb], tw(6 * tstep));
+        const cd v7 = cmul(x[j + 7 * nb], tw(7 * tstep));
+        const double h = 0.70710678118654752440, sd = (double)dir;
+        const cd a0 = cadd(v0, v4), a1 = cadd(v1, v5), a2 = cadd(v2, v6), a3 = cadd(v3, v7);
+        const cd b0 = csub(v0, v4), c1 = csub(v1, v5), c2 = csub(v2, v6), c3 = csub(v3, v7);
+        const cd b1 = mk(h * (c1.x - sd * c1.y), h * (c1.y + sd * c1.x));
+        const cd b2 = cimul(sd, c2);
+        const cd b3 = mk(h * (-c3.x - sd * c3.y), h * (-c3.y + sd * c3.x));
+        {
+            const cd s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = cimul(sd, csub(a1, a3));
+            v[0] = cadd(s02, s13);
+            v[2] = cadd(d02, d13);
+            v[4] = csub(s02, s13);
+            v[6] = csub(d02, d13);
+        }
+        {
+            const cd s02 = cadd(b0, b2), d02 = csub(b0, b2), s13 = cadd(b1, b3), d13 = cimul(sd, csub(b1, b3));
+            v[1] = cadd(s02, s13);
+            v[3] = cadd(d02, d13);
+            v[5] = csub(s02, s13);
+            v[7] = csub(d02, d13);
+        }
+    } else {
+        static_assert(R == 3, "radix");
+        const double s3 = 0.86602540378443864676;
+        const cd v0 = x[j];
+        const cd v1 = cmul(x[j + nb], tw(tstep));
+        const cd v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
+        const cd t1 = cadd(v1, v2);
+        const cd t2 = cimul((double)dir * s3, csub(v1, v2));
+        const cd m0 = mk(v0.x - 0.5 * t1.x, v0.y - 0.5 * t1.y);
+        v[0] = cadd(v0, t1);
+        v[1] = cadd(m0, t2);
+        v[2] = csub(m0, t2);
+    }
+}
+
 template <class TW>
 __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *radices, int nrad, TW tw)
 {
@@ -170,65 +233,28 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
         }
         const int tot = nb * ntr;
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-            int tr = t / nb, j = t - tr * nb;
+            const int tr = t / nb, j = t - tr * nb, jm = j % Ns;
+            const int tstep = (n / (Ns * R)) * jm, idxD = (j / Ns) * Ns * R + jm;
             const cd *x = a + tr * n;
             cd *y = b + tr * n;
-            int jm = j % Ns;
-            int tstep = (n / (Ns * R)) * jm;
-            int idxD = (j / Ns) * Ns * R + jm;
             if (R == 2) {
-                cd v0 = x[j], v1 = cmul(x[j + nb], tw(tstep));
-                y[idxD] = cadd(v0, v1);
-                y[idxD + Ns] = csub(v0, v1);
+                cd v[2];
+                stockham_bfly<2>(x, j, nb, tstep, dir, tw, v);
+                y[idxD] = v[0]; y[idxD + Ns] = v[1];
             } else if (R == 4) {
-                cd v0 = x[j];
-                cd v1 = cmul(x[j + nb], tw(tstep));
-                cd v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
-                cd v3 = cmul(x[j + 3 * nb], tw(3 * tstep));
-                cd a0 = cadd(v0, v2), b0 = csub(v0, v2), c0 = cadd(v1, v3);
-                cd d0 = cimul((double)dir, csub(v1, v3));
-                y[idxD] = cadd(a0, c0);
-                y[idxD + Ns] = cadd(b0, d0);
-                y[idxD + 2 * Ns] = csub(a0, c0);
-                y[idxD + 3 * Ns] = csub(b0, d0);
+                cd v[4];
+                stockham_bfly<4>(x, j, nb, tstep, dir, tw, v);
+#pragma unroll
+                for (int r = 0; r < 4; r++) y[idxD + r * Ns] = v[r];
             } else if (R == 8) {
-                // radix 8 as 2 x 4: a_r = v_r + v_{r+4} gives the even outputs,
-                // (v_r - v_{r+4}) w8^r the odd ones, w8 = exp(dir i pi/4)
-                const cd v0 = x[j], v1 = cmul(x[j + nb], tw(tstep)), v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
-                const cd v3 = cmul(x[j + 3 * nb], tw(3 * tstep)), v4 = cmul(x[j + 4 * nb], tw(4 * tstep));
-                const cd v5 = cmul(x[j + 5 * nb], tw(5 * tstep)), v6 = cmul(x[j + 6 * nb], tw(6 * tstep));
-                const cd v7 = cmul(x[j + 7 * nb], tw(7 * tstep));
-                const double h = 0.70710678118654752440, sd = (double)dir;
-                const cd a0 = cadd(v0, v4), a1 = cadd(v1, v5), a2 = cadd(v2, v6), a3 = cadd(v3, v7);
-                const cd b0 = csub(v0, v4), c1 = csub(v1, v5), c2 = csub(v2, v6), c3 = csub(v3, v7);
-                const cd b1 = mk(h * (c1.x - sd * c1.y), h * (c1.y + sd * c1.x));
-                const cd b2 = cimul(sd, c2);
-                const cd b3 = mk(h * (-c3.x - sd * c3.y), h * (-c3.y + sd * c3.x));
-                {
-                    const cd s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = cimul(sd, csub(a1, a3));
-                    y[idxD] = cadd(s02, s13);
-                    y[idxD + 2 * Ns] = cadd(d02, d13);
-                    y[idxD + 4 * Ns] = csub(s02, s13);
-                    y[idxD + 6 * Ns] = csub(d02, d13);
-                }
-                {
-                    const cd s02 = cadd(b0, b2), d02 = csub(b0, b2), s13 = cadd(b1, b3), d13 = cimul(sd, csub(b1, b3));
-                    y[idxD + Ns] = cadd(s02, s13);
-                    y[idxD + 3 * Ns] = cadd(d02, d13);
-                    y[idxD + 5 * Ns] = csub(s02, s13);
-                    y[idxD + 7 * Ns] = csub(d02, d13);
-                }
+                cd v[8];
+                stockham_bfly<8>(x, j, nb, tstep, dir, tw, v);
+#pragma unroll
+                for (int r = 0; r < 8; r++) y[idxD + r * Ns] = v[r];
             } else if (R == 3) {
-                const double s3 = 0.86602540378443864676;
-                cd v0 = x[j];
-                cd v1 = cmul(x[j + nb], tw(tstep));
-                cd v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
-                cd t1 = cadd(v1, v2);
-                cd t2 = cimul((double)dir * s3, csub(v1, v2));
-                cd m0 = mk(v0.x - 0.5 * t1.x, v0.y - 0.5 * t1.y);
-                y[idxD] = cadd(v0, t1);
-                y[idxD + Ns] = cadd(m0, t2);
-                y[idxD + 2 * Ns] = csub(m0, t2);
+                cd v[3];
+                stockham_bfly<3>(x, j, nb, tstep, dir, tw, v);
+                y[idxD] = v[0]; y[idxD + Ns] = v[1]; y[idxD + 2 * Ns] = v[2];
             }
         }
         __syncthreads();
@@ -238,6 +264,113 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
         Ns *= R;
     }
     return a;
+}
+
+// The same transforms in place, for the long azimuthal grids: every stage
+// computes its outputs into registers, then a barrier, the stores, and a
+// barrier, so one LDS column per transform is enough.  A thread takes at
+// most IT items of a stage (fft_inplace_fits checks the plan on the host):
+// a butterfly of radix 2, 3 or 4, or four consecutive outputs of another
+// radix (8 included), which share their input reads (Horner's rule as
+// above).
+template <int R, int IT, class TW>
+__device__ __forceinline__ void inplace_bfly(cd *a, int n, int ntr, int Ns, int dir, TW tw)
+{
+    const int nb = n / R, tot = ntr * nb;
+    cd y[IT][R];
+    int d[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t = threadIdx.x + it * blockDim.x;
+        d[it] = -1;
+        if (t < tot) {
+            const int tr = t / nb, j = t - tr * nb, jm = j % Ns;
+            stockham_bfly<R>(a + tr * n, j, nb, (n / (Ns * R)) * jm, dir, tw, y[it]);
+            d[it] = tr * n + (j / Ns) * Ns * R + jm;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+        if (d[it] >= 0)
+#pragma unroll
+            for (int r = 0; r < R; r++) a[d[it] + r * Ns] = y[it][r];
+    __syncthreads();
+}
+// one Horner step a z + v
+__device__ __forceinline__ cd hstep(cd a, cd z, cd v)
+{
+    return mk(fma(a.x, z.x, fma(-a.y, z.y, v.x)), fma(a.x, z.y, fma(a.y, z.x, v.y)));
+}
+template <int IT, int G, class TW>
+__device__ __forceinline__ void inplace_prime(cd *a, int n, int ntr, int Ns, int R, TW tw)
+{
+    const int nb = n / R, ng = (R + G - 1) / G, tot = ntr * nb * ng;
+    cd y[IT][G];
+    int d[IT], cnt[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t = threadIdx.x + it * blockDim.x;
+        cnt[it] = 0;
+        d[it] = 0;
+        if (t < tot) {
+            const int u = t / ng, g = t - u * ng, tr = u / nb, j = u - tr * nb;
+            const int jm = j % Ns, tstep = (n / (Ns * R)) * jm, k0 = G * g;
+            const cd *x = a + tr * n + j;
+            cd z[G], acc[G];
+            const cd top = x[(R - 1) * nb];
+#pragma unroll
+            for (int c = 0; c < G; c++) {
+                z[c] = tw(tstep + min(k0 + c, R - 1) * nb);
+                acc[c] = top;
+            }
+            for (int r = R - 2; r >= 0; r--) {
+                const cd v = x[r * nb];
+#pragma unroll
+                for (int c = 0; c < G; c++) acc[c] = hstep(acc[c], z[c], v);
+            }
+#pragma unroll
+            for (int c = 0; c < G; c++) y[it][c] = acc[c];
+            d[it] = tr * n + (j / Ns) * Ns * R + jm + k0 * Ns;
+            cnt[it] = min(G, R - k0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int c = 0; c < G; c++)
+            if (c < cnt[it]) a[d[it] + c * Ns] = y[it][c];
+    __syncthreads();
+}
+template <class TW>
+__device__ void fft_inplace(cd *a, int n, int ntr, int dir, const int *radices, int nrad, TW tw)
+{
+    int Ns = 1;
+    for (int s = 0; s < nrad; s++) {
+        const int R = radices[s];
+        switch (R) {
+        case 2: inplace_bfly<2, 2>(a, n, ntr, Ns, dir, tw); break;
+        case 3: inplace_bfly<3, 2>(a, n, ntr, Ns, dir, tw); break;
+        case 4: inplace_bfly<4, 1>(a, n, ntr, Ns, dir, tw); break;
+        default: inplace_prime<2, 4>(a, n, ntr, Ns, R, tw); break;
+        }
+        Ns *= R;
+    }
+}
+// host check: every stage of the plan gives each of nthreads threads at
+// most the items that fft_inplace holds in registers
+inline bool fft_inplace_fits(int n, int ntr, const int *radices, int nrad, int nthreads)
+{
+    for (int s = 0; s < nrad; s++) {
+        const int R = radices[s], nb = n / R;
+        long items, cap;
+        items = (long)ntr * nb;
+        cap = R == 4 ? 1 : 2;
+        if (R != 2 && R != 3 && R != 4) items *= (R + 3) / 4;
+        if (items > cap * nthreads) return false;
+    }
+    return true;
 }
 
 // azimuthal FFT of length n_phi (twiddles exp(-2 pi i k / n_phi) in p.twid)
@@ -668,10 +801,150 @@ __device__ __forceinline__ void radial_loop_big_body(const PzDev &p, int nr, int
     }
 }
 
+// Long azimuthal grids without composition, where two columns of n_phi and
+// the in-place transforms fit (fft_inplace_fits): the spectra by radius
+// come from k_gather_r (contiguous rows), the pairs (u_s, u_phi) and
+// (omega, T) travel through their in-place complex FFTs together, and LDS
+// holds the two columns and a two-level twiddle table (64 + n_phi / 64
+// entries).
+__device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr, int is, int courant)
+{
+    extern __shared__ double smem[];
+    const int n = p.nphi, nm = p.nm, N = p.N;
+    const size_t F = (size_t)nm * N;
+    cd *s0 = (cd *)smem, *s1 = s0 + n, *t1 = s1 + n, *t2 = t1 + 64;
+    const int n2 = (n + 63) >> 6;
+    double *red = (double *)(t2 + n2);
+    const cd *Rus = p.rrow + (size_t)nr * nm, *Rup = Rus + F, *Rom = Rus + 2 * F, *RT = Rus + 3 * F;
+    // exp(-2 pi i k / n) = t1[k mod 64] t2[k / 64]
+    for (int k = threadIdx.x; k < 64 + n2; k += blockDim.x) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
+    auto fwd = [=](int k) { return cmul(t1[k & 63], t2[k >> 6]); };
+    auto bwd = [=](int k) {
+        const cd w = cmul(t1[k & 63], t2[k >> 6]);
+        return mk(w.x, -w.y);
+    };
+    // Reynolds-stress m-sum (radial_loop.f90:267-278)
+    double part = 0.0;
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) {
+        const int m = p.idx2m[i];
+        const cd usf = m == 0 ? cscale(0.5 * p.ek * p.CorFac * p.ekpump[nr], Rup[i]) : Rus[i];
+        part += cc22real(usf, Rom[i], m);
+    }
+    const double usom = block_reduce(part, red, 0);
+    // Hermitian spectra of z = a + i b for (us, up), (om, T)
+    for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) {
+        const int tr = t >= n, k = t - tr * n;
+        const cd *A = tr == 0 ? Rus : Rom;
+        const cd *B = tr == 0 ? Rup : (p.l_heat ? RT : nullptr);
+        cd z = mk(0, 0);
+        if (k == 0) {
+            z = mk(A[0].x, B ? B[0].x : 0.0);
+        } else if (k < nm) {
+            const cd a = A[k], b = B ? B[k] : mk(0, 0);
+            z = mk(a.x - b.y, a.y + b.x);
+        } else if (n - k < nm) {
+            const int kk = n - k;
+            const cd a = cconj(A[kk]), b = B ? cconj(B[kk]) : mk(0, 0);
+            z = mk(a.x - b.y, a.y + b.x);
+        }
+        s0[t] = z;
+    }
+    __syncthreads();
+    fft_inplace(s0, n, 2, +1, p.radices, p.nrad, bwd);
+    if (courant) {
+        double vr = 0.0, vh = 0.0;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const cd z = s0[j];
+            vr = fmax(vr, z.x * z.x);
+            vh = fmax(vh, z.y * z.y);
+        }
+        vr = block_reduce(vr, red, 1);
+        vh = block_reduce(vh, red, 1);
+        if (threadIdx.x == 0) {
+            const double cf = p.courfac;
+            vr = cf * cf * vr;
+            vh = cf * cf * vh;
+            double dtrk = 1.0e10, dthk = 1.0e10;
+            if (vr != 0.0) dtrk = fmin(dtrk, sqrt(p.delxr2[nr] / vr));
+            if (vh != 0.0) dthk = fmin(dthk, sqrt(p.delxh2[nr] / vh));
+            p.dtr[nr] = dtrk;
+            p.dth[nr] = dthk;
+        }
+    }
+    // nonlinear products in place (radial_loop.f90:316-325): (up X + i r us X)
+    const double rr = p.r[nr];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const cd z0 = s0[j], z1 = s1[j];
+        const double us = z0.x, up = z0.y, om = z1.x, T = z1.y;
+        s0[j] = mk(up * om, rr * us * om);
+        s1[j] = mk(up * T, rr * us * T);
+    }
+    __syncthreads();
+    fft_inplace(s0, n, p.l_heat ? 2 : 1, -1, p.radices, p.nrad, fwd);
+    // unpack, normalise and post-scale (radial_loop.f90:350-386)
+    const double or1 = p.or1[nr];
+    const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
+    const int nout = p.l_heat ? 2 : 1;
+    for (int t = threadIdx.x; t < nout * nm; t += blockDim.x) {
+        const int tr = t >= nm, i = t - tr * nm;
+        const cd *Z = tr == 0 ? s0 : s1;
+        const cd zk = Z[i], zc = cconj(Z[i == 0 ? 0 : n - i]);
+        cd A = mk(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+        cd B = mk(0.5 * (zk.y - zc.y), -0.5 * (zk.x - zc.x));
+        A = cdivr(A, (double)n);
+        B = cdivr(B, (double)n);
+        const int m = p.idx2m[i];
+        const size_t q = (size_t)i * N + nr;
+        const cd cim = mk(0.0, -or1 * (double)m);
+        if (tr == 0) {
+            p.P_expl[is * F + q] = (i == 0) ? mk(-usom, 0.0) : cmul(cim, A);
+            p.dVsOm[q] = B;
+        } else {
+            const cd v = cmul(cim, A);
+            p.T_expl[is * F + q] = mk(v.x - fbeta * B.x, v.y - fbeta * B.y);
+            p.dVsT[q] = B;
+        }
+    }
+}
+
+// u_s, u_phi, omega (and T) from the mode-major layout [m][n_r] to rows by
+// radius [f][n_r][n_m]: 16 x 32 tiles through LDS, coalesced on both sides
+#define PZ_GATHER_TM 16
+#define PZ_GATHER_TR 32
+__global__ void __launch_bounds__(256) k_gather_r(const PzDev *__restrict__ pd)
+{
+    const PzDev &p = *pd;
+    __shared__ cd tile[PZ_GATHER_TM][PZ_GATHER_TR + 1];
+    const int N = p.N, nm = p.nm;
+    const size_t F = (size_t)nm * N;
+    const int m0 = blockIdx.x * PZ_GATHER_TM, r0 = blockIdx.y * PZ_GATHER_TR;
+    const int nf = p.l_heat ? 4 : 3;
+    for (int f = 0; f < nf; f++) {
+        const cd *src = f == 0 ? p.us : f == 1 ? p.up : f == 2 ? p.om : p.temp;
+        cd *dst = p.rrow + f * F;
+        for (int t = threadIdx.x; t < PZ_GATHER_TM * PZ_GATHER_TR; t += blockDim.x) {
+            const int mm = t / PZ_GATHER_TR, rr = t % PZ_GATHER_TR;
+            if (m0 + mm < nm && r0 + rr < N) tile[mm][rr] = src[(size_t)(m0 + mm) * N + r0 + rr];
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < PZ_GATHER_TM * PZ_GATHER_TR; t += blockDim.x) {
+            const int rr = t / PZ_GATHER_TM, mm = t % PZ_GATHER_TM;
+            if (m0 + mm < nm && r0 + rr < N) dst[(size_t)(r0 + rr) * nm + m0 + mm] = tile[mm][rr];
+        }
+        __syncthreads();
+    }
+}
+
 #define PZ_RADIAL_BIG_THREADS 1024
 __global__ void __launch_bounds__(PZ_RADIAL_BIG_THREADS) k_radial_loop_big(const PzDev *__restrict__ pd, int is, int courant)
 {
     radial_loop_big_body(*pd, blockIdx.x, is, courant);
+}
+#define PZ_RADIAL_INPLACE_THREADS 1024
+__global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS) k_radial_loop_inplace(const PzDev *__restrict__ pd, int is,
+                                                                                    int courant)
+{
+    radial_loop_inplace_body(*pd, blockIdx.x, is, courant);
 }
 
 __global__ void __launch_bounds__(256) k_radial_loop(const PzDev *__restrict__ pd, int is, int courant)

@@ -145,6 +145,9 @@ struct PzDev {
     int dmat;
     double *OmI, *PsI, *dhom, *dinfl;
     const double *bcrow;
+    // long azimuthal grids: u_s, u_phi, omega and T by radius, [f][n_r][n_m]
+    // (k_gather_r), for k_radial_loop_inplace
+    cd *rrow;
 
     PzScheme *ss;
 };
