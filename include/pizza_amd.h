@@ -107,7 +107,14 @@ typedef struct pz_config {
     /* &grid / &control: radial_scheme='FD' (second-order finite differences,
        radial.f90:86-99) on the stretched grid of finite_differences.f90:87-188 */
     double fd_stretch, fd_ratio;
-    int l_finite_diff, fd_pad;
+    int l_finite_diff;
+    /* 1: the finite-difference T and xi equations take the t_top/t_bot
+       patterns as wall values (fixed value) or wall gradients (fixed flux,
+       through the ghost points), as the spectral methods do.  No reference
+       counterpart: the reference's finite differences hold the perturbation
+       at zero on the walls (update_temp_fd.f90:85-110, its flux terms
+       commented out at :142,162).  Default 0, the reference's behaviour. */
+    int l_fd_bc_pattern;
 } pz_config;
 
 typedef struct pz_handle pz_handle;

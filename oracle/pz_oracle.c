@@ -46,8 +46,11 @@ static cplx *tl_cbuf(int k, size_t n)
  * 4 flips the sign of the compositional buoyancy (ChemFac),
  * 5 doubles the compositional diffusivity (XidiffFac),
  * 6 flips the sign of the collocation influence-matrix method's explicit
- *   Ekman-pumping term (update_psi_coll_dmat.f90:527-549).
- * Read when a run is created (4, 5) or in the explicit finish (1-3, 6). */
+ *   Ekman-pumping term (update_psi_coll_dmat.f90:527-549),
+ * 7 drops the flux pattern's part of the finite-difference wall row,
+ * 8 flips the sign of the flux pattern in the finite-difference ghost point.
+ * Read when a run is created (4, 5), in the explicit finish (1-3, 6) or in
+ * the finite-difference boundary handling (7, 8). */
 int or_plant = 0;
 
 static double *tl_dbuf(size_t n)
@@ -3750,13 +3753,54 @@ static void fd_finish_exp(or_handle *h, int is)
     }
 }
 
-/* fill_ghosts_temp / fill_ghosts_xi (update_temp_fd.f90:117-169) */
-static void fd_fill_ghosts_scalar(or_handle *h, cplx *g, int ktop, int kbot)
+/* the wall patterns (t_top, t_bot per m) of the finite-difference T
+ * (is_temp) or xi equation when l_fd_bc_pattern is set, else NULL: the
+ * reference holds the perturbation at zero on the walls */
+static void fd_patterns(const or_handle *h, int is_temp, const cplx **top, const cplx **bot)
+{
+    *top = *bot = NULL;
+    if (!h->c.l_fd_bc_pattern) return;
+    *top = is_temp ? h->topt : h->topxi;
+    *bot = is_temp ? h->bott : h->botxi;
+}
+
+/* fill_ghosts_temp / fill_ghosts_xi (update_temp_fd.f90:117-169): fixed
+ * value, 2 f_0 - f_1; fixed flux, the mirror point f_1, plus
+ * 2 (r_0 - r_1) q at the outer wall and minus 2 (r_{N-2} - r_{N-1}) q at
+ * the inner one for a flux pattern q (the terms the reference comments out
+ * at :142,162) */
+static void fd_fill_ghosts_scalar(or_handle *h, cplx *g, int ktop, int kbot, const cplx *qt, const cplx *qb)
 {
     int N = h->N;
+    double dt = h->r[0] - h->r[1], db = h->r[N - 2] - h->r[N - 1];
     for (int i = 0; i < h->nm; i++) {
-        TG(g, i, -1) = ktop == 1 ? 2.0 * TG(g, i, 0) - TG(g, i, 1) : TG(g, i, 1);
-        TG(g, i, N) = kbot == 1 ? 2.0 * TG(g, i, N - 1) - TG(g, i, N - 2) : TG(g, i, N - 2);
+        TG(g, i, -1) = ktop == 1 ? 2.0 * TG(g, i, 0) - TG(g, i, 1)
+                                 : TG(g, i, 1) + (qt ? (or_plant == 8 ? -2.0 : 2.0) * dt * qt[i] : 0.0);
+        TG(g, i, N) = kbot == 1 ? 2.0 * TG(g, i, N - 1) - TG(g, i, N - 2)
+                                : TG(g, i, N - 2) - (qb ? 2.0 * db * qb[i] : 0.0);
+    }
+}
+
+/* the wall rows of a finite-difference T / xi right-hand side: the
+ * pattern value on a fixed-value wall (zero without patterns), and on a
+ * fixed-flux wall the inhomogeneous part of the ghost point moved to the
+ * right-hand side, minus the wall row's ghost coefficient times it */
+static void fd_wall_rows(or_handle *h, cplx *g, int i, int ktop, int kbot, double diff, const cplx *qt,
+                         const cplx *qb)
+{
+    int N = h->N;
+    double w = h->ts.wimp_lin[0];
+    if (ktop == 1) TG(g, i, 0) = qt ? qt[i] : 0.0;
+    else if (qt) {
+        const double *a = h->fdr, *b = h->fddr;
+        double lo = -w * diff * (b[0] + h->or1[0] * a[0]);
+        if (or_plant != 7) TG(g, i, 0) -= lo * 2.0 * (h->r[0] - h->r[1]) * qt[i];
+    }
+    if (kbot == 1) TG(g, i, N - 1) = qb ? qb[i] : 0.0;
+    else if (qb) {
+        const double *a = h->fdr + 3 * (N - 1), *b = h->fddr + 3 * (N - 1);
+        double up = -w * diff * (b[2] + h->or1[N - 1] * a[2]);
+        TG(g, i, N - 1) += up * 2.0 * (h->r[N - 2] - h->r[N - 1]) * qb[i];
     }
 }
 
@@ -3873,12 +3917,13 @@ static void fd_mloop(or_handle *h)
         if ((f == 0 && !h->l_heat) || (f == 1 && !h->l_chem)) continue;
         cplx *g = f == 0 ? h->tg : h->xg;
         int ktop = f == 0 ? h->c.ktopt : h->c.ktopxi, kbot = f == 0 ? h->c.kbott : h->c.kbotxi;
+        const cplx *qt, *qb;
+        fd_patterns(h, f == 0, &qt, &qb);
         ts_set_imex_rhs(h, t, f == 0 ? &h->dTdt : &h->dxidt, rhs);
         for (int i = 0; i < nm; i++) {
             for (int k = 0; k < N; k++) TG(g, i, k) = rhs[(size_t)i * N + k];
-            if (ktop == 1) TG(g, i, 0) = 0.0;
+            fd_wall_rows(h, g, i, ktop, kbot, f == 0 ? h->TdiffFac : h->XidiffFac, qt, qb);
             TG(g, i, -1) = 0.0;
-            if (kbot == 1) TG(g, i, N - 1) = 0.0;
             TG(g, i, N) = 0.0;
         }
     }
@@ -3910,8 +3955,13 @@ static void fd_mloop(or_handle *h)
             fd_solve5(N, h->pD + o, h->pL1 + o, h->pL2 + o, h->pU1 + o, h->pU2 + o, &PG(h->pg, i, -2));
     }
     /* ghost zones, then the arrays of the next stage */
-    if (h->l_heat) fd_fill_ghosts_scalar(h, h->tg, h->c.ktopt, h->c.kbott);
-    if (h->l_chem) fd_fill_ghosts_scalar(h, h->xg, h->c.ktopxi, h->c.kbotxi);
+    {
+        const cplx *qt, *qb;
+        fd_patterns(h, 1, &qt, &qb);
+        if (h->l_heat) fd_fill_ghosts_scalar(h, h->tg, h->c.ktopt, h->c.kbott, qt, qb);
+        fd_patterns(h, 0, &qt, &qb);
+        if (h->l_chem) fd_fill_ghosts_scalar(h, h->xg, h->c.ktopxi, h->c.kbotxi, qt, qb);
+    }
     fd_fill_ghosts_psi(h);
     int inext = t->istage == t->nstages ? 1 : t->istage + 1;
     int lcalc = t->l_imp_calc_rhs[inext - 1];
@@ -3940,13 +3990,17 @@ static void fd_from_fields(or_handle *h, int with_om)
     if (h->l_heat) {
         for (int i = 0; i < nm; i++)
             for (int k = 0; k < N; k++) TG(h->tg, i, k) = h->temp[(size_t)i * N + k];
-        fd_fill_ghosts_scalar(h, h->tg, h->c.ktopt, h->c.kbott);
+        const cplx *qt, *qb;
+        fd_patterns(h, 1, &qt, &qb);
+        fd_fill_ghosts_scalar(h, h->tg, h->c.ktopt, h->c.kbott, qt, qb);
         fd_rhs_imp_scalar(h, h->tg, h->dtemp, &h->dTdt, 1, 1, h->TdiffFac);
     }
     if (h->l_chem) {
         for (int i = 0; i < nm; i++)
             for (int k = 0; k < N; k++) TG(h->xg, i, k) = h->xi[(size_t)i * N + k];
-        fd_fill_ghosts_scalar(h, h->xg, h->c.ktopxi, h->c.kbotxi);
+        const cplx *qt, *qb;
+        fd_patterns(h, 0, &qt, &qb);
+        fd_fill_ghosts_scalar(h, h->xg, h->c.ktopxi, h->c.kbotxi, qt, qb);
         fd_rhs_imp_scalar(h, h->xg, h->dxi, &h->dxidt, 1, 1, h->XidiffFac);
     }
     /* psi is unknown: rebuild it from u_s */
@@ -4027,16 +4081,18 @@ static void fd_assemble_scalar(or_handle *h, int is_temp)
     tarray *a = is_temp ? &h->dTdt : &h->dxidt;
     cplx *g = is_temp ? h->tg : h->xg, *fld = is_temp ? h->temp : h->xi;
     int ktop = is_temp ? h->c.ktopt : h->c.ktopxi, kbot = is_temp ? h->c.kbott : h->c.kbotxi;
+    const cplx *qt, *qb;
+    fd_patterns(h, is_temp, &qt, &qb);
     ts_assemble(h, t, a, h->work);
     for (int i = 0; i < h->nm; i++) {
         for (int k = 0; k < N; k++) {
             cplx w = h->work[(size_t)i * N + k];
             TG(g, i, k) = h->idx2m[i] == 0 ? creal(w) : w;
         }
-        if (ktop == 1) TG(g, i, 0) = 0.0;
-        if (kbot == 1) TG(g, i, N - 1) = 0.0;
+        if (ktop == 1) TG(g, i, 0) = qt ? qt[i] : 0.0;
+        if (kbot == 1) TG(g, i, N - 1) = qb ? qb[i] : 0.0;
     }
-    fd_fill_ghosts_scalar(h, g, ktop, kbot);
+    fd_fill_ghosts_scalar(h, g, ktop, kbot, qt, qb);
     fd_rhs_imp_scalar(h, g, is_temp ? h->dtemp : h->dxi, a, 1, t->l_imp_calc_rhs[0],
                       is_temp ? h->TdiffFac : h->XidiffFac);
     for (int i = 0; i < h->nm; i++)
@@ -4878,9 +4934,13 @@ int or_time_series(or_handle *h, double *out, double *spec)
                 const cplx *c = f == 0 ? t : x;
                 int ktop = f == 0 ? h->c.ktopt : h->c.ktopxi, kbot = f == 0 ? h->c.kbott : h->c.kbotxi;
                 cplx *d = (f == 0 ? dT : dX) + (size_t)i * N;
+                const cplx *qt, *qb;
+                fd_patterns(h, f == 0, &qt, &qb);
+                cplx gt = qt ? 2.0 * (h->r[0] - h->r[1]) * qt[i] : 0.0;
+                cplx gb = qb ? 2.0 * (h->r[N - 2] - h->r[N - 1]) * qb[i] : 0.0;
                 for (int k = 0; k < N; k++) {
-                    cplx fm = k > 0 ? c[k - 1] : (ktop == 1 ? 2.0 * c[0] - c[1] : c[1]);
-                    cplx fp = k < N - 1 ? c[k + 1] : (kbot == 1 ? 2.0 * c[N - 1] - c[N - 2] : c[N - 2]);
+                    cplx fm = k > 0 ? c[k - 1] : (ktop == 1 ? 2.0 * c[0] - c[1] : c[1] + gt);
+                    cplx fp = k < N - 1 ? c[k + 1] : (kbot == 1 ? 2.0 * c[N - 1] - c[N - 2] : c[N - 2] - gb);
                     const double *a = h->fdr + 3 * k;
                     d[k] = a[0] * fm + a[1] * c[k] + a[2] * fp;
                 }

@@ -76,7 +76,7 @@ class PzConfig(ctypes.Structure):
         ("xi_top", ctypes.c_double * (3 * N_T_BOUNDS)),
         ("xi_bot", ctypes.c_double * (3 * N_T_BOUNDS)),
         ("fd_stretch", ctypes.c_double), ("fd_ratio", ctypes.c_double),
-        ("l_finite_diff", ctypes.c_int), ("fd_pad", ctypes.c_int),
+        ("l_finite_diff", ctypes.c_int), ("l_fd_bc_pattern", ctypes.c_int),
     ]
 
 

@@ -976,6 +976,7 @@ static void setup(pz_handle *h)
         drx[k] = 2.0 / (h->r_cmb - h->r_icb);
     }
     d.fd = c.l_finite_diff ? 1 : 0;
+    d.fd_pattern = c.l_finite_diff && c.l_fd_bc_pattern ? 1 : 0;
     if (d.fd) r = fd_grid(N, h->r_cmb, h->r_icb, c.fd_stretch, c.fd_ratio);
     for (int k = 0; k < N; k++) {
         or1[k] = 1.0 / r[k];
@@ -2166,10 +2167,10 @@ static void series_derivatives(pz_handle *h, cd *dT, cd *dX)
         const int blocks = (int)((F + 255) / 256);
         if (h->l_heat)
             hipLaunchKernelGGL(k_fd_dr_ghost, dim3(blocks), dim3(256), 0, h->stream, h->d, (const cd *)h->d.temp, dT,
-                               h->c.ktopt, h->c.kbott);
+                               h->c.ktopt, h->c.kbott, (const cd *)h->d.topt, (const cd *)h->d.bott);
         if (h->l_chem)
             hipLaunchKernelGGL(k_fd_dr_ghost, dim3(blocks), dim3(256), 0, h->stream, h->d, (const cd *)h->d.xi, dX,
-                               h->c.ktopxi, h->c.kbotxi);
+                               h->c.ktopxi, h->c.kbotxi, (const cd *)h->d.topxi, (const cd *)h->d.botxi);
     } else {
         const size_t lds = (4 * h->N + 4 * (size_t)h->d.L + 16) * sizeof(cd);
         if (h->l_heat)

@@ -354,7 +354,8 @@ __global__ void __launch_bounds__(ASM_THREADS) k_assemble_coll(const PzDev *__re
 // workgroup (assemble_stage_Rdist, mloop_fd.f90:241-274):
 // assemble_temp_Rdist / assemble_xi_Rdist (update_temp_fd.f90:317-385,
 // update_xi_fd.f90:293-361) take the assembled right-hand side as the new
-// field (real at m = 0, zero on a fixed-value wall); assemble_psi_Rloc
+// field (real at m = 0, zero on a fixed-value wall, or its pattern with
+// l_fd_bc_pattern); assemble_psi_Rloc
 // (update_psi_fd.f90:472-606) takes it as u_phi for m = 0 (zero on a
 // no-slip wall) and solves -L psi = omega with psi = 0 at both walls for
 // m /= 0 (get_elliptic_mat_Rdist, :838-888: tridiagonal, factorised here
@@ -417,13 +418,15 @@ __global__ void __launch_bounds__(256) k_assemble_fd(const PzDev *__restrict__ p
         if (hT) {
             cd v = assemble_elem(p, ss, p.T_old, p.T_impl, p.T_expl, q);
             if (m == 0) v.y = 0.0;
-            if ((k == 0 && p.ktopt == 1) || (k == N - 1 && p.kbott == 1)) v = z;
+            if (k == 0 && p.ktopt == 1) v = fd_wall(p, p.topt, mi);
+            if (k == N - 1 && p.kbott == 1) v = fd_wall(p, p.bott, mi);
             gT[k + 1] = v;
         }
         if (hX) {
             cd v = assemble_elem(p, ss, p.X_old, p.X_impl, p.X_expl, q);
             if (m == 0) v.y = 0.0;
-            if ((k == 0 && p.ktopxi == 1) || (k == N - 1 && p.kbotxi == 1)) v = z;
+            if (k == 0 && p.ktopxi == 1) v = fd_wall(p, p.topxi, mi);
+            if (k == N - 1 && p.kbotxi == 1) v = fd_wall(p, p.botxi, mi);
             gX[k + 1] = v;
         }
         cd v = assemble_elem(p, ss, p.P_old, p.P_impl, p.P_expl, q);
@@ -452,8 +455,8 @@ __global__ void __launch_bounds__(256) k_assemble_fd(const PzDev *__restrict__ p
     if (tid < 64 && m != 0) fd_tri_solve_wave(N, eD, eL, eU, gP + 1);
     __syncthreads();
     if (tid == 0) {
-        if (hT) fd_ghost_scalar(gT, N, p.ktopt, p.kbott);
-        if (hX) fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi);
+        if (hT) fd_ghost_scalar(gT, N, p.ktopt, p.kbott, R, fd_wall(p, p.topt, mi), fd_wall(p, p.bott, mi));
+        if (hX) fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi, R, fd_wall(p, p.topxi, mi), fd_wall(p, p.botxi, mi));
         fd_ghost_psi(p, R, m, gP);
     }
     __syncthreads();

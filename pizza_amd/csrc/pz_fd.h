@@ -242,11 +242,48 @@ struct FdLayout {
     }
 };
 
-// fill_ghosts_temp / fill_ghosts_xi (update_temp_fd.f90:117-169)
-__device__ __forceinline__ void fd_ghost_scalar(cd *g, int N, int ktop, int kbot)
+// fill_ghosts_temp / fill_ghosts_xi (update_temp_fd.f90:117-169): fixed
+// value 2 f_0 - f_1; fixed flux, the mirror point f_1 plus 2 (r_0 - r_1) qt
+// at the outer wall and minus 2 (r_{N-2} - r_{N-1}) qb at the inner one for
+// a flux pattern (zero unless l_fd_bc_pattern: the reference comments these
+// terms out at :142,162)
+__device__ __forceinline__ void fd_ghost_scalar(cd *g, int N, int ktop, int kbot, const RadView &R, cd qt, cd qb)
 {
-    g[0] = ktop == 1 ? csub(cscale(2.0, g[1]), g[2]) : g[2];
-    g[N + 1] = kbot == 1 ? csub(cscale(2.0, g[N]), g[N - 1]) : g[N - 1];
+    const double dt = R.r(0) - R.r(1), db = R.r(N - 2) - R.r(N - 1);
+    g[0] = ktop == 1 ? csub(cscale(2.0, g[1]), g[2]) : cadd(g[2], cscale(2.0 * dt, qt));
+    g[N + 1] = kbot == 1 ? csub(cscale(2.0, g[N]), g[N - 1]) : csub(g[N - 1], cscale(2.0 * db, qb));
+}
+
+// the wall pattern of mode mi when the finite-difference walls take them
+__device__ __forceinline__ cd fd_wall(const PzDev &p, const cd *pat, int mi)
+{
+    return p.fd_pattern ? pat[mi] : mk(0.0, 0.0);
+}
+
+// a wall row of the finite-difference T / xi right-hand side (prepare_*_fd):
+// the pattern value on a fixed-value wall (zero without patterns); on a
+// fixed-flux wall the ghost point's flux term, moved to the right-hand side
+// times minus the wall row's ghost coefficient (-w kappa (d2 + d/r) of the
+// ghost)
+__device__ __forceinline__ cd fd_wall_row(const PzDev &p, const RadView &R, const FdView &S, int k, int N, int ktop,
+                                          int kbot, double diff, cd qt, cd qb, cd v)
+{
+    const double w = p.ss->wimp_lin[0];
+    if (k == 0) {
+        if (ktop == 1) return qt;
+        if (p.fd_pattern) {
+            const double lo = -w * diff * (S.ddr(0, 0) + R.or1(0) * S.dr(0, 0));
+            v = csub(v, cscale(lo * 2.0 * (R.r(0) - R.r(1)), qt));
+        }
+    }
+    if (k == N - 1) {
+        if (kbot == 1) return qb;
+        if (p.fd_pattern) {
+            const double up = -w * diff * (S.ddr(N - 1, 2) + R.or1(N - 1) * S.dr(N - 1, 2));
+            v = cadd(v, cscale(up * 2.0 * (R.r(N - 2) - R.r(N - 1)), qb));
+        }
+    }
+    return v;
 }
 
 // fill_ghosts_psi (update_psi_fd.f90:140-215): u_phi(m = 0) at offset 1 or
@@ -575,7 +612,9 @@ __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage
                 p.T_expl[qe] = e;
             }
             cd v = fd_rhs_point(p, ss, istage, p.T_old, p.T_impl, p.T_expl, off, k, wcur, e);
-            if ((k == 0 && p.ktopt == 1) || (k == N - 1 && p.kbott == 1)) v = z;
+            if (k == 0 || k == N - 1)
+                v = fd_wall_row(p, R, S, k, N, p.ktopt, p.kbott, p.TdiffFac, fd_wall(p, p.topt, mi),
+                                fd_wall(p, p.bott, mi), v);
             gT[k + 1] = v;
         }
         if (hX) {
@@ -585,7 +624,9 @@ __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage
                 p.X_expl[qe] = e;
             }
             cd v = fd_rhs_point(p, ss, istage, p.X_old, p.X_impl, p.X_expl, off, k, wcur, e);
-            if ((k == 0 && p.ktopxi == 1) || (k == N - 1 && p.kbotxi == 1)) v = z;
+            if (k == 0 || k == N - 1)
+                v = fd_wall_row(p, R, S, k, N, p.ktopxi, p.kbotxi, p.XidiffFac, fd_wall(p, p.topxi, mi),
+                                fd_wall(p, p.botxi, mi), v);
             gX[k + 1] = v;
         }
         cd e = gld(p.P_expl, qe);
@@ -621,8 +662,8 @@ __device__ __forceinline__ void fd_mloop_body(const PzDev &p, int mi, int istage
     }
     __syncthreads();
     if (tid == 0) {
-        if (hT) fd_ghost_scalar(gT, N, p.ktopt, p.kbott);
-        if (hX) fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi);
+        if (hT) fd_ghost_scalar(gT, N, p.ktopt, p.kbott, R, fd_wall(p, p.topt, mi), fd_wall(p, p.bott, mi));
+        if (hX) fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi, R, fd_wall(p, p.topxi, mi), fd_wall(p, p.botxi, mi));
         fd_ghost_psi(p, R, m, gP);
     }
     __syncthreads();
@@ -663,8 +704,9 @@ __global__ void __launch_bounds__(256) k_fd_start(const PzDev *__restrict__ pd)
     }
     __syncthreads();
     if (tid == 0) {
-        if (p.l_heat) fd_ghost_scalar(gT, N, p.ktopt, p.kbott);
-        if (p.l_chem) fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi);
+        if (p.l_heat) fd_ghost_scalar(gT, N, p.ktopt, p.kbott, R, fd_wall(p, p.topt, mi), fd_wall(p, p.bott, mi));
+        if (p.l_chem)
+            fd_ghost_scalar(gX, N, p.ktopxi, p.kbotxi, R, fd_wall(p, p.topxi, mi), fd_wall(p, p.botxi, mi));
         fd_ghost_psi(p, R, m, gP);
     }
     __syncthreads();
@@ -686,16 +728,20 @@ __global__ void __launch_bounds__(256) k_fd_start(const PzDev *__restrict__ pd)
 // update_temp_fd.f90:117-169: fixed value 2 f_0 - f_1, fixed flux f_1),
 // then the centred three-point stencil at every point.  One thread per
 // (m, r).
-__global__ void k_fd_dr_ghost(const PzDev p __attribute__((grid_constant)), const cd *f, cd *df, int ktop, int kbot)
+__global__ void k_fd_dr_ghost(const PzDev p __attribute__((grid_constant)), const cd *f, cd *df, int ktop, int kbot,
+                              const cd *pt, const cd *pb)
 {
     const int N = p.N;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (size_t)p.nm * N) return;
-    const int k = (int)(q % N);
+    const int k = (int)(q % N), mi = (int)(q / N);
     const cd *c = f + (q - k);
     const FdView S(p.fdc, N);
-    const cd fm = k > 0 ? c[k - 1] : (ktop == 1 ? csub(cscale(2.0, c[0]), c[1]) : c[1]);
-    const cd fp = k < N - 1 ? c[k + 1] : (kbot == 1 ? csub(cscale(2.0, c[N - 1]), c[N - 2]) : c[N - 2]);
+    const RadView R(p.radc, N);
+    const cd gt = cscale(2.0 * (R.r(0) - R.r(1)), fd_wall(p, pt, mi));
+    const cd gb = cscale(2.0 * (R.r(N - 2) - R.r(N - 1)), fd_wall(p, pb, mi));
+    const cd fm = k > 0 ? c[k - 1] : (ktop == 1 ? csub(cscale(2.0, c[0]), c[1]) : cadd(c[1], gt));
+    const cd fp = k < N - 1 ? c[k + 1] : (kbot == 1 ? csub(cscale(2.0, c[N - 1]), c[N - 2]) : csub(c[N - 2], gb));
     df[q] = cadd3(cscale(S.dr(k, 0), fm), cscale(S.dr(k, 1), c[k]), cscale(S.dr(k, 2), fp));
 }
 

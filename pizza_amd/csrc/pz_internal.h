@@ -145,6 +145,9 @@ struct PzDev {
     int dmat;
     double *OmI, *PsI, *dhom, *dinfl;
     const double *bcrow;
+    // finite differences: 1 when the T and xi walls take the t_top/t_bot
+    // patterns (l_fd_bc_pattern; the reference holds them at zero)
+    int fd_pattern;
     // long azimuthal grids: u_s, u_phi, omega and T by radius, [f][n_r][n_m]
     // (k_gather_r), for k_radial_loop_inplace
     cd *rrow;

@@ -35,7 +35,7 @@ module pizza_amd_gpu
       real(c_double) :: t_top(3*PZ_N_T_BOUNDS), t_bot(3*PZ_N_T_BOUNDS)
       real(c_double) :: xi_top(3*PZ_N_T_BOUNDS), xi_bot(3*PZ_N_T_BOUNDS)
       real(c_double) :: fd_stretch, fd_ratio
-      integer(c_int) :: l_finite_diff, fd_pad
+      integer(c_int) :: l_finite_diff, l_fd_bc_pattern
    end type pz_config
 
    ! one log-step row of every time-series file, layout of struct pz_series

@@ -255,7 +255,7 @@ contains
       write(*, '(a, es24.16)') 'fd_stretch ', c%fd_stretch
       write(*, '(a, es24.16)') 'fd_ratio ', c%fd_ratio
       write(*, '(a, i0)') 'l_finite_diff ', c%l_finite_diff
-      write(*, '(a, i0)') 'fd_pad ', c%fd_pad
+      write(*, '(a, i0)') 'l_fd_bc_pattern ', c%l_fd_bc_pattern
       write(*, '(a, i0)') 'init_t ', init_t
       write(*, '(a, es24.16)') 'amp_t ', amp_t
       write(*, '(a, i0)') 'init_xi ', init_xi

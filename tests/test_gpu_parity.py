@@ -439,6 +439,44 @@ def test_finite_difference_parity(scheme):
     np.testing.assert_allclose(g.nusselt(), o.nusselt(), rtol=1e-10)
 
 
+@pytest.mark.parametrize("scheme", ["SBDF3", "CNAB2", "ARS233"])
+def test_fd_wall_pattern_parity(scheme):
+    """Finite differences with wall patterns (l_fd_bc_pattern = 1, an
+    extension whose oracle is pinned in tests/test_fd_patterns.py): an
+    m = 2 flux pattern on the outer T wall and an m = 3 fixed value on the
+    inner one (the composition takes the same patterns as fixed values),
+    the thermochemical physics at n_r = 65, against the oracle after 1 and
+    12 steps; ARS233 adds the assembly stage.  The time series (dT/dr at
+    the walls through the ghost points) are compared too."""
+    from pizza_amd import workloads as W
+    from pizza_amd.config import scheme_id
+    cfg = W.small_bpr353(65, 32)
+    cfg.l_finite_diff = 1
+    cfg.time_scheme = scheme_id(scheme)
+    cfg.dtMax = 2.0e-6
+    cfg.ktopt, cfg.kbott = 2, 1
+    cfg.l_fd_bc_pattern = 1
+    for a, v in ((cfg.t_top, (2, 0.3, -0.1)), (cfg.t_bot, (3, 0.2, 0.05))):
+        a[0], a[1], a[2] = v
+    print(f"case: FD wall patterns, scheme {scheme}", flush=True)
+    floors = _floor(cfg, [1, 12], lambda x: W.seed_run(x, cfg))
+    g, o = _pair(cfg)
+    for x in (g, o):
+        W.seed_run(x, cfg)
+        x.start()
+    for i, n in enumerate((1, 11)):
+        g.step(n)
+        o.step(n)
+        assert abs(g.time()[0] - o.time()[0]) < 1e-15
+        for f in _fields(cfg):
+            err = rel(g.get_field(f), o.get_field(f))
+            assert err < max(1e-11 if i == 0 else 1e-10, 10 * floors[i][f]), (f, i, err, floors[i][f])
+    # dT/dr of the m = 2 mode at the outer wall is the flux pattern
+    for x in (g, o):
+        assert abs(x.get_field("dtemp")[2, 0] - (0.3 - 0.1j)) < 1e-12
+    np.testing.assert_allclose(g.nusselt(), o.nusselt(), rtol=1e-10)
+
+
 def test_long_azimuthal_grid_radial_loop():
     """A grid whose radial loop does not fit the staged variant (n_phi 2046)
     runs the one-pair-at-a-time variant: FD at n_r = 33, m_max = 682,
