@@ -475,7 +475,7 @@ __device__ __forceinline__ void fft_wave_p(int P, int ntr, LD ld, ST st, TW tw)
     }
 }
 
-// The azimuthal transforms of an n_phi = 64 P grid, P in {2, 3, 4, 6, 8}
+// The azimuthal transforms of an n_phi = 64 P grid, P in {2, 3, 4, 6}
 template <class LD, class ST, class TW>
 __device__ __forceinline__ void fft_wave_phi(int P, int ntr, LD ld, ST st, TW tw)
 {
@@ -483,14 +483,13 @@ __device__ __forceinline__ void fft_wave_phi(int P, int ntr, LD ld, ST st, TW tw
     case 2: fft_wave<2>(ntr, ld, st, tw); break;
     case 3: fft_wave<3>(ntr, ld, st, tw); break;
     case 4: fft_wave<4>(ntr, ld, st, tw); break;
-    case 6: fft_wave<6>(ntr, ld, st, tw); break;
-    default: fft_wave<8>(ntr, ld, st, tw); break;
+    default: fft_wave<6>(ntr, ld, st, tw); break;
     }
 }
 __device__ __forceinline__ int wave_fft_p_phi(int n)
 {
     const int P = n >> 6;
-    return (n & 63) == 0 && (P == 2 || P == 3 || P == 4 || P == 6 || P == 8) ? P : 0;
+    return (n & 63) == 0 && (P == 2 || P == 3 || P == 4 || P == 6) ? P : 0;
 }
 
 // P of the register FFTs for this run's Chebyshev length, 0 when L is not
@@ -555,36 +554,32 @@ __device__ __forceinline__ void radial_loop_body(const PzDev &p, int nr, int is,
         part += cc22real(usf, spec[2 * nm + i], m);
     }
     double usom = block_reduce(part, red, 0);
-    // pack pairs (us,up), (om,T), (xi,0) as Hermitian spectra of z = a + i b
-    for (int t = threadIdx.x; t < nin * n; t += blockDim.x) {
-        int tr = t / n, k = t - tr * n;
-        const cd *A = spec + (2 * tr) * nm, *B = spec + (2 * tr + 1) * nm;
-        bool hasB = (2 * tr + 1) < 5;
-        cd z = mk(0, 0);
-        if (k == 0) {
-            z = mk(A[0].x, hasB ? B[0].x : 0.0);
-        } else if (k < nm) {
-            cd a = A[k], b = hasB ? B[k] : mk(0, 0);
-            z = mk(a.x - b.y, a.y + b.x); // a + i b
-        } else if (n - k < nm && n - k > 0) {
-            int kk = n - k;
-            cd a = cconj(A[kk]), b = hasB ? cconj(B[kk]) : mk(0, 0);
-            z = mk(a.x - b.y, a.y + b.x);
-        }
-        bufa[t] = z;
-    }
-    __syncthreads();
-    // register FFTs (one wave per transform) where n_phi = 64 P, else LDS
-    // Stockham FFTs
+    // pairs (us,up), (om,T), (xi,0) as Hermitian spectra of z = a + i b
+    auto packed = [=](int tr, int k) {
+        // a + i b from the m < n_m half, its conjugate from the other
+        const bool lo = k < nm, hi = !lo && n - k < nm;
+        const int kk = lo ? k : n - k;
+        if (!lo && !hi) return mk(0.0, 0.0);
+        const cd a = spec[(2 * tr) * nm + kk];
+        const cd b = (2 * tr + 1) < 5 ? spec[(2 * tr + 1) * nm + kk] : mk(0.0, 0.0);
+        if (k == 0) return mk(a.x, b.x);
+        return lo ? mk(a.x - b.y, a.y + b.x) : mk(a.x + b.y, b.x - a.y);
+    };
+    // register FFTs (one wave per transform, packing as they load) where
+    // n_phi = 64 P, else LDS Stockham FFTs of the packed columns
     const int Pw = wave_fft_p_phi(n);
     cd *g;
     if (Pw) {
-        fft_wave_phi(Pw, nin, [=](int tr, int i) { return bufa[tr * n + i]; },
-                     [=](int tr, int k, cd v) { bufb[tr * n + k] = v; },
+        fft_wave_phi(Pw, nin, packed, [=](int tr, int k, cd v) { bufb[tr * n + k] = v; },
                      [=](int k) { const cd w = twl[k]; return mk(w.x, -w.y); });
         __syncthreads();
         g = bufb;
     } else {
+        for (int t = threadIdx.x; t < nin * n; t += blockDim.x) {
+            const int tr = t / n;
+            bufa[t] = packed(tr, t - tr * n);
+        }
+        __syncthreads();
         g = fft_stockham(bufa, bufb, n, nin, +1, p.radices, p.nrad,
                          [&](int k) { const cd w = twl[k]; return mk(w.x, -w.y); });
     }
