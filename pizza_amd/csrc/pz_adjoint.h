@@ -1,5 +1,6 @@
 // Device code of libpizza_amd, part 5: the discrete adjoint of the IMEX
-// step for the collocation method and the multistep schemes.  The reference
+// step for the collocation and finite-difference methods and the multistep
+// schemes.  The reference
 // has no adjoint; every kernel here is the exact transpose of the forward
 // kernel it names, so that <J dx, y> = <dx, J^T y> holds to rounding:
 //   k_adj_probe       the forward's radial operators as dense matrices
@@ -7,12 +8,14 @@
 //   k_radial_adj      transpose of k_radial_loop, linearised at the
 //                     state the step started from
 //   k_start_coll_adj  transpose of k_start_coll with respect to T
+//   k_fd_mloop_adj    transpose of k_fd_mloop (stage 1, multistep family)
+//   k_fd_start_adj    transpose of k_fd_start with respect to T
 // Inner product: the real one, <a, b> = sum Re(conj(a) b) over every
 // complex entry, so complex-linear maps transpose to their conjugates.
 // Adjoint variables share the state layout (arena_layout), addressed
 // through a PzDev view of their own.  Included once by pz_api.hip.
 #pragma once
-#include "pz_coll.h"
+#include "pz_fd.h"
 
 struct PzAdjDev {
     // dense operators, entry (i, j) = component i of the image of the unit
@@ -39,6 +42,9 @@ struct PzAdjDev {
     // inverses may be rebuilt by a replay running beside the adjoint
     const double *T_inv_s, *X_inv_s, *U_inv_s, *T_inv_b, *X_inv_b, *U_inv_b;
     const cd *PC_inv_s, *PC_inv_b;
+    // finite differences: copies of the factorised systems (fT, fX, fU, fP)
+    const double *fT_s, *fX_s, *fU_s, *fT_b, *fX_b, *fU_b;
+    const cd *fP_s, *fP_b;
 };
 
 // out_f[j] (+)= sum_{i<n} M[j ld + i] y_f[i] for j < n: one wave per output
@@ -232,7 +238,7 @@ __global__ void __launch_bounds__(512) k_mcoll_adj(const PzDev *__restrict__ pfw
     const double *Uinv = brg ? A.U_inv_b : A.U_inv_s;
     const cd *PCinv = brg ? A.PC_inv_b : A.PC_inv_s;
     if (mi == 0 && tid == 0 &&
-        (ss->l_new_dt || (ss->lMat && ss->n_steps > 1 && !(p.bridge && ss->n_steps == p.nold))))
+        (ss->l_new_dt || (ss->lMat && ss->n_steps > 1 && !(p.bridge && ss->n_steps <= p.nold))))
         *A.flag = 1;
 
     cd *V = (cd *)smem;
@@ -705,4 +711,370 @@ __global__ void __launch_bounds__(256) k_start_coll_adj(const PzDev *__restrict_
     matTr<1>(N, A.C, VecSet<const cd>(c), VecSet<cd>(t), false);
     __syncthreads();
     for (int k = tid; k < N; k += nb) grad[off + k] = cadd(g[k], t[k]);
+}
+
+// ---------------------------------------------------------------------------
+// Finite differences
+// ---------------------------------------------------------------------------
+// sum_k D(k, j) v[k]: the transpose of fd_dr_at (one-sided at the walls)
+__device__ __forceinline__ cd fd_drT_at(const FdView &S, const cd *v, int j, int N)
+{
+    cd s = mk(0, 0);
+    for (int k = max(1, j - 1); k <= min(N - 2, j + 1); k++) s = cadd(s, cscale(S.dr(k, j - k + 1), v[k]));
+    if (j <= 2) s = cadd(s, cscale(S.top(j), v[0]));
+    if (j >= N - 3) s = cadd(s, cscale(S.bot(N - 1 - j), v[N - 1]));
+    return s;
+}
+
+// transposed substitutions of the factorised systems (the forward ones are
+// fd_tri_solve_wave / fd_penta_solve_wave), one thread, on x[0..N)
+// tridiagonal (real): A^-1 = U^-1 L^-1 with L unit lower (c_k = d_{k-1}
+// lo_k), U = diag(1/d) + up on the superdiagonal; A^-T = L^-T U^-T
+__device__ void fd_tri_solveT(int N, const double *d, const double *lo, const double *up, cd *x)
+{
+    cd s = cscale(d[0], x[0]);
+    x[0] = s;
+    for (int k = 1; k < N; k++) {
+        s = cscale(d[k], csub(x[k], cscale(up[k - 1], s)));
+        x[k] = s;
+    }
+    cd t = x[N - 1];
+    for (int k = N - 2; k >= 0; k--) {
+        t = csub(x[k], cscale(d[k] * lo[k + 1], t));
+        x[k] = t;
+    }
+}
+// pentadiagonal (complex): A^-1 = U^-1 L^-1 D with L, U unit triangular
+// (l1, l2 below, u1, u2 above); A^-H = conj(D) L^-H U^-H
+__device__ void fd_penta_solveH(int N, const cd *d, const cd *l1, const cd *l2, const cd *u1, const cd *u2, cd *x)
+{
+    cd s1 = mk(0, 0), s2 = mk(0, 0);
+    for (int k = 0; k < N; k++) {
+        cd v = x[k];
+        if (k >= 1) v = csub(v, cmul(cconj(u1[k - 1]), s1));
+        if (k >= 2) v = csub(v, cmul(cconj(u2[k - 2]), s2));
+        x[k] = v;
+        s2 = s1;
+        s1 = v;
+    }
+    cd t1 = mk(0, 0), t2 = mk(0, 0);
+    for (int k = N - 1; k >= 0; k--) {
+        cd v = x[k];
+        if (k + 1 < N) v = csub(v, cmul(cconj(l1[k + 1]), t1));
+        if (k + 2 < N) v = csub(v, cmul(cconj(l2[k + 2]), t2));
+        t2 = t1;
+        t1 = v;
+        x[k] = v;
+    }
+    for (int k = 0; k < N; k++) x[k] = cmul(cconj(d[k]), x[k]);
+}
+
+// the adjoints of a T or xi column of fd_implicit (outputs temp, T_old(1)
+// in gi, dtemp in gd, T_impl(1) in gm) on the ghosted layout G[0..N+2):
+// G[j] = gi[j-1] + sum over the 3-point stencils of rows j-2 .. j
+__device__ __forceinline__ cd fd_scalar_outT(const RadView &R, const FdView &S, int N, double kap, double dm2,
+                                             const cd *gi, const cd *gd, const cd *gm, int j)
+{
+    cd s = (j >= 1 && j <= N) ? gi[j - 1] : mk(0, 0);
+    for (int k = max(0, j - 2); k <= min(N - 1, j); k++) {
+        const int c = j - k;
+        double im = S.ddr(k, c) + R.or1(k) * S.dr(k, c);
+        if (c == 1) im -= dm2 * R.or2(k);
+        s = cadd(s, cadd(cscale(S.dr(k, c), gd[k]), cscale(kap * im, gm[k])));
+    }
+    return s;
+}
+
+// ghost points of a T / xi column (fd_ghost_scalar) transposed in place;
+// returns the adjoints of the flux patterns (outer, inner)
+__device__ __forceinline__ void fd_ghost_scalarT(cd *G, int N, int ktop, int kbot, const RadView &R, cd &qt, cd &qb)
+{
+    const cd g0 = G[0], gn = G[N + 1];
+    if (ktop == 1) { G[1] = cadd(G[1], cscale(2.0, g0)); G[2] = csub(G[2], g0); }
+    else { G[2] = cadd(G[2], g0); qt = cadd(qt, cscale(2.0 * (R.r(0) - R.r(1)), g0)); }
+    if (kbot == 1) { G[N] = cadd(G[N], cscale(2.0, gn)); G[N - 1] = csub(G[N - 1], gn); }
+    else { G[N - 1] = cadd(G[N - 1], gn); qb = csub(qb, cscale(2.0 * (R.r(N - 2) - R.r(N - 1)), gn)); }
+    G[0] = mk(0, 0);
+    G[N + 1] = mk(0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Transpose of fd_mloop_body (stage 1 of a multistep scheme, explicit part
+// included) for one m, in the layout of k_mcoll_adj: reads the adjoints of
+// the step's outputs from lam (row m) and replaces them by those of its
+// inputs; the adjoints of the radial loop's outputs go to ad, those of the
+// wall patterns (l_fd_bc_pattern) are added to ad.gtop / gbot (T) and
+// ad.gtopxi / gbotxi (xi).  The factorised systems are the copies the
+// window's steps ran with.  LDS: 3 (N + 4) + 10 N complex.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fd_mloop_adj(const PzDev *__restrict__ pfw, const PzDev *__restrict__ plam,
+                                                      const PzAdjDev *__restrict__ pad)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pfw;
+    const PzDev &L = *plam;
+    const PzAdjDev &A = *pad;
+    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi], tid = threadIdx.x, nb = blockDim.x;
+    const size_t F = (size_t)p.nm * N, off = (size_t)mi * N;
+    const PzScheme *ss = p.ss;
+    const RadView R(p.radc, N);
+    const FdView S(p.fdc, N);
+    const bool hT = p.l_heat, hX = p.l_chem;
+    const double dm = (double)m, dm2 = dm * dm, nu = p.ViscFac, cf = p.CorFac;
+    const int lcalc = p.l_imp_calc_rhs[0] || (p.bridge && ss->n_steps + 1 <= p.nold - 1);
+    const double wcur = ss->wexp[0], w = ss->wimp_lin[0];
+    const bool brg = p.bridge && ss->n_steps <= p.nold - 1;
+    const double *fT = (brg ? A.fT_b : A.fT_s) + off * 3, *fX = (brg ? A.fX_b : A.fX_s) + off * 3;
+    const double *fU = brg ? A.fU_b : A.fU_s;
+    const cd *fP = (brg ? A.fP_b : A.fP_s) + off * 5;
+    if (mi == 0 && tid == 0 &&
+        (ss->l_new_dt || (ss->lMat && ss->n_steps > 1 && !(p.bridge && ss->n_steps <= p.nold))))
+        *A.flag = 1;
+
+    cd *GT = (cd *)smem, *GX = GT + (N + 4), *GP = GX + (N + 4);
+    cd *V = GP + (N + 4);
+    cd *eT = V, *eX = V + N, *eP = V + 2 * N, *Lus0 = V + 3 * N, *LT0 = V + 4 * N, *LX0 = V + 5 * N;
+    cd *a0 = V + 6 * N, *a1 = V + 7 * N, *a2 = V + 8 * N, *a3 = V + 9 * N; // per-row adjoints, reused
+    __shared__ cd s_q[4]; // adjoints of the patterns: T outer, inner, xi outer, inner
+    if (tid < 4) s_q[tid] = mk(0, 0);
+
+    // ---- T and xi: adjoints of the outputs on the ghosted columns ----------
+    for (int f = 0; f < 2; f++) {
+        if ((f == 0 && !hT) || (f == 1 && !hX)) continue;
+        cd *G = f == 0 ? GT : GX, *e = f == 0 ? eT : eX;
+        cd *old = f == 0 ? L.T_old : L.X_old, *impl = f == 0 ? L.T_impl : L.X_impl, *expl = f == 0 ? L.T_expl : L.X_expl;
+        const cd *fld = f == 0 ? L.temp : L.xi;
+        for (int k = tid; k < N; k += nb) {
+            const size_t q = off + k;
+            a0[k] = cadd(fld[q], old[q]);
+            a1[k] = f == 0 ? L.dtemp[q] : mk(0, 0);
+            a2[k] = lcalc ? impl[q] : mk(0, 0);
+            cd ev = expl[q];
+            if (p.nexp > 1) ev = cadd(ev, expl[F + q]);
+            e[k] = ev;
+            unrotate(p, old, impl, expl, off, k, lcalc);
+        }
+        __syncthreads();
+        const double kap = f == 0 ? p.TdiffFac : p.XidiffFac;
+        for (int j = tid; j < N + 2; j += nb) G[j] = fd_scalar_outT(R, S, N, kap, dm2, a0, a1, a2, j);
+        __syncthreads();
+        if (tid == 0) {
+            cd qt = mk(0, 0), qb = mk(0, 0);
+            fd_ghost_scalarT(G, N, f == 0 ? p.ktopt : p.ktopxi, f == 0 ? p.kbott : p.kbotxi, R, qt, qb);
+            s_q[2 * f] = cadd(s_q[2 * f], qt);
+            s_q[2 * f + 1] = cadd(s_q[2 * f + 1], qb);
+            const double *M = f == 0 ? fT : fX;
+            fd_tri_solveT(N, M, M + N, M + 2 * N, G + 1);
+        }
+        __syncthreads();
+    }
+
+    // ---- psi (m /= 0) or u_phi (m = 0): adjoints of the outputs ------------
+    for (int k = tid; k < N; k += nb) {
+        const size_t q = off + k;
+        const cd lold = L.P_old[q], lim = lcalc ? L.P_impl[q] : mk(0, 0);
+        cd ev = L.P_expl[q];
+        if (p.nexp > 1) ev = cadd(ev, L.P_expl[F + q]);
+        eP[k] = ev;
+        const double o1 = R.or1(k), o2 = R.or2(k);
+        const cd lom = cadd(L.om[q], m == 0 ? mk(0, 0) : lold), lus = L.us[q], lup = L.up[q], lps = L.psi[q];
+        if (m == 0) {
+            // u1 direct, du, ddu
+            a0[k] = cadd(cadd(lold, lup), cadd(cscale(o1, L.om[q]), cscale(-nu * o2 - cf * R.ekpump(k), lim)));
+            a1[k] = cadd(L.om[q], cscale(nu * o1, lim));
+            a2[k] = cscale(nu, lim);
+        } else {
+            const double be = R.beta(k), db = R.dbeta(k), o3 = R.or3(k), d2b = R.d2beta(k), d3b = R.d3beta(k);
+            const double c3 = be + 2.0 * o1, c2 = 3.0 * db + 2.0 * be * o1 - o2 - 2.0 * dm2 * o2;
+            const double c1 = 3.0 * d2b + 4.0 * db * o1 - be * o2 + o3 + dm2 * o2 * (2.0 * o1 - be);
+            const double c0 = d3b + 2.0 * d2b * o1 - db * o2 + be * o3 - dm2 * o2 * (4.0 * o2 + be * o1 + db - dm2 * o2);
+            const double ek = p.l_ek_pump ? cf * R.ekpump(k) : 0.0, eu = p.l_ek_pump ? R.ekp_up(k) : 0.0;
+            // lddp, ldp, lps, ld3, ld4
+            const cd lddp = cadd(cscale(-1.0, lom), cscale(-nu * c2 + ek, lim));
+            const cd ldp = csub(cadd(cscale(-(o1 + be), lom), cscale(-nu * c1 + ek * (o1 + be + eu), lim)), lup);
+            cd lp = csub(csub(lps, cscale(be, lup)), cscale(be * o1 + db - dm2 * o2, lom));
+            lp = cadd(lp, ci(-dm * o1, lus));
+            lp = cadd(lp, cscale(-nu * c0, lim));
+            if (p.l_coriolis_imp) lp = cadd(lp, ci(-cf * be * dm * o1, lim));
+            if (p.l_ek_pump) {
+                const cd P0 = mk(db + be * o1 - dm2 * o2 + be * R.ekp_up(k) + dm2 * o1 * R.ekp_dusdp(k),
+                                 -dm * o1 * R.ekp_us(k));
+                lp = cadd(lp, cscale(ek, cmul(cconj(P0), lim)));
+            }
+            a0[k] = lp;
+            a1[k] = ldp;
+            a2[k] = lddp;
+            a3[k] = cscale(-nu, lim);        // d4 weight 1
+            Lus0[k] = cscale(-nu * c3, lim); // d3 (weight c3), kept in Lus0 until the solve
+        }
+        unrotate(p, L.P_old, L.P_impl, L.P_expl, off, k, lcalc);
+    }
+    __syncthreads();
+    if (m == 0) {
+        for (int j = tid; j < N + 2; j += nb) {
+            cd s = (j >= 1 && j <= N) ? a0[j - 1] : mk(0, 0);
+            for (int k = max(0, j - 2); k <= min(N - 1, j); k++) {
+                const int c = j - k;
+                s = cadd(s, cadd(cscale(S.dr(k, c), a1[k]), cscale(S.ddr(k, c), a2[k])));
+            }
+            GP[j] = s;
+        }
+    } else {
+        for (int j = tid; j < N + 4; j += nb) {
+            cd s = (j >= 2 && j <= N + 1) ? a0[j - 2] : mk(0, 0);
+            for (int k = max(0, j - 4); k <= min(N - 1, j); k++) {
+                const int c = j - k; // f_c of row k
+                if (c >= 1 && c <= 3) s = cadd(s, cadd(cscale(S.dr(k, c - 1), a1[k]), cscale(S.ddr(k, c - 1), a2[k])));
+                s = cadd(s, cadd(cscale(S.d3(k, c), Lus0[k]), cscale(S.d4(k, c), a3[k])));
+            }
+            GP[j] = s;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (m == 0) {
+            const double dr = R.r(1) - R.r(0), db = R.r(N - 1) - R.r(N - 2);
+            const cd g0 = GP[0], gn = GP[N + 1];
+            if (p.ktopv == 1) { GP[2] = cadd(GP[2], g0); GP[1] = cadd(GP[1], cscale(-2.0 * dr * R.or1(0), g0)); }
+            else { GP[2] = csub(GP[2], g0); GP[1] = cadd(GP[1], cscale(2.0, g0)); }
+            if (p.kbotv == 1) { GP[N - 1] = cadd(GP[N - 1], gn); GP[N] = cadd(GP[N], cscale(2.0 * db * R.or1(N - 1), gn)); }
+            else { GP[N - 1] = csub(GP[N - 1], gn); GP[N] = cadd(GP[N], cscale(2.0, gn)); }
+            fd_tri_solveT(N, fU, fU + N, fU + 2 * N, GP + 1);
+        } else {
+            const double dr = R.r(1) - R.r(0), db = R.r(N - 1) - R.r(N - 2);
+            const double ct = p.ktopv == 1 ? -(1.0 - 0.5 * R.or1(0) * dr) / (1.0 + 0.5 * R.or1(0) * dr) : 1.0;
+            const double cb = p.kbotv == 1 ? -(1.0 + 0.5 * R.or1(N - 1) * db) / (1.0 - 0.5 * R.or1(N - 1) * db) : 1.0;
+            GP[3] = cadd(GP[3], cscale(ct, GP[1]));
+            GP[N] = cadd(GP[N], cscale(cb, GP[N + 2]));
+            fd_penta_solveH(N, fP, fP + N, fP + 2 * N, fP + 3 * N, fP + 4 * N, GP + 2);
+        }
+    }
+    __syncthreads();
+
+    // ---- right-hand sides back to the slots and the explicit terms --------
+    for (int k = tid; k < N; k += nb) {
+        Lus0[k] = mk(0, 0);
+        LT0[k] = mk(0, 0);
+        LX0[k] = mk(0, 0);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // wall rows of T and xi: pattern values or flux terms
+        for (int f = 0; f < 2; f++) {
+            if ((f == 0 && !hT) || (f == 1 && !hX)) continue;
+            cd *G = f == 0 ? GT : GX;
+            const int ktop = f == 0 ? p.ktopt : p.ktopxi, kbot = f == 0 ? p.kbott : p.kbotxi;
+            const double kap = f == 0 ? p.TdiffFac : p.XidiffFac;
+            if (ktop == 1) { s_q[2 * f] = cadd(s_q[2 * f], G[1]); G[1] = mk(0, 0); }
+            else {
+                const double lo = -w * kap * (S.ddr(0, 0) + R.or1(0) * S.dr(0, 0));
+                s_q[2 * f] = cadd(s_q[2 * f], cscale(-lo * 2.0 * (R.r(0) - R.r(1)), G[1]));
+            }
+            if (kbot == 1) { s_q[2 * f + 1] = cadd(s_q[2 * f + 1], G[N]); G[N] = mk(0, 0); }
+            else {
+                const double up = -w * kap * (S.ddr(N - 1, 2) + R.or1(N - 1) * S.dr(N - 1, 2));
+                s_q[2 * f + 1] = cadd(s_q[2 * f + 1], cscale(up * 2.0 * (R.r(N - 2) - R.r(N - 1)), G[N]));
+            }
+        }
+        if (m == 0) {
+            if (p.ktopv != 1) GP[1] = mk(0, 0);
+            if (p.kbotv != 1) GP[N] = mk(0, 0);
+        } else {
+            GP[2] = mk(0, 0);
+            GP[N + 1] = mk(0, 0);
+        }
+        if (p.fd_pattern) {
+            A.gtop[mi] = cadd(A.gtop[mi], s_q[0]);
+            A.gbot[mi] = cadd(A.gbot[mi], s_q[1]);
+            A.gtopxi[mi] = cadd(A.gtopxi[mi], s_q[2]);
+            A.gbotxi[mi] = cadd(A.gbotxi[mi], s_q[3]);
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < N; k += nb) {
+        const double o1 = R.or1(k);
+        if (hT) {
+            const cd rb = GT[k + 1];
+            eT[k] = cadd(eT[k], cscale(wcur, rb));
+            rhs_to_slots(p, ss, L.T_old, L.T_impl, L.T_expl, off, k, rb);
+            A.eT0[off + k] = eT[k];
+            Lus0[k] = csub(Lus0[k], cscale(R.dtcond(k) - p.tadvz * R.beta(k) * R.tcond(k), eT[k]));
+        }
+        if (hX) {
+            const cd rb = GX[k + 1];
+            eX[k] = cadd(eX[k], cscale(wcur, rb));
+            rhs_to_slots(p, ss, L.X_old, L.X_impl, L.X_expl, off, k, rb);
+            A.eX0[off + k] = eX[k];
+            Lus0[k] = csub(Lus0[k], cscale(R.dxicond(k), eX[k]));
+        }
+        const cd rb = m == 0 ? GP[k + 1] : GP[k + 2];
+        eP[k] = cadd(eP[k], cscale(wcur, rb));
+        rhs_to_slots(p, ss, L.P_old, L.P_impl, L.P_expl, off, k, rb);
+        A.eP0[off + k] = eP[k];
+        if (m != 0) {
+            if (!p.l_coriolis_imp) Lus0[k] = cadd(Lus0[k], cscale(cf * R.beta(k), eP[k]));
+            if (hT) LT0[k] = ci(dm * p.BuoFac * R.rgrav(k) * o1, eP[k]);
+            if (hX) LX0[k] = ci(dm * p.ChemFac * R.rgrav(k) * o1, eP[k]);
+        }
+    }
+    __syncthreads();
+    // adjoints of the dV fields: -o1 times the transposed one-sided stencil
+    for (int k = tid; k < N; k += nb) {
+        a0[k] = cscale(-R.or1(k), eT[k]);
+        a1[k] = cscale(-R.or1(k), eX[k]);
+        a2[k] = m == 0 ? mk(0, 0) : cscale(-R.or1(k), eP[k]);
+    }
+    __syncthreads();
+    for (int j = tid; j < N; j += nb) {
+        const size_t q = off + j;
+        if (hT) A.aT0[q] = fd_drT_at(S, a0, j, N);
+        if (hX) A.aX0[q] = fd_drT_at(S, a1, j, N);
+        A.aP0[q] = fd_drT_at(S, a2, j, N);
+        L.us[q] = Lus0[j];
+        L.up[q] = mk(0, 0);
+        L.om[q] = mk(0, 0);
+        L.psi[q] = mk(0, 0);
+        L.temp[q] = LT0[j];
+        L.xi[q] = LX0[j];
+        L.dtemp[q] = mk(0, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transpose of k_fd_start with respect to T: the initial T field fills the
+// ghosted column, whose fd_implicit outputs are temp, dtemp, T_old(1) and
+// T_impl(1); the flux patterns' adjoints of the ghost points go to
+// ad.gtop / gbot.  LDS: 4 N + 4 complex.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fd_start_adj(const PzDev *__restrict__ pd, const PzDev *__restrict__ plam,
+                                                      const PzAdjDev *__restrict__ pad, cd *grad)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pd;
+    const PzDev &L = *plam;
+    const PzAdjDev &A = *pad;
+    const int mi = blockIdx.x, N = p.N, m = p.idx2m[mi], tid = threadIdx.x, nb = blockDim.x;
+    const size_t off = (size_t)mi * N;
+    const RadView R(p.radc, N);
+    const FdView S(p.fdc, N);
+    cd *G = (cd *)smem, *gi = G + N + 4, *gd = gi + N, *gm = gd + N;
+    for (int k = tid; k < N; k += nb) {
+        gi[k] = cadd(L.temp[off + k], L.T_old[off + k]);
+        gd[k] = L.dtemp[off + k];
+        gm[k] = L.T_impl[off + k];
+    }
+    __syncthreads();
+    for (int j = tid; j < N + 2; j += nb)
+        G[j] = fd_scalar_outT(R, S, N, p.TdiffFac, (double)m * (double)m, gi, gd, gm, j);
+    __syncthreads();
+    if (tid == 0) {
+        cd qt = mk(0, 0), qb = mk(0, 0);
+        fd_ghost_scalarT(G, N, p.ktopt, p.kbott, R, qt, qb);
+        if (p.fd_pattern) {
+            A.gtop[mi] = cadd(A.gtop[mi], qt);
+            A.gbot[mi] = cadd(A.gbot[mi], qb);
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < N; k += nb) grad[off + k] = G[k + 1];
 }

@@ -866,6 +866,7 @@ static void allow_full_lds(int lim)
                              (const void *)k_radial_loop_inplace,
                              (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
                              (const void *)k_start_coll_adj, (const void *)k_assemble_coll,
+                             (const void *)k_fd_mloop_adj, (const void *)k_fd_start_adj,
                              (const void *)k_assemble_fd, (const void *)k_build_coll,
                              (const void *)k_build_coll_batch, (const void *)k_mcoll_l2};
     for (const void *k : kernels) {
@@ -2501,6 +2502,19 @@ extern "C" int pz_nusselt(pz_handle *h, double out[2])
 // segment from its snapshot, keeping the state before every step of it,
 // and runs the transposed step kernels backwards through the segment.
 // ---------------------------------------------------------------------------
+// the implicit systems a step reads (pointer, bytes): the collocation
+// inverses, or the factorised finite-difference bands
+static std::vector<std::pair<double *, size_t>> adj_live(pz_handle *h)
+{
+    const size_t nm = h->nm, N = h->N;
+    if (h->d.fd)
+        return {{h->d.fT, nm * 3 * N * sizeof(double)}, {h->d.fX, nm * 3 * N * sizeof(double)},
+                {h->d.fU, 3 * N * sizeof(double)}, {(double *)h->d.fP, nm * 5 * N * sizeof(cd)}};
+    const size_t ti = nm * N * h->d.ldi, pc = nm * N * 2 * N, ui = N * h->d.ldi;
+    return {{h->d.T_inv, ti * sizeof(double)}, {h->d.X_inv, ti * sizeof(double)}, {h->d.U_inv, ui * sizeof(double)},
+            {(double *)h->d.PC_inv, pc * sizeof(cd)}};
+}
+
 static void adj_init(pz_handle *h)
 {
     if (h->adj) return;
@@ -2534,18 +2548,28 @@ static void adj_init(pz_handle *h)
     ad.gtopxi = h->dalloc<cd>(nm); ad.gbotxi = h->dalloc<cd>(nm);
     ad.flag = h->dalloc<int>(1);
     {
-        // copies of the inverses: S = those of the scheme's own weights,
-        // B = those of the CNAB2 start-up steps of a bridged scheme
-        const size_t ti = (size_t)nm * N * h->d.ldi, pc = (size_t)nm * N * 2 * N, ui = (size_t)N * h->d.ldi;
-        a->mat_bytes = {ti * sizeof(double), ti * sizeof(double), ui * sizeof(double), pc * sizeof(cd)};
-        a->mats_s = {h->dalloc<double>(ti), h->dalloc<double>(ti), h->dalloc<double>(ui), (double *)h->dalloc<cd>(pc)};
-        a->mats_b = h->d.bridge ? std::vector<double *>{h->dalloc<double>(ti), h->dalloc<double>(ti),
-                                                        h->dalloc<double>(ui), (double *)h->dalloc<cd>(pc)}
-                                : a->mats_s;
-        ad.T_inv_s = a->mats_s[0]; ad.X_inv_s = a->mats_s[1]; ad.U_inv_s = a->mats_s[2];
-        ad.PC_inv_s = (const cd *)a->mats_s[3];
-        ad.T_inv_b = a->mats_b[0]; ad.X_inv_b = a->mats_b[1]; ad.U_inv_b = a->mats_b[2];
-        ad.PC_inv_b = (const cd *)a->mats_b[3];
+        // copies of the implicit systems (the inverses of the collocation
+        // method, the factorised bands of finite differences): S = those of
+        // the scheme's own weights, B = those of the CNAB2 start-up steps of
+        // a bridged scheme
+        a->mat_bytes.clear();
+        for (auto &lv : adj_live(h)) a->mat_bytes.push_back(lv.second);
+        auto alloc_set = [&]() {
+            std::vector<double *> v;
+            for (size_t b : a->mat_bytes) v.push_back((double *)h->dalloc<char>(b));
+            return v;
+        };
+        a->mats_s = alloc_set();
+        a->mats_b = h->d.bridge ? alloc_set() : a->mats_s;
+        if (h->d.fd) {
+            ad.fT_s = a->mats_s[0]; ad.fX_s = a->mats_s[1]; ad.fU_s = a->mats_s[2]; ad.fP_s = (const cd *)a->mats_s[3];
+            ad.fT_b = a->mats_b[0]; ad.fX_b = a->mats_b[1]; ad.fU_b = a->mats_b[2]; ad.fP_b = (const cd *)a->mats_b[3];
+        } else {
+            ad.T_inv_s = a->mats_s[0]; ad.X_inv_s = a->mats_s[1]; ad.U_inv_s = a->mats_s[2];
+            ad.PC_inv_s = (const cd *)a->mats_s[3];
+            ad.T_inv_b = a->mats_b[0]; ad.X_inv_b = a->mats_b[1]; ad.U_inv_b = a->mats_b[2];
+            ad.PC_inv_b = (const cd *)a->mats_b[3];
+        }
     }
     HIPCHK(hipStreamCreate(&a->s2));
     HIPCHK(hipEventCreateWithFlags(&a->ev[0], hipEventDisableTiming));
@@ -2561,6 +2585,17 @@ static void adj_init(pz_handle *h)
     a->acc = h->dalloc<double>(1);
     a->W = h->dalloc<cd>(F);
     a->grad = h->dalloc<cd>(F);
+    if (h->d.fd) {
+        // dT/dr at the outer wall as the step takes it (the ghost-point
+        // stencil of row 0, fd_implicit): a_1 T_0 + a_2 T_1 + a_0 T_-1 with
+        // T_-1 = 2 T_0 - T_1 (fixed value) or T_1 (fixed flux)
+        std::vector<double> row(N, 0.0), fdc(3);
+        HIPCHK(hipMemcpy(fdc.data(), h->d.fdc, 3 * sizeof(double), hipMemcpyDeviceToHost));
+        if (h->c.ktopt == 1) { row[0] = fdc[1] + 2.0 * fdc[0]; row[1] = fdc[2] - fdc[0]; }
+        else { row[0] = fdc[1]; row[1] = fdc[2] + fdc[0]; }
+        HIPCHK(hipMemcpy(lnu, row.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        return;
+    }
     const size_t lds = (6 * (size_t)N + 6 * (size_t)h->d.L + 16) * sizeof(cd) + 3 * (size_t)N * sizeof(double);
     hipLaunchKernelGGL(k_adj_probe, dim3(N), dim3(256), lds, h->stream, (const PzDev *)h->ddev, C, G, D1, D2, E1, U,
                        lnu);
@@ -2608,18 +2643,22 @@ static void adj_window(pz_handle *h, int K, int ck)
 // copies the live inverses to a set (to_live = false) or back
 static void adj_mats(pz_handle *h, const std::vector<double *> &set, bool to_live)
 {
-    double *live[4] = {h->d.T_inv, h->d.X_inv, h->d.U_inv, (double *)h->d.PC_inv};
-    for (int i = 0; i < 4; i++)
-        HIPCHK(hipMemcpyAsync(to_live ? live[i] : set[i], to_live ? set[i] : live[i], h->adj->mat_bytes[i],
-                              hipMemcpyDeviceToDevice, h->stream));
+    const auto live = adj_live(h);
+    for (size_t i = 0; i < live.size(); i++)
+        if (live[i].first && live[i].second)
+            HIPCHK(hipMemcpyAsync(to_live ? live[i].first : set[i], to_live ? set[i] : live[i].first,
+                                  h->adj->mat_bytes[i], hipMemcpyDeviceToDevice, h->stream));
 }
 
 static int adj_check(pz_handle *h)
 {
     if (!h->started) return fail(h, -5, "pz_start has not been called");
-    if (!h->d.coll || h->d.fd || h->sch.family != 0)
-        return fail(h, -2, "the adjoint covers the collocation method with a multistep time scheme");
-    if (h->c.l_rerror_fix) return fail(h, -2, "the adjoint needs l_rerror_fix = .false. (thresholds are not linear)");
+    if (!(h->d.coll || h->d.fd) || h->sch.family != 0)
+        return fail(h, -2, "the adjoint covers the collocation and finite-difference methods with a multistep "
+                           "time scheme");
+    if (h->d.coll && h->c.l_rerror_fix)
+        return fail(h, -2, "the adjoint needs l_rerror_fix = .false. (thresholds are not linear)");
+    if (h->radial_big) return fail(h, -2, "the adjoint needs the staged radial loop (n_phi too long)");
     if (!h->l_heat) return fail(h, -2, "the adjoint objectives need the temperature equation");
     if (h->nphi < 2 * h->nm) return fail(h, -2, "the adjoint needs n_phi >= 2 n_m");
     if (h->d.dmat) return fail(h, -2, "the adjoint covers matrix_solve='direct'");
@@ -2708,6 +2747,7 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
         const int K = a->K, ck = a->ck, nm = h->nm, N = h->N;
         const size_t lds_radj = (11 * (size_t)nm + 9 * (size_t)h->nphi) * sizeof(cd);
         const size_t lds_madj = ADJ_NVEC * (size_t)N * sizeof(cd);
+        const size_t lds_fadj = (3 * ((size_t)N + 4) + 10 * (size_t)N) * sizeof(cd);
         const double tiny = 10.0 * 2.220446049250313e-16;
         double wnu = 0.0;
         if (a->objective == PZ_OBJ_NUTOP && std::fabs(h->dtcond_top) > tiny) wnu = dJ / ((double)K * h->dtcond_top);
@@ -2763,8 +2803,12 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
                 } else if (n == K)
                     hipLaunchKernelGGL(k_adj_seed_w, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, a->s2,
                                        (const PzDev *)a->dlam, (const cd *)a->grad);
-                hipLaunchKernelGGL(k_mcoll_adj, dim3(nm), dim3(512), lds_madj, a->s2, fw, (const PzDev *)a->dlam,
-                                   (const PzAdjDev *)a->dad);
+                if (h->d.fd)
+                    hipLaunchKernelGGL(k_fd_mloop_adj, dim3(nm), dim3(256), lds_fadj, a->s2, fw,
+                                       (const PzDev *)a->dlam, (const PzAdjDev *)a->dad);
+                else
+                    hipLaunchKernelGGL(k_mcoll_adj, dim3(nm), dim3(512), lds_madj, a->s2, fw, (const PzDev *)a->dlam,
+                                       (const PzAdjDev *)a->dad);
                 hipLaunchKernelGGL(k_radial_adj, dim3(N), dim3(256), lds_radj, a->s2, fw, (const PzDev *)a->dlam,
                                    (const PzAdjDev *)a->dad);
             }
@@ -2783,8 +2827,12 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
             adjoint(sg);
         }
         HIPCHK(hipStreamWaitEvent(h->stream, a->ev[2], 0));
-        hipLaunchKernelGGL(k_start_coll_adj, dim3(nm), dim3(256), 6 * (size_t)N * sizeof(cd), h->stream,
-                           (const PzDev *)h->ddev, (const PzDev *)a->dlam, (const PzAdjDev *)a->dad, a->grad);
+        if (h->d.fd)
+            hipLaunchKernelGGL(k_fd_start_adj, dim3(nm), dim3(256), (4 * (size_t)N + 4) * sizeof(cd), h->stream,
+                               (const PzDev *)h->ddev, (const PzDev *)a->dlam, (const PzAdjDev *)a->dad, a->grad);
+        else
+            hipLaunchKernelGGL(k_start_coll_adj, dim3(nm), dim3(256), 6 * (size_t)N * sizeof(cd), h->stream,
+                               (const PzDev *)h->ddev, (const PzDev *)a->dlam, (const PzAdjDev *)a->dad, a->grad);
         HIPCHK(hipMemcpyAsync(h->arena, a->fin, A, hipMemcpyDeviceToDevice, h->stream));
         adj_mats(h, a->mats_s, true);
         HIPCHK(hipGetLastError());
