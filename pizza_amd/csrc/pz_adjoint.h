@@ -605,6 +605,101 @@ __global__ void __launch_bounds__(256) k_radial_adj(const PzDev *__restrict__ pf
     }
 }
 
+// ---------------------------------------------------------------------------
+// Transpose of radial_loop_inplace_body (long azimuthal grids without
+// composition), linearised at fw, for one radius; the steps of k_radial_adj
+// with the spectra by radius from k_gather_r (run on fw, into fw's rrow),
+// the transforms of fft_inplace (two at a time) and four LDS columns: the
+// forward grid (us, up), (omega, T) and the adjoint product columns.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS)
+k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ plam, const PzAdjDev *__restrict__ pad)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pfw;
+    const PzDev &L = *plam;
+    const PzAdjDev &A = *pad;
+    const int nr = blockIdx.x, n = p.nphi, nm = p.nm, N = p.N, tid = threadIdx.x, nb = blockDim.x;
+    const size_t F = (size_t)nm * N;
+    cd *gf = (cd *)smem, *sa = gf + 2 * n, *t1 = sa + 2 * n, *t2 = t1 + 64;
+    const int n2 = (n + 63) >> 6;
+    const bool hT = p.l_heat;
+    const cd *Rus = p.rrow + (size_t)nr * nm, *Rup = Rus + F, *Rom = Rus + 2 * F, *RT = Rus + 3 * F;
+    for (int k = tid; k < 64 + n2; k += nb) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
+    auto fwd = [=](int k) { return cmul(t1[k & 63], t2[k >> 6]); };
+    auto bwd = [=](int k) {
+        const cd w = cmul(t1[k & 63], t2[k >> 6]);
+        return mk(w.x, -w.y);
+    };
+    const double or1 = p.or1[nr], rr = p.r[nr];
+    const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
+    // forward grid columns (us, up), (om, T); adjoint product spectra
+    // (halved above m = 0) packed the same way
+    for (int t = tid; t < 2 * n; t += nb) {
+        const int tr = t >= n, k = t - tr * n;
+        gf[t] = herm_pack(tr == 0 ? Rus : Rom, tr == 0 ? Rup : (hT ? RT : nullptr), k, n, nm);
+        const bool lo = k < nm, hi = !lo && n - k < nm;
+        cd z = mk(0, 0);
+        if (lo || hi) {
+            const int i = lo ? k : n - k, m = p.idx2m[i];
+            const size_t q = (size_t)i * N + nr;
+            cd a, b;
+            if (tr == 0) {
+                a = m == 0 ? mk(0, 0) : ci(or1 * m, A.eP0[q]);
+                b = A.aP0[q];
+            } else if (hT) {
+                const cd e = A.eT0[q];
+                a = ci(or1 * m, e);
+                b = csub(A.aT0[q], cscale(fbeta, e));
+            } else {
+                a = b = mk(0, 0);
+            }
+            const double h = i == 0 ? 1.0 : 0.5;
+            a = cscale(h, a);
+            b = cscale(h, b);
+            if (k == 0) z = mk(a.x, b.x);
+            else if (lo) z = mk(a.x - b.y, a.y + b.x);
+            else z = mk(a.x + b.y, b.x - a.y);
+        }
+        sa[t] = z;
+    }
+    __syncthreads();
+    fft_inplace(gf, n, 2, +1, p.radices, p.nrad, bwd);
+    fft_inplace(sa, n, 2, +1, p.radices, p.nrad, bwd);
+    const double inv_n = 1.0 / (double)n;
+    for (int j = tid; j < n; j += nb) {
+        const double us = gf[j].x, up = gf[j].y, om = gf[n + j].x, T = gf[n + j].y;
+        const double ao = sa[j].x * inv_n, bo = sa[j].y * inv_n;
+        const double aT = hT ? sa[n + j].x * inv_n : 0.0, bT = hT ? sa[n + j].y * inv_n : 0.0;
+        sa[j] = mk(rr * (bo * om + bT * T), ao * om + aT * T);
+        sa[n + j] = mk(ao * up + bo * rr * us, aT * up + bT * rr * us);
+    }
+    __syncthreads();
+    fft_inplace(sa, n, 2, -1, p.radices, p.nrad, fwd);
+    const double lsom = -A.eP0[nr].x; // P_expl(m = 0) = -usom
+    const double cfac = 0.5 * p.ek * p.CorFac * p.ekpump[nr];
+    for (int t = tid; t < 2 * nm; t += nb) {
+        const int tr = t >= nm, i = t - tr * nm;
+        const cd zk = sa[tr * n + i], zc = cconj(sa[tr * n + (i == 0 ? 0 : n - i)]);
+        cd a = mk(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+        cd b = mk(0.5 * (zk.y - zc.y), -0.5 * (zk.x - zc.x));
+        if (i == 0) { a = mk(a.x, 0.0); b = mk(b.x, 0.0); }
+        else { a = cscale(2.0, a); b = cscale(2.0, b); }
+        const size_t q = (size_t)i * N + nr;
+        if (tr == 0) {
+            if (i == 0) b = mk(b.x + lsom * cfac * Rom[0].x, b.y);
+            else a = cadd(a, cscale(2.0 * lsom, Rom[i]));
+            L.us[q] = cadd(L.us[q], a);
+            L.up[q] = cadd(L.up[q], b);
+        } else {
+            if (i == 0) a = mk(a.x + lsom * cfac * Rup[0].x, a.y);
+            else a = cadd(a, cscale(2.0 * lsom, Rus[i]));
+            L.om[q] = cadd(L.om[q], a);
+            if (hT) L.temp[q] = cadd(L.temp[q], b);
+        }
+    }
+}
+
 // adjoint seeds of the objectives at the state after a step
 // NuTop: lam(T, m = 0) += w lnu (real)
 __global__ void __launch_bounds__(256) k_adj_seed_nu(const PzDev *__restrict__ plam, const PzAdjDev *__restrict__ pad,

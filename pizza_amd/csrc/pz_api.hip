@@ -428,6 +428,7 @@ struct AdjCtx {
     char *lam = nullptr;     // adjoint variables (state layout)
     double *acc = nullptr;   // objective sum
     cd *W = nullptr, *grad = nullptr;
+    cd *rrow = nullptr; // spectra by radius of the replayed states (long grids)
     // window: nseg checkpoints every ck steps, ck + 1 segment snapshots
     // and their ck forward views, the state at the end of the window
     char *ckpt = nullptr, *seg = nullptr, *fin = nullptr;
@@ -867,6 +868,7 @@ static void allow_full_lds(int lim)
                              (const void *)k_adj_probe, (const void *)k_mcoll_adj, (const void *)k_radial_adj,
                              (const void *)k_start_coll_adj, (const void *)k_assemble_coll,
                              (const void *)k_fd_mloop_adj, (const void *)k_fd_start_adj,
+                             (const void *)k_radial_adj_inplace,
                              (const void *)k_assemble_fd, (const void *)k_build_coll,
                              (const void *)k_build_coll_batch, (const void *)k_mcoll_l2};
     for (const void *k : kernels) {
@@ -2585,6 +2587,7 @@ static void adj_init(pz_handle *h)
     a->acc = h->dalloc<double>(1);
     a->W = h->dalloc<cd>(F);
     a->grad = h->dalloc<cd>(F);
+    if (h->radial_inplace) a->rrow = h->dalloc<cd>(4 * F);
     if (h->d.fd) {
         // dT/dr at the outer wall as the step takes it (the ghost-point
         // stencil of row 0, fd_implicit): a_1 T_0 + a_2 T_1 + a_0 T_-1 with
@@ -2631,6 +2634,7 @@ static void adj_window(pz_handle *h, int K, int ck)
                 arena_layout(v[b * ck + j], base, F, h->sch.nold, h->sch.nimp, h->sch.nexp);
                 arena_layout(next, base + A, F, h->sch.nold, h->sch.nimp, h->sch.nexp);
                 v[b * ck + j].ss = next.ss;
+                if (a->rrow) v[b * ck + j].rrow = a->rrow; // apart from the replay's own
             }
         HIPCHK(hipMemcpy(a->dviews, v.data(), sizeof(PzDev) * 2 * ck, hipMemcpyHostToDevice));
         a->views_ck = ck;
@@ -2658,7 +2662,8 @@ static int adj_check(pz_handle *h)
                            "time scheme");
     if (h->d.coll && h->c.l_rerror_fix)
         return fail(h, -2, "the adjoint needs l_rerror_fix = .false. (thresholds are not linear)");
-    if (h->radial_big) return fail(h, -2, "the adjoint needs the staged radial loop (n_phi too long)");
+    if (h->radial_big && !h->radial_inplace)
+        return fail(h, -2, "the adjoint of the long azimuthal grids needs the run without composition");
     if (!h->l_heat) return fail(h, -2, "the adjoint objectives need the temperature equation");
     if (h->nphi < 2 * h->nm) return fail(h, -2, "the adjoint needs n_phi >= 2 n_m");
     if (h->d.dmat) return fail(h, -2, "the adjoint covers matrix_solve='direct'");
@@ -2679,8 +2684,11 @@ extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *wei
         HIPCHK(hipMemcpyAsync(&s0, h->d.ss, sizeof(s0), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         if (s0.n_steps != 0) return fail(h, -5, "the adjoint window starts right after pz_start");
-        const size_t lds_radj = (11 * (size_t)h->nm + 9 * (size_t)h->nphi) * sizeof(cd);
-        const size_t lds_madj = ADJ_NVEC * (size_t)h->N * sizeof(cd);
+        const size_t lds_radj = h->radial_inplace
+                                    ? (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64) * sizeof(cd)
+                                    : (11 * (size_t)h->nm + 9 * (size_t)h->nphi) * sizeof(cd);
+        const size_t lds_madj = h->d.fd ? (3 * ((size_t)h->N + 4) + 10 * (size_t)h->N) * sizeof(cd)
+                                        : ADJ_NVEC * (size_t)h->N * sizeof(cd);
         int dev = 0, lim = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -2748,6 +2756,7 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
         const size_t lds_radj = (11 * (size_t)nm + 9 * (size_t)h->nphi) * sizeof(cd);
         const size_t lds_madj = ADJ_NVEC * (size_t)N * sizeof(cd);
         const size_t lds_fadj = (3 * ((size_t)N + 4) + 10 * (size_t)N) * sizeof(cd);
+        const size_t lds_radj_in = (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64) * sizeof(cd);
         const double tiny = 10.0 * 2.220446049250313e-16;
         double wnu = 0.0;
         if (a->objective == PZ_OBJ_NUTOP && std::fabs(h->dtcond_top) > tiny) wnu = dJ / ((double)K * h->dtcond_top);
@@ -2809,8 +2818,16 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
                 else
                     hipLaunchKernelGGL(k_mcoll_adj, dim3(nm), dim3(512), lds_madj, a->s2, fw, (const PzDev *)a->dlam,
                                        (const PzAdjDev *)a->dad);
-                hipLaunchKernelGGL(k_radial_adj, dim3(N), dim3(256), lds_radj, a->s2, fw, (const PzDev *)a->dlam,
-                                   (const PzAdjDev *)a->dad);
+                if (h->radial_inplace) {
+                    hipLaunchKernelGGL(k_gather_r,
+                                       dim3((nm + PZ_GATHER_TM - 1) / PZ_GATHER_TM, (N + PZ_GATHER_TR - 1) / PZ_GATHER_TR),
+                                       dim3(256), 0, a->s2, fw);
+                    hipLaunchKernelGGL(k_radial_adj_inplace, dim3(N), dim3(PZ_RADIAL_INPLACE_THREADS), lds_radj_in,
+                                       a->s2, fw, (const PzDev *)a->dlam, (const PzAdjDev *)a->dad);
+                } else {
+                    hipLaunchKernelGGL(k_radial_adj, dim3(N), dim3(256), lds_radj, a->s2, fw, (const PzDev *)a->dlam,
+                                       (const PzAdjDev *)a->dad);
+                }
             }
             HIPCHK(hipEventRecord(a->ev[2 + b], a->s2));
         };

@@ -143,3 +143,24 @@ def test_fd_wall_pattern_gradient(ktopt, objective):
             fd = (js[0] - js[1]) / (2 * h)
             ad = gbc[m, 0].real if part == 1 else gbc[m, 0].imag
             assert abs(fd - ad) <= tol * max(abs(fd), abs(ad), 1e-30), (m, part, fd, ad)
+
+
+def test_fd_long_grid_two_step_dot_product():
+    """The long azimuthal grid (n_phi 2046 = 2 3 11 31, the C5 grid) takes
+    the transposed in-place radial loop (k_radial_adj_inplace after
+    k_gather_r on the replayed state): exact two-step dot-product test."""
+    cfg = _cfg("SBDF3", n_r=33, m_max=682)
+    print("case: FD adjoint dot product, n_phi 2046", flush=True)
+    run, T0, us, up, field = _setup(cfg, amp_u=1.0e-2)
+    rng = np.random.default_rng(13)
+    Wt = rng.standard_normal((run.n_m, run.n_r)) + 1j * rng.standard_normal((run.n_m, run.n_r))
+    _J(run, T0, us, up, "temp", 2, Wt)
+    g, _ = run.adjoint_backward()
+    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
+    for k in range(2):
+        d = field(1.0e-3)
+        jp = _J(run, T0 + d, us, up, "temp", 2, Wt)
+        jm = _J(run, T0 - d, us, up, "temp", 2, Wt)
+        fd = (jp - jm) / 2.0
+        ad = _dot(g, d)
+        assert abs(fd - ad) <= 1e-9 * max(abs(fd), abs(ad)), (k, fd, ad)
