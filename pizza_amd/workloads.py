@@ -157,3 +157,26 @@ def c5_fd(n_r=257, m_max=682, steps_scheme="SBDF3"):
     return default_config(
         n_r_max=n_r, n_cheb_max=n_r, m_max=m_max, minc=1, time_scheme=steps_scheme,
         radial_scheme="FD", dtMax=1.0e-7, ra=1.0e9, ek=1.0e-6, pr=0.1, radratio=0.35)
+
+
+C5_PATTERN_MODES = 16
+
+
+def c5_flux_pattern(seed=SEED, amp=0.1, modes=C5_PATTERN_MODES):
+    """The seeded outer-wall heat-flux pattern of the C5 gradient: modes
+    m = 1 .. 16, complex coefficients of modulus amp with random phases."""
+    rng = np.random.Generator(np.random.PCG64([seed, 55]))
+    ph = rng.uniform(0.0, 2.0 * np.pi, modes)
+    return [(m, amp * np.cos(p), amp * np.sin(p)) for m, p in zip(range(1, modes + 1), ph)]
+
+
+def c5_gradient(pattern=None, n_r=257, m_max=682):
+    """C5 with the heat-flux pattern on the outer wall (ktopt = 2, the
+    finite-difference wall patterns switched on): the configuration of the
+    gradient of the mean kinetic energy with respect to that pattern."""
+    c = c5_fd(n_r=n_r, m_max=m_max)
+    c.ktopt = 2
+    c.l_fd_bc_pattern = 1
+    for n, (m, tr, ti) in enumerate(pattern if pattern is not None else c5_flux_pattern()):
+        c.t_top[3 * n], c.t_top[3 * n + 1], c.t_top[3 * n + 2] = m, tr, ti
+    return c
