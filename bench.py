@@ -253,6 +253,50 @@ def fd_bench(steps=50, warmup=5):
             "steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps}
 
 
+def adjoint_fd_bench(steps=100, ckpt=10, reps=2):
+    """Adjoint cost of the C5 gradient: finite differences at n_r 257,
+    n_phi 2046, SBDF3, the mean kinetic energy over `steps` steps with
+    respect to the seeded 16-mode outer-wall heat-flux pattern (and the
+    initial temperature), snapshots every `ckpt` steps.  cost = (forward
+    sweep + backward sweep) / plain forward; best of `reps`."""
+    import numpy as np
+    cfg = W.c5_gradient()
+    run = Run(cfg)
+    W.seed_run(run, cfg, amp_t=1e-3)
+    T0 = run.get_field("temp")
+    z = np.zeros_like(T0)
+
+    def reset():
+        run.set_field("us", z)
+        run.set_field("up", z)
+        run.set_field("temp", T0)
+        run.start()
+
+    reset()
+    run.step(5)
+    run.time()
+    t_fwd = t_adj = float("inf")
+    for _ in range(reps):
+        reset()
+        t0 = time.perf_counter()
+        run.step(steps)
+        run.time()
+        t_fwd = min(t_fwd, time.perf_counter() - t0)
+    J = gbc = None
+    for _ in range(reps):
+        reset()
+        t0 = time.perf_counter()
+        J = run.adjoint_forward("ekin", steps, ckpt)
+        _, gbc = run.adjoint_backward()
+        t_adj = min(t_adj, time.perf_counter() - t0)
+    run.close()
+    return {"workload": "FD-SBDF3 n_r=257 n_phi=2046 E=1e-6 Ra=1e9 Pr=0.1 dt=1e-7, d<E_kin>/d(outer heat-flux "
+                        "pattern, m=1..16)",
+            "steps": steps, "checkpoint_every": ckpt, "forward_ms": 1e3 * t_fwd,
+            "forward_plus_adjoint_ms": 1e3 * t_adj, "cost": t_adj / t_fwd, "objective": J,
+            "grad_pattern_max_abs": float(np.abs(gbc[1:17, 0]).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,7 +326,7 @@ def main():
         print(json.dumps({"ensemble": ensemble_bench(steps=args.ensemble_steps)}))
         return
     if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_bench()}))
+        print(json.dumps({"adjoint": adjoint_bench(), "adjoint_fd": adjoint_fd_bench()}))
         return
     if args.fd_only:
         print(json.dumps({"fd": fd_bench()}))
@@ -339,6 +383,7 @@ def main():
         line["ensemble"] = ensemble_bench(steps=args.ensemble_steps)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
+        line["adjoint_fd"] = adjoint_fd_bench()
     line["fd"] = fd_bench()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
