@@ -736,20 +736,30 @@ __global__ void __launch_bounds__(256) k_obj_acc(const PzDev *__restrict__ pd, c
 
 // kinetic energy of a state (outputs.f90:552-570), us2_3D + up2_3D + uz2_3D
 // = sum over m and r of c_m (kus_r |u_s|^2 + kup_r |u_phi|^2), c_0 = 1 on
-// the real parts only, c_m = 2 for m >= 1; accumulated in acc[0]
+// the real parts only, c_m = 2 for m >= 1: one workgroup per m writes its
+// row's sum to acc[1 + mi]; k_obj_fin adds the rows, in order, to acc[0]
 __global__ void __launch_bounds__(256) k_obj_ke(const PzDev *__restrict__ pd, const PzAdjDev *__restrict__ pad,
                                                 double *acc)
 {
     __shared__ double red[8];
     const PzDev &p = *pd;
-    const size_t F = (size_t)p.nm * p.N;
+    const int mi = blockIdx.x, N = p.N;
+    const size_t off = (size_t)mi * N;
+    const bool m0 = p.idx2m[mi] == 0;
     double v = 0.0;
-    for (size_t q = threadIdx.x; q < F; q += blockDim.x) {
-        const int i = (int)(q / p.N), k = (int)(q % p.N);
-        const cd u = p.us[q], w = p.up[q];
-        if (p.idx2m[i] == 0) v += pad->kus[k] * u.x * u.x + pad->kup[k] * w.x * w.x;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        const cd u = p.us[off + k], w = p.up[off + k];
+        if (m0) v += pad->kus[k] * u.x * u.x + pad->kup[k] * w.x * w.x;
         else v += 2.0 * (pad->kus[k] * (u.x * u.x + u.y * u.y) + pad->kup[k] * (w.x * w.x + w.y * w.y));
     }
+    v = block_reduce(v, red, 0);
+    if (threadIdx.x == 0) acc[1 + mi] = v;
+}
+__global__ void __launch_bounds__(256) k_obj_fin(int nm, double *acc)
+{
+    __shared__ double red[8];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) v += acc[1 + i];
     v = block_reduce(v, red, 0);
     if (threadIdx.x == 0) acc[0] += v;
 }
@@ -822,46 +832,91 @@ __device__ __forceinline__ cd fd_drT_at(const FdView &S, const cd *v, int j, int
 }
 
 // transposed substitutions of the factorised systems (the forward ones are
-// fd_tri_solve_wave / fd_penta_solve_wave), one thread, on x[0..N)
-// tridiagonal (real): A^-1 = U^-1 L^-1 with L unit lower (c_k = d_{k-1}
-// lo_k), U = diag(1/d) + up on the superdiagonal; A^-T = L^-T U^-T
-__device__ void fd_tri_solveT(int N, const double *d, const double *lo, const double *up, cd *x)
+// fd_tri_solve_wave / fd_penta_solve_wave), on x[0..N), by one whole wave:
+// lane l owns rows [l C, l C + C), C = ceil(N / 64); each sweep is a linear
+// recurrence that every lane reduces to one affine map of the state entering
+// its rows, the maps are composed by a scan over the lanes and each lane
+// re-runs its rows from the state the scan hands it (as in the forward
+// substitutions).  The states entering the first row are zero.
+//
+// first order, y_k = c(k) y_{k-+1} + b(k, x_k), x_k <- y_k
+template <bool FWD, class CF, class BF>
+__device__ __forceinline__ void wave_sweep1(int N, cd *x, CF cf, BF bf)
 {
-    cd s = cscale(d[0], x[0]);
-    x[0] = s;
-    for (int k = 1; k < N; k++) {
-        s = cscale(d[k], csub(x[k], cscale(up[k - 1], s)));
-        x[k] = s;
+    const int lane = threadIdx.x & 63, C = (N + 63) >> 6;
+    const int k0 = min(N, lane * C), k1 = min(N, k0 + C), n = k1 - k0;
+    double a = 1.0;
+    cd g = mk(0.0, 0.0);
+    for (int i = 0; i < n; i++) {
+        const int k = FWD ? k0 + i : k1 - 1 - i;
+        const double c = cf(k);
+        g = cadd(cscale(c, g), bf(k, x[k]));
+        a *= c;
     }
-    cd t = x[N - 1];
-    for (int k = N - 2; k >= 0; k--) {
-        t = csub(x[k], cscale(d[k] * lo[k + 1], t));
-        x[k] = t;
+    lane_scan1<FWD>(a, g);
+    cd y = FWD ? lane_up(g, 1) : lane_dn(g, 1);
+    if (lane == (FWD ? 0 : 63)) y = mk(0.0, 0.0);
+    for (int i = 0; i < n; i++) {
+        const int k = FWD ? k0 + i : k1 - 1 - i;
+        y = cadd(cscale(cf(k), y), bf(k, x[k]));
+        x[k] = y;
     }
+}
+// second order, v_k = x_k - c1(k) v_{k-+1} - c2(k) v_{k-+2}, x_k <- out(k, v_k)
+template <bool FWD, class C1, class C2, class OF>
+__device__ __forceinline__ void wave_sweep2(int N, cd *x, C1 c1f, C2 c2f, OF of)
+{
+    const int lane = threadIdx.x & 63, C = (N + 63) >> 6;
+    const int k0 = min(N, lane * C), k1 = min(N, k0 + C), n = k1 - k0;
+    const cd z = mk(0.0, 0.0), one = mk(1.0, 0.0);
+    LaneAff2 t{one, z, z, one, z, z};
+    for (int i = 0; i < n; i++) {
+        const int k = FWD ? k0 + i : k1 - 1 - i;
+        const cd a1 = c1f(k), a2 = c2f(k);
+        const cd n0 = csub(csub(x[k], cmul(a1, t.g0)), cmul(a2, t.g1));
+        t.g1 = t.g0;
+        t.g0 = n0;
+        const cd r00 = cneg(cfma(a1, t.m00, cmul(a2, t.m10))), r01 = cneg(cfma(a1, t.m01, cmul(a2, t.m11)));
+        t.m10 = t.m00;
+        t.m11 = t.m01;
+        t.m00 = r00;
+        t.m01 = r01;
+    }
+    lane_scan2<FWD>(t);
+    cd pv = FWD ? lane_up(t.g0, 1) : lane_dn(t.g0, 1), qv = FWD ? lane_up(t.g1, 1) : lane_dn(t.g1, 1);
+    if (lane == (FWD ? 0 : 63)) { pv = z; qv = z; }
+    for (int i = 0; i < n; i++) {
+        const int k = FWD ? k0 + i : k1 - 1 - i;
+        const cd v = csub(csub(x[k], cmul(c1f(k), pv)), cmul(c2f(k), qv));
+        x[k] = of(k, v);
+        qv = pv;
+        pv = v;
+    }
+}
+// tridiagonal (real): A^-1 = U^-1 L^-1 with L unit lower (c_k = d_{k-1}
+// lo_k), U = diag(1/d) + up on the superdiagonal; A^-T = L^-T U^-T:
+// s_k = d_k (x_k - up_{k-1} s_{k-1}), then t_k = s_k - d_k lo_{k+1} t_{k+1}
+__device__ __forceinline__ void fd_tri_solveT_wave(int N, const double *__restrict__ d,
+                                                   const double *__restrict__ lo, const double *__restrict__ up,
+                                                   cd *x)
+{
+    wave_sweep1<true>(N, x, [&](int k) { return k == 0 ? 0.0 : -d[k] * up[k - 1]; },
+                      [&](int k, cd v) { return cscale(d[k], v); });
+    wave_sweep1<false>(N, x, [&](int k) { return k == N - 1 ? 0.0 : -d[k] * lo[k + 1]; },
+                       [&](int, cd v) { return v; });
 }
 // pentadiagonal (complex): A^-1 = U^-1 L^-1 D with L, U unit triangular
 // (l1, l2 below, u1, u2 above); A^-H = conj(D) L^-H U^-H
-__device__ void fd_penta_solveH(int N, const cd *d, const cd *l1, const cd *l2, const cd *u1, const cd *u2, cd *x)
+__device__ __forceinline__ void fd_penta_solveH_wave(int N, const cd *__restrict__ d, const cd *__restrict__ l1,
+                                                     const cd *__restrict__ l2, const cd *__restrict__ u1,
+                                                     const cd *__restrict__ u2, cd *x)
 {
-    cd s1 = mk(0, 0), s2 = mk(0, 0);
-    for (int k = 0; k < N; k++) {
-        cd v = x[k];
-        if (k >= 1) v = csub(v, cmul(cconj(u1[k - 1]), s1));
-        if (k >= 2) v = csub(v, cmul(cconj(u2[k - 2]), s2));
-        x[k] = v;
-        s2 = s1;
-        s1 = v;
-    }
-    cd t1 = mk(0, 0), t2 = mk(0, 0);
-    for (int k = N - 1; k >= 0; k--) {
-        cd v = x[k];
-        if (k + 1 < N) v = csub(v, cmul(cconj(l1[k + 1]), t1));
-        if (k + 2 < N) v = csub(v, cmul(cconj(l2[k + 2]), t2));
-        t2 = t1;
-        t1 = v;
-        x[k] = v;
-    }
-    for (int k = 0; k < N; k++) x[k] = cmul(cconj(d[k]), x[k]);
+    const cd z = mk(0.0, 0.0);
+    wave_sweep2<true>(N, x, [&](int k) { return k >= 1 ? cconj(u1[k - 1]) : z; },
+                      [&](int k) { return k >= 2 ? cconj(u2[k - 2]) : z; }, [&](int, cd v) { return v; });
+    wave_sweep2<false>(N, x, [&](int k) { return k + 1 < N ? cconj(l1[k + 1]) : z; },
+                       [&](int k) { return k + 2 < N ? cconj(l2[k + 2]) : z; },
+                       [&](int k, cd v) { return cmul(cconj(d[k]), v); });
 }
 
 // the adjoints of a T or xi column of fd_implicit (outputs temp, T_old(1)
@@ -958,8 +1013,11 @@ __global__ void __launch_bounds__(256) k_fd_mloop_adj(const PzDev *__restrict__ 
             fd_ghost_scalarT(G, N, f == 0 ? p.ktopt : p.ktopxi, f == 0 ? p.kbott : p.kbotxi, R, qt, qb);
             s_q[2 * f] = cadd(s_q[2 * f], qt);
             s_q[2 * f + 1] = cadd(s_q[2 * f + 1], qb);
+        }
+        __syncthreads();
+        if (tid < 64) {
             const double *M = f == 0 ? fT : fX;
-            fd_tri_solveT(N, M, M + N, M + 2 * N, G + 1);
+            fd_tri_solveT_wave(N, M, M + N, M + 2 * N, G + 1);
         }
         __syncthreads();
     }
@@ -1034,15 +1092,18 @@ __global__ void __launch_bounds__(256) k_fd_mloop_adj(const PzDev *__restrict__ 
             else { GP[2] = csub(GP[2], g0); GP[1] = cadd(GP[1], cscale(2.0, g0)); }
             if (p.kbotv == 1) { GP[N - 1] = cadd(GP[N - 1], gn); GP[N] = cadd(GP[N], cscale(2.0 * db * R.or1(N - 1), gn)); }
             else { GP[N - 1] = csub(GP[N - 1], gn); GP[N] = cadd(GP[N], cscale(2.0, gn)); }
-            fd_tri_solveT(N, fU, fU + N, fU + 2 * N, GP + 1);
         } else {
             const double dr = R.r(1) - R.r(0), db = R.r(N - 1) - R.r(N - 2);
             const double ct = p.ktopv == 1 ? -(1.0 - 0.5 * R.or1(0) * dr) / (1.0 + 0.5 * R.or1(0) * dr) : 1.0;
             const double cb = p.kbotv == 1 ? -(1.0 + 0.5 * R.or1(N - 1) * db) / (1.0 - 0.5 * R.or1(N - 1) * db) : 1.0;
             GP[3] = cadd(GP[3], cscale(ct, GP[1]));
             GP[N] = cadd(GP[N], cscale(cb, GP[N + 2]));
-            fd_penta_solveH(N, fP, fP + N, fP + 2 * N, fP + 3 * N, fP + 4 * N, GP + 2);
         }
+    }
+    __syncthreads();
+    if (tid < 64) {
+        if (m == 0) fd_tri_solveT_wave(N, fU, fU + N, fU + 2 * N, GP + 1);
+        else fd_penta_solveH_wave(N, fP, fP + N, fP + 2 * N, fP + 3 * N, fP + 4 * N, GP + 2);
     }
     __syncthreads();
 

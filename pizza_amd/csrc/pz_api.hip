@@ -2584,7 +2584,7 @@ static void adj_init(pz_handle *h)
     arena_layout(lv, a->lam, F, h->sch.nold, h->sch.nimp, h->sch.nexp);
     lv.ss = h->d.ss;
     a->dlam = h->upload(std::vector<PzDev>{lv});
-    a->acc = h->dalloc<double>(1);
+    a->acc = h->dalloc<double>(1 + (size_t)h->d.nm); // total, then one partial sum per m
     a->W = h->dalloc<cd>(F);
     a->grad = h->dalloc<cd>(F);
     if (h->radial_inplace) a->rrow = h->dalloc<cd>(4 * F);
@@ -2716,9 +2716,11 @@ extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *wei
                 if (objective == PZ_OBJ_NUTOP)
                     hipLaunchKernelGGL(k_obj_acc, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev,
                                        (const PzAdjDev *)a->dad, (const cd *)nullptr, a->acc);
-                else if (objective == PZ_OBJ_EKIN)
-                    hipLaunchKernelGGL(k_obj_ke, dim3(1), dim3(256), 0, h->stream, (const PzDev *)h->ddev,
+                else if (objective == PZ_OBJ_EKIN) {
+                    hipLaunchKernelGGL(k_obj_ke, dim3(h->d.nm), dim3(256), 0, h->stream, (const PzDev *)h->ddev,
                                        (const PzAdjDev *)a->dad, a->acc);
+                    hipLaunchKernelGGL(k_obj_fin, dim3(1), dim3(256), 0, h->stream, h->d.nm, a->acc);
+                }
             }
         }
         if (objective == PZ_OBJ_TEMP)
