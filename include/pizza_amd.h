@@ -390,9 +390,9 @@ int pz_ensemble_policy(const pz_ensemble *e, int *policy, double *trial_ms);
 
 /*
  * Discrete adjoint of the step.  The reference has no adjoint; the forward
- * map differentiated is pz_step's (collocation method, multistep schemes,
- * dt fixed over the window), its objectives are the diagnostics of
- * outputs.f90:
+ * map differentiated is pz_step's (collocation or finite-difference method,
+ * multistep schemes, dt fixed over the window), its objectives are the
+ * diagnostics of outputs.f90:
  *   PZ_OBJ_NUTOP  J = (1/K) sum_{n=1..K} NuTop(n), the outer-wall Nusselt
  *                 number (outputs.f90:729-755) after each of the K steps;
  *   PZ_OBJ_EKIN   J = (1/K) sum_{n=1..K} E(n), the kinetic energy
@@ -408,8 +408,9 @@ int pz_ensemble_policy(const pz_ensemble *e, int *policy, double *trial_ms);
  * temperature field given to pz_start (grad_temp, layout of pz_set_field)
  * and to the boundary values of T in the implicit solves (grad_bc,
  * complex(8) (n_m_max, 2): outer wall, inner wall; the fixed temperature or
- * flux pattern of ktopt/kbott, with the conducting state held fixed), and
- * restores the state after K steps.  Either output may be null.  Returns -7
+ * flux pattern of ktopt/kbott, with the conducting state held fixed; with
+ * finite differences the wall patterns of l_fd_bc_pattern, zero without
+ * them), and restores the state after K steps.  Either output may be null.  Returns -7
  * if dt (and so the matrices) changed inside the window.
  */
 enum pz_objective { PZ_OBJ_NUTOP = 0, PZ_OBJ_EKIN = 1, PZ_OBJ_TEMP = 2 };
