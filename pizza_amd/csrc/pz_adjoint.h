@@ -623,6 +623,7 @@ k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ pl
     const size_t F = (size_t)nm * N;
     cd *gf = (cd *)smem, *sa = gf + 2 * n, *t1 = sa + 2 * n, *t2 = t1 + 64;
     const int n2 = (n + 63) >> 6;
+    cd *om = t2 + n2;
     const bool hT = p.l_heat;
     const cd *Rus = p.rrow + (size_t)nr * nm, *Rup = Rus + F, *Rom = Rus + 2 * F, *RT = Rus + 3 * F;
     for (int k = tid; k < 64 + n2; k += nb) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
@@ -664,8 +665,8 @@ k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ pl
         sa[t] = z;
     }
     __syncthreads();
-    fft_inplace(gf, n, 2, +1, p.radices, p.nrad, bwd);
-    fft_inplace(sa, n, 2, +1, p.radices, p.nrad, bwd);
+    fft_inplace(gf, n, 2, +1, p.radices, p.nrad, bwd, om);
+    fft_inplace(sa, n, 2, +1, p.radices, p.nrad, bwd, om);
     const double inv_n = 1.0 / (double)n;
     for (int j = tid; j < n; j += nb) {
         const double us = gf[j].x, up = gf[j].y, om = gf[n + j].x, T = gf[n + j].y;
@@ -675,7 +676,7 @@ k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ pl
         sa[n + j] = mk(ao * up + bo * rr * us, aT * up + bT * rr * us);
     }
     __syncthreads();
-    fft_inplace(sa, n, 2, -1, p.radices, p.nrad, fwd);
+    fft_inplace(sa, n, 2, -1, p.radices, p.nrad, fwd, om);
     const double lsom = -A.eP0[nr].x; // P_expl(m = 0) = -usom
     const double cfac = 0.5 * p.ek * p.CorFac * p.ekpump[nr];
     for (int t = tid; t < 2 * nm; t += nb) {

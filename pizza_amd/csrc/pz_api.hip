@@ -1246,9 +1246,16 @@ static void setup(pz_handle *h)
         // without composition: two columns and in-place transforms where the
         // plan allows (k_radial_loop_inplace), the spectra by radius from
         // k_gather_r
-        if (!d.l_chem && fft_inplace_fits(h->nphi, 2, d.radices, d.nrad, PZ_RADIAL_INPLACE_THREADS)) {
+        // (the plan then starts with its largest odd prime, fft_inplace's
+        // twiddle-free conjugate-pair stage)
+        int plan[PZ_MAX_RADICES];
+        for (int s = 0; s < d.nrad; s++) plan[s] = d.radices[s];
+        inplace_plan(plan, d.nrad);
+        if (!d.l_chem && fft_inplace_fits(h->nphi, 2, plan, d.nrad, PZ_RADIAL_INPLACE_THREADS)) {
+            for (int s = 0; s < d.nrad; s++) d.radices[s] = plan[s];
             h->radial_inplace = true;
-            h->lds_radial = (2 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64) * sizeof(cd) + 16 * sizeof(double);
+            h->lds_radial = (2 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd) +
+                            16 * sizeof(double);
             d.rrow = h->dalloc<cd>(4 * (size_t)h->nm * N);
         }
     }
@@ -2685,7 +2692,7 @@ extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *wei
         HIPCHK(hipStreamSynchronize(h->stream));
         if (s0.n_steps != 0) return fail(h, -5, "the adjoint window starts right after pz_start");
         const size_t lds_radj = h->radial_inplace
-                                    ? (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64) * sizeof(cd)
+                                    ? (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd)
                                     : (11 * (size_t)h->nm + 9 * (size_t)h->nphi) * sizeof(cd);
         const size_t lds_madj = h->d.fd ? (3 * ((size_t)h->N + 4) + 10 * (size_t)h->N) * sizeof(cd)
                                         : ADJ_NVEC * (size_t)h->N * sizeof(cd);
@@ -2758,7 +2765,7 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
         const size_t lds_radj = (11 * (size_t)nm + 9 * (size_t)h->nphi) * sizeof(cd);
         const size_t lds_madj = ADJ_NVEC * (size_t)N * sizeof(cd);
         const size_t lds_fadj = (3 * ((size_t)N + 4) + 10 * (size_t)N) * sizeof(cd);
-        const size_t lds_radj_in = (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64) * sizeof(cd);
+        const size_t lds_radj_in = (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd);
         const double tiny = 10.0 * 2.220446049250313e-16;
         double wnu = 0.0;
         if (a->objective == PZ_OBJ_NUTOP && std::fabs(h->dtcond_top) > tiny) wnu = dJ / ((double)K * h->dtcond_top);
@@ -2880,7 +2887,11 @@ extern "C" int pz_fft(pz_handle *h, const double *grid, double *spec, int n_rad)
         HIPCHK(hipMalloc(&g, (size_t)n_rad * h->nphi * sizeof(double)));
         HIPCHK(hipMalloc(&s, (size_t)n_rad * h->nm * sizeof(cd)));
         HIPCHK(hipMemcpy(g, grid, (size_t)n_rad * h->nphi * sizeof(double), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_phi_fft, dim3(n_rad), dim3(256), 2 * h->nphi * sizeof(cd), h->stream, h->d, g, s);
+        if (h->radial_inplace)
+            hipLaunchKernelGGL(k_phi_fft_inplace, dim3(n_rad), dim3(PZ_RADIAL_INPLACE_THREADS), h->lds_radial,
+                               h->stream, (const PzDev *)h->ddev, g, s);
+        else
+            hipLaunchKernelGGL(k_phi_fft, dim3(n_rad), dim3(256), 2 * h->nphi * sizeof(cd), h->stream, h->d, g, s);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipMemcpy(spec, s, (size_t)n_rad * h->nm * sizeof(cd), hipMemcpyDeviceToHost));
@@ -2901,7 +2912,11 @@ extern "C" int pz_ifft(pz_handle *h, const double *spec, double *grid, int n_rad
         HIPCHK(hipMalloc(&g, (size_t)n_rad * h->nphi * sizeof(double)));
         HIPCHK(hipMalloc(&s, (size_t)n_rad * h->nm * sizeof(cd)));
         HIPCHK(hipMemcpy(s, spec, (size_t)n_rad * h->nm * sizeof(cd), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_phi_ifft, dim3(n_rad), dim3(256), 2 * h->nphi * sizeof(cd), h->stream, h->d, s, g);
+        if (h->radial_inplace)
+            hipLaunchKernelGGL(k_phi_ifft_inplace, dim3(n_rad), dim3(PZ_RADIAL_INPLACE_THREADS), h->lds_radial,
+                               h->stream, (const PzDev *)h->ddev, s, g);
+        else
+            hipLaunchKernelGGL(k_phi_ifft, dim3(n_rad), dim3(256), 2 * h->nphi * sizeof(cd), h->stream, h->d, s, g);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipMemcpy(grid, g, (size_t)n_rad * h->nphi * sizeof(double), hipMemcpyDeviceToHost));

@@ -343,12 +343,89 @@ __device__ __forceinline__ void inplace_prime(cd *a, int n, int ntr, int Ns, int
             if (c < cnt[it]) a[d[it] + c * Ns] = y[it][c];
     __syncthreads();
 }
+// The first stage of a plan (no twiddles) with an odd prime R, 5 <= R <=
+// 31: the R-point DFT by conjugate pairs.  With s_j = x_j + x_{R-j}, d_j =
+// x_j - x_{R-j} and w^{jk} = c + i sigma (the direction's sign in sigma),
+// X_k = x_0 + A_k + i B_k and X_{R-k} = x_0 + A_k - i B_k, where A_k = sum_j
+// c_{jk} s_j and B_k = sum_j sigma_{jk} d_j (j, k = 1 .. h, h = (R-1)/2):
+// two real-by-complex products per term and pair instead of two complex
+// Horner steps per output.  om[(j-1) PZ_SYM_ROW + k-1] = w^{jk mod R} (a
+// row per j, filled here); an item takes P consecutive pairs of one
+// butterfly (the first item also X_0 = x_0 + sum s_j).
+#define PZ_SYM_PAIRS 4
+#define PZ_SYM_ROW 16
+#define PZ_FFT_OM_MAX (15 * PZ_SYM_ROW)
+__host__ __device__ inline bool sym_first_stage(int R) { return R >= 5 && (R & 1) && R <= 31; }
+template <int P, class TW>
+__device__ __forceinline__ void inplace_prime_first(cd *a, int n, int ntr, int R, cd *om, TW tw)
+{
+    const int nb = n / R, h = (R - 1) >> 1, ng = (h + P - 1) / P, tot = ntr * nb * ng;
+    for (int e = threadIdx.x; e < h * PZ_SYM_ROW; e += blockDim.x) {
+        const int j = e / PZ_SYM_ROW + 1, k = e - (j - 1) * PZ_SYM_ROW + 1;
+        om[e] = tw(((j * k) % R) * nb);
+    }
+    __syncthreads();
+    cd y[2 * P + 1];
+    int base = 0, kl = 1, cnt = 0;
+    const int t = threadIdx.x;
+    if (t < tot) {
+        const int u = t / ng, g = t - u * ng, tr = u / nb, j = u - tr * nb;
+        const cd *xp = a + tr * n + j, *xm = xp + R * nb;
+        const cd x0 = *xp;
+        kl = 1 + P * g;
+        const cd *w = om + (kl - 1);
+        cd A[P], B[P];
+#pragma unroll
+        for (int c = 0; c < P; c++) { A[c] = mk(0.0, 0.0); B[c] = mk(0.0, 0.0); }
+        cd sum = x0;
+        for (int jj = 1; jj <= h; jj++) {
+            xp += nb;
+            xm -= nb;
+            const cd vp = *xp, vm = *xm;
+            const cd sj = cadd(vp, vm), dj = csub(vp, vm);
+            sum = cadd(sum, sj);
+#pragma unroll
+            for (int c = 0; c < P; c++) {
+                const cd wc = w[c];
+                A[c] = mk(fma(wc.x, sj.x, A[c].x), fma(wc.x, sj.y, A[c].y));
+                B[c] = mk(fma(wc.y, dj.x, B[c].x), fma(wc.y, dj.y, B[c].y));
+            }
+            w += PZ_SYM_ROW;
+        }
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+            y[2 * c] = mk(x0.x + A[c].x - B[c].y, x0.y + A[c].y + B[c].x);
+            y[2 * c + 1] = mk(x0.x + A[c].x + B[c].y, x0.y + A[c].y - B[c].x);
+        }
+        y[2 * P] = g == 0 ? sum : mk(0.0, 0.0);
+        base = tr * n + j * R;
+        cnt = min(P, h + 1 - kl);
+        if (g == 0) cnt = -1 - cnt; // X_0 as well
+    }
+    __syncthreads();
+    const int np = cnt < 0 ? -1 - cnt : cnt;
+#pragma unroll
+    for (int c = 0; c < P; c++)
+        if (c < np) {
+            a[base + kl + c] = y[2 * c];
+            a[base + R - kl - c] = y[2 * c + 1];
+        }
+    if (cnt < 0) a[base] = y[2 * P];
+    __syncthreads();
+}
+// om: LDS for PZ_FFT_OM_MAX entries, used when the plan starts with an odd
+// prime 5 <= R <= 31 (one item per thread there: fft_inplace_fits)
 template <class TW>
-__device__ void fft_inplace(cd *a, int n, int ntr, int dir, const int *radices, int nrad, TW tw)
+__device__ void fft_inplace(cd *a, int n, int ntr, int dir, const int *radices, int nrad, TW tw, cd *om)
 {
     int Ns = 1;
     for (int s = 0; s < nrad; s++) {
         const int R = radices[s];
+        if (s == 0 && sym_first_stage(R)) {
+            inplace_prime_first<PZ_SYM_PAIRS>(a, n, ntr, R, om, tw);
+            Ns *= R;
+            continue;
+        }
         switch (R) {
         case 2: inplace_bfly<2, 2>(a, n, ntr, Ns, dir, tw); break;
         case 3: inplace_bfly<3, 2>(a, n, ntr, Ns, dir, tw); break;
@@ -367,10 +444,27 @@ inline bool fft_inplace_fits(int n, int ntr, const int *radices, int nrad, int n
         long items, cap;
         items = (long)ntr * nb;
         cap = R == 4 ? 1 : 2;
-        if (R != 2 && R != 3 && R != 4) items *= (R + 3) / 4;
+        if (s == 0 && sym_first_stage(R)) {
+            items *= ((R - 1) / 2 + PZ_SYM_PAIRS - 1) / PZ_SYM_PAIRS;
+            cap = 1;
+        } else if (R != 2 && R != 3 && R != 4) {
+            items *= (R + 3) / 4;
+        }
         if (items > cap * nthreads) return false;
     }
     return true;
+}
+// the plan of the in-place transforms: the largest odd prime up to 31
+// first (its stage has no twiddles), the other radices in their order
+inline void inplace_plan(int *radices, int nrad)
+{
+    int best = -1;
+    for (int s = 0; s < nrad; s++)
+        if (sym_first_stage(radices[s]) && (best < 0 || radices[s] > radices[best])) best = s;
+    if (best <= 0) return;
+    const int R = radices[best];
+    for (int s = best; s > 0; s--) radices[s] = radices[s - 1];
+    radices[0] = R;
 }
 
 // azimuthal FFT of length n_phi (twiddles exp(-2 pi i k / n_phi) in p.twid)
@@ -809,7 +903,8 @@ __device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr,
     const size_t F = (size_t)nm * N;
     cd *s0 = (cd *)smem, *s1 = s0 + n, *t1 = s1 + n, *t2 = t1 + 64;
     const int n2 = (n + 63) >> 6;
-    double *red = (double *)(t2 + n2);
+    cd *om = t2 + n2;
+    double *red = (double *)(om + PZ_FFT_OM_MAX);
     const cd *Rus = p.rrow + (size_t)nr * nm, *Rup = Rus + F, *Rom = Rus + 2 * F, *RT = Rus + 3 * F;
     // exp(-2 pi i k / n) = t1[k mod 64] t2[k / 64]
     for (int k = threadIdx.x; k < 64 + n2; k += blockDim.x) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
@@ -845,7 +940,7 @@ __device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr,
         s0[t] = z;
     }
     __syncthreads();
-    fft_inplace(s0, n, 2, +1, p.radices, p.nrad, bwd);
+    fft_inplace(s0, n, 2, +1, p.radices, p.nrad, bwd, om);
     if (courant) {
         double vr = 0.0, vh = 0.0;
         for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -875,7 +970,7 @@ __device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr,
         s1[j] = mk(up * T, rr * us * T);
     }
     __syncthreads();
-    fft_inplace(s0, n, p.l_heat ? 2 : 1, -1, p.radices, p.nrad, fwd);
+    fft_inplace(s0, n, p.l_heat ? 2 : 1, -1, p.radices, p.nrad, fwd, om);
     // unpack, normalise and post-scale (radial_loop.f90:350-386)
     const double or1 = p.or1[nr];
     const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
@@ -940,6 +1035,45 @@ __global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS) k_radial_loop_inpla
                                                                                     int courant)
 {
     radial_loop_inplace_body(*pd, blockIdx.x, is, courant);
+}
+// pz_fft / pz_ifft by the in-place transforms of the long grids, one
+// radius per block (grid real [rad][n_phi], spectral [rad][nm])
+__global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS)
+k_phi_fft_inplace(const PzDev *__restrict__ pd, const double *grid, cd *spec)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pd;
+    const int n = p.nphi, nm = p.nm, n2 = (n + 63) >> 6;
+    cd *a = (cd *)smem, *t1 = a + 2 * n, *t2 = t1 + 64, *om = t2 + n2;
+    for (int k = threadIdx.x; k < 64 + n2; k += blockDim.x) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
+    const double *g = grid + (size_t)blockIdx.x * n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) a[j] = mk(g[j], 0.0);
+    __syncthreads();
+    fft_inplace(a, n, 1, -1, p.radices, p.nrad, [=](int k) { return cmul(t1[k & 63], t2[k >> 6]); }, om);
+    for (int i = threadIdx.x; i < nm; i += blockDim.x) spec[(size_t)blockIdx.x * nm + i] = cdivr(a[i], (double)n);
+}
+__global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS)
+k_phi_ifft_inplace(const PzDev *__restrict__ pd, const cd *spec, double *grid)
+{
+    extern __shared__ double smem[];
+    const PzDev &p = *pd;
+    const int n = p.nphi, nm = p.nm, n2 = (n + 63) >> 6;
+    cd *a = (cd *)smem, *t1 = a + 2 * n, *t2 = t1 + 64, *om = t2 + n2;
+    for (int k = threadIdx.x; k < 64 + n2; k += blockDim.x) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
+    const cd *s = spec + (size_t)blockIdx.x * nm;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        cd z = mk(0, 0);
+        if (k == 0) z = mk(s[0].x, 0.0);
+        else if (k < nm) z = s[k];
+        else if (n - k < nm) z = cconj(s[n - k]);
+        a[k] = z;
+    }
+    __syncthreads();
+    fft_inplace(a, n, 1, +1, p.radices, p.nrad, [=](int k) {
+        const cd w = cmul(t1[k & 63], t2[k >> 6]);
+        return mk(w.x, -w.y);
+    }, om);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) grid[(size_t)blockIdx.x * n + j] = a[j].x;
 }
 
 __global__ void __launch_bounds__(256) k_radial_loop(const PzDev *__restrict__ pd, int is, int courant)

@@ -57,8 +57,9 @@ def test_unit_operators():
 def test_fft_prime_radices(m_max):
     """Azimuthal transforms whose n_phi has prime factors above 8, where
     the stages evaluate a direct DFT per output by Horner's rule: n_phi 255
-    (3 5 17), 510 (2 3 5 17) and 2046 (2 3 11 31), against the oracle's
-    recursive FFT to 1e-14."""
+    (3 5 17), 510 (2 3 5 17) and 2046 (2 3 11 31, the in-place transforms
+    of the long grids, whose first stage takes radix 31 by conjugate
+    pairs), against the oracle's recursive FFT to 1e-14."""
     from pizza_amd import workloads as W
     cfg = W.c1_collocation()
     cfg.m_max = m_max
