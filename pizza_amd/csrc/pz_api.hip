@@ -1247,15 +1247,16 @@ static void setup(pz_handle *h)
         // plan allows (k_radial_loop_inplace), the spectra by radius from
         // k_gather_r
         // (the plan then starts with its largest odd prime, fft_inplace's
-        // twiddle-free conjugate-pair stage)
+        // twiddle-free conjugate-pair stage, and takes 2 x 3 as radix 6)
         int plan[PZ_MAX_RADICES];
         for (int s = 0; s < d.nrad; s++) plan[s] = d.radices[s];
-        inplace_plan(plan, d.nrad);
-        if (!d.l_chem && fft_inplace_fits(h->nphi, 2, plan, d.nrad, PZ_RADIAL_INPLACE_THREADS)) {
+        const int nplan = inplace_plan(plan, d.nrad);
+        if (!d.l_chem && fft_inplace_fits(h->nphi, 2, plan, nplan, PZ_RADIAL_INPLACE_THREADS)) {
+            d.nrad = nplan;
             for (int s = 0; s < d.nrad; s++) d.radices[s] = plan[s];
             h->radial_inplace = true;
             h->lds_radial = (2 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd) +
-                            16 * sizeof(double);
+                            32 * sizeof(double);
             d.rrow = h->dalloc<cd>(4 * (size_t)h->nm * N);
         }
     }

@@ -181,6 +181,30 @@ __device__ __forceinline__ void stockham_bfly(const cd *x, int j, int nb, int ts
             v[5] = csub(s02, s13);
             v[7] = csub(d02, d13);
         }
+    } else if constexpr (R == 6) {
+        // radix 6 as 2 x 3: the DFT3s E of the even and O of the odd taps,
+        // X_k = E_k + w6^k O_k, X_{k+3} = E_k - w6^k O_k, w6 = exp(dir i pi/3)
+        const double s3 = 0.86602540378443864676, sd = (double)dir;
+        cd e[3], o[3];
+        const cd v0 = x[j], v1 = cmul(x[j + nb], tw(tstep)), v2 = cmul(x[j + 2 * nb], tw(2 * tstep));
+        const cd v3 = cmul(x[j + 3 * nb], tw(3 * tstep)), v4 = cmul(x[j + 4 * nb], tw(4 * tstep));
+        const cd v5 = cmul(x[j + 5 * nb], tw(5 * tstep));
+        auto dft3 = [&](cd a0, cd a1, cd a2, cd(&r)[3]) {
+            const cd t1 = cadd(a1, a2), t2 = cimul(sd * s3, csub(a1, a2));
+            const cd m0 = mk(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
+            r[0] = cadd(a0, t1);
+            r[1] = cadd(m0, t2);
+            r[2] = csub(m0, t2);
+        };
+        dft3(v0, v2, v4, e);
+        dft3(v1, v3, v5, o);
+        const cd o1 = cmul(mk(0.5, sd * s3), o[1]), o2 = cmul(mk(-0.5, sd * s3), o[2]);
+        v[0] = cadd(e[0], o[0]);
+        v[3] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[4] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[5] = csub(e[2], o2);
     } else {
         static_assert(R == 3, "radix");
         const double s3 = 0.86602540378443864676;
@@ -270,7 +294,7 @@ __device__ cd *fft_stockham(cd *a, cd *b, int n, int ntr, int dir, const int *ra
 // computes its outputs into registers, then a barrier, the stores, and a
 // barrier, so one LDS column per transform is enough.  A thread takes at
 // most IT items of a stage (fft_inplace_fits checks the plan on the host):
-// a butterfly of radix 2, 3 or 4, or four consecutive outputs of another
+// a butterfly of radix 2, 3, 4 or 6, or four consecutive outputs of another
 // radix (8 included), which share their input reads (Horner's rule as
 // above).
 template <int R, int IT, class TW>
@@ -430,7 +454,12 @@ __device__ void fft_inplace(cd *a, int n, int ntr, int dir, const int *radices, 
         case 2: inplace_bfly<2, 2>(a, n, ntr, Ns, dir, tw); break;
         case 3: inplace_bfly<3, 2>(a, n, ntr, Ns, dir, tw); break;
         case 4: inplace_bfly<4, 1>(a, n, ntr, Ns, dir, tw); break;
-        default: inplace_prime<2, 4>(a, n, ntr, Ns, R, tw); break;
+        case 6: inplace_bfly<6, 1>(a, n, ntr, Ns, dir, tw); break;
+        default:
+            // one item per thread when six outputs an item cover the stage
+            if (ntr * (n / R) * ((R + 5) / 6) <= (int)blockDim.x) inplace_prime<1, 6>(a, n, ntr, Ns, R, tw);
+            else inplace_prime<2, 4>(a, n, ntr, Ns, R, tw);
+            break;
         }
         Ns *= R;
     }
@@ -443,11 +472,11 @@ inline bool fft_inplace_fits(int n, int ntr, const int *radices, int nrad, int n
         const int R = radices[s], nb = n / R;
         long items, cap;
         items = (long)ntr * nb;
-        cap = R == 4 ? 1 : 2;
+        cap = (R == 4 || R == 6) ? 1 : 2;
         if (s == 0 && sym_first_stage(R)) {
             items *= ((R - 1) / 2 + PZ_SYM_PAIRS - 1) / PZ_SYM_PAIRS;
             cap = 1;
-        } else if (R != 2 && R != 3 && R != 4) {
+        } else if (R != 2 && R != 3 && R != 4 && R != 6) {
             items *= (R + 3) / 4;
         }
         if (items > cap * nthreads) return false;
@@ -455,16 +484,30 @@ inline bool fft_inplace_fits(int n, int ntr, const int *radices, int nrad, int n
     return true;
 }
 // the plan of the in-place transforms: the largest odd prime up to 31
-// first (its stage has no twiddles), the other radices in their order
-inline void inplace_plan(int *radices, int nrad)
+// first (its stage has no twiddles), a radix 2 and a radix 3 merged into
+// one radix-6 stage, the other radices in their order; returns the number
+// of stages
+inline int inplace_plan(int *radices, int nrad)
 {
+    int i2 = -1, i3 = -1;
+    for (int s = 0; s < nrad; s++) {
+        if (radices[s] == 2 && i2 < 0) i2 = s;
+        if (radices[s] == 3 && i3 < 0) i3 = s;
+    }
+    if (i2 >= 0 && i3 >= 0) {
+        radices[std::min(i2, i3)] = 6;
+        for (int s = std::max(i2, i3); s + 1 < nrad; s++) radices[s] = radices[s + 1];
+        nrad--;
+    }
     int best = -1;
     for (int s = 0; s < nrad; s++)
         if (sym_first_stage(radices[s]) && (best < 0 || radices[s] > radices[best])) best = s;
-    if (best <= 0) return;
-    const int R = radices[best];
-    for (int s = best; s > 0; s--) radices[s] = radices[s - 1];
-    radices[0] = R;
+    if (best > 0) {
+        const int R = radices[best];
+        for (int s = best; s > 0; s--) radices[s] = radices[s - 1];
+        radices[0] = R;
+    }
+    return nrad;
 }
 
 // azimuthal FFT of length n_phi (twiddles exp(-2 pi i k / n_phi) in p.twid)
@@ -904,7 +947,7 @@ __device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr,
     cd *s0 = (cd *)smem, *s1 = s0 + n, *t1 = s1 + n, *t2 = t1 + 64;
     const int n2 = (n + 63) >> 6;
     cd *om = t2 + n2;
-    double *red = (double *)(om + PZ_FFT_OM_MAX);
+    double *red = (double *)(om + PZ_FFT_OM_MAX), *msr = red + 16;
     const cd *Rus = p.rrow + (size_t)nr * nm, *Rup = Rus + F, *Rom = Rus + 2 * F, *RT = Rus + 3 * F;
     // exp(-2 pi i k / n) = t1[k mod 64] t2[k / 64]
     for (int k = threadIdx.x; k < 64 + n2; k += blockDim.x) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
@@ -913,32 +956,36 @@ __device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr,
         const cd w = cmul(t1[k & 63], t2[k >> 6]);
         return mk(w.x, -w.y);
     };
-    // Reynolds-stress m-sum (radial_loop.f90:267-278)
+    // Hermitian spectra of z = a + i b for (us, up), (om, T), and the
+    // Reynolds-stress m-sum (radial_loop.f90:267-278) from the same loads,
+    // one partial sum per wave in msr (added at the unpack)
+    const double ekc = 0.5 * p.ek * p.CorFac * p.ekpump[nr];
     double part = 0.0;
-    for (int i = threadIdx.x; i < nm; i += blockDim.x) {
-        const int m = p.idx2m[i];
-        const cd usf = m == 0 ? cscale(0.5 * p.ek * p.CorFac * p.ekpump[nr], Rup[i]) : Rus[i];
-        part += cc22real(usf, Rom[i], m);
-    }
-    const double usom = block_reduce(part, red, 0);
-    // Hermitian spectra of z = a + i b for (us, up), (om, T)
-    for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) {
-        const int tr = t >= n, k = t - tr * n;
-        const cd *A = tr == 0 ? Rus : Rom;
-        const cd *B = tr == 0 ? Rup : (p.l_heat ? RT : nullptr);
-        cd z = mk(0, 0);
-        if (k == 0) {
-            z = mk(A[0].x, B ? B[0].x : 0.0);
-        } else if (k < nm) {
-            const cd a = A[k], b = B ? B[k] : mk(0, 0);
-            z = mk(a.x - b.y, a.y + b.x);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        cd z0 = mk(0, 0), z1 = mk(0, 0);
+        if (k < nm) {
+            const cd us = Rus[k], up = Rup[k], o = Rom[k], T = p.l_heat ? RT[k] : mk(0, 0);
+            const int m = p.idx2m[k];
+            part += cc22real(m == 0 ? cscale(ekc, up) : us, o, m);
+            if (k == 0) {
+                z0 = mk(us.x, up.x);
+                z1 = mk(o.x, T.x);
+            } else {
+                z0 = mk(us.x - up.y, us.y + up.x);
+                z1 = mk(o.x - T.y, o.y + T.x);
+            }
         } else if (n - k < nm) {
             const int kk = n - k;
-            const cd a = cconj(A[kk]), b = B ? cconj(B[kk]) : mk(0, 0);
-            z = mk(a.x - b.y, a.y + b.x);
+            const cd us = cconj(Rus[kk]), up = cconj(Rup[kk]), o = cconj(Rom[kk]);
+            const cd T = p.l_heat ? cconj(RT[kk]) : mk(0, 0);
+            z0 = mk(us.x - up.y, us.y + up.x);
+            z1 = mk(o.x - T.y, o.y + T.x);
         }
-        s0[t] = z;
+        s0[k] = z0;
+        s1[k] = z1;
     }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    if ((threadIdx.x & 63) == 0) msr[threadIdx.x >> 6] = part;
     __syncthreads();
     fft_inplace(s0, n, 2, +1, p.radices, p.nrad, bwd, om);
     if (courant) {
@@ -987,7 +1034,13 @@ __device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr,
         const size_t q = (size_t)i * N + nr;
         const cd cim = mk(0.0, -or1 * (double)m);
         if (tr == 0) {
-            p.P_expl[is * F + q] = (i == 0) ? mk(-usom, 0.0) : cmul(cim, A);
+            if (i == 0) {
+                double usom = msr[0];
+                for (int w = 1; w < (int)((blockDim.x + 63) >> 6); w++) usom += msr[w];
+                p.P_expl[is * F + q] = mk(-usom, 0.0);
+            } else {
+                p.P_expl[is * F + q] = cmul(cim, A);
+            }
             p.dVsOm[q] = B;
         } else {
             const cd v = cmul(cim, A);

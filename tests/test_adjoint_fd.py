@@ -171,8 +171,9 @@ def test_c5_flux_pattern_gradient():
     E 1e-6, Ra 1e9, Pr 0.1, SBDF3 dt 1e-7, 100 steps with checkpoints every
     10; the kinetic energy averaged over the window with respect to the
     seeded outer-wall heat-flux pattern (16 modes, modulus 0.1), against
-    central differences of the GPU forward along 8 seeded directions of
-    the 16 complex coefficients (step 8e-3; tolerance 1e-6)."""
+    fourth-order central differences of the GPU forward along 8 seeded
+    directions of the 16 complex coefficients (steps 3.2e-2 and 6.4e-2;
+    tolerance 1e-6)."""
     from pizza_amd import workloads as W
     from pizza_amd.runtime import Run
     base = W.c5_flux_pattern()
@@ -207,13 +208,16 @@ def test_c5_flux_pattern_gradient():
     for k in range(8):
         d = rng.standard_normal(len(base)) + 1j * rng.standard_normal(len(base))
         # J is about 3.3e6 while its change along d is a few times 10 h: the
-        # rounding of J (about 3e-8 here) limits the central difference to
-        # 3e-8 / h, so h is large (8e-3 of the modulus-0.1 coefficients);
-        # measured: 2e-8 .. 7e-7 relative over the 8 directions (at h = 1e-4
-        # the same differences stray by 1e-5, at 2e-3 by up to 1.3e-6)
-        fd = central(d, 8.0e-3)
+        # rounding of J (a few 1e-8) limits a central difference to about
+        # 3e-8 / h, and the h^2 term grows with h, so the fourth-order
+        # combination (4 D(h) - D(2h)) / 3 of two large steps is used
+        # (h = 3.2e-2 of the modulus-0.1 coefficients).  Measured over the 8
+        # directions: 1e-9 .. 9e-8 relative; the plain central differences
+        # stray by up to 2e-6 at h = 8e-3 and 1.2e-5 at 3.2e-2.
+        h = 3.2e-2
+        fd = (4.0 * central(d, h) - central(d, 2.0 * h)) / 3.0
         ad = float(np.sum(g.real * d.real + g.imag * d.imag))
-        print(f"direction {k}: central difference {fd:.10e}, adjoint {ad:.10e}", flush=True)
+        print(f"direction {k}: fourth-order central difference {fd:.10e}, adjoint {ad:.10e}", flush=True)
         assert abs(fd - ad) <= 1e-6 * max(abs(fd), abs(ad)), (k, fd, ad, J0)
 
 
