@@ -440,7 +440,7 @@ __device__ __forceinline__ void inplace_prime_first(cd *a, int n, int ntr, int R
 // om: LDS for PZ_FFT_OM_MAX entries, used when the plan starts with an odd
 // prime 5 <= R <= 31 (one item per thread there: fft_inplace_fits)
 template <class TW>
-__device__ void fft_inplace(cd *a, int n, int ntr, int dir, const int *radices, int nrad, TW tw, cd *om)
+__device__ __forceinline__ void fft_inplace(cd *a, int n, int ntr, int dir, const int *radices, int nrad, TW tw, cd *om)
 {
     int Ns = 1;
     for (int s = 0; s < nrad; s++) {
