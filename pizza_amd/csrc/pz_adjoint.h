@@ -612,7 +612,8 @@ __global__ void __launch_bounds__(256) k_radial_adj(const PzDev *__restrict__ pf
 // the transforms of fft_inplace (two at a time) and four LDS columns: the
 // forward grid (us, up), (omega, T) and the adjoint product columns.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS)
+#define PZ_RADIAL_ADJ_THREADS 1024
+__global__ void __launch_bounds__(PZ_RADIAL_ADJ_THREADS)
 k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ plam, const PzAdjDev *__restrict__ pad)
 {
     extern __shared__ double smem[];

@@ -2832,7 +2832,7 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
                     hipLaunchKernelGGL(k_gather_r,
                                        dim3((nm + PZ_GATHER_TM - 1) / PZ_GATHER_TM, (N + PZ_GATHER_TR - 1) / PZ_GATHER_TR),
                                        dim3(256), 0, a->s2, fw);
-                    hipLaunchKernelGGL(k_radial_adj_inplace, dim3(N), dim3(PZ_RADIAL_INPLACE_THREADS), lds_radj_in,
+                    hipLaunchKernelGGL(k_radial_adj_inplace, dim3(N), dim3(PZ_RADIAL_ADJ_THREADS), lds_radj_in,
                                        a->s2, fw, (const PzDev *)a->dlam, (const PzAdjDev *)a->dad);
                 } else {
                     hipLaunchKernelGGL(k_radial_adj, dim3(N), dim3(256), lds_radj, a->s2, fw, (const PzDev *)a->dlam,

@@ -376,7 +376,7 @@ __device__ __forceinline__ void inplace_prime(cd *a, int n, int ntr, int Ns, int
 // Horner steps per output.  om[(j-1) PZ_SYM_ROW + k-1] = w^{jk mod R} (a
 // row per j, filled here); an item takes P consecutive pairs of one
 // butterfly (the first item also X_0 = x_0 + sum s_j).
-#define PZ_SYM_PAIRS 4
+#define PZ_SYM_PAIRS 5
 #define PZ_SYM_ROW 16
 #define PZ_FFT_OM_MAX (15 * PZ_SYM_ROW)
 __host__ __device__ inline bool sym_first_stage(int R) { return R >= 5 && (R & 1) && R <= 31; }
@@ -454,11 +454,12 @@ __device__ __forceinline__ void fft_inplace(cd *a, int n, int ntr, int dir, cons
         case 2: inplace_bfly<2, 2>(a, n, ntr, Ns, dir, tw); break;
         case 3: inplace_bfly<3, 2>(a, n, ntr, Ns, dir, tw); break;
         case 4: inplace_bfly<4, 1>(a, n, ntr, Ns, dir, tw); break;
-        case 6: inplace_bfly<6, 1>(a, n, ntr, Ns, dir, tw); break;
+        case 6: inplace_bfly<6, 2>(a, n, ntr, Ns, dir, tw); break;
         default:
-            // one item per thread when six outputs an item cover the stage
+            // one item per thread when six outputs an item cover the stage,
+            // else up to three items of four outputs
             if (ntr * (n / R) * ((R + 5) / 6) <= (int)blockDim.x) inplace_prime<1, 6>(a, n, ntr, Ns, R, tw);
-            else inplace_prime<2, 4>(a, n, ntr, Ns, R, tw);
+            else inplace_prime<3, 4>(a, n, ntr, Ns, R, tw);
             break;
         }
         Ns *= R;
@@ -470,14 +471,15 @@ inline bool fft_inplace_fits(int n, int ntr, const int *radices, int nrad, int n
 {
     for (int s = 0; s < nrad; s++) {
         const int R = radices[s], nb = n / R;
-        long items, cap;
-        items = (long)ntr * nb;
-        cap = (R == 4 || R == 6) ? 1 : 2;
+        long items = (long)ntr * nb, cap = R == 4 ? 1 : 2; // radix 2, 3, 6: two butterflies
         if (s == 0 && sym_first_stage(R)) {
             items *= ((R - 1) / 2 + PZ_SYM_PAIRS - 1) / PZ_SYM_PAIRS;
             cap = 1;
         } else if (R != 2 && R != 3 && R != 4 && R != 6) {
+            // six outputs an item, one item a thread, or four outputs, three items
+            if (items * ((R + 5) / 6) <= nthreads) continue;
             items *= (R + 3) / 4;
+            cap = 3;
         }
         if (items > cap * nthreads) return false;
     }
@@ -1083,9 +1085,11 @@ __global__ void __launch_bounds__(PZ_RADIAL_BIG_THREADS) k_radial_loop_big(const
 {
     radial_loop_big_body(*pd, blockIdx.x, is, courant);
 }
-#define PZ_RADIAL_INPLACE_THREADS 1024
-__global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS) k_radial_loop_inplace(const PzDev *__restrict__ pd, int is,
-                                                                                    int courant)
+// 512 threads within 128 registers: two workgroups (2 x 67 KB of LDS) share
+// a CU, so the 257 radii of n_r = 257 run in one round on 256 CUs
+#define PZ_RADIAL_INPLACE_THREADS 512
+__global__ void __launch_bounds__(PZ_RADIAL_INPLACE_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+k_radial_loop_inplace(const PzDev *__restrict__ pd, int is, int courant)
 {
     radial_loop_inplace_body(*pd, blockIdx.x, is, courant);
 }
