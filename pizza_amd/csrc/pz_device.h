@@ -1053,30 +1053,44 @@ __device__ __forceinline__ void radial_loop_inplace_body(const PzDev &p, int nr,
 }
 
 // u_s, u_phi, omega (and T) from the mode-major layout [m][n_r] to rows by
-// radius [f][n_r][n_m]: 16 x 32 tiles through LDS, coalesced on both sides
+// radius [f][n_r][n_m]: 16 x 32 tiles of all fields through LDS, coalesced
+// on both sides, every load of a thread in flight before the one barrier
 #define PZ_GATHER_TM 16
 #define PZ_GATHER_TR 32
 __global__ void __launch_bounds__(256) k_gather_r(const PzDev *__restrict__ pd)
 {
     const PzDev &p = *pd;
-    __shared__ cd tile[PZ_GATHER_TM][PZ_GATHER_TR + 1];
+    __shared__ cd tile[4][PZ_GATHER_TM][PZ_GATHER_TR + 1];
     const int N = p.N, nm = p.nm;
     const size_t F = (size_t)nm * N;
     const int m0 = blockIdx.x * PZ_GATHER_TM, r0 = blockIdx.y * PZ_GATHER_TR;
     const int nf = p.l_heat ? 4 : 3;
-    for (int f = 0; f < nf; f++) {
+    constexpr int PER = PZ_GATHER_TM * PZ_GATHER_TR / 256; // elements a thread moves per field
+    cd v[4][PER];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
         const cd *src = f == 0 ? p.us : f == 1 ? p.up : f == 2 ? p.om : p.temp;
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int t = threadIdx.x + e * 256, mm = t / PZ_GATHER_TR, rr = t % PZ_GATHER_TR;
+            v[f][e] = (f < nf && m0 + mm < nm && r0 + rr < N) ? src[(size_t)(m0 + mm) * N + r0 + rr] : mk(0, 0);
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int t = threadIdx.x + e * 256;
+            tile[f][t / PZ_GATHER_TR][t % PZ_GATHER_TR] = v[f][e];
+        }
+    __syncthreads();
+    for (int f = 0; f < nf; f++) {
         cd *dst = p.rrow + f * F;
-        for (int t = threadIdx.x; t < PZ_GATHER_TM * PZ_GATHER_TR; t += blockDim.x) {
-            const int mm = t / PZ_GATHER_TR, rr = t % PZ_GATHER_TR;
-            if (m0 + mm < nm && r0 + rr < N) tile[mm][rr] = src[(size_t)(m0 + mm) * N + r0 + rr];
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            const int t = threadIdx.x + e * 256, rr = t / PZ_GATHER_TM, mm = t % PZ_GATHER_TM;
+            if (m0 + mm < nm && r0 + rr < N) dst[(size_t)(r0 + rr) * nm + m0 + mm] = tile[f][mm][rr];
         }
-        __syncthreads();
-        for (int t = threadIdx.x; t < PZ_GATHER_TM * PZ_GATHER_TR; t += blockDim.x) {
-            const int rr = t / PZ_GATHER_TM, mm = t % PZ_GATHER_TM;
-            if (m0 + mm < nm && r0 + rr < N) dst[(size_t)(r0 + rr) * nm + m0 + mm] = tile[mm][rr];
-        }
-        __syncthreads();
     }
 }
 
