@@ -455,12 +455,7 @@ __device__ __forceinline__ void fft_inplace(cd *a, int n, int ntr, int dir, cons
         case 3: inplace_bfly<3, 2>(a, n, ntr, Ns, dir, tw); break;
         case 4: inplace_bfly<4, 1>(a, n, ntr, Ns, dir, tw); break;
         case 6: inplace_bfly<6, 2>(a, n, ntr, Ns, dir, tw); break;
-        default:
-            // one item per thread when six outputs an item cover the stage,
-            // else up to three items of four outputs
-            if (ntr * (n / R) * ((R + 5) / 6) <= (int)blockDim.x) inplace_prime<1, 6>(a, n, ntr, Ns, R, tw);
-            else inplace_prime<3, 4>(a, n, ntr, Ns, R, tw);
-            break;
+        default: inplace_prime<3, 4>(a, n, ntr, Ns, R, tw); break; // up to three items of four outputs
         }
         Ns *= R;
     }
@@ -476,9 +471,7 @@ inline bool fft_inplace_fits(int n, int ntr, const int *radices, int nrad, int n
             items *= ((R - 1) / 2 + PZ_SYM_PAIRS - 1) / PZ_SYM_PAIRS;
             cap = 1;
         } else if (R != 2 && R != 3 && R != 4 && R != 6) {
-            // six outputs an item, one item a thread, or four outputs, three items
-            if (items * ((R + 5) / 6) <= nthreads) continue;
-            items *= (R + 3) / 4;
+            items *= (R + 3) / 4; // four outputs an item, three items a thread
             cap = 3;
         }
         if (items > cap * nthreads) return false;
