@@ -45,6 +45,7 @@ struct PzAdjDev {
     // finite differences: copies of the factorised systems (fT, fX, fU, fP)
     const double *fT_s, *fX_s, *fU_s, *fT_b, *fX_b, *fU_b;
     const cd *fP_s, *fP_b;
+    cd *gfs; // long grids: the forward grid columns by radius, 2 n_phi each
 };
 
 // out_f[j] (+)= sum_{i<n} M[j ld + i] y_f[i] for j < n: one wave per output
@@ -608,12 +609,16 @@ __global__ void __launch_bounds__(256) k_radial_adj(const PzDev *__restrict__ pf
 // ---------------------------------------------------------------------------
 // Transpose of radial_loop_inplace_body (long azimuthal grids without
 // composition), linearised at fw, for one radius; the steps of k_radial_adj
-// with the spectra by radius from k_gather_r (run on fw, into fw's rrow),
-// the transforms of fft_inplace (two at a time) and four LDS columns: the
-// forward grid (us, up), (omega, T) and the adjoint product columns.
+// with the spectra by radius from k_gather_r (run on fw, into fw's rrow)
+// and the transforms of fft_inplace (two at a time) in one pair of LDS
+// columns: the forward grid (us, up), (omega, T) is transformed there and
+// parked in ad.gfs (2 n_phi per radius, read back by the same workgroup),
+// then the adjoint product columns take the LDS.  67 KB of LDS and 512
+// threads within 128 registers: two workgroups share a CU, as in the
+// forward.
 // ---------------------------------------------------------------------------
-#define PZ_RADIAL_ADJ_THREADS 1024
-__global__ void __launch_bounds__(PZ_RADIAL_ADJ_THREADS)
+#define PZ_RADIAL_ADJ_THREADS 512
+__global__ void __launch_bounds__(PZ_RADIAL_ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ plam, const PzAdjDev *__restrict__ pad)
 {
     extern __shared__ double smem[];
@@ -622,9 +627,10 @@ k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ pl
     const PzAdjDev &A = *pad;
     const int nr = blockIdx.x, n = p.nphi, nm = p.nm, N = p.N, tid = threadIdx.x, nb = blockDim.x;
     const size_t F = (size_t)nm * N;
-    cd *gf = (cd *)smem, *sa = gf + 2 * n, *t1 = sa + 2 * n, *t2 = t1 + 64;
+    cd *sa = (cd *)smem, *t1 = sa + 2 * n, *t2 = t1 + 64;
     const int n2 = (n + 63) >> 6;
     cd *om = t2 + n2;
+    cd *G = A.gfs + (size_t)nr * 2 * n;
     const bool hT = p.l_heat;
     const cd *Rus = p.rrow + (size_t)nr * nm, *Rup = Rus + F, *Rom = Rus + 2 * F, *RT = Rus + 3 * F;
     for (int k = tid; k < 64 + n2; k += nb) t1[k] = p.twid[k < 64 ? k : 64 * (k - 64)];
@@ -633,48 +639,58 @@ k_radial_adj_inplace(const PzDev *__restrict__ pfw, const PzDev *__restrict__ pl
         const cd w = cmul(t1[k & 63], t2[k >> 6]);
         return mk(w.x, -w.y);
     };
-    const double or1 = p.or1[nr], rr = p.r[nr];
-    const double fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
-    // forward grid columns (us, up), (om, T); adjoint product spectra
-    // (halved above m = 0) packed the same way
+    // the forward grid columns (us, up), (om, T), parked in G
     for (int t = tid; t < 2 * n; t += nb) {
         const int tr = t >= n, k = t - tr * n;
-        gf[t] = herm_pack(tr == 0 ? Rus : Rom, tr == 0 ? Rup : (hT ? RT : nullptr), k, n, nm);
-        const bool lo = k < nm, hi = !lo && n - k < nm;
-        cd z = mk(0, 0);
-        if (lo || hi) {
-            const int i = lo ? k : n - k, m = p.idx2m[i];
-            const size_t q = (size_t)i * N + nr;
-            cd a, b;
-            if (tr == 0) {
-                a = m == 0 ? mk(0, 0) : ci(or1 * m, A.eP0[q]);
-                b = A.aP0[q];
-            } else if (hT) {
-                const cd e = A.eT0[q];
-                a = ci(or1 * m, e);
-                b = csub(A.aT0[q], cscale(fbeta, e));
-            } else {
-                a = b = mk(0, 0);
-            }
-            const double h = i == 0 ? 1.0 : 0.5;
-            a = cscale(h, a);
-            b = cscale(h, b);
-            if (k == 0) z = mk(a.x, b.x);
-            else if (lo) z = mk(a.x - b.y, a.y + b.x);
-            else z = mk(a.x + b.y, b.x - a.y);
-        }
-        sa[t] = z;
+        sa[t] = herm_pack(tr == 0 ? Rus : Rom, tr == 0 ? Rup : (hT ? RT : nullptr), k, n, nm);
     }
     __syncthreads();
-    fft_inplace(gf, n, 2, +1, p.radices, p.nrad, bwd, om);
     fft_inplace(sa, n, 2, +1, p.radices, p.nrad, bwd, om);
-    const double inv_n = 1.0 / (double)n;
-    for (int j = tid; j < n; j += nb) {
-        const double us = gf[j].x, up = gf[j].y, om = gf[n + j].x, T = gf[n + j].y;
-        const double ao = sa[j].x * inv_n, bo = sa[j].y * inv_n;
-        const double aT = hT ? sa[n + j].x * inv_n : 0.0, bT = hT ? sa[n + j].y * inv_n : 0.0;
-        sa[j] = mk(rr * (bo * om + bT * T), ao * om + aT * T);
-        sa[n + j] = mk(ao * up + bo * rr * us, aT * up + bT * rr * us);
+    for (int t = tid; t < 2 * n; t += nb) G[t] = sa[t];
+    __syncthreads();
+    // adjoint product spectra (halved above m = 0) packed the same way
+    {
+        const double or1 = p.or1[nr], fbeta = (1.0 - p.tadvz) * p.beta[nr] * or1;
+        for (int t = tid; t < 2 * n; t += nb) {
+            const int tr = t >= n, k = t - tr * n;
+            const bool lo = k < nm, hi = !lo && n - k < nm;
+            cd z = mk(0, 0);
+            if (lo || hi) {
+                const int i = lo ? k : n - k, m = p.idx2m[i];
+                const size_t q = (size_t)i * N + nr;
+                cd a, b;
+                if (tr == 0) {
+                    a = m == 0 ? mk(0, 0) : ci(or1 * m, A.eP0[q]);
+                    b = A.aP0[q];
+                } else if (hT) {
+                    const cd e = A.eT0[q];
+                    a = ci(or1 * m, e);
+                    b = csub(A.aT0[q], cscale(fbeta, e));
+                } else {
+                    a = b = mk(0, 0);
+                }
+                const double h = i == 0 ? 1.0 : 0.5;
+                a = cscale(h, a);
+                b = cscale(h, b);
+                if (k == 0) z = mk(a.x, b.x);
+                else if (lo) z = mk(a.x - b.y, a.y + b.x);
+                else z = mk(a.x + b.y, b.x - a.y);
+            }
+            sa[t] = z;
+        }
+    }
+    __syncthreads();
+    fft_inplace(sa, n, 2, +1, p.radices, p.nrad, bwd, om);
+    {
+        const double rr = p.r[nr], inv_n = 1.0 / (double)n;
+        for (int j = tid; j < n; j += nb) {
+            const cd g0 = G[j], g1 = G[n + j];
+            const double us = g0.x, up = g0.y, omv = g1.x, T = g1.y;
+            const double ao = sa[j].x * inv_n, bo = sa[j].y * inv_n;
+            const double aT = hT ? sa[n + j].x * inv_n : 0.0, bT = hT ? sa[n + j].y * inv_n : 0.0;
+            sa[j] = mk(rr * (bo * omv + bT * T), ao * omv + aT * T);
+            sa[n + j] = mk(ao * up + bo * rr * us, aT * up + bT * rr * us);
+        }
     }
     __syncthreads();
     fft_inplace(sa, n, 2, -1, p.radices, p.nrad, fwd, om);

@@ -2586,6 +2586,7 @@ static void adj_init(pz_handle *h)
     HIPCHK(hipEventCreateWithFlags(&a->ev[1], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&a->ev[2], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&a->ev[3], hipEventDisableTiming));
+    if (h->radial_inplace) ad.gfs = h->dalloc<cd>(2 * (size_t)h->nphi * h->N);
     a->dad = h->upload(std::vector<PzAdjDev>{ad});
     a->lam = h->dalloc<char>(h->arena_bytes);
     PzDev lv = h->d;
@@ -2693,7 +2694,7 @@ extern "C" int pz_adjoint_forward(pz_handle *h, int objective, const double *wei
         HIPCHK(hipStreamSynchronize(h->stream));
         if (s0.n_steps != 0) return fail(h, -5, "the adjoint window starts right after pz_start");
         const size_t lds_radj = h->radial_inplace
-                                    ? (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd)
+                                    ? (2 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd)
                                     : (11 * (size_t)h->nm + 9 * (size_t)h->nphi) * sizeof(cd);
         const size_t lds_madj = h->d.fd ? (3 * ((size_t)h->N + 4) + 10 * (size_t)h->N) * sizeof(cd)
                                         : ADJ_NVEC * (size_t)h->N * sizeof(cd);
@@ -2766,7 +2767,7 @@ extern "C" int pz_adjoint_backward(pz_handle *h, double dJ, double *grad_temp, d
         const size_t lds_radj = (11 * (size_t)nm + 9 * (size_t)h->nphi) * sizeof(cd);
         const size_t lds_madj = ADJ_NVEC * (size_t)N * sizeof(cd);
         const size_t lds_fadj = (3 * ((size_t)N + 4) + 10 * (size_t)N) * sizeof(cd);
-        const size_t lds_radj_in = (4 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd);
+        const size_t lds_radj_in = (2 * (size_t)h->nphi + 64 + (h->nphi + 63) / 64 + PZ_FFT_OM_MAX) * sizeof(cd);
         const double tiny = 10.0 * 2.220446049250313e-16;
         double wnu = 0.0;
         if (a->objective == PZ_OBJ_NUTOP && std::fabs(h->dtcond_top) > tiny) wnu = dJ / ((double)K * h->dtcond_top);
